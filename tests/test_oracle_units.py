@@ -1,0 +1,163 @@
+"""Oracle unit checks fixed by mathematics (closed forms, invariants, FD)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import basis, fe, kernels
+from oracle import solver as osv
+
+
+@pytest.mark.parametrize("ele", ["HEX8", "HEX27"])
+def test_basis_invariants(ele):
+    order = 2 if ele == "HEX8" else 4
+    v, g, w = basis.get_shape_vals_and_grads(ele, order)
+    assert abs(w.sum() - 1) < 1e-14
+    np.testing.assert_allclose(v.sum(1), 1, atol=1e-13)
+    np.testing.assert_allclose(g.sum(1), 0, atol=1e-12)
+    X, _ = basis.hex_quadrature(order)
+    nc = basis.node_coords(ele)
+    # reproduction of polynomial fields of the element's degree
+    f = lambda x: 1 + 2 * x[:, 0] - x[:, 1] + 0.5 * x[:, 2] + (x[:, 0] * x[:, 1] * x[:, 2] if ele == "HEX8" else x[:, 0] ** 2 * x[:, 1] ** 2 * x[:, 2] ** 2)
+    np.testing.assert_allclose(v @ f(nc), f(X), atol=1e-12)
+    # Kronecker property at nodes
+    vn, _ = basis.tabulate(ele, nc)
+    np.testing.assert_allclose(vn, np.eye(len(nc)), atol=1e-12)
+
+
+def test_gauss_exactness_and_hex27_order():
+    x, w = basis.gauss_legendre_01(3)
+    for k in range(6):
+        assert abs(np.sum(w * x ** k) - 1.0 / (k + 1)) < 1e-14
+    # HEX27 mesh order = VTK triquadratic hexahedron (basis.py:60-61 with basix numbering)
+    nc = basis.node_coords("HEX27")
+    assert np.allclose(nc[8], [0.5, 0, 0]) and np.allclose(nc[11], [0, 0.5, 0])
+    assert np.allclose(nc[16], [0, 0, 0.5]) and np.allclose(nc[20], [0, 0.5, 0.5])
+    assert np.allclose(nc[24], [0.5, 0.5, 0]) and np.allclose(nc[26], [0.5, 0.5, 0.5])
+
+
+def test_face_areas_and_volume():
+    pts, cells = fe.box_mesh(3, 2, 4, 1.5, 1.0, 2.0)
+    v, g, w = basis.get_shape_vals_and_grads("HEX8")
+    _, JxW = fe.shape_grads(pts, cells, g, w)
+    assert abs(JxW.sum() - 3.0) < 1e-12
+    fsv, fsg, fw, fn, finds = basis.get_face_shape_vals_and_grads("HEX8")
+    bi = fe.boundary_inds(pts, cells, [lambda p: abs(p[0] - 1.5) < 1e-9], finds)[0]
+    assert len(bi) == 8
+    _, nanson = fe.face_shape_grads(pts, cells, bi, fsg, fn, fw)
+    assert abs(nanson.sum() - 2.0) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["linear_elastic", "neo_hookean"])
+def test_tangent_is_fd_of_stress(kind):
+    mat = kernels.Material(kind, E=10.0, nu=0.3)
+    rng = np.random.default_rng(1)
+    ug = 0.1 * rng.standard_normal((4, 3, 3))
+    m1, m2, _, _ = mat.moduli(None)
+    A = mat.tangent(ug, m1, m2)
+    h = 1e-6
+    for k in range(3):
+        for l in range(3):
+            d = np.zeros((3, 3)); d[k, l] = h
+            fd = (mat.stress(ug + d, m1, m2) - mat.stress(ug - d, m1, m2)) / (2 * h)
+            np.testing.assert_allclose(A[..., k, l], fd, rtol=1e-6, atol=1e-7)
+
+
+def test_stress_is_fd_of_energy():
+    mat = kernels.Material("neo_hookean", E=10.0, nu=0.3)
+    rng = np.random.default_rng(2)
+    ug = 0.1 * rng.standard_normal((3, 3))
+    m1, m2, _, _ = mat.moduli(None)
+    P = mat.stress(ug, m1, m2)
+    h = 1e-6
+    for k in range(3):
+        for l in range(3):
+            d = np.zeros((3, 3)); d[k, l] = h
+            fd = (mat.energy(ug + d, m1, m2) - mat.energy(ug - d, m1, m2)) / (2 * h)
+            assert abs(fd - P[k, l]) < 1e-6
+
+
+def _small_problem(kind="neo_hookean", ele="HEX8"):
+    if ele == "HEX8":
+        pts, cells = fe.box_mesh(2, 2, 2)
+        order = None
+    else:
+        pts, cells = fe.box_mesh_hex27(1, 1, 2)
+        order = 4
+    P = osv.OracleProblem(pts, cells, 3, kernels.Material(kind, E=10.0, nu=0.3), ele_type=ele,
+                          quadrature_order=order)
+    rng = np.random.default_rng(3)
+    u = 0.05 * rng.standard_normal(P.N)
+    return P, u
+
+
+@pytest.mark.parametrize("ele", ["HEX8", "HEX27"])
+def test_element_jacobian_fd_symmetry_rigid(ele):
+    P, u = _small_problem("neo_hookean", ele)
+    cu = P.cell_u(u)
+    K = kernels.cell_jacobian(P.mat, cu, P.grads, P.JxW)
+    nd = K.shape[1]
+    h = 1e-6
+    for c in range(nd):
+        du = np.zeros_like(cu).reshape(len(cu), -1)
+        du[:, c] = h
+        rp = kernels.cell_residual(P.mat, cu + du.reshape(cu.shape), P.grads, P.JxW)
+        rm = kernels.cell_residual(P.mat, cu - du.reshape(cu.shape), P.grads, P.JxW)
+        np.testing.assert_allclose(K[:, :, c], (rp - rm) / (2 * h), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(K, np.swapaxes(K, 1, 2), atol=1e-10)
+    # at u = 0 the rigid translations are in the null space of K (linear elastic)
+    Kl = kernels.cell_jacobian(kernels.Material("linear_elastic", E=10.0, nu=0.3), 0 * cu, P.grads, P.JxW)
+    t = np.tile([1.0, 0.0, 0.0], nd // 3)
+    assert np.abs(Kl @ t).max() < 1e-10
+
+
+def test_residual_is_fd_of_energy():
+    P, u = _small_problem()
+    R = P.compute_residual(u)
+    h = 1e-6
+    for i in [0, 5, 17, 40]:
+        e = np.zeros_like(u); e[i] = h
+        fd = (kernels.energy_total(P.mat, P.cell_u(u + e), P.grads, P.JxW)
+              - kernels.energy_total(P.mat, P.cell_u(u - e), P.grads, P.JxW)) / (2 * h)
+        assert abs(fd - R[i]) < 1e-6
+
+
+def test_textbook_cube_stiffness_trace():
+    # unit-cube hex8, linear elastic, 2x2x2 Gauss: K is exact for the trilinear
+    # field; check the energy of a uniform strain equals volume * sigma:eps / 2.
+    pts, cells = fe.box_mesh(1, 1, 1)
+    P = osv.OracleProblem(pts, cells, 3, kernels.Material("linear_elastic", E=7.0, nu=0.25))
+    K = kernels.cell_jacobian(P.mat, np.zeros((1, 8, 3)), P.grads, P.JxW)[0]
+    eps = np.array([[1e-3, 2e-4, 0], [2e-4, -5e-4, 1e-4], [0, 1e-4, 3e-4]])
+    u = (pts @ eps.T)[cells[0]].reshape(-1)
+    lam = 7.0 * 0.25 / (1.25 * 0.5); mu = 7.0 / 2.5
+    sig = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    assert abs(0.5 * u @ K @ u - 0.5 * np.sum(sig * eps)) < 1e-14
+
+
+def test_pattern_and_row_elimination():
+    P, u = _small_problem()
+    P.update_bcs([[lambda p: abs(p[0]) < 1e-9] * 2, [0, 2], [lambda p: 0.0, lambda p: 0.1]])
+    A = P.assemble(u)
+    I, J = P.coo_pattern()
+    ref = sp.coo_matrix((np.ones(len(I)), (I, J)), shape=(P.N, P.N)).tocsr()
+    ref.sum_duplicates(); ref.sort_indices()
+    assert np.array_equal(A.indptr, ref.indptr) and np.array_equal(A.indices, ref.indices)
+    # structured closed form nnz = 9 (3nx+1)(3ny+1)(3nz+1)
+    assert A.nnz == 9 * 7 ** 3
+    for rows in P.bc_rows():
+        for r in rows:
+            row = A.getrow(r)
+            assert row.nnz == A.indptr[r + 1] - A.indptr[r]           # pattern kept
+            assert row[0, r] == 1.0 and abs(row).sum() == 1.0
+
+
+def test_bicgstab_matches_direct_solve():
+    P, u = _small_problem("linear_elastic")
+    P.update_bcs([[lambda p: abs(p[0]) < 1e-9] * 3, [0, 1, 2], [lambda p: 0.0] * 3])
+    A = P.assemble(u)
+    b = np.random.default_rng(4).standard_normal(P.N)
+    b[np.concatenate(P.bc_rows())] = 0
+    x, k, info = osv.bicgstab(A, b, np.zeros_like(b), lambda v: v / A.diagonal())
+    import scipy.sparse.linalg as spla
+    xd = spla.spsolve(A.tocsc(), b)
+    assert info == 0 and np.abs(x - xd).max() < 1e-8 * np.abs(xd).max()
