@@ -1,0 +1,166 @@
+/* libhipfem — MI355X (gfx950) kernels for JAX-FEM's implicit solve path.
+ *
+ * C ABI: plain pointers and sizes.  Every array argument is a caller-owned
+ * DEVICE pointer unless stated otherwise; `stream` is a hipStream_t (the
+ * caller's current stream).  Every function returns 0 on success, -1 on an
+ * argument error and -2 on a HIP error; hf_last_error() gives the message.
+ * Calls allocate no device memory except hf_pattern_build / hf_pattern_destroy
+ * (set-up time).  Calls on one stream are ordered; calls sharing workspace
+ * must not run concurrently on different streams.
+ *
+ * DOF numbering is the reference's interleaved 3*node+comp (problem.py:86-92).
+ * Element matrices, the block-sparse (BSR) matrix and its CSR view follow the
+ * reference's COO layout jac[r,c] = dR_r/du_c (problem.py:262-266, 453).
+ */
+#ifndef HIPFEM_H
+#define HIPFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* hf_stream_t; /* hipStream_t */
+
+/* ---- constitutive laws (the Problem's get_tensor_map, problem.py:189-214) ---- */
+enum { HF_MAT_POISSON = 0, HF_MAT_LINEAR_ELASTIC = 1, HF_MAT_NEO_HOOKEAN = 2 };
+/* how the per-quadrature parameter p (Problem.internal_vars[0], problem.py:493-556)
+ * enters the moduli */
+enum { HF_PARAM_NONE = 0, HF_PARAM_SCALE_E = 1, HF_PARAM_SIMP = 2, HF_PARAM_MU = 3,
+       HF_PARAM_SCALE_K = 4 };
+
+typedef struct hf_material {
+  int32_t kind;       /* HF_MAT_* */
+  int32_t param_mode; /* HF_PARAM_* */
+  double E, nu;       /* Young's modulus, Poisson ratio */
+  double E0, Emin, penal; /* SIMP: E = Emin + (E0-Emin) p^penal */
+  double kappa;       /* bulk modulus when mu = p (HF_PARAM_MU) */
+  double k;           /* Poisson coefficient (flux = k grad u) */
+} hf_material;
+
+/* Message of the last failed call on this thread; returns its length. */
+int hf_last_error(char* buf, size_t n);
+/* Library version string (host pointer, static). */
+const char* hf_version(void);
+
+/* ---- sparsity pattern: replaces _PetscTangentCache.__init__'s
+ *      setPreallocationCOO(I, J) (solver.py:469-475) over the COO (I, J) of
+ *      Problem.__post_init__ (problem.py:86-107).  Built on the device by a
+ *      radix sort of node-pair keys; the scalar CSR view is bit-identical to
+ *      the sorted-unique (I, J) pattern. ---------------------------------- */
+typedef struct hf_pattern hf_pattern;
+int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nodes_per_cell, int64_t nnode,
+                     int32_t vec, hf_stream_t stream, hf_pattern** out);
+/* host outputs: number of blocks (node pairs), scalar nnz, contributions */
+int hf_pattern_sizes(const hf_pattern* p, int64_t* nblocks, int64_t* nnz, int64_t* ncontrib);
+/* device pointers owned by the pattern: block rowptr (nnode+1), block colidx (nblocks),
+ * diag block index (nnode), transposed-block index (nblocks) */
+int hf_pattern_arrays(const hf_pattern* p, const int32_t** rowptr, const int32_t** colidx,
+                      const int32_t** diag, const int32_t** trans);
+/* scalar CSR view: rowptr (N+1), colind (nnz) into caller buffers (solver.py:65, getValuesCSR) */
+int hf_pattern_csr(const hf_pattern* p, int32_t* rowptr, int32_t* colind, hf_stream_t stream);
+int hf_pattern_destroy(hf_pattern* p);
+
+/* ---- cell kernels: replace the jit(vmap(kernel)) / jit(vmap(kernel_jac)) of
+ *      pre_jit_fns (problem.py:261-356) and split_and_compute_cell (:359-396).
+ *      points (nnode,3) f64, cells (ncell,nn) i32, u (nnode*vec) f64,
+ *      params (ncell,nq) f64 or NULL.  elem_res (ncell, nn*vec); elem_jac
+ *      block-major (ncell, nn, nn, vec, vec).  ele: 8 (HEX8, 2x2x2 Gauss)
+ *      or 27 (HEX27, 3x3x3 Gauss). ----------------------------------------- */
+int hf_cell_residual_f64(int32_t ele, const hf_material* mat, const double* points,
+                         const int32_t* cells, int64_t ncell, const double* u,
+                         const double* params, double* elem_res, hf_stream_t stream);
+int hf_cell_residual_f32(int32_t ele, const hf_material* mat, const double* points,
+                         const int32_t* cells, int64_t ncell, const double* u,
+                         const double* params, float* elem_res, hf_stream_t stream);
+int hf_cell_jacobian_f64(int32_t ele, const hf_material* mat, const double* points,
+                         const int32_t* cells, int64_t ncell, const double* u,
+                         const double* params, double* elem_jac, hf_stream_t stream);
+int hf_cell_jacobian_f32(int32_t ele, const hf_material* mat, const double* points,
+                         const int32_t* cells, int64_t ncell, const double* u,
+                         const double* params, float* elem_jac, hf_stream_t stream);
+/* per-cell parameter VJP: grad[c,q] = -sum lam . dr/dp[c,q]; replaces the
+ * jax.vjp over compute_residual in implicit_vjp (solver.py:1348-1378).
+ * lam (nnode*vec) f64 must be zero on Dirichlet rows. grad (ncell, nq) f64. */
+int hf_cell_param_vjp_f64(int32_t ele, const hf_material* mat, const double* points,
+                          const int32_t* cells, int64_t ncell, const double* u,
+                          const double* params, const double* lam, double* grad,
+                          hf_stream_t stream);
+
+/* ---- assembly: replaces compute_residual_vars_helper's scatter-add
+ *      (problem.py:426-437) and PETSc setValuesCOO + assemble (solver.py:485-488)
+ *      by deterministic segmented reductions over the pattern. -------------- */
+/* R[node] = sum over cells of elem_res in cell order (+ f_ext if not NULL) */
+int hf_gather_residual_f64(const hf_pattern* p, const double* elem_res, const double* f_ext,
+                           double* R, hf_stream_t stream);
+/* BSR values (nblocks, vec, vec) = sum of element blocks in cell order */
+int hf_assemble_bsr_f64(const hf_pattern* p, const double* elem_jac, double* bsr, hf_stream_t stream);
+int hf_assemble_bsr_f32(const hf_pattern* p, const float* elem_jac, float* bsr, hf_stream_t stream);
+/* scalar CSR values from BSR values (the CSR order of hf_pattern_csr) */
+int hf_bsr_to_csr_f64(const hf_pattern* p, const double* bsr, double* csr, hf_stream_t stream);
+
+/* ---- Dirichlet row elimination (solver.py:290-363, 489-490). bc_dofs are
+ *      unique DOF indices (later specs already applied), bc_vals their values. */
+/* zeroRows(rows, diag=1) on the BSR values */
+int hf_bc_rows_f64(const hf_pattern* p, const int32_t* bc_dofs, int64_t nbc, double* bsr, hf_stream_t stream);
+int hf_bc_rows_f32(const hf_pattern* p, const int32_t* bc_dofs, int64_t nbc, float* bsr, hf_stream_t stream);
+/* apply_bc_vec: R[d] = u[d] - scale*g[d] */
+int hf_bc_residual_f64(const int32_t* bc_dofs, const double* bc_vals, int64_t nbc, double scale,
+                       const double* u, double* R, hf_stream_t stream);
+/* x0 = assign_bc(0) - copy_bc(u): x0[d] = g[d]-u[d] on B, 0 elsewhere (x0 of length N) */
+int hf_bc_initial_guess_f64(const int32_t* bc_dofs, const double* bc_vals, int64_t nbc,
+                            const double* u, double* x0, int64_t N, hf_stream_t stream);
+
+/* ---- sparse matrix-vector products (the BCOO matvec of jax_solve, solver.py:67-84) */
+int hf_spmv_bsr_f64(const hf_pattern* p, const double* bsr, const double* x, double* y, hf_stream_t stream);
+int hf_spmv_bsr_f32(const hf_pattern* p, const float* bsr, const float* x, float* y, hf_stream_t stream);
+/* y = A^T x through the transposed-block map (PETSc A.transpose, solver.py:1369-1370) */
+int hf_spmv_bsr_t_f64(const hf_pattern* p, const double* bsr, const double* x, double* y, hf_stream_t stream);
+int hf_spmv_bsr_t_f32(const hf_pattern* p, const float* bsr, const float* x, float* y, hf_stream_t stream);
+/* inverse of the scalar diagonal (Jacobi M = 1/diag(A), solver.py:68-69) */
+int hf_jacobi_f64(const hf_pattern* p, const double* bsr, double* dinv, hf_stream_t stream);
+int hf_jacobi_f32(const hf_pattern* p, const float* bsr, float* dinv, hf_stream_t stream);
+
+/* ---- device-resident Krylov solves: replace jax.scipy.sparse.linalg.bicgstab
+ *      inside jax_solve (solver.py:78-84) and the linear_solver plugin slot
+ *      (solver.py:261-284).  Stop rule ||r||^2 <= max(rtol^2 ||b||^2, atol^2)
+ *      with r the unpreconditioned recursive residual; dot products accumulate
+ *      in FP64.  The host polls a device flag every `poll` iterations.
+ *      transpose != 0 solves A^T x = b.  workspace: hf_krylov_workspace_size. */
+typedef struct hf_krylov_info {
+  int32_t iterations;
+  int32_t status;      /* 0 converged, 1 max iterations, 2 breakdown, 3 non-finite */
+  double res_norm;     /* final recursive ||r||_2 */
+  double b_norm;       /* ||b||_2 */
+} hf_krylov_info;
+
+enum { HF_KRYLOV_PCG = 0, HF_KRYLOV_BICGSTAB = 1 };
+size_t hf_krylov_workspace_size(int32_t method, int64_t N, int32_t elem_bytes);
+int hf_krylov_solve_f64(int32_t method, const hf_pattern* p, const double* bsr, const double* dinv,
+                        const double* b, double* x, int32_t transpose, int32_t maxiter,
+                        double rtol, double atol, int32_t poll, void* workspace,
+                        hf_krylov_info* info, hf_stream_t stream);
+int hf_krylov_solve_f32(int32_t method, const hf_pattern* p, const float* bsr, const float* dinv,
+                        const float* b, float* x, int32_t transpose, int32_t maxiter,
+                        double rtol, double atol, int32_t poll, void* workspace,
+                        hf_krylov_info* info, hf_stream_t stream);
+
+/* ---- measurement: HIP-event timing of the Krylov SpMV kernel (the first SpMV
+ *      of every poll batch, on the solve's stream).  No reference counterpart:
+ *      replaces the reference's wall-clock buckets (solver.py:22-57) at kernel
+ *      granularity.  hf_profile_enable also resets the counters. */
+int hf_profile_enable(int32_t on);
+int hf_profile_read(int64_t* launches, double* total_ms);
+
+/* ---- small vector helpers (FP64 accumulate) ---- */
+size_t hf_dot_workspace_size(void);
+/* *result (host) = sum x*y accumulated in FP64, deterministic order */
+int hf_dot_f64(const double* x, const double* y, int64_t n, void* workspace, double* result, hf_stream_t stream);
+int hf_dot_f32(const float* x, const float* y, int64_t n, void* workspace, double* result, hf_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIPFEM_H */

@@ -1,0 +1,122 @@
+"""ctypes binding of libhipfem.so (the C ABI declared in include/hipfem.h).
+
+The library is loaded after ``import torch`` so that it binds to the HIP
+runtime torch already loaded (same SONAME), sharing streams and allocations.
+There is no fallback: a missing library or a failed call raises.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhipfem.so")
+
+c_i32, c_i64, c_f64, c_sz = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_size_t
+vp = ctypes.c_void_p
+
+HF_MAT_POISSON, HF_MAT_LINEAR_ELASTIC, HF_MAT_NEO_HOOKEAN = 0, 1, 2
+HF_PARAM_NONE, HF_PARAM_SCALE_E, HF_PARAM_SIMP, HF_PARAM_MU, HF_PARAM_SCALE_K = 0, 1, 2, 3, 4
+HF_KRYLOV_PCG, HF_KRYLOV_BICGSTAB = 0, 1
+
+
+class HFMaterial(ctypes.Structure):
+    _fields_ = [("kind", c_i32), ("param_mode", c_i32), ("E", c_f64), ("nu", c_f64),
+                ("E0", c_f64), ("Emin", c_f64), ("penal", c_f64), ("kappa", c_f64), ("k", c_f64)]
+
+
+class HFKrylovInfo(ctypes.Structure):
+    _fields_ = [("iterations", c_i32), ("status", c_i32), ("res_norm", c_f64), ("b_norm", c_f64)]
+
+
+class HipFemError(RuntimeError):
+    pass
+
+
+_SIGS = {
+    "hf_last_error": (c_i32, [ctypes.c_char_p, c_sz]),
+    "hf_version": (ctypes.c_char_p, []),
+    "hf_pattern_build": (c_i32, [vp, c_i64, c_i32, c_i64, c_i32, vp, ctypes.POINTER(vp)]),
+    "hf_pattern_sizes": (c_i32, [vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
+    "hf_pattern_arrays": (c_i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+    "hf_pattern_csr": (c_i32, [vp, vp, vp, vp]),
+    "hf_pattern_destroy": (c_i32, [vp]),
+    "hf_cell_residual_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
+    "hf_cell_residual_f32": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
+    "hf_cell_jacobian_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
+    "hf_cell_jacobian_f32": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
+    "hf_cell_param_vjp_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp, vp]),
+    "hf_gather_residual_f64": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_assemble_bsr_f64": (c_i32, [vp, vp, vp, vp]),
+    "hf_assemble_bsr_f32": (c_i32, [vp, vp, vp, vp]),
+    "hf_bsr_to_csr_f64": (c_i32, [vp, vp, vp, vp]),
+    "hf_bc_rows_f64": (c_i32, [vp, vp, c_i64, vp, vp]),
+    "hf_bc_rows_f32": (c_i32, [vp, vp, c_i64, vp, vp]),
+    "hf_bc_residual_f64": (c_i32, [vp, vp, c_i64, c_f64, vp, vp, vp]),
+    "hf_bc_initial_guess_f64": (c_i32, [vp, vp, c_i64, vp, vp, c_i64, vp]),
+    "hf_spmv_bsr_f64": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_spmv_bsr_f32": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_spmv_bsr_t_f64": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_spmv_bsr_t_f32": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_jacobi_f64": (c_i32, [vp, vp, vp, vp]),
+    "hf_jacobi_f32": (c_i32, [vp, vp, vp, vp]),
+    "hf_krylov_workspace_size": (c_sz, [c_i32, c_i64, c_i32]),
+    "hf_krylov_solve_f64": (c_i32, [c_i32, vp, vp, vp, vp, vp, c_i32, c_i32, c_f64, c_f64, c_i32, vp,
+                                    ctypes.POINTER(HFKrylovInfo), vp]),
+    "hf_krylov_solve_f32": (c_i32, [c_i32, vp, vp, vp, vp, vp, c_i32, c_i32, c_f64, c_f64, c_i32, vp,
+                                    ctypes.POINTER(HFKrylovInfo), vp]),
+    "hf_profile_enable": (c_i32, [c_i32]),
+    "hf_profile_read": (c_i32, [ctypes.POINTER(c_i64), ctypes.POINTER(c_f64)]),
+    "hf_dot_workspace_size": (c_sz, []),
+    "hf_dot_f64": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),
+    "hf_dot_f32": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),
+}
+
+_lib = None
+
+
+def declared_symbols():
+    """Entry points declared in include/hipfem.h that this binding uses."""
+    return sorted(_SIGS)
+
+
+def load(path=LIB_PATH):
+    """Load libhipfem.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise HipFemError("libhipfem.so not found at %s; build it with "
+                          "`python jax-fem_amd/build.py`" % path)
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load().hf_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def call(name, *args):
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        raise HipFemError("%s failed (%d): %s" % (name, rc, last_error()))
+    return rc
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous(), "expected a contiguous device tensor"
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream(dev=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)

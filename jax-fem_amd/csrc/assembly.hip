@@ -1,0 +1,420 @@
+// Sparsity pattern, deterministic assembly, Dirichlet row elimination.
+//
+// Reference: the COO pattern of Problem.__post_init__ (problem.py:86-107) is
+// preallocated by PETSc (solver.py:469-475) as the sorted unique (I, J) pairs;
+// setValuesCOO sums duplicates (solver.py:487) and zeroRows(rows) puts identity
+// rows on Dirichlet DOFs, columns kept (solver.py:489-490).
+//
+// Here every (cell, a, b) node pair is one contribution; a device radix sort of
+// the 64-bit keys row_node * nnode + col_node (stable, so equal keys keep
+// ascending contribution = ascending cell order) gives the unique node-pair
+// blocks in CSR order.  For vec components every node pair is a dense vec x vec
+// block of the scalar COO pattern, so the scalar CSR view (row 3n+i, columns
+// 3m+j over the sorted neighbours m) is exactly the sorted-unique (I, J).
+// Assembly then is a segmented reduction over each block's contribution list,
+// summed in cell order: deterministic, no float atomics.
+#include "common.hpp"
+#include "hipfem.h"
+#include <hipcub/hipcub.hpp>
+
+struct hf_pattern {
+  int64_t nnode, ncell, nblocks, ncontrib;
+  int nn, vec;
+  int32_t* rowptr;   // nnode+1 block-row pointer
+  int32_t* colidx;   // nblocks
+  int32_t* seg;      // nblocks+1 start of each block's contribution segment
+  int32_t* contrib;  // ncontrib sorted contribution ids (cell*nn + a)*nn + b
+  int32_t* diag;     // nnode diagonal block index (-1 for a node in no cell)
+  int32_t* trans;    // nblocks index of the transposed block
+  int32_t* brow;     // nblocks block row
+};
+
+namespace hf {
+
+__global__ void make_keys(const int32_t* __restrict__ cells, int64_t ncell, int nn, int64_t nnode,
+                          uint64_t* __restrict__ keys, int32_t* __restrict__ ids) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)nn * nn;
+  if (k >= ncell * per) return;
+  const int64_t c = k / per;
+  const int r = (int)(k - c * per);
+  const int a = r / nn, b = r - (r / nn) * nn;
+  const uint64_t na = (uint64_t)cells[c * nn + a], nb = (uint64_t)cells[c * nn + b];
+  keys[k] = na * (uint64_t)nnode + nb;
+  ids[k] = (int32_t)k;
+}
+
+__global__ void head_flags(const uint64_t* __restrict__ keys, int64_t n, int32_t* __restrict__ flags) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  flags[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1 : 0;
+}
+
+__global__ void fill_blocks(const uint64_t* __restrict__ keys, const int32_t* __restrict__ scan,
+                            int64_t n, int64_t nnode, int32_t* __restrict__ seg,
+                            int32_t* __restrict__ colidx, int32_t* __restrict__ brow,
+                            int32_t* __restrict__ rowcount, int32_t* __restrict__ diag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (k != 0 && keys[k] == keys[k - 1]) return;
+  const int32_t b = scan[k] - 1;
+  const uint64_t key = keys[k];
+  const int32_t row = (int32_t)(key / (uint64_t)nnode), col = (int32_t)(key % (uint64_t)nnode);
+  seg[b] = (int32_t)k;
+  colidx[b] = col;
+  brow[b] = row;
+  atomicAdd(&rowcount[row], 1);  // integer count, order-independent
+  if (row == col) diag[row] = b;
+}
+
+__global__ void transpose_map(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                              const int32_t* __restrict__ brow, int64_t nblocks,
+                              int32_t* __restrict__ trans) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const int32_t r = brow[b], c = colidx[b];
+  int lo = rowptr[c], hi = rowptr[c + 1] - 1;
+  int found = -1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int v = colidx[mid];
+    if (v == r) { found = mid; break; }
+    if (v < r) lo = mid + 1; else hi = mid - 1;
+  }
+  trans[b] = found;
+}
+
+// scalar CSR view of a vec-block pattern
+__global__ void csr_view(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                         int64_t nnode, int vec, int32_t* __restrict__ srow, int32_t* __restrict__ scol) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // scalar row
+  const int64_t N = nnode * vec;
+  if (r > N) return;
+  if (r == N) { srow[N] = rowptr[nnode] * vec * vec; return; }
+  const int64_t n = r / vec;
+  const int i = (int)(r - n * vec);
+  const int32_t b0 = rowptr[n], b1 = rowptr[n + 1];
+  const int32_t len = (b1 - b0) * vec;
+  const int64_t start = (int64_t)b0 * vec * vec + (int64_t)i * len;
+  srow[r] = (int32_t)start;
+  for (int32_t b = b0; b < b1; ++b)
+    for (int j = 0; j < vec; ++j) scol[start + (int64_t)(b - b0) * vec + j] = colidx[b] * vec + j;
+}
+
+template <typename T>
+__global__ void bsr_to_csr_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ brow,
+                                  int64_t nblocks, int vec, const T* __restrict__ bsr, T* __restrict__ csr) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int bb = vec * vec;
+  if (t >= nblocks * bb) return;
+  const int64_t b = t / bb;
+  const int e = (int)(t - b * bb), i = e / vec, j = e - (e / vec) * vec;
+  const int32_t n = brow[b];
+  const int32_t b0 = rowptr[n], len = (rowptr[n + 1] - b0) * vec;
+  csr[(int64_t)b0 * bb + (int64_t)i * len + (b - b0) * vec + j] = bsr[t];
+}
+
+// One thread per (block, entry): sum the entry over the block's contributions
+// in cell order.  Element matrices are block-major (cell, a, b, i, k).
+template <typename T, int BB>
+__global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ seg,
+                                                       const int32_t* __restrict__ contrib,
+                                                       int64_t nblocks, const T* __restrict__ elem,
+                                                       T* __restrict__ bsr) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblocks * BB) return;
+  const int64_t b = t / BB;
+  const int e = (int)(t - b * BB);
+  const int32_t s0 = seg[b], s1 = seg[b + 1];
+  T acc = 0;
+  for (int32_t s = s0; s < s1; ++s) acc += elem[(int64_t)contrib[s] * BB + e];
+  bsr[t] = acc;
+}
+
+// R[n*vec+i] = sum over the cells holding node n (diag-block contributions,
+// cell order) of elem_res[(cell*nn + a)*vec + i]  (+ f_ext)
+template <int VEC>
+__global__ void gather_kernel(const int32_t* __restrict__ diag, const int32_t* __restrict__ seg,
+                              const int32_t* __restrict__ contrib, int64_t nnode, int nn,
+                              const double* __restrict__ elem, const double* __restrict__ fext,
+                              double* __restrict__ R) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  double acc[VEC];
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) acc[i] = fext ? fext[n * VEC + i] : 0.0;
+  const int32_t b = diag[n];
+  if (b >= 0) {
+    for (int32_t s = seg[b]; s < seg[b + 1]; ++s) {
+      const int64_t ca = contrib[s] / nn;  // = cell*nn + a  (contribution (cell,a,a))
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) acc[i] += elem[ca * VEC + i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) R[n * VEC + i] = acc[i];
+}
+
+template <typename T>
+__global__ void bc_rows_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ diag,
+                               const int32_t* __restrict__ dofs, int64_t nbc, int vec,
+                               T* __restrict__ bsr) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbc) return;
+  const int32_t d = dofs[t];
+  const int32_t n = d / vec, i = d - (d / vec) * vec;
+  const int bb = vec * vec;
+  for (int32_t b = rowptr[n]; b < rowptr[n + 1]; ++b)
+    for (int k = 0; k < vec; ++k) bsr[(int64_t)b * bb + i * vec + k] = (T)0;
+  if (diag[n] >= 0) bsr[(int64_t)diag[n] * bb + i * vec + i] = (T)1;
+}
+
+__global__ void bc_residual_kernel(const int32_t* __restrict__ dofs, const double* __restrict__ vals,
+                                   int64_t nbc, double scale, const double* __restrict__ u,
+                                   double* __restrict__ R) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbc) return;
+  const int32_t d = dofs[t];
+  R[d] = u[d] - vals[t] * scale;
+}
+
+__global__ void bc_x0_kernel(const int32_t* __restrict__ dofs, const double* __restrict__ vals,
+                             int64_t nbc, const double* __restrict__ u, double* __restrict__ x0) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbc) return;
+  const int32_t d = dofs[t];
+  x0[d] = vals[t] - u[d];
+}
+
+template <typename T>
+__global__ void jacobi_kernel(const int32_t* __restrict__ diag, int64_t nnode, int vec,
+                              const T* __restrict__ bsr, T* __restrict__ dinv) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nnode * vec) return;
+  const int64_t n = r / vec;
+  const int i = (int)(r - n * vec);
+  const int32_t b = diag[n];
+  dinv[r] = b >= 0 ? (T)1 / bsr[(int64_t)b * vec * vec + i * vec + i] : (T)1;
+}
+
+}  // namespace hf
+
+using namespace hf;
+
+static inline unsigned grid_for(int64_t n, int b = 256) { return (unsigned)ceil_div(n, b); }
+
+extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn, int64_t nnode,
+                                int32_t vec, hf_stream_t stream_, hf_pattern** out) {
+  hipStream_t stream = (hipStream_t)stream_;
+  HF_REQUIRE(out != nullptr && cells != nullptr, "hf_pattern_build: null argument");
+  HF_REQUIRE(ncell > 0 && nnode > 0 && nn > 0 && vec > 0, "hf_pattern_build: empty mesh");
+  const int64_t n = ncell * (int64_t)nn * nn;
+  HF_REQUIRE(n < (int64_t)INT32_MAX, "hf_pattern_build: %lld contributions exceed int32", (long long)n);
+  HF_REQUIRE(nnode * vec < (int64_t)INT32_MAX, "hf_pattern_build: too many DOFs");
+  hf_pattern* p = new hf_pattern();
+  p->nnode = nnode; p->ncell = ncell; p->nn = nn; p->vec = vec; p->ncontrib = n;
+  uint64_t *keys = nullptr, *keys_s = nullptr;
+  int32_t *ids = nullptr, *flags = nullptr, *scan = nullptr, *rowcount = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0, tmp2 = 0;
+  int end_bit = 1;
+  while (end_bit < 64 && ((uint64_t)1 << end_bit) < (uint64_t)nnode * (uint64_t)nnode) ++end_bit;
+#define PB_CHECK(expr)                                                                    \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess) {                                                               \
+      set_error("hf_pattern_build: %s -> %s", #expr, hipGetErrorString(_e));              \
+      hipFree(keys); hipFree(keys_s); hipFree(ids); hipFree(flags); hipFree(scan);        \
+      hipFree(rowcount); hipFree(tmp);                                                    \
+      hf_pattern_destroy(p);                                                              \
+      return -2;                                                                          \
+    }                                                                                     \
+  } while (0)
+  p->rowptr = p->colidx = p->seg = p->contrib = p->diag = p->trans = p->brow = nullptr;
+  PB_CHECK(hipMalloc(&keys, n * sizeof(uint64_t)));
+  PB_CHECK(hipMalloc(&keys_s, n * sizeof(uint64_t)));
+  PB_CHECK(hipMalloc(&ids, n * sizeof(int32_t)));
+  PB_CHECK(hipMalloc(&p->contrib, n * sizeof(int32_t)));
+  make_keys<<<grid_for(n), 256, 0, stream>>>(cells, ncell, nn, nnode, keys, ids);
+  PB_CHECK(hipGetLastError());
+  PB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_s, ids, p->contrib, (int)n, 0,
+                                              end_bit, stream));
+  PB_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tmp2, flags, scan, (int)n, stream));
+  if (tmp2 > tmp_bytes) tmp_bytes = tmp2;
+  PB_CHECK(hipMalloc(&tmp, tmp_bytes));
+  PB_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_s, ids, p->contrib, (int)n, 0,
+                                              end_bit, stream));
+  hipFree(keys); keys = nullptr;
+  hipFree(ids); ids = nullptr;
+  PB_CHECK(hipMalloc(&flags, n * sizeof(int32_t)));
+  PB_CHECK(hipMalloc(&scan, n * sizeof(int32_t)));
+  head_flags<<<grid_for(n), 256, 0, stream>>>(keys_s, n, flags);
+  PB_CHECK(hipGetLastError());
+  PB_CHECK(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, flags, scan, (int)n, stream));
+  int32_t nb = 0;
+  PB_CHECK(hipMemcpyAsync(&nb, scan + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+  PB_CHECK(hipStreamSynchronize(stream));
+  p->nblocks = nb;
+  PB_CHECK(hipMalloc(&p->seg, (nb + 1) * sizeof(int32_t)));
+  PB_CHECK(hipMalloc(&p->colidx, nb * sizeof(int32_t)));
+  PB_CHECK(hipMalloc(&p->brow, nb * sizeof(int32_t)));
+  PB_CHECK(hipMalloc(&p->trans, nb * sizeof(int32_t)));
+  PB_CHECK(hipMalloc(&p->diag, nnode * sizeof(int32_t)));
+  PB_CHECK(hipMalloc(&p->rowptr, (nnode + 1) * sizeof(int32_t)));
+  PB_CHECK(hipMalloc(&rowcount, (nnode + 1) * sizeof(int32_t)));
+  PB_CHECK(hipMemsetAsync(rowcount, 0, (nnode + 1) * sizeof(int32_t), stream));
+  PB_CHECK(hipMemsetAsync(p->diag, 0xff, nnode * sizeof(int32_t), stream));
+  fill_blocks<<<grid_for(n), 256, 0, stream>>>(keys_s, scan, n, nnode, p->seg, p->colidx, p->brow, rowcount,
+                                                 p->diag);
+  PB_CHECK(hipGetLastError());
+  const int32_t nn32 = (int32_t)n;
+  PB_CHECK(hipMemcpyAsync(p->seg + nb, &nn32, sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  PB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, rowcount, p->rowptr, (int)(nnode + 1), stream));
+  if (tmp2 > tmp_bytes) {
+    hipFree(tmp); tmp = nullptr;
+    tmp_bytes = tmp2;
+    PB_CHECK(hipMalloc(&tmp, tmp_bytes));
+  }
+  PB_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, rowcount, p->rowptr, (int)(nnode + 1), stream));
+  transpose_map<<<grid_for(nb), 256, 0, stream>>>(p->rowptr, p->colidx, p->brow, nb, p->trans);
+  PB_CHECK(hipGetLastError());
+  PB_CHECK(hipStreamSynchronize(stream));
+  hipFree(keys_s); hipFree(flags); hipFree(scan); hipFree(rowcount); hipFree(tmp);
+#undef PB_CHECK
+  *out = p;
+  return 0;
+}
+
+extern "C" int hf_pattern_sizes(const hf_pattern* p, int64_t* nblocks, int64_t* nnz, int64_t* ncontrib) {
+  HF_REQUIRE(p != nullptr, "null pattern");
+  if (nblocks) *nblocks = p->nblocks;
+  if (nnz) *nnz = p->nblocks * p->vec * p->vec;
+  if (ncontrib) *ncontrib = p->ncontrib;
+  return 0;
+}
+
+extern "C" int hf_pattern_arrays(const hf_pattern* p, const int32_t** rowptr, const int32_t** colidx,
+                                 const int32_t** diag, const int32_t** trans) {
+  HF_REQUIRE(p != nullptr, "null pattern");
+  if (rowptr) *rowptr = p->rowptr;
+  if (colidx) *colidx = p->colidx;
+  if (diag) *diag = p->diag;
+  if (trans) *trans = p->trans;
+  return 0;
+}
+
+extern "C" int hf_pattern_csr(const hf_pattern* p, int32_t* rowptr, int32_t* colind, hf_stream_t s) {
+  HF_REQUIRE(p && rowptr && colind, "hf_pattern_csr: null argument");
+  HF_REQUIRE(p->nblocks * p->vec * p->vec < (int64_t)INT32_MAX, "hf_pattern_csr: nnz exceeds int32");
+  csr_view<<<grid_for(p->nnode * p->vec + 1), 256, 0, (hipStream_t)s>>>(p->rowptr, p->colidx, p->nnode, p->vec,
+                                                                          rowptr, colind);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int hf_pattern_destroy(hf_pattern* p) {
+  if (!p) return 0;
+  hipFree(p->rowptr); hipFree(p->colidx); hipFree(p->seg); hipFree(p->contrib);
+  hipFree(p->diag); hipFree(p->trans); hipFree(p->brow);
+  delete p;
+  return 0;
+}
+
+extern "C" int hf_bsr_to_csr_f64(const hf_pattern* p, const double* bsr, double* csr, hf_stream_t s) {
+  HF_REQUIRE(p && bsr && csr, "hf_bsr_to_csr_f64: null argument");
+  const int64_t n = p->nblocks * p->vec * p->vec;
+  bsr_to_csr_kernel<double><<<grid_for(n), 256, 0, (hipStream_t)s>>>(p->rowptr, p->brow, p->nblocks, p->vec,
+                                                                        bsr, csr);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+static int assemble_impl(const hf_pattern* p, const T* elem, T* bsr, hipStream_t s) {
+  HF_REQUIRE(p && elem && bsr, "hf_assemble_bsr: null argument");
+  const int64_t n = p->nblocks * p->vec * p->vec;
+  if (p->vec == 3)
+    assemble_kernel<T, 9><<<grid_for(n), 256, 0, s>>>(p->seg, p->contrib, p->nblocks, elem, bsr);
+  else if (p->vec == 1)
+    assemble_kernel<T, 1><<<grid_for(n), 256, 0, s>>>(p->seg, p->contrib, p->nblocks, elem, bsr);
+  else
+    HF_REQUIRE(false, "hf_assemble_bsr: vec=%d not supported", p->vec);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+extern "C" int hf_assemble_bsr_f64(const hf_pattern* p, const double* e, double* b, hf_stream_t s) {
+  return assemble_impl<double>(p, e, b, (hipStream_t)s);
+}
+extern "C" int hf_assemble_bsr_f32(const hf_pattern* p, const float* e, float* b, hf_stream_t s) {
+  return assemble_impl<float>(p, e, b, (hipStream_t)s);
+}
+
+extern "C" int hf_gather_residual_f64(const hf_pattern* p, const double* elem, const double* fext, double* R,
+                                      hf_stream_t s) {
+  HF_REQUIRE(p && elem && R, "hf_gather_residual_f64: null argument");
+  if (p->vec == 3)
+    gather_kernel<3><<<grid_for(p->nnode), 256, 0, (hipStream_t)s>>>(p->diag, p->seg, p->contrib, p->nnode, p->nn,
+                                                                       elem, fext, R);
+  else if (p->vec == 1)
+    gather_kernel<1><<<grid_for(p->nnode), 256, 0, (hipStream_t)s>>>(p->diag, p->seg, p->contrib, p->nnode, p->nn,
+                                                                       elem, fext, R);
+  else
+    HF_REQUIRE(false, "vec=%d not supported", p->vec);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+static int bc_rows_impl(const hf_pattern* p, const int32_t* dofs, int64_t nbc, T* bsr, hipStream_t s) {
+  HF_REQUIRE(p && bsr, "hf_bc_rows: null argument");
+  if (nbc == 0) return 0;
+  bc_rows_kernel<T><<<grid_for(nbc), 256, 0, s>>>(p->rowptr, p->diag, dofs, nbc, p->vec, bsr);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+extern "C" int hf_bc_rows_f64(const hf_pattern* p, const int32_t* d, int64_t n, double* b, hf_stream_t s) {
+  return bc_rows_impl<double>(p, d, n, b, (hipStream_t)s);
+}
+extern "C" int hf_bc_rows_f32(const hf_pattern* p, const int32_t* d, int64_t n, float* b, hf_stream_t s) {
+  return bc_rows_impl<float>(p, d, n, b, (hipStream_t)s);
+}
+
+extern "C" int hf_bc_residual_f64(const int32_t* dofs, const double* vals, int64_t nbc, double scale,
+                                  const double* u, double* R, hf_stream_t s) {
+  if (nbc == 0) return 0;
+  bc_residual_kernel<<<grid_for(nbc), 256, 0, (hipStream_t)s>>>(dofs, vals, nbc, scale, u, R);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int hf_bc_initial_guess_f64(const int32_t* dofs, const double* vals, int64_t nbc, const double* u,
+                                       double* x0, int64_t N, hf_stream_t s) {
+  HF_CHECK_HIP(hipMemsetAsync(x0, 0, N * sizeof(double), (hipStream_t)s));
+  if (nbc == 0) return 0;
+  bc_x0_kernel<<<grid_for(nbc), 256, 0, (hipStream_t)s>>>(dofs, vals, nbc, u, x0);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+static int jacobi_impl(const hf_pattern* p, const T* bsr, T* dinv, hipStream_t s) {
+  HF_REQUIRE(p && bsr && dinv, "hf_jacobi: null argument");
+  jacobi_kernel<T><<<grid_for(p->nnode * p->vec), 256, 0, s>>>(p->diag, p->nnode, p->vec, bsr, dinv);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+extern "C" int hf_jacobi_f64(const hf_pattern* p, const double* b, double* d, hf_stream_t s) {
+  return jacobi_impl<double>(p, b, d, (hipStream_t)s);
+}
+extern "C" int hf_jacobi_f32(const hf_pattern* p, const float* b, float* d, hf_stream_t s) {
+  return jacobi_impl<float>(p, b, d, (hipStream_t)s);
+}
+
+// internal accessors for the Krylov module
+namespace hf {
+const int32_t* pattern_rowptr(const hf_pattern* p) { return p->rowptr; }
+const int32_t* pattern_colidx(const hf_pattern* p) { return p->colidx; }
+const int32_t* pattern_trans(const hf_pattern* p) { return p->trans; }
+int64_t pattern_nnode(const hf_pattern* p) { return p->nnode; }
+int pattern_vec(const hf_pattern* p) { return p->vec; }
+}  // namespace hf

@@ -1,0 +1,338 @@
+// HEX8 cell kernels: element residual, element Jacobian, parameter VJP.
+//
+// Math (reference: jax_fem/problem.py:189-214 laplace kernel, :262-266 jacfwd;
+// jax_fem/fe.py:112-141 geometry):
+//   J = sum_n x_n (x) dN_n/dxi, g_n = dN_n/dxi . J^-1, w = det(J) * w_ref
+//   u_grad[i][j] = sum_n u[n][i] g_n[j],  r[n][i] = sum_q w sum_j P[i][j] g_n[j]
+//   K[a i][b k]  = sum_q w g_a . A[i,:,k,:] . g_b, contracted in closed form:
+//     linear elastic  w (lam g_a[i] g_b[k] + mu (d_ik g_a.g_b + g_a[k] g_b[i]))
+//     Neo-Hookean     (H = F^-T, c = J^-2/3, s1 = mu c w, s2 = kappa w)
+//        alpha_i Hg_b[k] + beta_i Fg_b[k] + Hg_b[i] gamma_k + d_ik s1 g_a.g_b
+//        alpha = s1 (-2/3 Fg_a + 2/9 I1 Hg_a) + s2 (2J-1) J Hg_a
+//        beta  = -2/3 s1 Hg_a,  gamma = (s1 I1/3 - s2 J (J-1)) Hg_a
+// Geometry is recomputed per quadrature point from the node coordinates (no
+// stored shape gradients: 0 bytes of per-cell geometry traffic).
+//
+// Thread mapping (wave64):
+//   residual / param VJP: one thread per cell (24 DOF in registers).
+//   Jacobian: one thread per (cell, row node a): 8 consecutive lanes hold one
+//   cell, each writes its 8 x 3 x 3 row of blocks = 288 contiguous bytes (FP32);
+//   the 8 lanes of a cell write one contiguous 2304-byte element matrix.
+#include "common.hpp"
+#include "element_math.hpp"
+
+namespace hf {
+
+// Reference gradient of mesh-order HEX8 node n at Gauss point q along d.
+// Node reference coordinates in VTK order; Gauss points (1 -+ 1/sqrt3)/2 with
+// quad index q = (i*2 + j)*2 + k (x from i) as in the oracle's tensor rule.
+__host__ __device__ constexpr int hex8_node_bit(int n, int d) {
+  // x: 0,1,1,0,0,1,1,0   y: 0,0,1,1,0,0,1,1   z: 0,0,0,0,1,1,1,1
+  return d == 0 ? ((n & 3) == 1 || (n & 3) == 2) : d == 1 ? ((n & 3) >= 2) : (n >= 4);
+}
+__host__ __device__ constexpr double hex8_gauss(int i) {
+  return i == 0 ? 0.21132486540518711775 : 0.78867513459481288225;
+}
+__host__ __device__ constexpr int hex8_qbit(int q, int d) {
+  return d == 0 ? (q >> 2) & 1 : d == 1 ? (q >> 1) & 1 : q & 1;
+}
+__host__ __device__ constexpr double hex8_l(int bit, double t) { return bit ? t : 1.0 - t; }
+__host__ __device__ constexpr double hex8_dl(int bit) { return bit ? 1.0 : -1.0; }
+__host__ __device__ constexpr double hex8_dN(int q, int n, int d) {
+  double v = 1.0;
+  for (int e = 0; e < 3; ++e) {
+    const int b = hex8_node_bit(n, e);
+    const double t = hex8_gauss(hex8_qbit(q, e));
+    v *= (e == d) ? hex8_dl(b) : hex8_l(b, t);
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Per-quadrature geometry: physical gradients g[n][3] and weight w.
+template <typename T>
+__device__ __forceinline__ void hex8_geometry(int q, const T X[8][3], T g[8][3], T& w) {
+  T Jm[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      T s = 0;
+#pragma unroll
+      for (int n = 0; n < 8; ++n) s += X[n][d] * (T)hex8_dN(q, n, e);
+      Jm[d][e] = s;
+    }
+  T inv[3][3];
+  const T det = inv3(Jm, inv);  // inv = (dxi/dx)
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      T s = 0;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) s += (T)hex8_dN(q, n, e) * inv[e][j];
+      g[n][j] = s;
+    }
+  w = det * (T)0.125;
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void load_cell(const double* __restrict__ pts, const int32_t* __restrict__ cells,
+                                          const double* __restrict__ u, int64_t c, T X[8][3], T U[8][VEC]) {
+  const int4* cp = reinterpret_cast<const int4*>(cells + c * 8);
+  const int4 c0 = cp[0], c1 = cp[1];
+  const int nid[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) X[n][d] = (T)pts[3 * (int64_t)nid[n] + d];
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) U[n][i] = (T)u[VEC * (int64_t)nid[n] + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, int MAT, int VEC>
+__global__ void __launch_bounds__(256) hex8_residual_kernel(
+    const double* __restrict__ pts, const int32_t* __restrict__ cells, int64_t ncell,
+    const double* __restrict__ u, const double* __restrict__ params, hf_material mat,
+    T* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  T X[8][3], U[8][VEC];
+  load_cell<T, VEC>(pts, cells, u, c, X, U);
+  T r[8][VEC];
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) r[n][i] = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    T g[8][3], w;
+    hex8_geometry<T>(q, X, g, w);
+    T ug[VEC][3];
+    grad_u<T, 8, VEC>(U, g, ug);
+    Moduli md = moduli(mat, params ? params[c * 8 + q] : 1.0, params != nullptr);
+    T P[VEC][3];
+    material_stress<T, MAT, VEC>(ug, (T)md.m1, (T)md.m2, P);
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+#pragma unroll
+      for (int i = 0; i < VEC; ++i)
+        r[n][i] += w * (P[i][0] * g[n][0] + P[i][1] * g[n][1] + P[i][2] * g[n][2]);
+  }
+  T* o = out + c * (8 * VEC);
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) o[n * VEC + i] = r[n][i];
+}
+
+// One thread per (cell, a).  Output block-major (cell, a, b, i, k).
+template <typename T, int MAT, int VEC>
+__global__ void __launch_bounds__(256) hex8_jacobian_kernel(
+    const double* __restrict__ pts, const int32_t* __restrict__ cells, int64_t ncell,
+    const double* __restrict__ u, const double* __restrict__ params, hf_material mat,
+    T* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = t >> 3;
+  const int a = (int)(t & 7);
+  if (c >= ncell) return;
+  T X[8][3], U[8][VEC];
+  load_cell<T, VEC>(pts, cells, u, c, X, U);
+  T K[8][VEC][VEC];
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+#pragma unroll
+    for (int i = 0; i < VEC; ++i)
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) K[b][i][k] = 0;
+
+#pragma unroll 1
+  for (int q = 0; q < 8; ++q) {
+    T g[8][3], w;
+    hex8_geometry<T>(q, X, g, w);
+    Moduli md = moduli(mat, params ? params[c * 8 + q] : 1.0, params != nullptr);
+    T ga[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ga[j] = select8(g, a, j);
+    if constexpr (MAT == MAT_NEO_HOOKEAN) {
+      T ug[3][3];
+      grad_u<T, 8, 3>(U, g, ug);
+      NHState<T> s;
+      nh_state<T>(ug, s);
+      const T s1 = (T)md.m1 * s.c * w, s2 = (T)md.m2 * w;
+      T Fga[3], Hga[3];
+      matvec3(s.F, ga, Fga);
+      matvec3(s.H, ga, Hga);
+      T al[3], be[3], ga_[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        al[i] = s1 * ((T)(-2.0 / 3.0) * Fga[i] + (T)(2.0 / 9.0) * s.I1 * Hga[i]) +
+                s2 * ((T)2 * s.J - (T)1) * s.J * Hga[i];
+        be[i] = (T)(-2.0 / 3.0) * s1 * Hga[i];
+        ga_[i] = (s1 * s.I1 * (T)(1.0 / 3.0) - s2 * s.J * (s.J - (T)1)) * Hga[i];
+      }
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        T Fgb[3], Hgb[3];
+        matvec3(s.F, g[b], Fgb);
+        matvec3(s.H, g[b], Hgb);
+        const T gab = s1 * (ga[0] * g[b][0] + ga[1] * g[b][1] + ga[2] * g[b][2]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            K[b][i][k] += al[i] * Hgb[k] + be[i] * Fgb[k] + Hgb[i] * ga_[k] + (i == k ? gab : (T)0);
+      }
+    } else if constexpr (MAT == MAT_LINEAR_ELASTIC) {
+      const T lw = (T)md.m2 * w, mw = (T)md.m1 * w;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const T gab = mw * (ga[0] * g[b][0] + ga[1] * g[b][1] + ga[2] * g[b][2]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            K[b][i][k] += lw * ga[i] * g[b][k] + mw * ga[k] * g[b][i] + (i == k ? gab : (T)0);
+      }
+    } else {  // Poisson, VEC == 1
+      const T kw = (T)md.m1 * w;
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+        K[b][0][0] += kw * (ga[0] * g[b][0] + ga[1] * g[b][1] + ga[2] * g[b][2]);
+    }
+  }
+  T* o = out + (c * 8 + a) * (8 * VEC * VEC);
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+#pragma unroll
+    for (int i = 0; i < VEC; ++i)
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) o[(b * VEC + i) * VEC + k] = K[b][i][k];
+}
+
+// grad[c,q] = - w sum_ij (sum_a lam[a][i] g_a[j]) dP_ij/dp
+template <int MAT, int VEC>
+__global__ void __launch_bounds__(256) hex8_param_vjp_kernel(
+    const double* __restrict__ pts, const int32_t* __restrict__ cells, int64_t ncell,
+    const double* __restrict__ u, const double* __restrict__ params, hf_material mat,
+    const double* __restrict__ lam, double* __restrict__ grad) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double X[8][3], U[8][VEC], L[8][VEC];
+  load_cell<double, VEC>(pts, cells, u, c, X, U);
+  {
+    const int32_t* cp = cells + c * 8;
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) L[n][i] = lam[VEC * (int64_t)cp[n] + i];
+  }
+#pragma unroll 1
+  for (int q = 0; q < 8; ++q) {
+    double g[8][3], w;
+    hex8_geometry<double>(q, X, g, w);
+    double ug[VEC][3], lg[VEC][3];
+    grad_u<double, 8, VEC>(U, g, ug);
+    grad_u<double, 8, VEC>(L, g, lg);
+    Moduli md = moduli(mat, params[c * 8 + q], true);
+    double P1[VEC][3], P2[VEC][3];
+    material_stress_parts<double, MAT, VEC>(ug, P1, P2);
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s += lg[i][j] * (md.d1 * P1[i][j] + md.d2 * P2[i][j]);
+    grad[c * 8 + q] = -w * s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch
+template <typename T>
+static int launch_hex8_residual(const hf_material* mat, const double* pts, const int32_t* cells,
+                                int64_t ncell, const double* u, const double* params, T* out,
+                                hipStream_t s) {
+  if (ncell == 0) return 0;
+  const dim3 blk(256), grd((unsigned)ceil_div(ncell, 256));
+  switch (mat->kind) {
+    case MAT_POISSON:
+      hex8_residual_kernel<T, MAT_POISSON, 1><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      break;
+    case MAT_LINEAR_ELASTIC:
+      hex8_residual_kernel<T, MAT_LINEAR_ELASTIC, 3><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      break;
+    case MAT_NEO_HOOKEAN:
+      hex8_residual_kernel<T, MAT_NEO_HOOKEAN, 3><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      break;
+    default:
+      HF_REQUIRE(false, "unknown material kind %d", mat->kind);
+  }
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T>
+static int launch_hex8_jacobian(const hf_material* mat, const double* pts, const int32_t* cells,
+                                int64_t ncell, const double* u, const double* params, T* out,
+                                hipStream_t s) {
+  if (ncell == 0) return 0;
+  const dim3 blk(256), grd((unsigned)ceil_div(ncell * 8, 256));
+  switch (mat->kind) {
+    case MAT_POISSON:
+      hex8_jacobian_kernel<T, MAT_POISSON, 1><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      break;
+    case MAT_LINEAR_ELASTIC:
+      hex8_jacobian_kernel<T, MAT_LINEAR_ELASTIC, 3><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      break;
+    case MAT_NEO_HOOKEAN:
+      hex8_jacobian_kernel<T, MAT_NEO_HOOKEAN, 3><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      break;
+    default:
+      HF_REQUIRE(false, "unknown material kind %d", mat->kind);
+  }
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+int hex8_param_vjp(const hf_material* mat, const double* pts, const int32_t* cells, int64_t ncell,
+                   const double* u, const double* params, const double* lam, double* grad,
+                   hipStream_t s) {
+  if (ncell == 0) return 0;
+  const dim3 blk(256), grd((unsigned)ceil_div(ncell, 256));
+  switch (mat->kind) {
+    case MAT_POISSON:
+      hex8_param_vjp_kernel<MAT_POISSON, 1><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, lam, grad);
+      break;
+    case MAT_LINEAR_ELASTIC:
+      hex8_param_vjp_kernel<MAT_LINEAR_ELASTIC, 3><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, lam, grad);
+      break;
+    case MAT_NEO_HOOKEAN:
+      hex8_param_vjp_kernel<MAT_NEO_HOOKEAN, 3><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, lam, grad);
+      break;
+    default:
+      HF_REQUIRE(false, "unknown material kind %d", mat->kind);
+  }
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+int hex8_residual_f64(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                      const double* prm, double* o, hipStream_t s) {
+  return launch_hex8_residual<double>(m, p, c, n, u, prm, o, s);
+}
+int hex8_residual_f32(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                      const double* prm, float* o, hipStream_t s) {
+  return launch_hex8_residual<float>(m, p, c, n, u, prm, o, s);
+}
+int hex8_jacobian_f64(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                      const double* prm, double* o, hipStream_t s) {
+  return launch_hex8_jacobian<double>(m, p, c, n, u, prm, o, s);
+}
+int hex8_jacobian_f32(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                      const double* prm, float* o, hipStream_t s) {
+  return launch_hex8_jacobian<float>(m, p, c, n, u, prm, o, s);
+}
+
+}  // namespace hf

@@ -1,0 +1,214 @@
+"""Set-up side of a finite-element variable (host, once per mesh / load step).
+
+Mirrors ``jax_fem.fe.FiniteElement`` (fe.py:20-322) for HEX8 and HEX27:
+reference tables on the [0,1]^3 cell (basis.py:141-250), Dirichlet index/value
+extraction (fe.py:220-269), boundary-face selection (fe.py:271-322), and the
+face / volume load vectors of u-independent surface and mass maps
+(problem.py:216-259).  None of this is on the solve path: the per-iteration
+work (cell kernels, assembly, Krylov) runs in libhipfem.
+"""
+import numpy as np
+
+from .mesh import HEX8_OFFSETS, HEX27_OFFSETS
+
+_RE_ORDER_NOTE = "mesh (VTK) node order; basis.py:53 / :60-61 map it onto basix"
+ELEMENTS = {
+    # nodes, degree, default quadrature order (basis.py:56, :64)
+    "HEX8": (HEX8_OFFSETS / 1.0, 1, 2),
+    "HEX27": (HEX27_OFFSETS / 2.0, 2, 10),
+}
+# basix hexahedron facets as mesh-order corner nodes and their outward normals
+# (facet order: z=0, y=0, x=0, x=1, y=1, z=1)
+FACES = np.array([[0, 1, 3, 2], [0, 1, 4, 5], [0, 3, 4, 7], [1, 2, 5, 6], [3, 2, 7, 6], [4, 5, 7, 6]])
+FACE_NORMALS = np.array([[0, 0, -1], [0, -1, 0], [-1, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
+
+
+def gauss_points(order):
+    """Gauss-Legendre on [0,1] with (order+2)//2 points, ascending; weights sum to 1."""
+    m = (order + 2) // 2
+    t, w = np.polynomial.legendre.leggauss(m)
+    return (t + 1.0) / 2.0, w / 2.0
+
+
+def lagrange(deg, t):
+    """1-D Lagrange values and derivatives through equispaced nodes on [0,1]: (len(t), deg+1)."""
+    t = np.asarray(t, dtype=np.float64)[:, None]
+    if deg == 1:
+        return np.hstack([1 - t, t]), np.hstack([-np.ones_like(t), np.ones_like(t)])
+    v = np.hstack([2 * (t - 0.5) * (t - 1), -4 * t * (t - 1), 2 * t * (t - 0.5)])
+    d = np.hstack([4 * t - 3, 4 - 8 * t, 4 * t - 1])
+    return v, d
+
+
+def tabulate(ele_type, X):
+    """Shape values (np, nn) and reference gradients (np, nn, 3) at points X (np, 3)."""
+    nodes, deg, _ = ELEMENTS[ele_type]
+    ijk = np.rint(nodes * deg).astype(int)
+    V, D = zip(*[lagrange(deg, X[:, c]) for c in range(3)])
+    vx, vy, vz = V[0][:, ijk[:, 0]], V[1][:, ijk[:, 1]], V[2][:, ijk[:, 2]]
+    dx, dy, dz = D[0][:, ijk[:, 0]], D[1][:, ijk[:, 1]], D[2][:, ijk[:, 2]]
+    return vx * vy * vz, np.stack([dx * vy * vz, vx * dy * vz, vx * vy * dz], axis=-1)
+
+
+def volume_rule(order):
+    t, w = gauss_points(order)
+    P = np.stack(np.meshgrid(t, t, t, indexing="ij"), -1).reshape(-1, 3)
+    W = np.einsum("i,j,k->ijk", w, w, w).reshape(-1)
+    return P, W
+
+
+def face_rule(order):
+    """Face quadrature points (6, nfq, 3) and weights (6, nfq) on the unit-cube facets."""
+    t, w = gauss_points(order)
+    s2 = np.stack(np.meshgrid(t, t, indexing="ij"), -1).reshape(-1, 2)
+    w2 = np.outer(w, w).reshape(-1)
+    corners = HEX8_OFFSETS.astype(np.float64)
+    pts = []
+    for f in FACES:
+        v0, v1, v2 = corners[f[0]], corners[f[1]], corners[f[2]]
+        pts.append(v0[None] + s2[:, :1] * (v1 - v0)[None] + s2[:, 1:] * (v2 - v0)[None])
+    return np.stack(pts), np.tile(w2, (6, 1))
+
+
+def _eval_points(fn, pts, extra=()):
+    """Evaluate a per-point function over many points.
+
+    Tries one call on the transposed batch (x[0] is then the vector of first
+    coordinates, which is what element-wise per-point code computes), verifies
+    it on a sample of single-point calls, and otherwise loops point by point."""
+    n = len(pts)
+    if n == 0:
+        return np.zeros((0,))
+    single = lambda i: np.asarray(fn(pts[i], *[e[i] for e in extra]), dtype=np.float64)
+    try:
+        out = np.asarray(fn(pts.T, *[np.asarray(e).T for e in extra]))
+        if out.dtype == object:
+            raise ValueError
+        out = np.asarray(out, dtype=np.float64)
+        ref0 = single(0)
+        if out.shape == ref0.shape:           # constant map
+            out = np.broadcast_to(out, (n,) + ref0.shape)
+        else:
+            out = np.moveaxis(out.reshape(ref0.shape + (n,)), -1, 0) if out.ndim else np.full(n, float(out))
+        for i in np.unique(np.linspace(0, n - 1, min(n, 7)).astype(int)):
+            if not np.allclose(out[i], single(i), rtol=0, atol=0):
+                raise ValueError
+        return np.ascontiguousarray(out)
+    except Exception:
+        return np.stack([single(i) for i in range(n)])
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _location_flags(fn, points):
+    nargs = fn.__code__.co_argcount
+    if nargs == 1:
+        f = lambda x: fn(x)
+        return _eval_points(f, points).astype(bool).reshape(-1)
+    if nargs == 2:
+        idx = np.arange(len(points))
+        return _eval_points(fn, points, (idx,)).astype(bool).reshape(-1)
+    raise ValueError("Wrong number of arguments for location_fn: must be 1 or 2, get %d" % nargs)
+
+
+class FiniteElement:
+    """One FE variable: mesh, tables, Dirichlet data, boundary faces (fe.py:20-110)."""
+
+    def __init__(self, mesh, vec, dim, ele_type="HEX8", quadrature_rule=None, quadrature_order=None,
+                 dirichlet_bc_info=None):
+        if ele_type not in ELEMENTS:
+            raise NotImplementedError("element %s (HEX8 and HEX27 are supported)" % ele_type)
+        if quadrature_rule not in (None, "default", "GLL") and quadrature_rule is not None:
+            raise NotImplementedError("quadrature_rule %r" % (quadrature_rule,))
+        self.mesh, self.vec, self.dim, self.ele_type = mesh, vec, dim, ele_type
+        self.points = np.asarray(mesh.points, dtype=np.float64)
+        self.cells = np.asarray(mesh.cells, dtype=np.int64)
+        self.num_cells = len(self.cells)
+        self.num_total_nodes = len(self.points)
+        self.num_total_dofs = self.num_total_nodes * vec
+        order = ELEMENTS[ele_type][2] if quadrature_order is None else quadrature_order
+        self.quadrature_order = order
+        qp, self.quad_weights = volume_rule(order)
+        self.shape_vals, self.shape_grads_ref = tabulate(ele_type, qp)
+        fp, self.face_quad_weights = face_rule(order)
+        nf, nfq, _ = fp.shape
+        fv, fg = tabulate(ele_type, fp.reshape(-1, 3))
+        self.face_shape_vals = fv.reshape(nf, nfq, -1)
+        self.face_shape_grads_ref = fg.reshape(nf, nfq, -1, 3)
+        self.face_normals = FACE_NORMALS
+        self.face_inds = FACES
+        self.num_quads, self.num_nodes = self.shape_vals.shape
+        self.num_faces, self.num_face_quads = nf, nfq
+        self.dirichlet_bc_info = dirichlet_bc_info
+        self.node_inds_list, self.vec_inds_list, self.vals_list = self.Dirichlet_boundary_conditions(
+            dirichlet_bc_info)
+
+    # -- Dirichlet data (fe.py:220-269) --------------------------------------
+    def Dirichlet_boundary_conditions(self, dirichlet_bc_info):
+        node_inds_list, vec_inds_list, vals_list = [], [], []
+        if dirichlet_bc_info is None:
+            return node_inds_list, vec_inds_list, vals_list
+        location_fns, vecs, value_fns = dirichlet_bc_info
+        assert len(location_fns) == len(value_fns) and len(value_fns) == len(vecs)
+        for loc, comp, val in zip(location_fns, vecs, value_fns):
+            nodes = np.flatnonzero(_location_flags(loc, self.points))
+            probe = val(self.points[nodes[0]]) if len(nodes) else 0.0
+            if _is_torch(probe):
+                # differentiable Dirichlet values (ad_wrapper through set_params)
+                import torch
+                values = torch.stack([torch.as_tensor(val(self.points[i]), dtype=torch.float64).reshape(())
+                                      for i in nodes.tolist()]) if len(nodes) else torch.zeros(0)
+            else:
+                values = _eval_points(val, self.points[nodes]).reshape(-1) if len(nodes) else np.zeros(0)
+                values = np.asarray(values, dtype=np.float64)
+            node_inds_list.append(nodes)
+            vec_inds_list.append(np.full(len(nodes), comp, dtype=np.int32))
+            vals_list.append(values)
+        return node_inds_list, vec_inds_list, vals_list
+
+    def update_Dirichlet_boundary_conditions(self, dirichlet_bc_info):
+        self.dirichlet_bc_info = dirichlet_bc_info
+        self.node_inds_list, self.vec_inds_list, self.vals_list = self.Dirichlet_boundary_conditions(
+            dirichlet_bc_info)
+
+    # -- boundary faces (fe.py:271-322) --------------------------------------
+    def get_boundary_conditions_inds(self, location_fns):
+        out = []
+        for loc in (location_fns or []):
+            flags = _location_flags(loc, self.points)
+            on = np.all(flags[self.cells[:, self.face_inds]], axis=-1)       # (ncell, 6)
+            out.append(np.argwhere(on))
+        return out
+
+    # -- geometry (fe.py:112-218), host FP64 ----------------------------------
+    def _jac(self, coords, grads_ref):
+        Jm = np.einsum("...nd,...ne->...de", coords, grads_ref)
+        return Jm, np.linalg.det(Jm), np.linalg.inv(Jm)
+
+    def get_shape_grads(self, points=None):
+        pts = self.points if points is None else points
+        X = pts[self.cells][:, None]                                         # (c,1,n,3)
+        _, det, inv = self._jac(X, self.shape_grads_ref[None])
+        grads = np.einsum("qnd,cqde->cqne", self.shape_grads_ref, inv)
+        return grads, det * self.quad_weights[None]
+
+    def JxW(self):
+        return self.get_shape_grads()[1]
+
+    def get_physical_quad_points(self):
+        return np.einsum("qn,cnd->cqd", self.shape_vals, self.points[self.cells])
+
+    def get_face_shape_grads(self, boundary_inds):
+        X = self.points[self.cells[boundary_inds[:, 0]]][:, None]
+        g = self.face_shape_grads_ref[boundary_inds[:, 1]]
+        _, det, inv = self._jac(X, g)
+        grads = np.einsum("sqnd,sqde->sqne", g, inv)
+        nJ = np.einsum("sd,sqde->sqe", self.face_normals[boundary_inds[:, 1]], inv)
+        nanson = np.linalg.norm(nJ, axis=-1) * det * self.face_quad_weights[boundary_inds[:, 1]]
+        return grads, nanson
+
+    def get_physical_surface_quad_points(self, boundary_inds):
+        return np.einsum("sqn,snd->sqd", self.face_shape_vals[boundary_inds[:, 1]],
+                         self.points[self.cells[boundary_inds[:, 0]]])

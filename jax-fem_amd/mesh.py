@@ -1,0 +1,69 @@
+"""Mesh container and structured box generators.
+
+``Mesh`` mirrors jax_fem/generate_mesh.py:13-27 (points (nnode, dim), cells
+(ncell, nn)).  ``box_mesh`` reproduces generate_mesh.py:151-189 (node index
+(i*(Ny+1)+j)*(Nz+1)+k, HEX8 corners in VTK order) and returns a ``Mesh``
+directly (the reference wraps it in a meshio object).  ``box_mesh_hex27`` is
+the structured HEX27 counterpart (the reference only has a gmsh path,
+generate_mesh.py:192-262): nodes on the doubled lattice, cell nodes in the
+VTK triquadratic order that the reference's HEX27 re_order maps to basix
+(basis.py:60-61).
+"""
+import numpy as np
+
+# VTK triquadratic-hexahedron node offsets on the doubled lattice (units of h/2):
+# 8 corners, 12 edge mid-points, 6 face centres (x-, x+, y-, y+, z-, z+), centre.
+HEX27_OFFSETS = np.array([
+    [0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0], [0, 0, 2], [2, 0, 2], [2, 2, 2], [0, 2, 2],
+    [1, 0, 0], [2, 1, 0], [1, 2, 0], [0, 1, 0], [1, 0, 2], [2, 1, 2], [1, 2, 2], [0, 1, 2],
+    [0, 0, 1], [2, 0, 1], [2, 2, 1], [0, 2, 1],
+    [0, 1, 1], [2, 1, 1], [1, 0, 1], [1, 2, 1], [1, 1, 0], [1, 1, 2], [1, 1, 1]])
+HEX8_OFFSETS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                         [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+
+
+class Mesh:
+    """points (nnode, dim) float64, cells (ncell, nn) int, optional ele_type."""
+
+    def __init__(self, points, cells, ele_type=None):
+        self.points = np.asarray(points, dtype=np.float64)
+        self.cells = np.asarray(cells)
+        self.ele_type = ele_type
+
+    def count_selected_faces(self, location_fn):
+        from .fe import FiniteElement
+        fe = FiniteElement(self, vec=1, dim=self.points.shape[1], ele_type=self.ele_type or "HEX8")
+        return len(fe.get_boundary_conditions_inds([location_fn])[0])
+
+
+def _lattice(n, L):
+    return np.linspace(0.0, L, n + 1)
+
+
+def box_mesh(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0):
+    """HEX8 box with Nx x Ny x Nz cells (generate_mesh.py:151-189)."""
+    X, Y, Z = np.meshgrid(_lattice(Nx, domain_x), _lattice(Ny, domain_y), _lattice(Nz, domain_z),
+                          indexing="ij")
+    points = np.stack((X, Y, Z), axis=-1).reshape(-1, 3)
+    idx = np.arange(len(points)).reshape(Nx + 1, Ny + 1, Nz + 1)
+    i, j, k = [a.reshape(-1) for a in np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")]
+    o = HEX8_OFFSETS
+    cells = idx[i[:, None] + o[None, :, 0], j[:, None] + o[None, :, 1], k[:, None] + o[None, :, 2]]
+    return Mesh(points, cells.astype(np.int64), "HEX8")
+
+
+def box_mesh_hex27(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0):
+    """HEX27 box with Nx x Ny x Nz cells on the (2N+1)^3 node lattice."""
+    X, Y, Z = np.meshgrid(_lattice(2 * Nx, domain_x), _lattice(2 * Ny, domain_y), _lattice(2 * Nz, domain_z),
+                          indexing="ij")
+    points = np.stack((X, Y, Z), axis=-1).reshape(-1, 3)
+    idx = np.arange(len(points)).reshape(2 * Nx + 1, 2 * Ny + 1, 2 * Nz + 1)
+    i, j, k = [a.reshape(-1) for a in np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")]
+    o = HEX27_OFFSETS
+    cells = idx[2 * i[:, None] + o[None, :, 0], 2 * j[:, None] + o[None, :, 1], 2 * k[:, None] + o[None, :, 2]]
+    return Mesh(points, cells.astype(np.int64), "HEX27")
+
+
+def get_meshio_cell_type(ele_type):
+    """Name of the cell block for an element type (generate_mesh.py helpers)."""
+    return {"HEX8": "hexahedron", "HEX27": "hexahedron27"}[ele_type]

@@ -1,0 +1,330 @@
+"""``Problem``: the reference's weak-form class, driving libhipfem on one GPU.
+
+Mirrors jax_fem/problem.py:15-611 for one variable (vec = 1 or 3, HEX8 or
+HEX27): same constructor, same user hooks (``get_tensor_map``,
+``get_mass_map``, ``get_surface_maps``, ``custom_init``, ``set_params``) and the
+engine methods ``compute_residual`` (:462) and ``newton_update`` (:477).
+
+Differences by design:
+  * ``get_tensor_map()`` returns a ``jax_fem_amd.materials`` law (callable like
+    the reference's map) so that the cell kernels know the law and its tangent;
+    a plain Python function raises ``NotImplementedError``.
+  * Mass and surface maps must not depend on u (they contribute a load vector
+    and no Jacobian); they are evaluated once per set-up or parameter change
+    (problem.py:216-259 evaluates them inside the element loop).
+  * ``newton_update`` assembles the tangent straight into a device BSR matrix
+    (3x3 node-pair blocks) instead of host COO values ``V``; ``problem.V``,
+    ``problem.I``, ``problem.J`` are still available on request (host copies).
+State on the device (torch tensors): points (nnode,3) f64, cells (ncell,nn)
+i32, per-quadrature parameters (ncell,nq) f64, Dirichlet DOFs/values, the load
+vector, the sparsity pattern (built once by a device radix sort) and the
+element/BSR buffers reused across Newton iterations.
+"""
+import ctypes
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _lib
+from .fe import FiniteElement, _eval_points
+from .materials import Material
+from .mesh import Mesh
+
+
+@dataclass
+class Problem:
+    mesh: Mesh
+    vec: int
+    dim: int
+    ele_type: str = "HEX8"
+    quadrature_rule: Any = None
+    quadrature_order: Any = None
+    dirichlet_bc_info: Any = None
+    location_fns: Any = None
+    additional_info: Any = ()
+    device: Any = None
+
+    def __post_init__(self):
+        if isinstance(self.mesh, list):
+            if len(self.mesh) != 1:
+                raise NotImplementedError("multi-variable problems are not supported by the GPU path")
+            self.mesh, self.vec, self.ele_type = self.mesh[0], self.vec[0], self.ele_type[0]
+            qo = self.quadrature_order
+            self.quadrature_order = qo[0] if isinstance(qo, list) else qo
+            dbc = self.dirichlet_bc_info
+            self.dirichlet_bc_info = dbc[0] if isinstance(dbc, list) and dbc and isinstance(dbc[0], list) \
+                and len(dbc) == 1 else dbc
+        if self.device is None:
+            self.device = torch.device("cuda", torch.cuda.current_device()) if torch.cuda.is_available() \
+                else torch.device("cpu")
+        self.fes = [FiniteElement(self.mesh, self.vec, self.dim, self.ele_type, self.quadrature_rule,
+                                  self.quadrature_order, self.dirichlet_bc_info)]
+        self.fe = self.fes[0]
+        self.num_vars = 1
+        self.cells_list = [self.fe.cells]
+        self.num_cells = self.fe.num_cells
+        self.offset = [0]
+        self.num_total_dofs_all_vars = self.fe.num_total_dofs
+        self.boundary_inds_list = self.fe.get_boundary_conditions_inds(self.location_fns)
+        self.internal_vars = ()
+        self.internal_vars_surfaces = [() for _ in self.boundary_inds_list]
+        self._pattern = None
+        self._bufs = {}
+        self._ext_key = None
+        self._bc_key = None
+        self.custom_init(*self.additional_info)
+        self._check_hooks()
+
+    # ------------------------------------------------------------------ hooks
+    def custom_init(self, *args):
+        pass
+
+    def set_params(self, params):
+        raise NotImplementedError("Child class must implement this function!")
+
+    def _check_hooks(self):
+        nsurf = len(self.boundary_inds_list)
+        if hasattr(self, "get_surface_maps"):
+            assert nsurf == len(self.get_surface_maps())
+        else:
+            assert nsurf == 0, "Missing definitions for surface integral"
+        if hasattr(self, "get_universal_kernel") or hasattr(self, "get_universal_kernels_surface"):
+            raise NotImplementedError("universal kernels have no HIP implementation yet")
+        if hasattr(self, "get_tensor_map"):
+            m = self.get_tensor_map()
+            if not isinstance(m, Material):
+                raise NotImplementedError(
+                    "get_tensor_map() must return a jax_fem_amd.materials law (LinearElastic, NeoHookean, "
+                    "Poisson); an arbitrary Python map has no HIP kernel")
+            if m.vec != self.vec:
+                raise ValueError("material expects vec=%d, problem has vec=%d" % (m.vec, self.vec))
+            self.material = m
+        else:
+            raise NotImplementedError("a get_tensor_map() law is required by the GPU path")
+
+    # ---------------------------------------------------------------- layout
+    def unflatten_fn_sol_list(self, dofs):
+        return [dofs.reshape(self.fe.num_total_nodes, self.vec)]
+
+    @property
+    def nn(self):
+        return self.fe.num_nodes
+
+    @property
+    def N(self):
+        return self.num_total_dofs_all_vars
+
+    # ---------------------------------------------------------- device state
+    def _dev(self):
+        if "points" not in self._bufs:
+            if self.device.type != "cuda":
+                raise RuntimeError("the jax_fem_amd solve path needs a GPU (torch.cuda is unavailable)")
+            _lib.load()
+            self._bufs["points"] = torch.as_tensor(self.fe.points, dtype=torch.float64, device=self.device).contiguous()
+            self._bufs["cells"] = torch.as_tensor(self.fe.cells, dtype=torch.int32, device=self.device).contiguous()
+        return self._bufs["points"], self._bufs["cells"]
+
+    @property
+    def pattern(self):
+        if self._pattern is None:
+            _, cells = self._dev()
+            h = ctypes.c_void_p()
+            _lib.call("hf_pattern_build", _lib.ptr(cells), self.num_cells, self.nn, self.fe.num_total_nodes,
+                      self.vec, _lib.stream(), ctypes.byref(h))
+            self._pattern = Pattern(h, self)
+        return self._pattern
+
+    def _params(self):
+        """Per-quadrature parameter (ncell, nq) f64 on the device, or None."""
+        iv = self.internal_vars
+        if iv is None or len(iv) == 0:
+            return None
+        if len(iv) > 1:
+            raise NotImplementedError("one per-quadrature parameter field is supported")
+        p = iv[0]
+        t = torch.as_tensor(p.detach() if torch.is_tensor(p) else p, dtype=torch.float64, device=self.device)
+        if t.dim() == 1 and t.shape[0] == self.num_cells:
+            t = t[:, None].expand(self.num_cells, self.fe.num_quads)
+        if tuple(t.shape) != (self.num_cells, self.fe.num_quads):
+            raise ValueError("internal_vars[0] must have shape (num_cells, num_quads)")
+        return t.contiguous()
+
+    def _buf(self, name, n, dtype):
+        b = self._bufs.get(name)
+        if b is None or b.numel() < n or b.dtype != dtype:
+            b = torch.empty(n, dtype=dtype, device=self.device)
+            self._bufs[name] = b
+        return b[:n]
+
+    # ------------------------------------------------------- Dirichlet data
+    def bc_arrays(self):
+        """Unique BC DOFs (int32, device) and values (f64, device); later specs win (solver.py:298-300)."""
+        fe = self.fe
+        key = tuple((id(v), len(v), float(v.detach().sum()) if torch.is_tensor(v) else float(np.sum(v)))
+                    for v in fe.vals_list) + tuple(id(n) for n in fe.node_inds_list)
+        if self._bc_key != key:
+            val = {}
+            for nodes, comps, vals in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list):
+                dofs = np.asarray(nodes, dtype=np.int64) * self.vec + np.asarray(comps, dtype=np.int64)
+                vals = vals.detach().cpu().numpy() if torch.is_tensor(vals) else vals
+                for d, v in zip(dofs.tolist(), np.asarray(vals, dtype=np.float64).reshape(-1).tolist()):
+                    val[d] = v
+            d = np.array(sorted(val), dtype=np.int32)
+            v = np.array([val[i] for i in d.tolist()], dtype=np.float64)
+            self._bc_dofs = torch.as_tensor(d, device=self.device)
+            self._bc_vals = torch.as_tensor(v, device=self.device)
+            self._bc_key = key
+        return self._bc_dofs, self._bc_vals
+
+    # ------------------------------------------------------------ load vector
+    def _external(self):
+        """u-independent mass and surface loads (problem.py:216-259), f64 device vector or None."""
+        key = (id(self.internal_vars_surfaces), id(self.internal_vars))
+        if self._ext_key == key:
+            return self._fext
+        fe = self.fe
+        f = np.zeros((fe.num_total_nodes, self.vec))
+        have = False
+        if hasattr(self, "get_mass_map"):
+            mmap = self.get_mass_map()
+            x = fe.get_physical_quad_points().reshape(-1, self.dim)
+            m = _eval_points(lambda xx: mmap(np.zeros(self.vec), xx), x).reshape(self.num_cells, fe.num_quads, -1)
+            JxW = fe.JxW()
+            r = np.einsum("cqi,qn,cq->cni", m, fe.shape_vals, JxW)
+            np.add.at(f, fe.cells.reshape(-1), r.reshape(-1, self.vec))
+            have = True
+        if hasattr(self, "get_surface_maps"):
+            for k, (bi, smap) in enumerate(zip(self.boundary_inds_list, self.get_surface_maps())):
+                if len(bi) == 0:
+                    continue
+                _, nanson = fe.get_face_shape_grads(bi)
+                xs = fe.get_physical_surface_quad_points(bi).reshape(-1, self.dim)
+                isv = [np.asarray(a).reshape(len(xs), -1) for a in (self.internal_vars_surfaces[k] or ())]
+                u0 = np.zeros((len(xs), self.vec))
+                t = _eval_points(lambda xx, uu, *a: smap(uu, xx, *a), xs, (u0, *isv)).reshape(len(bi), fe.num_face_quads, -1)
+                u1 = np.random.default_rng(0).standard_normal((len(xs), self.vec))
+                t1 = np.stack([np.asarray(smap(u1[i], xs[i], *[a[i] for a in isv]), dtype=np.float64)
+                               for i in range(min(3, len(xs)))])
+                if not np.allclose(t1, t.reshape(len(xs), -1)[:len(t1)]):
+                    raise NotImplementedError("surface maps that depend on u have no HIP kernel yet")
+                vals = fe.face_shape_vals[bi[:, 1]]
+                r = np.einsum("sqi,sqn,sq->sni", t, vals, nanson)
+                np.add.at(f, fe.cells[bi[:, 0]].reshape(-1), r.reshape(-1, self.vec))
+                have = True
+        self._fext = torch.as_tensor(f.reshape(-1), dtype=torch.float64, device=self.device) if have else None
+        self._ext_key = key
+        return self._fext
+
+    # -------------------------------------------------------------- engines
+    def residual_vec(self, u):
+        """Unconstrained residual R(u) (N,) f64 on the device: cells + loads (problem.py:439-445)."""
+        pts, cells = self._dev()
+        u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
+        er = self._buf("elem_res", self.num_cells * self.nn * self.vec, torch.float64)
+        mat = self.material.hf()
+        prm = self._params()
+        s = _lib.stream()
+        _lib.call("hf_cell_residual_f64", self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(er), s)
+        R = torch.empty(self.N, dtype=torch.float64, device=self.device)
+        _lib.call("hf_gather_residual_f64", self.pattern.handle, _lib.ptr(er), _lib.ptr(self._external()),
+                  _lib.ptr(R), s)
+        return R
+
+    def jacobian_bsr(self, u, dtype=torch.float64, bc=True):
+        """Tangent as BSR values (nblocks, vec, vec) flattened; Dirichlet rows -> identity rows."""
+        pts, cells = self._dev()
+        u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
+        p = self.pattern
+        bb = self.vec * self.vec
+        tag = "64" if dtype == torch.float64 else "32"
+        ej = self._buf("elem_jac" + tag, self.num_cells * self.nn * self.nn * bb, dtype)
+        mat = self.material.hf()
+        prm = self._params()
+        s = _lib.stream()
+        _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
+        vals = torch.empty(p.nblocks * bb, dtype=dtype, device=self.device)
+        _lib.call("hf_assemble_bsr_f" + tag, p.handle, _lib.ptr(ej), _lib.ptr(vals), s)
+        if bc:
+            dofs, _ = self.bc_arrays()
+            _lib.call("hf_bc_rows_f" + tag, p.handle, _lib.ptr(dofs), dofs.numel(), _lib.ptr(vals), s)
+        return vals
+
+    def element_jacobians(self, u, dtype=torch.float64):
+        """Element matrices (ncell, nd, nd), jac[r,c] = dR_r/du_c (problem.py:262-266)."""
+        pts, cells = self._dev()
+        u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
+        nn, v = self.nn, self.vec
+        ej = torch.empty(self.num_cells * nn * nn * v * v, dtype=dtype, device=self.device)
+        mat = self.material.hf()
+        tag = "64" if dtype == torch.float64 else "32"
+        _lib.call("hf_cell_jacobian_f" + tag, nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                  self.num_cells, _lib.ptr(u), _lib.ptr(self._params()), _lib.ptr(ej), _lib.stream())
+        # block-major (c, a, b, i, k) -> (c, a*v+i, b*v+k)
+        return ej.reshape(self.num_cells, nn, nn, v, v).permute(0, 1, 3, 2, 4).reshape(
+            self.num_cells, nn * v, nn * v)
+
+    # ------------------------------------------- reference-compatible methods
+    def compute_residual(self, sol_list):
+        u = sol_list[0] if isinstance(sol_list, (list, tuple)) else sol_list
+        return self.unflatten_fn_sol_list(self.residual_vec(torch.as_tensor(u)))
+
+    def newton_update(self, sol_list):
+        u = sol_list[0] if isinstance(sol_list, (list, tuple)) else sol_list
+        u = torch.as_tensor(u).reshape(-1)
+        self._newton_u = u.to(device=self.device, dtype=torch.float64).clone()
+        self._A_cache = None
+        return self.compute_residual([u])
+
+    @property
+    def V(self):
+        """Host COO values of the last newton_update: cells, then zero face blocks (problem.py:453-458)."""
+        K = self.element_jacobians(self._newton_u).reshape(-1).cpu().numpy()
+        nd = self.nn * self.vec
+        nface = sum(len(b) for b in self.boundary_inds_list) * nd * nd
+        return np.concatenate([K, np.zeros(nface)])
+
+    def _dof_inds(self, cells):
+        return (self.vec * cells[:, :, None] + np.arange(self.vec)[None, None, :]).reshape(len(cells), -1)
+
+    @property
+    def I(self):
+        inds = [self._dof_inds(self.fe.cells)] + [self._dof_inds(self.fe.cells[b[:, 0]]) for b in self.boundary_inds_list]
+        return np.concatenate([np.repeat(x[:, :, None], x.shape[1], axis=2).reshape(-1) for x in inds])
+
+    @property
+    def J(self):
+        inds = [self._dof_inds(self.fe.cells)] + [self._dof_inds(self.fe.cells[b[:, 0]]) for b in self.boundary_inds_list]
+        return np.concatenate([np.repeat(x[:, None, :], x.shape[1], axis=1).reshape(-1) for x in inds])
+
+
+class Pattern:
+    """Owner of an hf_pattern handle (destroyed with the Problem)."""
+
+    def __init__(self, handle, problem):
+        self.handle = handle
+        nb, nnz, nc = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("hf_pattern_sizes", handle, ctypes.byref(nb), ctypes.byref(nnz), ctypes.byref(nc))
+        self.nblocks, self.nnz, self.ncontrib = nb.value, nnz.value, nc.value
+        self.N = problem.N
+        self.vec = problem.vec
+        self.device = problem.device
+
+    def csr(self):
+        """Scalar CSR pattern (rowptr, colind) as device int32 tensors."""
+        rp = torch.empty(self.N + 1, dtype=torch.int32, device=self.device)
+        ci = torch.empty(self.nnz, dtype=torch.int32, device=self.device)
+        _lib.call("hf_pattern_csr", self.handle, _lib.ptr(rp), _lib.ptr(ci), _lib.stream())
+        return rp, ci
+
+    def __del__(self):
+        try:
+            if self.handle:
+                torch.cuda.synchronize()
+                _lib.load().hf_pattern_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

@@ -1,0 +1,466 @@
+"""Newton / Krylov / adjoint drivers on one GPU (drop-in for jax_fem/solver.py).
+
+Entry points with the reference's names and meaning:
+  * ``solver(problem, solver_options)`` (solver.py:1074-1318): Newton on the
+    row-eliminated system, loop while ||R||/||R0|| > rel_tol and ||R|| > tol.
+  * ``linear_solver(A, b, x0, linear_options)`` (solver.py:261-284).
+  * ``get_A(problem)`` (solver.py:502-515), ``apply_bc_vec`` / ``assign_bc`` /
+    ``copy_bc`` (solver.py:290-363).
+  * ``implicit_vjp`` / ``ad_wrapper`` (solver.py:1324-1417), the latter as a
+    ``torch.autograd.Function``.
+
+``solver_options`` keeps the reference schema (one ``newton`` method block or
+a legacy flat dict, solver.py:1014-1068).  Linear keys:
+  * ``jax_solver`` {precond}: the default, FP64 Jacobi-BiCGSTAB, tol = atol
+    = 1e-10, maxiter 10000 (solver.py:63-92), on the device.
+  * ``pcg`` / ``bicgstab`` {precond: 'jacobi', tol, atol, maxiter}: the same
+    device Krylov methods with explicit settings.
+  * ``custom_solver``: callable(A, b, x0, linear_options) -> x (solver.py:278-280);
+    A is the device ``TangentMatrix``.
+  * ``petsc_solver`` / ``spsolve_solver`` / ``amgx_solver`` are host or NVIDIA
+    back ends and raise ``NotImplementedError``.
+Added Newton keys:
+  * ``precision``: 'fp64' (default: everything FP64, as the reference) or
+    'mixed' (FP32 element Jacobian, FP32 SpMV/Krylov with FP64 dot products;
+    the Newton residual and update stay FP64, i.e. FP64 iterative refinement).
+  * ``forcing``: 'fixed' (default for fp64: the linear tolerance as given) or
+    'ew' (Eisenstat-Walker inexact Newton; default for mixed).
+Unknown keys are ignored, as in the reference.
+"""
+import ctypes
+import logging
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+logger = logging.getLogger("jax_fem_amd")
+
+_METHOD_KEYS = ("newton", "arc_length", "dynamic_relax")
+_LINEAR_KEYS = ("jax_solver", "amgx_solver", "spsolve_solver", "petsc_solver", "custom_solver", "pcg", "bicgstab")
+_NEWTON_KEYS = ("tol", "rel_tol", "line_search_flag", "initial_guess", "precision", "forcing", "max_iter",
+                "eta_max", "eta_min")
+
+
+def _resolve_solver_options(solver_options):
+    """(method, cfg); legacy flat dicts become Newton (solver.py:1050-1068)."""
+    opts = solver_options or {}
+    methods = [m for m in _METHOD_KEYS if m in opts]
+    if not methods:
+        cfg = {k: opts[k] for k in _NEWTON_KEYS if k in opts}
+        lin = {k: opts[k] for k in _LINEAR_KEYS if k in opts}
+        if lin:
+            cfg["linear"] = lin
+        return "newton", cfg
+    if len(methods) > 1:
+        raise ValueError("Pick one nonlinear method, got %s." % methods)
+    m = methods[0]
+    if not isinstance(opts[m], dict):
+        raise ValueError("solver_options['%s'] must be a dict." % m)
+    return m, opts[m]
+
+
+# ---------------------------------------------------------------------------
+class TangentMatrix:
+    """Row-eliminated tangent on the device: BSR values over the problem's pattern.
+
+    The object ``get_A`` returns (the reference returns a PETSc Mat)."""
+
+    def __init__(self, problem, vals):
+        self.problem = problem
+        self.pattern = problem.pattern
+        self.vals = vals
+        self.dtype = vals.dtype
+        self.shape = (problem.N, problem.N)
+        self._dinv = None
+
+    @property
+    def tag(self):
+        return "64" if self.dtype == torch.float64 else "32"
+
+    def matvec(self, x, transpose=False):
+        x = x.to(self.dtype).contiguous()
+        y = torch.empty_like(x)
+        name = "hf_spmv_bsr_t_f" if transpose else "hf_spmv_bsr_f"
+        _lib.call(name + self.tag, self.pattern.handle, _lib.ptr(self.vals), _lib.ptr(x), _lib.ptr(y), _lib.stream())
+        return y
+
+    def __matmul__(self, x):
+        return self.matvec(x)
+
+    def dinv(self):
+        if self._dinv is None:
+            d = torch.empty(self.problem.N, dtype=self.dtype, device=self.vals.device)
+            _lib.call("hf_jacobi_f" + self.tag, self.pattern.handle, _lib.ptr(self.vals), _lib.ptr(d), _lib.stream())
+            self._dinv = d
+        return self._dinv
+
+    def diagonal(self):
+        return 1.0 / self.dinv()
+
+    def csr(self):
+        """(rowptr, colind, values) of the scalar CSR view (FP64), device tensors."""
+        rp, ci = self.pattern.csr()
+        v = torch.empty(self.pattern.nnz, dtype=torch.float64, device=self.vals.device)
+        _lib.call("hf_bsr_to_csr_f64", self.pattern.handle, _lib.ptr(self.vals.to(torch.float64).contiguous()),
+                  _lib.ptr(v), _lib.stream())
+        return rp, ci, v
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp, ci, v = self.csr()
+        return sp.csr_matrix((v.cpu().numpy(), ci.cpu().numpy(), rp.cpu().numpy()), shape=self.shape)
+
+    def getValuesCSR(self):
+        rp, ci, v = self.csr()
+        return rp.cpu().numpy(), ci.cpu().numpy(), v.cpu().numpy()
+
+    def getSize(self):
+        return self.shape
+
+
+def get_A(problem, dtype=torch.float64):
+    """Assembled, row-eliminated tangent at the last ``newton_update`` state (solver.py:502-515)."""
+    u = getattr(problem, "_newton_u", None)
+    if u is None:
+        raise RuntimeError("call problem.newton_update(sol_list) first")
+    cache = getattr(problem, "_A_cache", None)
+    if cache is not None and cache.dtype == dtype:
+        return cache
+    A = TangentMatrix(problem, problem.jacobian_bsr(u, dtype))
+    problem._A_cache = A
+    return A
+
+
+# ---------------------------------------------------------------------------
+def apply_bc_vec(res_vec, dofs, problem, scale=1.0):
+    """R[B] = u[B] - scale*g (solver.py:290-304); returns a new vector."""
+    d, v = problem.bc_arrays()
+    R = res_vec.to(torch.float64).clone().contiguous()
+    _lib.call("hf_bc_residual_f64", _lib.ptr(d), _lib.ptr(v), d.numel(), float(scale),
+              _lib.ptr(dofs.to(torch.float64).contiguous()), _lib.ptr(R), _lib.stream())
+    return R
+
+
+def assign_bc(dofs, problem):
+    d, v = problem.bc_arrays()
+    out = dofs.to(torch.float64).clone()
+    out[d.long()] = v
+    return out
+
+
+def copy_bc(dofs, problem):
+    d, _ = problem.bc_arrays()
+    out = torch.zeros_like(dofs, dtype=torch.float64)
+    out[d.long()] = dofs.to(torch.float64)[d.long()]
+    return out
+
+
+def _initial_guess(problem, u):
+    """x0 = assign_bc(0) - copy_bc(u) (solver.py:402-409)."""
+    d, v = problem.bc_arrays()
+    x0 = torch.empty(problem.N, dtype=torch.float64, device=u.device)
+    _lib.call("hf_bc_initial_guess_f64", _lib.ptr(d), _lib.ptr(v), d.numel(), _lib.ptr(u), _lib.ptr(x0),
+              problem.N, _lib.stream())
+    return x0
+
+
+class KrylovStats:
+    def __init__(self):
+        self.calls = []
+
+    def add(self, method, iters, status, res, bnorm, seconds):
+        self.calls.append({"method": method, "iterations": iters, "status": status, "res": res,
+                           "b_norm": bnorm, "seconds": seconds})
+
+    @property
+    def iterations(self):
+        return sum(c["iterations"] for c in self.calls)
+
+
+def krylov(A, b, x0=None, method="pcg", rtol=1e-10, atol=1e-10, maxiter=10000, precond=True,
+           transpose=False, poll=8, stats=None, dinv=None):
+    """Device Krylov solve on a TangentMatrix; returns x (dtype of A)."""
+    dt = A.dtype
+    N = A.shape[0]
+    b = b.to(dt).contiguous()
+    x = (torch.zeros(N, dtype=dt, device=b.device) if x0 is None else x0.to(dt).clone()).contiguous()
+    if dinv is None:
+        dinv = A.dinv() if precond else torch.ones(N, dtype=dt, device=b.device)
+    mcode = _lib.HF_KRYLOV_PCG if method == "pcg" else _lib.HF_KRYLOV_BICGSTAB
+    esz = 8 if dt == torch.float64 else 4
+    wsz = _lib.load().hf_krylov_workspace_size(mcode, N, esz)
+    ws = _workspace(b.device, wsz)
+    info = _lib.HFKrylovInfo()
+    t0 = time.perf_counter()
+    _lib.call("hf_krylov_solve_f" + A.tag, mcode, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(dinv), _lib.ptr(b),
+              _lib.ptr(x), 1 if transpose else 0, int(maxiter), float(rtol), float(atol), int(poll),
+              _lib.ptr(ws), ctypes.byref(info), _lib.stream())
+    if stats is not None:
+        stats.add(method, info.iterations, info.status, info.res_norm, info.b_norm, time.perf_counter() - t0)
+    if info.status == 3:
+        raise FloatingPointError("Krylov solve produced non-finite values")
+    return x, info
+
+
+_WS = {}
+
+
+def _workspace(device, nbytes):
+    w = _WS.get(device)
+    if w is None or w.numel() < nbytes:
+        w = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+        _WS[device] = w
+    return w
+
+
+def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
+    """Dispatch on linear_options (solver.py:261-284)."""
+    opts = linear_options if linear_options is not None else {}
+    if not any(k in opts for k in _LINEAR_KEYS):
+        opts = dict(opts)
+        opts["jax_solver"] = {}
+    if "jax_solver" in opts:
+        precond = opts["jax_solver"].get("precond", True)
+        x, info = krylov(A, b, x0, "bicgstab", 1e-10 if rtol is None else rtol, 1e-10, 10000, precond,
+                         stats=stats)
+        if rtol is None:
+            err = torch.linalg.norm(A.matvec(x).to(torch.float64) - b.to(torch.float64)).item()
+            assert err < 0.1, "linear solver failed to converge with err = %g" % err
+        return x
+    for key in ("pcg", "bicgstab"):
+        if key in opts:
+            o = opts[key] or {}
+            pc = o.get("precond", "jacobi")
+            if pc not in ("jacobi", True, None, False, "none"):
+                raise NotImplementedError("preconditioner %r" % (pc,))
+            x, info = krylov(A, b, x0, key, o.get("tol", 1e-10) if rtol is None else rtol, o.get("atol", 1e-10),
+                             o.get("maxiter", 10000), pc not in (False, "none"), stats=stats,
+                             poll=o.get("poll", 8))
+            return x
+    if "custom_solver" in opts:
+        return opts["custom_solver"](A, b, x0, opts)
+    for key in ("petsc_solver", "spsolve_solver", "amgx_solver"):
+        if key in opts:
+            raise NotImplementedError("%s is a host/NVIDIA back end; use jax_solver, pcg or bicgstab" % key)
+    raise NotImplementedError("Unknown linear solver.")
+
+
+# ---------------------------------------------------------------------------
+def _flat_initial_guess(problem, cfg):
+    if "initial_guess" in cfg and cfg["initial_guess"] is not None:
+        g = cfg["initial_guess"]
+        g = g[0] if isinstance(g, (list, tuple)) else g
+        return torch.as_tensor(g, dtype=torch.float64).detach().reshape(-1).to(problem.device).clone()
+    return torch.zeros(problem.N, dtype=torch.float64, device=problem.device)
+
+
+def solver(problem, solver_options={}):
+    """Newton solve (solver.py:1074-1318); returns sol_list = [u (nnode, vec) f64 device]."""
+    method, cfg = _resolve_solver_options(solver_options)
+    if method != "newton":
+        raise NotImplementedError("%s is not part of the GPU solve path" % method)
+    precision = cfg.get("precision", "fp64")
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    forcing = cfg.get("forcing", "ew" if precision == "mixed" else "fixed")
+    rel_tol = cfg.get("rel_tol", 1e-8)
+    tol = cfg.get("tol", 1e-6)
+    max_iter = cfg.get("max_iter", 100)
+    lin_opts = dict(cfg.get("linear", {}))
+    if precision == "mixed" and not any(k in lin_opts for k in _LINEAR_KEYS):
+        lin_opts["pcg"] = {}
+    adt = torch.float32 if precision == "mixed" else torch.float64
+    eta_max = cfg.get("eta_max", 0.1)
+    eta_min = cfg.get("eta_min", 1e-6 if precision == "mixed" else 1e-10)
+
+    timing = {"local_assembly": 0.0, "global_matrix": 0.0, "linear": 0.0}
+    stats = KrylovStats()
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    u = _flat_initial_guess(problem, cfg)
+
+    def residual(u):
+        t0 = time.perf_counter()
+        R = apply_bc_vec(problem.residual_vec(u), u, problem)
+        res = float(torch.linalg.norm(R).item())
+        timing["local_assembly"] += time.perf_counter() - t0
+        return R, res
+
+    R, res = residual(u)
+    res0 = res
+    hist = [res]
+    n = 0
+    eta_prev = None
+    while (res / res0 > rel_tol) and (res > tol) and n < max_iter:
+        n += 1
+        t0 = time.perf_counter()
+        A = TangentMatrix(problem, problem.jacobian_bsr(u, adt))
+        torch.cuda.synchronize()
+        timing["global_matrix"] += time.perf_counter() - t0
+        x0 = _initial_guess(problem, u)
+        if forcing == "ew":
+            if eta_prev is None:
+                eta = eta_max
+            else:
+                eta = 0.9 * (hist[-1] / hist[-2]) ** 2
+                if 0.9 * eta_prev ** 2 > 0.1:
+                    eta = max(eta, 0.9 * eta_prev ** 2)
+            eta = max(eta, 0.5 * rel_tol * res0 / res)     # do not oversolve past the target
+            eta = min(max(eta, eta_min), eta_max)
+            eta_prev = eta
+        else:
+            eta = None
+        t0 = time.perf_counter()
+        inc = linear_solver(A, -R, x0, lin_opts, stats=stats, rtol=eta)
+        torch.cuda.synchronize()
+        timing["linear"] += time.perf_counter() - t0
+        u = u + inc.to(torch.float64)
+        R, res = residual(u)
+        hist.append(res)
+        logger.debug("newton %d: res %.3e rel %.3e", n, res, res / res0)
+    if not (np.isfinite(res) and bool(torch.isfinite(u).all())):
+        raise FloatingPointError("Newton produced non-finite values")
+    torch.cuda.synchronize()
+    problem.solve_info = {"newton_iters": n, "res_hist": hist, "res0": res0, "rel": res / res0,
+                          "krylov": stats.calls, "krylov_iters": stats.iterations,
+                          "timing": timing, "wall": time.perf_counter() - wall0, "precision": precision}
+    problem._newton_u = u
+    problem._A_cache = None
+    return problem.unflatten_fn_sol_list(u)
+
+
+# ---------------------------------------------------------------------------
+def _flatten(params):
+    if isinstance(params, (list, tuple)):
+        out, specs = [], []
+        for p in params:
+            f, s = _flatten(p)
+            out += f
+            specs.append(s)
+        return out, (type(params), specs)
+    return [params], None
+
+
+def _unflatten(flat, spec):
+    if spec is None:
+        return flat.pop(0)
+    typ, specs = spec
+    return typ(_unflatten(flat, s) for s in specs)
+
+
+def adjoint_solve(problem, u, v, adjoint_solver_options=None):
+    """lam with A^T lam = v at u (solver.py:1358-1371); returns (lam, lam_interior)."""
+    opts = adjoint_solver_options or {}
+    A = TangentMatrix(problem, problem.jacobian_bsr(u, torch.float64))
+    v = v.to(torch.float64).contiguous()
+    d, _ = problem.bc_arrays()
+    dl = d.long()
+    if "pcg" in opts:
+        # A_II is symmetric for these laws: solve A_II lam_I = v_I, then
+        # lam_B = v_B - (A_IB^T lam_I), the transposed product of the kept columns.
+        o = opts["pcg"] or {}
+        vI = v.clone()
+        vI[dl] = 0.0
+        lamI, _ = krylov(A, vI, None, "pcg", o.get("tol", 1e-10), o.get("atol", 1e-10), o.get("maxiter", 10000))
+        t = A.matvec(lamI, transpose=True)
+        lam = lamI.clone()
+        lam[dl] = v[dl] - t[dl]
+    else:
+        o = opts.get("bicgstab", opts.get("jax_solver", {})) or {}
+        lam, _ = krylov(A, v, None, "bicgstab", o.get("tol", 1e-10), o.get("atol", 1e-10), o.get("maxiter", 10000),
+                        o.get("precond", True) not in (False, "none"), transpose=True)
+    lamI = lam.clone()
+    lamI[dl] = 0.0
+    return lam, lamI
+
+
+def param_vjp(problem, u, lamI):
+    """-(d R / d p)^T lam per quadrature point, (ncell, nq) f64 (the material part of implicit_vjp)."""
+    pts, cells = problem._dev()
+    prm = problem._params()
+    if prm is None:
+        return None
+    grad = torch.empty_like(prm)
+    mat = problem.material.hf()
+    _lib.call("hf_cell_param_vjp_f64", problem.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+              problem.num_cells, _lib.ptr(u.contiguous()), _lib.ptr(prm), _lib.ptr(lamI.contiguous()),
+              _lib.ptr(grad), _lib.stream())
+    return grad
+
+
+def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options={}):
+    """Gradient of <v, u*(params)> w.r.t. params (solver.py:1324-1380).
+
+    Differentiates through ``problem.set_params`` with torch autograd: the
+    per-quadrature field ``internal_vars[0]`` receives -(dR/dp)^T lam, and
+    Dirichlet values given as torch tensors (``fe.vals_list``) receive lam_B."""
+    flat, spec = _flatten(params)
+    leaves = [torch.as_tensor(p, dtype=torch.float64).detach().clone().requires_grad_(True)
+              if (torch.is_tensor(p) or np.ndim(p) >= 0) else p for p in flat]
+    with torch.enable_grad():
+        problem.set_params(_unflatten(list(leaves), spec))
+        iv = problem.internal_vars[0] if problem.internal_vars else None
+        bc_vals = list(problem.fe.vals_list)
+    u = torch.as_tensor(sol_list[0]).reshape(-1).to(problem.device, torch.float64)
+    v = torch.as_tensor(v_list[0]).reshape(-1).to(problem.device, torch.float64)
+    problem._newton_u = u
+    lam, lamI = adjoint_solve(problem, u, v, adjoint_solver_options)
+    outs, gouts = [], []
+    if torch.is_tensor(iv) and iv.requires_grad:
+        g = param_vjp(problem, u, lamI)
+        outs.append(iv)
+        gouts.append(g.reshape(iv.shape).to(iv.device, iv.dtype))
+    # Dirichlet values: c_B = u_B - g  =>  d<lam, c>/dg = -lam_B, grad = +lam_B (later specs win)
+    fe = problem.fe
+    owner = {}
+    for s, (nodes, comps) in enumerate(zip(fe.node_inds_list, fe.vec_inds_list)):
+        for j, dof in enumerate((np.asarray(nodes) * problem.vec + np.asarray(comps)).tolist()):
+            owner[dof] = (s, j)
+    lam_h = lam.detach().cpu().numpy()
+    for s, vals in enumerate(bc_vals):
+        if torch.is_tensor(vals) and vals.requires_grad:
+            dofs = (np.asarray(fe.node_inds_list[s]) * problem.vec + np.asarray(fe.vec_inds_list[s])).tolist()
+            gv = np.array([lam_h[d] if owner[d] == (s, j) else 0.0 for j, d in enumerate(dofs)])
+            outs.append(vals)
+            gouts.append(torch.as_tensor(gv, dtype=vals.dtype, device=vals.device))
+    req = [l for l in leaves if torch.is_tensor(l) and l.requires_grad]
+    if outs:
+        grads = torch.autograd.grad(outs, req, grad_outputs=gouts, allow_unused=True)
+    else:
+        grads = [None] * len(req)
+    grads = [torch.zeros_like(l) if g is None else g for g, l in zip(grads, req)]
+    problem.set_params(_unflatten([l.detach() for l in leaves], spec))
+    return _unflatten(list(grads), spec)
+
+
+def ad_wrapper(problem, solver_options={}, adjoint_solver_options={}):
+    """fwd_pred(params) -> sol_list, differentiable with torch autograd (solver.py:1383-1417)."""
+
+    class _ImplicitSolve(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, spec_holder, *flat):
+            params = _unflatten([f.detach() for f in flat], spec_holder[0])
+            problem.set_params(params)
+            sol = solver(problem, solver_options)[0]
+            ctx.spec = spec_holder[0]
+            ctx.save_for_backward(*flat)
+            ctx.sol = sol.detach()
+            return sol.clone()
+
+        @staticmethod
+        def backward(ctx, v):
+            flat = ctx.saved_tensors
+            params = _unflatten(list(flat), ctx.spec)
+            g = implicit_vjp(problem, [ctx.sol], params, [v], adjoint_solver_options)
+            gflat, _ = _flatten(g)
+            return (None,) + tuple(gflat)
+
+    def fwd_pred(params):
+        flat, spec = _flatten(params)
+        flat = [f if torch.is_tensor(f) else torch.as_tensor(f, dtype=torch.float64) for f in flat]
+        return [_ImplicitSolve.apply([spec], *flat)]
+
+    return fwd_pred

@@ -1,0 +1,355 @@
+"""GPU parity: libhipfem (through the C ABI) vs the FP64 oracle and the reference's gold data.
+
+Tolerances: FP64 kernels 1e-11 relative to the oracle (summation order is the
+only difference); FP32 kernels 2e-5 relative; integer work (pattern,
+contributions) bit-exact; end-to-end solutions at the reference tests' own
+decimal places and vs the oracle at 1e-8.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from conftest import GOLDEN
+from oracle import fe as ofe
+from oracle import kernels as ok
+from oracle import solver as osv
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def iscl(a, b):
+    return np.abs(a - b) < 1e-5
+
+
+def make_problem(jfa, mesh, material, vec=3, dirichlet=None, location_fns=None, surface=None, mass=None,
+                 params=None, ele="HEX8", order=None):
+    class P(jfa.Problem):
+        def get_tensor_map(self):
+            return material
+
+    if surface is not None:
+        P.get_surface_maps = lambda self: surface
+    if mass is not None:
+        P.get_mass_map = lambda self: mass
+    prob = P(mesh, vec=vec, dim=3, ele_type=ele, quadrature_order=order, dirichlet_bc_info=dirichlet,
+             location_fns=location_fns)
+    if params is not None:
+        prob.internal_vars = [params]
+    return prob
+
+
+MATS = [
+    ("linear_elastic", None), ("neo_hookean", None), ("neo_hookean", "scale_E"),
+    ("linear_elastic", "simp"), ("neo_hookean", "mu"),
+]
+
+
+def _mats(jfa, kind, param):
+    kw = dict(E=10.0, nu=0.3, param=param)
+    if param == "mu":
+        kw["kappa"] = 8.0
+    if kind == "linear_elastic":
+        return jfa.LinearElastic(**kw), ok.Material(kind, E=10.0, nu=0.3, param=param)
+    return jfa.NeoHookean(**kw), ok.Material(kind, E=10.0, nu=0.3, param=param, kappa=8.0)
+
+
+def _distorted_box(n=(3, 2, 2), seed=0):
+    pts, cells = ofe.box_mesh(*n)
+    rng = np.random.default_rng(seed)
+    pts = pts + 0.08 * rng.uniform(-1, 1, pts.shape) / max(n)
+    return pts, cells
+
+
+@pytest.mark.parametrize("kind,param", MATS)
+def test_cell_kernels_vs_oracle(jfa, kind, param):
+    say("cell kernels", kind, param)
+    pts, cells = _distorted_box()
+    mat, omat = _mats(jfa, kind, param)
+    rng = np.random.default_rng(1)
+    prm = rng.uniform(0.3, 1.0, (len(cells), 8)) if param else None
+    prob = make_problem(jfa, jfa.Mesh(pts, cells), mat, params=prm)
+    oprob = osv.OracleProblem(pts, cells, 3, omat, params=prm)
+    u = 0.05 * rng.standard_normal(oprob.N)
+    R = prob.residual_vec(torch.tensor(u, device=DEV)).cpu().numpy()
+    Ro = oprob.compute_residual(u)
+    np.testing.assert_allclose(R, Ro, rtol=1e-11, atol=1e-11 * np.abs(Ro).max())
+    K64 = prob.element_jacobians(torch.tensor(u, device=DEV)).cpu().numpy()
+    Ko = oprob.element_jacobians(u)
+    np.testing.assert_allclose(K64, Ko, rtol=1e-11, atol=1e-11 * np.abs(Ko).max())
+    K32 = prob.element_jacobians(torch.tensor(u, device=DEV), torch.float32).cpu().numpy()
+    np.testing.assert_allclose(K32, Ko, rtol=2e-5, atol=2e-5 * np.abs(Ko).max())
+    if param:
+        lam = rng.standard_normal(oprob.N)
+        from importlib import import_module
+        g = import_module("jax_fem_amd.solver").param_vjp(prob, torch.tensor(u, device=DEV),
+                                                          torch.tensor(lam, device=DEV)).cpu().numpy()
+        go = -ok.cell_param_vjp(omat, oprob.cell_u(u), oprob.grads, oprob.JxW, prm,
+                                lam.reshape(-1, 3)[cells])
+        np.testing.assert_allclose(g, go, rtol=1e-10, atol=1e-12 * np.abs(go).max())
+
+
+def test_poisson_kernels_vs_oracle(jfa):
+    say("poisson kernels")
+    pts, cells = _distorted_box((2, 3, 2), 3)
+    prob = make_problem(jfa, jfa.Mesh(pts, cells), jfa.Poisson(), vec=1)
+    oprob = osv.OracleProblem(pts, cells, 1, ok.Material("poisson"))
+    u = np.random.default_rng(2).standard_normal(oprob.N)
+    R = prob.residual_vec(torch.tensor(u, device=DEV)).cpu().numpy()
+    np.testing.assert_allclose(R, oprob.compute_residual(u), rtol=1e-11, atol=1e-12)
+    K = prob.element_jacobians(torch.tensor(u, device=DEV)).cpu().numpy()
+    np.testing.assert_allclose(K, oprob.element_jacobians(u), rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["box", "cylinder"])
+def test_pattern_bit_exact_and_assembly(jfa, case):
+    say("pattern", case)
+    if case == "box":
+        pts, cells = _distorted_box((4, 3, 2), 5)
+    else:
+        d = np.load(os.path.join(GOLDEN, "fenicsx_hyperelasticity.npz"))
+        pts, cells = d["points"], d["cells"]
+    mat, omat = _mats(jfa, "neo_hookean", None)
+    dirichlet = [[lambda p: iscl(p[2], 0.0)] * 2, [0, 2], [lambda p: 0.0, lambda p: 0.1]]
+    prob = make_problem(jfa, jfa.Mesh(pts, cells), mat, dirichlet=dirichlet)
+    oprob = osv.OracleProblem(pts, cells, 3, omat, dirichlet_bc_info=dirichlet)
+    u = 0.02 * np.random.default_rng(3).standard_normal(oprob.N)
+    Ao = oprob.assemble(u)
+    # pattern: bit-exact against scipy's coo->csr of the reference (I, J)
+    rp, ci = prob.pattern.csr()
+    I, J = prob.I, prob.J
+    ref = sp.coo_matrix((np.ones(len(I)), (I, J)), shape=(prob.N, prob.N)).tocsr()
+    ref.sum_duplicates()
+    ref.sort_indices()
+    assert np.array_equal(rp.cpu().numpy(), ref.indptr)
+    assert np.array_equal(ci.cpu().numpy(), ref.indices)
+    # values: FP64 segmented reduction + row elimination vs scipy FP64 sums
+    from importlib import import_module
+    S = import_module("jax_fem_amd.solver")
+    A = S.TangentMatrix(prob, prob.jacobian_bsr(torch.tensor(u, device=DEV)))
+    rp2, ci2, v = A.csr()
+    np.testing.assert_allclose(v.cpu().numpy(), Ao.data, rtol=1e-11, atol=1e-11 * np.abs(Ao.data).max())
+    # SpMV / transposed SpMV in FP64 and FP32
+    x = np.random.default_rng(4).standard_normal(prob.N)
+    xt = torch.tensor(x, device=DEV)
+    np.testing.assert_allclose(A.matvec(xt).cpu().numpy(), Ao @ x, rtol=1e-11, atol=1e-11 * np.abs(Ao @ x).max())
+    np.testing.assert_allclose(A.matvec(xt, True).cpu().numpy(), Ao.T @ x, rtol=1e-11,
+                               atol=1e-11 * np.abs(Ao.T @ x).max())
+    A32 = S.TangentMatrix(prob, prob.jacobian_bsr(torch.tensor(u, device=DEV), torch.float32))
+    y32 = A32.matvec(xt.float()).double().cpu().numpy()
+    np.testing.assert_allclose(y32, Ao @ x, rtol=1e-4, atol=2e-5 * np.abs(Ao @ x).max())
+    # determinism: assembly and residual are bitwise repeatable
+    v2 = prob.jacobian_bsr(torch.tensor(u, device=DEV))
+    assert torch.equal(A.vals, v2)
+    r1 = prob.residual_vec(xt)
+    r2 = prob.residual_vec(xt)
+    assert torch.equal(r1, r2)
+
+
+@pytest.mark.parametrize("method", ["pcg", "bicgstab"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_krylov_vs_direct(jfa, method, dtype):
+    say("krylov", method, dtype)
+    pts, cells = _distorted_box((6, 5, 4), 7)
+    mat, omat = _mats(jfa, "linear_elastic", None)
+    dirichlet = [[lambda p: iscl(p[0], 0.0)] * 3, [0, 1, 2], [lambda p: 0.0] * 3]
+    prob = make_problem(jfa, jfa.Mesh(pts, cells), mat, dirichlet=dirichlet)
+    oprob = osv.OracleProblem(pts, cells, 3, omat, dirichlet_bc_info=dirichlet)
+    Ao = oprob.assemble(np.zeros(oprob.N))
+    b = np.random.default_rng(5).standard_normal(oprob.N)
+    b[np.concatenate(oprob.bc_rows())] = 0.0
+    import scipy.sparse.linalg as spla
+    xd = spla.spsolve(Ao.tocsc(), b)
+    from importlib import import_module
+    S = import_module("jax_fem_amd.solver")
+    A = S.TangentMatrix(prob, prob.jacobian_bsr(torch.zeros(prob.N, device=DEV, dtype=torch.float64), dtype))
+    tol = 1e-10 if dtype == torch.float64 else 1e-6
+    x, info = S.krylov(A, torch.tensor(b, device=DEV), None, method, tol, 0.0, 20000)
+    assert info.status == 0, (info.status, info.iterations)
+    xh = x.double().cpu().numpy()
+    if dtype == torch.float64:
+        assert np.linalg.norm(xh - xd) / np.linalg.norm(xd) < 1e-8
+    else:  # FP32 Krylov: true FP64 residual at the FP32 attainable level
+        assert np.linalg.norm(Ao @ xh - b) / np.linalg.norm(b) < 2e-5
+    # determinism of the whole Krylov loop
+    x2, _ = S.krylov(A, torch.tensor(b, device=DEV), None, method, tol, 0.0, 20000)
+    assert torch.equal(x, x2)
+    # transposed solve: A^T y = b
+    y, info = S.krylov(A, torch.tensor(b, device=DEV), None, "bicgstab", tol, 0.0, 20000, transpose=True)
+    assert info.status == 0
+    yh = y.double().cpu().numpy()
+    assert np.linalg.norm(Ao.T @ yh - b) / np.linalg.norm(b) < (1e-9 if dtype == torch.float64 else 2e-5)
+
+
+# --------------------------------------------------------------------- fixtures
+def fixture(name):
+    return np.load(os.path.join(GOLDEN, "fenicsx_%s.npz" % name), allow_pickle=False)
+
+
+def test_fixture_linear_poisson(jfa):
+    say("fixture poisson")
+    d = fixture("linear_poisson")
+    prob = make_problem(jfa, jfa.Mesh(d["points"], d["cells"]), jfa.Poisson(), vec=1,
+                        dirichlet=[[lambda p: iscl(p[0], 0), lambda p: iscl(p[0], 1)], [0, 0],
+                                   [lambda p: 0.0, lambda p: 1.0]])
+    u = jfa.solver(prob)[0].cpu().numpy()
+    np.testing.assert_array_almost_equal(d["sol"], u, decimal=5)
+    np.testing.assert_allclose(u[:, 0], d["points"][:, 0], atol=1e-8)
+
+
+def test_fixture_linear_elasticity_cube(jfa):
+    say("fixture cube")
+    d = fixture("linear_elasticity_cube")
+    left = lambda p: iscl(p[0], 0.0)
+    prob = make_problem(jfa, jfa.Mesh(d["points"], d["cells"]), jfa.LinearElastic(E=70e3, nu=0.3),
+                        dirichlet=[[left] * 3, [0, 1, 2], [lambda p: 1.0] * 3],
+                        location_fns=[lambda p: iscl(p[0], 1.0)],
+                        surface=[lambda u, x: -np.array([10.0, 0.0, 0.0])],
+                        mass=lambda u, x: -np.array([0.0, 10.0, 10.0]))
+    u = jfa.solver(prob)[0].cpu().numpy()
+    np.testing.assert_array_almost_equal(d["sol"], u, decimal=5)
+
+
+def test_fixture_linear_elasticity_cylinder(jfa):
+    say("fixture cylinder")
+    d = fixture("linear_elasticity_cylinder")
+    prob = make_problem(jfa, jfa.Mesh(d["points"], d["cells"]), jfa.LinearElastic(E=70e3, nu=0.3),
+                        dirichlet=[[lambda p: iscl(p[2], 0.0)] * 3, [0, 1, 2], [lambda p: 0.0] * 3],
+                        location_fns=[lambda p: iscl(p[2], 10.0)],
+                        surface=[lambda u, x: -np.array([1e3 * x[0] ** 2 + 1e3 * x[1] ** 2, 0.0, 0.0])],
+                        mass=lambda u, x: -np.array([1e3 * x[0], 2e3 * x[1], 3e3 * x[2]]))
+    u = jfa.solver(prob)[0].cpu().numpy()
+    np.testing.assert_array_almost_equal(d["sol"], u, decimal=3)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fixture_hyperelasticity(jfa, precision):
+    say("fixture hyperelasticity", precision)
+    d = fixture("hyperelasticity")
+    top = lambda p: iscl(p[2], 10.0)
+    bot = lambda p: iscl(p[2], 0.0)
+    prob = make_problem(jfa, jfa.Mesh(d["points"], d["cells"]), jfa.NeoHookean(E=1e3, nu=0.3),
+                        dirichlet=[[bot, bot, bot, top, top, top], [0, 1, 2, 0, 1, 2],
+                                   [lambda p: 0.0] * 5 + [lambda p: 1.0]])
+    u = jfa.solver(prob, {"newton": {"precision": precision}})[0].cpu().numpy()
+    info = prob.solve_info
+    say("newton", info["newton_iters"], "krylov", info["krylov_iters"], "rel", info["rel"])
+    assert info["rel"] <= 1e-8 or info["res_hist"][-1] <= 1e-6
+    np.testing.assert_array_almost_equal(d["sol"], u, decimal=5)
+    # traction on the top face (test_hyper_elasticity.py:121-127)
+    fe = prob.fe
+    bi = fe.get_boundary_conditions_inds([top])[0]
+    g, nanson = fe.get_face_shape_grads(bi)
+    ug = np.einsum("sni,sqnj->sqij", u[fe.cells[bi[:, 0]]], g)
+    P = prob.material(ug)
+    tz = np.einsum("sqij,j,sq->i", P, np.array([0, 0, 1.0]), nanson)[2]
+    np.testing.assert_almost_equal(float(d["traction"]), tz, decimal=5)
+
+
+# ----------------------------------------------------------------------- adjoint
+class _Grad:
+    """compute_gradients notebook problem (cells 11-21) on a 5^3 box_mesh."""
+
+    @staticmethod
+    def build(jfa):
+        class HyperElasticity(jfa.Problem):
+            def custom_init(self):
+                self.fe = self.fes[0]
+
+            def get_tensor_map(self):
+                return jfa.NeoHookean(E=1e6, nu=0.3, param="scale_E")
+
+            def get_surface_maps(self):
+                return [lambda u, x: np.array([0.0, 0.0, 1e3])]
+
+            def set_params(self, params):
+                rho, scale_d = params
+                self.internal_vars = [rho]
+                info = self.fe.dirichlet_bc_info
+                info[-1][-1] = lambda p: scale_d * 1.0
+                self.fe.update_Dirichlet_boundary_conditions(info)
+
+        bottom = lambda p: np.isclose(p[2], 0.0, atol=1e-5)
+        top = lambda p: np.isclose(p[2], 1.0, atol=1e-5)
+        info = [[bottom] * 3, [0, 1, 2], [lambda p: 0.0, lambda p: 0.0, lambda p: 1.0]]
+        return HyperElasticity(jfa.box_mesh(5, 5, 5), vec=3, dim=3, dirichlet_bc_info=info, location_fns=[top])
+
+
+@pytest.mark.parametrize("adj", [{}, {"pcg": {}}])
+def test_adjoint_notebook_golden(jfa, adj):
+    say("adjoint golden", adj)
+    prob = _Grad.build(jfa)
+    rho = torch.full((125, 8), 0.5, dtype=torch.float64, device=DEV, requires_grad=True)
+    scale = torch.tensor(1.0, dtype=torch.float64, device=DEV, requires_grad=True)
+    fwd = jfa.ad_wrapper(prob, {}, adj)
+    sol = fwd([rho, scale])[0]
+    J = torch.sum(sol ** 2)
+    J.backward()
+    # example.ipynb md cell 31
+    assert abs(scale.grad.item() / 431.59223609853564 - 1) < 1e-6, scale.grad.item()
+    assert abs(rho.grad[0, 0].item() / 0.002266954599447443 - 1) < 1e-6, rho.grad[0, 0].item()
+
+
+def test_adjoint_vs_oracle_and_fd(jfa):
+    say("adjoint simp vs oracle")
+    pts, cells = ofe.box_mesh(6, 3, 3, 2.0, 1.0, 1.0)
+    rng = np.random.default_rng(11)
+    theta = np.clip(0.5 + 0.05 * rng.standard_normal(len(cells)), 1e-3, 1.0)
+    th_q = np.repeat(theta[:, None], 8, axis=1)
+    fixed = lambda p: iscl(p[0], 0.0)
+    load = lambda p: iscl(p[0], 2.0) & (p[2] <= 0.34)
+    mat = jfa.LinearElastic(nu=0.3, param="simp", E0=1.0, Emin=1e-3, penal=3.0)
+
+    class Topo(jfa.Problem):
+        def get_tensor_map(self):
+            return mat
+
+        def get_surface_maps(self):
+            return [lambda u, x: np.array([0.0, 0.0, 1.0])]
+
+        def set_params(self, params):
+            self.internal_vars = [params[:, None].expand(-1, 8) if torch.is_tensor(params) else params]
+
+    prob = Topo(jfa.Mesh(pts, cells), vec=3, dim=3, dirichlet_bc_info=[[fixed] * 3, [0, 1, 2], [lambda p: 0.0] * 3],
+                location_fns=[load])
+    th = torch.tensor(theta, device=DEV, requires_grad=True)
+    fwd = jfa.ad_wrapper(prob, {"newton": {"linear": {"pcg": {"tol": 1e-12}}}}, {"pcg": {"tol": 1e-12}})
+    u = fwd(th)[0]
+    # compliance-like objective: sum of u_z on the loaded nodes
+    bi = prob.boundary_inds_list[0]
+    nodes = np.unique(prob.fe.cells[bi[:, 0]][:, [1, 2, 5, 6]])
+    obj = u[torch.tensor(nodes, device=DEV), 2].sum()
+    obj.backward()
+    g = th.grad.cpu().numpy()
+    # oracle
+    om = ok.Material("linear_elastic", nu=0.3, param="simp", E0=1.0, Emin=1e-3, penal=3.0)
+    op = osv.OracleProblem(pts, cells, 3, om, dirichlet_bc_info=[[fixed] * 3, [0, 1, 2], [lambda p: 0.0] * 3],
+                           location_fns=[load], surface_maps=[lambda x: np.broadcast_to(np.array([0, 0, 1.0]), x.shape)],
+                           params=th_q)
+    uo, _ = osv.newton_solve(op, rel_tol=1e-12)
+    np.testing.assert_allclose(u.detach().cpu().numpy().ravel(), uo, rtol=1e-7, atol=1e-9 * np.abs(uo).max())
+    v = np.zeros(op.N)
+    v[nodes * 3 + 2] = 1.0
+    gq, _, _ = osv.implicit_vjp(op, uo, v)
+    go = gq.sum(axis=1)
+    np.testing.assert_allclose(g, go, rtol=1e-6, atol=1e-9 * np.abs(go).max())
+    # central FD on 3 cells (h = 1e-4) through the GPU forward solve
+    for c in rng.choice(len(cells), 3, replace=False):
+        vals = []
+        for s in (1, -1):
+            t2 = th.detach().clone()
+            t2[c] += s * 1e-4
+            prob.set_params(t2)
+            uu = jfa.solver(prob, {"newton": {"linear": {"pcg": {"tol": 1e-13, "atol": 1e-16}}}})[0]
+            vals.append(uu[torch.tensor(nodes, device=DEV), 2].sum().item())
+        fd = (vals[0] - vals[1]) / 2e-4
+        assert abs(fd - g[c]) <= 1e-5 * abs(g[c]) + 1e-10, (c, fd, g[c])
