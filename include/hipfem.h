@@ -9,7 +9,7 @@
  * must not run concurrently on different streams.
  *
  * DOF numbering is the reference's interleaved 3*node+comp (problem.py:86-92).
- * Element matrices, the block-sparse (BSR) matrix and its CSR view follow the
+ * Element matrices, the node-pair block matrix and its CSR view follow the
  * reference's COO layout jac[r,c] = dR_r/du_c (problem.py:262-266, 453).
  */
 #ifndef HIPFEM_H
@@ -95,17 +95,21 @@ int hf_cell_param_vjp_f64(int32_t ele, const hf_material* mat, const double* poi
 /* R[node] = sum over cells of elem_res in cell order (+ f_ext if not NULL) */
 int hf_gather_residual_f64(const hf_pattern* p, const double* elem_res, const double* f_ext,
                            double* R, hf_stream_t stream);
-/* BSR values (nblocks, vec, vec) = sum of element blocks in cell order */
-int hf_assemble_bsr_f64(const hf_pattern* p, const double* elem_jac, double* bsr, hf_stream_t stream);
-int hf_assemble_bsr_f32(const hf_pattern* p, const float* elem_jac, float* bsr, hf_stream_t stream);
-/* scalar CSR values from BSR values (the CSR order of hf_pattern_csr) */
-int hf_bsr_to_csr_f64(const hf_pattern* p, const double* bsr, double* csr, hf_stream_t stream);
+/* Matrix values live in a sliced-ELL layout over the pattern's node-pair blocks
+ * (slices of 21 node rows for vec 3, 64 for vec 1; each slice padded to its
+ * longest row): hf_matrix_values() gives the element count of a value buffer.
+ * assemble: every block = sum of its element blocks in cell order. */
+int64_t hf_matrix_values(const hf_pattern* p);
+int hf_assemble_f64(const hf_pattern* p, const double* elem_jac, double* vals, hf_stream_t stream);
+int hf_assemble_f32(const hf_pattern* p, const float* elem_jac, float* vals, hf_stream_t stream);
+/* scalar CSR values (the order of hf_pattern_csr) from matrix values */
+int hf_matrix_to_csr_f64(const hf_pattern* p, const double* vals, double* csr, hf_stream_t stream);
 
 /* ---- Dirichlet row elimination (solver.py:290-363, 489-490). bc_dofs are
  *      unique DOF indices (later specs already applied), bc_vals their values. */
-/* zeroRows(rows, diag=1) on the BSR values */
-int hf_bc_rows_f64(const hf_pattern* p, const int32_t* bc_dofs, int64_t nbc, double* bsr, hf_stream_t stream);
-int hf_bc_rows_f32(const hf_pattern* p, const int32_t* bc_dofs, int64_t nbc, float* bsr, hf_stream_t stream);
+/* zeroRows(rows, diag=1) on the matrix values */
+int hf_bc_rows_f64(const hf_pattern* p, const int32_t* bc_dofs, int64_t nbc, double* vals, hf_stream_t stream);
+int hf_bc_rows_f32(const hf_pattern* p, const int32_t* bc_dofs, int64_t nbc, float* vals, hf_stream_t stream);
 /* apply_bc_vec: R[d] = u[d] - scale*g[d] */
 int hf_bc_residual_f64(const int32_t* bc_dofs, const double* bc_vals, int64_t nbc, double scale,
                        const double* u, double* R, hf_stream_t stream);
@@ -114,21 +118,23 @@ int hf_bc_initial_guess_f64(const int32_t* bc_dofs, const double* bc_vals, int64
                             const double* u, double* x0, int64_t N, hf_stream_t stream);
 
 /* ---- sparse matrix-vector products (the BCOO matvec of jax_solve, solver.py:67-84) */
-int hf_spmv_bsr_f64(const hf_pattern* p, const double* bsr, const double* x, double* y, hf_stream_t stream);
-int hf_spmv_bsr_f32(const hf_pattern* p, const float* bsr, const float* x, float* y, hf_stream_t stream);
+int hf_spmv_f64(const hf_pattern* p, const double* vals, const double* x, double* y, hf_stream_t stream);
+int hf_spmv_f32(const hf_pattern* p, const float* vals, const float* x, float* y, hf_stream_t stream);
 /* y = A^T x through the transposed-block map (PETSc A.transpose, solver.py:1369-1370) */
-int hf_spmv_bsr_t_f64(const hf_pattern* p, const double* bsr, const double* x, double* y, hf_stream_t stream);
-int hf_spmv_bsr_t_f32(const hf_pattern* p, const float* bsr, const float* x, float* y, hf_stream_t stream);
+int hf_spmv_t_f64(const hf_pattern* p, const double* vals, const double* x, double* y, hf_stream_t stream);
+int hf_spmv_t_f32(const hf_pattern* p, const float* vals, const float* x, float* y, hf_stream_t stream);
 /* inverse of the scalar diagonal (Jacobi M = 1/diag(A), solver.py:68-69) */
-int hf_jacobi_f64(const hf_pattern* p, const double* bsr, double* dinv, hf_stream_t stream);
-int hf_jacobi_f32(const hf_pattern* p, const float* bsr, float* dinv, hf_stream_t stream);
+int hf_jacobi_f64(const hf_pattern* p, const double* vals, double* dinv, hf_stream_t stream);
+int hf_jacobi_f32(const hf_pattern* p, const float* vals, float* dinv, hf_stream_t stream);
 
 /* ---- device-resident Krylov solves: replace jax.scipy.sparse.linalg.bicgstab
  *      inside jax_solve (solver.py:78-84) and the linear_solver plugin slot
  *      (solver.py:261-284).  Stop rule ||r||^2 <= max(rtol^2 ||b||^2, atol^2)
- *      with r the unpreconditioned recursive residual; dot products accumulate
- *      in FP64.  The host polls a device flag every `poll` iterations.
- *      transpose != 0 solves A^T x = b.  workspace: hf_krylov_workspace_size. */
+ *      with r the unpreconditioned recursive residual (HF_KRYLOV_REL_R0: relative
+ *      to the initial residual ||b - A x0|| instead of ||b||, the inexact-Newton
+ *      form); dot products accumulate in FP64.  The host polls a device flag
+ *      every `poll` iterations.  HF_KRYLOV_TRANSPOSE solves A^T x = b.
+ *      workspace: hf_krylov_workspace_size. */
 typedef struct hf_krylov_info {
   int32_t iterations;
   int32_t status;      /* 0 converged, 1 max iterations, 2 breakdown, 3 non-finite */
@@ -137,13 +143,14 @@ typedef struct hf_krylov_info {
 } hf_krylov_info;
 
 enum { HF_KRYLOV_PCG = 0, HF_KRYLOV_BICGSTAB = 1 };
+enum { HF_KRYLOV_TRANSPOSE = 1, HF_KRYLOV_REL_R0 = 2 };
 size_t hf_krylov_workspace_size(int32_t method, int64_t N, int32_t elem_bytes);
-int hf_krylov_solve_f64(int32_t method, const hf_pattern* p, const double* bsr, const double* dinv,
-                        const double* b, double* x, int32_t transpose, int32_t maxiter,
+int hf_krylov_solve_f64(int32_t method, const hf_pattern* p, const double* vals, const double* dinv,
+                        const double* b, double* x, int32_t flags, int32_t maxiter,
                         double rtol, double atol, int32_t poll, void* workspace,
                         hf_krylov_info* info, hf_stream_t stream);
-int hf_krylov_solve_f32(int32_t method, const hf_pattern* p, const float* bsr, const float* dinv,
-                        const float* b, float* x, int32_t transpose, int32_t maxiter,
+int hf_krylov_solve_f32(int32_t method, const hf_pattern* p, const float* vals, const float* dinv,
+                        const float* b, float* x, int32_t flags, int32_t maxiter,
                         double rtol, double atol, int32_t poll, void* workspace,
                         hf_krylov_info* info, hf_stream_t stream);
 

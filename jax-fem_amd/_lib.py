@@ -47,17 +47,18 @@ _SIGS = {
     "hf_cell_jacobian_f32": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
     "hf_cell_param_vjp_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp, vp]),
     "hf_gather_residual_f64": (c_i32, [vp, vp, vp, vp, vp]),
-    "hf_assemble_bsr_f64": (c_i32, [vp, vp, vp, vp]),
-    "hf_assemble_bsr_f32": (c_i32, [vp, vp, vp, vp]),
-    "hf_bsr_to_csr_f64": (c_i32, [vp, vp, vp, vp]),
+    "hf_matrix_values": (c_i64, [vp]),
+    "hf_assemble_f64": (c_i32, [vp, vp, vp, vp]),
+    "hf_assemble_f32": (c_i32, [vp, vp, vp, vp]),
+    "hf_matrix_to_csr_f64": (c_i32, [vp, vp, vp, vp]),
     "hf_bc_rows_f64": (c_i32, [vp, vp, c_i64, vp, vp]),
     "hf_bc_rows_f32": (c_i32, [vp, vp, c_i64, vp, vp]),
     "hf_bc_residual_f64": (c_i32, [vp, vp, c_i64, c_f64, vp, vp, vp]),
     "hf_bc_initial_guess_f64": (c_i32, [vp, vp, c_i64, vp, vp, c_i64, vp]),
-    "hf_spmv_bsr_f64": (c_i32, [vp, vp, vp, vp, vp]),
-    "hf_spmv_bsr_f32": (c_i32, [vp, vp, vp, vp, vp]),
-    "hf_spmv_bsr_t_f64": (c_i32, [vp, vp, vp, vp, vp]),
-    "hf_spmv_bsr_t_f32": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_spmv_f64": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_spmv_f32": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_spmv_t_f64": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_spmv_t_f32": (c_i32, [vp, vp, vp, vp, vp]),
     "hf_jacobi_f64": (c_i32, [vp, vp, vp, vp]),
     "hf_jacobi_f32": (c_i32, [vp, vp, vp, vp]),
     "hf_krylov_workspace_size": (c_sz, [c_i32, c_i64, c_i32]),

@@ -27,6 +27,15 @@ struct hf_pattern {
   int32_t* diag;     // nnode diagonal block index (-1 for a node in no cell)
   int32_t* trans;    // nblocks index of the transposed block
   int32_t* brow;     // nblocks block row
+  int32_t max_row_blocks;
+  // sliced-ELL value layout (see below)
+  int R;             // node rows per slice (21 for vec 3, 64 for vec 1)
+  int64_t nslices, nslots;
+  int32_t* slice_ptr;  // nslices+1 first slot of each slice
+  int32_t* sell_col;   // nslots*R block column per (slot, row); padding -> own row
+  int32_t* sell_tpos;  // nslots*R value base of the transposed block, -1 for padding
+  int32_t* blkpos;     // nblocks value base of each block
+  int32_t* dpos;       // nnode value base of the diagonal block (-1: none)
 };
 
 namespace hf {
@@ -84,6 +93,12 @@ __global__ void transpose_map(const int32_t* __restrict__ rowptr, const int32_t*
   trans[b] = found;
 }
 
+__global__ void row_max(const int32_t* __restrict__ rowptr, int64_t nnode, int32_t* __restrict__ mx) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  atomicMax(mx, rowptr[n + 1] - rowptr[n]);  // integer max, order-independent
+}
+
 // scalar CSR view of a vec-block pattern
 __global__ void csr_view(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
                          int64_t nnode, int vec, int32_t* __restrict__ srow, int32_t* __restrict__ scol) {
@@ -101,34 +116,129 @@ __global__ void csr_view(const int32_t* __restrict__ rowptr, const int32_t* __re
     for (int j = 0; j < vec; ++j) scol[start + (int64_t)(b - b0) * vec + j] = colidx[b] * vec + j;
 }
 
-template <typename T>
-__global__ void bsr_to_csr_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ brow,
-                                  int64_t nblocks, int vec, const T* __restrict__ bsr, T* __restrict__ csr) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int bb = vec * vec;
-  if (t >= nblocks * bb) return;
-  const int64_t b = t / bb;
-  const int e = (int)(t - b * bb), i = e / vec, j = e - (e / vec) * vec;
-  const int32_t n = brow[b];
-  const int32_t b0 = rowptr[n], len = (rowptr[n + 1] - b0) * vec;
-  csr[(int64_t)b0 * bb + (int64_t)i * len + (b - b0) * vec + j] = bsr[t];
+// Matrix values: sliced ELL ("SELL").  Node rows are grouped in slices of R
+// consecutive rows (R = 21 for vec = 3, so a slice is 63 scalar rows = one
+// wavefront lane each; R = 64 for vec = 1).  Slice s has width W_s = its
+// longest row (in blocks); its slots are slice_ptr[s] .. slice_ptr[s+1]-1.
+// Slot t holds, for each row r of the slice, the r-th row's j-th block
+// (j = t - slice_ptr[s]) or zero padding.  With L = R*vec lanes and
+// entry (i, jj) of a block:
+//     value  vals[t*R*vec*vec + (jj*R + r)*vec + i]   (= base + jj*L + i,
+//            base = t*R*vec*vec + r*vec)
+//     column sell_col[t*R + r]
+// so for fixed (t, jj) the L lanes read L consecutive values.
+__global__ void slice_width(const int32_t* __restrict__ rowptr, int64_t nnode, int R, int64_t nslices,
+                            int32_t* __restrict__ W) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nslices) return;
+  if (s == nslices) { W[s] = 0; return; }
+  int32_t w = 0;
+  for (int r = 0; r < R; ++r) {
+    const int64_t n = s * R + r;
+    if (n < nnode) w = max(w, rowptr[n + 1] - rowptr[n]);
+  }
+  W[s] = w;
 }
 
-// One thread per (block, entry): sum the entry over the block's contributions
-// in cell order.  Element matrices are block-major (cell, a, b, i, k).
-template <typename T, int BB>
-__global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ seg,
+__global__ void sell_fill(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                          const int32_t* __restrict__ slice_ptr, int64_t nnode, int R, int vec, int64_t nslices,
+                          int32_t* __restrict__ sell_col, int32_t* __restrict__ blkpos) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // row slot of its slice
+  if (n >= nslices * R) return;
+  const int64_t s = n / R;
+  const int r = (int)(n - s * R);
+  const int32_t t0 = slice_ptr[s], t1 = slice_ptr[s + 1];
+  const int32_t b0 = n < nnode ? rowptr[n] : 0, len = n < nnode ? rowptr[n + 1] - b0 : 0;
+  for (int32_t t = t0; t < t1; ++t) {
+    const int32_t j = t - t0;
+    if (j < len) {
+      sell_col[(int64_t)t * R + r] = colidx[b0 + j];
+      blkpos[b0 + j] = (int32_t)((int64_t)t * R * vec * vec + (int64_t)r * vec);
+    } else {
+      sell_col[(int64_t)t * R + r] = n < nnode ? (int32_t)n : 0;
+    }
+  }
+}
+
+__global__ void sell_tpos_fill(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ trans,
+                               const int32_t* __restrict__ blkpos, const int32_t* __restrict__ slice_ptr,
+                               int64_t nnode, int R, int64_t nslices, int32_t* __restrict__ sell_tpos) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nslices * R) return;
+  const int64_t s = n / R;
+  const int r = (int)(n - s * R);
+  const int32_t t0 = slice_ptr[s], t1 = slice_ptr[s + 1];
+  const int32_t b0 = n < nnode ? rowptr[n] : 0, len = n < nnode ? rowptr[n + 1] - b0 : 0;
+  for (int32_t t = t0; t < t1; ++t) {
+    const int32_t j = t - t0;
+    int32_t v = -1;
+    if (j < len) {
+      const int32_t kt = trans[b0 + j];
+      v = kt >= 0 ? blkpos[kt] : -1;
+    }
+    sell_tpos[(int64_t)t * R + r] = v;
+  }
+}
+
+__global__ void dpos_fill(const int32_t* __restrict__ diag, const int32_t* __restrict__ blkpos, int64_t nnode,
+                          int32_t* __restrict__ dpos) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  dpos[n] = diag[n] >= 0 ? blkpos[diag[n]] : -1;
+}
+
+// scalar CSR values (the order of hf_pattern_csr) from the SELL values
+template <typename T>
+__global__ void sell_to_csr_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ brow,
+                                   const int32_t* __restrict__ blkpos, int64_t nblocks, int vec, int R,
+                                   const T* __restrict__ vals, T* __restrict__ csr) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const int32_t n = brow[b];
+  const int32_t b0 = rowptr[n], len = (rowptr[n + 1] - b0) * vec;
+  const int64_t base = blkpos[b], L = (int64_t)R * vec;
+  for (int i = 0; i < vec; ++i)
+    for (int j = 0; j < vec; ++j)
+      csr[(int64_t)b0 * vec * vec + (int64_t)i * len + (b - b0) * vec + j] = vals[base + j * L + i];
+}
+
+// One thread per node row: every slot of the row's slice gets the sum of the
+// block's contributions (element blocks, block-major (cell, a, b, i, k), each
+// BB contiguous values) in cell order, or zeros for padding.
+template <typename T, int VEC>
+__global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ seg,
                                                        const int32_t* __restrict__ contrib,
-                                                       int64_t nblocks, const T* __restrict__ elem,
-                                                       T* __restrict__ bsr) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nblocks * BB) return;
-  const int64_t b = t / BB;
-  const int e = (int)(t - b * BB);
-  const int32_t s0 = seg[b], s1 = seg[b + 1];
-  T acc = 0;
-  for (int32_t s = s0; s < s1; ++s) acc += elem[(int64_t)contrib[s] * BB + e];
-  bsr[t] = acc;
+                                                       const int32_t* __restrict__ slice_ptr, int64_t nnode,
+                                                       int R, int64_t nslices, const T* __restrict__ elem,
+                                                       T* __restrict__ vals) {
+  constexpr int BB = VEC * VEC;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nslices * R) return;
+  const int64_t s = n / R;
+  const int r = (int)(n - s * R);
+  const int64_t L = (int64_t)R * VEC;
+  const int32_t t0 = slice_ptr[s], t1 = slice_ptr[s + 1];
+  const int32_t b0 = n < nnode ? rowptr[n] : 0, len = n < nnode ? rowptr[n + 1] - b0 : 0;
+  for (int32_t t = t0; t < t1; ++t) {
+    const int32_t j = t - t0;
+    T acc[BB];
+#pragma unroll
+    for (int e = 0; e < BB; ++e) acc[e] = 0;
+    if (j < len) {
+      const int32_t b = b0 + j;
+      for (int32_t q = seg[b]; q < seg[b + 1]; ++q) {
+        const T* src = elem + (int64_t)contrib[q] * BB;
+#pragma unroll
+        for (int e = 0; e < BB; ++e) acc[e] += src[e];
+      }
+    }
+    T* dst = vals + (int64_t)t * R * BB + (int64_t)r * VEC;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i)
+#pragma unroll
+      for (int jj = 0; jj < VEC; ++jj) dst[jj * L + i] = acc[i * VEC + jj];
+  }
 }
 
 // R[n*vec+i] = sum over the cells holding node n (diag-block contributions,
@@ -156,17 +266,22 @@ __global__ void gather_kernel(const int32_t* __restrict__ diag, const int32_t* _
 }
 
 template <typename T>
-__global__ void bc_rows_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ diag,
-                               const int32_t* __restrict__ dofs, int64_t nbc, int vec,
-                               T* __restrict__ bsr) {
+__global__ void bc_rows_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ slice_ptr,
+                               const int32_t* __restrict__ dpos, const int32_t* __restrict__ dofs, int64_t nbc,
+                               int vec, int R, T* __restrict__ vals) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbc) return;
   const int32_t d = dofs[t];
   const int32_t n = d / vec, i = d - (d / vec) * vec;
-  const int bb = vec * vec;
-  for (int32_t b = rowptr[n]; b < rowptr[n + 1]; ++b)
-    for (int k = 0; k < vec; ++k) bsr[(int64_t)b * bb + i * vec + k] = (T)0;
-  if (diag[n] >= 0) bsr[(int64_t)diag[n] * bb + i * vec + i] = (T)1;
+  const int64_t s = n / R;
+  const int r = (int)(n - s * R);
+  const int64_t L = (int64_t)R * vec;
+  const int32_t len = rowptr[n + 1] - rowptr[n];
+  for (int32_t j = 0; j < len; ++j) {
+    T* base = vals + (int64_t)(slice_ptr[s] + j) * R * vec * vec + (int64_t)r * vec;
+    for (int jj = 0; jj < vec; ++jj) base[jj * L + i] = (T)0;
+  }
+  if (dpos[n] >= 0) vals[(int64_t)dpos[n] + i * L + i] = (T)1;
 }
 
 __global__ void bc_residual_kernel(const int32_t* __restrict__ dofs, const double* __restrict__ vals,
@@ -187,14 +302,14 @@ __global__ void bc_x0_kernel(const int32_t* __restrict__ dofs, const double* __r
 }
 
 template <typename T>
-__global__ void jacobi_kernel(const int32_t* __restrict__ diag, int64_t nnode, int vec,
-                              const T* __restrict__ bsr, T* __restrict__ dinv) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nnode * vec) return;
-  const int64_t n = r / vec;
-  const int i = (int)(r - n * vec);
-  const int32_t b = diag[n];
-  dinv[r] = b >= 0 ? (T)1 / bsr[(int64_t)b * vec * vec + i * vec + i] : (T)1;
+__global__ void jacobi_kernel(const int32_t* __restrict__ dpos, int64_t nnode, int vec, int R,
+                              const T* __restrict__ vals, T* __restrict__ dinv) {
+  const int64_t rr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (rr >= nnode * vec) return;
+  const int64_t n = rr / vec;
+  const int i = (int)(rr - n * vec);
+  const int32_t b = dpos[n];
+  dinv[rr] = b >= 0 ? (T)1 / vals[(int64_t)b + (int64_t)i * R * vec + i] : (T)1;
 }
 
 }  // namespace hf
@@ -231,6 +346,9 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
     }                                                                                     \
   } while (0)
   p->rowptr = p->colidx = p->seg = p->contrib = p->diag = p->trans = p->brow = nullptr;
+  p->slice_ptr = p->sell_col = p->sell_tpos = p->blkpos = p->dpos = nullptr;
+  p->R = (vec == 3) ? 21 : 64;
+  p->nslices = (nnode + p->R - 1) / p->R;
   PB_CHECK(hipMalloc(&keys, n * sizeof(uint64_t)));
   PB_CHECK(hipMalloc(&keys_s, n * sizeof(uint64_t)));
   PB_CHECK(hipMalloc(&ids, n * sizeof(int32_t)));
@@ -278,6 +396,50 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
   PB_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, rowcount, p->rowptr, (int)(nnode + 1), stream));
   transpose_map<<<grid_for(nb), 256, 0, stream>>>(p->rowptr, p->colidx, p->brow, nb, p->trans);
   PB_CHECK(hipGetLastError());
+  {
+    // longest block row (selects the SpMV variant); rowcount is no longer needed
+    int32_t* mx = rowcount + nnode;
+    PB_CHECK(hipMemsetAsync(mx, 0, sizeof(int32_t), stream));
+    row_max<<<grid_for(nnode), 256, 0, stream>>>(p->rowptr, nnode, mx);
+    PB_CHECK(hipGetLastError());
+    PB_CHECK(hipMemcpyAsync(&p->max_row_blocks, mx, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+  }
+  {
+    // sliced-ELL layout
+    const int64_t ns = p->nslices;
+    int32_t* W = nullptr;
+    PB_CHECK(hipMalloc(&W, (ns + 1) * sizeof(int32_t)));
+    PB_CHECK(hipMalloc(&p->slice_ptr, (ns + 1) * sizeof(int32_t)));
+    slice_width<<<grid_for(ns + 1), 256, 0, stream>>>(p->rowptr, nnode, p->R, ns, W);
+    PB_CHECK(hipGetLastError());
+    size_t need = 0;
+    PB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, W, p->slice_ptr, (int)(ns + 1), stream));
+    if (need > tmp_bytes) {
+      hipFree(tmp); tmp = nullptr;
+      tmp_bytes = need;
+      PB_CHECK(hipMalloc(&tmp, tmp_bytes));
+    }
+    PB_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, W, p->slice_ptr, (int)(ns + 1), stream));
+    int32_t nslots = 0;
+    PB_CHECK(hipMemcpyAsync(&nslots, p->slice_ptr + ns, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    PB_CHECK(hipStreamSynchronize(stream));
+    hipFree(W);
+    p->nslots = nslots;
+    const int64_t cells_sell = (int64_t)nslots * p->R;
+    HF_REQUIRE(cells_sell * vec * vec < (int64_t)INT32_MAX, "hf_pattern_build: SELL values exceed int32 indexing");
+    PB_CHECK(hipMalloc(&p->sell_col, cells_sell * sizeof(int32_t)));
+    PB_CHECK(hipMalloc(&p->sell_tpos, cells_sell * sizeof(int32_t)));
+    PB_CHECK(hipMalloc(&p->blkpos, nb * sizeof(int32_t)));
+    PB_CHECK(hipMalloc(&p->dpos, nnode * sizeof(int32_t)));
+    sell_fill<<<grid_for(ns * p->R), 256, 0, stream>>>(p->rowptr, p->colidx, p->slice_ptr, nnode, p->R, vec, ns,
+                                                      p->sell_col, p->blkpos);
+    PB_CHECK(hipGetLastError());
+    sell_tpos_fill<<<grid_for(ns * p->R), 256, 0, stream>>>(p->rowptr, p->trans, p->blkpos, p->slice_ptr, nnode,
+                                                           p->R, ns, p->sell_tpos);
+    PB_CHECK(hipGetLastError());
+    dpos_fill<<<grid_for(nnode), 256, 0, stream>>>(p->diag, p->blkpos, nnode, p->dpos);
+    PB_CHECK(hipGetLastError());
+  }
   PB_CHECK(hipStreamSynchronize(stream));
   hipFree(keys_s); hipFree(flags); hipFree(scan); hipFree(rowcount); hipFree(tmp);
 #undef PB_CHECK
@@ -291,6 +453,10 @@ extern "C" int hf_pattern_sizes(const hf_pattern* p, int64_t* nblocks, int64_t* 
   if (nnz) *nnz = p->nblocks * p->vec * p->vec;
   if (ncontrib) *ncontrib = p->ncontrib;
   return 0;
+}
+
+extern "C" int64_t hf_matrix_values(const hf_pattern* p) {
+  return p ? p->nslots * (int64_t)p->R * p->vec * p->vec : -1;
 }
 
 extern "C" int hf_pattern_arrays(const hf_pattern* p, const int32_t** rowptr, const int32_t** colidx,
@@ -316,37 +482,39 @@ extern "C" int hf_pattern_destroy(hf_pattern* p) {
   if (!p) return 0;
   hipFree(p->rowptr); hipFree(p->colidx); hipFree(p->seg); hipFree(p->contrib);
   hipFree(p->diag); hipFree(p->trans); hipFree(p->brow);
+  hipFree(p->slice_ptr); hipFree(p->sell_col); hipFree(p->sell_tpos); hipFree(p->blkpos); hipFree(p->dpos);
   delete p;
   return 0;
 }
 
-extern "C" int hf_bsr_to_csr_f64(const hf_pattern* p, const double* bsr, double* csr, hf_stream_t s) {
-  HF_REQUIRE(p && bsr && csr, "hf_bsr_to_csr_f64: null argument");
-  const int64_t n = p->nblocks * p->vec * p->vec;
-  bsr_to_csr_kernel<double><<<grid_for(n), 256, 0, (hipStream_t)s>>>(p->rowptr, p->brow, p->nblocks, p->vec,
-                                                                        bsr, csr);
+extern "C" int hf_matrix_to_csr_f64(const hf_pattern* p, const double* vals, double* csr, hf_stream_t s) {
+  HF_REQUIRE(p && vals && csr, "hf_matrix_to_csr_f64: null argument");
+  sell_to_csr_kernel<double><<<grid_for(p->nblocks), 256, 0, (hipStream_t)s>>>(p->rowptr, p->brow, p->blkpos,
+                                                                                 p->nblocks, p->vec, p->R, vals, csr);
   HF_LAUNCH_CHECK();
   return 0;
 }
 
 template <typename T>
-static int assemble_impl(const hf_pattern* p, const T* elem, T* bsr, hipStream_t s) {
-  HF_REQUIRE(p && elem && bsr, "hf_assemble_bsr: null argument");
-  const int64_t n = p->nblocks * p->vec * p->vec;
+static int assemble_impl(const hf_pattern* p, const T* elem, T* vals, hipStream_t s) {
+  HF_REQUIRE(p && elem && vals, "hf_assemble: null argument");
+  const int64_t n = p->nslices * p->R;
   if (p->vec == 3)
-    assemble_kernel<T, 9><<<grid_for(n), 256, 0, s>>>(p->seg, p->contrib, p->nblocks, elem, bsr);
+    assemble_kernel<T, 3><<<grid_for(n), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr, p->nnode, p->R,
+                                                       p->nslices, elem, vals);
   else if (p->vec == 1)
-    assemble_kernel<T, 1><<<grid_for(n), 256, 0, s>>>(p->seg, p->contrib, p->nblocks, elem, bsr);
+    assemble_kernel<T, 1><<<grid_for(n), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr, p->nnode, p->R,
+                                                       p->nslices, elem, vals);
   else
-    HF_REQUIRE(false, "hf_assemble_bsr: vec=%d not supported", p->vec);
+    HF_REQUIRE(false, "hf_assemble: vec=%d not supported", p->vec);
   HF_LAUNCH_CHECK();
   return 0;
 }
-extern "C" int hf_assemble_bsr_f64(const hf_pattern* p, const double* e, double* b, hf_stream_t s) {
-  return assemble_impl<double>(p, e, b, (hipStream_t)s);
+extern "C" int hf_assemble_f64(const hf_pattern* p, const double* e, double* v, hf_stream_t s) {
+  return assemble_impl<double>(p, e, v, (hipStream_t)s);
 }
-extern "C" int hf_assemble_bsr_f32(const hf_pattern* p, const float* e, float* b, hf_stream_t s) {
-  return assemble_impl<float>(p, e, b, (hipStream_t)s);
+extern "C" int hf_assemble_f32(const hf_pattern* p, const float* e, float* v, hf_stream_t s) {
+  return assemble_impl<float>(p, e, v, (hipStream_t)s);
 }
 
 extern "C" int hf_gather_residual_f64(const hf_pattern* p, const double* elem, const double* fext, double* R,
@@ -365,10 +533,10 @@ extern "C" int hf_gather_residual_f64(const hf_pattern* p, const double* elem, c
 }
 
 template <typename T>
-static int bc_rows_impl(const hf_pattern* p, const int32_t* dofs, int64_t nbc, T* bsr, hipStream_t s) {
-  HF_REQUIRE(p && bsr, "hf_bc_rows: null argument");
+static int bc_rows_impl(const hf_pattern* p, const int32_t* dofs, int64_t nbc, T* vals, hipStream_t s) {
+  HF_REQUIRE(p && vals, "hf_bc_rows: null argument");
   if (nbc == 0) return 0;
-  bc_rows_kernel<T><<<grid_for(nbc), 256, 0, s>>>(p->rowptr, p->diag, dofs, nbc, p->vec, bsr);
+  bc_rows_kernel<T><<<grid_for(nbc), 256, 0, s>>>(p->rowptr, p->slice_ptr, p->dpos, dofs, nbc, p->vec, p->R, vals);
   HF_LAUNCH_CHECK();
   return 0;
 }
@@ -399,7 +567,7 @@ extern "C" int hf_bc_initial_guess_f64(const int32_t* dofs, const double* vals, 
 template <typename T>
 static int jacobi_impl(const hf_pattern* p, const T* bsr, T* dinv, hipStream_t s) {
   HF_REQUIRE(p && bsr && dinv, "hf_jacobi: null argument");
-  jacobi_kernel<T><<<grid_for(p->nnode * p->vec), 256, 0, s>>>(p->diag, p->nnode, p->vec, bsr, dinv);
+  jacobi_kernel<T><<<grid_for(p->nnode * p->vec), 256, 0, s>>>(p->dpos, p->nnode, p->vec, p->R, bsr, dinv);
   HF_LAUNCH_CHECK();
   return 0;
 }
@@ -416,5 +584,12 @@ const int32_t* pattern_rowptr(const hf_pattern* p) { return p->rowptr; }
 const int32_t* pattern_colidx(const hf_pattern* p) { return p->colidx; }
 const int32_t* pattern_trans(const hf_pattern* p) { return p->trans; }
 int64_t pattern_nnode(const hf_pattern* p) { return p->nnode; }
+int64_t pattern_nblocks(const hf_pattern* p) { return p->nblocks; }
+int pattern_R(const hf_pattern* p) { return p->R; }
+int64_t pattern_nslices(const hf_pattern* p) { return p->nslices; }
+const int32_t* pattern_slice_ptr(const hf_pattern* p) { return p->slice_ptr; }
+const int32_t* pattern_sell_col(const hf_pattern* p) { return p->sell_col; }
+const int32_t* pattern_sell_tpos(const hf_pattern* p) { return p->sell_tpos; }
+int pattern_max_row_blocks(const hf_pattern* p) { return p->max_row_blocks; }
 int pattern_vec(const hf_pattern* p) { return p->vec; }
 }  // namespace hf

@@ -17,9 +17,12 @@
 // so that each XCD (workgroups are dealt round-robin over 8 XCDs) works on one
 // contiguous range of rows: neighbouring rows share x entries in that XCD's L2.
 //
-// Reductions: every workgroup writes FP64 partial sums; the last workgroup to
-// finish (atomic ticket) adds them in a fixed order and updates the scalars.
-// Results are bitwise reproducible run to run.
+// Reductions: every workgroup of a vector kernel writes its FP64 partial sums
+// with plain stores; a one-workgroup scalar kernel launched next adds them in
+// a fixed order and updates the scalars (alpha, beta, flags).  No fences or
+// atomics between workgroups: on a multi-XCD GPU a device-scope release per
+// workgroup costs an L2 write-back, while the kernel boundary already makes
+// the partials visible.  Results are bitwise reproducible run to run.
 #include "common.hpp"
 #include "hipfem.h"
 
@@ -28,18 +31,23 @@ const int32_t* pattern_rowptr(const hf_pattern* p);
 const int32_t* pattern_colidx(const hf_pattern* p);
 const int32_t* pattern_trans(const hf_pattern* p);
 int64_t pattern_nnode(const hf_pattern* p);
+int64_t pattern_nblocks(const hf_pattern* p);
+int pattern_max_row_blocks(const hf_pattern* p);
+int64_t pattern_nslices(const hf_pattern* p);
+const int32_t* pattern_slice_ptr(const hf_pattern* p);
+const int32_t* pattern_sell_col(const hf_pattern* p);
+const int32_t* pattern_sell_tpos(const hf_pattern* p);
 int pattern_vec(const hf_pattern* p);
 
 constexpr int kBlock = 256;         // threads per workgroup
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kMaxGrid = 2048;      // workgroups of the reduction kernels
+constexpr int kMaxGrid = 8192;      // workgroups of the reduction kernels
 constexpr int kNumXcd = 8;
 
 struct Scalars {
   double rho, rho_, alpha, omega, rr, bb, thresh, pq, ss, ts, tt;
   int32_t iter, done, status, exit_early;
-  uint32_t ticket;
-  int32_t maxiter;
+  int32_t maxiter, pad;
 };
 
 // Logical workgroup id, remapped so that each XCD owns a contiguous range.
@@ -49,13 +57,10 @@ __device__ __forceinline__ int logical_block() {
   return (b % kNumXcd) * (g / kNumXcd) + b / kNumXcd;
 }
 
-// Block + grid reduction of NV doubles.  Returns true in the last workgroup,
-// where `tot` then holds the grid totals (identical in every thread).
+// Block reduction of NV doubles; thread 0 writes this workgroup's partials.
 template <int NV>
-__device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, uint32_t* ticket,
-                            double (&tot)[NV]) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
   __shared__ double sm[kWavesPerBlock][NV];
-  __shared__ int last;
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
@@ -70,17 +75,19 @@ __device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, uint
       for (int k = 0; k < kWavesPerBlock; ++k) s += sm[k][j];
       partials[(int64_t)blockIdx.x * NV + j] = s;
     }
-    __threadfence();
-    const uint32_t t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+
+// Sum of nparts partials in a fixed order, for a one-workgroup kernel; every
+// thread receives the totals.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nparts, double (&tot)[NV]) {
+  __shared__ double sm[kWavesPerBlock][NV];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   double acc[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) acc[j] = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
+  for (int b = threadIdx.x; b < nparts; b += kBlock)
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] += partials[(int64_t)b * NV + j];
 #pragma unroll
@@ -95,162 +102,174 @@ __device__ bool grid_reduce(double (&v)[NV], double* __restrict__ partials, uint
     for (int k = 0; k < kWavesPerBlock; ++k) s += sm[k][j];
     tot[j] = s;
   }
-  if (threadIdx.x == 0) *ticket = 0;
-  return true;
 }
 
-// y_i (all lanes) = sum over the row's blocks of A_blk[i][j] * xv(col*VEC + j).
-// TRANS: use the transposed block trans[k], entry [j][i].
-template <typename T, int VEC, bool TRANS, typename XF>
-__device__ __forceinline__ void bsr_row(int64_t n, const int32_t* __restrict__ rowptr,
-                                        const int32_t* __restrict__ colidx, const int32_t* __restrict__ trans,
-                                        const T* __restrict__ vals, XF xv, double (&y)[VEC]) {
-  constexpr int BB = VEC * VEC;
-  constexpr int G = kWave / BB;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int e = lane % BB, boff = lane / BB;
-  const int i = e / VEC, j = e - (e / VEC) * VEC;
-  const int32_t k0 = rowptr[n], k1 = rowptr[n + 1];
-  double acc = 0;
-  if (boff < G) {
-    for (int32_t k = k0 + boff; k < k1; k += G) {
-      const int32_t c = colidx[k];
-      T a;
-      if constexpr (TRANS) {
-        const int32_t kt = trans[k];
-        a = kt >= 0 ? vals[(int64_t)kt * BB + j * VEC + i] : (T)0;
-      } else {
-        a = __builtin_nontemporal_load(&vals[(int64_t)k * BB + e]);
-      }
-      acc += (double)(a * xv((int64_t)c * VEC + j));
-    }
-  }
+// ---------------------------------------------------------------------------
+// Slice sweeps over the sliced-ELL matrix (layout in assembly.hip): one
+// wavefront per slice of R node rows, lane = r*VEC + i owns scalar row
+// VEC*(slice*R + r) + i.  For every slot t of the slice the lane reads the
+// block column c = sell_col[t*R + r] and the VEC values of its row of the
+// block (each a coalesced run of L = R*VEC floats across the wavefront),
+// gathers x at c, and accumulates.  Slots are independent, so the unrolled
+// loop keeps several slots' loads in flight; no cross-lane reduction.
+// TRANS (A^T x): the block of A^T in slot t is the transpose of the block
+// whose value base is sell_tpos[t*R + r]; entry (i, jj) = vals[base + i*L + jj].
+struct MatArgs {
+  int64_t nrows, nslices;
+  const int32_t* slice_ptr;
+  const int32_t* sell_col;
+  const int32_t* sell_tpos;
+};
+
+template <int VEC>
+struct SellShape {
+  static constexpr int R = (VEC == 3) ? 21 : 64;
+  static constexpr int L = R * VEC;
+  static constexpr int BB = VEC * VEC;
+};
+
+template <typename T, int VEC, bool TRANS, typename XF, typename EPI>
+__device__ __forceinline__ void sweep(const MatArgs& m, int64_t s0, int64_t s1, const T* __restrict__ vals, XF xv,
+                                      EPI epi) {
+  using S = SellShape<VEC>;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int r = lane / VEC, i = lane - (lane / VEC) * VEC;
+  const bool in_slice = lane < S::L;
+  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    const int32_t t0 = m.slice_ptr[s], t1 = m.slice_ptr[s + 1];
+    T y = 0;
+    if (in_slice) {
+#pragma unroll 4
+      for (int32_t t = t0; t < t1; ++t) {
+        const int64_t cr = (int64_t)t * S::R + r;
+        const int64_t c = (int64_t)m.sell_col[cr] * VEC;
+        if constexpr (TRANS) {
+          const int32_t tp = m.sell_tpos[cr];
 #pragma unroll
-  for (int ii = 0; ii < VEC; ++ii) y[ii] = wave_sum((boff < G && i == ii) ? acc : 0.0);
+          for (int jj = 0; jj < VEC; ++jj) {
+            const T a = tp >= 0 ? vals[(int64_t)tp + i * S::L + jj] : (T)0;
+            y += a * xv(c + jj);
+          }
+        } else {
+          const T* vrow = vals + (int64_t)t * S::R * S::BB + lane;
+#pragma unroll
+          for (int jj = 0; jj < VEC; ++jj) y += __builtin_nontemporal_load(vrow + jj * S::L) * xv(c + jj);
+        }
+      }
+    }
+    const int64_t n = s * S::R + r;
+    if (in_slice && n < m.nrows) epi(n, i, y);
+  }
 }
 
-// Contiguous row range of this wave's workgroup.
-__device__ __forceinline__ void row_range(int64_t nrows, int64_t& r0, int64_t& r1) {
-  const int64_t per = ceil_div(nrows, gridDim.x);
+// Contiguous slice range of this workgroup (XCD-remapped).
+__device__ __forceinline__ void row_range(int64_t nslices, int64_t& r0, int64_t& r1) {
+  const int64_t per = ceil_div(nslices, gridDim.x);
   r0 = (int64_t)logical_block() * per;
-  r1 = r0 + per < nrows ? r0 + per : nrows;
+  r1 = r0 + per < nslices ? r0 + per : nslices;
 }
 
 // ---------------------------------------------------------------------------
 // plain SpMV y = A x (or A^T x)
 template <typename T, int VEC, bool TRANS>
-__global__ void __launch_bounds__(kBlock) spmv_kernel(int64_t nrows, const int32_t* __restrict__ rowptr,
-                                                      const int32_t* __restrict__ colidx,
-                                                      const int32_t* __restrict__ trans,
-                                                      const T* __restrict__ vals, const T* __restrict__ x,
-                                                      T* __restrict__ y) {
+__global__ void __launch_bounds__(kBlock) spmv_kernel(MatArgs m, const T* __restrict__ vals,
+                                                      const T* __restrict__ x, T* __restrict__ y) {
   int64_t r0, r1;
-  row_range(nrows, r0, r1);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  row_range(m.nslices, r0, r1);
   auto xv = [&](int64_t c) { return x[c]; };
-  for (int64_t n = r0 + w; n < r1; n += kWavesPerBlock) {
-    double yy[VEC];
-    bsr_row<T, VEC, TRANS>(n, rowptr, colidx, trans, vals, xv, yy);
-    if (lane < VEC) {
-      double o = yy[0];
-#pragma unroll
-      for (int ii = 1; ii < VEC; ++ii) o = (lane == ii) ? yy[ii] : o;
-      y[n * VEC + lane] = (T)o;
-    }
-  }
+  auto epi = [&](int64_t n, int i, T v) { y[n * VEC + i] = v; };
+  sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
 }
 
 // ---------------------------------------------------------------------------
-// PCG (Jacobi).  init: r = b - A x, z = dinv r, p_old = 0, beta = 0.
+// PCG (Jacobi).  init: r = b - A x, z = dinv r, p_old = 0; partials r.r, r.z, b.b
 template <typename T, int VEC, bool TRANS>
-__global__ void __launch_bounds__(kBlock) pcg_init_kernel(
-    int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-    const int32_t* __restrict__ trans, const T* __restrict__ vals, const T* __restrict__ dinv,
-    const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-    T* __restrict__ p_old, Scalars* sc, double* partials, double rtol2, double atol2) {
+__global__ void __launch_bounds__(kBlock) pcg_init_kernel(MatArgs m, const T* __restrict__ vals,
+                                                          const T* __restrict__ dinv, const T* __restrict__ b,
+                                                          const T* __restrict__ x, T* __restrict__ r,
+                                                          T* __restrict__ z, T* __restrict__ p_old,
+                                                          double* partials) {
   int64_t r0, r1;
-  row_range(nrows, r0, r1);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  row_range(m.nslices, r0, r1);
   auto xv = [&](int64_t c) { return x[c]; };
-  double loc[3] = {0, 0, 0};  // r.r, r.z, b.b
-  for (int64_t n = r0 + w; n < r1; n += kWavesPerBlock) {
-    double yy[VEC];
-    bsr_row<T, VEC, TRANS>(n, rowptr, colidx, trans, vals, xv, yy);
-    if (lane < VEC) {
-      double ax = yy[0];
-#pragma unroll
-      for (int ii = 1; ii < VEC; ++ii) ax = (lane == ii) ? yy[ii] : ax;
-      const int64_t d = n * VEC + lane;
-      const T rr = (T)((double)b[d] - ax);
-      const T zz = dinv[d] * rr;
-      r[d] = rr;
-      z[d] = zz;
-      p_old[d] = (T)0;
-      loc[0] += (double)rr * (double)rr;
-      loc[1] += (double)rr * (double)zz;
-      loc[2] += (double)b[d] * (double)b[d];
-    }
-  }
-  double tot[3];
-  if (grid_reduce<3>(loc, partials, &sc->ticket, tot) && threadIdx.x == 0) {
-    sc->rr = tot[0];
-    sc->rho = tot[1];
-    sc->bb = tot[2];
-    sc->thresh = fmax(rtol2 * tot[2], atol2);
-    sc->alpha = 0;
-    sc->iter = 0;
-    sc->status = 0;
-    sc->done = (tot[0] <= sc->thresh) ? 1 : 0;
-    if (!isfinite(tot[0])) { sc->done = 1; sc->status = 3; }
-    sc->omega = 0;  // beta for the first direction (p_new = z)
-  }
+  double loc[3] = {0, 0, 0};
+  auto epi = [&](int64_t n, int i, T ax) {
+    const int64_t d = n * VEC + i;
+    const T rr = b[d] - ax;
+    const T zz = dinv[d] * rr;
+    r[d] = rr;
+    z[d] = zz;
+    p_old[d] = (T)0;
+    loc[0] += (double)rr * (double)rr;
+    loc[1] += (double)rr * (double)zz;
+    loc[2] += (double)b[d] * (double)b[d];
+  };
+  sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
+  block_partials<3>(loc, partials);
 }
 
-// p_new = z + beta p_old (on the fly), q = A p_new, alpha = rho / (p_new . q)
+// Stop threshold: max(rtol^2 * ref, atol^2), ref = ||b||^2 or ||r0||^2 (rel_r0).
+__global__ void __launch_bounds__(kBlock) pcg_init_scalars(Scalars* sc, const double* partials, int nparts,
+                                                           double rtol2, double atol2, int rel_r0, int maxiter) {
+  double tot[3];
+  sum_partials<3>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->rr = tot[0];
+  sc->rho = tot[1];
+  sc->bb = tot[2];
+  sc->thresh = fmax(rtol2 * (rel_r0 ? tot[0] : tot[2]), atol2);
+  sc->alpha = 0;
+  sc->omega = 0;  // beta of the first direction (p_new = z)
+  sc->iter = 0;
+  sc->status = 0;
+  sc->maxiter = maxiter;
+  sc->done = (tot[0] <= sc->thresh) ? 1 : 0;
+  if (!isfinite(tot[0])) { sc->done = 1; sc->status = 3; }
+  else if (maxiter <= 0 && !sc->done) { sc->done = 1; sc->status = 1; }
+}
+
+// p_new = z + beta p_old (formed at the gathered columns), q = A p_new; partial p_new.q
 template <typename T, int VEC, bool TRANS>
-__global__ void __launch_bounds__(kBlock) pcg_spmv_kernel(
-    int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-    const int32_t* __restrict__ trans, const T* __restrict__ vals, const T* __restrict__ z,
-    const T* __restrict__ p_old, T* __restrict__ p_new, T* __restrict__ q, Scalars* sc,
-    double* partials) {
+__global__ void __launch_bounds__(kBlock) pcg_spmv_kernel(MatArgs m, const T* __restrict__ vals,
+                                                          const T* __restrict__ z, const T* __restrict__ p_old,
+                                                          T* __restrict__ p_new, T* __restrict__ q,
+                                                          const Scalars* sc, double* partials) {
   if (sc->done) return;
   const T beta = (T)sc->omega;
   int64_t r0, r1;
-  row_range(nrows, r0, r1);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  row_range(m.nslices, r0, r1);
   auto xv = [&](int64_t c) { return z[c] + beta * p_old[c]; };
   double loc[1] = {0};
-  for (int64_t n = r0 + w; n < r1; n += kWavesPerBlock) {
-    double yy[VEC];
-    bsr_row<T, VEC, TRANS>(n, rowptr, colidx, trans, vals, xv, yy);
-    if (lane < VEC) {
-      double ap = yy[0];
-#pragma unroll
-      for (int ii = 1; ii < VEC; ++ii) ap = (lane == ii) ? yy[ii] : ap;
-      const int64_t d = n * VEC + lane;
-      const T pn = z[d] + beta * p_old[d];
-      p_new[d] = pn;
-      q[d] = (T)ap;
-      loc[0] += (double)pn * ap;
-    }
-  }
+  auto epi = [&](int64_t n, int i, T ap) {
+    const int64_t d = n * VEC + i;
+    const T pn = z[d] + beta * p_old[d];
+    p_new[d] = pn;
+    q[d] = ap;
+    loc[0] += (double)pn * (double)ap;
+  };
+  sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
+  block_partials<1>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) pcg_alpha_scalars(Scalars* sc, const double* partials, int nparts) {
+  if (sc->done) return;
   double tot[1];
-  if (grid_reduce<1>(loc, partials, &sc->ticket, tot) && threadIdx.x == 0) {
-    sc->pq = tot[0];
-    if (!(tot[0] > 0.0) || !isfinite(tot[0])) {
-      sc->done = 1;
-      sc->status = isfinite(tot[0]) ? 2 : 3;
-    } else {
-      sc->alpha = sc->rho / tot[0];
-    }
+  sum_partials<1>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->pq = tot[0];
+  if (!(tot[0] > 0.0) || !isfinite(tot[0])) {
+    sc->done = 1;
+    sc->status = isfinite(tot[0]) ? 2 : 3;
+  } else {
+    sc->alpha = sc->rho / tot[0];
   }
 }
 
-// x += alpha p, r -= alpha q, z = dinv r, beta = r.z / rho
+// x += alpha p, r -= alpha q, z = dinv r; partials r.z, r.r
 template <typename T>
 __global__ void __launch_bounds__(kBlock) pcg_update_kernel(
     int64_t N, const T* __restrict__ dinv, const T* __restrict__ p, const T* __restrict__ q,
-    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, Scalars* sc, double* partials) {
+    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const Scalars* sc, double* partials) {
   if (sc->done) return;
   const T alpha = (T)sc->alpha;
   const int64_t per = ceil_div(N, gridDim.x);
@@ -266,56 +285,60 @@ __global__ void __launch_bounds__(kBlock) pcg_update_kernel(
     loc[0] += (double)rr * (double)zz;
     loc[1] += (double)rr * (double)rr;
   }
+  block_partials<2>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) pcg_beta_scalars(Scalars* sc, const double* partials, int nparts) {
+  if (sc->done) return;
   double tot[2];
-  if (grid_reduce<2>(loc, partials, &sc->ticket, tot) && threadIdx.x == 0) {
-    sc->iter += 1;
-    sc->rr = tot[1];
-    sc->omega = tot[0] / sc->rho;  // beta
-    sc->rho = tot[0];
-    if (!isfinite(tot[1])) { sc->done = 1; sc->status = 3; }
-    else if (tot[1] <= sc->thresh) { sc->done = 1; sc->status = 0; }
-    else if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; }
-  }
+  sum_partials<2>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->iter += 1;
+  sc->rr = tot[1];
+  sc->omega = tot[0] / sc->rho;  // beta
+  sc->rho = tot[0];
+  if (!isfinite(tot[1])) { sc->done = 1; sc->status = 3; }
+  else if (tot[1] <= sc->thresh) { sc->done = 1; sc->status = 0; }
+  else if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; }
 }
 
 // ---------------------------------------------------------------------------
 // BiCGSTAB (jax semantics).  Vectors: r, rhat, p, q, phat, s, shat, t.
 template <typename T, int VEC, bool TRANS>
-__global__ void __launch_bounds__(kBlock) bicg_init_kernel(
-    int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-    const int32_t* __restrict__ trans, const T* __restrict__ vals, const T* __restrict__ b,
-    const T* __restrict__ x, T* __restrict__ r, T* __restrict__ rhat, T* __restrict__ p,
-    T* __restrict__ q, Scalars* sc, double* partials, double rtol2, double atol2) {
+__global__ void __launch_bounds__(kBlock) bicg_init_kernel(MatArgs m, const T* __restrict__ vals,
+                                                           const T* __restrict__ b, const T* __restrict__ x,
+                                                           T* __restrict__ r, T* __restrict__ rhat, T* __restrict__ p,
+                                                           T* __restrict__ q, double* partials) {
   int64_t r0, r1;
-  row_range(nrows, r0, r1);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  row_range(m.nslices, r0, r1);
   auto xv = [&](int64_t c) { return x[c]; };
   double loc[2] = {0, 0};  // r.r (= rhat.r), b.b
-  for (int64_t n = r0 + w; n < r1; n += kWavesPerBlock) {
-    double yy[VEC];
-    bsr_row<T, VEC, TRANS>(n, rowptr, colidx, trans, vals, xv, yy);
-    if (lane < VEC) {
-      double ax = yy[0];
-#pragma unroll
-      for (int ii = 1; ii < VEC; ++ii) ax = (lane == ii) ? yy[ii] : ax;
-      const int64_t d = n * VEC + lane;
-      const T rr = (T)((double)b[d] - ax);
-      r[d] = rr; rhat[d] = rr; p[d] = rr; q[d] = rr;
-      loc[0] += (double)rr * (double)rr;
-      loc[1] += (double)b[d] * (double)b[d];
-    }
-  }
+  auto epi = [&](int64_t n, int i, T ax) {
+    const int64_t d = n * VEC + i;
+    const T rr = b[d] - ax;
+    r[d] = rr; rhat[d] = rr; p[d] = rr; q[d] = rr;
+    loc[0] += (double)rr * (double)rr;
+    loc[1] += (double)b[d] * (double)b[d];
+  };
+  sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
+  block_partials<2>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) bicg_init_scalars(Scalars* sc, const double* partials, int nparts,
+                                                            double rtol2, double atol2, int rel_r0, int maxiter) {
   double tot[2];
-  if (grid_reduce<2>(loc, partials, &sc->ticket, tot) && threadIdx.x == 0) {
-    sc->rr = tot[0];
-    sc->rho_ = tot[0];
-    sc->bb = tot[1];
-    sc->rho = 1.0; sc->alpha = 1.0; sc->omega = 1.0;
-    sc->thresh = fmax(rtol2 * tot[1], atol2);
-    sc->iter = 0; sc->status = 0; sc->exit_early = 0;
-    sc->done = (tot[0] <= sc->thresh) ? 1 : 0;
-    if (!isfinite(tot[0])) { sc->done = 1; sc->status = 3; }
-  }
+  sum_partials<2>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->rr = tot[0];
+  sc->rho_ = tot[0];
+  sc->bb = tot[1];
+  sc->rho = 1.0; sc->alpha = 1.0; sc->omega = 1.0;
+  sc->thresh = fmax(rtol2 * (rel_r0 ? tot[0] : tot[1]), atol2);
+  sc->iter = 0; sc->status = 0; sc->exit_early = 0;
+  sc->maxiter = maxiter;
+  sc->done = (tot[0] <= sc->thresh) ? 1 : 0;
+  if (!isfinite(tot[0])) { sc->done = 1; sc->status = 3; }
+  else if (maxiter <= 0 && !sc->done) { sc->done = 1; sc->status = 1; }
 }
 
 // p = r + beta (p - omega q), phat = dinv p
@@ -334,51 +357,50 @@ __global__ void __launch_bounds__(kBlock) bicg_p_kernel(int64_t N, const T* __re
   }
 }
 
-// y = A v with a grid-reduced dot.  MODE 0: q = A phat, alpha = rho_ / (rhat . q)
-//                                   MODE 1: t = A shat, omega = (t.s) / (t.t)
+// out = A v; partials w1.out, out.out.  MODE 0: q = A phat (alpha); MODE 1: t = A shat (omega)
 template <typename T, int VEC, bool TRANS, int MODE>
-__global__ void __launch_bounds__(kBlock) bicg_spmv_kernel(
-    int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-    const int32_t* __restrict__ trans, const T* __restrict__ vals, const T* __restrict__ v,
-    const T* __restrict__ w1, T* __restrict__ out, Scalars* sc, double* partials) {
+__global__ void __launch_bounds__(kBlock) bicg_spmv_kernel(MatArgs m, const T* __restrict__ vals,
+                                                           const T* __restrict__ v, const T* __restrict__ w1,
+                                                           T* __restrict__ out, const Scalars* sc, double* partials) {
   if (sc->done || (MODE == 1 && sc->exit_early)) return;
   int64_t r0, r1;
-  row_range(nrows, r0, r1);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  row_range(m.nslices, r0, r1);
   auto xv = [&](int64_t c) { return v[c]; };
   double loc[2] = {0, 0};
-  for (int64_t n = r0 + w; n < r1; n += kWavesPerBlock) {
-    double yy[VEC];
-    bsr_row<T, VEC, TRANS>(n, rowptr, colidx, trans, vals, xv, yy);
-    if (lane < VEC) {
-      double o = yy[0];
-#pragma unroll
-      for (int ii = 1; ii < VEC; ++ii) o = (lane == ii) ? yy[ii] : o;
-      const int64_t d = n * VEC + lane;
-      const T ot = (T)o;
-      out[d] = ot;
-      loc[0] += (double)w1[d] * (double)ot;
-      loc[1] += (double)ot * (double)ot;
-    }
-  }
-  double tot[2];
-  if (grid_reduce<2>(loc, partials, &sc->ticket, tot) && threadIdx.x == 0) {
-    if (MODE == 0) {
-      sc->alpha = sc->rho_ / tot[0];
-      if (!isfinite(sc->alpha)) { sc->done = 1; sc->status = 3; }
-    } else {
-      sc->ts = tot[0];
-      sc->tt = tot[1];
-      sc->omega = tot[0] / tot[1];
-    }
-  }
+  auto epi = [&](int64_t n, int i, T o) {
+    const int64_t d = n * VEC + i;
+    out[d] = o;
+    loc[0] += (double)w1[d] * (double)o;
+    loc[1] += (double)o * (double)o;
+  };
+  sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
+  block_partials<2>(loc, partials);
 }
 
-// s = r - alpha q, shat = dinv s, exit_early = s.s < thresh
+__global__ void __launch_bounds__(kBlock) bicg_alpha_scalars(Scalars* sc, const double* partials, int nparts) {
+  if (sc->done) return;
+  double tot[2];
+  sum_partials<2>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->alpha = sc->rho_ / tot[0];
+  if (!isfinite(sc->alpha)) { sc->done = 1; sc->status = 3; }
+}
+
+__global__ void __launch_bounds__(kBlock) bicg_omega_scalars(Scalars* sc, const double* partials, int nparts) {
+  if (sc->done || sc->exit_early) return;
+  double tot[2];
+  sum_partials<2>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->ts = tot[0];
+  sc->tt = tot[1];
+  sc->omega = tot[0] / tot[1];
+}
+
+// s = r - alpha q, shat = dinv s; partial s.s
 template <typename T>
 __global__ void __launch_bounds__(kBlock) bicg_s_kernel(int64_t N, const T* __restrict__ dinv,
                                                         const T* __restrict__ r, const T* __restrict__ q,
-                                                        T* __restrict__ s, T* __restrict__ shat, Scalars* sc,
+                                                        T* __restrict__ s, T* __restrict__ shat, const Scalars* sc,
                                                         double* partials) {
   if (sc->done) return;
   const T alpha = (T)sc->alpha;
@@ -392,19 +414,24 @@ __global__ void __launch_bounds__(kBlock) bicg_s_kernel(int64_t N, const T* __re
     shat[i] = dinv[i] * ss;
     loc[0] += (double)ss * (double)ss;
   }
-  double tot[1];
-  if (grid_reduce<1>(loc, partials, &sc->ticket, tot) && threadIdx.x == 0) {
-    sc->ss = tot[0];
-    sc->exit_early = (tot[0] < sc->thresh) ? 1 : 0;
-  }
+  block_partials<1>(loc, partials);
 }
 
-// x += alpha phat (+ omega shat); r = s (- omega t); rho_ = rhat . r; convergence
+__global__ void __launch_bounds__(kBlock) bicg_s_scalars(Scalars* sc, const double* partials, int nparts) {
+  if (sc->done) return;
+  double tot[1];
+  sum_partials<1>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->ss = tot[0];
+  sc->exit_early = (tot[0] < sc->thresh) ? 1 : 0;
+}
+
+// x += alpha phat (+ omega shat); r = s (- omega t); partials rhat.r, r.r
 template <typename T>
 __global__ void __launch_bounds__(kBlock) bicg_x_kernel(int64_t N, const T* __restrict__ rhat,
                                                         const T* __restrict__ phat, const T* __restrict__ shat,
                                                         const T* __restrict__ s, const T* __restrict__ t,
-                                                        T* __restrict__ x, T* __restrict__ r, Scalars* sc,
+                                                        T* __restrict__ x, T* __restrict__ r, const Scalars* sc,
                                                         double* partials) {
   if (sc->done) return;
   const bool early = sc->exit_early != 0;
@@ -426,47 +453,45 @@ __global__ void __launch_bounds__(kBlock) bicg_x_kernel(int64_t N, const T* __re
     loc[0] += (double)rhat[i] * (double)rr;
     loc[1] += (double)rr * (double)rr;
   }
+  block_partials<2>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) bicg_end_scalars(Scalars* sc, const double* partials, int nparts) {
+  if (sc->done) return;
   double tot[2];
-  if (grid_reduce<2>(loc, partials, &sc->ticket, tot) && threadIdx.x == 0) {
-    sc->iter += 1;
-    const double rho_used = sc->rho_;
-    sc->rho = rho_used;
-    sc->rho_ = tot[0];
-    sc->rr = tot[1];
-    if (!isfinite(tot[1])) { sc->done = 1; sc->status = 3; }
-    else if (tot[1] <= sc->thresh) { sc->done = 1; sc->status = 0; }
-    else if ((!early && sc->omega == 0.0) || sc->alpha == 0.0 || rho_used == 0.0) { sc->done = 1; sc->status = 2; }
-    else if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; }
-    if (early && sc->status == 0) sc->done = 1;
-  }
+  sum_partials<2>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  const bool early = sc->exit_early != 0;
+  sc->iter += 1;
+  const double rho_used = sc->rho_;
+  sc->rho = rho_used;
+  sc->rho_ = tot[0];
+  sc->rr = tot[1];
+  if (!isfinite(tot[1])) { sc->done = 1; sc->status = 3; }
+  else if (tot[1] <= sc->thresh || early) { sc->done = 1; sc->status = 0; }
+  else if (sc->omega == 0.0 || sc->alpha == 0.0 || rho_used == 0.0) { sc->done = 1; sc->status = 2; }
+  else if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; }
 }
 
 // ---------------------------------------------------------------------------
-__global__ void set_maxiter_kernel(Scalars* sc, int32_t maxiter) {
-  sc->maxiter = maxiter;
-  sc->ticket = 0;
-  sc->done = 0;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(kBlock) dot_kernel(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
-                                                     double* partials, uint32_t* ticket, double* out) {
+                                                     double* partials) {
   const int64_t per = ceil_div(n, gridDim.x);
   const int64_t i0 = (int64_t)blockIdx.x * per;
   const int64_t i1 = i0 + per < n ? i0 + per : n;
   double loc[1] = {0};
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) loc[0] += (double)x[i] * (double)y[i];
+  block_partials<1>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) dot_final(const double* partials, int nparts, double* out) {
   double tot[1];
-  if (grid_reduce<1>(loc, partials, ticket, tot) && threadIdx.x == 0) *out = tot[0];
+  sum_partials<1>(partials, nparts, tot);
+  if (threadIdx.x == 0) *out = tot[0];
 }
 
 // ---------------------------------------------------------------------------
-struct Workspace {
-  Scalars* sc;
-  double* partials;
-  char* vec;  // vectors follow
-};
-
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 static constexpr size_t kHeader = 4096 + (size_t)kMaxGrid * 4 * sizeof(double);
 
@@ -500,8 +525,8 @@ static int prof_collect() {
   return 0;
 }
 
-static inline int rows_grid(int64_t nrows) {
-  int64_t g = ceil_div(nrows, kWavesPerBlock * 2);
+static inline int rows_grid(int64_t nslices) {
+  int64_t g = ceil_div(nslices, kWavesPerBlock);
   if (g > kMaxGrid) g = kMaxGrid;
   if (g >= 64) g = (g / kNumXcd) * kNumXcd;
   return (int)(g < 1 ? 1 : g);
@@ -513,11 +538,20 @@ static inline int elem_grid(int64_t n) {
   return (int)(g < 1 ? 1 : g);
 }
 
+static MatArgs mat_args(const hf_pattern* p) {
+  MatArgs m;
+  m.nrows = pattern_nnode(p);
+  m.nslices = pattern_nslices(p);
+  m.slice_ptr = pattern_slice_ptr(p);
+  m.sell_col = pattern_sell_col(p);
+  m.sell_tpos = pattern_sell_tpos(p);
+  return m;
+}
+
 template <typename T, int VEC, bool TRANS>
 static int spmv_launch(const hf_pattern* p, const T* vals, const T* x, T* y, hipStream_t s) {
-  const int64_t nrows = pattern_nnode(p);
-  spmv_kernel<T, VEC, TRANS><<<rows_grid(nrows), kBlock, 0, s>>>(nrows, pattern_rowptr(p), pattern_colidx(p),
-                                                                  pattern_trans(p), vals, x, y);
+  const MatArgs m = mat_args(p);
+  spmv_kernel<T, VEC, TRANS><<<rows_grid(m.nslices), kBlock, 0, s>>>(m, vals, x, y);
   HF_LAUNCH_CHECK();
   return 0;
 }
@@ -534,18 +568,17 @@ static int spmv_dispatch(const hf_pattern* p, const T* vals, const T* x, T* y, h
 
 template <typename T, int VEC, bool TRANS>
 static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b, T* x, int maxiter, double rtol,
-                   double atol, int poll, void* ws, hf_krylov_info* info, hipStream_t s) {
-  const int64_t nrows = pattern_nnode(p), N = nrows * VEC;
+                   double atol, int poll, int rel_r0, void* ws, hf_krylov_info* info, hipStream_t s) {
+  const MatArgs m = mat_args(p);
+  const int64_t N = m.nrows * VEC;
   Scalars* sc = (Scalars*)ws;
   double* partials = (double*)((char*)ws + 4096);
   T* v = (T*)((char*)ws + kHeader);
   const size_t stride = align256(N * sizeof(T)) / sizeof(T);
   T *r = v, *z = v + stride, *p0 = v + 2 * stride, *p1 = v + 3 * stride, *q = v + 4 * stride;
-  const int32_t *rp = pattern_rowptr(p), *ci = pattern_colidx(p), *tr = pattern_trans(p);
-  const int gr = rows_grid(nrows), ge = elem_grid(N);
-  set_maxiter_kernel<<<1, 1, 0, s>>>(sc, maxiter);
-  pcg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(nrows, rp, ci, tr, vals, dinv, b, x, r, z, p0, sc,
-                                                       partials, rtol * rtol, atol * atol);
+  const int gr = rows_grid(m.nslices), ge = elem_grid(N);
+  pcg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, dinv, b, x, r, z, p0, partials);
+  pcg_init_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   HF_LAUNCH_CHECK();
   Scalars host{};
   int it = 0;
@@ -554,9 +587,11 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
       T* pold = (it & 1) ? p1 : p0;
       T* pnew = (it & 1) ? p0 : p1;
       if (k == 0 && prof_begin(s)) return -2;
-      pcg_spmv_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(nrows, rp, ci, tr, vals, z, pold, pnew, q, sc, partials);
+      pcg_spmv_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, z, pold, pnew, q, sc, partials);
       if (k == 0 && prof_end(s)) return -2;
+      pcg_alpha_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr);
       pcg_update_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, pnew, q, x, r, z, sc, partials);
+      pcg_beta_scalars<<<1, kBlock, 0, s>>>(sc, partials, ge);
     }
     HF_LAUNCH_CHECK();
     HF_CHECK_HIP(hipMemcpyAsync(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
@@ -577,19 +612,18 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
 
 template <typename T, int VEC, bool TRANS>
 static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b, T* x, int maxiter, double rtol,
-                    double atol, int poll, void* ws, hf_krylov_info* info, hipStream_t s) {
-  const int64_t nrows = pattern_nnode(p), N = nrows * VEC;
+                    double atol, int poll, int rel_r0, void* ws, hf_krylov_info* info, hipStream_t s) {
+  const MatArgs m = mat_args(p);
+  const int64_t N = m.nrows * VEC;
   Scalars* sc = (Scalars*)ws;
   double* partials = (double*)((char*)ws + 4096);
   T* v = (T*)((char*)ws + kHeader);
   const size_t stride = align256(N * sizeof(T)) / sizeof(T);
   T *r = v, *rhat = v + stride, *pp = v + 2 * stride, *q = v + 3 * stride, *phat = v + 4 * stride,
     *sv = v + 5 * stride, *shat = v + 6 * stride, *t = v + 7 * stride;
-  const int32_t *rp = pattern_rowptr(p), *ci = pattern_colidx(p), *tr = pattern_trans(p);
-  const int gr = rows_grid(nrows), ge = elem_grid(N);
-  set_maxiter_kernel<<<1, 1, 0, s>>>(sc, maxiter);
-  bicg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(nrows, rp, ci, tr, vals, b, x, r, rhat, pp, q, sc, partials,
-                                                        rtol * rtol, atol * atol);
+  const int gr = rows_grid(m.nslices), ge = elem_grid(N);
+  bicg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, b, x, r, rhat, pp, q, partials);
+  bicg_init_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   HF_LAUNCH_CHECK();
   Scalars host{};
   int it = 0;
@@ -597,11 +631,15 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
     for (int k = 0; k < poll; ++k, ++it) {
       bicg_p_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, r, q, pp, phat, sc);
       if (k == 0 && prof_begin(s)) return -2;
-      bicg_spmv_kernel<T, VEC, TRANS, 0><<<gr, kBlock, 0, s>>>(nrows, rp, ci, tr, vals, phat, rhat, q, sc, partials);
+      bicg_spmv_kernel<T, VEC, TRANS, 0><<<gr, kBlock, 0, s>>>(m, vals, phat, rhat, q, sc, partials);
       if (k == 0 && prof_end(s)) return -2;
+      bicg_alpha_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr);
       bicg_s_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, r, q, sv, shat, sc, partials);
-      bicg_spmv_kernel<T, VEC, TRANS, 1><<<gr, kBlock, 0, s>>>(nrows, rp, ci, tr, vals, shat, sv, t, sc, partials);
+      bicg_s_scalars<<<1, kBlock, 0, s>>>(sc, partials, ge);
+      bicg_spmv_kernel<T, VEC, TRANS, 1><<<gr, kBlock, 0, s>>>(m, vals, shat, sv, t, sc, partials);
+      bicg_omega_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr);
       bicg_x_kernel<T><<<ge, kBlock, 0, s>>>(N, rhat, phat, shat, sv, t, x, r, sc, partials);
+      bicg_end_scalars<<<1, kBlock, 0, s>>>(sc, partials, ge);
     }
     HF_LAUNCH_CHECK();
     HF_CHECK_HIP(hipMemcpyAsync(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
@@ -622,15 +660,17 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
 
 template <typename T>
 static int krylov_dispatch(int method, const hf_pattern* p, const T* vals, const T* dinv, const T* b, T* x,
-                           int transpose, int maxiter, double rtol, double atol, int poll, void* ws,
+                           int flags, int maxiter, double rtol, double atol, int poll, void* ws,
                            hf_krylov_info* info, hipStream_t s) {
+  const int transpose = flags & HF_KRYLOV_TRANSPOSE;
+  const int rel_r0 = (flags & HF_KRYLOV_REL_R0) ? 1 : 0;
   HF_REQUIRE(p && vals && dinv && b && x && ws, "hf_krylov_solve: null argument");
   HF_REQUIRE(maxiter >= 0, "hf_krylov_solve: maxiter < 0");
   if (poll < 1) poll = 1;
   const int vec = pattern_vec(p);
   HF_REQUIRE(vec == 1 || vec == 3, "hf_krylov_solve: vec=%d not supported", vec);
   HF_REQUIRE(method == HF_KRYLOV_PCG || method == HF_KRYLOV_BICGSTAB, "hf_krylov_solve: unknown method %d", method);
-#define KRY(M, V, TR) M<T, V, TR>(p, vals, dinv, b, x, maxiter, rtol, atol, poll, ws, info, s)
+#define KRY(M, V, TR) M<T, V, TR>(p, vals, dinv, b, x, maxiter, rtol, atol, poll, rel_r0, ws, info, s)
   if (method == HF_KRYLOV_PCG) {
     if (vec == 3) return transpose ? KRY(pcg_run, 3, true) : KRY(pcg_run, 3, false);
     return transpose ? KRY(pcg_run, 1, true) : KRY(pcg_run, 1, false);
@@ -649,29 +689,29 @@ extern "C" size_t hf_krylov_workspace_size(int32_t method, int64_t N, int32_t el
   return kHeader + (size_t)nvec * align256((size_t)N * elem_bytes);
 }
 
-extern "C" int hf_krylov_solve_f64(int32_t method, const hf_pattern* p, const double* bsr, const double* dinv,
-                                   const double* b, double* x, int32_t transpose, int32_t maxiter, double rtol,
+extern "C" int hf_krylov_solve_f64(int32_t method, const hf_pattern* p, const double* vals, const double* dinv,
+                                   const double* b, double* x, int32_t flags, int32_t maxiter, double rtol,
                                    double atol, int32_t poll, void* ws, hf_krylov_info* info, hf_stream_t s) {
-  return krylov_dispatch<double>(method, p, bsr, dinv, b, x, transpose, maxiter, rtol, atol, poll, ws, info,
+  return krylov_dispatch<double>(method, p, vals, dinv, b, x, flags, maxiter, rtol, atol, poll, ws, info,
                                  (hipStream_t)s);
 }
-extern "C" int hf_krylov_solve_f32(int32_t method, const hf_pattern* p, const float* bsr, const float* dinv,
-                                   const float* b, float* x, int32_t transpose, int32_t maxiter, double rtol,
+extern "C" int hf_krylov_solve_f32(int32_t method, const hf_pattern* p, const float* vals, const float* dinv,
+                                   const float* b, float* x, int32_t flags, int32_t maxiter, double rtol,
                                    double atol, int32_t poll, void* ws, hf_krylov_info* info, hf_stream_t s) {
-  return krylov_dispatch<float>(method, p, bsr, dinv, b, x, transpose, maxiter, rtol, atol, poll, ws, info,
+  return krylov_dispatch<float>(method, p, vals, dinv, b, x, flags, maxiter, rtol, atol, poll, ws, info,
                                 (hipStream_t)s);
 }
 
-extern "C" int hf_spmv_bsr_f64(const hf_pattern* p, const double* a, const double* x, double* y, hf_stream_t s) {
+extern "C" int hf_spmv_f64(const hf_pattern* p, const double* a, const double* x, double* y, hf_stream_t s) {
   return spmv_dispatch<double, false>(p, a, x, y, (hipStream_t)s);
 }
-extern "C" int hf_spmv_bsr_f32(const hf_pattern* p, const float* a, const float* x, float* y, hf_stream_t s) {
+extern "C" int hf_spmv_f32(const hf_pattern* p, const float* a, const float* x, float* y, hf_stream_t s) {
   return spmv_dispatch<float, false>(p, a, x, y, (hipStream_t)s);
 }
-extern "C" int hf_spmv_bsr_t_f64(const hf_pattern* p, const double* a, const double* x, double* y, hf_stream_t s) {
+extern "C" int hf_spmv_t_f64(const hf_pattern* p, const double* a, const double* x, double* y, hf_stream_t s) {
   return spmv_dispatch<double, true>(p, a, x, y, (hipStream_t)s);
 }
-extern "C" int hf_spmv_bsr_t_f32(const hf_pattern* p, const float* a, const float* x, float* y, hf_stream_t s) {
+extern "C" int hf_spmv_t_f32(const hf_pattern* p, const float* a, const float* x, float* y, hf_stream_t s) {
   return spmv_dispatch<float, true>(p, a, x, y, (hipStream_t)s);
 }
 
@@ -688,17 +728,16 @@ extern "C" int hf_profile_read(int64_t* launches, double* total_ms) {
   return 0;
 }
 
-extern "C" size_t hf_dot_workspace_size(void) { return 4096 + (size_t)kMaxGrid * sizeof(double) + 256; }
+extern "C" size_t hf_dot_workspace_size(void) { return 4096 + (size_t)kMaxGrid * sizeof(double); }
 
 template <typename T>
 static int dot_impl(const T* x, const T* y, int64_t n, void* ws, double* result, hipStream_t s) {
   HF_REQUIRE(x && y && ws && result, "hf_dot: null argument");
-  uint32_t* ticket = (uint32_t*)ws;
-  double* out = (double*)((char*)ws + 256);
+  double* out = (double*)ws;
   double* partials = (double*)((char*)ws + 4096);
-  HF_CHECK_HIP(hipMemsetAsync(ticket, 0, sizeof(uint32_t), s));
   const int g = elem_grid(n);
-  dot_kernel<T><<<g, kBlock, 0, s>>>(n, x, y, partials, ticket, out);
+  dot_kernel<T><<<g, kBlock, 0, s>>>(n, x, y, partials);
+  dot_final<<<1, kBlock, 0, s>>>(partials, g, out);
   HF_LAUNCH_CHECK();
   HF_CHECK_HIP(hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, s));
   HF_CHECK_HIP(hipStreamSynchronize(s));

@@ -234,7 +234,7 @@ class Problem:
         return R
 
     def jacobian_bsr(self, u, dtype=torch.float64, bc=True):
-        """Tangent as BSR values (nblocks, vec, vec) flattened; Dirichlet rows -> identity rows."""
+        """Tangent values (the library's sliced-ELL layout of node-pair blocks); Dirichlet rows -> identity rows."""
         pts, cells = self._dev()
         u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
         p = self.pattern
@@ -246,8 +246,8 @@ class Problem:
         s = _lib.stream()
         _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
                   self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
-        vals = torch.empty(p.nblocks * bb, dtype=dtype, device=self.device)
-        _lib.call("hf_assemble_bsr_f" + tag, p.handle, _lib.ptr(ej), _lib.ptr(vals), s)
+        vals = torch.empty(p.nvals, dtype=dtype, device=self.device)
+        _lib.call("hf_assemble_f" + tag, p.handle, _lib.ptr(ej), _lib.ptr(vals), s)
         if bc:
             dofs, _ = self.bc_arrays()
             _lib.call("hf_bc_rows_f" + tag, p.handle, _lib.ptr(dofs), dofs.numel(), _lib.ptr(vals), s)
@@ -309,6 +309,7 @@ class Pattern:
         nb, nnz, nc = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _lib.call("hf_pattern_sizes", handle, ctypes.byref(nb), ctypes.byref(nnz), ctypes.byref(nc))
         self.nblocks, self.nnz, self.ncontrib = nb.value, nnz.value, nc.value
+        self.nvals = int(_lib.load().hf_matrix_values(handle))
         self.N = problem.N
         self.vec = problem.vec
         self.device = problem.device

@@ -24,7 +24,12 @@ Added Newton keys:
     'mixed' (FP32 element Jacobian, FP32 SpMV/Krylov with FP64 dot products;
     the Newton residual and update stay FP64, i.e. FP64 iterative refinement).
   * ``forcing``: 'fixed' (default for fp64: the linear tolerance as given) or
-    'ew' (Eisenstat-Walker inexact Newton; default for mixed).
+    'ew' (Eisenstat-Walker inexact Newton; default for mixed).  With 'ew' the
+    first Newton step of a solve uses ``eta_first`` (default 1e-3): it carries
+    the Dirichlet increment of a load step, and a loose solve would leave it
+    in a thin boundary layer whose tangent can be indefinite.  A step that
+    raises ||R|| is halved up to 8 times (``backtrack``, default on with 'ew'),
+    and a PCG breakdown (p.Ap <= 0) re-solves that step with BiCGSTAB.
 Unknown keys are ignored, as in the reference.
 """
 import ctypes
@@ -41,7 +46,7 @@ logger = logging.getLogger("jax_fem_amd")
 _METHOD_KEYS = ("newton", "arc_length", "dynamic_relax")
 _LINEAR_KEYS = ("jax_solver", "amgx_solver", "spsolve_solver", "petsc_solver", "custom_solver", "pcg", "bicgstab")
 _NEWTON_KEYS = ("tol", "rel_tol", "line_search_flag", "initial_guess", "precision", "forcing", "max_iter",
-                "eta_max", "eta_min")
+                "eta_max", "eta_min", "eta_first", "backtrack")
 
 
 def _resolve_solver_options(solver_options):
@@ -64,7 +69,7 @@ def _resolve_solver_options(solver_options):
 
 # ---------------------------------------------------------------------------
 class TangentMatrix:
-    """Row-eliminated tangent on the device: BSR values over the problem's pattern.
+    """Row-eliminated tangent on the device: node-pair block values over the problem's pattern.
 
     The object ``get_A`` returns (the reference returns a PETSc Mat)."""
 
@@ -83,7 +88,7 @@ class TangentMatrix:
     def matvec(self, x, transpose=False):
         x = x.to(self.dtype).contiguous()
         y = torch.empty_like(x)
-        name = "hf_spmv_bsr_t_f" if transpose else "hf_spmv_bsr_f"
+        name = "hf_spmv_t_f" if transpose else "hf_spmv_f"
         _lib.call(name + self.tag, self.pattern.handle, _lib.ptr(self.vals), _lib.ptr(x), _lib.ptr(y), _lib.stream())
         return y
 
@@ -104,7 +109,7 @@ class TangentMatrix:
         """(rowptr, colind, values) of the scalar CSR view (FP64), device tensors."""
         rp, ci = self.pattern.csr()
         v = torch.empty(self.pattern.nnz, dtype=torch.float64, device=self.vals.device)
-        _lib.call("hf_bsr_to_csr_f64", self.pattern.handle, _lib.ptr(self.vals.to(torch.float64).contiguous()),
+        _lib.call("hf_matrix_to_csr_f64", self.pattern.handle, _lib.ptr(self.vals.to(torch.float64).contiguous()),
                   _lib.ptr(v), _lib.stream())
         return rp, ci, v
 
@@ -181,8 +186,11 @@ class KrylovStats:
 
 
 def krylov(A, b, x0=None, method="pcg", rtol=1e-10, atol=1e-10, maxiter=10000, precond=True,
-           transpose=False, poll=8, stats=None, dinv=None):
-    """Device Krylov solve on a TangentMatrix; returns x (dtype of A)."""
+           transpose=False, poll=8, stats=None, dinv=None, rel_r0=False):
+    """Device Krylov solve on a TangentMatrix; returns (x, info) with x in A's dtype.
+
+    Stops at ||r||^2 <= max(rtol^2 ref, atol^2) with ref = ||b||^2 (the reference's
+    jax_solve rule) or, with ``rel_r0``, ref = ||b - A x0||^2 (inexact Newton)."""
     dt = A.dtype
     N = A.shape[0]
     b = b.to(dt).contiguous()
@@ -196,7 +204,8 @@ def krylov(A, b, x0=None, method="pcg", rtol=1e-10, atol=1e-10, maxiter=10000, p
     info = _lib.HFKrylovInfo()
     t0 = time.perf_counter()
     _lib.call("hf_krylov_solve_f" + A.tag, mcode, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(dinv), _lib.ptr(b),
-              _lib.ptr(x), 1 if transpose else 0, int(maxiter), float(rtol), float(atol), int(poll),
+              _lib.ptr(x), (1 if transpose else 0) | (2 if rel_r0 else 0), int(maxiter), float(rtol), float(atol),
+              int(poll),
               _lib.ptr(ws), ctypes.byref(info), _lib.stream())
     if stats is not None:
         stats.add(method, info.iterations, info.status, info.res_norm, info.b_norm, time.perf_counter() - t0)
@@ -217,7 +226,10 @@ def _workspace(device, nbytes):
 
 
 def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
-    """Dispatch on linear_options (solver.py:261-284)."""
+    """Dispatch on linear_options (solver.py:261-284).
+
+    ``rtol`` (set by inexact Newton) overrides the configured tolerance and makes
+    it relative to the initial linear residual ||b - A x0||."""
     opts = linear_options if linear_options is not None else {}
     if not any(k in opts for k in _LINEAR_KEYS):
         opts = dict(opts)
@@ -225,7 +237,7 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
     if "jax_solver" in opts:
         precond = opts["jax_solver"].get("precond", True)
         x, info = krylov(A, b, x0, "bicgstab", 1e-10 if rtol is None else rtol, 1e-10, 10000, precond,
-                         stats=stats)
+                         stats=stats, rel_r0=rtol is not None)
         if rtol is None:
             err = torch.linalg.norm(A.matvec(x).to(torch.float64) - b.to(torch.float64)).item()
             assert err < 0.1, "linear solver failed to converge with err = %g" % err
@@ -236,9 +248,10 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
             pc = o.get("precond", "jacobi")
             if pc not in ("jacobi", True, None, False, "none"):
                 raise NotImplementedError("preconditioner %r" % (pc,))
-            x, info = krylov(A, b, x0, key, o.get("tol", 1e-10) if rtol is None else rtol, o.get("atol", 1e-10),
+            x, info = krylov(A, b, x0, key, o.get("tol", 1e-10) if rtol is None else rtol,
+                             o.get("atol", 1e-10) if rtol is None else 0.0,
                              o.get("maxiter", 10000), pc not in (False, "none"), stats=stats,
-                             poll=o.get("poll", 8))
+                             poll=o.get("poll", 8), rel_r0=rtol is not None)
             return x
     if "custom_solver" in opts:
         return opts["custom_solver"](A, b, x0, opts)
@@ -275,6 +288,8 @@ def solver(problem, solver_options={}):
     adt = torch.float32 if precision == "mixed" else torch.float64
     eta_max = cfg.get("eta_max", 0.1)
     eta_min = cfg.get("eta_min", 1e-6 if precision == "mixed" else 1e-10)
+    eta_first = cfg.get("eta_first", 1e-3)
+    backtrack = cfg.get("backtrack", forcing == "ew")
 
     timing = {"local_assembly": 0.0, "global_matrix": 0.0, "linear": 0.0}
     stats = KrylovStats()
@@ -303,7 +318,7 @@ def solver(problem, solver_options={}):
         x0 = _initial_guess(problem, u)
         if forcing == "ew":
             if eta_prev is None:
-                eta = eta_max
+                eta = eta_first
             else:
                 eta = 0.9 * (hist[-1] / hist[-2]) ** 2
                 if 0.9 * eta_prev ** 2 > 0.1:
@@ -315,12 +330,38 @@ def solver(problem, solver_options={}):
             eta = None
         t0 = time.perf_counter()
         inc = linear_solver(A, -R, x0, lin_opts, stats=stats, rtol=eta)
+        if stats.calls and stats.calls[-1]["status"] == 2 and any(k in lin_opts for k in ("pcg",)):
+            # p.Ap <= 0: the tangent is not positive definite here; BiCGSTAB does not need it
+            logger.debug("newton %d: PCG breakdown, re-solving with BiCGSTAB", n + 1)
+            o = dict(lin_opts["pcg"] or {})
+            inc = linear_solver(A, -R, x0, {"bicgstab": o}, stats=stats, rtol=eta)
         torch.cuda.synchronize()
         timing["linear"] += time.perf_counter() - t0
-        u = u + inc.to(torch.float64)
-        R, res = residual(u)
+        inc = inc.to(torch.float64)
+        if cfg.get("line_search_flag", False):
+            u, R, res = _line_search(problem, u, inc, residual)
+        else:
+            u_new = u + inc
+            R_new, res_new = residual(u_new)
+            if backtrack:
+                # keep the Dirichlet part of the step exact: only the interior part is scaled
+                d, _ = problem.bc_arrays()
+                alpha = 1.0
+                for _ in range(8):
+                    if np.isfinite(res_new) and res_new <= res:
+                        break
+                    alpha *= 0.5
+                    step = inc * alpha
+                    step[d.long()] = inc[d.long()]
+                    u_new = u + step
+                    R_new, res_new = residual(u_new)
+                if alpha < 1.0:
+                    logger.debug("newton %d: backtracked to alpha %.3g", n + 1, alpha)
+            u, R, res = u_new, R_new, res_new
         hist.append(res)
-        logger.debug("newton %d: res %.3e rel %.3e", n, res, res / res0)
+        last = stats.calls[-1] if stats.calls else {}
+        logger.debug("newton %d: res %.3e rel %.3e | eta %s krylov %s status %s", n, res, res / res0,
+                     "%.2e" % eta if eta is not None else "-", last.get("iterations"), last.get("status"))
     if not (np.isfinite(res) and bool(torch.isfinite(u).all())):
         raise FloatingPointError("Newton produced non-finite values")
     torch.cuda.synchronize()
@@ -330,6 +371,19 @@ def solver(problem, solver_options={}):
     problem._newton_u = u
     problem._A_cache = None
     return problem.unflatten_fn_sol_list(u)
+
+
+def _line_search(problem, u, inc, residual):
+    """Up to 3 step halvings while the residual norm keeps decreasing (solver.py:424-460)."""
+    alpha = 1.0
+    R, res = residual(u + alpha * inc)
+    for _ in range(3):
+        R_half, res_half = residual(u + 0.5 * alpha * inc)
+        if res_half > res:
+            break
+        alpha *= 0.5
+        R, res = R_half, res_half
+    return u + alpha * inc, R, res
 
 
 # ---------------------------------------------------------------------------

@@ -64,9 +64,13 @@ def _mats(jfa, kind, param):
 
 
 def _distorted_box(n=(3, 2, 2), seed=0):
+    """Box mesh with interior nodes jittered (boundary planes stay planar, so
+    location functions such as x == 0 still select them)."""
     pts, cells = ofe.box_mesh(*n)
     rng = np.random.default_rng(seed)
-    pts = pts + 0.08 * rng.uniform(-1, 1, pts.shape) / max(n)
+    interior = np.all((pts > 1e-9) & (pts < 1 - 1e-9), axis=1)
+    pts = pts.copy()
+    pts[interior] += 0.08 * rng.uniform(-1, 1, (interior.sum(), 3)) / max(n)
     return pts, cells
 
 
