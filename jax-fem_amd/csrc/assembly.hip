@@ -121,12 +121,13 @@ __global__ void csr_view(const int32_t* __restrict__ rowptr, const int32_t* __re
 // wavefront lane each; R = 64 for vec = 1).  Slice s has width W_s = its
 // longest row (in blocks); its slots are slice_ptr[s] .. slice_ptr[s+1]-1.
 // Slot t holds, for each row r of the slice, the r-th row's j-th block
-// (j = t - slice_ptr[s]) or zero padding.  With L = R*vec lanes and
-// entry (i, jj) of a block:
-//     value  vals[t*R*vec*vec + (jj*R + r)*vec + i]   (= base + jj*L + i,
-//            base = t*R*vec*vec + r*vec)
-//     column sell_col[t*R + r]
-// so for fixed (t, jj) the L lanes read L consecutive values.
+// (j = t - slice_ptr[s]) or zero padding; the block's vec x vec entries are
+// contiguous and row-major:
+//     entry (i, jj)  vals[(t*R + r)*vec*vec + i*vec + jj]   (= base + i*vec + jj,
+//                    base = (t*R + r)*vec*vec)
+//     column         sell_col[t*R + r]
+// so lane (r, i) of a wavefront reads its vec values of slot t as one
+// contiguous vec-word load, and the L = R*vec lanes read L*vec consecutive values.
 __global__ void slice_width(const int32_t* __restrict__ rowptr, int64_t nnode, int R, int64_t nslices,
                             int32_t* __restrict__ W) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -153,7 +154,7 @@ __global__ void sell_fill(const int32_t* __restrict__ rowptr, const int32_t* __r
     const int32_t j = t - t0;
     if (j < len) {
       sell_col[(int64_t)t * R + r] = colidx[b0 + j];
-      blkpos[b0 + j] = (int32_t)((int64_t)t * R * vec * vec + (int64_t)r * vec);
+      blkpos[b0 + j] = (int32_t)(((int64_t)t * R + r) * vec * vec);
     } else {
       sell_col[(int64_t)t * R + r] = n < nnode ? (int32_t)n : 0;
     }
@@ -196,10 +197,11 @@ __global__ void sell_to_csr_kernel(const int32_t* __restrict__ rowptr, const int
   if (b >= nblocks) return;
   const int32_t n = brow[b];
   const int32_t b0 = rowptr[n], len = (rowptr[n + 1] - b0) * vec;
-  const int64_t base = blkpos[b], L = (int64_t)R * vec;
+  const int64_t base = blkpos[b];
+  (void)R;
   for (int i = 0; i < vec; ++i)
     for (int j = 0; j < vec; ++j)
-      csr[(int64_t)b0 * vec * vec + (int64_t)i * len + (b - b0) * vec + j] = vals[base + j * L + i];
+      csr[(int64_t)b0 * vec * vec + (int64_t)i * len + (b - b0) * vec + j] = vals[base + i * vec + j];
 }
 
 // One thread per node row: every slot of the row's slice gets the sum of the
@@ -233,11 +235,10 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
         for (int e = 0; e < BB; ++e) acc[e] += src[e];
       }
     }
-    T* dst = vals + (int64_t)t * R * BB + (int64_t)r * VEC;
+    T* dst = vals + ((int64_t)t * R + r) * BB;
+    (void)L;
 #pragma unroll
-    for (int i = 0; i < VEC; ++i)
-#pragma unroll
-      for (int jj = 0; jj < VEC; ++jj) dst[jj * L + i] = acc[i * VEC + jj];
+    for (int e = 0; e < BB; ++e) dst[e] = acc[e];
   }
 }
 
@@ -275,13 +276,12 @@ __global__ void bc_rows_kernel(const int32_t* __restrict__ rowptr, const int32_t
   const int32_t n = d / vec, i = d - (d / vec) * vec;
   const int64_t s = n / R;
   const int r = (int)(n - s * R);
-  const int64_t L = (int64_t)R * vec;
   const int32_t len = rowptr[n + 1] - rowptr[n];
   for (int32_t j = 0; j < len; ++j) {
-    T* base = vals + (int64_t)(slice_ptr[s] + j) * R * vec * vec + (int64_t)r * vec;
-    for (int jj = 0; jj < vec; ++jj) base[jj * L + i] = (T)0;
+    T* base = vals + ((int64_t)(slice_ptr[s] + j) * R + r) * vec * vec;
+    for (int jj = 0; jj < vec; ++jj) base[i * vec + jj] = (T)0;
   }
-  if (dpos[n] >= 0) vals[(int64_t)dpos[n] + i * L + i] = (T)1;
+  if (dpos[n] >= 0) vals[(int64_t)dpos[n] + i * vec + i] = (T)1;
 }
 
 __global__ void bc_residual_kernel(const int32_t* __restrict__ dofs, const double* __restrict__ vals,
@@ -309,7 +309,8 @@ __global__ void jacobi_kernel(const int32_t* __restrict__ dpos, int64_t nnode, i
   const int64_t n = rr / vec;
   const int i = (int)(rr - n * vec);
   const int32_t b = dpos[n];
-  dinv[rr] = b >= 0 ? (T)1 / vals[(int64_t)b + (int64_t)i * R * vec + i] : (T)1;
+  (void)R;
+  dinv[rr] = b >= 0 ? (T)1 / vals[(int64_t)b + (int64_t)i * vec + i] : (T)1;
 }
 
 }  // namespace hf

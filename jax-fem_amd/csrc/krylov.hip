@@ -80,14 +80,17 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restri
 
 // Sum of nparts partials in a fixed order, for a one-workgroup kernel; every
 // thread receives the totals.
+constexpr int kScalarBlock = 1024;   // threads of the one-workgroup scalar kernels
+constexpr int kScalarWaves = kScalarBlock / kWave;
+
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nparts, double (&tot)[NV]) {
-  __shared__ double sm[kWavesPerBlock][NV];
+  __shared__ double sm[kScalarWaves][NV];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   double acc[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) acc[j] = 0;
-  for (int b = threadIdx.x; b < nparts; b += kBlock)
+  for (int b = threadIdx.x; b < nparts; b += kScalarBlock)
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] += partials[(int64_t)b * NV + j];
 #pragma unroll
@@ -99,7 +102,7 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ partials
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     double s = 0;
-    for (int k = 0; k < kWavesPerBlock; ++k) s += sm[k][j];
+    for (int k = 0; k < kScalarWaves; ++k) s += sm[k][j];
     tot[j] = s;
   }
 }
@@ -109,11 +112,11 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ partials
 // wavefront per slice of R node rows, lane = r*VEC + i owns scalar row
 // VEC*(slice*R + r) + i.  For every slot t of the slice the lane reads the
 // block column c = sell_col[t*R + r] and the VEC values of its row of the
-// block (each a coalesced run of L = R*VEC floats across the wavefront),
+// block (one VEC-word load; the wavefront reads L*VEC consecutive values),
 // gathers x at c, and accumulates.  Slots are independent, so the unrolled
 // loop keeps several slots' loads in flight; no cross-lane reduction.
 // TRANS (A^T x): the block of A^T in slot t is the transpose of the block
-// whose value base is sell_tpos[t*R + r]; entry (i, jj) = vals[base + i*L + jj].
+// whose value base is sell_tpos[t*R + r]; entry (i, jj) = vals[base + jj*VEC + i].
 struct MatArgs {
   int64_t nrows, nslices;
   const int32_t* slice_ptr;
@@ -147,13 +150,13 @@ __device__ __forceinline__ void sweep(const MatArgs& m, int64_t s0, int64_t s1, 
           const int32_t tp = m.sell_tpos[cr];
 #pragma unroll
           for (int jj = 0; jj < VEC; ++jj) {
-            const T a = tp >= 0 ? vals[(int64_t)tp + i * S::L + jj] : (T)0;
+            const T a = tp >= 0 ? vals[(int64_t)tp + jj * VEC + i] : (T)0;
             y += a * xv(c + jj);
           }
         } else {
-          const T* vrow = vals + (int64_t)t * S::R * S::BB + lane;
+          const T* vrow = vals + ((int64_t)t * S::L + lane) * VEC;
 #pragma unroll
-          for (int jj = 0; jj < VEC; ++jj) y += __builtin_nontemporal_load(vrow + jj * S::L) * xv(c + jj);
+          for (int jj = 0; jj < VEC; ++jj) y += __builtin_nontemporal_load(vrow + jj) * xv(c + jj);
         }
       }
     }
@@ -209,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) pcg_init_kernel(MatArgs m, const T* __
 }
 
 // Stop threshold: max(rtol^2 * ref, atol^2), ref = ||b||^2 or ||r0||^2 (rel_r0).
-__global__ void __launch_bounds__(kBlock) pcg_init_scalars(Scalars* sc, const double* partials, int nparts,
+__global__ void __launch_bounds__(kScalarBlock) pcg_init_scalars(Scalars* sc, const double* partials, int nparts,
                                                            double rtol2, double atol2, int rel_r0, int maxiter) {
   double tot[3];
   sum_partials<3>(partials, nparts, tot);
@@ -251,7 +254,7 @@ __global__ void __launch_bounds__(kBlock) pcg_spmv_kernel(MatArgs m, const T* __
   block_partials<1>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kBlock) pcg_alpha_scalars(Scalars* sc, const double* partials, int nparts) {
+__global__ void __launch_bounds__(kScalarBlock) pcg_alpha_scalars(Scalars* sc, const double* partials, int nparts) {
   if (sc->done) return;
   double tot[1];
   sum_partials<1>(partials, nparts, tot);
@@ -288,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) pcg_update_kernel(
   block_partials<2>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kBlock) pcg_beta_scalars(Scalars* sc, const double* partials, int nparts) {
+__global__ void __launch_bounds__(kScalarBlock) pcg_beta_scalars(Scalars* sc, const double* partials, int nparts) {
   if (sc->done) return;
   double tot[2];
   sum_partials<2>(partials, nparts, tot);
@@ -324,7 +327,7 @@ __global__ void __launch_bounds__(kBlock) bicg_init_kernel(MatArgs m, const T* _
   block_partials<2>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kBlock) bicg_init_scalars(Scalars* sc, const double* partials, int nparts,
+__global__ void __launch_bounds__(kScalarBlock) bicg_init_scalars(Scalars* sc, const double* partials, int nparts,
                                                             double rtol2, double atol2, int rel_r0, int maxiter) {
   double tot[2];
   sum_partials<2>(partials, nparts, tot);
@@ -377,7 +380,7 @@ __global__ void __launch_bounds__(kBlock) bicg_spmv_kernel(MatArgs m, const T* _
   block_partials<2>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kBlock) bicg_alpha_scalars(Scalars* sc, const double* partials, int nparts) {
+__global__ void __launch_bounds__(kScalarBlock) bicg_alpha_scalars(Scalars* sc, const double* partials, int nparts) {
   if (sc->done) return;
   double tot[2];
   sum_partials<2>(partials, nparts, tot);
@@ -386,7 +389,7 @@ __global__ void __launch_bounds__(kBlock) bicg_alpha_scalars(Scalars* sc, const 
   if (!isfinite(sc->alpha)) { sc->done = 1; sc->status = 3; }
 }
 
-__global__ void __launch_bounds__(kBlock) bicg_omega_scalars(Scalars* sc, const double* partials, int nparts) {
+__global__ void __launch_bounds__(kScalarBlock) bicg_omega_scalars(Scalars* sc, const double* partials, int nparts) {
   if (sc->done || sc->exit_early) return;
   double tot[2];
   sum_partials<2>(partials, nparts, tot);
@@ -417,7 +420,7 @@ __global__ void __launch_bounds__(kBlock) bicg_s_kernel(int64_t N, const T* __re
   block_partials<1>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kBlock) bicg_s_scalars(Scalars* sc, const double* partials, int nparts) {
+__global__ void __launch_bounds__(kScalarBlock) bicg_s_scalars(Scalars* sc, const double* partials, int nparts) {
   if (sc->done) return;
   double tot[1];
   sum_partials<1>(partials, nparts, tot);
@@ -456,7 +459,7 @@ __global__ void __launch_bounds__(kBlock) bicg_x_kernel(int64_t N, const T* __re
   block_partials<2>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kBlock) bicg_end_scalars(Scalars* sc, const double* partials, int nparts) {
+__global__ void __launch_bounds__(kScalarBlock) bicg_end_scalars(Scalars* sc, const double* partials, int nparts) {
   if (sc->done) return;
   double tot[2];
   sum_partials<2>(partials, nparts, tot);
@@ -485,7 +488,7 @@ __global__ void __launch_bounds__(kBlock) dot_kernel(int64_t n, const T* __restr
   block_partials<1>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kBlock) dot_final(const double* partials, int nparts, double* out) {
+__global__ void __launch_bounds__(kScalarBlock) dot_final(const double* partials, int nparts, double* out) {
   double tot[1];
   sum_partials<1>(partials, nparts, tot);
   if (threadIdx.x == 0) *out = tot[0];
@@ -522,6 +525,35 @@ static int prof_collect() {
   HF_CHECK_HIP(hipEventElapsedTime(&ms, g_ev[0], g_ev[1]));
   g_prof_ms += ms;
   g_prof_n += 1;
+  return 0;
+}
+
+// Host side of the convergence poll: after each batch of iterations the
+// scalars are copied to pinned host memory behind an event; the host launches
+// the next batch before it waits for the previous batch's copy, so the GPU
+// always has a batch queued.  One pinned pair + events per host thread.
+struct PollState {
+  Scalars* host = nullptr;  // 2 slots, pinned
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+static thread_local PollState g_poll;
+
+static int poll_init() {
+  if (!g_poll.host) {
+    HF_CHECK_HIP(hipHostMalloc((void**)&g_poll.host, 2 * sizeof(Scalars), hipHostMallocDefault));
+    HF_CHECK_HIP(hipEventCreateWithFlags(&g_poll.ev[0], hipEventDisableTiming));
+    HF_CHECK_HIP(hipEventCreateWithFlags(&g_poll.ev[1], hipEventDisableTiming));
+  }
+  return 0;
+}
+static int poll_post(const Scalars* sc, int slot, hipStream_t s) {
+  HF_CHECK_HIP(hipMemcpyAsync(&g_poll.host[slot], sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  HF_CHECK_HIP(hipEventRecord(g_poll.ev[slot], s));
+  return 0;
+}
+static int poll_wait(int slot, Scalars& out) {
+  HF_CHECK_HIP(hipEventSynchronize(g_poll.ev[slot]));
+  out = g_poll.host[slot];
   return 0;
 }
 
@@ -578,29 +610,53 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
   T *r = v, *z = v + stride, *p0 = v + 2 * stride, *p1 = v + 3 * stride, *q = v + 4 * stride;
   const int gr = rows_grid(m.nslices), ge = elem_grid(N);
   pcg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, dinv, b, x, r, z, p0, partials);
-  pcg_init_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
+  pcg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   HF_LAUNCH_CHECK();
+  if (poll_init()) return -2;
   Scalars host{};
-  int it = 0;
+  int it = 0, batch = 0;
+  bool timed_pending = false;
+  int timed_batch = -1;
   for (;;) {
     for (int k = 0; k < poll; ++k, ++it) {
       T* pold = (it & 1) ? p1 : p0;
       T* pnew = (it & 1) ? p0 : p1;
-      if (k == 0 && prof_begin(s)) return -2;
+      const bool timed = (k == 0 && !timed_pending);
+      if (timed && prof_begin(s)) return -2;
       pcg_spmv_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, z, pold, pnew, q, sc, partials);
-      if (k == 0 && prof_end(s)) return -2;
-      pcg_alpha_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr);
+      if (timed && prof_end(s)) return -2;
+      if (timed && g_prof) {
+        timed_pending = true;
+        timed_batch = batch;
+      }
+      pcg_alpha_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr);
       pcg_update_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, pnew, q, x, r, z, sc, partials);
-      pcg_beta_scalars<<<1, kBlock, 0, s>>>(sc, partials, ge);
+      pcg_beta_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, ge);
     }
     HF_LAUNCH_CHECK();
-    HF_CHECK_HIP(hipMemcpyAsync(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
-    HF_CHECK_HIP(hipStreamSynchronize(s));
-    if (!host.done || host.iter > it - poll) {
-      if (prof_collect()) return -2;  // the timed SpMV did real work
+    if (poll_post(sc, batch & 1, s)) return -2;
+    if (batch > 0) {
+      // wait for the previous batch while this one runs
+      if (poll_wait((batch - 1) & 1, host)) return -2;
+      if (timed_pending && timed_batch == batch - 1) {
+        // the timed SpMV ran in the batch just waited for; it did work unless
+        // the solve had already finished before that batch
+        if (!host.done || host.iter > (batch - 1) * poll) {
+          if (prof_collect()) return -2;
+        }
+        timed_pending = false;
+      }
+      if (host.done) break;
     }
-    if (host.done || it >= maxiter + poll) break;
+    ++batch;
+    if (it >= maxiter + 2 * poll) {
+      if (poll_wait((batch - 1) & 1, host)) return -2;
+      break;
+    }
   }
+  HF_CHECK_HIP(hipStreamSynchronize(s));
+  HF_CHECK_HIP(hipMemcpy(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+  timed_pending = false;
   if (info) {
     info->iterations = host.iter;
     info->status = host.done ? host.status : 1;
@@ -623,32 +679,54 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
     *sv = v + 5 * stride, *shat = v + 6 * stride, *t = v + 7 * stride;
   const int gr = rows_grid(m.nslices), ge = elem_grid(N);
   bicg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, b, x, r, rhat, pp, q, partials);
-  bicg_init_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
+  bicg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   HF_LAUNCH_CHECK();
+  if (poll_init()) return -2;
   Scalars host{};
-  int it = 0;
+  int it = 0, batch = 0;
+  bool timed_pending = false;
+  int timed_batch = -1;
   for (;;) {
     for (int k = 0; k < poll; ++k, ++it) {
+      const bool timed = (k == 0 && !timed_pending);
       bicg_p_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, r, q, pp, phat, sc);
-      if (k == 0 && prof_begin(s)) return -2;
+      if (timed && prof_begin(s)) return -2;
       bicg_spmv_kernel<T, VEC, TRANS, 0><<<gr, kBlock, 0, s>>>(m, vals, phat, rhat, q, sc, partials);
-      if (k == 0 && prof_end(s)) return -2;
-      bicg_alpha_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr);
+      if (timed && prof_end(s)) return -2;
+      if (timed && g_prof) {
+        timed_pending = true;
+        timed_batch = batch;
+      }
+      bicg_alpha_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr);
       bicg_s_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, r, q, sv, shat, sc, partials);
-      bicg_s_scalars<<<1, kBlock, 0, s>>>(sc, partials, ge);
+      bicg_s_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, ge);
       bicg_spmv_kernel<T, VEC, TRANS, 1><<<gr, kBlock, 0, s>>>(m, vals, shat, sv, t, sc, partials);
-      bicg_omega_scalars<<<1, kBlock, 0, s>>>(sc, partials, gr);
+      bicg_omega_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr);
       bicg_x_kernel<T><<<ge, kBlock, 0, s>>>(N, rhat, phat, shat, sv, t, x, r, sc, partials);
-      bicg_end_scalars<<<1, kBlock, 0, s>>>(sc, partials, ge);
+      bicg_end_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, ge);
     }
     HF_LAUNCH_CHECK();
-    HF_CHECK_HIP(hipMemcpyAsync(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
-    HF_CHECK_HIP(hipStreamSynchronize(s));
-    if (!host.done || host.iter > it - poll) {
-      if (prof_collect()) return -2;
+    if (poll_post(sc, batch & 1, s)) return -2;
+    if (batch > 0) {
+      if (poll_wait((batch - 1) & 1, host)) return -2;
+      if (timed_pending && timed_batch == batch - 1) {
+        // the timed SpMV ran in the batch just waited for; it did work unless
+        // the solve had already finished before that batch
+        if (!host.done || host.iter > (batch - 1) * poll) {
+          if (prof_collect()) return -2;
+        }
+        timed_pending = false;
+      }
+      if (host.done) break;
     }
-    if (host.done || it >= maxiter + poll) break;
+    ++batch;
+    if (it >= maxiter + 2 * poll) {
+      if (poll_wait((batch - 1) & 1, host)) return -2;
+      break;
+    }
   }
+  HF_CHECK_HIP(hipStreamSynchronize(s));
+  HF_CHECK_HIP(hipMemcpy(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
   if (info) {
     info->iterations = host.iter;
     info->status = host.done ? host.status : 1;
@@ -737,7 +815,7 @@ static int dot_impl(const T* x, const T* y, int64_t n, void* ws, double* result,
   double* partials = (double*)((char*)ws + 4096);
   const int g = elem_grid(n);
   dot_kernel<T><<<g, kBlock, 0, s>>>(n, x, y, partials);
-  dot_final<<<1, kBlock, 0, s>>>(partials, g, out);
+  dot_final<<<1, kScalarBlock, 0, s>>>(partials, g, out);
   HF_LAUNCH_CHECK();
   HF_CHECK_HIP(hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, s));
   HF_CHECK_HIP(hipStreamSynchronize(s));
