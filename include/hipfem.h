@@ -154,6 +154,42 @@ int hf_krylov_solve_f32(int32_t method, const hf_pattern* p, const float* vals, 
                         double rtol, double atol, int32_t poll, void* workspace,
                         hf_krylov_info* info, hf_stream_t stream);
 
+/* ---- learned preconditioner (no reference counterpart; the reference only has
+ *      Jacobi, solver.py:68-69): a 3x3x3 convolutional network on the structured
+ *      node grid (nx, ny, nz nodes, box_mesh order n = (i*ny + j)*nz + k,
+ *      generate_mesh.py:170-187), channels-last tensors (batch, node, channel),
+ *      weights W[tap][cin][cout] with tap = ((di+1)*3 + (dj+1))*3 + (dk+1),
+ *      zero padding.  Built layers: 4->16, 16->16, 16->3.  act: 0 none, 1 tanh. */
+int hf_cnn_conv3d_fwd_f32(const float* in, int32_t cin, const float* w, const float* b, int32_t cout,
+                          int32_t act, int32_t nx, int32_t ny, int32_t nz, int32_t batch, float* out,
+                          hf_stream_t stream);
+/* gin = conv^T(gpre): gradient w.r.t. the layer input */
+int hf_cnn_conv3d_bwd_data_f32(const float* gpre, int32_t cout, const float* w, int32_t cin, int32_t nx,
+                               int32_t ny, int32_t nz, int32_t batch, float* gin, hf_stream_t stream);
+/* gw, gb: weight and bias gradients, deterministic two-stage reduction */
+size_t hf_cnn_bwd_weight_workspace(int32_t cin, int32_t cout, int32_t nx, int32_t ny, int32_t nz, int32_t batch);
+int hf_cnn_conv3d_bwd_weight_f32(const float* in, int32_t cin, const float* gpre, int32_t cout, int32_t nx,
+                                 int32_t ny, int32_t nz, int32_t batch, void* workspace, float* gw, float* gb,
+                                 hf_stream_t stream);
+/* gpre = gout * (1 - y^2) */
+int hf_cnn_tanh_bwd_f32(const float* y, const float* gout, int64_t n, float* gpre, hf_stream_t stream);
+/* preconditioner input f = [D^-1/2 r / s, g] and output z = D^-1 r + s D^-1/2 mask y
+ * (r, z: (batch, 3*nnode); dinv, mask: 3*nnode; g: nnode; scale: s per batch entry) */
+int hf_lpc_features_f32(const float* r, const float* dinv, const float* gfield, int64_t nnode, int32_t batch,
+                        const float* scale, float* f, hf_stream_t stream);
+int hf_lpc_output_f32(const float* r, const float* dinv, const float* mask, const float* y, int64_t nnode,
+                      int32_t batch, const float* scale, float* z, hf_stream_t stream);
+/* flexible CG with the learned preconditioner (weights: w1,b1,w2,b2,w3,b3 device
+ * pointers).  status 0 converged, 1 max iterations, 2 M not positive or
+ * p.Ap <= 0, 3 non-finite, 4 stagnation (no 10 % drop of ||r|| in
+ * stall_window iterations); on 2-4 the caller continues with Jacobi-PCG. */
+size_t hf_lpc_workspace_size(int64_t nnode);
+int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const float* dinv, const float* b, float* x,
+                     const float* const* weights, const float* gfield, const float* mask, int32_t nx,
+                     int32_t ny, int32_t nz, int32_t maxiter, double rtol, double atol, int32_t poll,
+                     int32_t flags, int32_t stall_window, void* workspace, hf_krylov_info* info,
+                     hf_stream_t stream);
+
 /* ---- measurement: HIP-event timing of the Krylov SpMV kernel (the first SpMV
  *      of every poll batch, on the solve's stream).  No reference counterpart:
  *      replaces the reference's wall-clock buckets (solver.py:22-57) at kernel

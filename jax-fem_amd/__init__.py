@@ -16,6 +16,7 @@ from .fe import FiniteElement
 from .materials import LinearElastic, NeoHookean, Poisson, Material
 from .mesh import Mesh, box_mesh, box_mesh_hex27, get_meshio_cell_type
 from .problem import Problem
+from .learned_pc import LearnedPC
 from .solver import (solver, ad_wrapper, implicit_vjp, get_A, linear_solver, apply_bc_vec, assign_bc, copy_bc,
                      TangentMatrix, krylov)
 
@@ -28,4 +29,5 @@ if not logger.handlers:
 
 __all__ = ["Problem", "solver", "ad_wrapper", "implicit_vjp", "get_A", "linear_solver", "apply_bc_vec",
            "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "get_meshio_cell_type",
-           "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov"]
+           "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov",
+           "LearnedPC"]

@@ -68,6 +68,16 @@ _SIGS = {
                                     ctypes.POINTER(HFKrylovInfo), vp]),
     "hf_profile_enable": (c_i32, [c_i32]),
     "hf_profile_read": (c_i32, [ctypes.POINTER(c_i64), ctypes.POINTER(c_f64)]),
+    "hf_cnn_conv3d_fwd_f32": (c_i32, [vp, c_i32, vp, vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp]),
+    "hf_cnn_conv3d_bwd_data_f32": (c_i32, [vp, c_i32, vp, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp]),
+    "hf_cnn_bwd_weight_workspace": (c_sz, [c_i32, c_i32, c_i32, c_i32, c_i32, c_i32]),
+    "hf_cnn_conv3d_bwd_weight_f32": (c_i32, [vp, c_i32, vp, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp, vp, vp]),
+    "hf_cnn_tanh_bwd_f32": (c_i32, [vp, vp, c_i64, vp, vp]),
+    "hf_lpc_features_f32": (c_i32, [vp, vp, vp, c_i64, c_i32, vp, vp, vp]),
+    "hf_lpc_output_f32": (c_i32, [vp, vp, vp, vp, c_i64, c_i32, vp, vp, vp]),
+    "hf_lpc_workspace_size": (c_sz, [c_i64]),
+    "hf_lpc_solve_f32": (c_i32, [vp, vp, vp, vp, vp, vp, vp, vp, c_i32, c_i32, c_i32, c_i32, c_f64, c_f64, c_i32,
+                                 c_i32, c_i32, vp, ctypes.POINTER(HFKrylovInfo), vp]),
     "hf_dot_workspace_size": (c_sz, []),
     "hf_dot_f64": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),
     "hf_dot_f32": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),

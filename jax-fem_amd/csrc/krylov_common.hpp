@@ -1,0 +1,228 @@
+// Device-side pieces shared by the Krylov drivers (krylov.hip, lpc.hip):
+// scalars, fence-free partial reductions, XCD-aware ranges, the sliced-ELL
+// slice sweep, workspace layout and the host-side poll / profiling hooks.
+#pragma once
+#include "common.hpp"
+#include "hipfem.h"
+
+namespace hf {
+const int32_t* pattern_rowptr(const hf_pattern* p);
+const int32_t* pattern_colidx(const hf_pattern* p);
+const int32_t* pattern_trans(const hf_pattern* p);
+int64_t pattern_nnode(const hf_pattern* p);
+int64_t pattern_nblocks(const hf_pattern* p);
+int pattern_max_row_blocks(const hf_pattern* p);
+int64_t pattern_nslices(const hf_pattern* p);
+const int32_t* pattern_slice_ptr(const hf_pattern* p);
+const int32_t* pattern_sell_col(const hf_pattern* p);
+const int32_t* pattern_sell_tpos(const hf_pattern* p);
+int pattern_vec(const hf_pattern* p);
+
+constexpr int kBlock = 256;         // threads per workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kMaxGrid = 8192;      // workgroups of the reduction kernels
+constexpr int kNumXcd = 8;
+
+struct Scalars {
+  double rho, rho_, alpha, omega, rr, bb, thresh, pq, ss, ts, tt;
+  int32_t iter, done, status, exit_early;
+  int32_t maxiter, pad;
+};
+
+// Logical workgroup id, remapped so that each XCD owns a contiguous range.
+__device__ __forceinline__ int logical_block() {
+  const int g = gridDim.x, b = blockIdx.x;
+  if (g % kNumXcd != 0) return b;
+  return (b % kNumXcd) * (g / kNumXcd) + b / kNumXcd;
+}
+
+// Block reduction of NV doubles; thread 0 writes this workgroup's partials.
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
+  __shared__ double sm[kWavesPerBlock][NV];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const double s = wave_sum(v[j]);
+    if (lane == 0) sm[w][j] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      double s = 0;
+      for (int k = 0; k < kWavesPerBlock; ++k) s += sm[k][j];
+      partials[(int64_t)blockIdx.x * NV + j] = s;
+    }
+  }
+}
+
+// Sum of nparts partials in a fixed order, for a one-workgroup kernel; every
+// thread receives the totals.
+constexpr int kScalarBlock = 1024;   // threads of the one-workgroup scalar kernels
+constexpr int kScalarWaves = kScalarBlock / kWave;
+
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nparts, double (&tot)[NV]) {
+  __shared__ double sm[kScalarWaves][NV];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  double acc[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) acc[j] = 0;
+  for (int b = threadIdx.x; b < nparts; b += kScalarBlock)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] += partials[(int64_t)b * NV + j];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const double s = wave_sum(acc[j]);
+    if (lane == 0) sm[w][j] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double s = 0;
+    for (int k = 0; k < kScalarWaves; ++k) s += sm[k][j];
+    tot[j] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Slice sweeps over the sliced-ELL matrix (layout in assembly.hip): one
+// wavefront per slice of R node rows, lane = r*VEC + i owns scalar row
+// VEC*(slice*R + r) + i.  For every slot t of the slice the lane reads the
+// block column c = sell_col[t*R + r] and the VEC values of its row of the
+// block (one VEC-word load; the wavefront reads L*VEC consecutive values),
+// gathers x at c, and accumulates.  Slots are independent, so the unrolled
+// loop keeps several slots' loads in flight; no cross-lane reduction.
+// TRANS (A^T x): the block of A^T in slot t is the transpose of the block
+// whose value base is sell_tpos[t*R + r]; entry (i, jj) = vals[base + jj*VEC + i].
+struct MatArgs {
+  int64_t nrows, nslices;
+  const int32_t* slice_ptr;
+  const int32_t* sell_col;
+  const int32_t* sell_tpos;
+};
+
+template <int VEC>
+struct SellShape {
+  static constexpr int R = (VEC == 3) ? 21 : 64;
+  static constexpr int L = R * VEC;
+  static constexpr int BB = VEC * VEC;
+};
+
+template <typename T, int VEC, bool TRANS, typename XF, typename EPI>
+__device__ __forceinline__ void sweep(const MatArgs& m, int64_t s0, int64_t s1, const T* __restrict__ vals, XF xv,
+                                      EPI epi) {
+  using S = SellShape<VEC>;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int r = lane / VEC, i = lane - (lane / VEC) * VEC;
+  const bool in_slice = lane < S::L;
+  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    const int32_t t0 = m.slice_ptr[s], t1 = m.slice_ptr[s + 1];
+    T y = 0;
+    if (in_slice) {
+#pragma unroll 4
+      for (int32_t t = t0; t < t1; ++t) {
+        const int64_t cr = (int64_t)t * S::R + r;
+        const int64_t c = (int64_t)m.sell_col[cr] * VEC;
+        if constexpr (TRANS) {
+          const int32_t tp = m.sell_tpos[cr];
+#pragma unroll
+          for (int jj = 0; jj < VEC; ++jj) {
+            const T a = tp >= 0 ? vals[(int64_t)tp + jj * VEC + i] : (T)0;
+            y += a * xv(c + jj);
+          }
+        } else {
+          const T* vrow = vals + ((int64_t)t * S::L + lane) * VEC;
+#pragma unroll
+          for (int jj = 0; jj < VEC; ++jj) y += __builtin_nontemporal_load(vrow + jj) * xv(c + jj);
+        }
+      }
+    }
+    const int64_t n = s * S::R + r;
+    if (in_slice && n < m.nrows) epi(n, i, y);
+  }
+}
+
+// Contiguous slice range of this workgroup (XCD-remapped).
+__device__ __forceinline__ void row_range(int64_t nslices, int64_t& r0, int64_t& r1) {
+  const int64_t per = ceil_div(nslices, gridDim.x);
+  r0 = (int64_t)logical_block() * per;
+  r1 = r0 + per < nslices ? r0 + per : nslices;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kHeader = 4096 + (size_t)kMaxGrid * 4 * sizeof(double);
+
+inline int rows_grid(int64_t nslices) {
+  int64_t g = ceil_div(nslices, kWavesPerBlock);
+  if (g > kMaxGrid) g = kMaxGrid;
+  if (g >= 64) g = (g / kNumXcd) * kNumXcd;
+  return (int)(g < 1 ? 1 : g);
+}
+inline int elem_grid(int64_t n) {
+  int64_t g = ceil_div(n, kBlock * 4);
+  if (g > kMaxGrid) g = kMaxGrid;
+  if (g >= 64) g = (g / kNumXcd) * kNumXcd;
+  return (int)(g < 1 ? 1 : g);
+}
+
+inline MatArgs mat_args(const hf_pattern* p) {
+  MatArgs m;
+  m.nrows = pattern_nnode(p);
+  m.nslices = pattern_nslices(p);
+  m.slice_ptr = pattern_slice_ptr(p);
+  m.sell_col = pattern_sell_col(p);
+  m.sell_tpos = pattern_sell_tpos(p);
+  return m;
+}
+
+// ---- PCG search direction + SpMV, and alpha (shared with the flexible CG)
+// p_new = z + beta p_old (formed at the gathered columns), q = A p_new; partial p_new.q
+template <typename T, int VEC, bool TRANS>
+static __global__ void __launch_bounds__(kBlock) pcg_spmv_kernel(MatArgs m, const T* __restrict__ vals,
+                                                          const T* __restrict__ z, const T* __restrict__ p_old,
+                                                          T* __restrict__ p_new, T* __restrict__ q,
+                                                          const Scalars* sc, double* partials) {
+  if (sc->done) return;
+  const T beta = (T)sc->omega;
+  int64_t r0, r1;
+  row_range(m.nslices, r0, r1);
+  auto xv = [&](int64_t c) { return z[c] + beta * p_old[c]; };
+  double loc[1] = {0};
+  auto epi = [&](int64_t n, int i, T ap) {
+    const int64_t d = n * VEC + i;
+    const T pn = z[d] + beta * p_old[d];
+    p_new[d] = pn;
+    q[d] = ap;
+    loc[0] += (double)pn * (double)ap;
+  };
+  sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
+  block_partials<1>(loc, partials);
+}
+
+static __global__ void __launch_bounds__(kScalarBlock) pcg_alpha_scalars(Scalars* sc, const double* partials, int nparts) {
+  if (sc->done) return;
+  double tot[1];
+  sum_partials<1>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->pq = tot[0];
+  if (!(tot[0] > 0.0) || !isfinite(tot[0])) {
+    sc->done = 1;
+    sc->status = isfinite(tot[0]) ? 2 : 3;
+  } else {
+    sc->alpha = sc->rho / tot[0];
+  }
+}
+
+
+// host-side hooks implemented in krylov.hip
+extern bool g_prof;
+int prof_begin(hipStream_t s);
+int prof_end(hipStream_t s);
+int prof_collect();
+int poll_init();
+int poll_post(const Scalars* sc, int slot, hipStream_t s);
+int poll_wait(int slot, Scalars& out);
+
+}  // namespace hf

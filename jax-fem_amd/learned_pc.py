@@ -1,0 +1,315 @@
+"""Learned preconditioner: a 3x3x3 CNN on the structured node grid (GPU, libhipfem).
+
+No reference counterpart (the reference's Krylov solves use Jacobi only,
+jax_fem/solver.py:68-69).  The network and the solve are defined in
+csrc/lpc.hip; this module holds the weights, builds the input fields, runs
+inference / training through the C ABI and implements the fallback:
+
+    M(r) = D^-1 r + s D^-1/2 mask N([D^-1/2 r / s, g]),   s = rms(D^-1/2 r)
+
+N: conv 4->16 (tanh), 16->16 (tanh), 16->3, all 3x3x3 with zero padding; g is
+the log Young's modulus averaged to the nodes and divided by 3 (0 for a
+homogeneous body).  With the last layer at zero M is Jacobi.
+
+Training minimises, over residual samples r with r^T D^-1 r = 1, the energy
+    L(z) = 1/2 z^T A z - z^T r,   z = M(r),
+whose minimiser is z = A^-1 r, so dL/dz = A z - r (one SpMV).  The residuals
+are the ones Jacobi-PCG produces on the training matrices (iterations 0..K),
+i.e. the method's own Jacobians and solves.  Forward and backward passes are
+the HIP kernels (hf_cnn_conv3d_fwd/bwd_data/bwd_weight, hf_lpc_features /
+output); only the Adam update of the ~10k weights is a torch operation.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+LAYERS = ((4, 16, 1), (16, 16, 1), (16, 3, 0))   # (cin, cout, tanh)
+
+
+def _dev():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class LearnedPC:
+    def __init__(self, seed=0, device=None, zero_last=True):
+        self.device = device or _dev()
+        g = torch.Generator().manual_seed(seed)
+        self.W, self.B = [], []
+        for li, (ci, co, _) in enumerate(LAYERS):
+            std = math.sqrt(1.0 / (27 * ci))
+            w = torch.randn(27, ci, co, generator=g) * std
+            if zero_last and li == len(LAYERS) - 1:
+                w.zero_()
+            self.W.append(w.float().to(self.device).contiguous())
+            self.B.append(torch.zeros(co, dtype=torch.float32, device=self.device))
+        self._ptrs = None
+
+    # -- parameters ------------------------------------------------------------
+    def params(self):
+        out = []
+        for w, b in zip(self.W, self.B):
+            out += [w, b]
+        return out
+
+    def weight_ptrs(self):
+        arr = (ctypes.c_void_p * 6)(*[p.data_ptr() for p in self.params()])
+        self._ptrs = arr
+        return arr
+
+    def save(self, path):
+        np.savez(path, **{"p%d" % i: p.detach().cpu().numpy() for i, p in enumerate(self.params())})
+
+    @classmethod
+    def load(cls, path, device=None):
+        m = cls(device=device)
+        with np.load(path, allow_pickle=False) as d:
+            for i, p in enumerate(m.params()):
+                p.copy_(torch.as_tensor(d["p%d" % i]))
+        return m
+
+    def poisoned(self, mode):
+        """A copy with broken weights (tests of the fallback): 'nan' or 'huge'."""
+        m = LearnedPC(device=self.device)
+        for a, b in zip(m.params(), self.params()):
+            a.copy_(b)
+        if mode == "nan":
+            m.W[-1].fill_(float("nan"))
+        else:
+            m.W[-1].normal_(0, 1e6)
+        return m
+
+    # -- problem fields ----------------------------------------------------------
+    @staticmethod
+    def grid(problem):
+        g = getattr(problem.fe.mesh, "node_grid", None)
+        if g is None or int(np.prod(g)) != problem.fe.num_total_nodes or problem.vec != 3:
+            raise NotImplementedError("the learned preconditioner needs a structured box_mesh HEX8 grid, vec=3")
+        return tuple(int(x) for x in g)
+
+    @staticmethod
+    def gfield(problem):
+        """log E averaged from cells to nodes, / 3 (float32 device, nnode)."""
+        mat = problem.material
+        iv = problem.internal_vars
+        nnode = problem.fe.num_total_nodes
+        if not iv or mat.param is None:
+            return torch.zeros(nnode, dtype=torch.float32, device=problem.device)
+        p = iv[0].detach().cpu().numpy() if torch.is_tensor(iv[0]) else np.asarray(iv[0])
+        pc = p.reshape(problem.num_cells, -1).mean(axis=1)
+        E = mat._E(pc) if mat.param != "mu" else pc
+        logE = np.log(np.maximum(E / max(mat.E if mat.param == "scale_E" else 1.0, 1e-30), 1e-30))
+        cells = problem.fe.cells
+        acc = np.zeros(nnode)
+        cnt = np.zeros(nnode)
+        np.add.at(acc, cells.reshape(-1), np.repeat(logE, cells.shape[1]))
+        np.add.at(cnt, cells.reshape(-1), 1.0)
+        return torch.as_tensor(acc / np.maximum(cnt, 1) / 3.0, dtype=torch.float32, device=problem.device)
+
+    @staticmethod
+    def mask(problem):
+        m = torch.ones(problem.N, dtype=torch.float32, device=problem.device)
+        d, _ = problem.bc_arrays()
+        m[d.long()] = 0.0
+        return m
+
+    # -- inference / training passes ---------------------------------------------
+    def forward(self, A, R, gfield, mask, grid, keep=False):
+        """z = M(r) for a batch R (B, N) float32; returns z and the cached activations."""
+        nx, ny, nz = grid
+        B = R.shape[0]
+        nnode = nx * ny * nz
+        dinv = A.dinv()
+        R = R.contiguous()
+        s = torch.sqrt(torch.clamp((R * R * dinv).sum(1) / R.shape[1], min=1e-30)).float().contiguous()
+        st = _lib.stream()
+        f = torch.empty(B, nnode, 4, dtype=torch.float32, device=R.device)
+        _lib.call("hf_lpc_features_f32", _lib.ptr(R), _lib.ptr(dinv), _lib.ptr(gfield), nnode, B, _lib.ptr(s),
+                  _lib.ptr(f), st)
+        acts = [f]
+        x = f
+        for (ci, co, act), w, b in zip(LAYERS, self.W, self.B):
+            y = torch.empty(B, nnode, co, dtype=torch.float32, device=R.device)
+            _lib.call("hf_cnn_conv3d_fwd_f32", _lib.ptr(x), ci, _lib.ptr(w), _lib.ptr(b), co, act, nx, ny, nz, B,
+                      _lib.ptr(y), st)
+            acts.append(y)
+            x = y
+        z = torch.empty_like(R)
+        _lib.call("hf_lpc_output_f32", _lib.ptr(R), _lib.ptr(dinv), _lib.ptr(mask), _lib.ptr(x), nnode, B,
+                  _lib.ptr(s), _lib.ptr(z), st)
+        cache = {"acts": acts, "s": s, "dinv": dinv, "mask": mask, "grid": grid, "B": B} if keep else None
+        return z, cache
+
+    def backward(self, cache, gz):
+        """Gradients of <gz, z> w.r.t. (W1, b1, W2, b2, W3, b3) through the HIP backward kernels."""
+        nx, ny, nz = cache["grid"]
+        B = cache["B"]
+        nnode = nx * ny * nz
+        st = _lib.stream()
+        # z = dinv r + s sqrt(dinv) mask y  =>  dL/dy = s sqrt(dinv) mask gz
+        gy = (gz * cache["s"][:, None] * torch.sqrt(cache["dinv"]) * cache["mask"]).reshape(B, nnode, 3).contiguous()
+        acts = cache["acts"]
+        grads = [None] * 6
+        g_out = gy
+        for li in reversed(range(len(LAYERS))):
+            ci, co, act = LAYERS[li]
+            y = acts[li + 1]
+            if act:
+                gpre = torch.empty_like(g_out)
+                _lib.call("hf_cnn_tanh_bwd_f32", _lib.ptr(y), _lib.ptr(g_out), g_out.numel(), _lib.ptr(gpre), st)
+            else:
+                gpre = g_out
+            xin = acts[li]
+            gw = torch.empty_like(self.W[li])
+            gb = torch.empty_like(self.B[li])
+            wsz = _lib.load().hf_cnn_bwd_weight_workspace(ci, co, nx, ny, nz, B)
+            ws = torch.empty(int(wsz), dtype=torch.uint8, device=gz.device)
+            _lib.call("hf_cnn_conv3d_bwd_weight_f32", _lib.ptr(xin), ci, _lib.ptr(gpre), co, nx, ny, nz, B,
+                      _lib.ptr(ws), _lib.ptr(gw), _lib.ptr(gb), st)
+            grads[2 * li], grads[2 * li + 1] = gw, gb
+            if li > 0:
+                gin = torch.empty(B, nnode, ci, dtype=torch.float32, device=gz.device)
+                _lib.call("hf_cnn_conv3d_bwd_data_f32", _lib.ptr(gpre), co, _lib.ptr(self.W[li]), ci, nx, ny, nz, B,
+                          _lib.ptr(gin), st)
+                g_out = gin
+        return grads
+
+    # -- the solve ---------------------------------------------------------------
+    def solve(self, A, b, x0, problem, rtol=1e-6, atol=0.0, maxiter=10000, rel_r0=False, poll=8, stall=60,
+              stats=None):
+        """Flexible CG with M; on breakdown / stagnation / non-finite values,
+        Jacobi-PCG continues (from the FCG iterate, or from x0 if it is not finite).
+        Returns (x, info, fallback_taken)."""
+        from .solver import krylov
+        if A.dtype != torch.float32:
+            raise ValueError("the learned preconditioner runs on the FP32 tangent")
+        grid = self.grid(problem)
+        gfield = self.gfield(problem)
+        mask = self.mask(problem)
+        N = A.shape[0]
+        b = b.to(torch.float32).contiguous()
+        x = (torch.zeros(N, dtype=torch.float32, device=b.device) if x0 is None else
+             x0.to(torch.float32).clone()).contiguous()
+        x_start = x.clone()
+        ws = torch.empty(int(_lib.load().hf_lpc_workspace_size(N // 3)), dtype=torch.uint8, device=b.device)
+        info = _lib.HFKrylovInfo()
+        ptrs = self.weight_ptrs()
+        import time
+        t0 = time.perf_counter()
+        _lib.call("hf_lpc_solve_f32", A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(A.dinv()), _lib.ptr(b), _lib.ptr(x),
+                  ptrs, _lib.ptr(gfield), _lib.ptr(mask), grid[0], grid[1], grid[2], int(maxiter), float(rtol),
+                  float(atol), int(poll), 2 if rel_r0 else 0, int(stall), _lib.ptr(ws), ctypes.byref(info),
+                  _lib.stream())
+        if stats is not None:
+            stats.add("fcg-learned", info.iterations, info.status, info.res_norm, info.b_norm, time.perf_counter() - t0)
+        if info.status in (0, 1):
+            return x, info, False
+        # fallback: Jacobi-PCG to the same absolute target
+        if rel_r0:
+            r0 = torch.linalg.norm((b - A.matvec(x_start)).double()).item()
+            target = max(rtol * r0, atol)
+        else:
+            target = max(rtol * info.b_norm, atol)
+        start = x if (info.status != 3 and bool(torch.isfinite(x).all())) else x_start
+        x2, info2 = krylov(A, b, start, "pcg", 0.0, target, maxiter, True, stats=stats)
+        info2.iterations += info.iterations
+        return x2, info2, True
+
+
+# ---------------------------------------------------------------------------
+# training data: heterogeneous linear elasticity with a log-normal modulus
+def gaussian_random_field(shape, spacing, sigma, ell, rng):
+    """Stationary field with squared-exponential covariance (length ell, std sigma),
+    spectral synthesis on a periodic grid padded to twice the size, cropped."""
+    pad = tuple(2 * n for n in shape)
+    noise = rng.standard_normal(pad)
+    ks = [np.fft.fftfreq(n, d=spacing) for n in pad]
+    K2 = ks[0][:, None, None] ** 2 + ks[1][None, :, None] ** 2 + ks[2][None, None, :] ** 2
+    spec = np.exp(-2.0 * np.pi ** 2 * ell ** 2 * K2)
+    f = np.real(np.fft.ifftn(np.fft.fftn(noise) * np.sqrt(spec)))
+    f = f[:shape[0], :shape[1], :shape[2]]
+    f = (f - f.mean()) / (f.std() + 1e-12) * sigma
+    return f
+
+
+def heterogeneous_modulus(n_cells, seed, sigma=0.5, ell=0.1, lo=0.05, hi=20.0, domain=(1.0, 1.0, 1.0)):
+    """E per cell (cell-centroid log-normal field, clipped), box_mesh cell order."""
+    rng = np.random.default_rng(seed)
+    h = domain[0] / n_cells[0]
+    g = gaussian_random_field(tuple(n_cells), h, sigma, ell, rng)
+    return np.clip(np.exp(g), lo, hi).reshape(-1)
+
+
+def elasticity_instance(jfa, n, E_cell, Lz=1.0):
+    """Linear elasticity on a box with E per cell, u = 0 on x = 0 (the c0/c2 boundary conditions)."""
+    nx, ny, nz = (n, n, n) if np.ndim(n) == 0 else n
+
+    class Elastic(jfa.Problem):
+        def get_tensor_map(self):
+            return jfa.LinearElastic(E=1.0, nu=0.3, param="scale_E")
+
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-7)
+    prob = Elastic(jfa.box_mesh(nx, ny, nz, 1.0, ny / nx, Lz * nz / nx), vec=3, dim=3,
+                   dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3])
+    prob.internal_vars = [np.repeat(np.asarray(E_cell, dtype=np.float64)[:, None], 8, axis=1)]
+    return prob
+
+
+def residual_samples(A, problem, k_list, seed):
+    """Jacobi-PCG residuals r_k = b - A x_k for a random right-hand side (zero on
+    Dirichlet rows), normalised to r^T D^-1 r = 1.  Returns (len(k_list), N) float32."""
+    from .solver import krylov
+    gen = torch.Generator(device=A.vals.device).manual_seed(int(seed))
+    mask = LearnedPC.mask(problem)
+    b = torch.randn(A.shape[0], generator=gen, device=A.vals.device, dtype=torch.float32) * mask
+    out = []
+    for k in k_list:
+        if k == 0:
+            r = b.clone()
+        else:
+            x, _ = krylov(A, b, None, "pcg", 0.0, 0.0, int(k), True)
+            r = b - A.matvec(x)
+        r = r * mask
+        nrm = torch.sqrt((r.double() * r.double() * A.dinv().double()).sum()).float()
+        out.append(r / nrm)
+    return torch.stack(out)
+
+
+def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None, log_every=50):
+    """Adam on the energy loss over (A, problem) pairs; returns the loss history.
+
+    matrices: callable i -> (TangentMatrix FP32, problem) or a list of such pairs."""
+    opt = torch.optim.Adam(model.params(), lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=max(steps, 1), eta_min=lr * 0.05)
+    rng = np.random.default_rng(seed)
+    hist = []
+    cache_fields = {}
+    for step in range(steps):
+        idx = step
+        A, prob = matrices(idx) if callable(matrices) else matrices[idx % len(matrices)]
+        key = id(prob)
+        if key not in cache_fields:
+            cache_fields = {key: (model.gfield(prob), model.mask(prob), model.grid(prob))}
+        g, mask, grid = cache_fields[key]
+        ks = sorted(int(k) for k in rng.integers(0, k_max + 1, size=batch))
+        R = residual_samples(A, prob, ks, seed=rng.integers(1 << 31))
+        z, cache = model.forward(A, R, g, mask, grid, keep=True)
+        Az = torch.stack([A.matvec(zi) for zi in z])
+        loss_per = 0.5 * (z * Az).sum(1) - (z * R).sum(1)
+        # Jacobi reference energy (the untrained model) for monitoring
+        zj = R * A.dinv()
+        Azj = torch.stack([A.matvec(zi) for zi in zj])
+        ej = 0.5 * (zj * Azj).sum(1) - (zj * R).sum(1)
+        gz = (Az - R) / batch
+        grads = model.backward(cache, gz)
+        for p, gp in zip(model.params(), grads):
+            p.grad = gp
+        opt.step()
+        sched.step()
+        hist.append((float(loss_per.mean()), float(ej.mean())))
+        if log and (step % log_every == 0 or step == steps - 1):
+            log("step %d loss %.5f (jacobi %.5f)" % (step, hist[-1][0], hist[-1][1]))
+    return hist

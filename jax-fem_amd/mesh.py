@@ -49,7 +49,10 @@ def box_mesh(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0):
     i, j, k = [a.reshape(-1) for a in np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")]
     o = HEX8_OFFSETS
     cells = idx[i[:, None] + o[None, :, 0], j[:, None] + o[None, :, 1], k[:, None] + o[None, :, 2]]
-    return Mesh(points, cells.astype(np.int64), "HEX8")
+    mesh = Mesh(points, cells.astype(np.int64), "HEX8")
+    mesh.node_grid = (Nx + 1, Ny + 1, Nz + 1)   # structured node lattice, node n = (i*(Ny+1)+j)*(Nz+1)+k
+    mesh.cell_grid = (Nx, Ny, Nz)
+    return mesh
 
 
 def box_mesh_hex27(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0):

@@ -15,6 +15,9 @@ a legacy flat dict, solver.py:1014-1068).  Linear keys:
     = 1e-10, maxiter 10000 (solver.py:63-92), on the device.
   * ``pcg`` / ``bicgstab`` {precond: 'jacobi', tol, atol, maxiter}: the same
     device Krylov methods with explicit settings.
+  * ``learned`` {model: LearnedPC, tol, atol, maxiter, stall}: flexible CG with
+    the learned CNN preconditioner on the FP32 tangent (precision 'mixed'),
+    Jacobi-PCG as its fallback; the Newton residual stays the FP64 judge.
   * ``custom_solver``: callable(A, b, x0, linear_options) -> x (solver.py:278-280);
     A is the device ``TangentMatrix``.
   * ``petsc_solver`` / ``spsolve_solver`` / ``amgx_solver`` are host or NVIDIA
@@ -44,7 +47,8 @@ from . import _lib
 logger = logging.getLogger("jax_fem_amd")
 
 _METHOD_KEYS = ("newton", "arc_length", "dynamic_relax")
-_LINEAR_KEYS = ("jax_solver", "amgx_solver", "spsolve_solver", "petsc_solver", "custom_solver", "pcg", "bicgstab")
+_LINEAR_KEYS = ("jax_solver", "amgx_solver", "spsolve_solver", "petsc_solver", "custom_solver", "pcg", "bicgstab",
+                "learned")
 _NEWTON_KEYS = ("tol", "rel_tol", "line_search_flag", "initial_guess", "precision", "forcing", "max_iter",
                 "eta_max", "eta_min", "eta_first", "backtrack")
 
@@ -253,6 +257,16 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
                              o.get("maxiter", 10000), pc not in (False, "none"), stats=stats,
                              poll=o.get("poll", 8), rel_r0=rtol is not None)
             return x
+    if "learned" in opts:
+        o = opts["learned"] or {}
+        model = o["model"]
+        x, info, fb = model.solve(A, b, x0, A.problem, o.get("tol", 1e-6) if rtol is None else rtol,
+                                  o.get("atol", 0.0) if rtol is not None else o.get("atol", 1e-10),
+                                  o.get("maxiter", 10000), rel_r0=rtol is not None, poll=o.get("poll", 8),
+                                  stall=o.get("stall", 60), stats=stats)
+        if stats is not None:
+            stats.calls[-1]["fallback"] = fb
+        return x
     if "custom_solver" in opts:
         return opts["custom_solver"](A, b, x0, opts)
     for key in ("petsc_solver", "spsolve_solver", "amgx_solver"):
@@ -285,6 +299,8 @@ def solver(problem, solver_options={}):
     lin_opts = dict(cfg.get("linear", {}))
     if precision == "mixed" and not any(k in lin_opts for k in _LINEAR_KEYS):
         lin_opts["pcg"] = {}
+    if "learned" in lin_opts and precision != "mixed":
+        raise ValueError("the learned preconditioner runs on the FP32 tangent: use precision='mixed'")
     adt = torch.float32 if precision == "mixed" else torch.float64
     eta_max = cfg.get("eta_max", 0.1)
     eta_min = cfg.get("eta_min", 1e-6 if precision == "mixed" else 1e-10)
@@ -330,7 +346,7 @@ def solver(problem, solver_options={}):
             eta = None
         t0 = time.perf_counter()
         inc = linear_solver(A, -R, x0, lin_opts, stats=stats, rtol=eta)
-        if stats.calls and stats.calls[-1]["status"] == 2 and any(k in lin_opts for k in ("pcg",)):
+        if stats.calls and stats.calls[-1]["status"] == 2 and "pcg" in lin_opts:
             # p.Ap <= 0: the tangent is not positive definite here; BiCGSTAB does not need it
             logger.debug("newton %d: PCG breakdown, re-solving with BiCGSTAB", n + 1)
             o = dict(lin_opts["pcg"] or {})
@@ -367,6 +383,7 @@ def solver(problem, solver_options={}):
     torch.cuda.synchronize()
     problem.solve_info = {"newton_iters": n, "res_hist": hist, "res0": res0, "rel": res / res0,
                           "krylov": stats.calls, "krylov_iters": stats.iterations,
+                          "fallbacks": sum(1 for c in stats.calls if c.get("fallback")),
                           "timing": timing, "wall": time.perf_counter() - wall0, "precision": precision}
     problem._newton_u = u
     problem._A_cache = None

@@ -1,0 +1,127 @@
+"""Learned preconditioner: HIP forward/backward vs the FP64 CPU oracle (torch autograd),
+fallback behaviour, and the trained network's gain over Jacobi-PCG on held-out instances.
+
+Tolerances: FP32 kernels vs the FP64 oracle at 2e-5 (forward) / 2e-4 (gradients)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+from oracle import cnn as ocnn
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+WEIGHTS = os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_elastic.npz")
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def small_case(jfa, n=(4, 3, 5), seed=3):
+    from importlib import import_module
+    lpc = import_module("jax_fem_amd.learned_pc")
+    E = lpc.heterogeneous_modulus(n, seed)
+    prob = lpc.elasticity_instance(jfa, n, E)
+    S = import_module("jax_fem_amd.solver")
+    A = S.TangentMatrix(prob, prob.jacobian_bsr(torch.zeros(prob.N, device=DEV, dtype=torch.float64), torch.float32))
+    return lpc, prob, A
+
+
+def random_model(lpc, seed=1, scale=0.5):
+    m = lpc.LearnedPC(seed=seed, zero_last=False)
+    g = torch.Generator().manual_seed(seed + 7)
+    for w in m.W:
+        w.mul_(scale)
+    for b in m.B:
+        b.copy_(0.1 * torch.randn(b.shape, generator=g))
+    return m
+
+
+def test_cnn_forward_backward_vs_oracle(jfa):
+    say("cnn fwd/bwd vs oracle")
+    lpc, prob, A = small_case(jfa)
+    m = random_model(lpc)
+    grid = m.grid(prob)
+    g = m.gfield(prob)
+    mask = m.mask(prob)
+    gen = torch.Generator(device=DEV).manual_seed(5)
+    R = (torch.randn(2, prob.N, generator=gen, device=DEV) * mask).contiguous()
+    z, cache = m.forward(A, R, g, mask, grid, keep=True)
+    # oracle (float64 CPU)
+    params = []
+    for w, b in zip(m.W, m.B):
+        params += [ocnn.to_torch_weight(w.cpu().numpy()).requires_grad_(True), b.double().cpu().requires_grad_(True)]
+    dinv = A.dinv().double().cpu()
+    zo = ocnn.forward(params, R.double().cpu(), dinv, g.double().cpu(), mask.double().cpu(), grid)
+    np.testing.assert_allclose(z.double().cpu().numpy(), zo.detach().numpy(), rtol=2e-5,
+                               atol=2e-5 * zo.abs().max().item())
+    gz = torch.randn(2, prob.N, generator=gen, device=DEV)
+    grads = m.backward(cache, gz.contiguous())
+    (zo * gz.double().cpu()).sum().backward()
+    for li in range(3):
+        gw_o = params[2 * li].grad          # torch layout (co, ci, 3, 3, 3)
+        gw = grads[2 * li].double().cpu()   # (27, ci, co)
+        gw_t = ocnn.to_torch_weight(gw.numpy())
+        scale = gw_o.abs().max().item()
+        np.testing.assert_allclose(gw_t.numpy(), gw_o.numpy(), rtol=2e-4, atol=2e-4 * scale)
+        gb_o = params[2 * li + 1].grad
+        np.testing.assert_allclose(grads[2 * li + 1].double().cpu().numpy(), gb_o.numpy(), rtol=2e-4,
+                                   atol=2e-4 * gb_o.abs().max().item())
+
+
+def test_untrained_model_is_jacobi(jfa):
+    say("zero last layer == Jacobi")
+    lpc, prob, A = small_case(jfa)
+    m = lpc.LearnedPC(seed=0, zero_last=True)
+    R = torch.randn(1, prob.N, device=DEV) * m.mask(prob)
+    z, _ = m.forward(A, R.contiguous(), m.gfield(prob), m.mask(prob), m.grid(prob))
+    torch.testing.assert_close(z, R * A.dinv(), rtol=1e-6, atol=1e-7)
+
+
+def _direct(prob, A, b):
+    import scipy.sparse.linalg as spla
+    K = A.to_scipy()
+    return spla.spsolve(K.tocsc(), b.double().cpu().numpy())
+
+
+@pytest.mark.parametrize("mode", ["random", "nan", "huge"])
+def test_fcg_solve_and_fallback(jfa, mode):
+    say("fcg solve", mode)
+    lpc, prob, A = small_case(jfa, (6, 5, 4), 11)
+    m = random_model(lpc, scale=0.3)
+    if mode != "random":
+        m = m.poisoned(mode)
+    b = torch.randn(prob.N, device=DEV) * m.mask(prob)
+    x, info, fb = m.solve(A, b, None, prob, rtol=1e-6, maxiter=5000, stall=40)
+    if mode == "nan":
+        assert fb, "a NaN network must hand over to Jacobi-PCG"
+    K = A.to_scipy()
+    res = np.linalg.norm(K @ x.double().cpu().numpy() - b.double().cpu().numpy()) / np.linalg.norm(b.cpu().numpy())
+    assert res < 5e-6, (res, info.status, info.iterations, fb)
+
+
+def test_trained_network_gain_on_held_out(jfa):
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained weights committed yet")
+    say("trained gain")
+    from importlib import import_module
+    lpc = import_module("jax_fem_amd.learned_pc")
+    S = import_module("jax_fem_amd.solver")
+    m = lpc.LearnedPC.load(WEIGHTS)
+    gains = []
+    for seed in (9001, 9002):          # never used in training (seeds 1..)
+        E = lpc.heterogeneous_modulus((32, 32, 32), seed)
+        prob = lpc.elasticity_instance(jfa, 32, E)
+        A = S.TangentMatrix(prob, prob.jacobian_bsr(torch.zeros(prob.N, device=DEV, dtype=torch.float64),
+                                                    torch.float32))
+        b = torch.randn(prob.N, device=DEV, generator=torch.Generator(device=DEV).manual_seed(seed)) * m.mask(prob)
+        _, info_j = S.krylov(A, b, None, "pcg", 1e-6, 0.0, 20000)
+        _, info_l, fb = m.solve(A, b, None, prob, rtol=1e-6, maxiter=20000)
+        say("seed", seed, "jacobi-pcg", info_j.iterations, "learned fcg", info_l.iterations, "fallback", fb)
+        assert not fb
+        gains.append(info_j.iterations / info_l.iterations)
+    assert min(gains) > 1.2, gains

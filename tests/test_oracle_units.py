@@ -161,3 +161,18 @@ def test_bicgstab_matches_direct_solve():
     import scipy.sparse.linalg as spla
     xd = spla.spsolve(A.tocsc(), b)
     assert info == 0 and np.abs(x - xd).max() < 1e-8 * np.abs(xd).max()
+
+
+def test_cnn_oracle_tap_convention():
+    # torch conv3d (cross-correlation, padding 1) == the defining sum over taps
+    import torch
+    from oracle import cnn as ocnn
+    rng = np.random.default_rng(0)
+    grid = (4, 3, 5)
+    nn = int(np.prod(grid))
+    x = rng.standard_normal((nn, 4))
+    W = rng.standard_normal((27, 4, 6))
+    b = rng.standard_normal(6)
+    y = ocnn.conv(torch.as_tensor(x)[None], ocnn.to_torch_weight(W), torch.as_tensor(b), grid)[0].numpy()
+    for n in (0, 7, 31, nn - 1):
+        np.testing.assert_allclose(y[n], ocnn.conv_direct_node(x, W, b, grid, n), rtol=1e-12, atol=1e-12)
