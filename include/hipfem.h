@@ -179,16 +179,25 @@ int hf_lpc_features_f32(const float* r, const float* dinv, const float* gfield, 
                         const float* scale, float* f, hf_stream_t stream);
 int hf_lpc_output_f32(const float* r, const float* dinv, const float* mask, const float* y, int64_t nnode,
                       int32_t batch, const float* scale, float* z, hf_stream_t stream);
-/* flexible CG with the learned preconditioner (weights: w1,b1,w2,b2,w3,b3 device
- * pointers).  status 0 converged, 1 max iterations, 2 M not positive or
- * p.Ap <= 0, 3 non-finite, 4 stagnation (no 10 % drop of ||r|| in
- * stall_window iterations); on 2-4 the caller continues with Jacobi-PCG. */
+/* the preconditioner applied once, z = M(r), on the inference path the solve
+ * uses (MFMA f16 x f16 -> f32 convolutions; s computed on the device).  The
+ * FP32 layer calls above are the training path it is tested against. */
+size_t hf_lpc_apply_workspace_size(int64_t nnode);
+int hf_lpc_apply_f32(const float* r, const float* dinv, const float* gfield, const float* mask,
+                     const float* const* weights, int32_t nx, int32_t ny, int32_t nz, void* workspace, float* z,
+                     hf_stream_t stream);
+/* truncated flexible CG (each direction A-orthogonal to the last `trunc`,
+ * 1..4) with the learned preconditioner (weights: w1,b1,w2,b2,w3,b3 device
+ * pointers); the network correction is scaled down where needed so that
+ * r.z >= 0.1 r.D^-1.r.  status 0 converged, 1 max iterations, 2 p.Ap <= 0,
+ * 3 non-finite, 4 stagnation (no 10 % drop of ||r|| in stall_window
+ * iterations); on 2-4 the caller continues with Jacobi-PCG. */
 size_t hf_lpc_workspace_size(int64_t nnode);
 int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const float* dinv, const float* b, float* x,
                      const float* const* weights, const float* gfield, const float* mask, int32_t nx,
                      int32_t ny, int32_t nz, int32_t maxiter, double rtol, double atol, int32_t poll,
-                     int32_t flags, int32_t stall_window, void* workspace, hf_krylov_info* info,
-                     hf_stream_t stream);
+                     int32_t flags, int32_t stall_window, int32_t trunc, void* workspace,
+                     hf_krylov_info* info, hf_stream_t stream);
 
 /* ---- measurement: HIP-event timing of the Krylov SpMV kernel (the first SpMV
  *      of every poll batch, on the solve's stream).  No reference counterpart:

@@ -75,9 +75,11 @@ _SIGS = {
     "hf_cnn_tanh_bwd_f32": (c_i32, [vp, vp, c_i64, vp, vp]),
     "hf_lpc_features_f32": (c_i32, [vp, vp, vp, c_i64, c_i32, vp, vp, vp]),
     "hf_lpc_output_f32": (c_i32, [vp, vp, vp, vp, c_i64, c_i32, vp, vp, vp]),
+    "hf_lpc_apply_workspace_size": (c_sz, [c_i64]),
+    "hf_lpc_apply_f32": (c_i32, [vp, vp, vp, vp, vp, c_i32, c_i32, c_i32, vp, vp, vp]),
     "hf_lpc_workspace_size": (c_sz, [c_i64]),
     "hf_lpc_solve_f32": (c_i32, [vp, vp, vp, vp, vp, vp, vp, vp, c_i32, c_i32, c_i32, c_i32, c_f64, c_f64, c_i32,
-                                 c_i32, c_i32, vp, ctypes.POINTER(HFKrylovInfo), vp]),
+                                 c_i32, c_i32, c_i32, vp, ctypes.POINTER(HFKrylovInfo), vp]),
     "hf_dot_workspace_size": (c_sz, []),
     "hf_dot_f64": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),
     "hf_dot_f32": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),

@@ -15,11 +15,17 @@
 //   M(r) = D^-1 r + s * D^-1/2 * mask * y
 // so M is Jacobi plus a learned correction, exactly zero on Dirichlet rows and
 // positively homogeneous in r.  M is nonlinear, so the Krylov method is
-// flexible CG (Polak-Ribiere beta); a breakdown, a non-finite value or
-// stagnation ends the flexible solve and Jacobi-PCG continues from its iterate.
+// truncated flexible CG (Notay): each direction is A-orthogonalised against
+// the last 1..4; the correction is scaled down where needed so that
+// r.z >= 0.1 r.D^-1.r; when the recursive residual meets the tolerance the true
+// residual is recomputed (up to 3 replacements).  Breakdown, a non-finite value
+// or stagnation ends the flexible solve and the caller continues with
+// Jacobi-PCG.
 //
-// Kernels are FP32 on the vector ALU: a thread per node holds COUT
-// accumulators; the layer's weights are staged in LDS and read as broadcasts.
+// Two implementations of the network: FP32 vector-ALU kernels (a thread per
+// node, weights in LDS) used for training with their backward kernels, and
+// the inference path of the solve on MFMA (f16 activations, f32
+// accumulation), tested against the FP32 path.
 #include "krylov_common.hpp"
 
 namespace hf {
@@ -213,6 +219,217 @@ __global__ void lpc_output(const float* __restrict__ r, const float* __restrict_
   z[t] = dinv[d] * r[t] + scale[b] * sqrtf(dinv[d]) * mask[d] * y[t];
 }
 
+// ---- MFMA inference path: f16 activations, f32 accumulation ----------------
+// v_mfma_f32_16x16x16f16 computes D(16x16) += A(16x16) B(16x16); lane l holds
+// A[l%16][4(l/16)+j], B[4(l/16)+j][l%16] and D[4(l/16)+j][l%16], j = 0..3.
+// A conv layer is run as D[co][node] = sum_tap W_tap^T[co][ci] in_tap[ci][node]
+// over tiles of 16 consecutive nodes, so a lane loads 4 channels of one
+// neighbour node (8 bytes) and stores 4 output channels of one node: the
+// channels-last f16 layout a layer writes is the one the next layer reads.
+// Layer 1 (4 channels) packs 4 taps into one K = 16 step (7 steps, the 28th tap
+// is zero); layer 3 pads its 3 output channels to 16.
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kPackW1 = 7 * 64, kPackW2 = 27 * 64, kPackW3 = 27 * 64;  // h4 entries
+
+struct PackedNet {
+  const h4 *w1, *w2, *w3;          // [step][lane] MFMA A operands
+  const float *b1, *b2, *b3;
+};
+
+// Pack W1 (27x4x16), W2 (27x16x16), W3 (27x16x3) fp32 into per-lane f16 A operands.
+__global__ void pack_weights(const float* __restrict__ w1, const float* __restrict__ w2,
+                             const float* __restrict__ w3, h4* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kPackW1 + kPackW2 + kPackW3) return;
+  const int lane = e & 63, co = lane & 15, grp = lane >> 4;
+  h4 v;
+  if (e < kPackW1) {
+    const int tap = 4 * (e >> 6) + grp;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (_Float16)(tap < 27 ? w1[(tap * 4 + j) * 16 + co] : 0.f);
+  } else if (e < kPackW1 + kPackW2) {
+    const int tap = (e - kPackW1) >> 6;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (_Float16)w2[(tap * 16 + 4 * grp + j) * 16 + co];
+  } else {
+    const int tap = (e - kPackW1 - kPackW2) >> 6;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = (_Float16)(co < 3 ? w3[(tap * 16 + 4 * grp + j) * 3 + co] : 0.f);
+  }
+  out[e] = v;
+}
+
+struct NodeIjk {
+  int i, j, k;
+};
+
+__device__ __forceinline__ NodeIjk node_ijk(const CnnGrid& g, int64_t n) {
+  const int64_t plane = (int64_t)g.ny * g.nz;
+  NodeIjk r;
+  r.i = (int)(n / plane);
+  r.j = (int)((n / g.nz) % g.ny);
+  r.k = (int)(n % g.nz);
+  return r;
+}
+
+// neighbour of (i,j,k) at tap, or -1 outside the grid
+__device__ __forceinline__ int64_t tap_nbr(const CnnGrid& g, const NodeIjk& c, int tap) {
+  const int ii = c.i + tap / 9 - 1, jj = c.j + (tap / 3) % 3 - 1, kk = c.k + tap % 3 - 1;
+  if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) return -1;
+  return ((int64_t)ii * g.ny + jj) * g.nz + kk;
+}
+
+__device__ __forceinline__ f4 bias4(const float* __restrict__ b, int grp, int nvalid) {
+  f4 a;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) a[j] = (4 * grp + j < nvalid) ? b[4 * grp + j] : 0.f;
+  return a;
+}
+
+// f[n] = (r[3n+i] sqrt(dinv[3n+i]) / s, i<3 ; gfield[n]) in f16
+__global__ void lpc_features_h(const float* __restrict__ r, const float* __restrict__ dinv,
+                               const float* __restrict__ gfield, int64_t nnode, const float* __restrict__ scale,
+                               h4* __restrict__ f, const int32_t* skip) {
+  if (skip && *skip) return;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  const float inv_s = 1.0f / scale[0];
+  h4 v;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] = (_Float16)(r[n * 3 + i] * sqrtf(dinv[n * 3 + i]) * inv_s);
+  v[3] = (_Float16)gfield[n];
+  f[n] = v;
+}
+
+// h1 = tanh(conv1(f)): 4 -> 16 channels
+__global__ void __launch_bounds__(256) conv1_mfma(const h4* __restrict__ f, PackedNet net, CnnGrid g,
+                                                  h4* __restrict__ h1, const int32_t* skip) {
+  if (skip && *skip) return;
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  h4 wa[7];
+#pragma unroll
+  for (int t = 0; t < 7; ++t) wa[t] = net.w1[t * 64 + lane];
+  const f4 b0 = bias4(net.b1, grp, 16);
+  const int64_t nn = (int64_t)g.nx * g.ny * g.nz, ntiles = (nn + 15) / 16;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const h4 zero = {0, 0, 0, 0};
+  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+    const int64_t n = tile * 16 + col;
+    const bool valid = n < nn;
+    const NodeIjk c = node_ijk(g, valid ? n : 0);
+    f4 acc = b0;
+    h4 bv[7];
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+      const int tap = 4 * t + grp;
+      const int64_t m = (valid && tap < 27) ? tap_nbr(g, c, tap) : -1;
+      bv[t] = m >= 0 ? f[m] : zero;
+    }
+#pragma unroll
+    for (int t = 0; t < 7; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wa[t], bv[t], acc, 0, 0, 0);
+    if (valid) {
+      h4 o;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) o[j] = (_Float16)tanhf(acc[j]);
+      h1[n * 4 + grp] = o;
+    }
+  }
+}
+
+// 27-tap 16-channel accumulation for one tile: acc += sum_tap W_tap^T in[nbr]
+// All 27 gathers are issued before the MFMA chain so their latencies overlap.
+__device__ __forceinline__ f4 conv16_tile(const h4 (&wa)[27], const h4* __restrict__ in, const CnnGrid& g,
+                                          const NodeIjk& c, bool valid, int grp, f4 acc) {
+  const h4 zero = {0, 0, 0, 0};
+  h4 bv[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) {
+    const int64_t m = valid ? tap_nbr(g, c, t) : -1;
+    bv[t] = m >= 0 ? in[m * 4 + grp] : zero;
+  }
+#pragma unroll
+  for (int t = 0; t < 27; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wa[t], bv[t], acc, 0, 0, 0);
+  return acc;
+}
+
+// h2 = tanh(conv2(h1)): 16 -> 16 channels
+__global__ void __launch_bounds__(256) conv2_mfma(const h4* __restrict__ h1, PackedNet net, CnnGrid g,
+                                                  h4* __restrict__ h2, const int32_t* skip) {
+  if (skip && *skip) return;
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  h4 wa[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) wa[t] = net.w2[t * 64 + lane];
+  const f4 b0 = bias4(net.b2, grp, 16);
+  const int64_t nn = (int64_t)g.nx * g.ny * g.nz, ntiles = (nn + 15) / 16;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+    const int64_t n = tile * 16 + col;
+    const bool valid = n < nn;
+    const NodeIjk c = node_ijk(g, valid ? n : 0);
+    const f4 acc = conv16_tile(wa, h1, g, c, valid, grp, b0);
+    if (valid) {
+      h4 o;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) o[j] = (_Float16)tanhf(acc[j]);
+      h2[n * 4 + grp] = o;
+    }
+  }
+}
+
+// z = D^-1 r + s D^-1/2 mask conv3(h2)  (the preconditioner applied once)
+__global__ void __launch_bounds__(256) conv3_apply_mfma(const h4* __restrict__ h2, PackedNet net, CnnGrid g,
+                                                        const float* __restrict__ r, const float* __restrict__ dinv,
+                                                        const float* __restrict__ mask,
+                                                        const float* __restrict__ scale, float* __restrict__ z) {
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  h4 wa[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) wa[t] = net.w3[t * 64 + lane];
+  const f4 b0 = bias4(net.b3, grp, 3);
+  const float s = scale[0];
+  const int64_t nn = (int64_t)g.nx * g.ny * g.nz, ntiles = (nn + 15) / 16;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+    const int64_t n = tile * 16 + col;
+    const bool valid = n < nn;
+    const NodeIjk c = node_ijk(g, valid ? n : 0);
+    const f4 acc = conv16_tile(wa, h2, g, c, valid, grp, b0);
+    if (valid && grp == 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int64_t d = n * 3 + i;
+        z[d] = dinv[d] * r[d] + s * sqrtf(dinv[d]) * mask[d] * acc[i];
+      }
+    }
+  }
+}
+
+// scale = rms(D^-1/2 r): partials then one workgroup
+__global__ void __launch_bounds__(kBlock) rdr_partials(const float* __restrict__ r, const float* __restrict__ dinv,
+                                                       int64_t N, double* partials) {
+  double loc[1] = {0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+    loc[0] += (double)r[i] * (double)r[i] * (double)dinv[i];
+  block_partials<1>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kScalarBlock) rms_scalar(const double* partials, int nparts, int64_t N,
+                                                           float* scale) {
+  double tot[1];
+  sum_partials<1>(partials, nparts, tot);
+  if (threadIdx.x == 0) scale[0] = (float)sqrt(fmax(tot[0], 1e-300) / (double)N);
+}
+
+static inline unsigned tile_grid(int64_t nnode) {
+  const int64_t ntiles = (nnode + 15) / 16;
+  int64_t g = (ntiles + 7) / 8;  // ~2 tiles per wave, 4 waves per workgroup
+  if (g < 1) g = 1;
+  return (unsigned)(g < 2048 ? g : 2048);
+}
+
 // ---- flexible CG with the learned preconditioner ----------------------------
 struct LpcArgs {
   const float *w1, *b1, *w2, *b2, *w3, *b3;
@@ -221,34 +438,44 @@ struct LpcArgs {
   CnnGrid grid;         // batch = 1
 };
 
+constexpr int kMaxTrunc = 4;  // directions kept by the truncated flexible CG
+
 struct FcgScalars {
   float s;       // rms of D^-1/2 r
   int32_t stall;
   double best_rr;
+  double rdr;    // r.D^-1.r of the current residual
+  double theta;  // weight of the network correction in this step's z
+  double pq[kMaxTrunc];    // p_j.A p_j of the kept directions (ring slots)
+  double beta[kMaxTrunc];  // this step's A-orthogonalisation coefficients
+  int32_t base;            // iteration of the last (re)start: directions before it are dropped
+};
+
+// z = D^-1 r + theta c keeps r.z >= kPosFloor r.D^-1.r (theta in [0, 1]).
+constexpr double kPosFloor = 0.1;
+
+struct Ring {
+  float* v[kMaxTrunc];
 };
 
 // init: r = b - A x; partials r.r, b.b, r.D^-1.r
-template <int VEC>
 __global__ void __launch_bounds__(kBlock) fcg_init_kernel(MatArgs m, const float* __restrict__ vals,
                                                           const float* __restrict__ dinv, const float* __restrict__ b,
                                                           const float* __restrict__ x, float* __restrict__ r,
-                                                          float* __restrict__ z, float* __restrict__ p_old,
                                                           double* partials) {
   int64_t r0, r1;
   row_range(m.nslices, r0, r1);
   auto xv = [&](int64_t c) { return x[c]; };
   double loc[3] = {0, 0, 0};
   auto epi = [&](int64_t n, int i, float ax) {
-    const int64_t d = n * VEC + i;
+    const int64_t d = n * 3 + i;
     const float rr = b[d] - ax;
     r[d] = rr;
-    z[d] = 0.f;
-    p_old[d] = 0.f;
     loc[0] += (double)rr * (double)rr;
     loc[1] += (double)b[d] * (double)b[d];
     loc[2] += (double)rr * (double)rr * (double)dinv[d];
   };
-  sweep<float, VEC, false>(m, r0, r1, vals, xv, epi);
+  sweep<float, 3, false>(m, r0, r1, vals, xv, epi);
   block_partials<3>(loc, partials);
 }
 
@@ -262,8 +489,7 @@ __global__ void __launch_bounds__(kScalarBlock) fcg_init_scalars(Scalars* sc, Fc
   sc->rr = tot[0];
   sc->bb = tot[1];
   sc->thresh = fmax(rtol2 * (rel_r0 ? tot[0] : tot[1]), atol2);
-  sc->rho = 0;     // r.z of the previous step (0: first step, beta = 0)
-  sc->omega = 0;   // beta
+  sc->rho = 0;
   sc->alpha = 0;
   sc->iter = 0;
   sc->status = 0;
@@ -274,72 +500,193 @@ __global__ void __launch_bounds__(kScalarBlock) fcg_init_scalars(Scalars* sc, Fc
   fs->s = (float)sqrt(fmax(tot[2], 1e-300) / (double)N);
   fs->stall = 0;
   fs->best_rr = tot[0];
+  fs->rdr = tot[2];
+  fs->theta = 1.0;
+  for (int j = 0; j < kMaxTrunc; ++j) { fs->pq[j] = 1.0; fs->beta[j] = 0.0; }
+  fs->base = 0;
   *scale = fs->s;
 }
 
-// conv3 fused with the output: z_new = dinv r + s sqrt(dinv) mask y; partials r.z_new, r.z_old
-template <int CIN>
-__global__ void __launch_bounds__(256) fcg_conv3_out(const float* __restrict__ in, const float* __restrict__ w,
-                                                     const float* __restrict__ bias, CnnGrid g,
+// Residual replacement after the recursive residual met the tolerance:
+// partials of r = b - A x (recomputed by fcg_init_kernel).  Accept, or restart
+// the flexible CG from the true residual with an empty direction ring.
+__global__ void __launch_bounds__(kScalarBlock) fcg_verify_scalars(Scalars* sc, FcgScalars* fs, float* scale,
+                                                                   const double* partials, int nparts, int64_t N) {
+  double tot[3];
+  sum_partials<3>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->rr = tot[0];
+  if (!isfinite(tot[0])) { sc->done = 1; sc->status = 3; return; }
+  if (tot[0] <= sc->thresh) { sc->done = 1; sc->status = 0; return; }
+  if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; return; }
+  sc->done = 0;
+  sc->status = 0;
+  fs->base = sc->iter;
+  fs->rdr = tot[2];
+  fs->s = (float)sqrt(fmax(tot[2], 1e-300) / (double)N);
+  fs->stall = 0;
+  fs->best_rr = tot[0];
+  *scale = fs->s;
+}
+
+// conv3 (MFMA) fused with the output scaling: c = s sqrt(dinv) mask y (the
+// network's correction to the Jacobi step).  Partials: r.c, then per kept
+// direction j (D^-1 r).q_j and c.q_j.  Grid-stride over tiles so the partials
+// stay bounded.
+template <int M>
+__global__ void __launch_bounds__(256) fcg_conv3_out(const h4* __restrict__ h2, PackedNet net, CnnGrid g,
                                                      const float* __restrict__ r, const float* __restrict__ dinv,
                                                      const float* __restrict__ mask, const FcgScalars* fs,
-                                                     const Scalars* sc, float* __restrict__ z, double* partials) {
-  __shared__ float sw[27 * CIN * 3];
+                                                     const Scalars* sc, Ring Q, float* __restrict__ c,
+                                                     double* partials) {
   if (sc->done) return;
-  for (int e = threadIdx.x; e < 27 * CIN * 3; e += blockDim.x) sw[e] = w[e];
-  __syncthreads();
-  const int64_t nn = (int64_t)g.nx * g.ny * g.nz;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double loc[2] = {0, 0};
-  if (n < nn) {
-    float acc[3] = {bias[0], bias[1], bias[2]};
-    for (int tap = 0; tap < 27; ++tap) {
-      int64_t m;
-      if (!grid_nbr(g, n, tap, m)) continue;
-      float x[CIN];
-      load_ch<CIN>(in + m * CIN, x);
-      const float* wt = sw + tap * CIN * 3;
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  h4 wa[27];
 #pragma unroll
-      for (int ci = 0; ci < CIN; ++ci)
+  for (int t = 0; t < 27; ++t) wa[t] = net.w3[t * 64 + lane];
+  const f4 b0 = bias4(net.b3, grp, 3);
+  const int k = sc->iter - fs->base;
+  const int nq = k < M ? k : M;
+  const float s = fs->s;
+  const int64_t nn = (int64_t)g.nx * g.ny * g.nz, ntiles = (nn + 15) / 16;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  double loc[1 + 2 * M];
 #pragma unroll
-        for (int co = 0; co < 3; ++co) acc[co] += wt[ci * 3 + co] * x[ci];
-    }
-    const float s = fs->s;
+  for (int j = 0; j < 1 + 2 * M; ++j) loc[j] = 0.0;
+  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+    const int64_t n = tile * 16 + col;
+    const bool valid = n < nn;
+    const NodeIjk cc = node_ijk(g, valid ? n : 0);
+    const f4 acc = conv16_tile(wa, h2, g, cc, valid, grp, b0);
+    if (valid && grp == 0) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const int64_t d = n * 3 + i;
-      const float zn = dinv[d] * r[d] + s * sqrtf(dinv[d]) * mask[d] * acc[i];
-      loc[0] += (double)r[d] * (double)zn;
-      loc[1] += (double)r[d] * (double)z[d];
-      z[d] = zn;
+      for (int i = 0; i < 3; ++i) {
+        const int64_t d = n * 3 + i;
+        const float cn = s * sqrtf(dinv[d]) * mask[d] * acc[i];
+        const float jr = dinv[d] * r[d];
+        c[d] = cn;
+        loc[0] += (double)r[d] * (double)cn;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+          if (j < nq) {
+            const double qj = Q.v[j][d];
+            loc[1 + 2 * j] += (double)jr * qj;
+            loc[2 + 2 * j] += (double)cn * qj;
+          }
+        }
+      }
     }
   }
+  block_partials<1 + 2 * M>(loc, partials);
+}
+
+// theta (positivity safeguard), rho = r.z, beta_j = -z.q_j / p_j.q_j
+template <int M>
+__global__ void __launch_bounds__(kScalarBlock) fcg_beta_scalars(Scalars* sc, FcgScalars* fs, const double* partials,
+                                                                 int nparts) {
+  if (sc->done) return;
+  double tot[1 + 2 * M];
+  sum_partials<1 + 2 * M>(partials, nparts, tot);
+  if (threadIdx.x != 0) return;
+  const double a = fs->rdr, rc = tot[0];
+  bool finite = isfinite(rc) && isfinite(a);
+  for (int j = 0; j < 2 * M; ++j) finite = finite && isfinite(tot[1 + j]);
+  if (!finite) {
+    sc->done = 1;
+    sc->status = 3;
+    return;
+  }
+  double theta = 1.0;
+  if (a + rc < kPosFloor * a) theta = (kPosFloor - 1.0) * a / rc;  // rc < 0 here, theta in [0, 1)
+  const double rz = a + theta * rc;
+  if (!(rz > 0.0)) {  // r = 0 on the free rows: nothing left to do
+    sc->done = 1;
+    sc->status = 2;
+    return;
+  }
+  fs->theta = theta;
+  const int k = sc->iter - fs->base;
+  const int nq = k < M ? k : M;
+  for (int j = 0; j < M; ++j)
+    fs->beta[j] = j < nq ? -(tot[1 + 2 * j] + theta * tot[2 + 2 * j]) / fs->pq[j] : 0.0;
+  sc->rho = rz;
+}
+
+// p = D^-1 r + theta c + sum_j beta_j p_j into the ring slot iter % M
+template <int M>
+__global__ void __launch_bounds__(kBlock) fcg_form_p(int64_t N, const float* __restrict__ r,
+                                                     const float* __restrict__ dinv, const float* __restrict__ c,
+                                                     Ring P, const Scalars* sc, const FcgScalars* fs) {
+  if (sc->done) return;
+  const float theta = (float)fs->theta;
+  float beta[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) beta[j] = (float)fs->beta[j];
+  const int k = sc->iter - fs->base;
+  const int nq = k < M ? k : M;
+  float* __restrict__ dst = P.v[k % M];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+    float p = dinv[i] * r[i] + theta * c[i];
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      if (j < nq) p += beta[j] * P.v[j][i];
+    dst[i] = p;
+  }
+}
+
+// q = A p (slot iter % M); partials p.q, r.p
+template <int M>
+__global__ void __launch_bounds__(kBlock) fcg_spmv_kernel(MatArgs m, const float* __restrict__ vals,
+                                                          const float* __restrict__ r, Ring P, Ring Q,
+                                                          const Scalars* sc, const FcgScalars* fs,
+                                                          double* partials) {
+  if (sc->done) return;
+  const int slot = (sc->iter - fs->base) % M;
+  const float* __restrict__ p = P.v[slot];
+  float* __restrict__ q = Q.v[slot];
+  int64_t r0, r1;
+  row_range(m.nslices, r0, r1);
+  auto xv = [&](int64_t j) { return p[j]; };
+  double loc[2] = {0, 0};
+  auto epi = [&](int64_t n, int i, float ap) {
+    const int64_t d = n * 3 + i;
+    q[d] = ap;
+    loc[0] += (double)p[d] * (double)ap;
+    loc[1] += (double)r[d] * (double)p[d];
+  };
+  sweep<float, 3, false>(m, r0, r1, vals, xv, epi);
   block_partials<2>(loc, partials);
 }
 
-// beta (Polak-Ribiere) = (r.z_new - r.z_old) / rz_prev; rho = r.z_new
-__global__ void __launch_bounds__(kScalarBlock) fcg_beta_scalars(Scalars* sc, const double* partials, int nparts) {
+template <int M>
+__global__ void __launch_bounds__(kScalarBlock) fcg_alpha_scalars(Scalars* sc, FcgScalars* fs, const double* partials,
+                                                                  int nparts) {
   if (sc->done) return;
   double tot[2];
   sum_partials<2>(partials, nparts, tot);
   if (threadIdx.x != 0) return;
-  const double rz = tot[0];
-  if (!(rz > 0.0) || !isfinite(rz)) {  // M not positive on r: hand over to Jacobi-PCG
+  sc->pq = tot[0];
+  if (!(tot[0] > 0.0) || !isfinite(tot[0]) || !isfinite(tot[1])) {
     sc->done = 1;
-    sc->status = isfinite(rz) ? 2 : 3;
-    return;
+    sc->status = isfinite(tot[0]) && isfinite(tot[1]) ? 2 : 3;
+  } else {
+    sc->alpha = tot[1] / tot[0];
+    fs->pq[(sc->iter - fs->base) % M] = tot[0];
   }
-  sc->omega = (sc->iter == 0 || sc->rho == 0.0) ? 0.0 : fmax((rz - tot[1]) / sc->rho, 0.0);
-  sc->rho = rz;
 }
 
 // x += alpha p, r -= alpha q; partials r.r, r.D^-1.r
-__global__ void __launch_bounds__(kBlock) fcg_update_kernel(int64_t N, const float* __restrict__ dinv,
-                                                            const float* __restrict__ p, const float* __restrict__ q,
-                                                            float* __restrict__ x, float* __restrict__ r,
-                                                            const Scalars* sc, double* partials) {
+template <int M>
+__global__ void __launch_bounds__(kBlock) fcg_update_kernel(int64_t N, const float* __restrict__ dinv, Ring P,
+                                                            Ring Q, float* __restrict__ x, float* __restrict__ r,
+                                                            const Scalars* sc, const FcgScalars* fs,
+                                                            double* partials) {
   if (sc->done) return;
   const float alpha = (float)sc->alpha;
+  const int slot = (sc->iter - fs->base) % M;
+  const float* __restrict__ p = P.v[slot];
+  const float* __restrict__ q = Q.v[slot];
   const int64_t per = ceil_div(N, gridDim.x);
   const int64_t i0 = (int64_t)logical_block() * per;
   const int64_t i1 = i0 + per < N ? i0 + per : N;
@@ -363,6 +710,7 @@ __global__ void __launch_bounds__(kScalarBlock) fcg_end_scalars(Scalars* sc, Fcg
   if (threadIdx.x != 0) return;
   sc->iter += 1;
   sc->rr = tot[0];
+  fs->rdr = tot[1];
   fs->s = (float)sqrt(fmax(tot[1], 1e-300) / (double)N);
   *scale = fs->s;
   if (tot[0] < 0.81 * fs->best_rr) {  // progress: ||r|| down by 10 %
@@ -485,22 +833,100 @@ extern "C" int hf_lpc_output_f32(const float* r, const float* dinv, const float*
 }
 
 // ---- the solve ---------------------------------------------------------------
+template <int M>
+static void fcg_batch(int poll, const MatArgs& m, const float* vals, const float* dinv, float* x, float* r, float* c,
+                      const Ring& P, const Ring& Q, h4* f, h4* h1, h4* h2, const PackedNet& net,
+                      const float* gfield, const float* mask, const CnnGrid& g, Scalars* sc, FcgScalars* fs,
+                      float* scale, double* partials, int stall_window, hipStream_t s) {
+  const int64_t nnode = m.nrows, N = nnode * 3;
+  const int gr = rows_grid(m.nslices), ge = elem_grid(N);
+  const unsigned gn = (unsigned)ceil_div(nnode, 256), gt = tile_grid(nnode);
+  for (int k = 0; k < poll; ++k) {
+    lpc_features_h<<<gn, 256, 0, s>>>(r, dinv, gfield, nnode, scale, f, &sc->done);
+    conv1_mfma<<<gt, 256, 0, s>>>(f, net, g, h1, &sc->done);
+    conv2_mfma<<<gt, 256, 0, s>>>(h1, net, g, h2, &sc->done);
+    fcg_conv3_out<M><<<gt, 256, 0, s>>>(h2, net, g, r, dinv, mask, fs, sc, Q, c, partials);
+    fcg_beta_scalars<M><<<1, kScalarBlock, 0, s>>>(sc, fs, partials, (int)gt);
+    fcg_form_p<M><<<ge, kBlock, 0, s>>>(N, r, dinv, c, P, sc, fs);
+    fcg_spmv_kernel<M><<<gr, kBlock, 0, s>>>(m, vals, r, P, Q, sc, fs, partials);
+    fcg_alpha_scalars<M><<<1, kScalarBlock, 0, s>>>(sc, fs, partials, gr);
+    fcg_update_kernel<M><<<ge, kBlock, 0, s>>>(N, dinv, P, Q, x, r, sc, fs, partials);
+    fcg_end_scalars<<<1, kScalarBlock, 0, s>>>(sc, fs, scale, partials, ge, N, stall_window);
+  }
+}
+
+static PackedNet pack_net(const float* const* weights, h4* packed, hipStream_t s) {
+  const int tot = kPackW1 + kPackW2 + kPackW3;
+  pack_weights<<<(tot + 255) / 256, 256, 0, s>>>(weights[0], weights[2], weights[4], packed);
+  PackedNet net;
+  net.w1 = packed;
+  net.w2 = packed + kPackW1;
+  net.w3 = packed + kPackW1 + kPackW2;
+  net.b1 = weights[1];
+  net.b2 = weights[3];
+  net.b3 = weights[5];
+  return net;
+}
+
+static inline size_t net_bytes(int64_t nnode) {
+  // packed weights, features (h4 per node), h1 and h2 (4 h4 per node)
+  return align256((size_t)(kPackW1 + kPackW2 + kPackW3) * sizeof(h4)) + align256(nnode * 8) +
+         2 * align256(nnode * 32);
+}
+
+extern "C" size_t hf_lpc_apply_workspace_size(int64_t nnode) { return kHeader + 256 + net_bytes(nnode); }
+
+extern "C" int hf_lpc_apply_f32(const float* r, const float* dinv, const float* gfield, const float* mask,
+                                const float* const* weights, int32_t nx, int32_t ny, int32_t nz, void* ws, float* z,
+                                hf_stream_t stream) {
+  HF_REQUIRE(r && dinv && gfield && mask && weights && ws && z, "hf_lpc_apply_f32: null argument");
+  HF_REQUIRE(nx > 0 && ny > 0 && nz > 0, "hf_lpc_apply_f32: empty grid");
+  for (int i = 0; i < 6; ++i) HF_REQUIRE(weights[i], "hf_lpc_apply_f32: null weight %d", i);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nnode = (int64_t)nx * ny * nz, N = nnode * 3;
+  double* partials = (double*)((char*)ws + 4096);
+  char* cur = (char*)ws + kHeader;
+  float* scale = (float*)cur;
+  cur += 256;
+  h4* packed = (h4*)cur;
+  cur += align256((size_t)(kPackW1 + kPackW2 + kPackW3) * sizeof(h4));
+  h4* f = (h4*)cur;
+  cur += align256(nnode * 8);
+  h4* h1 = (h4*)cur;
+  cur += align256(nnode * 32);
+  h4* h2 = (h4*)cur;
+  const CnnGrid g = mkgrid(nx, ny, nz, 1);
+  const PackedNet net = pack_net(weights, packed, s);
+  const int ge = elem_grid(N);
+  rdr_partials<<<ge, kBlock, 0, s>>>(r, dinv, N, partials);
+  rms_scalar<<<1, kScalarBlock, 0, s>>>(partials, ge, N, scale);
+  const unsigned gn = (unsigned)ceil_div(nnode, 256), gt = tile_grid(nnode);
+  lpc_features_h<<<gn, 256, 0, s>>>(r, dinv, gfield, nnode, scale, f, nullptr);
+  conv1_mfma<<<gt, 256, 0, s>>>(f, net, g, h1, nullptr);
+  conv2_mfma<<<gt, 256, 0, s>>>(h1, net, g, h2, nullptr);
+  conv3_apply_mfma<<<gt, 256, 0, s>>>(h2, net, g, r, dinv, mask, scale, z);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
 extern "C" size_t hf_lpc_workspace_size(int64_t nnode) {
   const int64_t N = nnode * 3;
-  // scalars/partials header, FcgScalars + scale, vectors r z p0 p1 q, features, h1, h2
-  return kHeader + 256 + 5 * align256(N * 4) + align256(nnode * 4 * 4) + 2 * align256(nnode * 16 * 4);
+  // scalars/partials header, FcgScalars + scale, vectors r c P[4] Q[4], network buffers
+  return kHeader + 256 + (2 + 2 * kMaxTrunc) * align256(N * 4) + net_bytes(nnode);
 }
 
 extern "C" int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const float* dinv, const float* b, float* x,
                                 const float* const* weights, const float* gfield, const float* mask, int32_t nx,
                                 int32_t ny, int32_t nz, int32_t maxiter, double rtol, double atol, int32_t poll,
-                                int32_t flags, int32_t stall_window, void* ws, hf_krylov_info* info,
+                                int32_t flags, int32_t stall_window, int32_t trunc, void* ws, hf_krylov_info* info,
                                 hf_stream_t stream) {
   HF_REQUIRE(p && vals && dinv && b && x && weights && gfield && mask && ws, "hf_lpc_solve_f32: null argument");
   HF_REQUIRE(pattern_vec(p) == 3, "hf_lpc_solve_f32: vec must be 3");
+  HF_REQUIRE(trunc >= 1 && trunc <= kMaxTrunc, "hf_lpc_solve_f32: trunc %d outside 1..%d", trunc, kMaxTrunc);
   const MatArgs m = mat_args(p);
   HF_REQUIRE((int64_t)nx * ny * nz == m.nrows, "hf_lpc_solve_f32: grid %dx%dx%d != %lld nodes", nx, ny, nz,
              (long long)m.nrows);
+  static_assert(2048 * (1 + 2 * kMaxTrunc) <= kMaxGrid * 4, "fcg_conv3_out partials exceed the header");
   hipStream_t s = (hipStream_t)stream;
   const int64_t nnode = m.nrows, N = nnode * 3;
   if (poll < 1) poll = 1;
@@ -509,43 +935,58 @@ extern "C" int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const fl
   double* partials = (double*)((char*)ws + 4096);
   char* cur = (char*)ws + kHeader;
   FcgScalars* fs = (FcgScalars*)cur;
+  static_assert(sizeof(FcgScalars) <= 128, "FcgScalars overlaps the scale slot");
   float* scale = (float*)(cur + 128);
   cur += 256;
   auto take = [&](size_t bytes) { char* r_ = cur; cur += align256(bytes); return (float*)r_; };
-  float *r = take(N * 4), *z = take(N * 4), *p0 = take(N * 4), *p1 = take(N * 4), *q = take(N * 4);
-  float *f = take(nnode * 16), *h1 = take(nnode * 64), *h2 = take(nnode * 64);
+  float *r = take(N * 4), *c = take(N * 4);
+  Ring P, Q;
+  for (int j = 0; j < kMaxTrunc; ++j) P.v[j] = take(N * 4);
+  for (int j = 0; j < kMaxTrunc; ++j) Q.v[j] = take(N * 4);
+  h4* packed = (h4*)take((size_t)(kPackW1 + kPackW2 + kPackW3) * sizeof(h4));
+  h4* f = (h4*)take(nnode * 8);
+  h4* h1 = (h4*)take(nnode * 32);
+  h4* h2 = (h4*)take(nnode * 32);
   const CnnGrid g = mkgrid(nx, ny, nz, 1);
-  const int gr = rows_grid(m.nslices), ge = elem_grid(N);
-  const unsigned gn = (unsigned)ceil_div(nnode, 256);
-  fcg_init_kernel<3><<<gr, kBlock, 0, s>>>(m, vals, dinv, b, x, r, z, p0, partials);
+  const PackedNet net = pack_net(weights, packed, s);
+  const int gr = rows_grid(m.nslices);
+  fcg_init_kernel<<<gr, kBlock, 0, s>>>(m, vals, dinv, b, x, r, partials);
   fcg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, fs, scale, partials, gr, N, rtol * rtol, atol * atol, rel_r0,
                                              maxiter);
   HF_LAUNCH_CHECK();
   if (poll_init()) return -2;
   Scalars host{};
-  int it = 0, batch = 0;
-  for (;;) {
-    for (int k = 0; k < poll; ++k, ++it) {
-      float* pold = (it & 1) ? p1 : p0;
-      float* pnew = (it & 1) ? p0 : p1;
-      lpc_features<<<gn, 256, 0, s>>>(r, dinv, gfield, nnode, 1, scale, f, &sc->done);
-      conv3d_fwd<4, 16, 1><<<gn, 256, 0, s>>>(f, weights[0], weights[1], g, h1, &sc->done);
-      conv3d_fwd<16, 16, 1><<<gn, 256, 0, s>>>(h1, weights[2], weights[3], g, h2, &sc->done);
-      fcg_conv3_out<16><<<gn, 256, 0, s>>>(h2, weights[4], weights[5], g, r, dinv, mask, fs, sc, z, partials);
-      fcg_beta_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, (int)gn);
-      pcg_spmv_kernel<float, 3, false><<<gr, kBlock, 0, s>>>(m, vals, z, pold, pnew, q, sc, partials);
-      pcg_alpha_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr);
-      fcg_update_kernel<<<ge, kBlock, 0, s>>>(N, dinv, pnew, q, x, r, sc, partials);
-      fcg_end_scalars<<<1, kScalarBlock, 0, s>>>(sc, fs, scale, partials, ge, N, stall_window);
+  constexpr int kMaxReplace = 3;
+  for (int replaced = 0;; ++replaced) {
+    int it = 0, batch = 0;
+    for (;;) {
+      switch (trunc) {
+#define X(MM) \
+  case MM: fcg_batch<MM>(poll, m, vals, dinv, x, r, c, P, Q, f, h1, h2, net, gfield, mask, g, sc, fs, scale, \
+                         partials, stall_window, s); break;
+        X(1) X(2) X(3) X(4)
+#undef X
+      }
+      it += poll;
+      HF_LAUNCH_CHECK();
+      if (poll_post(sc, batch & 1, s)) return -2;
+      if (batch > 0) {
+        if (poll_wait((batch - 1) & 1, host)) return -2;
+        if (host.done) break;
+      }
+      ++batch;
+      if (it >= maxiter + 2 * poll) break;
     }
+    HF_CHECK_HIP(hipStreamSynchronize(s));
+    HF_CHECK_HIP(hipMemcpy(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    if (!host.done || host.status != 0 || replaced == kMaxReplace) break;
+    // the recursive residual met the tolerance: check the true one
+    fcg_init_kernel<<<gr, kBlock, 0, s>>>(m, vals, dinv, b, x, r, partials);
+    fcg_verify_scalars<<<1, kScalarBlock, 0, s>>>(sc, fs, scale, partials, gr, N);
     HF_LAUNCH_CHECK();
-    if (poll_post(sc, batch & 1, s)) return -2;
-    if (batch > 0) {
-      if (poll_wait((batch - 1) & 1, host)) return -2;
-      if (host.done) break;
-    }
-    ++batch;
-    if (it >= maxiter + 2 * poll) break;
+    HF_CHECK_HIP(hipMemcpyAsync(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+    HF_CHECK_HIP(hipStreamSynchronize(s));
+    if (host.done) break;
   }
   HF_CHECK_HIP(hipStreamSynchronize(s));
   HF_CHECK_HIP(hipMemcpy(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost));

@@ -14,10 +14,16 @@ homogeneous body).  With the last layer at zero M is Jacobi.
 Training minimises, over residual samples r with r^T D^-1 r = 1, the energy
     L(z) = 1/2 z^T A z - z^T r,   z = M(r),
 whose minimiser is z = A^-1 r, so dL/dz = A z - r (one SpMV).  The residuals
-are the ones Jacobi-PCG produces on the training matrices (iterations 0..K),
-i.e. the method's own Jacobians and solves.  Forward and backward passes are
-the HIP kernels (hf_cnn_conv3d_fwd/bwd_data/bwd_weight, hf_lpc_features /
-output); only the Adam update of the ~10k weights is a torch operation.
+are the ones Jacobi-PCG and the model's own flexible CG produce on the
+training matrices (iterations 0..K).  Forward and backward passes are the FP32
+HIP kernels (hf_cnn_conv3d_fwd/bwd_data/bwd_weight, hf_lpc_features /
+output); only the Adam update of the ~10k weights is a torch operation.  The
+solve applies the network on MFMA with f16 activations (hf_lpc_apply_f32 is
+that path for one residual).
+
+The shipped weights (weights/lpc_elastic.npz) come from
+    python jax-fem_amd/train_lpc.py --instances 2000 --epochs 1
+(32^3 heterogeneous elasticity, seeds 1..2000, 2000 Adam steps).
 """
 import ctypes
 import math
@@ -35,6 +41,8 @@ def _dev():
 
 
 class LearnedPC:
+    trunc = 2   # directions kept by the flexible CG
+
     def __init__(self, seed=0, device=None, zero_last=True):
         self.device = device or _dev()
         g = torch.Generator().manual_seed(seed)
@@ -177,10 +185,24 @@ class LearnedPC:
                 g_out = gin
         return grads
 
+    def apply(self, A, r, problem):
+        """z = M(r) for one residual on the solve's inference path (MFMA, f16 activations)."""
+        grid = self.grid(problem)
+        r = r.to(torch.float32).contiguous()
+        nnode = grid[0] * grid[1] * grid[2]
+        ws = torch.empty(int(_lib.load().hf_lpc_apply_workspace_size(nnode)), dtype=torch.uint8, device=r.device)
+        z = torch.empty_like(r)
+        # every tensor passed by pointer stays referenced until the kernels are queued
+        gfield, mask, dinv = self.gfield(problem), self.mask(problem), A.dinv()
+        _lib.call("hf_lpc_apply_f32", _lib.ptr(r), _lib.ptr(dinv), _lib.ptr(gfield), _lib.ptr(mask),
+                  self.weight_ptrs(), grid[0], grid[1], grid[2], _lib.ptr(ws), _lib.ptr(z), _lib.stream())
+        return z
+
     # -- the solve ---------------------------------------------------------------
     def solve(self, A, b, x0, problem, rtol=1e-6, atol=0.0, maxiter=10000, rel_r0=False, poll=8, stall=60,
-              stats=None):
-        """Flexible CG with M; on breakdown / stagnation / non-finite values,
+              trunc=None, stats=None):
+        """Truncated flexible CG with M (each direction A-orthogonal to the last
+        `trunc`, default self.trunc); on breakdown / stagnation / non-finite values,
         Jacobi-PCG continues (from the FCG iterate, or from x0 if it is not finite).
         Returns (x, info, fallback_taken)."""
         from .solver import krylov
@@ -201,8 +223,9 @@ class LearnedPC:
         t0 = time.perf_counter()
         _lib.call("hf_lpc_solve_f32", A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(A.dinv()), _lib.ptr(b), _lib.ptr(x),
                   ptrs, _lib.ptr(gfield), _lib.ptr(mask), grid[0], grid[1], grid[2], int(maxiter), float(rtol),
-                  float(atol), int(poll), 2 if rel_r0 else 0, int(stall), _lib.ptr(ws), ctypes.byref(info),
-                  _lib.stream())
+                  float(atol), int(poll), 2 if rel_r0 else 0, int(stall), int(trunc or self.trunc), _lib.ptr(ws),
+                  ctypes.byref(info), _lib.stream())
+        self.last_fcg = (int(info.iterations), int(info.status))
         if stats is not None:
             stats.add("fcg-learned", info.iterations, info.status, info.res_norm, info.b_norm, time.perf_counter() - t0)
         if info.status in (0, 1):
@@ -213,7 +236,13 @@ class LearnedPC:
             target = max(rtol * r0, atol)
         else:
             target = max(rtol * info.b_norm, atol)
-        start = x if (info.status != 3 and bool(torch.isfinite(x).all())) else x_start
+        # continue from the FCG iterate only if it is finite and better than the start
+        start = x_start
+        if info.status != 3 and bool(torch.isfinite(x).all()):
+            rx = torch.linalg.norm((b - A.matvec(x)).double()).item()
+            rs = torch.linalg.norm((b - A.matvec(x_start)).double()).item()
+            if rx < rs:
+                start = x
         x2, info2 = krylov(A, b, start, "pcg", 0.0, target, maxiter, True, stats=stats)
         info2.iterations += info.iterations
         return x2, info2, True
@@ -278,10 +307,42 @@ def residual_samples(A, problem, k_list, seed):
     return torch.stack(out)
 
 
-def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None, log_every=50):
+def fcg_residuals(model, A, problem, k_list, seed, gfield=None):
+    """Residuals after k steps of the model's own flexible CG (no fallback) on a
+    random right-hand side, normalised like residual_samples."""
+    gen = torch.Generator(device=A.vals.device).manual_seed(int(seed))
+    mask = LearnedPC.mask(problem)
+    b = torch.randn(A.shape[0], generator=gen, device=A.vals.device, dtype=torch.float32) * mask
+    grid = model.grid(problem)
+    gfield = model.gfield(problem) if gfield is None else gfield
+    N = A.shape[0]
+    ws = torch.empty(int(_lib.load().hf_lpc_workspace_size(N // 3)), dtype=torch.uint8, device=b.device)
+    ptrs = model.weight_ptrs()
+    out = []
+    for k in k_list:
+        x = torch.zeros_like(b)
+        if k > 0:
+            info = _lib.HFKrylovInfo()
+            _lib.call("hf_lpc_solve_f32", A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(A.dinv()), _lib.ptr(b),
+                      _lib.ptr(x), ptrs, _lib.ptr(gfield), _lib.ptr(mask), grid[0], grid[1], grid[2], int(k), 0.0,
+                      0.0, 8, 0, 1 << 30, model.trunc, _lib.ptr(ws), ctypes.byref(info), _lib.stream())
+        r = (b - A.matvec(x)) * mask
+        if not bool(torch.isfinite(r).all()):
+            r = b.clone()
+        nrm = torch.sqrt((r.double() * r.double() * A.dinv().double()).sum()).float()
+        out.append(r / nrm)
+    return torch.stack(out)
+
+
+def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None, log_every=50, self_frac=0.5):
+    """self_frac: share of the steps whose residuals come from the model's own
+    flexible CG (fcg_residuals) instead of Jacobi-PCG (residual_samples)."""
     """Adam on the energy loss over (A, problem) pairs; returns the loss history.
 
-    matrices: callable i -> (TangentMatrix FP32, problem) or a list of such pairs."""
+    matrices: callable i -> (TangentMatrix FP32, problem[, key]) or a list of such
+    tuples; the input fields are rebuilt whenever the key (default id(problem))
+    changes, so a caller that reuses one problem object for several instances
+    passes the instance id as the key."""
     opt = torch.optim.Adam(model.params(), lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=max(steps, 1), eta_min=lr * 0.05)
     rng = np.random.default_rng(seed)
@@ -289,13 +350,16 @@ def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None,
     cache_fields = {}
     for step in range(steps):
         idx = step
-        A, prob = matrices(idx) if callable(matrices) else matrices[idx % len(matrices)]
-        key = id(prob)
+        item = matrices(idx) if callable(matrices) else matrices[idx % len(matrices)]
+        A, prob = item[0], item[1]
+        key = item[2] if len(item) > 2 else id(prob)
         if key not in cache_fields:
             cache_fields = {key: (model.gfield(prob), model.mask(prob), model.grid(prob))}
         g, mask, grid = cache_fields[key]
         ks = sorted(int(k) for k in rng.integers(0, k_max + 1, size=batch))
-        R = residual_samples(A, prob, ks, seed=rng.integers(1 << 31))
+        sd = rng.integers(1 << 31)
+        R = (fcg_residuals(model, A, prob, ks, sd, gfield=g) if rng.random() < self_frac else
+             residual_samples(A, prob, ks, seed=sd))
         z, cache = model.forward(A, R, g, mask, grid, keep=True)
         Az = torch.stack([A.matvec(zi) for zi in z])
         loss_per = 0.5 * (z * Az).sum(1) - (z * R).sum(1)

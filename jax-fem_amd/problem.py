@@ -229,8 +229,8 @@ class Problem:
         _lib.call("hf_cell_residual_f64", self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
                   self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(er), s)
         R = torch.empty(self.N, dtype=torch.float64, device=self.device)
-        _lib.call("hf_gather_residual_f64", self.pattern.handle, _lib.ptr(er), _lib.ptr(self._external()),
-                  _lib.ptr(R), s)
+        fext = self._external()
+        _lib.call("hf_gather_residual_f64", self.pattern.handle, _lib.ptr(er), _lib.ptr(fext), _lib.ptr(R), s)
         return R
 
     def jacobian_bsr(self, u, dtype=torch.float64, bc=True):
@@ -261,8 +261,9 @@ class Problem:
         ej = torch.empty(self.num_cells * nn * nn * v * v, dtype=dtype, device=self.device)
         mat = self.material.hf()
         tag = "64" if dtype == torch.float64 else "32"
+        prm = self._params()
         _lib.call("hf_cell_jacobian_f" + tag, nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-                  self.num_cells, _lib.ptr(u), _lib.ptr(self._params()), _lib.ptr(ej), _lib.stream())
+                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), _lib.stream())
         # block-major (c, a, b, i, k) -> (c, a*v+i, b*v+k)
         return ej.reshape(self.num_cells, nn, nn, v, v).permute(0, 1, 3, 2, 4).reshape(
             self.num_cells, nn * v, nn * v)

@@ -113,8 +113,8 @@ class TangentMatrix:
         """(rowptr, colind, values) of the scalar CSR view (FP64), device tensors."""
         rp, ci = self.pattern.csr()
         v = torch.empty(self.pattern.nnz, dtype=torch.float64, device=self.vals.device)
-        _lib.call("hf_matrix_to_csr_f64", self.pattern.handle, _lib.ptr(self.vals.to(torch.float64).contiguous()),
-                  _lib.ptr(v), _lib.stream())
+        vals = self.vals.to(torch.float64).contiguous()
+        _lib.call("hf_matrix_to_csr_f64", self.pattern.handle, _lib.ptr(vals), _lib.ptr(v), _lib.stream())
         return rp, ci, v
 
     def to_scipy(self):
@@ -148,8 +148,9 @@ def apply_bc_vec(res_vec, dofs, problem, scale=1.0):
     """R[B] = u[B] - scale*g (solver.py:290-304); returns a new vector."""
     d, v = problem.bc_arrays()
     R = res_vec.to(torch.float64).clone().contiguous()
-    _lib.call("hf_bc_residual_f64", _lib.ptr(d), _lib.ptr(v), d.numel(), float(scale),
-              _lib.ptr(dofs.to(torch.float64).contiguous()), _lib.ptr(R), _lib.stream())
+    dofs = dofs.to(torch.float64).contiguous()
+    _lib.call("hf_bc_residual_f64", _lib.ptr(d), _lib.ptr(v), d.numel(), float(scale), _lib.ptr(dofs), _lib.ptr(R),
+              _lib.stream())
     return R
 
 
@@ -456,9 +457,9 @@ def param_vjp(problem, u, lamI):
         return None
     grad = torch.empty_like(prm)
     mat = problem.material.hf()
+    u, lamI = u.contiguous(), lamI.contiguous()
     _lib.call("hf_cell_param_vjp_f64", problem.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-              problem.num_cells, _lib.ptr(u.contiguous()), _lib.ptr(prm), _lib.ptr(lamI.contiguous()),
-              _lib.ptr(grad), _lib.stream())
+              problem.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(lamI), _lib.ptr(grad), _lib.stream())
     return grad
 
 

@@ -68,7 +68,7 @@ def main():
             cache[seed] = (TangentMatrix(prob, prob.jacobian_bsr(u0, torch.float32)), E)
         A, E = cache[seed]
         prob.internal_vars = [np.repeat(E[:, None], 8, axis=1)]
-        return A, prob
+        return A, prob, seed
 
     model = L.LearnedPC.load(args.init) if args.init else L.LearnedPC(seed=0)
     steps = int(args.epochs * args.instances)
@@ -90,7 +90,8 @@ def main():
         b = b * L.LearnedPC.mask(prob)
         _, ij = krylov(A, b, None, "pcg", 1e-6, 0.0, 20000)
         _, il, fb = model.solve(A, b, None, prob, rtol=1e-6, maxiter=20000)
-        res.append({"seed": seed, "jacobi_pcg": ij.iterations, "learned_fcg": il.iterations, "fallback": fb})
+        res.append({"seed": seed, "jacobi_pcg": ij.iterations, "learned_fcg": il.iterations, "fallback": fb,
+                    "fcg_part": model.last_fcg})
         print("[eval]", res[-1], flush=True)
     print(json.dumps({"train_seconds": t_train, "steps": steps, "final_loss": hist[-1], "eval": res}))
 

@@ -73,6 +73,30 @@ def test_cnn_forward_backward_vs_oracle(jfa):
                                    atol=2e-4 * gb_o.abs().max().item())
 
 
+@pytest.mark.parametrize("shape", [(6, 5, 4), (17, 9, 33)])
+def test_mfma_apply_matches_fp32_path(jfa, shape):
+    """The solve's inference path (f16 MFMA convolutions) against the FP32
+    training path of the same network.  Tolerance: f16 activations carry
+    about 3 decimal digits, so the learned correction z - D^-1 r agrees to
+    1e-2 relative (the Jacobi part is FP32 in both)."""
+    say("mfma apply", shape)
+    lpc, prob, A = small_case(jfa, shape, 3)
+    m = random_model(lpc, scale=0.5)
+    grid, g, mask = m.grid(prob), m.gfield(prob), m.mask(prob)
+    gen = torch.Generator(device=DEV).manual_seed(7)
+    r = (torch.randn(prob.N, generator=gen, device=DEV) * mask).contiguous()
+    z_ref, _ = m.forward(A, r[None], g, mask, grid)
+    z = m.apply(A, r, prob)
+    jac = r * A.dinv()
+    corr_ref = (z_ref[0] - jac).double()
+    err = torch.linalg.norm(z.double() - z_ref[0].double()) / torch.linalg.norm(corr_ref)
+    assert corr_ref.abs().max() > 0
+    assert err < 1e-2, err.item()
+    # Dirichlet rows carry no correction on either path
+    d = (mask == 0)
+    assert torch.equal(z[d], jac[d])
+
+
 def test_untrained_model_is_jacobi(jfa):
     say("zero last layer == Jacobi")
     lpc, prob, A = small_case(jfa)
@@ -95,13 +119,16 @@ def test_fcg_solve_and_fallback(jfa, mode):
     m = random_model(lpc, scale=0.3)
     if mode != "random":
         m = m.poisoned(mode)
-    b = torch.randn(prob.N, device=DEV) * m.mask(prob)
+    gen = torch.Generator(device=DEV).manual_seed(21)
+    b = torch.randn(prob.N, device=DEV, generator=gen) * m.mask(prob)
     x, info, fb = m.solve(A, b, None, prob, rtol=1e-6, maxiter=5000, stall=40)
     if mode == "nan":
         assert fb, "a NaN network must hand over to Jacobi-PCG"
     K = A.to_scipy()
     res = np.linalg.norm(K @ x.double().cpu().numpy() - b.double().cpu().numpy()) / np.linalg.norm(b.cpu().numpy())
-    assert res < 5e-6, (res, info.status, info.iterations, fb)
+    # rtol 1e-6 on the FP32 recursive residual; the true FP64 residual of an FP32
+    # Krylov iterate sits up to ~1e-5 above it (rounding in x and r updates)
+    assert res < 2e-5, (res, info.status, info.iterations, fb)
 
 
 def test_trained_network_gain_on_held_out(jfa):
