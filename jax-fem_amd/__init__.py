@@ -12,6 +12,7 @@ import logging
 import torch  # noqa: F401  (load torch's HIP runtime before libhipfem)
 
 from . import _lib
+from ._lib import HipFemError
 from .fe import FiniteElement
 from .materials import LinearElastic, NeoHookean, Poisson, Material
 from .mesh import Mesh, box_mesh, box_mesh_hex27, get_meshio_cell_type
@@ -27,7 +28,7 @@ if not logger.handlers:
     logger.addHandler(_h)
     logger.setLevel(logging.INFO)
 
-__all__ = ["Problem", "solver", "ad_wrapper", "implicit_vjp", "get_A", "linear_solver", "apply_bc_vec",
+__all__ = ["HipFemError", "Problem", "solver", "ad_wrapper", "implicit_vjp", "get_A", "linear_solver", "apply_bc_vec",
            "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "get_meshio_cell_type",
            "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov",
            "LearnedPC"]

@@ -416,7 +416,7 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
     for (int k = 0; k < poll; ++k, ++it) {
       T* pold = (it & 1) ? p1 : p0;
       T* pnew = (it & 1) ? p0 : p1;
-      const bool timed = (k == 0 && !timed_pending);
+      const bool timed = (k == poll / 2 && !timed_pending);  // a steady-state launch, mid-batch
       if (timed && prof_begin(s)) return -2;
       pcg_spmv_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, z, pold, pnew, q, sc, partials);
       if (timed && prof_end(s)) return -2;
@@ -435,8 +435,8 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
       if (poll_wait((batch - 1) & 1, host)) return -2;
       if (timed_pending && timed_batch == batch - 1) {
         // the timed SpMV ran in the batch just waited for; it did work unless
-        // the solve had already finished before that batch
-        if (!host.done || host.iter > (batch - 1) * poll) {
+        // the solve had already finished before it
+        if (!host.done || host.iter > (batch - 1) * poll + poll / 2) {
           if (prof_collect()) return -2;
         }
         timed_pending = false;
@@ -483,7 +483,7 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
   int timed_batch = -1;
   for (;;) {
     for (int k = 0; k < poll; ++k, ++it) {
-      const bool timed = (k == 0 && !timed_pending);
+      const bool timed = (k == poll / 2 && !timed_pending);  // a steady-state launch, mid-batch
       bicg_p_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, r, q, pp, phat, sc);
       if (timed && prof_begin(s)) return -2;
       bicg_spmv_kernel<T, VEC, TRANS, 0><<<gr, kBlock, 0, s>>>(m, vals, phat, rhat, q, sc, partials);
@@ -506,8 +506,8 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
       if (poll_wait((batch - 1) & 1, host)) return -2;
       if (timed_pending && timed_batch == batch - 1) {
         // the timed SpMV ran in the batch just waited for; it did work unless
-        // the solve had already finished before that batch
-        if (!host.done || host.iter > (batch - 1) * poll) {
+        // the solve had already finished before it
+        if (!host.done || host.iter > (batch - 1) * poll + poll / 2) {
           if (prof_collect()) return -2;
         }
         timed_pending = false;
