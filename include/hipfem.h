@@ -38,6 +38,10 @@ typedef struct hf_material {
   double E0, Emin, penal; /* SIMP: E = Emin + (E0-Emin) p^penal */
   double kappa;       /* bulk modulus when mu = p (HF_PARAM_MU) */
   double k;           /* Poisson coefficient (flux = k grad u) */
+  int32_t quad1d;     /* Gauss points per direction of the volume rule (quadrature_order
+                         -> (order+2)//2, basis.py:56,64); 0 = HEX8's 2.  Built: HEX8 2,
+                         HEX27 3 (quadrature_order 4 or 5). */
+  int32_t pad_;
 } hf_material;
 
 /* Message of the last failed call on this thread; returns its length. */

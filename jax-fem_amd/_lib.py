@@ -22,7 +22,8 @@ HF_KRYLOV_PCG, HF_KRYLOV_BICGSTAB = 0, 1
 
 class HFMaterial(ctypes.Structure):
     _fields_ = [("kind", c_i32), ("param_mode", c_i32), ("E", c_f64), ("nu", c_f64),
-                ("E0", c_f64), ("Emin", c_f64), ("penal", c_f64), ("kappa", c_f64), ("k", c_f64)]
+                ("E0", c_f64), ("Emin", c_f64), ("penal", c_f64), ("kappa", c_f64), ("k", c_f64),
+                ("quad1d", c_i32), ("pad_", c_i32)]
 
 
 class HFKrylovInfo(ctypes.Structure):

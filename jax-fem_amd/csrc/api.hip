@@ -53,6 +53,12 @@ static int check_args(int ele, const hf_material* m, const double* p, const int3
   HF_REQUIRE(ele == 8 || ele == 27, "cell kernel: element with %d nodes not supported", ele);
   HF_REQUIRE(m->kind >= 0 && m->kind <= 2, "cell kernel: unknown material %d", m->kind);
   HF_REQUIRE(!(m->kind == HF_MAT_POISSON && ele == 27), "cell kernel: HEX27 Poisson not supported");
+  if (ele == 8)
+    HF_REQUIRE(m->quad1d == 0 || m->quad1d == 2, "cell kernel: HEX8 kernels use 2x2x2 Gauss points "
+               "(quadrature_order 2 or 3); got %d per direction", m->quad1d);
+  else
+    HF_REQUIRE(m->quad1d == 3, "cell kernel: HEX27 kernels use 3x3x3 Gauss points (quadrature_order 4 or 5); "
+               "got %d per direction", m->quad1d);
   return 0;
 }
 

@@ -1,0 +1,319 @@
+// HEX27 cell kernels: element residual, element Jacobian, parameter VJP, on
+// 3x3x3 Gauss points (quadrature_order 4 or 5, basis.py:64 rule (order+2)//2).
+//
+// Math as the HEX8 kernels (jax_fem/problem.py:189-214 laplace kernel,
+// :262-266 jacfwd; jax_fem/fe.py:112-141 geometry), with the triquadratic
+// Lagrange basis on [0,1]^3 through the nodes 0, 1/2, 1 per direction, nodes in
+// VTK order (the reference's HEX27 re_order, basis.py:60-61, maps them onto
+// basix).  Quadrature index q = (i*3 + j)*3 + k with x from i.
+//
+// Residual (one wavefront per cell): lane q < 27 builds J = sum_n x_n (x)
+// dN_n/dxi, grad u = (sum_n u_n (x) dN_n/dxi) J^-1, P and S_q = w P J^-T;
+// lane (n, i) then sums r_n[i] = sum_q sum_e dN_n/dxi_e(q) S_q[i][e].
+//
+// Jacobian (one 256-thread workgroup per cell), K[(a,i),(b,k)] =
+// sum_q sum_{J,L} G_a[q][J] C_q[iJkL] G_b[q][L] with C_q the full tangent (w
+// included) and G the physical gradients, all staged in LDS:
+//   M_k[(a,i)][(q,L)] = sum_J G_a[q][J] C_q[iJkL]            (vector ALU)
+//   K_k[(a,i)][b]     = sum_{(q,L)} M_k[(a,i)][(q,L)] G_b[q][L]  (81 x 81 x 27 GEMM)
+// The GEMM runs on MFMA in FP32 (v_mfma_f32_16x16x4f32: 6 x 2 output tiles of
+// 16 x 16, 21 K-steps of 4; 81 padded to 96 rows / 84 inner, 27 to 32 columns)
+// and on the vector ALU in FP64.  2 * 3 * 81 * 81 * 27 = 1.06 MFLOP per cell in
+// the GEMM, 0.12 MFLOP in forming M.  Output block-major (cell, a, b, i, k).
+#include "common.hpp"
+#include "element_math.hpp"
+
+namespace hf {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Lattice position (0, 1, 2 = reference coordinate 0, 1/2, 1) of each VTK node.
+__constant__ int8_t c_h27_node[27][3] = {
+    {0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2}, {0, 2, 2}, {1, 0, 0},
+    {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {1, 0, 2}, {2, 1, 2}, {1, 2, 2}, {0, 1, 2}, {0, 0, 1}, {2, 0, 1},
+    {2, 2, 1}, {0, 2, 1}, {0, 1, 1}, {2, 1, 1}, {1, 0, 1}, {1, 2, 1}, {1, 1, 0}, {1, 1, 2}, {1, 1, 1}};
+
+// 3-point Gauss-Legendre on [0,1]: 1/2 -+ sqrt(3/5)/2, weights 5/18, 8/18, 5/18.
+__host__ __device__ constexpr double g3_point(int i) {
+  return i == 0 ? 0.5 - 0.38729833462074168852 : (i == 1 ? 0.5 : 0.5 + 0.38729833462074168852);
+}
+__host__ __device__ constexpr double g3_weight(int i) { return i == 1 ? 8.0 / 18.0 : 5.0 / 18.0; }
+
+// quadratic Lagrange through 0, 1/2, 1 and its derivative
+template <typename T>
+__device__ __forceinline__ T lag2(int a, T t) {
+  return a == 0 ? (T)2 * (t - (T)0.5) * (t - (T)1) : (a == 1 ? (T)(-4) * t * (t - (T)1) : (T)2 * t * (t - (T)0.5));
+}
+template <typename T>
+__device__ __forceinline__ T dlag2(int a, T t) {
+  return a == 0 ? (T)4 * t - (T)3 : (a == 1 ? (T)4 - (T)8 * t : (T)4 * t - (T)1);
+}
+
+// reference gradient dN_n/dxi at quadrature point q
+template <typename T>
+__device__ __forceinline__ void ref_grad27(int n, int q, T dN[3]) {
+  const T tx = (T)g3_point(q / 9), ty = (T)g3_point((q / 3) % 3), tz = (T)g3_point(q % 3);
+  const int a = c_h27_node[n][0], b = c_h27_node[n][1], c = c_h27_node[n][2];
+  const T vx = lag2<T>(a, tx), vy = lag2<T>(b, ty), vz = lag2<T>(c, tz);
+  dN[0] = dlag2<T>(a, tx) * vy * vz;
+  dN[1] = vx * dlag2<T>(b, ty) * vz;
+  dN[2] = vx * vy * dlag2<T>(c, tz);
+}
+
+__device__ __forceinline__ double g3_qweight(int q) {
+  return g3_weight(q / 9) * g3_weight((q / 3) % 3) * g3_weight(q % 3);
+}
+
+// Geometry and displacement gradient at q from LDS node data: inv = (dx/dxi)^-1,
+// ug = grad u, returns w = det * weight.
+template <typename T>
+__device__ __forceinline__ T hex27_point(int q, const T (*X)[3], const T (*U)[3], T inv[3][3], T ug[3][3]) {
+  T Jm[3][3], Hr[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int e = 0; e < 3; ++e) Jm[d][e] = Hr[d][e] = (T)0;
+  for (int n = 0; n < 27; ++n) {
+    T dN[3];
+    ref_grad27<T>(n, q, dN);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        Jm[d][e] += X[n][d] * dN[e];
+        Hr[d][e] += U[n][d] * dN[e];
+      }
+  }
+  const T det = inv3(Jm, inv);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ug[i][j] = Hr[i][0] * inv[0][j] + Hr[i][1] * inv[1][j] + Hr[i][2] * inv[2][j];
+  return det * (T)g3_qweight(q);
+}
+
+template <typename T>
+__device__ __forceinline__ void hex27_load(const double* __restrict__ pts, const int32_t* __restrict__ cells,
+                                           const double* __restrict__ v, int64_t c, int t, T (*X)[3], T (*U)[3]) {
+  if (t < 27) {
+    const int64_t nid = cells[c * 27 + t];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      X[t][d] = (T)pts[3 * nid + d];
+      U[t][d] = (T)v[3 * nid + d];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, int MAT>
+__global__ void __launch_bounds__(64) hex27_residual_kernel(const double* __restrict__ pts,
+                                                            const int32_t* __restrict__ cells, int64_t ncell,
+                                                            const double* __restrict__ u,
+                                                            const double* __restrict__ params, hf_material mat,
+                                                            T* __restrict__ out) {
+  __shared__ T sX[27][3], sU[27][3], sS[27][9];
+  const int64_t c = blockIdx.x;
+  const int t = threadIdx.x;
+  hex27_load<T>(pts, cells, u, c, t, sX, sU);
+  __syncthreads();
+  if (t < 27) {
+    const int q = t;
+    T inv[3][3], ug[3][3];
+    const T w = hex27_point<T>(q, sX, sU, inv, ug);
+    const Moduli md = moduli(mat, params ? params[c * 27 + q] : 1.0, params != nullptr);
+    T P[3][3];
+    material_stress<T, MAT, 3>(ug, (T)md.m1, (T)md.m2, P);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) sS[q][i * 3 + e] = w * (P[i][0] * inv[e][0] + P[i][1] * inv[e][1] + P[i][2] * inv[e][2]);
+  }
+  __syncthreads();
+  for (int o = t; o < 81; o += 64) {
+    const int n = o / 3, i = o - 3 * (o / 3);
+    T r = 0;
+    for (int q = 0; q < 27; ++q) {
+      T dN[3];
+      ref_grad27<T>(n, q, dN);
+      r += dN[0] * sS[q][i * 3] + dN[1] * sS[q][i * 3 + 1] + dN[2] * sS[q][i * 3 + 2];
+    }
+    out[c * 81 + o] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+constexpr int kMStride = 85;  // row stride of M_k in LDS (81 columns + padding)
+
+template <typename T, int MAT, bool MFMA>
+__global__ void __launch_bounds__(256) hex27_jacobian_kernel(const double* __restrict__ pts,
+                                                             const int32_t* __restrict__ cells, int64_t ncell,
+                                                             const double* __restrict__ u,
+                                                             const double* __restrict__ params, hf_material mat,
+                                                             T* __restrict__ out) {
+  __shared__ T sX[27][3], sU[27][3];
+  __shared__ T sC[27][81];    // tangent per quadrature point, w included
+  __shared__ T sInv[27][9];
+  __shared__ T sG[27][27][3];  // physical gradients [q][node][l]
+  __shared__ T sM[81 * kMStride];
+  const int64_t c = blockIdx.x;
+  const int t = threadIdx.x;
+  hex27_load<T>(pts, cells, u, c, t, sX, sU);
+  __syncthreads();
+  if (t < 27) {
+    const int q = t;
+    T inv[3][3], ug[3][3];
+    const T w = hex27_point<T>(q, sX, sU, inv, ug);
+    const Moduli md = moduli(mat, params ? params[c * 27 + q] : 1.0, params != nullptr);
+    material_tangent<T, MAT>(ug, (T)md.m1, (T)md.m2, w, &sC[q][0]);
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) sInv[q][e * 3 + l] = inv[e][l];
+  }
+  __syncthreads();
+  for (int p = t; p < 27 * 27; p += 256) {
+    const int q = p / 27, b = p - 27 * (p / 27);
+    T dN[3];
+    ref_grad27<T>(b, q, dN);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) sG[q][b][l] = dN[0] * sInv[q][l] + dN[1] * sInv[q][3 + l] + dN[2] * sInv[q][6 + l];
+  }
+  __syncthreads();
+  T* o = out + c * (27 * 27 * 9);
+  for (int k = 0; k < 3; ++k) {
+    for (int p = t; p < 81 * 81; p += 256) {
+      const int row = p / 81, col = p - 81 * (p / 81);
+      const int a = row / 3, i = row - 3 * (row / 3), q = col / 3, l = col - 3 * (col / 3);
+      const T* Cq = &sC[q][0];
+      sM[row * kMStride + col] = sG[q][a][0] * Cq[((i * 3 + 0) * 3 + k) * 3 + l] +
+                                 sG[q][a][1] * Cq[((i * 3 + 1) * 3 + k) * 3 + l] +
+                                 sG[q][a][2] * Cq[((i * 3 + 2) * 3 + k) * 3 + l];
+    }
+    __syncthreads();
+    if constexpr (MFMA) {
+      const int lane = t & 63, wv = t >> 6;
+      for (int tile = wv; tile < 12; tile += 4) {  // wave-uniform
+        const int rt = tile >> 1, ct = tile & 1;
+        const int row = rt * 16 + (lane & 15), b = ct * 16 + (lane & 15);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 3
+        for (int ks = 0; ks < 21; ++ks) {
+          const int kk = ks * 4 + (lane >> 4);
+          const float av = (row < 81 && kk < 81) ? (float)sM[row * kMStride + kk] : 0.f;
+          const float bv = (b < 27 && kk < 81) ? (float)sG[kk / 3][b][kk - 3 * (kk / 3)] : 0.f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+        }
+        const int bo = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int ro = rt * 16 + 4 * (lane >> 4) + r;
+          if (ro < 81 && bo < 27) {
+            const int a = ro / 3, i = ro - 3 * (ro / 3);
+            o[((a * 27 + bo) * 3 + i) * 3 + k] = (T)acc[r];
+          }
+        }
+      }
+    } else {
+      for (int p = t; p < 81 * 27; p += 256) {
+        const int row = p / 27, b = p - 27 * (p / 27);
+        const T* mr = sM + row * kMStride;
+        T s = 0;
+        for (int q = 0; q < 27; ++q)
+          s += mr[3 * q] * sG[q][b][0] + mr[3 * q + 1] * sG[q][b][1] + mr[3 * q + 2] * sG[q][b][2];
+        const int a = row / 3, i = row - 3 * (row / 3);
+        o[((a * 27 + b) * 3 + i) * 3 + k] = s;
+      }
+    }
+    __syncthreads();  // sM is rewritten for the next k
+  }
+}
+
+// grad[c,q] = - w sum_ij (grad lam)_ij dP_ij/dp
+template <int MAT>
+__global__ void __launch_bounds__(64) hex27_param_vjp_kernel(const double* __restrict__ pts,
+                                                             const int32_t* __restrict__ cells, int64_t ncell,
+                                                             const double* __restrict__ u,
+                                                             const double* __restrict__ params, hf_material mat,
+                                                             const double* __restrict__ lam,
+                                                             double* __restrict__ grad) {
+  __shared__ double sX[27][3], sU[27][3], sL[27][3];
+  const int64_t c = blockIdx.x;
+  const int t = threadIdx.x;
+  hex27_load<double>(pts, cells, u, c, t, sX, sU);
+  if (t < 27) {
+    const int64_t nid = cells[c * 27 + t];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sL[t][d] = lam[3 * nid + d];
+  }
+  __syncthreads();
+  if (t < 27) {
+    const int q = t;
+    double inv[3][3], ug[3][3], inv2[3][3], lg[3][3];
+    const double w = hex27_point<double>(q, sX, sU, inv, ug);
+    hex27_point<double>(q, sX, sL, inv2, lg);
+    const Moduli md = moduli(mat, params[c * 27 + q], true);
+    double P1[3][3], P2[3][3];
+    material_stress_parts<double, MAT, 3>(ug, P1, P2);
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s += lg[i][j] * (md.d1 * P1[i][j] + md.d2 * P2[i][j]);
+    grad[c * 27 + q] = -w * s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+static int launch_hex27_residual(const hf_material* m, const double* p, const int32_t* c, int64_t n,
+                                 const double* u, const double* prm, T* o, hipStream_t s) {
+  if (n == 0) return 0;
+  if (m->kind == MAT_LINEAR_ELASTIC)
+    hex27_residual_kernel<T, MAT_LINEAR_ELASTIC><<<(unsigned)n, 64, 0, s>>>(p, c, n, u, prm, *m, o);
+  else
+    hex27_residual_kernel<T, MAT_NEO_HOOKEAN><<<(unsigned)n, 64, 0, s>>>(p, c, n, u, prm, *m, o);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T, bool MFMA>
+static int launch_hex27_jacobian(const hf_material* m, const double* p, const int32_t* c, int64_t n,
+                                 const double* u, const double* prm, T* o, hipStream_t s) {
+  if (n == 0) return 0;
+  if (m->kind == MAT_LINEAR_ELASTIC)
+    hex27_jacobian_kernel<T, MAT_LINEAR_ELASTIC, MFMA><<<(unsigned)n, 256, 0, s>>>(p, c, n, u, prm, *m, o);
+  else
+    hex27_jacobian_kernel<T, MAT_NEO_HOOKEAN, MFMA><<<(unsigned)n, 256, 0, s>>>(p, c, n, u, prm, *m, o);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+int hex27_residual_f64(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                       const double* prm, double* o, hipStream_t s) {
+  return launch_hex27_residual<double>(m, p, c, n, u, prm, o, s);
+}
+int hex27_residual_f32(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                       const double* prm, float* o, hipStream_t s) {
+  return launch_hex27_residual<float>(m, p, c, n, u, prm, o, s);
+}
+int hex27_jacobian_f64(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                       const double* prm, double* o, hipStream_t s) {
+  return launch_hex27_jacobian<double, false>(m, p, c, n, u, prm, o, s);
+}
+int hex27_jacobian_f32(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                       const double* prm, float* o, hipStream_t s) {
+  return launch_hex27_jacobian<float, true>(m, p, c, n, u, prm, o, s);
+}
+int hex27_param_vjp(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                    const double* prm, const double* lam, double* grad, hipStream_t s) {
+  if (n == 0) return 0;
+  if (m->kind == MAT_LINEAR_ELASTIC)
+    hex27_param_vjp_kernel<MAT_LINEAR_ELASTIC><<<(unsigned)n, 64, 0, s>>>(p, c, n, u, prm, *m, lam, grad);
+  else
+    hex27_param_vjp_kernel<MAT_NEO_HOOKEAN><<<(unsigned)n, 64, 0, s>>>(p, c, n, u, prm, *m, lam, grad);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace hf

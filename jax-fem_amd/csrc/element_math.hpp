@@ -182,6 +182,46 @@ __device__ __forceinline__ void material_stress_parts(const T ug[VEC][3], T P1[V
   }
 }
 
+// Full tangent C[((i*3+J)*3+k)*3+L] = dP_iJ / dF_kL, scaled by w (vec = 3).
+//   linear elastic (m1 = mu, m2 = lambda): mu (d_ik d_JL + d_iL d_Jk) + lambda d_iJ d_kL
+//   Neo-Hookean (m1 = mu, m2 = kappa; c = J^-2/3, H = F^-T):
+//     mu c [d_ik d_JL - 2/3 H_kL F_iJ - 2/3 F_kL H_iJ + 2/9 I1 H_kL H_iJ + 1/3 I1 H_iL H_kJ]
+//     + kappa [(2J - 1) J H_kL H_iJ - (J^2 - J) H_iL H_kJ]
+// Contracted with g_a, g_b this is the closed form the HEX8 Jacobian kernel uses.
+template <typename T, int MAT>
+__device__ __forceinline__ void material_tangent(const T ug[3][3], T m1, T m2, T w, T* __restrict__ C) {
+  if constexpr (MAT == MAT_LINEAR_ELASTIC) {
+    const T mu = m1 * w, la = m2 * w;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int J = 0; J < 3; ++J)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int L = 0; L < 3; ++L)
+            C[((i * 3 + J) * 3 + k) * 3 + L] = mu * ((T)(i == k && J == L) + (T)(i == L && J == k)) +
+                                               la * (T)(i == J && k == L);
+  } else {
+    NHState<T> s;
+    nh_state<T>(ug, s);
+    const T s1 = m1 * s.c * w, s2 = m2 * w;
+    const T a1 = (T)(-2.0 / 3.0) * s1, a2 = (T)(2.0 / 9.0) * s1 * s.I1 + s2 * ((T)2 * s.J - (T)1) * s.J;
+    const T a3 = (T)(1.0 / 3.0) * s1 * s.I1 - s2 * s.J * (s.J - (T)1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int J = 0; J < 3; ++J)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int L = 0; L < 3; ++L)
+            C[((i * 3 + J) * 3 + k) * 3 + L] = s1 * (T)(i == k && J == L) +
+                                               a1 * (s.H[k][L] * s.F[i][J] + s.F[k][L] * s.H[i][J]) +
+                                               a2 * s.H[k][L] * s.H[i][J] + a3 * s.H[i][L] * s.H[k][J];
+  }
+}
+
 template <typename T, int MAT, int VEC>
 __device__ __forceinline__ void material_stress(const T ug[VEC][3], T m1, T m2, T P[VEC][3]) {
   T P1[VEC][3], P2[VEC][3];

@@ -64,7 +64,10 @@ def box_mesh_hex27(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0):
     i, j, k = [a.reshape(-1) for a in np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")]
     o = HEX27_OFFSETS
     cells = idx[2 * i[:, None] + o[None, :, 0], 2 * j[:, None] + o[None, :, 1], 2 * k[:, None] + o[None, :, 2]]
-    return Mesh(points, cells.astype(np.int64), "HEX27")
+    mesh = Mesh(points, cells.astype(np.int64), "HEX27")
+    mesh.node_grid = (2 * Nx + 1, 2 * Ny + 1, 2 * Nz + 1)   # node n = (i*(2Ny+1)+j)*(2Nz+1)+k
+    mesh.cell_grid = (Nx, Ny, Nz)
+    return mesh
 
 
 def get_meshio_cell_type(ele_type):

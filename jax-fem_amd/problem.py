@@ -136,6 +136,12 @@ class Problem:
             self._pattern = Pattern(h, self)
         return self._pattern
 
+    def _hfmat(self):
+        """The material's C struct with the volume rule's Gauss points per direction."""
+        mat = self.material.hf()
+        mat.quad1d = (int(self.fe.quadrature_order) + 2) // 2
+        return mat
+
     def _params(self):
         """Per-quadrature parameter (ncell, nq) f64 on the device, or None."""
         iv = self.internal_vars
@@ -223,7 +229,7 @@ class Problem:
         pts, cells = self._dev()
         u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
         er = self._buf("elem_res", self.num_cells * self.nn * self.vec, torch.float64)
-        mat = self.material.hf()
+        mat = self._hfmat()
         prm = self._params()
         s = _lib.stream()
         _lib.call("hf_cell_residual_f64", self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
@@ -241,7 +247,7 @@ class Problem:
         bb = self.vec * self.vec
         tag = "64" if dtype == torch.float64 else "32"
         ej = self._buf("elem_jac" + tag, self.num_cells * self.nn * self.nn * bb, dtype)
-        mat = self.material.hf()
+        mat = self._hfmat()
         prm = self._params()
         s = _lib.stream()
         _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
@@ -259,7 +265,7 @@ class Problem:
         u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
         nn, v = self.nn, self.vec
         ej = torch.empty(self.num_cells * nn * nn * v * v, dtype=dtype, device=self.device)
-        mat = self.material.hf()
+        mat = self._hfmat()
         tag = "64" if dtype == torch.float64 else "32"
         prm = self._params()
         _lib.call("hf_cell_jacobian_f" + tag, nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),

@@ -456,7 +456,7 @@ def param_vjp(problem, u, lamI):
     if prm is None:
         return None
     grad = torch.empty_like(prm)
-    mat = problem.material.hf()
+    mat = problem._hfmat()
     u, lamI = u.contiguous(), lamI.contiguous()
     _lib.call("hf_cell_param_vjp_f64", problem.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
               problem.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(lamI), _lib.ptr(grad), _lib.stream())
