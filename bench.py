@@ -62,6 +62,103 @@ def build_problem(jfa, n):
     return prob, bc
 
 
+def build_hex27(jfa, n):
+    """Neo-Hookean HEX27 n^3 (3x3x3 Gauss), per-cell shear modulus mu_e (the
+    adjoint parameter, uniform E/(2(1+nu)) here) with kappa fixed; BCs as the
+    HEX8 workload, u_x = 0.1 applied in 10 load steps."""
+    import numpy as np
+    E, nu = 10.0, 0.3
+    mat = jfa.NeoHookean(E=E, nu=nu, param="mu", kappa=E / (3 * (1 - 2 * nu)))
+
+    class HyperElasticity27(jfa.Problem):
+        def get_tensor_map(self):
+            return mat
+
+        def set_params(self, p):
+            self.internal_vars = [p[:, None].expand(-1, self.fe.num_quads)]
+
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-5)
+
+    def bc(d):
+        return [[left, left, left, right], [0, 1, 2, 0], [lambda p: 0.0] * 3 + [lambda p: d]]
+
+    prob = HyperElasticity27(jfa.box_mesh_hex27(n, n, n), vec=3, dim=3, ele_type="HEX27", quadrature_order=4,
+                             dirichlet_bc_info=bc(0.0))
+    import torch
+    prob.set_params(torch.full((prob.num_cells,), E / (2 * (1 + nu)), dtype=torch.float64, device="cuda"))
+    return prob, bc
+
+
+def build_hetero(jfa, nx=128, ny=128, nz=64, seed=0):
+    """Heterogeneous linear elasticity on [0,1]x[0,1]x[0,0.5]: E per cell a clipped
+    log-normal Gaussian random field (sigma 0.5, correlation length 0.1), nu 0.3,
+    u = 0 on x = 0, traction (0,0,-10) on x = 1 (surface map (0,0,+10))."""
+    import numpy as np
+    from jax_fem_amd.learned_pc import heterogeneous_modulus
+    E = heterogeneous_modulus((nx, ny, nz), seed)
+
+    class Hetero(jfa.Problem):
+        def get_tensor_map(self):
+            return jfa.LinearElastic(E=1.0, nu=0.3, param="scale_E")
+
+        def get_surface_maps(self):
+            return [lambda u, x: np.array([0.0, 0.0, 10.0])]
+
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-7)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-7)
+    prob = Hetero(jfa.box_mesh(nx, ny, nz, 1.0, ny / nx, nz / nx), vec=3, dim=3,
+                  dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3], location_fns=[right])
+    prob.internal_vars = [np.repeat(E[:, None], 8, axis=1)]
+    return prob
+
+
+def build_simp(jfa, nx=96, ny=48, nz=48, seed=0):
+    """SIMP cantilever on [0,2]x[0,1]x[0,1]: theta = clip(0.5 + N(0, 0.05^2), 1e-3, 1)
+    per cell, E = Emin + (E0 - Emin) theta^3 (E0 = 1, Emin = 1e-3), nu 0.3, fixed
+    on x = 0, surface map (0,0,+1) on x = 2, z <= 0.1."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    theta = np.clip(0.5 + 0.05 * rng.standard_normal(nx * ny * nz), 1e-3, 1.0)
+    mat = jfa.LinearElastic(nu=0.3, param="simp", E0=1.0, Emin=1e-3, penal=3.0)
+
+    class Topo(jfa.Problem):
+        def get_tensor_map(self):
+            return mat
+
+        def get_surface_maps(self):
+            return [lambda u, x: np.array([0.0, 0.0, 1.0])]
+
+        def set_params(self, p):
+            self.internal_vars = [p[:, None].expand(-1, 8) if torch.is_tensor(p) else np.repeat(p[:, None], 8, 1)]
+
+    fixed = lambda p: np.isclose(p[0], 0.0, atol=1e-7)
+    load = lambda p: np.isclose(p[0], 2.0, atol=1e-7) & (p[2] <= 0.1 + 1e-7)
+    prob = Topo(jfa.box_mesh(nx, ny, nz, 2.0, 1.0, 1.0), vec=3, dim=3,
+                dirichlet_bc_info=[[fixed] * 3, [0, 1, 2], [lambda p: 0.0] * 3], location_fns=[load])
+    prob.set_params(theta)
+    return prob, theta
+
+
+def build_elastic20(jfa, n=20):
+    """Linear elasticity on the unit cube, E = 70e3, nu = 0.3, u = 0 on x = 0,
+    traction (0,0,-10) on x = 1 (surface map (0,0,+10))."""
+    import numpy as np
+
+    class Elastic(jfa.Problem):
+        def get_tensor_map(self):
+            return jfa.LinearElastic(E=70e3, nu=0.3)
+
+        def get_surface_maps(self):
+            return [lambda u, x: np.array([0.0, 0.0, 10.0])]
+
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-5)
+    return Elastic(jfa.box_mesh(n, n, n), vec=3, dim=3, dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3],
+                   location_fns=[right])
+
+
 def run_sequence(jfa, prob, bc, opts, nload=5, dmax=0.1):
     import torch
     sol = None
@@ -149,15 +246,116 @@ def _cpu_model():
     return "unknown"
 
 
+WORKLOADS = {
+    "neohookean-hex8-64": "neo-hookean hex8 64^3, 5 load steps, u_x=0.1 on x=1",
+    "neohookean-hex27-48": "neo-hookean hex27 48^3 (3x3x3 Gauss), per-cell mu, 10 load steps, u_x=0.1 on x=1",
+    "hetero-elastic-hex8-128x128x64": "heterogeneous linear elasticity hex8 128x128x64, log-normal E (seed 0), "
+                                      "traction (0,0,-10) on x=1",
+    "simp-hex8-96x48x48": "SIMP cantilever hex8 96x48x48 on [0,2]x[0,1]x[0,1], theta=0.5+N(0,0.05^2) (seed 0), "
+                          "p=3, load (0,0,-1) on x=2, z<=0.1; forward + adjoint of the compliance",
+    "elastic-hex8-20": "linear elasticity hex8 20^3, E=70e3, traction (0,0,-10) on x=1",
+}
+
+
+def solve_case(jfa, prob, bc, opts, nload, dmax):
+    if bc is None:
+        sol = jfa.solver(prob, opts)
+        return sol, [dict(prob.solve_info)]
+    return run_sequence(jfa, prob, bc, opts, nload, dmax)
+
+
+def hex27_jacobian_mfma(jfa, prob):
+    """HIP-event time of one FP32 HEX27 element-Jacobian launch (MFMA) at u = 0."""
+    import ctypes
+    import torch
+    pts, cells = prob._dev()
+    u = torch.zeros(prob.N, dtype=torch.float64, device="cuda")
+    ej = torch.empty(prob.num_cells * 27 * 27 * 9, dtype=torch.float32, device="cuda")
+    mat, prm = prob._hfmat(), prob._params()
+    st = jfa._lib.stream()
+
+    def launch():
+        jfa._lib.call("hf_cell_jacobian_f32", 27, ctypes.byref(mat), jfa._lib.ptr(pts), jfa._lib.ptr(cells),
+                      prob.num_cells, jfa._lib.ptr(u), jfa._lib.ptr(prm), jfa._lib.ptr(ej), st)
+    launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    reps = 5
+    for _ in range(reps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flop_cell = 2 * 3 * 81 * 81 * 27 + 2 * 3 * 81 * 81 * 3   # GEMM + forming M_k
+    tf = prob.num_cells * flop_cell / (ms * 1e-3) / 1e12
+    return {"kernel": "hex27_jacobian_kernel<float, NEO_HOOKEAN, MFMA>", "avg_ms": ms, "flop_per_cell": flop_cell,
+            "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
+            "output_GBps": prob.num_cells * 6561 * 4 / (ms * 1e-3) / 1e9,
+            "note": "flop count of the factored form used (K_k = M_k G per k); the dense B^T D B form counts "
+                    "3,542,940 flop/cell"}
+
+
+def simp_adjoint(jfa, prob, theta, fd_cells, opts_fwd):
+    """Compliance C = integral of t.u over the loaded face = -f_ext.u (the surface
+    map is s = -t): gradient w.r.t. theta by ad_wrapper (transposed solve +
+    parameter VJP), timed, and checked by central differences (h = 1e-4) on
+    fd_cells seeded cells with FP64 forward solves to a 1e-12 relative residual."""
+    import numpy as np
+    import torch
+    th = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+    adj_opts = {"pcg": {"tol": 1e-12, "atol": 0.0}}
+    fwd = jfa.ad_wrapper(prob, opts_fwd, adj_opts)
+    fext = prob._external()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u = fwd(th)[0]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    obj = -(fext * u.reshape(-1)).sum()
+    obj.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = th.grad.detach().cpu().numpy()
+    out = {"objective": "compliance", "value": float(obj.detach()), "forward_s": t1 - t0, "adjoint_s": t2 - t1,
+           "grad_norm": float(np.linalg.norm(g))}
+    if fd_cells:
+        rng = np.random.default_rng(0)
+        cells = rng.choice(len(theta), fd_cells, replace=False)
+        # warm start from u*: the relative tolerance then applies to the O(h) residual
+        # of the perturbation, i.e. 1e-10 of the change in C, not of C itself
+        fd_opts = {"newton": {"precision": "fp64", "tol": 0.0, "rel_tol": 1e-10, "forcing": "fixed", "max_iter": 5,
+                              "initial_guess": [u.detach()], "linear": {"pcg": {"tol": 1e-12, "atol": 0.0}}}}
+        errs = []
+        for c in cells:
+            vals = []
+            for sgn in (1, -1):
+                t = th.detach().clone()
+                t[c] += sgn * 1e-4
+                prob.set_params(t)
+                uu = jfa.solver(prob, fd_opts)[0]
+                vals.append(float(-(fext * uu.reshape(-1)).sum()))
+            fd = (vals[0] - vals[1]) / 2e-4
+            errs.append(abs(fd - g[c]) / max(abs(g[c]), 1e-30))
+            log("fd cell %d: fd %.10e adjoint %.10e rel err %.2e" % (c, fd, g[c], errs[-1]))
+        prob.set_params(th.detach())
+        out["fd_cells"] = int(fd_cells)
+        out["fd_max_rel_err"] = float(max(errs))
+        out["fd_h"] = 1e-4
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=64, help="cells per direction")
+    ap.add_argument("--workload", default="neohookean-hex8-64", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=None, help="cells per direction (cube workloads)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--fd-cells", type=int, default=0, help="SIMP workload: cells checked by central FD")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--debug", action="store_true")
     args = ap.parse_args()
@@ -172,15 +370,36 @@ def main():
     import numpy as np
     import torch
     jfa = load_pkg()
-    lib = jfa._lib.load()
+    jfa._lib.load()
     if args.debug:
         import logging
         logging.getLogger("jax_fem_amd").setLevel(logging.DEBUG)
     torch.cuda.init()
-    n = args.n
-    prob, bc = build_problem(jfa, n)
+    wl = args.workload
+    default = wl == "neohookean-hex8-64"
+    bc, nload, dmax, theta = None, 1, 0.0, None
+    if default:
+        n = args.n or 64
+        prob, bc = build_problem(jfa, n)
+        nload, dmax = 5, 0.1
+        desc = "neo-hookean hex8 %d^3, 5 load steps, u_x=0.1 on x=1" % n
+    elif wl == "neohookean-hex27-48":
+        n = args.n or 48
+        prob, bc = build_hex27(jfa, n)
+        nload, dmax = 10, 0.1
+        desc = WORKLOADS[wl].replace("48^3", "%d^3" % n)
+    elif wl == "hetero-elastic-hex8-128x128x64":
+        prob = build_hetero(jfa)
+        desc = WORKLOADS[wl]
+    elif wl == "simp-hex8-96x48x48":
+        prob, theta = build_simp(jfa)
+        desc = WORKLOADS[wl]
+    else:
+        n = args.n or 20
+        prob = build_elastic20(jfa, n)
+        desc = WORKLOADS[wl].replace("20^3", "%d^3" % n)
     N = prob.N
-    log("problem: hex8 %d^3, %d DOF" % (n, N))
+    log("problem: %s, %d DOF" % (desc, N))
     t0 = time.perf_counter()
     _ = prob.pattern
     torch.cuda.synchronize()
@@ -188,11 +407,16 @@ def main():
 
     # tol=0: stop on the relative FP64 residual 1e-8 only (the reference also stops at ||R|| <= 1e-6)
     mixed = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4, "linear": {"pcg": {}}}}
+    # plain: the reference's jax_solve tolerance 1e-10, relative only, so that small-load
+    # problems (SIMP) still reach the 1e-8 Newton target instead of stalling at an atol
     plain = {"newton": {"precision": "fp64", "tol": 0.0, "forcing": "fixed",
-                        "linear": {"pcg": {"tol": 1e-10, "atol": 1e-10}}}}
+                        "linear": {"pcg": {"tol": 1e-10, "atol": 0.0}}}}
+    if wl == "elastic-hex8-20":
+        # the reference's own configuration: one Newton step, FP64 Jacobi-BiCGSTAB to 1e-10
+        plain = {"newton": {"tol": 0.0, "linear": {"jax_solver": {}}}}
 
     for w in range(args.warmup):
-        run_sequence(jfa, prob, bc, mixed)
+        solve_case(jfa, prob, bc, mixed, nload, dmax)
         log("warmup %d done" % w)
 
     import ctypes
@@ -200,7 +424,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        sol_m, infos_m = run_sequence(jfa, prob, bc, mixed)
+        sol_m, infos_m = solve_case(jfa, prob, bc, mixed, nload, dmax)
         log("timed step %d done" % s)
     torch.cuda.synchronize()
     wall_m = (time.perf_counter() - t0) / args.steps
@@ -208,7 +432,6 @@ def main():
     jfa._lib.call("hf_profile_read", ctypes.byref(nl), ctypes.byref(ms))
     jfa._lib.call("hf_profile_enable", 0)
     spmv_us = 1e3 * ms.value / max(nl.value, 1)
-    nnode = prob.fe.num_total_nodes
     nnz = prob.pattern.nnz
     # algorithmic bytes of one fused PCG SpMV launch: values (4 B/nnz), one block
     # column per node-pair block, z and p_old read, p_new and q written (4 B each)
@@ -220,16 +443,16 @@ def main():
         achieved))
 
     learned_out = None
-    if not args.no_learned:
+    if not args.no_learned and wl != "elastic-hex8-20":
         from jax_fem_amd.learned_pc import LearnedPC
         model = LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_elastic.npz"))
         learned = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4,
                               "linear": {"learned": {"model": model}}}}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        sol_l, infos_l = run_sequence(jfa, prob, bc, learned)
+        sol_l, infos_l = solve_case(jfa, prob, bc, learned, nload, dmax)
         wall_l = time.perf_counter() - t0
-        learned_out = {"dof_per_s": N * 5 / wall_l, "wall_s": wall_l,
+        learned_out = {"dof_per_s": N * nload / wall_l, "wall_s": wall_l,
                        "newton_iters": [i["newton_iters"] for i in infos_l],
                        "krylov_iters": [i["krylov_iters"] for i in infos_l],
                        "fallbacks": [i["fallbacks"] for i in infos_l],
@@ -241,33 +464,36 @@ def main():
 
     plain_out = None
     cpu = None
+    rel_l2 = None
     if not args.no_plain:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        sol_p, infos_p = run_sequence(jfa, prob, bc, plain)
+        sol_p, infos_p = solve_case(jfa, prob, bc, plain, nload, dmax)
         wall_p = time.perf_counter() - t0
         um = sol_m[0].reshape(-1)
         up = sol_p[0].reshape(-1)
         rel_l2 = float(torch.linalg.norm(um - up) / torch.linalg.norm(up))
-        plain_out = {"dof_per_s": N * 5 / wall_p, "wall_s": wall_p,
+        plain_out = {"dof_per_s": N * nload / wall_p, "wall_s": wall_p,
+                     "method": "FP64 Jacobi-BiCGSTAB 1e-10 (jax_solver)" if wl == "elastic-hex8-20"
+                     else "FP64 Jacobi-PCG 1e-10",
                      "newton_iters": [i["newton_iters"] for i in infos_p],
                      "krylov_iters": [i["krylov_iters"] for i in infos_p],
                      "max_rel_residual": max(i["rel"] for i in infos_p)}
         log("plain: %.3f s, newton %s, krylov %s; mixed vs plain rel-L2 %.2e" % (
             wall_p, plain_out["newton_iters"], plain_out["krylov_iters"], rel_l2))
-        if not args.no_cpu:
+        if default and not args.no_cpu:
             t0 = time.perf_counter()
             cpu = cpu_baseline(n, infos_p)
             log("cpu baseline %.1f s of sampling" % (time.perf_counter() - t0))
     out = {
         "metric": "DOF/s + wall-to-1e-8 FP64 residual: learned-PC/FP32-refined vs plain PCG",
-        "value": N * 5 / wall_m,
+        "value": N * nload / wall_m,
         "unit": "DOF/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall_m * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": "neo-hookean hex8 %d^3, 5 load steps, u_x=0.1 on x=1" % n, "dof": N,
+        "config": {"workload": desc, "dof": N,
                    "nnz": nnz, "method": "value: FP32 Jacobi-PCG + FP64 Newton refinement (EW forcing); the "
                                          "learned-PC variant is reported under 'learned_pc'",
                    "wall_to_tol_s": wall_m, "max_rel_residual": rel_m,
@@ -276,15 +502,21 @@ def main():
         "roofline": {"bound": "hbm",
                      "kernel": "pcg_spmv_kernel<float,3> (SELL node-block SpMV fused with p = z + beta p and p.Ap)",
                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-                     "traffic": _pmc_traffic() if n == 64 else None, "avg_launch_us": spmv_us,
+                     "traffic": _pmc_traffic() if (default and n == 64) else None, "avg_launch_us": spmv_us,
                      "algorithmic_bytes": spmv_bytes},
         "learned_pc": learned_out,
         "plain": plain_out,
         "speedup_vs_plain": (plain_out["wall_s"] / wall_m) if plain_out else None,
         "cpu_baseline": cpu,
     }
-    if plain_out:
+    if rel_l2 is not None:
         out["config"]["rel_l2_vs_fp64_plain"] = rel_l2
+    if wl == "neohookean-hex27-48":
+        out["mfma"] = hex27_jacobian_mfma(jfa, prob)
+        log("hex27 jacobian: %.3f ms, %.1f TFLOP/s" % (out["mfma"]["avg_ms"], out["mfma"]["achieved"]))
+    if wl == "simp-hex8-96x48x48":
+        out["adjoint"] = simp_adjoint(jfa, prob, theta, args.fd_cells, mixed)
+        log("adjoint: %s" % json.dumps(out["adjoint"]))
     print(json.dumps(out))
 
 
