@@ -122,12 +122,14 @@ __global__ void csr_view(const int32_t* __restrict__ rowptr, const int32_t* __re
 // longest row (in blocks); its slots are slice_ptr[s] .. slice_ptr[s+1]-1.
 // Slot t holds, for each row r of the slice, the r-th row's j-th block
 // (j = t - slice_ptr[s]) or zero padding; the block's vec x vec entries are
-// contiguous and row-major:
-//     entry (i, jj)  vals[(t*R + r)*vec*vec + i*vec + jj]   (= base + i*vec + jj,
+// contiguous and column-major:
+//     entry (i, jj)  vals[(t*R + r)*vec*vec + jj*vec + i]   (= base + jj*vec + i,
 //                    base = (t*R + r)*vec*vec)
 //     column         sell_col[t*R + r]
-// so lane (r, i) of a wavefront reads its vec values of slot t as one
-// contiguous vec-word load, and the L = R*vec lanes read L*vec consecutive values.
+// so lane (r, i) of a wavefront reads column i of its row's block of slot t as
+// one contiguous vec-word load (the L = R*vec lanes read L*vec consecutive
+// values), multiplies it by the single gathered entry x[col*vec + i] and keeps
+// vec partial outputs; the vec lanes of a row are summed once per slice.
 __global__ void slice_width(const int32_t* __restrict__ rowptr, int64_t nnode, int R, int64_t nslices,
                             int32_t* __restrict__ W) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -201,7 +203,7 @@ __global__ void sell_to_csr_kernel(const int32_t* __restrict__ rowptr, const int
   (void)R;
   for (int i = 0; i < vec; ++i)
     for (int j = 0; j < vec; ++j)
-      csr[(int64_t)b0 * vec * vec + (int64_t)i * len + (b - b0) * vec + j] = vals[base + i * vec + j];
+      csr[(int64_t)b0 * vec * vec + (int64_t)i * len + (b - b0) * vec + j] = vals[base + j * vec + i];
 }
 
 // One thread per node row: every slot of the row's slice gets the sum of the
@@ -237,8 +239,9 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
     }
     T* dst = vals + ((int64_t)t * R + r) * BB;
     (void)L;
+    // element blocks are row-major (i, k); matrix blocks are stored column-major
 #pragma unroll
-    for (int e = 0; e < BB; ++e) dst[e] = acc[e];
+    for (int e = 0; e < BB; ++e) dst[(e % VEC) * VEC + e / VEC] = acc[e];
   }
 }
 
@@ -279,7 +282,7 @@ __global__ void bc_rows_kernel(const int32_t* __restrict__ rowptr, const int32_t
   const int32_t len = rowptr[n + 1] - rowptr[n];
   for (int32_t j = 0; j < len; ++j) {
     T* base = vals + ((int64_t)(slice_ptr[s] + j) * R + r) * vec * vec;
-    for (int jj = 0; jj < vec; ++jj) base[i * vec + jj] = (T)0;
+    for (int jj = 0; jj < vec; ++jj) base[jj * vec + i] = (T)0;  // row i of a column-major block
   }
   if (dpos[n] >= 0) vals[(int64_t)dpos[n] + i * vec + i] = (T)1;
 }

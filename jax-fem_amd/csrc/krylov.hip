@@ -9,13 +9,16 @@
 // matrix when the initial guess is exact on the Dirichlet rows (then every
 // Krylov vector is 0 on them and the iteration runs on the symmetric A_II).
 //
-// SpMV mapping (wave64): one wavefront per block row.  A block row's values
-// are contiguous (blocks of vec*vec); lane l handles entry e = l % BB of block
-// rowstart + l / BB + G*it with G = 64 / BB blocks per step (7 for 3x3 blocks,
-// lanes 63 idle), so each step streams 63 consecutive values.  Rows are given
-// to workgroups in contiguous chunks, and consecutive workgroups are remapped
-// so that each XCD (workgroups are dealt round-robin over 8 XCDs) works on one
+// SpMV: sliced ELL over node blocks, column-major blocks, one wavefront per
+// slice of 21 node rows (krylov_common.hpp, sweep).  Slices are given to
+// workgroups in contiguous chunks, and consecutive workgroups are remapped so
+// that each XCD (workgroups are dealt round-robin over 8 XCDs) works on one
 // contiguous range of rows: neighbouring rows share x entries in that XCD's L2.
+//
+// PCG is the Chronopoulos-Gear form: the SpMV multiplies the preconditioned
+// residual u (one gathered vector), the direction p and s = A p follow by
+// recurrences in the elementwise kernel, and all dot products of an iteration
+// are reduced in one phase (one scalar kernel per iteration).
 //
 // Reductions: every workgroup of a vector kernel writes its FP64 partial sums
 // with plain stores; a one-workgroup scalar kernel launched next adds them in
@@ -23,6 +26,8 @@
 // atomics between workgroups: on a multi-XCD GPU a device-scope release per
 // workgroup costs an L2 write-back, while the kernel boundary already makes
 // the partials visible.  Results are bitwise reproducible run to run.
+#include <hip/hip_ext.h>
+
 #include "common.hpp"
 #include "hipfem.h"
 
@@ -42,13 +47,21 @@ __global__ void __launch_bounds__(kBlock) spmv_kernel(MatArgs m, const T* __rest
 }
 
 // ---------------------------------------------------------------------------
-// PCG (Jacobi).  init: r = b - A x, z = dinv r, p_old = 0; partials r.r, r.z, b.b
+// PCG (Jacobi), Chronopoulos-Gear form:
+//   u = M r, w = A u, gamma = r.u, delta = w.u,
+//   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev),
+//   p = u + beta p, s = w + beta s (= A p), x += alpha p, r -= alpha s.
+// delta - beta gamma / alpha_prev equals p.Ap: <= 0 is a breakdown (status 2).
+// Partials: region A (the elementwise / init kernels) and region B (the SpMV).
+constexpr int kPartB = 3 * kMaxGrid;
+
+// init: r = b - A x, u = dinv r, p = s = 0; partials (A) r.u, r.r, b.b
 template <typename T, int VEC, bool TRANS>
-__global__ void __launch_bounds__(kBlock) pcg_init_kernel(MatArgs m, const T* __restrict__ vals,
-                                                          const T* __restrict__ dinv, const T* __restrict__ b,
-                                                          const T* __restrict__ x, T* __restrict__ r,
-                                                          T* __restrict__ z, T* __restrict__ p_old,
-                                                          double* partials) {
+__global__ void __launch_bounds__(kBlock) cg_init_kernel(MatArgs m, const T* __restrict__ vals,
+                                                         const T* __restrict__ dinv, const T* __restrict__ b,
+                                                         const T* __restrict__ x, T* __restrict__ r,
+                                                         T* __restrict__ u, T* __restrict__ p, T* __restrict__ sv,
+                                                         double* partials) {
   int64_t r0, r1;
   row_range(m.nslices, r0, r1);
   auto xv = [&](int64_t c) { return x[c]; };
@@ -56,73 +69,113 @@ __global__ void __launch_bounds__(kBlock) pcg_init_kernel(MatArgs m, const T* __
   auto epi = [&](int64_t n, int i, T ax) {
     const int64_t d = n * VEC + i;
     const T rr = b[d] - ax;
-    const T zz = dinv[d] * rr;
+    const T uu = dinv[d] * rr;
     r[d] = rr;
-    z[d] = zz;
-    p_old[d] = (T)0;
-    loc[0] += (double)rr * (double)rr;
-    loc[1] += (double)rr * (double)zz;
+    u[d] = uu;
+    p[d] = (T)0;
+    sv[d] = (T)0;
+    loc[0] += (double)rr * (double)uu;
+    loc[1] += (double)rr * (double)rr;
     loc[2] += (double)b[d] * (double)b[d];
   };
   sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
   block_partials<3>(loc, partials);
 }
 
+// w = A u; partial (B) u.w.  sc = nullptr at set-up (the flags are not yet reset).
+template <typename T, int VEC, bool TRANS>
+__global__ void __launch_bounds__(kBlock) cg_spmv_kernel(MatArgs m, const T* __restrict__ vals,
+                                                         const T* __restrict__ u, T* __restrict__ w,
+                                                         const Scalars* sc, double* partials) {
+  if (sc && sc->done) return;
+  int64_t r0, r1;
+  row_range(m.nslices, r0, r1);
+  auto xv = [&](int64_t c) { return u[c]; };
+  double loc[1] = {0};
+  auto epi = [&](int64_t n, int i, T v) {
+    const int64_t d = n * VEC + i;
+    w[d] = v;
+    loc[0] += (double)u[d] * (double)v;
+  };
+  sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
+  block_partials<1>(loc, partials);
+}
+
 // Stop threshold: max(rtol^2 * ref, atol^2), ref = ||b||^2 or ||r0||^2 (rel_r0).
-__global__ void __launch_bounds__(kScalarBlock) pcg_init_scalars(Scalars* sc, const double* partials, int nparts,
-                                                           double rtol2, double atol2, int rel_r0, int maxiter) {
-  double tot[3];
-  sum_partials<3>(partials, nparts, tot);
+__global__ void __launch_bounds__(kScalarBlock) cg_init_scalars(Scalars* sc, const double* partA, int nA,
+                                                                const double* partB, int nB, double rtol2,
+                                                                double atol2, int rel_r0, int maxiter) {
+  double a[3], d[1];
+  sum_partials<3>(partA, nA, a);
+  sum_partials<1>(partB, nB, d);
   if (threadIdx.x != 0) return;
-  sc->rr = tot[0];
-  sc->rho = tot[1];
-  sc->bb = tot[2];
-  sc->thresh = fmax(rtol2 * (rel_r0 ? tot[0] : tot[2]), atol2);
-  sc->alpha = 0;
-  sc->omega = 0;  // beta of the first direction (p_new = z)
+  sc->rho = a[0];   // gamma
+  sc->rr = a[1];
+  sc->bb = a[2];
+  sc->pq = d[0];    // delta
+  sc->thresh = fmax(rtol2 * (rel_r0 ? a[1] : a[2]), atol2);
+  sc->omega = 0;    // beta of the first direction (p = u)
   sc->iter = 0;
   sc->status = 0;
   sc->maxiter = maxiter;
-  sc->done = (tot[0] <= sc->thresh) ? 1 : 0;
-  if (!isfinite(tot[0])) { sc->done = 1; sc->status = 3; }
-  else if (maxiter <= 0 && !sc->done) { sc->done = 1; sc->status = 1; }
+  sc->done = (a[1] <= sc->thresh) ? 1 : 0;
+  if (!isfinite(a[1])) { sc->done = 1; sc->status = 3; }
+  else if (!sc->done && maxiter <= 0) { sc->done = 1; sc->status = 1; }
+  else if (!sc->done && !(d[0] > 0.0)) { sc->done = 1; sc->status = isfinite(d[0]) ? 2 : 3; }
+  sc->alpha = sc->done ? 0.0 : a[0] / d[0];
 }
 
-// x += alpha p, r -= alpha q, z = dinv r; partials r.z, r.r
+// p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = dinv r; partials (A) r.u, r.r
 template <typename T>
-__global__ void __launch_bounds__(kBlock) pcg_update_kernel(
-    int64_t N, const T* __restrict__ dinv, const T* __restrict__ p, const T* __restrict__ q,
-    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const Scalars* sc, double* partials) {
+__global__ void __launch_bounds__(kBlock) cg_update_kernel(int64_t N, const T* __restrict__ dinv,
+                                                           T* __restrict__ u, const T* __restrict__ w,
+                                                           T* __restrict__ p, T* __restrict__ sv, T* __restrict__ x,
+                                                           T* __restrict__ r, const Scalars* sc, double* partials) {
   if (sc->done) return;
-  const T alpha = (T)sc->alpha;
+  const T alpha = (T)sc->alpha, beta = (T)sc->omega;
   const int64_t per = ceil_div(N, gridDim.x);
   const int64_t i0 = (int64_t)logical_block() * per;
   const int64_t i1 = i0 + per < N ? i0 + per : N;
   double loc[2] = {0, 0};
   for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
-    x[i] += alpha * p[i];
-    const T rr = r[i] - alpha * q[i];
-    const T zz = dinv[i] * rr;
+    const T pp = u[i] + beta * p[i];
+    const T ss = w[i] + beta * sv[i];
+    p[i] = pp;
+    sv[i] = ss;
+    x[i] += alpha * pp;
+    const T rr = r[i] - alpha * ss;
+    const T uu = dinv[i] * rr;
     r[i] = rr;
-    z[i] = zz;
-    loc[0] += (double)rr * (double)zz;
+    u[i] = uu;
+    loc[0] += (double)rr * (double)uu;
     loc[1] += (double)rr * (double)rr;
   }
   block_partials<2>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kScalarBlock) pcg_beta_scalars(Scalars* sc, const double* partials, int nparts) {
+__global__ void __launch_bounds__(kScalarBlock) cg_scalars(Scalars* sc, const double* partA, int nA,
+                                                           const double* partB, int nB) {
   if (sc->done) return;
-  double tot[2];
-  sum_partials<2>(partials, nparts, tot);
+  double a[2], d[1];
+  sum_partials<2>(partA, nA, a);
+  sum_partials<1>(partB, nB, d);
   if (threadIdx.x != 0) return;
   sc->iter += 1;
-  sc->rr = tot[1];
-  sc->omega = tot[0] / sc->rho;  // beta
-  sc->rho = tot[0];
-  if (!isfinite(tot[1])) { sc->done = 1; sc->status = 3; }
-  else if (tot[1] <= sc->thresh) { sc->done = 1; sc->status = 0; }
-  else if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; }
+  sc->rr = a[1];
+  if (!isfinite(a[1])) { sc->done = 1; sc->status = 3; return; }
+  if (a[1] <= sc->thresh) { sc->done = 1; sc->status = 0; return; }
+  if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; return; }
+  const double gamma = a[0], beta = gamma / sc->rho;
+  const double denom = d[0] - beta * gamma / sc->alpha;  // p.Ap of the next direction
+  if (!(denom > 0.0) || !isfinite(denom)) {
+    sc->done = 1;
+    sc->status = isfinite(denom) ? 2 : 3;
+    return;
+  }
+  sc->omega = beta;
+  sc->alpha = gamma / denom;
+  sc->rho = gamma;
+  sc->pq = denom;
 }
 
 // ---------------------------------------------------------------------------
@@ -331,6 +384,17 @@ int prof_begin(hipStream_t s) {
   HF_CHECK_HIP(hipEventRecord(g_ev[0], s));
   return 0;
 }
+// The pair of timing events, created on first use; a launch given to
+// hipExtLaunchKernelGGL with them records at the kernel's own start and end.
+int prof_events(hipEvent_t* e0, hipEvent_t* e1) {
+  if (!g_ev[0]) {
+    HF_CHECK_HIP(hipEventCreate(&g_ev[0]));
+    HF_CHECK_HIP(hipEventCreate(&g_ev[1]));
+  }
+  *e0 = g_ev[0];
+  *e1 = g_ev[1];
+  return 0;
+}
 int prof_end(hipStream_t s) {
   if (!g_prof) return 0;
   HF_CHECK_HIP(hipEventRecord(g_ev[1], s));
@@ -399,13 +463,16 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
   const MatArgs m = mat_args(p);
   const int64_t N = m.nrows * VEC;
   Scalars* sc = (Scalars*)ws;
-  double* partials = (double*)((char*)ws + 4096);
+  double* partA = (double*)((char*)ws + 4096);
+  double* partB = partA + kPartB;
   T* v = (T*)((char*)ws + kHeader);
   const size_t stride = align256(N * sizeof(T)) / sizeof(T);
-  T *r = v, *z = v + stride, *p0 = v + 2 * stride, *p1 = v + 3 * stride, *q = v + 4 * stride;
+  T *r = v, *u = v + stride, *pv = v + 2 * stride, *sv = v + 3 * stride, *w = v + 4 * stride;
   const int gr = rows_grid(m.nslices), ge = elem_grid(N);
-  pcg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, dinv, b, x, r, z, p0, partials);
-  pcg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
+  static_assert(kPartB + kMaxGrid <= 4 * kMaxGrid, "partials exceed the workspace header");
+  cg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, dinv, b, x, r, u, pv, sv, partA);
+  cg_spmv_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, u, w, nullptr, partB);
+  cg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, gr, partB, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   HF_LAUNCH_CHECK();
   if (poll_init()) return -2;
   Scalars host{};
@@ -414,19 +481,19 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
   int timed_batch = -1;
   for (;;) {
     for (int k = 0; k < poll; ++k, ++it) {
-      T* pold = (it & 1) ? p1 : p0;
-      T* pnew = (it & 1) ? p0 : p1;
       const bool timed = (k == poll / 2 && !timed_pending);  // a steady-state launch, mid-batch
-      if (timed && prof_begin(s)) return -2;
-      pcg_spmv_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, z, pold, pnew, q, sc, partials);
-      if (timed && prof_end(s)) return -2;
+      cg_update_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, u, w, pv, sv, x, r, sc, partA);
       if (timed && g_prof) {
+        hipEvent_t e0, e1;
+        if (prof_events(&e0, &e1)) return -2;
+        hipExtLaunchKernelGGL((cg_spmv_kernel<T, VEC, TRANS>), dim3(gr), dim3(kBlock), 0, s, e0, e1, 0, m, vals, u,
+                              w, (const Scalars*)sc, partB);
         timed_pending = true;
         timed_batch = batch;
+      } else {
+        cg_spmv_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, u, w, sc, partB);
       }
-      pcg_alpha_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr);
-      pcg_update_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, pnew, q, x, r, z, sc, partials);
-      pcg_beta_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, ge);
+      cg_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, ge, partB, gr);
     }
     HF_LAUNCH_CHECK();
     if (poll_post(sc, batch & 1, s)) return -2;

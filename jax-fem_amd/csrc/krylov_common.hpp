@@ -110,6 +110,22 @@ struct SellShape {
   static constexpr int BB = VEC * VEC;
 };
 
+// LDS written by the lanes of one wavefront and read back by other lanes of it.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// y = A x (or A^T x) over the slices [s0, s1), one wavefront per slice; epi(n, i, y)
+// receives scalar row (node n, component i).  xv(k) returns entry k of x.
+// VEC = 3 (column form): lane (r, i), lane = 3r + i < 63, streams column i of
+// its row's block in each slot (3 contiguous values), gathers the single entry
+// x[col*3 + i] and keeps 3 partial outputs; the 3 lanes of a row are summed
+// through LDS once per slice.  Each entry of x is gathered once per block,
+// not once per lane of the row.
+// TRANS: the block of A^T at (n, c) is the transpose of the stored block at
+// sell_tpos (block (c, n), column-major), so lane (r, i) reads row i of it.
 template <typename T, int VEC, bool TRANS, typename XF, typename EPI>
 __device__ __forceinline__ void sweep(const MatArgs& m, int64_t s0, int64_t s1, const T* __restrict__ vals, XF xv,
                                       EPI epi) {
@@ -117,30 +133,61 @@ __device__ __forceinline__ void sweep(const MatArgs& m, int64_t s0, int64_t s1, 
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int r = lane / VEC, i = lane - (lane / VEC) * VEC;
   const bool in_slice = lane < S::L;
-  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int32_t t0 = m.slice_ptr[s], t1 = m.slice_ptr[s + 1];
-    T y = 0;
-    if (in_slice) {
-#pragma unroll 4
-      for (int32_t t = t0; t < t1; ++t) {
-        const int64_t cr = (int64_t)t * S::R + r;
-        const int64_t c = (int64_t)m.sell_col[cr] * VEC;
-        if constexpr (TRANS) {
-          const int32_t tp = m.sell_tpos[cr];
-#pragma unroll
-          for (int jj = 0; jj < VEC; ++jj) {
-            const T a = tp >= 0 ? vals[(int64_t)tp + jj * VEC + i] : (T)0;
-            y += a * xv(c + jj);
+  if constexpr (VEC == 1) {
+    for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+      const int32_t t0 = m.slice_ptr[s], t1 = m.slice_ptr[s + 1];
+      T y = 0;
+      if (in_slice) {
+#pragma unroll 8
+        for (int32_t t = t0; t < t1; ++t) {
+          const int64_t cr = (int64_t)t * S::R + r;
+          const T x = xv((int64_t)m.sell_col[cr]);
+          if constexpr (TRANS) {
+            const int32_t tp = m.sell_tpos[cr];
+            y += (tp >= 0 ? vals[tp] : (T)0) * x;
+          } else {
+            y += __builtin_nontemporal_load(vals + (int64_t)t * S::L + lane) * x;
           }
-        } else {
-          const T* vrow = vals + ((int64_t)t * S::L + lane) * VEC;
-#pragma unroll
-          for (int jj = 0; jj < VEC; ++jj) y += __builtin_nontemporal_load(vrow + jj) * xv(c + jj);
         }
       }
+      const int64_t n = s * S::R + r;
+      if (in_slice && n < m.nrows) epi(n, 0, y);
     }
-    const int64_t n = s * S::R + r;
-    if (in_slice && n < m.nrows) epi(n, i, y);
+  } else {
+    __shared__ T red[kWavesPerBlock][kWave * VEC];
+    for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+      const int32_t t0 = m.slice_ptr[s], t1 = m.slice_ptr[s + 1];
+      T y[VEC];
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) y[k] = 0;
+      if (in_slice) {
+#pragma unroll 8
+        for (int32_t t = t0; t < t1; ++t) {
+          const int64_t cr = (int64_t)t * S::R + r;
+          const T x = xv((int64_t)m.sell_col[cr] * VEC + i);
+          if constexpr (TRANS) {
+            const int32_t tp = m.sell_tpos[cr];
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) y[k] += (tp >= 0 ? vals[(int64_t)tp + k * VEC + i] : (T)0) * x;
+          } else {
+            const T* vc = vals + ((int64_t)t * S::L + lane) * VEC;
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) y[k] += __builtin_nontemporal_load(vc + k) * x;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) red[w][lane * VEC + k] = y[k];
+      wave_lds_sync();
+      if (in_slice) {
+        T yi = 0;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) yi += red[w][(r * VEC + j) * VEC + i];
+        const int64_t n = s * S::R + r;
+        if (n < m.nrows) epi(n, i, yi);
+      }
+      wave_lds_sync();  // the next slice rewrites red
+    }
   }
 }
 
@@ -176,45 +223,6 @@ inline MatArgs mat_args(const hf_pattern* p) {
   m.sell_tpos = pattern_sell_tpos(p);
   return m;
 }
-
-// ---- PCG search direction + SpMV, and alpha (shared with the flexible CG)
-// p_new = z + beta p_old (formed at the gathered columns), q = A p_new; partial p_new.q
-template <typename T, int VEC, bool TRANS>
-static __global__ void __launch_bounds__(kBlock) pcg_spmv_kernel(MatArgs m, const T* __restrict__ vals,
-                                                          const T* __restrict__ z, const T* __restrict__ p_old,
-                                                          T* __restrict__ p_new, T* __restrict__ q,
-                                                          const Scalars* sc, double* partials) {
-  if (sc->done) return;
-  const T beta = (T)sc->omega;
-  int64_t r0, r1;
-  row_range(m.nslices, r0, r1);
-  auto xv = [&](int64_t c) { return z[c] + beta * p_old[c]; };
-  double loc[1] = {0};
-  auto epi = [&](int64_t n, int i, T ap) {
-    const int64_t d = n * VEC + i;
-    const T pn = z[d] + beta * p_old[d];
-    p_new[d] = pn;
-    q[d] = ap;
-    loc[0] += (double)pn * (double)ap;
-  };
-  sweep<T, VEC, TRANS>(m, r0, r1, vals, xv, epi);
-  block_partials<1>(loc, partials);
-}
-
-static __global__ void __launch_bounds__(kScalarBlock) pcg_alpha_scalars(Scalars* sc, const double* partials, int nparts) {
-  if (sc->done) return;
-  double tot[1];
-  sum_partials<1>(partials, nparts, tot);
-  if (threadIdx.x != 0) return;
-  sc->pq = tot[0];
-  if (!(tot[0] > 0.0) || !isfinite(tot[0])) {
-    sc->done = 1;
-    sc->status = isfinite(tot[0]) ? 2 : 3;
-  } else {
-    sc->alpha = sc->rho / tot[0];
-  }
-}
-
 
 // host-side hooks implemented in krylov.hip
 extern bool g_prof;
