@@ -261,24 +261,45 @@ __global__ void pack_weights(const float* __restrict__ w1, const float* __restri
   out[e] = v;
 }
 
-struct NodeIjk {
-  int i, j, k;
+// Tiles: a workgroup computes TI x TJ x TK output nodes (8 k-lines of 16, two per
+// wavefront) from its (TI+2) x (TJ+2) x (TK+2) input halo staged in LDS, zero
+// outside the grid, so the 27 taps of a lane are LDS reads without bounds checks.
+// Each input node is read from global memory about 1.7 times instead of 27.
+constexpr int TI = 2, TJ = 4, TK = 16;
+constexpr int HI = TI + 2, HJ = TJ + 2, HK = TK + 2, HN = HI * HJ * HK;  // 432 halo nodes
+constexpr int kTileGridMax = 2048;
+
+struct TileGeom {
+  int nbj, nbk;
+  int64_t ntiles;
 };
 
-__device__ __forceinline__ NodeIjk node_ijk(const CnnGrid& g, int64_t n) {
-  const int64_t plane = (int64_t)g.ny * g.nz;
-  NodeIjk r;
-  r.i = (int)(n / plane);
-  r.j = (int)((n / g.nz) % g.ny);
-  r.k = (int)(n % g.nz);
-  return r;
+__host__ __device__ inline TileGeom tile_geom(const CnnGrid& g) {
+  TileGeom t;
+  const int nbi = (g.nx + TI - 1) / TI;
+  t.nbj = (g.ny + TJ - 1) / TJ;
+  t.nbk = (g.nz + TK - 1) / TK;
+  t.ntiles = (int64_t)nbi * t.nbj * t.nbk;
+  return t;
 }
 
-// neighbour of (i,j,k) at tap, or -1 outside the grid
-__device__ __forceinline__ int64_t tap_nbr(const CnnGrid& g, const NodeIjk& c, int tap) {
-  const int ii = c.i + tap / 9 - 1, jj = c.j + (tap / 3) % 3 - 1, kk = c.k + tap % 3 - 1;
+__device__ __forceinline__ void tile_origin(const TileGeom& tg, int64_t t, int& i0, int& j0, int& k0) {
+  k0 = (int)(t % tg.nbk) * TK;
+  j0 = (int)((t / tg.nbk) % tg.nbj) * TJ;
+  i0 = (int)(t / ((int64_t)tg.nbk * tg.nbj)) * TI;
+}
+
+// halo node h -> global node index, or -1 outside the grid
+__device__ __forceinline__ int64_t halo_node(const CnnGrid& g, int i0, int j0, int k0, int h) {
+  const int hk = h % HK, hj = (h / HK) % HJ, hi = h / (HK * HJ);
+  const int ii = i0 - 1 + hi, jj = j0 - 1 + hj, kk = k0 - 1 + hk;
   if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) return -1;
   return ((int64_t)ii * g.ny + jj) * g.nz + kk;
+}
+
+// halo index of the tap neighbour of output node (li, lj, kk) of the tile
+__device__ __forceinline__ int halo_tap(int li, int lj, int kk, int tap) {
+  return ((li + 1 + tap / 9 - 1) * HJ + (lj + 1 + (tap / 3) % 3 - 1)) * HK + (kk + 1 + tap % 3 - 1);
 }
 
 __device__ __forceinline__ f4 bias4(const float* __restrict__ b, int grp, int nvalid) {
@@ -288,122 +309,146 @@ __device__ __forceinline__ f4 bias4(const float* __restrict__ b, int grp, int nv
   return a;
 }
 
-// f[n] = (r[3n+i] sqrt(dinv[3n+i]) / s, i<3 ; gfield[n]) in f16
-__global__ void lpc_features_h(const float* __restrict__ r, const float* __restrict__ dinv,
-                               const float* __restrict__ gfield, int64_t nnode, const float* __restrict__ scale,
-                               h4* __restrict__ f, const int32_t* skip) {
+// Layer 1 (4 -> 16, tanh) with the input features formed while staging the halo:
+// f = (r_i sqrt(dinv_i) / s, i < 3; gfield).  Four taps share one K = 16 step.
+__global__ void __launch_bounds__(256) conv1_tile(const float* __restrict__ r, const float* __restrict__ dinv,
+                                                  const float* __restrict__ gfield, const float* __restrict__ scale,
+                                                  PackedNet net, CnnGrid g, h4* __restrict__ h1,
+                                                  const int32_t* skip) {
+  __shared__ h4 sf[HN];
   if (skip && *skip) return;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nnode) return;
-  const float inv_s = 1.0f / scale[0];
-  h4 v;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) v[i] = (_Float16)(r[n * 3 + i] * sqrtf(dinv[n * 3 + i]) * inv_s);
-  v[3] = (_Float16)gfield[n];
-  f[n] = v;
-}
-
-// h1 = tanh(conv1(f)): 4 -> 16 channels
-__global__ void __launch_bounds__(256) conv1_mfma(const h4* __restrict__ f, PackedNet net, CnnGrid g,
-                                                  h4* __restrict__ h1, const int32_t* skip) {
-  if (skip && *skip) return;
-  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kk = lane & 15, grp = lane >> 4;
   h4 wa[7];
 #pragma unroll
   for (int t = 0; t < 7; ++t) wa[t] = net.w1[t * 64 + lane];
   const f4 b0 = bias4(net.b1, grp, 16);
-  const int64_t nn = (int64_t)g.nx * g.ny * g.nz, ntiles = (nn + 15) / 16;
-  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const float inv_s = 1.0f / scale[0];
+  const TileGeom tg = tile_geom(g);
   const h4 zero = {0, 0, 0, 0};
-  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
-    const int64_t n = tile * 16 + col;
-    const bool valid = n < nn;
-    const NodeIjk c = node_ijk(g, valid ? n : 0);
-    f4 acc = b0;
-    h4 bv[7];
+  for (int64_t tile = blockIdx.x; tile < tg.ntiles; tile += gridDim.x) {
+    int i0, j0, k0;
+    tile_origin(tg, tile, i0, j0, k0);
+    for (int h = threadIdx.x; h < HN; h += blockDim.x) {
+      const int64_t m = halo_node(g, i0, j0, k0, h);
+      h4 v = zero;
+      if (m >= 0) {
 #pragma unroll
-    for (int t = 0; t < 7; ++t) {
-      const int tap = 4 * t + grp;
-      const int64_t m = (valid && tap < 27) ? tap_nbr(g, c, tap) : -1;
-      bv[t] = m >= 0 ? f[m] : zero;
+        for (int i = 0; i < 3; ++i) v[i] = (_Float16)(r[m * 3 + i] * sqrtf(dinv[m * 3 + i]) * inv_s);
+        v[3] = (_Float16)gfield[m];
+      }
+      sf[h] = v;
     }
+    __syncthreads();
 #pragma unroll
-    for (int t = 0; t < 7; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wa[t], bv[t], acc, 0, 0, 0);
-    if (valid) {
-      h4 o;
+    for (int x = 0; x < 2; ++x) {
+      const int line = wv * 2 + x, li = line / TJ, lj = line % TJ;
+      f4 acc = b0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) o[j] = (_Float16)tanhf(acc[j]);
-      h1[n * 4 + grp] = o;
+      for (int t = 0; t < 7; ++t) {
+        const int tap = 4 * t + grp;
+        const h4 bv = tap < 27 ? sf[halo_tap(li, lj, kk, tap)] : zero;
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wa[t], bv, acc, 0, 0, 0);
+      }
+      const int ii = i0 + li, jj = j0 + lj, kn = k0 + kk;
+      if (ii < g.nx && jj < g.ny && kn < g.nz) {
+        const int64_t n = ((int64_t)ii * g.ny + jj) * g.nz + kn;
+        h4 o;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[j] = (_Float16)tanhf(acc[j]);
+        h1[n * 4 + grp] = o;
+      }
     }
+    __syncthreads();  // the halo is restaged for the next tile
   }
 }
 
-// 27-tap 16-channel accumulation for one tile: acc += sum_tap W_tap^T in[nbr]
-// All 27 gathers are issued before the MFMA chain so their latencies overlap.
-__device__ __forceinline__ f4 conv16_tile(const h4 (&wa)[27], const h4* __restrict__ in, const CnnGrid& g,
-                                          const NodeIjk& c, bool valid, int grp, f4 acc) {
+// Stage the 16-channel halo of a tile (4 h4 per node).
+__device__ __forceinline__ void stage16(const h4* __restrict__ in, const CnnGrid& g, int i0, int j0, int k0,
+                                        h4* __restrict__ sh) {
   const h4 zero = {0, 0, 0, 0};
+  for (int e = threadIdx.x; e < HN * 4; e += blockDim.x) {
+    const int64_t m = halo_node(g, i0, j0, k0, e >> 2);
+    sh[e] = m >= 0 ? in[m * 4 + (e & 3)] : zero;
+  }
+}
+
+// 27-tap 16-channel MFMA chain for output node (li, lj, kk) of the staged tile.
+__device__ __forceinline__ f4 conv16_lds(const h4 (&wa)[27], const h4* __restrict__ sh, int li, int lj, int kk,
+                                         int grp, f4 acc) {
   h4 bv[27];
 #pragma unroll
-  for (int t = 0; t < 27; ++t) {
-    const int64_t m = valid ? tap_nbr(g, c, t) : -1;
-    bv[t] = m >= 0 ? in[m * 4 + grp] : zero;
-  }
+  for (int t = 0; t < 27; ++t) bv[t] = sh[halo_tap(li, lj, kk, t) * 4 + grp];
 #pragma unroll
   for (int t = 0; t < 27; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wa[t], bv[t], acc, 0, 0, 0);
   return acc;
 }
 
-// h2 = tanh(conv2(h1)): 16 -> 16 channels
-__global__ void __launch_bounds__(256) conv2_mfma(const h4* __restrict__ h1, PackedNet net, CnnGrid g,
+// Layer 2 (16 -> 16, tanh)
+__global__ void __launch_bounds__(256) conv2_tile(const h4* __restrict__ h1, PackedNet net, CnnGrid g,
                                                   h4* __restrict__ h2, const int32_t* skip) {
+  __shared__ h4 sh[HN * 4];
   if (skip && *skip) return;
-  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kk = lane & 15, grp = lane >> 4;
   h4 wa[27];
 #pragma unroll
   for (int t = 0; t < 27; ++t) wa[t] = net.w2[t * 64 + lane];
   const f4 b0 = bias4(net.b2, grp, 16);
-  const int64_t nn = (int64_t)g.nx * g.ny * g.nz, ntiles = (nn + 15) / 16;
-  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
-    const int64_t n = tile * 16 + col;
-    const bool valid = n < nn;
-    const NodeIjk c = node_ijk(g, valid ? n : 0);
-    const f4 acc = conv16_tile(wa, h1, g, c, valid, grp, b0);
-    if (valid) {
-      h4 o;
+  const TileGeom tg = tile_geom(g);
+  for (int64_t tile = blockIdx.x; tile < tg.ntiles; tile += gridDim.x) {
+    int i0, j0, k0;
+    tile_origin(tg, tile, i0, j0, k0);
+    stage16(h1, g, i0, j0, k0, sh);
+    __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 4; ++j) o[j] = (_Float16)tanhf(acc[j]);
-      h2[n * 4 + grp] = o;
+    for (int x = 0; x < 2; ++x) {
+      const int line = wv * 2 + x, li = line / TJ, lj = line % TJ;
+      const f4 acc = conv16_lds(wa, sh, li, lj, kk, grp, b0);
+      const int ii = i0 + li, jj = j0 + lj, kn = k0 + kk;
+      if (ii < g.nx && jj < g.ny && kn < g.nz) {
+        const int64_t n = ((int64_t)ii * g.ny + jj) * g.nz + kn;
+        h4 o;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[j] = (_Float16)tanhf(acc[j]);
+        h2[n * 4 + grp] = o;
+      }
     }
+    __syncthreads();
   }
 }
 
-// z = D^-1 r + s D^-1/2 mask conv3(h2)  (the preconditioner applied once)
-__global__ void __launch_bounds__(256) conv3_apply_mfma(const h4* __restrict__ h2, PackedNet net, CnnGrid g,
+// Layer 3 (16 -> 3) and z = D^-1 r + s D^-1/2 mask y (the preconditioner applied once)
+__global__ void __launch_bounds__(256) conv3_apply_tile(const h4* __restrict__ h2, PackedNet net, CnnGrid g,
                                                         const float* __restrict__ r, const float* __restrict__ dinv,
                                                         const float* __restrict__ mask,
                                                         const float* __restrict__ scale, float* __restrict__ z) {
-  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  __shared__ h4 sh[HN * 4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kk = lane & 15, grp = lane >> 4;
   h4 wa[27];
 #pragma unroll
   for (int t = 0; t < 27; ++t) wa[t] = net.w3[t * 64 + lane];
   const f4 b0 = bias4(net.b3, grp, 3);
   const float s = scale[0];
-  const int64_t nn = (int64_t)g.nx * g.ny * g.nz, ntiles = (nn + 15) / 16;
-  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
-    const int64_t n = tile * 16 + col;
-    const bool valid = n < nn;
-    const NodeIjk c = node_ijk(g, valid ? n : 0);
-    const f4 acc = conv16_tile(wa, h2, g, c, valid, grp, b0);
-    if (valid && grp == 0) {
+  const TileGeom tg = tile_geom(g);
+  for (int64_t tile = blockIdx.x; tile < tg.ntiles; tile += gridDim.x) {
+    int i0, j0, k0;
+    tile_origin(tg, tile, i0, j0, k0);
+    stage16(h2, g, i0, j0, k0, sh);
+    __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const int64_t d = n * 3 + i;
-        z[d] = dinv[d] * r[d] + s * sqrtf(dinv[d]) * mask[d] * acc[i];
+    for (int x = 0; x < 2; ++x) {
+      const int line = wv * 2 + x, li = line / TJ, lj = line % TJ;
+      const f4 acc = conv16_lds(wa, sh, li, lj, kk, grp, b0);
+      const int ii = i0 + li, jj = j0 + lj, kn = k0 + kk;
+      if (grp == 0 && ii < g.nx && jj < g.ny && kn < g.nz) {
+        const int64_t n = ((int64_t)ii * g.ny + jj) * g.nz + kn;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const int64_t d = n * 3 + i;
+          z[d] = dinv[d] * r[d] + s * sqrtf(dinv[d]) * mask[d] * acc[i];
+        }
       }
     }
+    __syncthreads();
   }
 }
 
@@ -423,11 +468,9 @@ __global__ void __launch_bounds__(kScalarBlock) rms_scalar(const double* partial
   if (threadIdx.x == 0) scale[0] = (float)sqrt(fmax(tot[0], 1e-300) / (double)N);
 }
 
-static inline unsigned tile_grid(int64_t nnode) {
-  const int64_t ntiles = (nnode + 15) / 16;
-  int64_t g = (ntiles + 7) / 8;  // ~2 tiles per wave, 4 waves per workgroup
-  if (g < 1) g = 1;
-  return (unsigned)(g < 2048 ? g : 2048);
+static inline unsigned tile_grid(const CnnGrid& g) {
+  const int64_t nt = tile_geom(g).ntiles;
+  return (unsigned)(nt < kTileGridMax ? nt : kTileGridMax);
 }
 
 // ---- flexible CG with the learned preconditioner ----------------------------
@@ -529,18 +572,19 @@ __global__ void __launch_bounds__(kScalarBlock) fcg_verify_scalars(Scalars* sc, 
   *scale = fs->s;
 }
 
-// conv3 (MFMA) fused with the output scaling: c = s sqrt(dinv) mask y (the
-// network's correction to the Jacobi step).  Partials: r.c, then per kept
-// direction j (D^-1 r).q_j and c.q_j.  Grid-stride over tiles so the partials
-// stay bounded.
+// conv3 (MFMA, LDS-tiled) fused with the output scaling: c = s sqrt(dinv) mask y
+// (the network's correction to the Jacobi step).  Partials: r.c, then per kept
+// direction j (D^-1 r).q_j and c.q_j.  The grid loops over tiles (at most
+// kTileGridMax workgroups), so the partials stay bounded.
 template <int M>
 __global__ void __launch_bounds__(256) fcg_conv3_out(const h4* __restrict__ h2, PackedNet net, CnnGrid g,
                                                      const float* __restrict__ r, const float* __restrict__ dinv,
                                                      const float* __restrict__ mask, const FcgScalars* fs,
                                                      const Scalars* sc, Ring Q, float* __restrict__ c,
                                                      double* partials) {
+  __shared__ h4 sh[HN * 4];
   if (sc->done) return;
-  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kk = lane & 15, grp = lane >> 4;
   h4 wa[27];
 #pragma unroll
   for (int t = 0; t < 27; ++t) wa[t] = net.w3[t * 64 + lane];
@@ -548,34 +592,41 @@ __global__ void __launch_bounds__(256) fcg_conv3_out(const h4* __restrict__ h2, 
   const int k = sc->iter - fs->base;
   const int nq = k < M ? k : M;
   const float s = fs->s;
-  const int64_t nn = (int64_t)g.nx * g.ny * g.nz, ntiles = (nn + 15) / 16;
-  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
   double loc[1 + 2 * M];
 #pragma unroll
   for (int j = 0; j < 1 + 2 * M; ++j) loc[j] = 0.0;
-  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
-    const int64_t n = tile * 16 + col;
-    const bool valid = n < nn;
-    const NodeIjk cc = node_ijk(g, valid ? n : 0);
-    const f4 acc = conv16_tile(wa, h2, g, cc, valid, grp, b0);
-    if (valid && grp == 0) {
+  const TileGeom tg = tile_geom(g);
+  for (int64_t tile = blockIdx.x; tile < tg.ntiles; tile += gridDim.x) {
+    int i0, j0, k0;
+    tile_origin(tg, tile, i0, j0, k0);
+    stage16(h2, g, i0, j0, k0, sh);
+    __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const int64_t d = n * 3 + i;
-        const float cn = s * sqrtf(dinv[d]) * mask[d] * acc[i];
-        const float jr = dinv[d] * r[d];
-        c[d] = cn;
-        loc[0] += (double)r[d] * (double)cn;
+    for (int x = 0; x < 2; ++x) {
+      const int line = wv * 2 + x, li = line / TJ, lj = line % TJ;
+      const f4 acc = conv16_lds(wa, sh, li, lj, kk, grp, b0);
+      const int ii = i0 + li, jj = j0 + lj, kn = k0 + kk;
+      if (grp == 0 && ii < g.nx && jj < g.ny && kn < g.nz) {
+        const int64_t n = ((int64_t)ii * g.ny + jj) * g.nz + kn;
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-          if (j < nq) {
-            const double qj = Q.v[j][d];
-            loc[1 + 2 * j] += (double)jr * qj;
-            loc[2 + 2 * j] += (double)cn * qj;
+        for (int i = 0; i < 3; ++i) {
+          const int64_t d = n * 3 + i;
+          const float cn = s * sqrtf(dinv[d]) * mask[d] * acc[i];
+          const float jr = dinv[d] * r[d];
+          c[d] = cn;
+          loc[0] += (double)r[d] * (double)cn;
+#pragma unroll
+          for (int j = 0; j < M; ++j) {
+            if (j < nq) {
+              const double qj = Q.v[j][d];
+              loc[1 + 2 * j] += (double)jr * qj;
+              loc[2 + 2 * j] += (double)cn * qj;
+            }
           }
         }
       }
     }
+    __syncthreads();
   }
   block_partials<1 + 2 * M>(loc, partials);
 }
@@ -840,11 +891,11 @@ static void fcg_batch(int poll, const MatArgs& m, const float* vals, const float
                       float* scale, double* partials, int stall_window, hipStream_t s) {
   const int64_t nnode = m.nrows, N = nnode * 3;
   const int gr = rows_grid(m.nslices), ge = elem_grid(N);
-  const unsigned gn = (unsigned)ceil_div(nnode, 256), gt = tile_grid(nnode);
+  const unsigned gt = tile_grid(g);
+  (void)f;
   for (int k = 0; k < poll; ++k) {
-    lpc_features_h<<<gn, 256, 0, s>>>(r, dinv, gfield, nnode, scale, f, &sc->done);
-    conv1_mfma<<<gt, 256, 0, s>>>(f, net, g, h1, &sc->done);
-    conv2_mfma<<<gt, 256, 0, s>>>(h1, net, g, h2, &sc->done);
+    conv1_tile<<<gt, 256, 0, s>>>(r, dinv, gfield, scale, net, g, h1, &sc->done);
+    conv2_tile<<<gt, 256, 0, s>>>(h1, net, g, h2, &sc->done);
     fcg_conv3_out<M><<<gt, 256, 0, s>>>(h2, net, g, r, dinv, mask, fs, sc, Q, c, partials);
     fcg_beta_scalars<M><<<1, kScalarBlock, 0, s>>>(sc, fs, partials, (int)gt);
     fcg_form_p<M><<<ge, kBlock, 0, s>>>(N, r, dinv, c, P, sc, fs);
@@ -900,11 +951,11 @@ extern "C" int hf_lpc_apply_f32(const float* r, const float* dinv, const float* 
   const int ge = elem_grid(N);
   rdr_partials<<<ge, kBlock, 0, s>>>(r, dinv, N, partials);
   rms_scalar<<<1, kScalarBlock, 0, s>>>(partials, ge, N, scale);
-  const unsigned gn = (unsigned)ceil_div(nnode, 256), gt = tile_grid(nnode);
-  lpc_features_h<<<gn, 256, 0, s>>>(r, dinv, gfield, nnode, scale, f, nullptr);
-  conv1_mfma<<<gt, 256, 0, s>>>(f, net, g, h1, nullptr);
-  conv2_mfma<<<gt, 256, 0, s>>>(h1, net, g, h2, nullptr);
-  conv3_apply_mfma<<<gt, 256, 0, s>>>(h2, net, g, r, dinv, mask, scale, z);
+  const unsigned gt = tile_grid(g);
+  (void)f;
+  conv1_tile<<<gt, 256, 0, s>>>(r, dinv, gfield, scale, net, g, h1, nullptr);
+  conv2_tile<<<gt, 256, 0, s>>>(h1, net, g, h2, nullptr);
+  conv3_apply_tile<<<gt, 256, 0, s>>>(h2, net, g, r, dinv, mask, scale, z);
   HF_LAUNCH_CHECK();
   return 0;
 }
@@ -926,7 +977,7 @@ extern "C" int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const fl
   const MatArgs m = mat_args(p);
   HF_REQUIRE((int64_t)nx * ny * nz == m.nrows, "hf_lpc_solve_f32: grid %dx%dx%d != %lld nodes", nx, ny, nz,
              (long long)m.nrows);
-  static_assert(2048 * (1 + 2 * kMaxTrunc) <= kMaxGrid * 4, "fcg_conv3_out partials exceed the header");
+  static_assert(kTileGridMax * (1 + 2 * kMaxTrunc) <= kMaxGrid * 4, "fcg_conv3_out partials exceed the header");
   hipStream_t s = (hipStream_t)stream;
   const int64_t nnode = m.nrows, N = nnode * 3;
   if (poll < 1) poll = 1;
