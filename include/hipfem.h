@@ -194,8 +194,9 @@ int hf_lpc_apply_f32(const float* r, const float* dinv, const float* gfield, con
  * 1..4) with the learned preconditioner (weights: w1,b1,w2,b2,w3,b3 device
  * pointers); the network correction is scaled down where needed so that
  * r.z >= 0.1 r.D^-1.r.  status 0 converged, 1 max iterations, 2 p.Ap <= 0,
- * 3 non-finite, 4 stagnation (no 10 % drop of ||r|| in stall_window
- * iterations); on 2-4 the caller continues with Jacobi-PCG. */
+ * 3 non-finite, 4 stagnation (after ||r|| first fell 10 % below its start: no new
+ * 10 % best in max(stall_window, iterations / 2) iterations; before: 10 windows);
+ * on 2-4 the caller continues with Jacobi-PCG. */
 size_t hf_lpc_workspace_size(int64_t nnode);
 int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const float* dinv, const float* b, float* x,
                      const float* const* weights, const float* gfield, const float* mask, int32_t nx,

@@ -492,6 +492,7 @@ struct FcgScalars {
   double pq[kMaxTrunc];    // p_j.A p_j of the kept directions (ring slots)
   double beta[kMaxTrunc];  // this step's A-orthogonalisation coefficients
   int32_t base;            // iteration of the last (re)start: directions before it are dropped
+  int32_t improved;        // ||r|| has dropped below 0.9 ||r_start|| since the last (re)start
 };
 
 // z = D^-1 r + theta c keeps r.z >= kPosFloor r.D^-1.r (theta in [0, 1]).
@@ -542,6 +543,7 @@ __global__ void __launch_bounds__(kScalarBlock) fcg_init_scalars(Scalars* sc, Fc
   else if (maxiter <= 0 && !sc->done) { sc->done = 1; sc->status = 1; }
   fs->s = (float)sqrt(fmax(tot[2], 1e-300) / (double)N);
   fs->stall = 0;
+  fs->improved = 0;
   fs->best_rr = tot[0];
   fs->rdr = tot[2];
   fs->theta = 1.0;
@@ -568,6 +570,7 @@ __global__ void __launch_bounds__(kScalarBlock) fcg_verify_scalars(Scalars* sc, 
   fs->rdr = tot[2];
   fs->s = (float)sqrt(fmax(tot[2], 1e-300) / (double)N);
   fs->stall = 0;
+  fs->improved = 0;
   fs->best_rr = tot[0];
   *scale = fs->s;
 }
@@ -767,12 +770,21 @@ __global__ void __launch_bounds__(kScalarBlock) fcg_end_scalars(Scalars* sc, Fcg
   if (tot[0] < 0.81 * fs->best_rr) {  // progress: ||r|| down by 10 %
     fs->best_rr = tot[0];
     fs->stall = 0;
+    fs->improved = 1;
   } else {
     fs->stall += 1;
   }
   if (!isfinite(tot[0])) { sc->done = 1; sc->status = 3; }
   else if (tot[0] <= sc->thresh) { sc->done = 1; sc->status = 0; }
-  else if (fs->stall >= stall_window) { sc->done = 1; sc->status = 4; }
+  // stagnation: once ||r|| has dropped 10 % below its start, no new best for
+  // max(stall_window, iter / 2) iterations; before that, 10 stall windows.  The
+  // residual of CG-type methods can grow for a long stretch while a localised load
+  // spreads through the body.
+  else if (fs->improved ? fs->stall >= max(stall_window, (sc->iter - fs->base) / 2)
+                        : (int64_t)(sc->iter - fs->base) >= (int64_t)10 * stall_window) {
+    sc->done = 1;
+    sc->status = 4;
+  }
   else if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; }
 }
 

@@ -22,8 +22,9 @@ solve applies the network on MFMA with f16 activations (hf_lpc_apply_f32 is
 that path for one residual).
 
 The shipped weights (weights/lpc_elastic.npz) come from
-    python jax-fem_amd/train_lpc.py --instances 2000 --epochs 1
-(32^3 heterogeneous elasticity, seeds 1..2000, 2000 Adam steps).
+    python jax-fem_amd/train_lpc.py --instances 2000 --epochs 1 --domains 1.0,0.25
+(32^3 heterogeneous elasticity, seeds 1..2000, fields at two grid spacings,
+2000 Adam steps, 73 s on one MI355X).
 """
 import ctypes
 import math
@@ -287,13 +288,50 @@ def elasticity_instance(jfa, n, E_cell, Lz=1.0):
     return prob
 
 
-def residual_samples(A, problem, k_list, seed):
-    """Jacobi-PCG residuals r_k = b - A x_k for a random right-hand side (zero on
-    Dirichlet rows), normalised to r^T D^-1 r = 1.  Returns (len(k_list), N) float32."""
+RHS_KINDS = ("noise", "face", "smooth")
+
+
+def random_rhs(A, problem, gen, kind):
+    """A right-hand side (zero on Dirichlet rows) of one of three kinds:
+    'noise'  white noise on every free DOF;
+    'face'   a constant load of random direction on the nodes of one random box face
+             (the shape of a traction load: large in one layer, zero elsewhere);
+    'smooth' A y for a smooth random displacement y (a few low Fourier modes)."""
+    mask = LearnedPC.mask(problem)
+    dev = A.vals.device
+    N = A.shape[0]
+    if kind == "noise":
+        return torch.randn(N, generator=gen, device=dev, dtype=torch.float32) * mask
+    nx, ny, nz = LearnedPC.grid(problem)
+    idx = torch.arange(N // 3, device=dev)
+    i, j, k = idx // (ny * nz), (idx // nz) % ny, idx % nz
+    if kind == "face":
+        face = int(torch.randint(0, 6, (1,), generator=gen, device=dev))
+        coord, ext = [(i, nx), (j, ny), (k, nz)][face // 2]
+        sel = (coord == (0 if face % 2 == 0 else ext - 1)).float()
+        d = torch.randn(3, generator=gen, device=dev)
+        b = (sel[:, None] * d[None, :]).reshape(-1) * mask
+        if not bool((b != 0).any()):   # the face is clamped: no load can act there
+            return torch.randn(N, generator=gen, device=dev, dtype=torch.float32) * mask
+        return b
+    y = torch.zeros(N // 3, 3, device=dev)
+    for _ in range(4):
+        f = torch.rand(3, generator=gen, device=dev) * 3.0
+        ph = torch.rand(3, generator=gen, device=dev) * 6.283
+        amp = torch.randn(3, generator=gen, device=dev)
+        w = torch.sin(f[0] * 3.1416 * i / nx + ph[0]) * torch.sin(f[1] * 3.1416 * j / ny + ph[1]) * \
+            torch.sin(f[2] * 3.1416 * k / nz + ph[2])
+        y += w[:, None] * amp[None, :]
+    return A.matvec((y.reshape(-1) * mask).contiguous()) * mask
+
+
+def residual_samples(A, problem, k_list, seed, kind="noise"):
+    """Jacobi-PCG residuals r_k = b - A x_k for a random right-hand side of the given
+    kind (random_rhs), normalised to r^T D^-1 r = 1.  Returns (len(k_list), N) float32."""
     from .solver import krylov
     gen = torch.Generator(device=A.vals.device).manual_seed(int(seed))
     mask = LearnedPC.mask(problem)
-    b = torch.randn(A.shape[0], generator=gen, device=A.vals.device, dtype=torch.float32) * mask
+    b = random_rhs(A, problem, gen, kind)
     out = []
     for k in k_list:
         if k == 0:
@@ -303,16 +341,16 @@ def residual_samples(A, problem, k_list, seed):
             r = b - A.matvec(x)
         r = r * mask
         nrm = torch.sqrt((r.double() * r.double() * A.dinv().double()).sum()).float()
-        out.append(r / nrm)
+        out.append(r / torch.clamp(nrm, min=1e-30))
     return torch.stack(out)
 
 
-def fcg_residuals(model, A, problem, k_list, seed, gfield=None):
+def fcg_residuals(model, A, problem, k_list, seed, gfield=None, kind="noise"):
     """Residuals after k steps of the model's own flexible CG (no fallback) on a
-    random right-hand side, normalised like residual_samples."""
+    random right-hand side of the given kind, normalised like residual_samples."""
     gen = torch.Generator(device=A.vals.device).manual_seed(int(seed))
     mask = LearnedPC.mask(problem)
-    b = torch.randn(A.shape[0], generator=gen, device=A.vals.device, dtype=torch.float32) * mask
+    b = random_rhs(A, problem, gen, kind)
     grid = model.grid(problem)
     gfield = model.gfield(problem) if gfield is None else gfield
     N = A.shape[0]
@@ -330,13 +368,15 @@ def fcg_residuals(model, A, problem, k_list, seed, gfield=None):
         if not bool(torch.isfinite(r).all()):
             r = b.clone()
         nrm = torch.sqrt((r.double() * r.double() * A.dinv().double()).sum()).float()
-        out.append(r / nrm)
+        out.append(r / torch.clamp(nrm, min=1e-30))
     return torch.stack(out)
 
 
-def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None, log_every=50, self_frac=0.5):
+def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None, log_every=50, self_frac=0.5,
+          rhs_kinds=("noise",)):
     """self_frac: share of the steps whose residuals come from the model's own
-    flexible CG (fcg_residuals) instead of Jacobi-PCG (residual_samples)."""
+    flexible CG (fcg_residuals) instead of Jacobi-PCG (residual_samples);
+    rhs_kinds: right-hand-side kinds (random_rhs) drawn uniformly per step."""
     """Adam on the energy loss over (A, problem) pairs; returns the loss history.
 
     matrices: callable i -> (TangentMatrix FP32, problem[, key]) or a list of such
@@ -358,8 +398,9 @@ def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None,
         g, mask, grid = cache_fields[key]
         ks = sorted(int(k) for k in rng.integers(0, k_max + 1, size=batch))
         sd = rng.integers(1 << 31)
-        R = (fcg_residuals(model, A, prob, ks, sd, gfield=g) if rng.random() < self_frac else
-             residual_samples(A, prob, ks, seed=sd))
+        kind = rhs_kinds[int(rng.integers(len(rhs_kinds)))]
+        R = (fcg_residuals(model, A, prob, ks, sd, gfield=g, kind=kind) if rng.random() < self_frac else
+             residual_samples(A, prob, ks, seed=sd, kind=kind))
         z, cache = model.forward(A, R, g, mask, grid, keep=True)
         Az = torch.stack([A.matvec(zi) for zi in z])
         loss_per = 0.5 * (z * Az).sum(1) - (z * R).sum(1)
@@ -369,6 +410,11 @@ def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None,
         ej = 0.5 * (zj * Azj).sum(1) - (zj * R).sum(1)
         gz = (Az - R) / batch
         grads = model.backward(cache, gz)
+        if not (bool(torch.isfinite(loss_per).all()) and all(bool(torch.isfinite(gp).all()) for gp in grads)):
+            sched.step()   # a degenerate sample: skip the update, keep the schedule
+            if log:
+                log("step %d: non-finite loss or gradient, update skipped" % step)
+            continue
         for p, gp in zip(model.params(), grads):
             p.grad = gp
         opt.step()

@@ -6,7 +6,10 @@
 Training set: heterogeneous linear elasticity on a 32^3 HEX8 box, Young's
 modulus a log-normal Gaussian random field per cell (mean log 0, sigma 0.5,
 correlation length 0.1, clipped to [0.05, 20]), nu = 0.3, u = 0 on x = 0;
-instance i uses seed i (1..instances).  Each step takes one instance's FP32
+instance i uses seed i (1..instances).  --domains d1,d2,... samples the field
+on the box [0, d]^3 of instance i (d cycling over the list), i.e. at grid spacing
+d/32: d = 1 is the unit cube, d = 0.25 the spacing of a 128-cell-wide mesh of
+the unit cube (correlation length 12.8 cells instead of 3.2).  Each step takes one instance's FP32
 tangent, draws `batch` Jacobi-PCG residuals (iterations 0..k_max) of a random
 right-hand side and takes an Adam step on the energy loss (learned_pc.train).
 Held-out seeds (9001, ...) measure iterations of learned FCG vs Jacobi-PCG.
@@ -42,6 +45,9 @@ def main():
     ap.add_argument("--kmax", type=int, default=150)
     ap.add_argument("--eval", type=int, default=3)
     ap.add_argument("--init", default=None, help="start from these weights")
+    ap.add_argument("--domains", default="1.0", help="comma-separated field box sizes, cycled over instances")
+    ap.add_argument("--self-frac", type=float, default=0.5)
+    ap.add_argument("--rhs", default="noise", help="comma-separated right-hand-side kinds: noise, face, smooth")
     ap.add_argument("--out", default=os.path.join(HERE, "weights", "lpc_elastic.npz"))
     args = ap.parse_args()
 
@@ -53,7 +59,13 @@ def main():
 
     torch.cuda.init()
     n = args.n
-    E0 = L.heterogeneous_modulus((n, n, n), 1)
+    domains = [float(x) for x in args.domains.split(",")]
+
+    def field(seed):
+        d = domains[(seed - 1) % len(domains)]
+        return L.heterogeneous_modulus((n, n, n), seed, domain=(d, d, d))
+
+    E0 = field(1)
     prob = L.elasticity_instance(jfa, n, E0)
     u0 = torch.zeros(prob.N, dtype=torch.float64, device="cuda")
     cache = {}
@@ -63,7 +75,7 @@ def main():
         if seed not in cache:
             if len(cache) > 256:
                 cache.pop(next(iter(cache)))
-            E = L.heterogeneous_modulus((n, n, n), seed)
+            E = field(seed)
             prob.internal_vars = [np.repeat(E[:, None], 8, axis=1)]
             cache[seed] = (TangentMatrix(prob, prob.jacobian_bsr(u0, torch.float32)), E)
         A, E = cache[seed]
@@ -74,24 +86,25 @@ def main():
     steps = int(args.epochs * args.instances)
     t0 = time.perf_counter()
     hist = L.train(model, matrix, steps, batch=args.batch, lr=args.lr, k_max=args.kmax,
-                   log=lambda s: print("[train]", s, flush=True), log_every=max(1, steps // 40))
+                   log=lambda s: print("[train]", s, flush=True), log_every=max(1, steps // 40),
+                   self_frac=args.self_frac, rhs_kinds=tuple(args.rhs.split(",")))
     t_train = time.perf_counter() - t0
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     model.save(args.out)
 
     # held-out evaluation
     res = []
-    for k in range(args.eval):
+    for k in range(args.eval * len(domains)):
         seed = 9001 + k
-        E = L.heterogeneous_modulus((n, n, n), seed)
+        E = field(seed)
         prob.internal_vars = [np.repeat(E[:, None], 8, axis=1)]
         A = TangentMatrix(prob, prob.jacobian_bsr(u0, torch.float32))
         b = torch.randn(prob.N, device="cuda", generator=torch.Generator(device="cuda").manual_seed(seed))
         b = b * L.LearnedPC.mask(prob)
         _, ij = krylov(A, b, None, "pcg", 1e-6, 0.0, 20000)
         _, il, fb = model.solve(A, b, None, prob, rtol=1e-6, maxiter=20000)
-        res.append({"seed": seed, "jacobi_pcg": ij.iterations, "learned_fcg": il.iterations, "fallback": fb,
-                    "fcg_part": model.last_fcg})
+        res.append({"seed": seed, "domain": domains[(seed - 1) % len(domains)], "jacobi_pcg": ij.iterations,
+                    "learned_fcg": il.iterations, "fallback": fb, "fcg_part": model.last_fcg})
         print("[eval]", res[-1], flush=True)
     print(json.dumps({"train_seconds": t_train, "steps": steps, "final_loss": hist[-1], "eval": res}))
 
