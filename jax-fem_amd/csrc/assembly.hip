@@ -36,6 +36,8 @@ struct hf_pattern {
   int32_t* sell_tpos;  // nslots*R value base of the transposed block, -1 for padding
   int32_t* blkpos;     // nblocks value base of each block
   int32_t* dpos;       // nnode value base of the diagonal block (-1: none)
+  int32_t* sell_row;   // nslices*R node of each slice row (-1: padding row)
+  int32_t* sell_pos;   // nnode slice row of each node
 };
 
 namespace hf {
@@ -116,9 +118,13 @@ __global__ void csr_view(const int32_t* __restrict__ rowptr, const int32_t* __re
     for (int j = 0; j < vec; ++j) scol[start + (int64_t)(b - b0) * vec + j] = colidx[b] * vec + j;
 }
 
-// Matrix values: sliced ELL ("SELL").  Node rows are grouped in slices of R
-// consecutive rows (R = 21 for vec = 3, so a slice is 63 scalar rows = one
-// wavefront lane each; R = 64 for vec = 1).  Slice s has width W_s = its
+// Matrix values: sliced ELL ("SELL-C-sigma").  Node rows are grouped in slices of
+// R rows (R = 21 for vec = 3, so a slice is 63 scalar rows = one wavefront lane
+// each; R = 64 for vec = 1).  Within windows of 8 slices the rows are ordered by
+// decreasing length (stable), so rows of similar length share a slice and the
+// padding stays small when row lengths vary (the vertex / edge / face / interior
+// nodes of HEX27 meshes have 125 / 75 / 45 / 27 blocks); sell_row[s*R + r] is the
+// node of slice row r of slice s and sell_pos its inverse.  Slice s has width W_s = its
 // longest row (in blocks); its slots are slice_ptr[s] .. slice_ptr[s+1]-1.
 // Slot t holds, for each row r of the slice, the r-th row's j-th block
 // (j = t - slice_ptr[s]) or zero padding; the block's vec x vec entries are
@@ -130,48 +136,75 @@ __global__ void csr_view(const int32_t* __restrict__ rowptr, const int32_t* __re
 // one contiguous vec-word load (the L = R*vec lanes read L*vec consecutive
 // values), multiplies it by the single gathered entry x[col*vec + i] and keeps
 // vec partial outputs; the vec lanes of a row are summed once per slice.
-__global__ void slice_width(const int32_t* __restrict__ rowptr, int64_t nnode, int R, int64_t nslices,
-                            int32_t* __restrict__ W) {
+// keys ordering the rows of each window of `window` consecutive nodes by
+// decreasing length: (window index, ~length)
+__global__ void perm_keys(const int32_t* __restrict__ rowptr, int64_t nnode, int64_t window,
+                          uint64_t* __restrict__ keys, int32_t* __restrict__ ids) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  const uint32_t len = (uint32_t)min(rowptr[n + 1] - rowptr[n], (int32_t)0xFFFFF);
+  keys[n] = ((uint64_t)(n / window) << 20) | (uint64_t)(0xFFFFFu - len);
+  ids[n] = (int32_t)n;
+}
+
+__global__ void sell_row_fill(const int32_t* __restrict__ order, int64_t nnode, int64_t npos,
+                              int32_t* __restrict__ sell_row, int32_t* __restrict__ sell_pos) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npos) return;
+  if (q < nnode) {
+    const int32_t n = order[q];
+    sell_row[q] = n;
+    sell_pos[n] = (int32_t)q;
+  } else {
+    sell_row[q] = -1;
+  }
+}
+
+__global__ void slice_width(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ sell_row, int R,
+                            int64_t nslices, int32_t* __restrict__ W) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s > nslices) return;
   if (s == nslices) { W[s] = 0; return; }
   int32_t w = 0;
   for (int r = 0; r < R; ++r) {
-    const int64_t n = s * R + r;
-    if (n < nnode) w = max(w, rowptr[n + 1] - rowptr[n]);
+    const int32_t n = sell_row[s * R + r];
+    if (n >= 0) w = max(w, rowptr[n + 1] - rowptr[n]);
   }
   W[s] = w;
 }
 
 __global__ void sell_fill(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-                          const int32_t* __restrict__ slice_ptr, int64_t nnode, int R, int vec, int64_t nslices,
-                          int32_t* __restrict__ sell_col, int32_t* __restrict__ blkpos) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // row slot of its slice
-  if (n >= nslices * R) return;
-  const int64_t s = n / R;
-  const int r = (int)(n - s * R);
+                          const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ sell_row, int R,
+                          int vec, int64_t nslices, int32_t* __restrict__ sell_col, int32_t* __restrict__ blkpos) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // slice row
+  if (q >= nslices * R) return;
+  const int64_t s = q / R;
+  const int r = (int)(q - s * R);
+  const int32_t n = sell_row[q];
   const int32_t t0 = slice_ptr[s], t1 = slice_ptr[s + 1];
-  const int32_t b0 = n < nnode ? rowptr[n] : 0, len = n < nnode ? rowptr[n + 1] - b0 : 0;
+  const int32_t b0 = n >= 0 ? rowptr[n] : 0, len = n >= 0 ? rowptr[n + 1] - b0 : 0;
   for (int32_t t = t0; t < t1; ++t) {
     const int32_t j = t - t0;
     if (j < len) {
       sell_col[(int64_t)t * R + r] = colidx[b0 + j];
       blkpos[b0 + j] = (int32_t)(((int64_t)t * R + r) * vec * vec);
     } else {
-      sell_col[(int64_t)t * R + r] = n < nnode ? (int32_t)n : 0;
+      sell_col[(int64_t)t * R + r] = n >= 0 ? n : 0;   // padding: a zero block on an existing column
     }
   }
 }
 
 __global__ void sell_tpos_fill(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ trans,
                                const int32_t* __restrict__ blkpos, const int32_t* __restrict__ slice_ptr,
-                               int64_t nnode, int R, int64_t nslices, int32_t* __restrict__ sell_tpos) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nslices * R) return;
-  const int64_t s = n / R;
-  const int r = (int)(n - s * R);
+                               const int32_t* __restrict__ sell_row, int R, int64_t nslices,
+                               int32_t* __restrict__ sell_tpos) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nslices * R) return;
+  const int64_t s = q / R;
+  const int r = (int)(q - s * R);
+  const int32_t n = sell_row[q];
   const int32_t t0 = slice_ptr[s], t1 = slice_ptr[s + 1];
-  const int32_t b0 = n < nnode ? rowptr[n] : 0, len = n < nnode ? rowptr[n + 1] - b0 : 0;
+  const int32_t b0 = n >= 0 ? rowptr[n] : 0, len = n >= 0 ? rowptr[n + 1] - b0 : 0;
   for (int32_t t = t0; t < t1; ++t) {
     const int32_t j = t - t0;
     int32_t v = -1;
@@ -213,17 +246,18 @@ template <typename T, int VEC>
 __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ seg,
                                                        const int32_t* __restrict__ contrib,
-                                                       const int32_t* __restrict__ slice_ptr, int64_t nnode,
-                                                       int R, int64_t nslices, const T* __restrict__ elem,
-                                                       T* __restrict__ vals) {
+                                                       const int32_t* __restrict__ slice_ptr,
+                                                       const int32_t* __restrict__ sell_row, int R, int64_t nslices,
+                                                       const T* __restrict__ elem, T* __restrict__ vals) {
   constexpr int BB = VEC * VEC;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nslices * R) return;
-  const int64_t s = n / R;
-  const int r = (int)(n - s * R);
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // slice row
+  if (q >= nslices * R) return;
+  const int64_t s = q / R;
+  const int r = (int)(q - s * R);
   const int64_t L = (int64_t)R * VEC;
+  const int32_t n = sell_row[q];
   const int32_t t0 = slice_ptr[s], t1 = slice_ptr[s + 1];
-  const int32_t b0 = n < nnode ? rowptr[n] : 0, len = n < nnode ? rowptr[n + 1] - b0 : 0;
+  const int32_t b0 = n >= 0 ? rowptr[n] : 0, len = n >= 0 ? rowptr[n + 1] - b0 : 0;
   for (int32_t t = t0; t < t1; ++t) {
     const int32_t j = t - t0;
     T acc[BB];
@@ -271,14 +305,15 @@ __global__ void gather_kernel(const int32_t* __restrict__ diag, const int32_t* _
 
 template <typename T>
 __global__ void bc_rows_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ slice_ptr,
-                               const int32_t* __restrict__ dpos, const int32_t* __restrict__ dofs, int64_t nbc,
-                               int vec, int R, T* __restrict__ vals) {
+                               const int32_t* __restrict__ sell_pos, const int32_t* __restrict__ dpos,
+                               const int32_t* __restrict__ dofs, int64_t nbc, int vec, int R, T* __restrict__ vals) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbc) return;
   const int32_t d = dofs[t];
   const int32_t n = d / vec, i = d - (d / vec) * vec;
-  const int64_t s = n / R;
-  const int r = (int)(n - s * R);
+  const int32_t q = sell_pos[n];
+  const int64_t s = q / R;
+  const int r = (int)(q - s * R);
   const int32_t len = rowptr[n + 1] - rowptr[n];
   for (int32_t j = 0; j < len; ++j) {
     T* base = vals + ((int64_t)(slice_ptr[s] + j) * R + r) * vec * vec;
@@ -350,7 +385,7 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
     }                                                                                     \
   } while (0)
   p->rowptr = p->colidx = p->seg = p->contrib = p->diag = p->trans = p->brow = nullptr;
-  p->slice_ptr = p->sell_col = p->sell_tpos = p->blkpos = p->dpos = nullptr;
+  p->slice_ptr = p->sell_col = p->sell_tpos = p->blkpos = p->dpos = p->sell_row = p->sell_pos = nullptr;
   p->R = (vec == 3) ? 21 : 64;
   p->nslices = (nnode + p->R - 1) / p->R;
   PB_CHECK(hipMalloc(&keys, n * sizeof(uint64_t)));
@@ -409,12 +444,41 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
     PB_CHECK(hipMemcpyAsync(&p->max_row_blocks, mx, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
   }
   {
-    // sliced-ELL layout
+    // sliced-ELL layout: rows ordered by decreasing length within windows of 8 slices
     const int64_t ns = p->nslices;
+    {
+      const int64_t window = (int64_t)p->R * 8;
+      uint64_t *pk = nullptr, *pk_s = nullptr;
+      int32_t *pid = nullptr, *order = nullptr;
+      PB_CHECK(hipMalloc(&pk, nnode * sizeof(uint64_t)));
+      PB_CHECK(hipMalloc(&pk_s, nnode * sizeof(uint64_t)));
+      PB_CHECK(hipMalloc(&pid, nnode * sizeof(int32_t)));
+      PB_CHECK(hipMalloc(&order, nnode * sizeof(int32_t)));
+      perm_keys<<<grid_for(nnode), 256, 0, stream>>>(p->rowptr, nnode, window, pk, pid);
+      PB_CHECK(hipGetLastError());
+      int pbits = 20;
+      while (pbits < 63 && ((uint64_t)1 << (pbits - 20)) <= (uint64_t)(nnode / window)) ++pbits;
+      size_t need = 0;
+      PB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, pk, pk_s, pid, order, (int)nnode, 0, pbits,
+                                                  stream));
+      if (need > tmp_bytes) {
+        hipFree(tmp); tmp = nullptr;
+        tmp_bytes = need;
+        PB_CHECK(hipMalloc(&tmp, tmp_bytes));
+      }
+      PB_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, pk, pk_s, pid, order, (int)nnode, 0, pbits,
+                                                  stream));
+      PB_CHECK(hipMalloc(&p->sell_row, ns * p->R * sizeof(int32_t)));
+      PB_CHECK(hipMalloc(&p->sell_pos, nnode * sizeof(int32_t)));
+      sell_row_fill<<<grid_for(ns * p->R), 256, 0, stream>>>(order, nnode, ns * p->R, p->sell_row, p->sell_pos);
+      PB_CHECK(hipGetLastError());
+      PB_CHECK(hipStreamSynchronize(stream));
+      hipFree(pk); hipFree(pk_s); hipFree(pid); hipFree(order);
+    }
     int32_t* W = nullptr;
     PB_CHECK(hipMalloc(&W, (ns + 1) * sizeof(int32_t)));
     PB_CHECK(hipMalloc(&p->slice_ptr, (ns + 1) * sizeof(int32_t)));
-    slice_width<<<grid_for(ns + 1), 256, 0, stream>>>(p->rowptr, nnode, p->R, ns, W);
+    slice_width<<<grid_for(ns + 1), 256, 0, stream>>>(p->rowptr, p->sell_row, p->R, ns, W);
     PB_CHECK(hipGetLastError());
     size_t need = 0;
     PB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, W, p->slice_ptr, (int)(ns + 1), stream));
@@ -435,11 +499,11 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
     PB_CHECK(hipMalloc(&p->sell_tpos, cells_sell * sizeof(int32_t)));
     PB_CHECK(hipMalloc(&p->blkpos, nb * sizeof(int32_t)));
     PB_CHECK(hipMalloc(&p->dpos, nnode * sizeof(int32_t)));
-    sell_fill<<<grid_for(ns * p->R), 256, 0, stream>>>(p->rowptr, p->colidx, p->slice_ptr, nnode, p->R, vec, ns,
-                                                      p->sell_col, p->blkpos);
+    sell_fill<<<grid_for(ns * p->R), 256, 0, stream>>>(p->rowptr, p->colidx, p->slice_ptr, p->sell_row, p->R, vec,
+                                                      ns, p->sell_col, p->blkpos);
     PB_CHECK(hipGetLastError());
-    sell_tpos_fill<<<grid_for(ns * p->R), 256, 0, stream>>>(p->rowptr, p->trans, p->blkpos, p->slice_ptr, nnode,
-                                                           p->R, ns, p->sell_tpos);
+    sell_tpos_fill<<<grid_for(ns * p->R), 256, 0, stream>>>(p->rowptr, p->trans, p->blkpos, p->slice_ptr,
+                                                           p->sell_row, p->R, ns, p->sell_tpos);
     PB_CHECK(hipGetLastError());
     dpos_fill<<<grid_for(nnode), 256, 0, stream>>>(p->diag, p->blkpos, nnode, p->dpos);
     PB_CHECK(hipGetLastError());
@@ -487,6 +551,7 @@ extern "C" int hf_pattern_destroy(hf_pattern* p) {
   hipFree(p->rowptr); hipFree(p->colidx); hipFree(p->seg); hipFree(p->contrib);
   hipFree(p->diag); hipFree(p->trans); hipFree(p->brow);
   hipFree(p->slice_ptr); hipFree(p->sell_col); hipFree(p->sell_tpos); hipFree(p->blkpos); hipFree(p->dpos);
+  hipFree(p->sell_row); hipFree(p->sell_pos);
   delete p;
   return 0;
 }
@@ -504,11 +569,11 @@ static int assemble_impl(const hf_pattern* p, const T* elem, T* vals, hipStream_
   HF_REQUIRE(p && elem && vals, "hf_assemble: null argument");
   const int64_t n = p->nslices * p->R;
   if (p->vec == 3)
-    assemble_kernel<T, 3><<<grid_for(n), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr, p->nnode, p->R,
-                                                       p->nslices, elem, vals);
+    assemble_kernel<T, 3><<<grid_for(n), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr, p->sell_row,
+                                                       p->R, p->nslices, elem, vals);
   else if (p->vec == 1)
-    assemble_kernel<T, 1><<<grid_for(n), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr, p->nnode, p->R,
-                                                       p->nslices, elem, vals);
+    assemble_kernel<T, 1><<<grid_for(n), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr, p->sell_row,
+                                                       p->R, p->nslices, elem, vals);
   else
     HF_REQUIRE(false, "hf_assemble: vec=%d not supported", p->vec);
   HF_LAUNCH_CHECK();
@@ -540,7 +605,8 @@ template <typename T>
 static int bc_rows_impl(const hf_pattern* p, const int32_t* dofs, int64_t nbc, T* vals, hipStream_t s) {
   HF_REQUIRE(p && vals, "hf_bc_rows: null argument");
   if (nbc == 0) return 0;
-  bc_rows_kernel<T><<<grid_for(nbc), 256, 0, s>>>(p->rowptr, p->slice_ptr, p->dpos, dofs, nbc, p->vec, p->R, vals);
+  bc_rows_kernel<T><<<grid_for(nbc), 256, 0, s>>>(p->rowptr, p->slice_ptr, p->sell_pos, p->dpos, dofs, nbc, p->vec,
+                                                   p->R, vals);
   HF_LAUNCH_CHECK();
   return 0;
 }
@@ -594,6 +660,7 @@ int64_t pattern_nslices(const hf_pattern* p) { return p->nslices; }
 const int32_t* pattern_slice_ptr(const hf_pattern* p) { return p->slice_ptr; }
 const int32_t* pattern_sell_col(const hf_pattern* p) { return p->sell_col; }
 const int32_t* pattern_sell_tpos(const hf_pattern* p) { return p->sell_tpos; }
+const int32_t* pattern_sell_row(const hf_pattern* p) { return p->sell_row; }
 int pattern_max_row_blocks(const hf_pattern* p) { return p->max_row_blocks; }
 int pattern_vec(const hf_pattern* p) { return p->vec; }
 }  // namespace hf

@@ -16,6 +16,7 @@ int64_t pattern_nslices(const hf_pattern* p);
 const int32_t* pattern_slice_ptr(const hf_pattern* p);
 const int32_t* pattern_sell_col(const hf_pattern* p);
 const int32_t* pattern_sell_tpos(const hf_pattern* p);
+const int32_t* pattern_sell_row(const hf_pattern* p);
 int pattern_vec(const hf_pattern* p);
 
 constexpr int kBlock = 256;         // threads per workgroup
@@ -101,6 +102,7 @@ struct MatArgs {
   const int32_t* slice_ptr;
   const int32_t* sell_col;
   const int32_t* sell_tpos;
+  const int32_t* sell_row;  // node of each slice row (-1: padding)
 };
 
 template <int VEC>
@@ -150,8 +152,8 @@ __device__ __forceinline__ void sweep(const MatArgs& m, int64_t s0, int64_t s1, 
           }
         }
       }
-      const int64_t n = s * S::R + r;
-      if (in_slice && n < m.nrows) epi(n, 0, y);
+      const int32_t n = in_slice ? m.sell_row[s * S::R + r] : -1;
+      if (n >= 0) epi((int64_t)n, 0, y);
     }
   } else {
     __shared__ T red[kWavesPerBlock][kWave * VEC];
@@ -183,8 +185,8 @@ __device__ __forceinline__ void sweep(const MatArgs& m, int64_t s0, int64_t s1, 
         T yi = 0;
 #pragma unroll
         for (int j = 0; j < VEC; ++j) yi += red[w][(r * VEC + j) * VEC + i];
-        const int64_t n = s * S::R + r;
-        if (n < m.nrows) epi(n, i, yi);
+        const int32_t n = m.sell_row[s * S::R + r];
+        if (n >= 0) epi((int64_t)n, i, yi);
       }
       wave_lds_sync();  // the next slice rewrites red
     }
@@ -221,6 +223,7 @@ inline MatArgs mat_args(const hf_pattern* p) {
   m.slice_ptr = pattern_slice_ptr(p);
   m.sell_col = pattern_sell_col(p);
   m.sell_tpos = pattern_sell_tpos(p);
+  m.sell_row = pattern_sell_row(p);
   return m;
 }
 
