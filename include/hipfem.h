@@ -104,8 +104,20 @@ int hf_gather_residual_f64(const hf_pattern* p, const double* elem_res, const do
  * longest row): hf_matrix_values() gives the element count of a value buffer.
  * assemble: every block = sum of its element blocks in cell order. */
 int64_t hf_matrix_values(const hf_pattern* p);
+/* blocks in the longest block row (node neighbours, itself included) */
+int32_t hf_pattern_max_row_blocks(const hf_pattern* p);
 int hf_assemble_f64(const hf_pattern* p, const double* elem_jac, double* vals, hf_stream_t stream);
 int hf_assemble_f32(const hf_pattern* p, const float* elem_jac, float* vals, hf_stream_t stream);
+/* HEX8 elasticity (vec 3, linear elastic or Neo-Hookean): the element Jacobians
+ * computed and assembled into the matrix values in one pass, without the
+ * element-matrix array; same summation order and bits as hf_cell_jacobian_* +
+ * hf_assemble_* (get_A over the jacfwd element matrices, problem.py:262-266,
+ * solver.py:466-515).  Rows must fit the LDS budget (max_row_blocks <= 42 in
+ * FP64, 85 in FP32), else the call fails and the two-stage path applies. */
+int hf_assemble_hex8_f64(const hf_pattern* p, const hf_material* mat, const double* points, const int32_t* cells,
+                         const double* u, const double* params, double* vals, hf_stream_t stream);
+int hf_assemble_hex8_f32(const hf_pattern* p, const hf_material* mat, const double* points, const int32_t* cells,
+                         const double* u, const double* params, float* vals, hf_stream_t stream);
 /* scalar CSR values (the order of hf_pattern_csr) from matrix values */
 int hf_matrix_to_csr_f64(const hf_pattern* p, const double* vals, double* csr, hf_stream_t stream);
 

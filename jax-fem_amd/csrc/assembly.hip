@@ -523,6 +523,8 @@ extern "C" int hf_pattern_sizes(const hf_pattern* p, int64_t* nblocks, int64_t* 
   return 0;
 }
 
+extern "C" int32_t hf_pattern_max_row_blocks(const hf_pattern* p) { return p ? p->max_row_blocks : -1; }
+
 extern "C" int64_t hf_matrix_values(const hf_pattern* p) {
   return p ? p->nslots * (int64_t)p->R * p->vec * p->vec : -1;
 }
@@ -661,6 +663,10 @@ const int32_t* pattern_slice_ptr(const hf_pattern* p) { return p->slice_ptr; }
 const int32_t* pattern_sell_col(const hf_pattern* p) { return p->sell_col; }
 const int32_t* pattern_sell_tpos(const hf_pattern* p) { return p->sell_tpos; }
 const int32_t* pattern_sell_row(const hf_pattern* p) { return p->sell_row; }
+const int32_t* pattern_seg(const hf_pattern* p) { return p->seg; }
+const int32_t* pattern_contrib(const hf_pattern* p) { return p->contrib; }
+const int32_t* pattern_diag(const hf_pattern* p) { return p->diag; }
+int pattern_nn(const hf_pattern* p) { return p->nn; }
 int pattern_max_row_blocks(const hf_pattern* p) { return p->max_row_blocks; }
 int pattern_vec(const hf_pattern* p) { return p->vec; }
 }  // namespace hf

@@ -20,6 +20,8 @@
 //   the 8 lanes of a cell write one contiguous 2304-byte element matrix.
 #include "common.hpp"
 #include "element_math.hpp"
+#include "hipfem.h"
+#include "krylov_common.hpp"
 
 namespace hf {
 
@@ -128,19 +130,13 @@ __global__ void __launch_bounds__(256) hex8_residual_kernel(
     for (int i = 0; i < VEC; ++i) o[n * VEC + i] = r[n][i];
 }
 
-// One thread per (cell, a).  Output block-major (cell, a, b, i, k).
+// Row a of the element Jacobian of one cell: K[b][i][k] = dR_(a,i) / du_(b,k).
+// Shared by the element-Jacobian kernel and the fused Jacobian + assembly kernel,
+// so both produce the same bits.
 template <typename T, int MAT, int VEC>
-__global__ void __launch_bounds__(256) hex8_jacobian_kernel(
-    const double* __restrict__ pts, const int32_t* __restrict__ cells, int64_t ncell,
-    const double* __restrict__ u, const double* __restrict__ params, hf_material mat,
-    T* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t c = t >> 3;
-  const int a = (int)(t & 7);
-  if (c >= ncell) return;
-  T X[8][3], U[8][VEC];
-  load_cell<T, VEC>(pts, cells, u, c, X, U);
-  T K[8][VEC][VEC];
+__device__ __forceinline__ void hex8_jacobian_row(const T X[8][3], const T U[8][VEC],
+                                                  const double* __restrict__ prm, const hf_material& mat, int a,
+                                                  T K[8][VEC][VEC]) {
 #pragma unroll
   for (int b = 0; b < 8; ++b)
 #pragma unroll
@@ -152,7 +148,7 @@ __global__ void __launch_bounds__(256) hex8_jacobian_kernel(
   for (int q = 0; q < 8; ++q) {
     T g[8][3], w;
     hex8_geometry<T>(q, X, g, w);
-    Moduli md = moduli(mat, params ? params[c * 8 + q] : 1.0, params != nullptr);
+    Moduli md = moduli(mat, prm ? prm[q] : 1.0, prm != nullptr);
     T ga[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) ga[j] = select8(g, a, j);
@@ -203,6 +199,22 @@ __global__ void __launch_bounds__(256) hex8_jacobian_kernel(
         K[b][0][0] += kw * (ga[0] * g[b][0] + ga[1] * g[b][1] + ga[2] * g[b][2]);
     }
   }
+}
+
+// One thread per (cell, a).  Output block-major (cell, a, b, i, k).
+template <typename T, int MAT, int VEC>
+__global__ void __launch_bounds__(256) hex8_jacobian_kernel(
+    const double* __restrict__ pts, const int32_t* __restrict__ cells, int64_t ncell,
+    const double* __restrict__ u, const double* __restrict__ params, hf_material mat,
+    T* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = t >> 3;
+  const int a = (int)(t & 7);
+  if (c >= ncell) return;
+  T X[8][3], U[8][VEC];
+  load_cell<T, VEC>(pts, cells, u, c, X, U);
+  T K[8][VEC][VEC];
+  hex8_jacobian_row<T, MAT, VEC>(X, U, params ? params + c * 8 : nullptr, mat, a, K);
   T* o = out + (c * 8 + a) * (8 * VEC * VEC);
 #pragma unroll
   for (int b = 0; b < 8; ++b)
@@ -246,6 +258,122 @@ __global__ void __launch_bounds__(256) hex8_param_vjp_kernel(
       for (int j = 0; j < 3; ++j) s += lg[i][j] * (md.d1 * P1[i][j] + md.d2 * P2[i][j]);
     grad[c * 8 + q] = -w * s;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Element Jacobians assembled straight into the sliced-ELL matrix values (HEX8,
+// vec 3), without the element-matrix array: 8 lanes per matrix row (node n of a
+// slice row), lane c of the group takes the c-th cell holding n (the cells are the
+// contributions of n's diagonal block, in cell order), computes row a = (n's local
+// index) of that cell's Jacobian and adds its 8 blocks into the row's accumulators
+// in LDS, one cell after the other in ascending cell order - the summation order
+// of the two-stage path (hf_cell_jacobian + hf_assemble), which gives the same bits.
+struct FusedArgs {
+  const int32_t *rowptr, *colidx, *seg, *contrib, *diag, *slice_ptr, *sell_row;
+  int64_t npos;  // slice rows (nslices * R)
+  int R, max_row;
+};
+
+template <typename T, int MAT>
+__global__ void __launch_bounds__(256) hex8_assemble_fused(FusedArgs fa, const double* __restrict__ pts,
+                                                           const int32_t* __restrict__ cells,
+                                                           const double* __restrict__ u,
+                                                           const double* __restrict__ params, hf_material mat,
+                                                           T* __restrict__ vals) {
+  extern __shared__ unsigned char hf_smem[];
+  T* acc = reinterpret_cast<T*>(hf_smem);  // [32 rows][max_row][9], row-major blocks
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ln = lane >> 3, cl = lane & 7;
+  const int64_t q = ((int64_t)blockIdx.x * 4 + wv) * 8 + ln;
+  T* my = acc + (int64_t)(wv * 8 + ln) * fa.max_row * 9;
+  const int32_t n = q < fa.npos ? fa.sell_row[q] : -1;
+  int32_t b0 = 0, len = 0, ncl = 0, s0 = 0;
+  if (n >= 0) {
+    b0 = fa.rowptr[n];
+    len = fa.rowptr[n + 1] - b0;
+    const int32_t db = fa.diag[n];
+    if (db >= 0) {
+      s0 = fa.seg[db];
+      ncl = fa.seg[db + 1] - s0;
+    }
+  }
+  for (int e = cl; e < len * 9; e += 8) my[e] = (T)0;
+  wave_lds_sync();
+  for (int cbase = 0; __ballot(ncl > cbase) != 0; cbase += 8) {
+    const int ci = cbase + cl;
+    const bool mine = ci < ncl;
+    T K[8][3][3];
+    int jslot[8];
+    if (mine) {
+      const int32_t id = fa.contrib[s0 + ci];
+      const int64_t c = id / 64;
+      const int a = (id / 8) % 8;
+      T X[8][3], U[8][3];
+      load_cell<T, 3>(pts, cells, u, c, X, U);
+      hex8_jacobian_row<T, MAT, 3>(X, U, params ? params + c * 8 : nullptr, mat, a, K);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int32_t m = cells[c * 8 + b];
+        int j = 0;
+        while (j < len && fa.colidx[b0 + j] != m) ++j;
+        jslot[b] = j;
+      }
+    }
+    for (int round = 0; round < 8; ++round) {  // cells in ascending order
+      if (mine && cl == round) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          if (jslot[b] < len) {
+            T* dst = my + jslot[b] * 9;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int k = 0; k < 3; ++k) dst[i * 3 + k] += K[b][i][k];
+          }
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+  if (q < fa.npos) {
+    const int64_t sl = q / fa.R;
+    const int r = (int)(q - sl * fa.R);
+    const int32_t t0 = fa.slice_ptr[sl], t1 = fa.slice_ptr[sl + 1];
+    for (int32_t j = cl; j < t1 - t0; j += 8) {
+      T* dst = vals + ((int64_t)(t0 + j) * fa.R + r) * 9;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) dst[(e % 3) * 3 + e / 3] = j < len ? my[j * 9 + e] : (T)0;  // column-major
+    }
+  }
+}
+
+template <typename T>
+static int launch_hex8_assemble_fused(const hf_pattern* p, const hf_material* mat, const double* pts,
+                                      const int32_t* cells, const double* u, const double* params, T* vals,
+                                      hipStream_t s) {
+  HF_REQUIRE(pattern_vec(p) == 3 && pattern_nn(p) == 8, "hf_assemble_hex8: needs a HEX8 vec-3 pattern");
+  HF_REQUIRE(mat->kind == MAT_LINEAR_ELASTIC || mat->kind == MAT_NEO_HOOKEAN,
+             "hf_assemble_hex8: material %d not supported", mat->kind);
+  FusedArgs fa;
+  fa.rowptr = pattern_rowptr(p);
+  fa.colidx = pattern_colidx(p);
+  fa.seg = pattern_seg(p);
+  fa.contrib = pattern_contrib(p);
+  fa.diag = pattern_diag(p);
+  fa.slice_ptr = pattern_slice_ptr(p);
+  fa.sell_row = pattern_sell_row(p);
+  fa.R = pattern_R(p);
+  fa.npos = pattern_nslices(p) * fa.R;
+  fa.max_row = pattern_max_row_blocks(p);
+  const size_t smem = (size_t)32 * fa.max_row * 9 * sizeof(T);
+  HF_REQUIRE(smem <= 96 * 1024, "hf_assemble_hex8: rows of %d blocks exceed the LDS budget", fa.max_row);
+  const unsigned grid = (unsigned)ceil_div(fa.npos, 32);
+  if (mat->kind == MAT_NEO_HOOKEAN)
+    hex8_assemble_fused<T, MAT_NEO_HOOKEAN><<<grid, 256, smem, s>>>(fa, pts, cells, u, params, *mat, vals);
+  else
+    hex8_assemble_fused<T, MAT_LINEAR_ELASTIC><<<grid, 256, smem, s>>>(fa, pts, cells, u, params, *mat, vals);
+  HF_LAUNCH_CHECK();
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -336,3 +464,16 @@ int hex8_jacobian_f32(const hf_material* m, const double* p, const int32_t* c, i
 }
 
 }  // namespace hf
+
+extern "C" int hf_assemble_hex8_f32(const hf_pattern* p, const hf_material* m, const double* pts, const int32_t* cells,
+                                    const double* u, const double* params, float* vals, hf_stream_t s) {
+  HF_REQUIRE(p && m && pts && cells && u && vals, "hf_assemble_hex8_f32: null argument");
+  HF_REQUIRE(m->quad1d == 0 || m->quad1d == 2, "hf_assemble_hex8_f32: HEX8 kernels use 2x2x2 Gauss points");
+  return hf::launch_hex8_assemble_fused<float>(p, m, pts, cells, u, params, vals, (hipStream_t)s);
+}
+extern "C" int hf_assemble_hex8_f64(const hf_pattern* p, const hf_material* m, const double* pts, const int32_t* cells,
+                                    const double* u, const double* params, double* vals, hf_stream_t s) {
+  HF_REQUIRE(p && m && pts && cells && u && vals, "hf_assemble_hex8_f64: null argument");
+  HF_REQUIRE(m->quad1d == 0 || m->quad1d == 2, "hf_assemble_hex8_f64: HEX8 kernels use 2x2x2 Gauss points");
+  return hf::launch_hex8_assemble_fused<double>(p, m, pts, cells, u, params, vals, (hipStream_t)s);
+}

@@ -17,6 +17,11 @@ const int32_t* pattern_slice_ptr(const hf_pattern* p);
 const int32_t* pattern_sell_col(const hf_pattern* p);
 const int32_t* pattern_sell_tpos(const hf_pattern* p);
 const int32_t* pattern_sell_row(const hf_pattern* p);
+const int32_t* pattern_seg(const hf_pattern* p);
+const int32_t* pattern_contrib(const hf_pattern* p);
+const int32_t* pattern_diag(const hf_pattern* p);
+int pattern_R(const hf_pattern* p);
+int pattern_nn(const hf_pattern* p);
 int pattern_vec(const hf_pattern* p);
 
 constexpr int kBlock = 256;         // threads per workgroup

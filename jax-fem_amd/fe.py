@@ -142,8 +142,21 @@ class FiniteElement:
         self.num_quads, self.num_nodes = self.shape_vals.shape
         self.num_faces, self.num_face_quads = nf, nfq
         self.dirichlet_bc_info = dirichlet_bc_info
+        self._loc_cache = {}
         self.node_inds_list, self.vec_inds_list, self.vals_list = self.Dirichlet_boundary_conditions(
             dirichlet_bc_info)
+
+    def _loc_nodes(self, loc):
+        """Nodes selected by a location function.  A location function is a pure
+        function of the point (and index), and the mesh of an element is fixed, so
+        the selection is evaluated once per function object; the cache holds the
+        function itself, so its id cannot be reused by another one."""
+        hit = self._loc_cache.get(id(loc))
+        if hit is not None and hit[0] is loc:
+            return hit[1]
+        nodes = np.flatnonzero(_location_flags(loc, self.points))
+        self._loc_cache[id(loc)] = (loc, nodes)
+        return nodes
 
     # -- Dirichlet data (fe.py:220-269) --------------------------------------
     def Dirichlet_boundary_conditions(self, dirichlet_bc_info):
@@ -153,7 +166,7 @@ class FiniteElement:
         location_fns, vecs, value_fns = dirichlet_bc_info
         assert len(location_fns) == len(value_fns) and len(value_fns) == len(vecs)
         for loc, comp, val in zip(location_fns, vecs, value_fns):
-            nodes = np.flatnonzero(_location_flags(loc, self.points))
+            nodes = self._loc_nodes(loc)
             probe = val(self.points[nodes[0]]) if len(nodes) else 0.0
             if _is_torch(probe):
                 # differentiable Dirichlet values (ad_wrapper through set_params)

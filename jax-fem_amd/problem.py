@@ -171,14 +171,16 @@ class Problem:
         key = tuple((id(v), len(v), float(v.detach().sum()) if torch.is_tensor(v) else float(np.sum(v)))
                     for v in fe.vals_list) + tuple(id(n) for n in fe.node_inds_list)
         if self._bc_key != key:
-            val = {}
+            ds, vs = [np.zeros(0, dtype=np.int64)], [np.zeros(0)]
             for nodes, comps, vals in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list):
-                dofs = np.asarray(nodes, dtype=np.int64) * self.vec + np.asarray(comps, dtype=np.int64)
+                ds.append(np.asarray(nodes, dtype=np.int64) * self.vec + np.asarray(comps, dtype=np.int64))
                 vals = vals.detach().cpu().numpy() if torch.is_tensor(vals) else vals
-                for d, v in zip(dofs.tolist(), np.asarray(vals, dtype=np.float64).reshape(-1).tolist()):
-                    val[d] = v
-            d = np.array(sorted(val), dtype=np.int32)
-            v = np.array([val[i] for i in d.tolist()], dtype=np.float64)
+                vs.append(np.broadcast_to(np.asarray(vals, dtype=np.float64).reshape(-1), ds[-1].shape))
+            all_d, all_v = np.concatenate(ds), np.concatenate(vs)
+            # the last spec naming a DOF wins: first occurrence in reversed order; sorted DOFs
+            d, first = np.unique(all_d[::-1], return_index=True)
+            v = all_v[::-1][first]
+            d = d.astype(np.int32)
             self._bc_dofs = torch.as_tensor(d, device=self.device)
             self._bc_vals = torch.as_tensor(v, device=self.device)
             self._bc_key = key
@@ -246,18 +248,33 @@ class Problem:
         p = self.pattern
         bb = self.vec * self.vec
         tag = "64" if dtype == torch.float64 else "32"
-        ej = self._buf("elem_jac" + tag, self.num_cells * self.nn * self.nn * bb, dtype)
         mat = self._hfmat()
         prm = self._params()
         s = _lib.stream()
-        _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
         vals = torch.empty(p.nvals, dtype=dtype, device=self.device)
-        _lib.call("hf_assemble_f" + tag, p.handle, _lib.ptr(ej), _lib.ptr(vals), s)
+        if self._fused_ok(dtype):
+            # element Jacobians assembled in the same pass (no element-matrix array)
+            _lib.call("hf_assemble_hex8_f" + tag, p.handle, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                      _lib.ptr(u), _lib.ptr(prm), _lib.ptr(vals), s)
+        else:
+            ej = self._buf("elem_jac" + tag, self.num_cells * self.nn * self.nn * bb, dtype)
+            _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                      self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
+            _lib.call("hf_assemble_f" + tag, p.handle, _lib.ptr(ej), _lib.ptr(vals), s)
         if bc:
             dofs, _ = self.bc_arrays()
             _lib.call("hf_bc_rows_f" + tag, p.handle, _lib.ptr(dofs), dofs.numel(), _lib.ptr(vals), s)
         return vals
+
+    fused_assembly = True   # HEX8 elasticity: hf_assemble_hex8 instead of element matrices + hf_assemble
+
+    def _fused_ok(self, dtype):
+        if not (self.fused_assembly and self.nn == 8 and self.vec == 3):
+            return False
+        if type(self.material).__name__ not in ("LinearElastic", "NeoHookean"):
+            return False
+        limit = 42 if dtype == torch.float64 else 85   # LDS: 32 rows x max_row x 9 values
+        return self.pattern.max_row_blocks <= limit
 
     def element_jacobians(self, u, dtype=torch.float64):
         """Element matrices (ncell, nd, nd), jac[r,c] = dR_r/du_c (problem.py:262-266)."""
@@ -317,6 +334,7 @@ class Pattern:
         _lib.call("hf_pattern_sizes", handle, ctypes.byref(nb), ctypes.byref(nnz), ctypes.byref(nc))
         self.nblocks, self.nnz, self.ncontrib = nb.value, nnz.value, nc.value
         self.nvals = int(_lib.load().hf_matrix_values(handle))
+        self.max_row_blocks = int(_lib.load().hf_pattern_max_row_blocks(handle))
         self.N = problem.N
         self.vec = problem.vec
         self.device = problem.device
