@@ -216,6 +216,46 @@ int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const float* dinv, 
                      int32_t flags, int32_t stall_window, int32_t trunc, void* workspace,
                      hf_krylov_info* info, hf_stream_t stream);
 
+/* ---- geometric multigrid on the structured node lattice (no reference
+ *      counterpart; the reference only has Jacobi, solver.py:68-69).  Box mesh
+ *      node order n = (i*ny + j)*nz + k (generate_mesh.py:170-187), vec = 3,
+ *      FP32.  Level 0 is the tangent with its Dirichlet DOFs masked out of rows
+ *      and columns; each coarser level halves the intervals per dimension and
+ *      holds the Galerkin product P^T A P of trilinear interpolation as a
+ *      27-point node-block stencil, stored slot-major
+ *      S[(o*9 + 3i + j)*nnode + n], o = (dx+1)*9 + (dy+1)*3 + (dz+1); the
+ *      coarsest (<= 88 DOF) is inverted densely in FP64.  V-cycles with a
+ *      polynomial smoother in Dinv A (Dinv: inverse 3x3 diagonal blocks) of
+ *      `degree` steps on each side: Chebyshev on [lam/ratio, lam] with lam the
+ *      Gershgorin bound of Dinv A (default: degree 2, ratio 8), or damped
+ *      block Jacobi (param = omega). */
+typedef struct hf_mg hf_mg;
+enum { HF_MG_JACOBI = 0, HF_MG_CHEBYSHEV = 1 };
+int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, hf_mg** out);   /* nodes per dimension */
+int hf_mg_destroy(hf_mg* mg);
+/* number of levels; dims (3 per level, up to max_levels) receives the node counts */
+int hf_mg_levels(const hf_mg* mg, int32_t* nlev, int32_t* dims, int32_t max_levels);
+int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float param);
+/* lam of every level but the coarsest from the last set-up */
+int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels);
+/* builds every level from the FP32 tangent (pattern + SELL values) and the
+ * level-0 DOF mask (1 free, 0 Dirichlet; kept by reference until the next
+ * set-up); *dropped (optional) = coarsest DOFs without support.  Fails if the
+ * pattern couples nodes that are not lattice neighbours.  Synchronises. */
+int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* mask, int32_t* dropped,
+                    hf_stream_t stream);
+/* z = V(r), one cycle */
+int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, hf_stream_t stream);
+/* copy level `level`'s stencil (243 * nnode floats) into dst */
+int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf_stream_t stream);
+/* flexible CG (one kept direction) preconditioned by V-cycles; the stop rule,
+ * flags (HF_KRYLOV_REL_R0; no transpose) and info of hf_krylov_solve.
+ * status 2: p.Ap <= 0.  workspace: hf_mg_workspace_size. */
+size_t hf_mg_workspace_size(const hf_mg* mg);
+int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* b, float* x, int32_t flags,
+                    int32_t maxiter, double rtol, double atol, int32_t poll, void* workspace,
+                    hf_krylov_info* info, hf_stream_t stream);
+
 /* ---- measurement: HIP-event timing of the Krylov SpMV kernel (the first SpMV
  *      of every poll batch, on the solve's stream).  No reference counterpart:
  *      replaces the reference's wall-clock buckets (solver.py:22-57) at kernel

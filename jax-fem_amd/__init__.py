@@ -18,6 +18,7 @@ from .materials import LinearElastic, NeoHookean, Poisson, Material
 from .mesh import Mesh, box_mesh, box_mesh_hex27, get_meshio_cell_type
 from .problem import Problem
 from .learned_pc import LearnedPC
+from .multigrid import Multigrid
 from .solver import (solver, ad_wrapper, implicit_vjp, get_A, linear_solver, apply_bc_vec, assign_bc, copy_bc,
                      TangentMatrix, krylov)
 
@@ -31,4 +32,4 @@ if not logger.handlers:
 __all__ = ["HipFemError", "Problem", "solver", "ad_wrapper", "implicit_vjp", "get_A", "linear_solver", "apply_bc_vec",
            "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "get_meshio_cell_type",
            "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov",
-           "LearnedPC"]
+           "LearnedPC", "Multigrid"]

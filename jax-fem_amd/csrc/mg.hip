@@ -1,0 +1,1386 @@
+// Geometric multigrid preconditioner on the structured node lattice of a box
+// mesh (node n = (x*ny + y)*nz + z, generate_mesh.py:170-187), vec = 3, FP32,
+// and the flexible CG it preconditions.  No reference counterpart: the
+// reference's Krylov solves use Jacobi only (jax_fem/solver.py:68-69).
+//
+// Levels.  Level 0 is the tangent's 27-point node-block stencil with the
+// Dirichlet DOFs masked out of its rows and columns (S0 = M A M).  Level l+1
+// halves the intervals of every dimension (n -> ceil(n/2); coarse node c sits
+// at fine position min(2c, n), so an odd count ends in one short interval) and
+// carries the Galerkin operator A_{l+1} = P^T A_l P of the trilinear
+// prolongation P (per dimension: weight 1 at the coarse node's own position,
+// 1/2 at a fine-only neighbour).  A_l P is formed first (fine rows, 27 slots of
+// coarse columns around x/2), then P^T (A_l P); both stay 27-point.  The
+// coarsest level (<= kDenseMax DOF) is inverted densely in FP64 (Gauss-Jordan
+// in LDS on the symmetrised matrix).
+//
+// Stencils are stored slot-major: S[(o*9 + 3i + j)*nnode + n] is entry (i, j)
+// of the block that couples node n to its neighbour at offset
+// o = (dx+1)*9 + (dy+1)*3 + (dz+1); slots that leave the lattice hold zeros.
+// A thread per node reads its 243 values with every wavefront load coalesced,
+// and neighbour indices are clamped to the lattice (the zero slots cancel them).
+//
+// V-cycle at level l (input r, output e), nu sweeps per side:
+//   e = w Dinv r;  t = r - A e;  r_{l+1} = P^T t;  e_{l+1} = V(r_{l+1});
+//   e += M P e_{l+1};  e = e + w Dinv (r - A e)
+// with Dinv the inverse of each node's 3x3 diagonal block and M the Dirichlet
+// mask (level 0 only).  The same smoother on both sides makes the cycle a
+// symmetric operator.
+//
+// Outer method: flexible CG with one kept direction (Notay's FCG(1)) in the
+// Chronopoulos-Gear arrangement of krylov.hip: z = V(r), w = A z and the dot
+// products r.z, z.w, z.q in one phase, then
+//   beta = -z.q / p.q,  p = z + beta p,  q = w + beta q  (q = A p),
+//   p.q <- z.w + 2 beta z.q + beta^2 p.q,  alpha = r.z / p.q,
+//   x += alpha p,  r -= alpha q.
+// The flexible beta keeps the method valid when the preconditioner is not
+// exactly symmetric (FP32 rounding, or a learned fine-level smoother).
+#include "common.hpp"
+#include "hipfem.h"
+
+#include "krylov_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace hf {
+const int32_t* pattern_blkpos(const hf_pattern* p);
+const int32_t* pattern_sell_pos(const hf_pattern* p);
+
+namespace {
+constexpr int kMgMaxLevels = 16;
+constexpr int kDenseMax = 88;   // FP64 dense coarsest matrix in 62 KB of LDS
+constexpr int kMgBlock = 256;
+constexpr int kSt = 243;        // 27 slots x 9 values
+constexpr int kMaxDegree = 8;   // smoothing steps per side
+
+struct Lat {
+  int nx, ny, nz;
+  int64_t n;
+};
+
+inline Lat make_lat(int nx, int ny, int nz) {
+  Lat g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.n = (int64_t)nx * ny * nz;
+  return g;
+}
+
+__device__ __forceinline__ void lat_xyz(const Lat& g, int64_t n, int& x, int& y, int& z) {
+  z = (int)(n % g.nz);
+  const int64_t q = n / g.nz;
+  y = (int)(q % g.ny);
+  x = (int)(q / g.ny);
+}
+__device__ __forceinline__ int64_t lat_id(const Lat& g, int x, int y, int z) {
+  return ((int64_t)x * g.ny + y) * g.nz + z;
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// 1-D transfer between nf fine intervals and their coarse nodes.
+__device__ __forceinline__ int cpos(int c, int nf) { return 2 * c < nf ? 2 * c : nf; }
+__device__ __forceinline__ bool is_cpos(int f, int nf) { return (f % 2 == 0) || f == nf; }
+__device__ __forceinline__ float p1(int f, int c, int nf) {
+  const int p = cpos(c, nf);
+  if (f == p) return 1.f;
+  if ((f == p - 1 || f == p + 1) && !is_cpos(f, nf)) return 0.5f;
+  return 0.f;
+}
+
+// Contiguous node range of this workgroup (XCD-remapped, as the Krylov kernels).
+__device__ __forceinline__ void node_range(int64_t n, int64_t& i0, int64_t& i1) {
+  const int64_t per = ceil_div(n, gridDim.x);
+  i0 = (int64_t)logical_block() * per;
+  i1 = i0 + per < n ? i0 + per : n;
+}
+inline int node_grid(int64_t n) {
+  int64_t g = ceil_div(n, kMgBlock);
+  if (g > kMaxGrid) g = kMaxGrid;
+  if (g >= 64) g = (g / kNumXcd) * kNumXcd;
+  return (int)(g < 1 ? 1 : g);
+}
+
+// out = sum over the 27 slots of S[n] v[neighbour]  (clamped neighbours)
+__device__ __forceinline__ void stencil_row(const float* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
+                                            const float* __restrict__ v, float (&out)[3]) {
+  out[0] = out[1] = out[2] = 0.f;
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx) {
+    const int xx = clampi(x + dx - 1, 0, g.nx - 1);
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      const int yy = clampi(y + dy - 1, 0, g.ny - 1);
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz) {
+        const int zz = clampi(z + dz - 1, 0, g.nz - 1);
+        const int o = dx * 9 + dy * 3 + dz;
+        const int64_t m = lat_id(g, xx, yy, zz) * 3;
+        const float v0 = v[m], v1 = v[m + 1], v2 = v[m + 2];
+        const float* s = S + (int64_t)o * 9 * g.n + n;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          out[i] += s[(3 * i) * g.n] * v0 + s[(3 * i + 1) * g.n] * v1 + s[(3 * i + 2) * g.n] * v2;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void block_apply(const float* __restrict__ D, int64_t n, int64_t node, const float (&v)[3],
+                                            float (&out)[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    out[i] = D[(3 * i) * n + node] * v[0] + D[(3 * i + 1) * n + node] * v[1] + D[(3 * i + 2) * n + node] * v[2];
+}
+
+// ---------------------------------------------------------------------------
+// set-up kernels
+
+// Level 0: S0 = M A M from the tangent's SELL values, one thread per (node,
+// slot = blockIdx.y).  On the lattice a row's columns are its in-lattice
+// neighbours in sorted order, which is the lexicographic order of the offsets,
+// so the block of offset o is the row's k-th, k the number of in-lattice
+// offsets before o, and sits in slot k of the node's slice row.
+// mg_check_lattice verifies that structure once per pattern (colidx and
+// blkpos against the rule, and the row lengths); mg_fine_stencil then reads
+// only the values and the mask.
+struct LatSlot {
+  bool inside;
+  int k;          // index of the offset's block within the row
+  int64_t c;      // neighbour node
+  int count;      // in-lattice neighbours of the node (row length)
+};
+__device__ __forceinline__ LatSlot lat_slot(const Lat& g, int64_t n, int o) {
+  int p[3];
+  lat_xyz(g, n, p[0], p[1], p[2]);
+  const int np[3] = {g.nx, g.ny, g.nz};
+  const int off[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+  LatSlot r;
+  r.inside = true;
+  int na[3], rank[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int lo = p[d] > 0, hi = p[d] < np[d] - 1;
+    na[d] = 1 + lo + hi;
+    rank[d] = off[d] < 0 ? 0 : (off[d] == 0 ? lo : lo + 1);
+    r.inside = r.inside && (off[d] >= 0 || lo) && (off[d] <= 0 || hi);
+  }
+  r.k = (rank[0] * na[1] + rank[1]) * na[2] + rank[2];
+  r.count = na[0] * na[1] * na[2];
+  r.c = r.inside ? lat_id(g, p[0] + off[0], p[1] + off[1], p[2] + off[2]) : -1;
+  return r;
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_check_lattice(const int32_t* __restrict__ rowptr,
+                                                             const int32_t* __restrict__ colidx,
+                                                             const int32_t* __restrict__ blkpos,
+                                                             const int32_t* __restrict__ sell_pos,
+                                                             const int32_t* __restrict__ slice_ptr, int R, Lat g,
+                                                             int32_t* __restrict__ bad) {
+  const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  const int o = blockIdx.y;
+  if (n >= g.n) return;
+  const LatSlot ls = lat_slot(g, n, o);
+  const int32_t r0 = rowptr[n], len = rowptr[n + 1] - r0;
+  if (o == 13 && len != ls.count) bad[0] = 1;
+  if (!ls.inside) return;
+  if (ls.k >= len) {
+    bad[0] = 1;
+    return;
+  }
+  const int32_t q = sell_pos[n];
+  const int64_t pos = ((int64_t)(slice_ptr[q / R] + ls.k) * R + q % R) * 9;
+  if (colidx[r0 + ls.k] != (int32_t)ls.c || blkpos[r0 + ls.k] != pos) bad[0] = 1;
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_fine_stencil(const int32_t* __restrict__ rowptr,
+                                                            const int32_t* __restrict__ sell_pos,
+                                                            const int32_t* __restrict__ slice_ptr, int R,
+                                                            const float* __restrict__ vals,
+                                                            const float* __restrict__ mask, Lat g,
+                                                            float* __restrict__ S) {
+  const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  const int o = blockIdx.y;
+  if (n >= g.n) return;
+  const LatSlot ls = lat_slot(g, n, o);
+  float out[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) out[e] = 0.f;
+  if (ls.inside && ls.k < rowptr[n + 1] - rowptr[n]) {
+    const int32_t q = sell_pos[n];
+    const float* v = vals + ((int64_t)(slice_ptr[q / R] + ls.k) * R + q % R) * 9;
+    const float mr[3] = {mask[3 * n], mask[3 * n + 1], mask[3 * n + 2]};
+    const float mc[3] = {mask[3 * ls.c], mask[3 * ls.c + 1], mask[3 * ls.c + 2]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) out[3 * i + j] = v[3 * j + i] * mr[i] * mc[j];
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) S[(int64_t)(o * 9 + e) * g.n + n] = out[e];
+}
+
+// Inverse of each node's diagonal block (slot 13); masked DOFs (mask 0) get a
+// unit diagonal.  A block that is not safely invertible falls back to the
+// inverse of its positive diagonal entries (0 elsewhere).
+__global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restrict__ S, const float* __restrict__ mask,
+                                                          int64_t n, float* __restrict__ Dinv) {
+  const int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (node >= n) return;
+  double a[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) a[e] = (double)S[(int64_t)(13 * 9 + e) * n + node];
+  if (mask) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (mask[3 * node + i] == 0.f) a[4 * i] += 1.0;
+  }
+  double c[9];
+  c[0] = a[4] * a[8] - a[5] * a[7];
+  c[1] = a[2] * a[7] - a[1] * a[8];
+  c[2] = a[1] * a[5] - a[2] * a[4];
+  c[3] = a[5] * a[6] - a[3] * a[8];
+  c[4] = a[0] * a[8] - a[2] * a[6];
+  c[5] = a[2] * a[3] - a[0] * a[5];
+  c[6] = a[3] * a[7] - a[4] * a[6];
+  c[7] = a[1] * a[6] - a[0] * a[7];
+  c[8] = a[0] * a[4] - a[1] * a[3];
+  const double det = a[0] * c[0] + a[1] * c[3] + a[2] * c[6];
+  const double sc = fmax(fabs(a[0]), fmax(fabs(a[4]), fabs(a[8])));
+  if (det > 1e-12 * sc * sc * sc && isfinite(det)) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Dinv[(int64_t)e * n + node] = (float)(c[e] / det);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      const bool d = (e % 4) == 0;
+      Dinv[(int64_t)e * n + node] = (d && a[e] > 0.0) ? (float)(1.0 / a[e]) : 0.f;
+    }
+  }
+}
+
+// Stage 1 of the Galerkin product: B = A P over fine rows i, with slot s of
+// row i holding the block of coarse column J = floor(i/2) + s - 1 (per
+// dimension).  One thread per (row, slot); the slot is blockIdx.y.
+__global__ void __launch_bounds__(kMgBlock) mg_rap_ap(const float* __restrict__ S, Lat f, Lat c,
+                                                      float* __restrict__ B) {
+  const int64_t i = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  const int slot = blockIdx.y;
+  if (i >= f.n) return;
+  int fi[3];
+  lat_xyz(f, i, fi[0], fi[1], fi[2]);
+  const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
+  const int nc[3] = {c.nx - 1, c.ny - 1, c.nz - 1};
+  const int sd[3] = {slot / 9, (slot / 3) % 3, slot % 3};
+  float W[3][3];  // [dim][offset]: P(fine row + offset - 1, coarse column of this slot)
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int J = fi[d] / 2 + sd[d] - 1;
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      const int j = fi[d] + o - 1;
+      W[d][o] = (J >= 0 && J <= nc[d] && j >= 0 && j <= nf[d]) ? p1(j, J, nf[d]) : 0.f;
+    }
+  }
+  float acc[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) acc[e] = 0.f;
+#pragma unroll
+  for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+      for (int oz = 0; oz < 3; ++oz) {
+        const float w = W[0][ox] * W[1][oy] * W[2][oz];
+        if (w == 0.f) continue;
+        const float* sp = S + (int64_t)((ox * 9 + oy * 3 + oz) * 9) * f.n + i;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] += w * sp[(int64_t)e * f.n];
+      }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) B[(int64_t)(slot * 9 + e) * f.n + i] = acc[e];
+}
+
+// Stage 2: A_c = P^T B, one thread per (coarse node, output slot O = blockIdx.y).
+__global__ void __launch_bounds__(kMgBlock) mg_rap_ptap(const float* __restrict__ B, Lat f, Lat c,
+                                                        float* __restrict__ Sc) {
+  const int64_t I = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  const int O = blockIdx.y;
+  if (I >= c.n) return;
+  int ci[3];
+  lat_xyz(c, I, ci[0], ci[1], ci[2]);
+  const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
+  const int nc[3] = {c.nx - 1, c.ny - 1, c.nz - 1};
+  const int J[3] = {ci[0] + O / 9 - 1, ci[1] + (O / 3) % 3 - 1, ci[2] + O % 3 - 1};
+  float acc[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) acc[e] = 0.f;
+  const bool inside = J[0] >= 0 && J[0] <= nc[0] && J[1] >= 0 && J[1] <= nc[1] && J[2] >= 0 && J[2] <= nc[2];
+  if (inside) {
+    // fine rows in the support of I per dimension, their weights and B slots
+    int fr[3][3], sl[3][3];
+    float wr[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int p = cpos(ci[d], nf[d]);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int fidx = p + a - 1;
+        const bool ok = fidx >= 0 && fidx <= nf[d];
+        const int s = ok ? J[d] - fidx / 2 : 5;
+        fr[d][a] = ok ? fidx : p;
+        sl[d][a] = s + 1;
+        wr[d][a] = (ok && s >= -1 && s <= 1) ? p1(fidx, ci[d], nf[d]) : 0.f;
+      }
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+      for (int ay = 0; ay < 3; ++ay)
+#pragma unroll
+        for (int az = 0; az < 3; ++az) {
+          const float w = wr[0][ax] * wr[1][ay] * wr[2][az];
+          if (w == 0.f) continue;
+          const int slot = sl[0][ax] * 9 + sl[1][ay] * 3 + sl[2][az];
+          const int64_t i = lat_id(f, fr[0][ax], fr[1][ay], fr[2][az]);
+          const float* bp = B + (int64_t)(slot * 9) * f.n + i;
+#pragma unroll
+          for (int e = 0; e < 9; ++e) acc[e] += w * bp[(int64_t)e * f.n];
+        }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Sc[(int64_t)(O * 9 + e) * c.n + I] = acc[e];
+}
+
+// Dense FP64 inverse of the coarsest level: one workgroup, matrix in LDS,
+// symmetrised, Gauss-Jordan without pivoting (SPD).  A non-positive pivot
+// (a coarse DOF with no support) drops that DOF: its row and column of the
+// inverse are 0; bad[1] counts such pivots.
+constexpr int kDenseThreads = 1024;
+__global__ void __launch_bounds__(kDenseThreads) mg_dense_setup(const float* __restrict__ S, Lat g,
+                                                                double* __restrict__ Ainv, int32_t* __restrict__ bad) {
+  constexpr int kCols = kDenseThreads / kDenseMax;  // threads per row (11): row ti, columns tj, tj + kCols, ...
+  __shared__ double A[kDenseMax * kDenseMax];
+  __shared__ double col[kDenseMax];
+  const int nd = (int)(3 * g.n);
+  const int tid = threadIdx.x;
+  const int ti = tid / kCols, tj = tid - (tid / kCols) * kCols;
+  const bool owner = ti < nd;
+  if (owner)
+    for (int j = tj; j < nd; j += kCols) A[ti * nd + j] = 0.0;
+  __syncthreads();
+  for (int q = tid; q < (int)g.n * 27; q += kDenseThreads) {
+    const int I = q / 27, O = q - (q / 27) * 27;
+    int x, y, z;
+    lat_xyz(g, I, x, y, z);
+    const int jx = x + O / 9 - 1, jy = y + (O / 3) % 3 - 1, jz = z + O % 3 - 1;
+    if (jx < 0 || jx >= g.nx || jy < 0 || jy >= g.ny || jz < 0 || jz >= g.nz) continue;
+    const int J = (int)lat_id(g, jx, jy, jz);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) A[(3 * I + a) * nd + 3 * J + b] = (double)S[(int64_t)(O * 9 + 3 * a + b) * g.n + I];
+  }
+  __syncthreads();
+  // symmetrise: each owner averages its entries above the diagonal with their mirrors
+  double sym[(kDenseMax + kCols - 1) / kCols];
+  if (owner) {
+    int m = 0;
+    for (int j = tj; j < nd; j += kCols, ++m) sym[m] = 0.5 * (A[ti * nd + j] + A[j * nd + ti]);
+  }
+  __syncthreads();
+  if (owner) {
+    int m = 0;
+    for (int j = tj; j < nd; j += kCols, ++m) A[ti * nd + j] = sym[m];
+  }
+  __syncthreads();
+  int dropped = 0;
+  for (int k = 0; k < nd; ++k) {
+    if (tid < nd) col[tid] = A[tid * nd + k];
+    __syncthreads();
+    const double piv = col[k];
+    const double pinv = (piv > 0.0 && isfinite(piv)) ? 1.0 / piv : 0.0;
+    dropped += (pinv == 0.0);
+    if (owner && ti == k)
+      for (int j = tj; j < nd; j += kCols) A[k * nd + j] = (j == k ? 1.0 : A[k * nd + j]) * pinv;
+    __syncthreads();
+    if (owner && ti != k) {
+      const double f = col[ti];
+      for (int j = tj; j < nd; j += kCols) A[ti * nd + j] = (j == k ? 0.0 : A[ti * nd + j]) - f * A[k * nd + j];
+    }
+    __syncthreads();
+  }
+  if (owner)
+    for (int j = tj; j < nd; j += kCols) Ainv[ti * nd + j] = A[ti * nd + j];
+  if (tid == 0) bad[1] = dropped;
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle kernels (sc: the outer solve's scalars; nullptr for a single apply)
+
+// Smoothing is a polynomial in Dinv A applied to the level's residual, run as
+// the three-term recurrence (Chebyshev; damped Jacobi is the case c1 = 0):
+//   d_0 = s0 Dinv r_0,  d_j = c1_j d_{j-1} + c2_j Dinv r_j,  r_j = r_{j-1} - A d_{j-1},
+//   e = sum_j d_j.
+// Each step is one pass over the stencil (the product A d_{j-1} needs the
+// previous direction at the neighbours, so directions alternate buffers).
+// The per-node bodies below are shared by the grid kernels (large levels) and
+// by mg_small_cycle (every level from the first small one down, one workgroup).
+
+// pre-smoother from e = 0: d = s0 Dinv r, e = d
+__device__ __forceinline__ void poly_first_node(const float* __restrict__ Dinv, int64_t n, float s0,
+                                                const float* __restrict__ r, float* __restrict__ d,
+                                                float* __restrict__ e, int64_t node) {
+  const float v[3] = {r[3 * node], r[3 * node + 1], r[3 * node + 2]};
+  float o[3];
+  block_apply(Dinv, n, node, v, o);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    d[3 * node + i] = s0 * o[i];
+    e[3 * node + i] = s0 * o[i];
+  }
+}
+
+// res = res_in - A d_in;  d_out = c1 d_in + c2 Dinv res;  e += d_out
+// (res_in may alias res_out: each node reads and writes only its own entries)
+__device__ __forceinline__ void poly_step_node(const float* __restrict__ S, const float* __restrict__ Dinv,
+                                               const Lat& g, float c1, float c2, const float* res_in,
+                                               const float* __restrict__ d_in, float* res_out,
+                                               float* __restrict__ d_out, float* __restrict__ e, int64_t node) {
+  int x, y, z;
+  lat_xyz(g, node, x, y, z);
+  float ad[3];
+  stencil_row(S, g, node, x, y, z, d_in, ad);
+  const float res[3] = {res_in[3 * node] - ad[0], res_in[3 * node + 1] - ad[1], res_in[3 * node + 2] - ad[2]};
+  float dr[3];
+  block_apply(Dinv, g.n, node, res, dr);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float dn = c1 * d_in[3 * node + i] + c2 * dr[i];
+    res_out[3 * node + i] = res[i];
+    d_out[3 * node + i] = dn;
+    e[3 * node + i] += dn;
+  }
+}
+
+// t = res_in - A d_in (the residual after the last pre-smoothing step)
+__device__ __forceinline__ void poly_resid_node(const float* __restrict__ S, const Lat& g,
+                                                const float* __restrict__ res_in, const float* __restrict__ d_in,
+                                                float* __restrict__ t, int64_t node) {
+  int x, y, z;
+  lat_xyz(g, node, x, y, z);
+  float ad[3];
+  stencil_row(S, g, node, x, y, z, d_in, ad);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t[3 * node + i] = res_in[3 * node + i] - ad[i];
+}
+
+// post-smoother start: res = r - A e_in;  d = s0 Dinv res;  e_out = e_in + d
+__device__ __forceinline__ void post_first_node(const float* __restrict__ S, const float* __restrict__ Dinv,
+                                                const Lat& g, float s0, const float* __restrict__ r,
+                                                const float* __restrict__ e_in, float* __restrict__ res,
+                                                float* __restrict__ d, float* __restrict__ e_out, int64_t node) {
+  int x, y, z;
+  lat_xyz(g, node, x, y, z);
+  float ae[3];
+  stencil_row(S, g, node, x, y, z, e_in, ae);
+  const float rr[3] = {r[3 * node] - ae[0], r[3 * node + 1] - ae[1], r[3 * node + 2] - ae[2]};
+  float dr[3];
+  block_apply(Dinv, g.n, node, rr, dr);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    res[3 * node + i] = rr[i];
+    d[3 * node + i] = s0 * dr[i];
+    e_out[3 * node + i] = e_in[3 * node + i] + s0 * dr[i];
+  }
+}
+
+// rc = P^T t at coarse node I; with Dc given, also the coarse pre-smoother's
+// start dc = ec = s0 Dc rc
+__device__ __forceinline__ void restrict_node(const float* __restrict__ t, const Lat& f, const Lat& c,
+                                              float* __restrict__ rc, const float* __restrict__ Dc, float s0,
+                                              float* __restrict__ dc, float* __restrict__ ec, int64_t I) {
+  const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
+  int ci[3];
+  lat_xyz(c, I, ci[0], ci[1], ci[2]);
+  int fr[3][3];
+  float wr[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int p = cpos(ci[d], nf[d]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int fidx = p + a - 1;
+      const bool ok = fidx >= 0 && fidx <= nf[d];
+      fr[d][a] = ok ? fidx : p;
+      wr[d][a] = ok ? p1(fidx, ci[d], nf[d]) : 0.f;
+    }
+  }
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int ay = 0; ay < 3; ++ay)
+#pragma unroll
+      for (int az = 0; az < 3; ++az) {
+        const float w = wr[0][ax] * wr[1][ay] * wr[2][az];
+        const int64_t i = lat_id(f, fr[0][ax], fr[1][ay], fr[2][az]) * 3;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] += w * t[i + k];
+      }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) rc[3 * I + k] = acc[k];
+  if (Dc) {
+    float o[3];
+    block_apply(Dc, c.n, I, acc, o);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dc[3 * I + k] = s0 * o[k];
+      ec[3 * I + k] = s0 * o[k];
+    }
+  }
+}
+
+// e += mask * P ec at fine node i
+__device__ __forceinline__ void prolong_node(const float* __restrict__ ec, const Lat& f, const Lat& c,
+                                             const float* __restrict__ mask, float* __restrict__ e, int64_t i) {
+  const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
+  const int nc[3] = {c.nx - 1, c.ny - 1, c.nz - 1};
+  int fi[3];
+  lat_xyz(f, i, fi[0], fi[1], fi[2]);
+  int pc[3][2];
+  float pw[3][2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int v = fi[d];
+    if (v % 2 == 0) {
+      pc[d][0] = v / 2; pw[d][0] = 1.f; pc[d][1] = v / 2; pw[d][1] = 0.f;
+    } else if (v == nf[d]) {
+      pc[d][0] = nc[d]; pw[d][0] = 1.f; pc[d][1] = nc[d]; pw[d][1] = 0.f;
+    } else {
+      pc[d][0] = (v - 1) / 2; pw[d][0] = 0.5f; pc[d][1] = (v + 1) / 2; pw[d][1] = 0.5f;
+    }
+  }
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const float w = pw[0][a] * pw[1][b] * pw[2][cc];
+        const int64_t J = lat_id(c, pc[0][a], pc[1][b], pc[2][cc]) * 3;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] += w * ec[J + k];
+      }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float m = mask ? mask[3 * i + k] : 1.f;
+    e[3 * i + k] += m * acc[k];
+  }
+}
+
+#define MG_NODE_LOOP(N, BODY)                                                   \
+  do {                                                                          \
+    int64_t i0_, i1_;                                                           \
+    node_range((N), i0_, i1_);                                                  \
+    for (int64_t node = i0_ + threadIdx.x; node < i1_; node += kMgBlock) BODY;  \
+  } while (0)
+
+__global__ void __launch_bounds__(kMgBlock) mg_poly_first(const float* __restrict__ Dinv, int64_t n, float s0,
+                                                          const float* __restrict__ r, float* __restrict__ d,
+                                                          float* __restrict__ e, const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(n, poly_first_node(Dinv, n, s0, r, d, e, node));
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_poly_step(const float* __restrict__ S, const float* __restrict__ Dinv,
+                                                         Lat g, float c1, float c2, const float* res_in,
+                                                         const float* __restrict__ d_in, float* res_out,
+                                                         float* __restrict__ d_out, float* __restrict__ e,
+                                                         const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(g.n, poly_step_node(S, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const float* __restrict__ S, Lat g,
+                                                          const float* __restrict__ res_in,
+                                                          const float* __restrict__ d_in, float* __restrict__ t,
+                                                          const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(g.n, poly_resid_node(S, g, res_in, d_in, t, node));
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_post_first(const float* __restrict__ S, const float* __restrict__ Dinv,
+                                                          Lat g, float s0, const float* __restrict__ r,
+                                                          const float* __restrict__ e_in, float* __restrict__ res,
+                                                          float* __restrict__ d, float* __restrict__ e_out,
+                                                          const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(g.n, post_first_node(S, Dinv, g, s0, r, e_in, res, d, e_out, node));
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_restrict(const float* __restrict__ t, Lat f, Lat c,
+                                                        float* __restrict__ rc, const float* __restrict__ Dc,
+                                                        float s0, float* __restrict__ dc, float* __restrict__ ec,
+                                                        const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(c.n, restrict_node(t, f, c, rc, Dc, s0, dc, ec, node));
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_prolong(const float* __restrict__ ec, Lat f, Lat c,
+                                                       const float* __restrict__ mask, float* __restrict__ e,
+                                                       const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(f.n, prolong_node(ec, f, c, mask, e, node));
+}
+
+// Gershgorin bound of Dinv A: per-workgroup max over nodes of the largest
+// absolute row sum of the node's block row of Dinv A.
+__global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S, const float* __restrict__ Dinv,
+                                                     int64_t n, float* __restrict__ part) {
+  __shared__ float sm[kMgBlock / kWave];
+  float best = 0.f;
+  for (int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x; node < n; node += (int64_t)gridDim.x * kMgBlock) {
+    float M[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) M[e] = Dinv[(int64_t)e * n + node];
+    float rs[3] = {0.f, 0.f, 0.f};
+    for (int o = 0; o < 27; ++o) {
+      float B[9];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) B[e] = S[(int64_t)(o * 9 + e) * n + node];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          rs[i] += fabsf(M[3 * i] * B[j] + M[3 * i + 1] * B[3 + j] + M[3 * i + 2] * B[6 + j]);
+    }
+    best = fmaxf(best, fmaxf(rs[0], fmaxf(rs[1], rs[2])));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o, 64));
+  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = sm[0];
+    for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
+    part[blockIdx.x] = m;
+  }
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_max_final(const float* __restrict__ part, int np,
+                                                         float* __restrict__ out) {
+  __shared__ float sm[kMgBlock / kWave];
+  float best = 0.f;
+  for (int k = threadIdx.x; k < np; k += kMgBlock) best = fmaxf(best, part[k]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o, 64));
+  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = sm[0];
+    for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
+    out[0] = m;
+  }
+}
+
+// coarsest level: e = Ainv r (FP64 accumulation), one workgroup
+constexpr int kDenseApplyThreads = 128;
+__global__ void __launch_bounds__(kDenseApplyThreads) mg_dense_apply(const double* __restrict__ Ainv, int nd,
+                                                                     const float* __restrict__ r,
+                                                                     float* __restrict__ e, const Scalars* sc) {
+  if (sc && sc->done) return;
+  __shared__ double rs[kDenseMax];
+  const int tid = threadIdx.x;
+  if (tid < nd) rs[tid] = (double)r[tid];
+  __syncthreads();
+  if (tid < nd) {
+    double acc = 0.0;
+    for (int j = 0; j < nd; ++j) acc += Ainv[(int64_t)tid * nd + j] * rs[j];
+    e[tid] = (float)acc;
+  }
+}
+
+// The levels from the first one with at most kSmallNodes nodes down, in one
+// workgroup: the whole sub-cycle with workgroup barriers between its phases.
+// A stencil product uses 8 lanes per node (slots o = q, q+8, ... on lane q,
+// summed with shuffles), so a phase is one round of loads rather than 27.
+// The first level's input and pre-smoother start were written by the
+// restriction above it; its result is left in its e2 buffer (the
+// post-smoother start swaps e and e2 once).
+constexpr int kSmallThreads = 1024;
+constexpr int kGroup = 8;                             // lanes per node
+constexpr int kSmallNodes = kSmallThreads / kGroup;   // 128
+constexpr int kSmallMax = 4;
+struct SmallLevel {
+  Lat g;
+  const float* S;
+  const float* Dinv;
+  float *r, *e, *e2, *da, *db, *res;
+  float s0;
+  float c1[kMaxDegree], c2[kMaxDegree];
+};
+struct SmallCycle {
+  int nlev;     // levels here, the last one dense
+  int degree;
+  const double* Ainv;
+  int nd;
+  SmallLevel lv[kSmallMax];
+};
+
+// product of node n's stencil row with v, by the 8 lanes of a group (q = lane in group)
+__device__ __forceinline__ void stencil_row_group(const float* __restrict__ S, const Lat& g, int64_t n,
+                                                  const float* __restrict__ v, int q, float (&out)[3]) {
+  int x, y, z;
+  lat_xyz(g, n, x, y, z);
+  out[0] = out[1] = out[2] = 0.f;
+  for (int o = q; o < 27; o += kGroup) {
+    const int xx = clampi(x + o / 9 - 1, 0, g.nx - 1);
+    const int yy = clampi(y + (o / 3) % 3 - 1, 0, g.ny - 1);
+    const int zz = clampi(z + o % 3 - 1, 0, g.nz - 1);
+    const int64_t m = lat_id(g, xx, yy, zz) * 3;
+    const float v0 = v[m], v1 = v[m + 1], v2 = v[m + 2];
+    const float* sp = S + (int64_t)o * 9 * g.n + n;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      out[i] += sp[(3 * i) * g.n] * v0 + sp[(3 * i + 1) * g.n] * v1 + sp[(3 * i + 2) * g.n] * v2;
+  }
+#pragma unroll
+  for (int w = 1; w < kGroup; w <<= 1)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] += __shfl_xor(out[i], w, kWave);
+}
+
+#define MG_GROUP_LOOP(N, ...)                                                                           \
+  do {                                                                                                  \
+    const int q = threadIdx.x % kGroup;                                                                 \
+    for (int64_t node = threadIdx.x / kGroup; node < (N); node += kSmallThreads / kGroup) __VA_ARGS__;  \
+    __syncthreads();                                                                                    \
+  } while (0)
+#define MG_WG_LOOP(N, ...)                                                                   \
+  do {                                                                                       \
+    for (int64_t node = threadIdx.x; node < (N); node += kSmallThreads) __VA_ARGS__;         \
+    __syncthreads();                                                                         \
+  } while (0)
+
+__global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, const Scalars* sc) {
+  if (sc && sc->done) return;
+  const int L = sc_.nlev, k = sc_.degree;
+  float* cur[kSmallMax];
+  float* oth[kSmallMax];
+#pragma unroll
+  for (int l = 0; l < kSmallMax; ++l) {
+    cur[l] = sc_.lv[l].e;
+    oth[l] = sc_.lv[l].e2;
+  }
+  for (int l = 0; l + 1 < L; ++l) {
+    const SmallLevel& V = sc_.lv[l];
+    const float* res = V.r;
+    float* din = V.da;
+    float* dout = V.db;
+    for (int j = 1; j < k; ++j) {
+      MG_GROUP_LOOP(V.g.n, {
+        float ad[3];
+        stencil_row_group(V.S, V.g, node, din, q, ad);
+        if (q == 0) {
+          const float rr[3] = {res[3 * node] - ad[0], res[3 * node + 1] - ad[1], res[3 * node + 2] - ad[2]};
+          float dr[3];
+          block_apply(V.Dinv, V.g.n, node, rr, dr);
+          for (int i = 0; i < 3; ++i) {
+            const float dn = V.c1[j] * din[3 * node + i] + V.c2[j] * dr[i];
+            V.res[3 * node + i] = rr[i];
+            dout[3 * node + i] = dn;
+            cur[l][3 * node + i] += dn;
+          }
+        }
+      });
+      res = V.res;
+      float* tmp = din; din = dout; dout = tmp;
+    }
+    MG_GROUP_LOOP(V.g.n, {
+      float ad[3];
+      stencil_row_group(V.S, V.g, node, din, q, ad);
+      if (q == 0)
+        for (int i = 0; i < 3; ++i) dout[3 * node + i] = res[3 * node + i] - ad[i];
+    });
+    const SmallLevel& C = sc_.lv[l + 1];
+    const bool smooth_c = (l + 2 < L);
+    MG_WG_LOOP(C.g.n, restrict_node(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.s0, C.da, cur[l + 1], node));
+  }
+  {
+    __shared__ double rs[kDenseMax];
+    const SmallLevel& B = sc_.lv[L - 1];
+    const int tid = threadIdx.x;
+    if (tid < sc_.nd) rs[tid] = (double)B.r[tid];
+    __syncthreads();
+    if (tid < sc_.nd) {
+      double acc = 0.0;
+      for (int j = 0; j < sc_.nd; ++j) acc += sc_.Ainv[(int64_t)tid * sc_.nd + j] * rs[j];
+      cur[L - 1][tid] = (float)acc;
+    }
+    __syncthreads();
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    const SmallLevel& V = sc_.lv[l];
+    const SmallLevel& C = sc_.lv[l + 1];
+    MG_WG_LOOP(V.g.n, prolong_node(cur[l + 1], V.g, C.g, nullptr, cur[l], node));
+    float* ein = cur[l];
+    float* eout = oth[l];
+    MG_GROUP_LOOP(V.g.n, {
+      float ae[3];
+      stencil_row_group(V.S, V.g, node, ein, q, ae);
+      if (q == 0) {
+        const float rr[3] = {V.r[3 * node] - ae[0], V.r[3 * node + 1] - ae[1], V.r[3 * node + 2] - ae[2]};
+        float dr[3];
+        block_apply(V.Dinv, V.g.n, node, rr, dr);
+        for (int i = 0; i < 3; ++i) {
+          V.res[3 * node + i] = rr[i];
+          V.da[3 * node + i] = V.s0 * dr[i];
+          eout[3 * node + i] = ein[3 * node + i] + V.s0 * dr[i];
+        }
+      }
+    });
+    cur[l] = eout;
+    oth[l] = ein;
+    float* din = V.da;
+    float* dout = V.db;
+    for (int j = 1; j < k; ++j) {
+      MG_GROUP_LOOP(V.g.n, {
+        float ad[3];
+        stencil_row_group(V.S, V.g, node, din, q, ad);
+        if (q == 0) {
+          const float rr[3] = {V.res[3 * node] - ad[0], V.res[3 * node + 1] - ad[1], V.res[3 * node + 2] - ad[2]};
+          float dr[3];
+          block_apply(V.Dinv, V.g.n, node, rr, dr);
+          for (int i = 0; i < 3; ++i) {
+            const float dn = V.c1[j] * din[3 * node + i] + V.c2[j] * dr[i];
+            V.res[3 * node + i] = rr[i];
+            dout[3 * node + i] = dn;
+            eout[3 * node + i] += dn;
+          }
+        }
+      });
+      float* t2 = din; din = dout; dout = t2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// outer flexible CG kernels
+
+// workspace: scalars, partials A (<= 2 per workgroup) and B (<= 3), then the vectors
+constexpr int kPartB = 2 * kMaxGrid;
+constexpr size_t kMgHeader = 4096 + (size_t)5 * kMaxGrid * sizeof(double);
+
+// r = b - A x, p = q = 0; partials (A) r.r, b.b
+__global__ void __launch_bounds__(kBlock) mgcg_init_kernel(MatArgs m, const float* __restrict__ vals,
+                                                           const float* __restrict__ b, const float* __restrict__ x,
+                                                           float* __restrict__ r, float* __restrict__ p,
+                                                           float* __restrict__ q, double* partials) {
+  int64_t r0, r1;
+  row_range(m.nslices, r0, r1);
+  auto xv = [&](int64_t c) { return x[c]; };
+  double loc[2] = {0, 0};
+  auto epi = [&](int64_t n, int i, float ax) {
+    const int64_t d = n * 3 + i;
+    const float rr = b[d] - ax;
+    r[d] = rr;
+    p[d] = 0.f;
+    q[d] = 0.f;
+    loc[0] += (double)rr * (double)rr;
+    loc[1] += (double)b[d] * (double)b[d];
+  };
+  sweep<float, 3, false>(m, r0, r1, vals, xv, epi);
+  block_partials<2>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kScalarBlock) mgcg_init_scalars(Scalars* sc, const double* partA, int nA,
+                                                                  double rtol2, double atol2, int rel_r0,
+                                                                  int maxiter) {
+  double a[2];
+  sum_partials<2>(partA, nA, a);
+  if (threadIdx.x != 0) return;
+  sc->rr = a[0];
+  sc->bb = a[1];
+  sc->thresh = fmax(rtol2 * (rel_r0 ? a[0] : a[1]), atol2);
+  sc->iter = 0;
+  sc->status = 0;
+  sc->maxiter = maxiter;
+  sc->alpha = 0.0;
+  sc->omega = 0.0;
+  sc->pq = 0.0;
+  sc->rho = 0.0;
+  sc->done = (a[0] <= sc->thresh) ? 1 : 0;
+  if (!isfinite(a[0])) { sc->done = 1; sc->status = 3; }
+  else if (!sc->done && maxiter <= 0) { sc->done = 1; sc->status = 1; }
+}
+
+// w = A z; partials (B) r.z, z.w, z.q
+__global__ void __launch_bounds__(kBlock) mgcg_spmv_kernel(MatArgs m, const float* __restrict__ vals,
+                                                           const float* __restrict__ z, const float* __restrict__ r,
+                                                           const float* __restrict__ q, float* __restrict__ w,
+                                                           const Scalars* sc, double* partials) {
+  if (sc->done) return;
+  int64_t r0, r1;
+  row_range(m.nslices, r0, r1);
+  auto xv = [&](int64_t c) { return z[c]; };
+  double loc[3] = {0, 0, 0};
+  auto epi = [&](int64_t n, int i, float v) {
+    const int64_t d = n * 3 + i;
+    w[d] = v;
+    const double zz = (double)z[d];
+    loc[0] += (double)r[d] * zz;
+    loc[1] += zz * (double)v;
+    loc[2] += zz * (double)q[d];
+  };
+  sweep<float, 3, false>(m, r0, r1, vals, xv, epi);
+  block_partials<3>(loc, partials);
+}
+
+// beta, p.q and alpha of the new direction
+__global__ void __launch_bounds__(kScalarBlock) mgcg_dir_scalars(Scalars* sc, const double* partB, int nB) {
+  if (sc->done) return;
+  double d[3];
+  sum_partials<3>(partB, nB, d);
+  if (threadIdx.x != 0) return;
+  const double gamma = d[0], delta = d[1], zeta = d[2];
+  double beta = 0.0, pq = delta;
+  if (sc->iter > 0) {
+    beta = -zeta / sc->pq;
+    pq = delta + 2.0 * beta * zeta + beta * beta * sc->pq;
+  }
+  if (!(pq > 0.0) || !isfinite(pq) || !isfinite(gamma)) {
+    sc->done = 1;
+    sc->status = (isfinite(pq) && isfinite(gamma)) ? 2 : 3;
+    return;
+  }
+  sc->omega = beta;
+  sc->pq = pq;
+  sc->rho = gamma;
+  sc->alpha = gamma / pq;
+}
+
+// p = z + beta p, q = w + beta q, x += alpha p, r -= alpha q; partial (A) r.r
+__global__ void __launch_bounds__(kBlock) mgcg_update_kernel(int64_t N, const float* __restrict__ z,
+                                                             const float* __restrict__ w, float* __restrict__ p,
+                                                             float* __restrict__ q, float* __restrict__ x,
+                                                             float* __restrict__ r, const Scalars* sc,
+                                                             double* partials) {
+  if (sc->done) return;
+  const float alpha = (float)sc->alpha, beta = (float)sc->omega;
+  const int64_t per = ceil_div(N, gridDim.x);
+  const int64_t i0 = (int64_t)logical_block() * per;
+  const int64_t i1 = i0 + per < N ? i0 + per : N;
+  double loc[1] = {0};
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
+    const float pp = z[i] + beta * p[i];
+    const float qq = w[i] + beta * q[i];
+    p[i] = pp;
+    q[i] = qq;
+    x[i] += alpha * pp;
+    const float rr = r[i] - alpha * qq;
+    r[i] = rr;
+    loc[0] += (double)rr * (double)rr;
+  }
+  block_partials<1>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kScalarBlock) mgcg_conv_scalars(Scalars* sc, const double* partA, int nA) {
+  if (sc->done) return;
+  double a[1];
+  sum_partials<1>(partA, nA, a);
+  if (threadIdx.x != 0) return;
+  sc->iter += 1;
+  sc->rr = a[0];
+  if (!isfinite(a[0])) { sc->done = 1; sc->status = 3; }
+  else if (a[0] <= sc->thresh) { sc->done = 1; sc->status = 0; }
+  else if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; }
+}
+
+}  // namespace
+}  // namespace hf
+
+// ---------------------------------------------------------------------------
+// host side
+using namespace hf;
+
+struct MgLevel {
+  Lat g;
+  float* S = nullptr;     // 243 n
+  float* Dinv = nullptr;  // 9 n
+  float* r = nullptr;     // 3 n (levels >= 1: the restricted residual)
+  float* e = nullptr;     // 3 n
+  float* e2 = nullptr;    // 3 n
+  float* da = nullptr;    // 3 n smoother directions (two, alternating)
+  float* db = nullptr;    // 3 n
+  float* res = nullptr;   // 3 n smoother residual / restriction input
+  float lam = 0.f;        // Gershgorin bound of Dinv A (last set-up)
+  float s0 = 0.f;         // polynomial coefficients (see mg_poly_step)
+  float c1[kMaxDegree] = {0};
+  float c2[kMaxDegree] = {0};
+};
+
+struct hf_mg {
+  int nlev = 0;
+  MgLevel lv[kMgMaxLevels];
+  float* B = nullptr;        // A P of the finest pair, reused by every level
+  double* Ainv = nullptr;    // coarsest dense inverse
+  int nd = 0;
+  int32_t* flags = nullptr;  // [0] pattern not on the lattice, [1] dropped coarse pivots
+  float* part = nullptr;     // kMaxGrid per-workgroup maxima + kMgMaxLevels bounds
+  int kind = HF_MG_CHEBYSHEV;
+  int degree = 2;
+  float param = 8.f;         // Chebyshev: lam / lower end of the interval; Jacobi: omega
+  const float* mask = nullptr;  // level-0 Dirichlet mask of the last set-up (caller's memory)
+  const hf_pattern* checked = nullptr;  // pattern whose lattice structure was verified
+  int64_t checked_nblocks = -1;
+  bool ready = false;
+};
+
+static int mg_free(hf_mg* mg) {
+  for (int l = 0; l < mg->nlev; ++l) {
+    MgLevel& L = mg->lv[l];
+    float* bufs[] = {L.S, L.Dinv, L.r, L.e, L.e2, L.da, L.db, L.res};
+    for (float* b : bufs)
+      if (b) (void)hipFree(b);
+  }
+  if (mg->B) (void)hipFree(mg->B);
+  if (mg->Ainv) (void)hipFree(mg->Ainv);
+  if (mg->flags) (void)hipFree(mg->flags);
+  if (mg->part) (void)hipFree(mg->part);
+  return 0;
+}
+
+extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, hf_mg** out) {
+  HF_REQUIRE(out, "hf_mg_create: null output");
+  HF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "hf_mg_create: lattice %dx%dx%d needs >= 2 nodes per dimension", nx,
+             ny, nz);
+  HF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 30, "hf_mg_create: lattice too large");
+  // level dimensions first (no allocation until they are known)
+  int dims[kMgMaxLevels][3];
+  int nlev = 0;
+  int a = nx, b = ny, c = nz;
+  for (;;) {
+    HF_REQUIRE(nlev < kMgMaxLevels, "hf_mg_create: too many levels");
+    dims[nlev][0] = a; dims[nlev][1] = b; dims[nlev][2] = c;
+    ++nlev;
+    if (3 * (int64_t)a * b * c <= kDenseMax) break;
+    // intervals n -> ceil(n/2): nodes a -> a/2 + 1 (a lattice of 2 nodes stays)
+    const int na = a / 2 + 1, nb = b / 2 + 1, nc = c / 2 + 1;
+    HF_REQUIRE(!(na == a && nb == b && nc == c), "hf_mg_create: cannot coarsen %dx%dx%d below %d DOF", a, b, c,
+               kDenseMax);
+    a = na; b = nb; c = nc;
+  }
+  hf_mg* mg = new hf_mg();
+  mg->nlev = nlev;
+  for (int l = 0; l < nlev; ++l) mg->lv[l].g = make_lat(dims[l][0], dims[l][1], dims[l][2]);
+  auto alloc = [&](float** p, int64_t n) -> int {
+    HF_CHECK_HIP(hipMalloc((void**)p, (size_t)n * sizeof(float)));
+    return 0;
+  };
+  int rc = 0;
+  for (int l = 0; l < mg->nlev && !rc; ++l) {
+    MgLevel& L = mg->lv[l];
+    const int64_t n = L.g.n;
+    rc |= alloc(&L.S, kSt * n);
+    if (l + 1 < mg->nlev) rc |= alloc(&L.Dinv, 9 * n) | alloc(&L.da, 3 * n) | alloc(&L.db, 3 * n) |
+                                alloc(&L.res, 3 * n);
+    if (l > 0) rc |= alloc(&L.r, 3 * n);
+    rc |= alloc(&L.e, 3 * n) | alloc(&L.e2, 3 * n);
+  }
+  if (!rc && mg->nlev > 1) rc |= alloc(&mg->B, kSt * mg->lv[0].g.n);
+  if (!rc) rc |= alloc(&mg->part, kMaxGrid + kMgMaxLevels);
+  mg->nd = (int)(3 * mg->lv[mg->nlev - 1].g.n);
+  if (!rc) {
+    hipError_t e1 = hipMalloc((void**)&mg->Ainv, (size_t)mg->nd * mg->nd * sizeof(double));
+    hipError_t e2 = hipMalloc((void**)&mg->flags, 4 * sizeof(int32_t));
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      set_error("hf_mg_create: hipMalloc failed");
+      rc = -2;
+    }
+  }
+  if (rc) {
+    mg_free(mg);
+    delete mg;
+    return rc;
+  }
+  *out = mg;
+  return 0;
+}
+
+extern "C" int hf_mg_destroy(hf_mg* mg) {
+  if (!mg) return 0;
+  mg_free(mg);
+  delete mg;
+  return 0;
+}
+
+extern "C" int hf_mg_levels(const hf_mg* mg, int32_t* nlev, int32_t* dims, int32_t max_levels) {
+  HF_REQUIRE(mg && nlev, "hf_mg_levels: null argument");
+  *nlev = mg->nlev;
+  if (dims)
+    for (int l = 0; l < mg->nlev && l < max_levels; ++l) {
+      dims[3 * l] = mg->lv[l].g.nx;
+      dims[3 * l + 1] = mg->lv[l].g.ny;
+      dims[3 * l + 2] = mg->lv[l].g.nz;
+    }
+  return 0;
+}
+
+// Polynomial coefficients of every level from its bound lam.
+static void mg_coefficients(hf_mg* mg) {
+  for (int l = 0; l + 1 < mg->nlev; ++l) {
+    MgLevel& L = mg->lv[l];
+    if (mg->kind == HF_MG_JACOBI) {
+      L.s0 = mg->param;
+      for (int j = 0; j < kMaxDegree; ++j) { L.c1[j] = 0.f; L.c2[j] = mg->param; }
+      continue;
+    }
+    // Chebyshev on [lam / param, lam]
+    const double hi = L.lam > 0.f ? (double)L.lam : 1.0, lo = hi / mg->param;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    L.s0 = (float)(1.0 / theta);
+    for (int j = 1; j < kMaxDegree; ++j) {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      L.c1[j] = (float)(rn * rho);
+      L.c2[j] = (float)(2.0 * rn / delta);
+      rho = rn;
+    }
+  }
+}
+
+extern "C" int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float param) {
+  HF_REQUIRE(mg, "hf_mg_set_smoother: null handle");
+  HF_REQUIRE(kind == HF_MG_JACOBI || kind == HF_MG_CHEBYSHEV, "hf_mg_set_smoother: unknown kind %d", kind);
+  HF_REQUIRE(degree >= 1 && degree <= kMaxDegree, "hf_mg_set_smoother: degree %d outside 1..%d", degree, kMaxDegree);
+  if (kind == HF_MG_JACOBI)
+    HF_REQUIRE(param > 0.f && param < 2.f, "hf_mg_set_smoother: omega %g outside (0, 2)", (double)param);
+  else
+    HF_REQUIRE(param > 1.f, "hf_mg_set_smoother: interval ratio %g must exceed 1", (double)param);
+  mg->kind = kind;
+  mg->degree = degree;
+  mg->param = param;
+  mg_coefficients(mg);
+  return 0;
+}
+
+extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* mask,
+                               int32_t* dropped, hf_stream_t stream) {
+  HF_REQUIRE(mg && p && vals && mask, "hf_mg_setup_f32: null argument");
+  HF_REQUIRE(pattern_vec(p) == 3, "hf_mg_setup_f32: vec=%d, the multigrid is built for vec=3", pattern_vec(p));
+  const Lat g0 = mg->lv[0].g;
+  HF_REQUIRE(pattern_nnode(p) == g0.n, "hf_mg_setup_f32: pattern has %lld nodes, lattice %lld",
+             (long long)pattern_nnode(p), (long long)g0.n);
+  hipStream_t s = (hipStream_t)stream;
+  mg->ready = false;
+  HF_CHECK_HIP(hipMemsetAsync(mg->flags, 0, 4 * sizeof(int32_t), s));
+  const unsigned gb0 = (unsigned)ceil_div(g0.n, kMgBlock);
+  const bool check = !(mg->checked == p && mg->checked_nblocks == pattern_nblocks(p));
+  if (check)
+    mg_check_lattice<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p),
+                                                        pattern_sell_pos(p), pattern_slice_ptr(p), pattern_R(p), g0,
+                                                        mg->flags);
+  mg_fine_stencil<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p), pattern_slice_ptr(p),
+                                                     pattern_R(p), vals, mask, g0, mg->lv[0].S);
+  float* lam_dev = mg->part + kMaxGrid;
+  for (int l = 0; l + 1 < mg->nlev; ++l) {
+    const Lat f = mg->lv[l].g, c = mg->lv[l + 1].g;
+    mg_block_dinv<<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(mg->lv[l].S, l == 0 ? mask : nullptr, f.n,
+                                                                         mg->lv[l].Dinv);
+    const int gg = (int)std::min<int64_t>(ceil_div(f.n, kMgBlock), kMaxGrid);
+    mg_gersh<<<gg, kMgBlock, 0, s>>>(mg->lv[l].S, mg->lv[l].Dinv, f.n, mg->part);
+    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gg, lam_dev + l);
+    mg_rap_ap<<<dim3((unsigned)ceil_div(f.n, kMgBlock), 27), kMgBlock, 0, s>>>(mg->lv[l].S, f, c, mg->B);
+    mg_rap_ptap<<<dim3((unsigned)ceil_div(c.n, kMgBlock), 27), kMgBlock, 0, s>>>(mg->B, f, c, mg->lv[l + 1].S);
+  }
+  const Lat gl = mg->lv[mg->nlev - 1].g;
+  mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
+  HF_LAUNCH_CHECK();
+  int32_t hflags[4] = {0, 0, 0, 0};
+  float lam[kMgMaxLevels] = {0};
+  HF_CHECK_HIP(hipMemcpyAsync(hflags, mg->flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
+  if (mg->nlev > 1)
+    HF_CHECK_HIP(hipMemcpyAsync(lam, lam_dev, (mg->nlev - 1) * sizeof(float), hipMemcpyDeviceToHost, s));
+  HF_CHECK_HIP(hipStreamSynchronize(s));
+  HF_REQUIRE(hflags[0] == 0, "hf_mg_setup_f32: the pattern couples nodes that are not lattice neighbours "
+                             "(the multigrid needs a HEX8 box lattice)");
+  mg->checked = p;
+  mg->checked_nblocks = pattern_nblocks(p);
+  for (int l = 0; l + 1 < mg->nlev; ++l) {
+    HF_REQUIRE(std::isfinite(lam[l]), "hf_mg_setup_f32: non-finite operator on level %d", l);
+    mg->lv[l].lam = lam[l];
+  }
+  mg_coefficients(mg);
+  if (dropped) *dropped = hflags[1];
+  mg->mask = mask;
+  mg->ready = true;
+  return 0;
+}
+
+extern "C" int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels) {
+  HF_REQUIRE(mg && lam, "hf_mg_bounds: null argument");
+  for (int l = 0; l < mg->nlev && l < max_levels; ++l) lam[l] = mg->lv[l].lam;
+  return 0;
+}
+
+// Enqueue one V-cycle: level-0 input r0, result in *out (one of the two level-0
+// buffers z0 / z1 given by the caller).  Levels with more than kSmallNodes
+// nodes run as grid kernels; the rest (from level lb down) in mg_small_cycle.
+static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out, const Scalars* sc, hipStream_t s) {
+  const int L = mg->nlev;
+  const int k = mg->degree;
+  float* cur[kMgMaxLevels];
+  float* oth[kMgMaxLevels];
+  for (int l = 0; l < L; ++l) {
+    cur[l] = l == 0 ? z0 : mg->lv[l].e;
+    oth[l] = l == 0 ? z1 : mg->lv[l].e2;
+  }
+  int lb = L - 1;
+  for (int l = 1; l < L; ++l)
+    if (mg->lv[l].g.n <= kSmallNodes && L - l <= kSmallMax) { lb = l; break; }
+  if (L == 1) lb = 0;
+  auto rin = [&](int l) -> const float* { return l == 0 ? r0 : mg->lv[l].r; };
+  for (int l = 0; l < lb; ++l) {
+    const MgLevel& V = mg->lv[l];
+    const int gg = node_grid(V.g.n);
+    // pre-smoothing from e = 0 (its start on levels >= 1 is fused into the
+    // restriction that produced rin(l)), then the residual for the coarse level
+    if (l == 0) mg_poly_first<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.s0, rin(l), V.da, cur[l], sc);
+    const float* res = rin(l);
+    float* din = V.da;
+    float* dout = V.db;
+    for (int j = 1; j < k; ++j) {
+      mg_poly_step<<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], res, din, V.res, dout, cur[l], sc);
+      res = V.res;
+      float* tmp = din; din = dout; dout = tmp;
+    }
+    mg_poly_resid<<<gg, kMgBlock, 0, s>>>(V.S, V.g, res, din, dout, sc);
+    const MgLevel& C = mg->lv[l + 1];
+    const bool smooth_c = (l + 2 < L);
+    mg_restrict<<<node_grid(C.g.n), kMgBlock, 0, s>>>(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.s0,
+                                                      C.da, cur[l + 1], sc);
+  }
+  if (L == 1) {
+    mg_dense_apply<<<1, kDenseApplyThreads, 0, s>>>(mg->Ainv, mg->nd, rin(0), cur[0], sc);
+  } else {
+    SmallCycle c;
+    c.nlev = L - lb;
+    c.degree = k;
+    c.Ainv = mg->Ainv;
+    c.nd = mg->nd;
+    for (int l = lb; l < L; ++l) {
+      const MgLevel& V = mg->lv[l];
+      SmallLevel& q = c.lv[l - lb];
+      q.g = V.g; q.S = V.S; q.Dinv = V.Dinv;
+      q.r = V.r; q.e = V.e; q.e2 = V.e2; q.da = V.da; q.db = V.db; q.res = V.res;
+      q.s0 = V.s0;
+      for (int j = 0; j < kMaxDegree; ++j) { q.c1[j] = V.c1[j]; q.c2[j] = V.c2[j]; }
+    }
+    mg_small_cycle<<<1, kSmallThreads, 0, s>>>(c, sc);
+    cur[lb] = (L - lb >= 2) ? mg->lv[lb].e2 : mg->lv[lb].e;
+  }
+  for (int l = lb - 1; l >= 0; --l) {
+    const MgLevel& V = mg->lv[l];
+    const MgLevel& C = mg->lv[l + 1];
+    const int gg = node_grid(V.g.n);
+    mg_prolong<<<gg, kMgBlock, 0, s>>>(cur[l + 1], V.g, C.g, l == 0 ? mg->mask : nullptr, cur[l], sc);
+    // post-smoothing: the same polynomial on the residual after the correction
+    mg_post_first<<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.s0, rin(l), cur[l], V.res, V.da, oth[l], sc);
+    float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
+    float* din = V.da;
+    float* dout = V.db;
+    for (int j = 1; j < k; ++j) {
+      mg_poly_step<<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], V.res, din, V.res, dout, cur[l], sc);
+      float* t2 = din; din = dout; dout = t2;
+    }
+  }
+  HF_LAUNCH_CHECK();
+  *out = cur[0];
+  return 0;
+}
+
+extern "C" int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, hf_stream_t stream) {
+  HF_REQUIRE(mg && r && z, "hf_mg_apply_f32: null argument");
+  HF_REQUIRE(mg->ready, "hf_mg_apply_f32: hf_mg_setup_f32 has not run");
+  hipStream_t s = (hipStream_t)stream;
+  float* res = nullptr;
+  if (vcycle(mg, r, z, mg->lv[0].e2, &res, nullptr, s)) return -2;
+  if (res != z) HF_CHECK_HIP(hipMemcpyAsync(z, res, (size_t)mg->lv[0].g.n * 3 * sizeof(float),
+                                            hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+extern "C" int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf_stream_t stream) {
+  HF_REQUIRE(mg && dst, "hf_mg_copy_stencil: null argument");
+  HF_REQUIRE(level >= 0 && level < mg->nlev, "hf_mg_copy_stencil: level %d outside 0..%d", level, mg->nlev - 1);
+  HF_REQUIRE(mg->ready, "hf_mg_copy_stencil: hf_mg_setup_f32 has not run");
+  HF_CHECK_HIP(hipMemcpyAsync(dst, mg->lv[level].S, (size_t)kSt * mg->lv[level].g.n * sizeof(float),
+                              hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+
+extern "C" size_t hf_mg_workspace_size(const hf_mg* mg) {
+  if (!mg) return 0;
+  return kMgHeader + 6 * align256((size_t)mg->lv[0].g.n * 3 * sizeof(float));
+}
+
+extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* b, float* x,
+                               int32_t flags, int32_t maxiter, double rtol, double atol, int32_t poll, void* ws,
+                               hf_krylov_info* info, hf_stream_t stream) {
+  HF_REQUIRE(mg && p && vals && b && x && ws, "hf_mg_solve_f32: null argument");
+  HF_REQUIRE(mg->ready, "hf_mg_solve_f32: hf_mg_setup_f32 has not run");
+  HF_REQUIRE(pattern_vec(p) == 3 && pattern_nnode(p) == mg->lv[0].g.n, "hf_mg_solve_f32: pattern does not match");
+  HF_REQUIRE(!(flags & HF_KRYLOV_TRANSPOSE), "hf_mg_solve_f32: transposed solves are not supported");
+  HF_REQUIRE(maxiter >= 0, "hf_mg_solve_f32: maxiter < 0");
+  if (poll < 1) poll = 1;
+  hipStream_t s = (hipStream_t)stream;
+  const int rel_r0 = (flags & HF_KRYLOV_REL_R0) ? 1 : 0;
+  const MatArgs m = mat_args(p);
+  const int64_t N = m.nrows * 3;
+  Scalars* sc = (Scalars*)ws;
+  double* partA = (double*)((char*)ws + 4096);
+  double* partB = partA + kPartB;
+  float* v = (float*)((char*)ws + kMgHeader);
+  const size_t stride = align256(N * sizeof(float)) / sizeof(float);
+  float *r = v, *z0 = v + stride, *z1 = v + 2 * stride, *w = v + 3 * stride, *pv = v + 4 * stride,
+        *q = v + 5 * stride;
+  const int gr = rows_grid(m.nslices), ge = elem_grid(N);
+  mgcg_init_kernel<<<gr, kBlock, 0, s>>>(m, vals, b, x, r, pv, q, partA);
+  mgcg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
+  float* z = nullptr;
+  if (vcycle(mg, r, z0, z1, &z, sc, s)) return -2;
+  mgcg_spmv_kernel<<<gr, kBlock, 0, s>>>(m, vals, z, r, q, w, sc, partB);
+  mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(sc, partB, gr);
+  HF_LAUNCH_CHECK();
+  if (poll_init()) return -2;
+  Scalars host{};
+  int it = 0, batch = 0;
+  for (;;) {
+    for (int k = 0; k < poll; ++k, ++it) {
+      mgcg_update_kernel<<<ge, kBlock, 0, s>>>(N, z, w, pv, q, x, r, sc, partA);
+      mgcg_conv_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, ge);
+      if (vcycle(mg, r, z0, z1, &z, sc, s)) return -2;
+      mgcg_spmv_kernel<<<gr, kBlock, 0, s>>>(m, vals, z, r, q, w, sc, partB);
+      mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(sc, partB, gr);
+    }
+    HF_LAUNCH_CHECK();
+    if (poll_post(sc, batch & 1, s)) return -2;
+    if (batch > 0) {
+      if (poll_wait((batch - 1) & 1, host)) return -2;
+      if (host.done) break;
+    }
+    ++batch;
+    if (it >= maxiter + 2 * poll) {
+      if (poll_wait((batch - 1) & 1, host)) return -2;
+      break;
+    }
+  }
+  HF_CHECK_HIP(hipStreamSynchronize(s));
+  HF_CHECK_HIP(hipMemcpy(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+  if (info) {
+    info->iterations = host.iter;
+    info->status = host.done ? host.status : 1;
+    info->res_norm = sqrt(host.rr);
+    info->b_norm = sqrt(host.bb);
+  }
+  return 0;
+}

@@ -1,0 +1,233 @@
+"""Geometric multigrid preconditioner on the structured node lattice (GPU, libhipfem).
+
+No reference counterpart: the reference's Krylov solves use Jacobi only
+(jax_fem/solver.py:68-69).  The hierarchy, the V-cycle and the flexible CG are
+csrc/mg.hip; this module owns the device object, builds the Dirichlet mask,
+re-runs the set-up when the tangent changes and implements the fallback.
+
+    level 0      the FP32 tangent with Dirichlet DOFs masked out (M A M)
+    level l+1    P^T A_l P, P trilinear interpolation from the lattice with
+                 every dimension's interval count halved (rounded up)
+    coarsest     <= 88 DOF, dense FP64 inverse
+    cycle        V-cycle, `degree` smoothing steps per side: Chebyshev in Dinv A on
+                 [lam/ratio, lam], lam the Gershgorin bound of Dinv A (Dinv: inverse
+                 3x3 diagonal blocks), or damped block Jacobi (omega)
+    outer        flexible CG (one kept direction), FP64 dot products
+
+Used through ``solver_options={'newton': {'precision': 'mixed', 'linear':
+{'mg': {...}}}}`` (solver.linear_solver) or directly:
+
+    mg = Multigrid.for_problem(problem)
+    x, info = mg.solve(A, b, x0, rtol=1e-8)
+"""
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def lattice(problem):
+    """Node counts (nx, ny, nz) of a structured box_mesh HEX8 problem with vec = 3."""
+    g = getattr(problem.fe.mesh, "node_grid", None)
+    if g is None or int(np.prod(g)) != problem.fe.num_total_nodes or problem.vec != 3 or \
+            problem.fe.ele_type != "HEX8":
+        raise NotImplementedError("the multigrid preconditioner needs a structured box_mesh HEX8 lattice, vec=3")
+    return tuple(int(x) for x in g)
+
+
+def dirichlet_mask(problem):
+    """1 on free DOFs, 0 on Dirichlet DOFs (float32 device, N)."""
+    m = torch.ones(problem.N, dtype=torch.float32, device=problem.device)
+    d, _ = problem.bc_arrays()
+    m[d.long()] = 0.0
+    return m
+
+
+class Multigrid:
+    def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None):
+        _lib.load()
+        self.grid = tuple(int(v) for v in grid)
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        h = ctypes.c_void_p()
+        _lib.call("hf_mg_create", self.grid[0], self.grid[1], self.grid[2], ctypes.byref(h))
+        self.handle = h
+        self.smoother_cfg = None
+        self.set_smoother(smoother, degree, param)
+        self._tangent = None     # the matrix values and mask the hierarchy was built from (kept alive)
+        self._mask = None
+        self._pattern = None
+        self.dropped = 0
+        self.setup_seconds = 0.0
+        self.setups = 0
+        self.last = None
+        self._token = None       # the solve sequence (Newton solve) the hierarchy was built in
+        self._iters_at_setup = None
+        self._stale = False
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None
+        if h is not None and lib is not None:
+            lib.hf_mg_destroy(h)
+            self.handle = None
+
+    @classmethod
+    def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None):
+        """The problem's cached hierarchy (one per lattice) with the given smoother."""
+        grid = lattice(problem)
+        mg = getattr(problem, "_mg", None)
+        if mg is None or mg.grid != grid:
+            mg = cls(grid, smoother, degree, param, device=problem.device)
+            problem._mg = mg
+        else:
+            mg.set_smoother(smoother, degree, param)
+        return mg
+
+    def set_smoother(self, smoother="chebyshev", degree=2, param=None):
+        """smoother 'chebyshev' (param: interval ratio, default 8) or 'jacobi' (param: omega,
+        default 0.5); degree: steps per side."""
+        kinds = {"jacobi": _lib.HF_MG_JACOBI, "chebyshev": _lib.HF_MG_CHEBYSHEV}
+        if smoother not in kinds:
+            raise ValueError("smoother must be 'chebyshev' or 'jacobi'")
+        if param is None:
+            param = 8.0 if smoother == "chebyshev" else 0.5
+        cfg = (smoother, int(degree), float(param))
+        if cfg != self.smoother_cfg:
+            _lib.call("hf_mg_set_smoother", self.handle, kinds[smoother], int(degree), ctypes.c_float(param))
+            self.smoother_cfg = cfg
+
+    def bounds(self):
+        """Gershgorin bound of Dinv A per smoothed level (last set-up)."""
+        lam = (ctypes.c_float * 16)()
+        _lib.call("hf_mg_bounds", self.handle, lam, 16)
+        return [lam[i] for i in range(len(self.levels()) - 1)]
+
+    def levels(self):
+        """[(nx, ny, nz), ...] finest first."""
+        n = ctypes.c_int32()
+        dims = (ctypes.c_int32 * 48)()
+        _lib.call("hf_mg_levels", self.handle, ctypes.byref(n), dims, 16)
+        return [tuple(dims[3 * i:3 * i + 3]) for i in range(n.value)]
+
+    def setup(self, A, mask):
+        """Build every level from the FP32 TangentMatrix A and the Dirichlet mask."""
+        if A.dtype != torch.float32:
+            raise ValueError("the multigrid preconditioner runs on the FP32 tangent")
+        mask = mask.to(torch.float32).contiguous()
+        dropped = ctypes.c_int32()
+        t0 = time.perf_counter()
+        _lib.call("hf_mg_setup_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(mask),
+                  ctypes.byref(dropped), _lib.stream())
+        self.setup_seconds += time.perf_counter() - t0
+        self._tangent = A.vals      # identity of the tangent the levels come from
+        self._mask = mask           # the cycle reads the mask by pointer
+        self._pattern = A.pattern
+        self.dropped = int(dropped.value)
+        self.setups += 1
+        self._iters_at_setup = None
+        self._stale = False
+
+    def is_current(self, A):
+        return self._tangent is A.vals
+
+    def needs_setup(self, A, rebuild, token):
+        """rebuild: 'always' (every tangent), 'solve' (once per Newton solve, identified
+        by token) or 'never' (the first tangent's hierarchy for every later one).  A
+        hierarchy is also rebuilt for a new pattern, or when a solve with it took more
+        than max(2 n0, n0 + 10) iterations, n0 the count of the first solve after its set-up."""
+        if self._tangent is None or self._pattern is not A.pattern or self._stale:
+            return True
+        if rebuild == "always":
+            return not self.is_current(A)
+        if rebuild == "solve":
+            return token is not self._token
+        if rebuild == "never":
+            return False
+        raise ValueError("rebuild must be 'always', 'solve' or 'never'")
+
+    def note_iterations(self, n):
+        if self._iters_at_setup is None:
+            self._iters_at_setup = n
+        elif n > max(2 * self._iters_at_setup, self._iters_at_setup + 10):
+            self._stale = True
+
+    def apply(self, r):
+        """z = V(r) (one cycle)."""
+        r = r.to(torch.float32).contiguous()
+        z = torch.empty_like(r)
+        _lib.call("hf_mg_apply_f32", self.handle, _lib.ptr(r), _lib.ptr(z), _lib.stream())
+        return z
+
+    def stencil(self, level):
+        """Level `level`'s stencil as a (27, 3, 3, nnode) float32 tensor: [o, i, j, n] couples
+        DOF i of node n to DOF j of its neighbour at offset o = (dx+1)*9 + (dy+1)*3 + (dz+1)."""
+        nx, ny, nz = self.levels()[level]
+        n = nx * ny * nz
+        out = torch.empty(243 * n, dtype=torch.float32, device=self.device)
+        _lib.call("hf_mg_copy_stencil", self.handle, int(level), _lib.ptr(out), _lib.stream())
+        return out.reshape(27, 3, 3, n)
+
+    def solve(self, A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=500, rel_r0=False, poll=4, stats=None,
+              mask=None, problem=None, rebuild="always", token=None):
+        """Flexible CG preconditioned by V-cycles; the operator is always the FP32
+        tangent A, the hierarchy is rebuilt from A as `rebuild` says (needs_setup).
+        Returns (x float32, info)."""
+        if self.needs_setup(A, rebuild, token):
+            self.setup(A, dirichlet_mask(problem or A.problem) if mask is None else mask)
+            self._token = token
+        N = A.shape[0]
+        b = b.to(torch.float32).contiguous()
+        x = (torch.zeros(N, dtype=torch.float32, device=b.device) if x0 is None else
+             x0.to(torch.float32).clone()).contiguous()
+        ws = _workspace(b.device, int(_lib.load().hf_mg_workspace_size(self.handle)))
+        info = _lib.HFKrylovInfo()
+        t0 = time.perf_counter()
+        _lib.call("hf_mg_solve_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(b), _lib.ptr(x),
+                  2 if rel_r0 else 0, int(maxiter), float(rtol), float(atol), int(poll), _lib.ptr(ws),
+                  ctypes.byref(info), _lib.stream())
+        self.last = (int(info.iterations), int(info.status))
+        self.note_iterations(int(info.iterations))
+        if info.status != 0:
+            self._stale = True
+        if stats is not None:
+            stats.add("fcg-mg", info.iterations, info.status, info.res_norm, info.b_norm, time.perf_counter() - t0)
+        return x, info
+
+
+_WS = {}
+
+
+def _workspace(device, nbytes):
+    w = _WS.get(device)
+    if w is None or w.numel() < nbytes:
+        w = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+        _WS[device] = w
+    return w
+
+
+def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=4, smoother="chebyshev", degree=2,
+                        param=None, rebuild="solve", stats=None):
+    """Multigrid FCG; on breakdown or non-finite values Jacobi-PCG solves to the same
+    absolute target from the better of the FCG iterate and x0.  `stats` (the Newton
+    solve's KrylovStats) identifies the solve for rebuild='solve'.  Returns (x, info, fallback)."""
+    from .solver import krylov
+    mg = Multigrid.for_problem(problem, smoother, degree, param)
+    x_start = (torch.zeros(A.shape[0], dtype=torch.float32, device=b.device) if x0 is None else
+               x0.to(torch.float32).clone())
+    x, info = mg.solve(A, b, x_start, rtol, atol, maxiter, rel_r0, poll, stats, problem=problem, rebuild=rebuild,
+                       token=stats if stats is not None else object())
+    if info.status in (0, 1):
+        return x, info, False
+    b32 = b.to(torch.float32)
+    r0 = torch.linalg.norm((b32 - A.matvec(x_start)).double()).item()
+    target = max(rtol * (r0 if rel_r0 else info.b_norm), atol)
+    start = x_start
+    if info.status != 3 and bool(torch.isfinite(x).all()):
+        if torch.linalg.norm((b32 - A.matvec(x)).double()).item() < r0:
+            start = x
+    x2, info2 = krylov(A, b32, start, "pcg", 0.0, target, max(maxiter, 10000), True, stats=stats)
+    info2.iterations += info.iterations
+    return x2, info2, True

@@ -18,6 +18,11 @@ a legacy flat dict, solver.py:1014-1068).  Linear keys:
   * ``learned`` {model: LearnedPC, tol, atol, maxiter, stall}: flexible CG with
     the learned CNN preconditioner on the FP32 tangent (precision 'mixed'),
     Jacobi-PCG as its fallback; the Newton residual stays the FP64 judge.
+  * ``mg`` {tol, atol, maxiter, smoother, degree, param, rebuild, poll}: flexible CG
+    preconditioned by geometric-multigrid V-cycles on the FP32 tangent (precision
+    'mixed'; box_mesh HEX8 lattices, multigrid.py), Jacobi-PCG as its fallback;
+    rebuild 'solve' (default: the hierarchy of a Newton solve's first tangent
+    preconditions its later steps), 'always' or 'never'.
   * ``custom_solver``: callable(A, b, x0, linear_options) -> x (solver.py:278-280);
     A is the device ``TangentMatrix``.
   * ``petsc_solver`` / ``spsolve_solver`` / ``amgx_solver`` are host or NVIDIA
@@ -48,7 +53,7 @@ logger = logging.getLogger("jax_fem_amd")
 
 _METHOD_KEYS = ("newton", "arc_length", "dynamic_relax")
 _LINEAR_KEYS = ("jax_solver", "amgx_solver", "spsolve_solver", "petsc_solver", "custom_solver", "pcg", "bicgstab",
-                "learned")
+                "learned", "mg")
 _NEWTON_KEYS = ("tol", "rel_tol", "line_search_flag", "initial_guess", "precision", "forcing", "max_iter",
                 "eta_max", "eta_min", "eta_first", "backtrack")
 
@@ -268,6 +273,19 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
         if stats is not None:
             stats.calls[-1]["fallback"] = fb
         return x
+    if "mg" in opts:
+        from .multigrid import solve_with_fallback
+        o = opts["mg"] or {}
+        if A.dtype != torch.float32:
+            raise ValueError("the multigrid preconditioner runs on the FP32 tangent: use precision='mixed'")
+        x, info, fb = solve_with_fallback(A, b, x0, A.problem, o.get("tol", 1e-8) if rtol is None else rtol,
+                                          o.get("atol", 1e-10) if rtol is None else 0.0, o.get("maxiter", 1000),
+                                          rel_r0=rtol is not None, poll=o.get("poll", 4),
+                                          smoother=o.get("smoother", "chebyshev"), degree=o.get("degree", 2),
+                                          param=o.get("param"), rebuild=o.get("rebuild", "solve"), stats=stats)
+        if stats is not None:
+            stats.calls[-1]["fallback"] = fb
+        return x
     if "custom_solver" in opts:
         return opts["custom_solver"](A, b, x0, opts)
     for key in ("petsc_solver", "spsolve_solver", "amgx_solver"):
@@ -300,8 +318,9 @@ def solver(problem, solver_options={}):
     lin_opts = dict(cfg.get("linear", {}))
     if precision == "mixed" and not any(k in lin_opts for k in _LINEAR_KEYS):
         lin_opts["pcg"] = {}
-    if "learned" in lin_opts and precision != "mixed":
-        raise ValueError("the learned preconditioner runs on the FP32 tangent: use precision='mixed'")
+    for key in ("learned", "mg"):
+        if key in lin_opts and precision != "mixed":
+            raise ValueError("the %s preconditioner runs on the FP32 tangent: use precision='mixed'" % key)
     adt = torch.float32 if precision == "mixed" else torch.float64
     eta_max = cfg.get("eta_max", 0.1)
     eta_min = cfg.get("eta_min", 1e-6 if precision == "mixed" else 1e-10)

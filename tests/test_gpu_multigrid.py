@@ -1,0 +1,284 @@
+"""Geometric multigrid (csrc/mg.hip) through the C ABI against explicit sparse algebra.
+
+* level operators vs P^T A P with P the Kronecker product of 1-D interpolation
+  matrices (scipy, FP64), on lattices with odd interval counts; the coarse
+  operators of an unconstrained body annihilate rigid translations;
+* one V-cycle vs the same cycle written with scipy matrices;
+* the preconditioned flexible CG vs the true FP64 residual and Jacobi-PCG's
+  iteration count; a Newton solve with it vs the FP64 plain path;
+* set-up errors (a pattern that is not the lattice stencil).
+
+Tolerances: level 0 is an exact copy; coarse levels 2e-5 of the operator's
+largest entry (FP32 accumulation); the V-cycle 1e-4 of its largest entry;
+solves by their FP64 residual; Newton rel-L2 1e-6."""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def interp_1d(nf):
+    """(nf+1, nc+1) linear interpolation from every second node (the last node kept)."""
+    nc = (nf + 1) // 2 if nf >= 2 else nf
+    pos = [min(2 * c, nf) for c in range(nc + 1)]
+    P = np.zeros((nf + 1, nc + 1))
+    for c, p in enumerate(pos):
+        P[p, c] = 1.0
+    for f in range(nf + 1):
+        if f in pos:
+            continue
+        left = max(p for p in pos if p < f)
+        right = min(p for p in pos if p > f)
+        P[f, pos.index(left)] = (right - f) / (right - left)
+        P[f, pos.index(right)] = (f - left) / (right - left)
+    return P
+
+
+def interp_3d(dims):
+    Px, Py, Pz = (interp_1d(n - 1) for n in dims)
+    P = sp.kron(sp.kron(sp.csr_matrix(Px), sp.csr_matrix(Py)), sp.csr_matrix(Pz))
+    return sp.kron(P, sp.identity(3)).tocsr()
+
+
+def stencil_to_csr(S, dims):
+    """(27, 3, 3, n) slot-major stencil -> scipy CSR; slots leaving the lattice must hold zeros."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    idx = np.arange(n)
+    x, y, z = idx // (ny * nz), (idx // nz) % ny, idx % nz
+    rows, cols, vals = [], [], []
+    for o in range(27):
+        dx, dy, dz = o // 9 - 1, (o // 3) % 3 - 1, o % 3 - 1
+        xx, yy, zz = x + dx, y + dy, z + dz
+        ok = (xx >= 0) & (xx < nx) & (yy >= 0) & (yy < ny) & (zz >= 0) & (zz < nz)
+        assert np.all(S[o][:, :, ~ok] == 0), "stencil slot %d leaves the lattice but is not zero" % o
+        nb = (xx * ny + yy) * nz + zz
+        for i in range(3):
+            for j in range(3):
+                rows.append(3 * idx[ok] + i)
+                cols.append(3 * nb[ok] + j)
+                vals.append(S[o, i, j, ok].astype(np.float64))
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(3 * n, 3 * n))
+
+
+def box_problem(jfa, cells, material="neo", dirichlet=True, E_cell=None):
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-7)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-7)
+    mat = jfa.NeoHookean(E=10.0, nu=0.3) if material == "neo" else \
+        jfa.LinearElastic(E=1.0, nu=0.3, param="scale_E")
+
+    class P(jfa.Problem):
+        def get_tensor_map(self):
+            return mat
+
+    bc = [[left] * 3 + [right], [0, 1, 2, 0], [lambda p: 0.0] * 3 + [lambda p: 0.05]] if dirichlet else None
+    prob = P(jfa.box_mesh(*cells), vec=3, dim=3, dirichlet_bc_info=bc)
+    if E_cell is not None:
+        prob.internal_vars = [np.repeat(np.asarray(E_cell, dtype=np.float64)[:, None], 8, axis=1)]
+    return prob
+
+
+def tangent(jfa, prob, seed=0, scale=0.02):
+    from importlib import import_module
+    S = import_module("jax_fem_amd.solver")
+    g = torch.Generator(device=DEV).manual_seed(seed)
+    u = scale * torch.randn(prob.N, generator=g, device=DEV, dtype=torch.float64)
+    return S.TangentMatrix(prob, prob.jacobian_bsr(u, torch.float32))
+
+
+def galerkin_levels(A0, dims_list):
+    As = [A0]
+    for l in range(len(dims_list) - 1):
+        P = interp_3d(dims_list[l])
+        assert P.shape[1] == 3 * int(np.prod(dims_list[l + 1]))
+        As.append((P.T @ As[-1] @ P).tocsr())
+    return As
+
+
+@pytest.mark.parametrize("cells", [(4, 3, 5), (8, 6, 7)])
+def test_galerkin_levels_vs_scipy(jfa, cells):
+    say("mg galerkin levels", cells)
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    prob = box_problem(jfa, cells)
+    A = tangent(jfa, prob)
+    mg = MG.Multigrid(MG.lattice(prob))
+    mask = MG.dirichlet_mask(prob)
+    mg.setup(A, mask)
+    dims = mg.levels()
+    assert dims[0] == tuple(c + 1 for c in cells) and 3 * np.prod(dims[-1]) <= 88
+    m = sp.diags(mask.double().cpu().numpy())
+    As = galerkin_levels((m @ A.to_scipy() @ m).tocsr(), dims)
+    for l, d in enumerate(dims):
+        G = stencil_to_csr(mg.stencil(l).cpu().numpy(), d)
+        ref = As[l].toarray()
+        tol = 0.0 if l == 0 else 2e-5 * np.abs(ref).max()
+        np.testing.assert_allclose(G.toarray(), ref, rtol=0, atol=tol, err_msg="level %d" % l)
+
+
+def test_coarse_operators_annihilate_translations(jfa):
+    say("mg translations")
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    prob = box_problem(jfa, (6, 5, 7), material="lin", dirichlet=False)
+    A = tangent(jfa, prob, scale=0.0)
+    mg = MG.Multigrid(MG.lattice(prob))
+    mg.setup(A, torch.ones(prob.N, device=DEV))
+    for l, d in enumerate(mg.levels()):
+        G = stencil_to_csr(mg.stencil(l).cpu().numpy(), d)
+        n = int(np.prod(d))
+        scale = abs(G).sum(axis=1).max()
+        for k in range(3):
+            t = np.zeros(3 * n)
+            t[k::3] = 1.0
+            assert np.abs(G @ t).max() <= 1e-5 * scale, (l, k)
+
+
+def block_dinv(A, unit=None):
+    n = A.shape[0] // 3
+    Ad = A.toarray()
+    D = []
+    for i in range(n):
+        B = Ad[3 * i:3 * i + 3, 3 * i:3 * i + 3].copy()
+        if unit is not None:
+            B += np.diag(1.0 - unit[3 * i:3 * i + 3])
+        D.append(np.linalg.inv(B))
+    return sp.block_diag(D).tocsr()
+
+
+def smoother_ref(A, D, r, e, kind, degree, param, lam):
+    """`degree` steps of Chebyshev (interval [lam/param, lam]) or damped Jacobi (omega = param)
+    for A e = r preconditioned by D, from e."""
+    res = r - A @ e
+    if kind == "jacobi":
+        for _ in range(degree):
+            e = e + param * (D @ res)
+            res = r - A @ e
+        return e
+    lo, hi = lam / param, lam
+    theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    sigma = theta / delta
+    rho = 1.0 / sigma
+    d = (D @ res) / theta
+    for k in range(degree):
+        e = e + d
+        if k == degree - 1:
+            break
+        res = res - A @ d
+        rho_new = 1.0 / (2.0 * sigma - rho)
+        d = rho_new * rho * d + 2.0 * rho_new / delta * (D @ res)
+        rho = rho_new
+    return e
+
+
+def vcycle_ref(As, dims, mask, r, kind, degree, param, lams):
+    L = len(As)
+    Dinv = [block_dinv(As[l], mask if l == 0 else None) for l in range(L - 1)]
+    Ps = [interp_3d(dims[l]) for l in range(L - 1)]
+    Ac = As[-1].toarray()
+    Ainv = np.linalg.pinv(0.5 * (Ac + Ac.T))
+
+    def V(l, rl):
+        if l == L - 1:
+            return Ainv @ rl
+        A = As[l]
+        e = smoother_ref(A, Dinv[l], rl, np.zeros_like(rl), kind, degree, param, lams[l])
+        ec = V(l + 1, Ps[l].T @ (rl - A @ e))
+        e = e + (mask if l == 0 else 1.0) * (Ps[l] @ ec)
+        return smoother_ref(A, Dinv[l], rl, e, kind, degree, param, lams[l])
+
+    return V(0, r), Dinv
+
+
+@pytest.mark.parametrize("kind,degree,param", [("chebyshev", 2, 8.0), ("chebyshev", 3, 4.0), ("jacobi", 2, 0.5)])
+def test_vcycle_vs_scipy(jfa, kind, degree, param):
+    say("mg vcycle", kind, degree, param)
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    prob = box_problem(jfa, (6, 5, 7))
+    A = tangent(jfa, prob)
+    mg = MG.Multigrid(MG.lattice(prob), kind, degree, param)
+    mask = MG.dirichlet_mask(prob)
+    mg.setup(A, mask)
+    dims = mg.levels()
+    m = mask.double().cpu().numpy()
+    As = galerkin_levels((sp.diags(m) @ A.to_scipy() @ sp.diags(m)).tocsr(), dims)
+    # the Gershgorin bounds of Dinv A, level by level
+    Dinv = [block_dinv(As[l], m if l == 0 else None) for l in range(len(As) - 1)]
+    lams = [abs(Dinv[l] @ As[l]).sum(axis=1).max() for l in range(len(As) - 1)]
+    np.testing.assert_allclose(mg.bounds(), lams, rtol=1e-5)
+    g = torch.Generator(device=DEV).manual_seed(11)
+    r = torch.randn(prob.N, generator=g, device=DEV) * mask
+    z = mg.apply(r).double().cpu().numpy()
+    zr, _ = vcycle_ref(As, dims, m, r.double().cpu().numpy(), kind, degree, param, lams)
+    np.testing.assert_allclose(z, zr, rtol=0, atol=1e-4 * np.abs(zr).max())
+
+
+def test_mg_fcg_converges(jfa):
+    say("mg fcg solve")
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    lpc = __import__("jax_fem_amd.learned_pc", fromlist=["x"])
+    from importlib import import_module
+    S = import_module("jax_fem_amd.solver")
+    cells = (16, 12, 10)
+    prob = box_problem(jfa, cells, material="lin", E_cell=lpc.heterogeneous_modulus(cells, 4))
+    A = S.TangentMatrix(prob, prob.jacobian_bsr(torch.zeros(prob.N, device=DEV, dtype=torch.float64),
+                                                torch.float32))
+    mask = MG.dirichlet_mask(prob)
+    g = torch.Generator(device=DEV).manual_seed(2)
+    b = torch.randn(prob.N, generator=g, device=DEV) * mask
+    mg = MG.Multigrid.for_problem(prob)
+    x, info = mg.solve(A, b, rtol=1e-6, atol=0.0, maxiter=200)
+    assert info.status == 0, (info.status, info.iterations)
+    A64 = A.to_scipy()
+    bb = b.double().cpu().numpy()
+    rel = np.linalg.norm(bb - A64 @ x.double().cpu().numpy()) / np.linalg.norm(bb)
+    assert rel <= 1e-5, rel
+    _, jinfo = S.krylov(A, b, None, "pcg", 1e-6, 0.0, 20000, True)
+    say("mg iterations", info.iterations, "jacobi-pcg", jinfo.iterations)
+    assert info.iterations <= 20 and 8 * info.iterations < jinfo.iterations
+
+
+def test_newton_with_mg_matches_fp64(jfa):
+    say("mg newton")
+    cells = (10, 8, 8)
+    prob = box_problem(jfa, cells)
+    u64 = jfa.solver(prob, {"newton": {"tol": 0.0, "rel_tol": 1e-10,
+                                       "linear": {"pcg": {"tol": 1e-12, "atol": 1e-14}}}})[0]
+    prob2 = box_problem(jfa, cells)
+    u = jfa.solver(prob2, {"newton": {"precision": "mixed", "tol": 0.0, "rel_tol": 1e-10,
+                                      "linear": {"mg": {}}}})[0]
+    assert prob2.solve_info["rel"] <= 1e-10
+    assert prob2.solve_info["fallbacks"] == 0
+    rel = (torch.linalg.norm(u - u64) / torch.linalg.norm(u64)).item()
+    say("newton mg rel-L2", rel, "krylov", prob2.solve_info["krylov_iters"])
+    assert rel <= 1e-6
+
+
+def test_setup_rejects_non_lattice_pattern(jfa):
+    say("mg non-lattice")
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    cells = (3, 3, 3)
+    mesh = jfa.box_mesh(*cells)
+    perm = np.random.default_rng(0).permutation(len(mesh.points))
+    inv = np.argsort(perm)
+    mesh2 = jfa.Mesh(mesh.points[perm], inv[mesh.cells], "HEX8")   # same body, nodes relabelled
+
+    class P(jfa.Problem):
+        def get_tensor_map(self):
+            return jfa.LinearElastic(E=1.0, nu=0.3)
+
+    prob = P(mesh2, vec=3, dim=3)
+    A = tangent(jfa, prob, scale=0.0)
+    mg = MG.Multigrid((4, 4, 4))
+    with pytest.raises(jfa.HipFemError, match="lattice"):
+        mg.setup(A, torch.ones(prob.N, device=DEV))
+    with pytest.raises(jfa.HipFemError, match="2 nodes"):
+        MG.Multigrid((1, 4, 4))
