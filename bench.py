@@ -5,16 +5,21 @@ Workload (default, 1 GPU): compressible Neo-Hookean, unit cube, HEX8 64^3
 for load steps k = 1..5 (u_y, u_z free there), each load step warm-started
 from the previous one and solved to ||R||/||R0|| <= 1e-8 (FP64 residual).
 
-A "step" is the whole 5-load-step solve.  Methods:
-  mixed   : FP32 element Jacobians + FP32 Jacobi-PCG with FP64 dot products,
-            FP64 Newton residual/update (iterative refinement), Eisenstat-Walker
-            forcing.  This is `value` (the fastest path on this workload).
-  learned : the same Newton with the learned CNN preconditioner (flexible CG,
-            Jacobi-PCG fallback), timed once and reported beside it.
+A "step" is the whole 5-load-step solve.  Methods (all FP32 tangent + FP64
+Newton residual/update, i.e. FP64 iterative refinement, Eisenstat-Walker
+forcing, unless stated):
+  mg      : flexible CG preconditioned by geometric-multigrid V-cycles (Galerkin
+            levels, Chebyshev smoothing; box lattices).  This is `value` where
+            the mesh is a HEX8 box lattice.
+  mixed   : Jacobi-PCG with FP64 dot products (`value` on other meshes).
+  learned : the learned CNN preconditioner (flexible CG, Jacobi-PCG fallback),
+            timed once and reported beside it.
   plain   : the drop-in: everything FP64, Jacobi-PCG to 1e-10 per Newton step
             (the reference's jax_solve tolerances).
-roofline.traffic is the HBM traffic per SpMV launch measured with rocprofv3
-PMC counters and committed in profiles/cg_spmv_hex8_64_pmc.md (null if absent).
+roofline: the dominant kernel of the `value` path, timed live with HIP events
+(the fine-level smoother pass of the multigrid, or the Krylov SpMV); traffic is
+its HBM bytes per launch from rocprofv3 PMC counters committed under profiles/
+(null if absent).
 `--calibrate-copy` runs only a 256 MiB device copy (the PMC calibration kernel).
 Prints one JSON line.
 """
@@ -212,9 +217,9 @@ def cpu_baseline(n, plain_infos, budget_cells=16384):
             "cpu_model": _cpu_model(), "host_cores_available": len(os.sched_getaffinity(0))}
 
 
-def _pmc_traffic():
-    """HBM bytes per pcg_spmv launch from the committed PMC summary, or None."""
-    path = os.path.join(ROOT, "profiles", "cg_spmv_hex8_64_pmc.md")
+def _pmc_traffic(name):
+    """HBM bytes per launch from a committed PMC summary under profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -355,6 +360,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-mg", action="store_true", help="value path: Jacobi-PCG even on box lattices")
+    ap.add_argument("--no-pcg", action="store_true", help="skip the mixed Jacobi-PCG leg beside the multigrid")
     ap.add_argument("--fd-cells", type=int, default=0, help="SIMP workload: cells checked by central FD")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--debug", action="store_true")
@@ -407,6 +414,7 @@ def main():
 
     # tol=0: stop on the relative FP64 residual 1e-8 only (the reference also stops at ||R|| <= 1e-6)
     mixed = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4, "linear": {"pcg": {}}}}
+    mgo = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4, "linear": {"mg": {}}}}
     # plain: the reference's jax_solve tolerances (tol = atol = 1e-10, solver.py:78-84).
     # The SIMP load is small (||R0|| ~ 1e-2), where the absolute 1e-10 would stop the
     # Krylov solves above the 1e-8 relative Newton target: there the tolerance is relative only.
@@ -415,76 +423,94 @@ def main():
     if wl == "elastic-hex8-20":
         # the reference's own configuration: one Newton step, FP64 Jacobi-BiCGSTAB to 1e-10
         plain = {"newton": {"tol": 0.0, "linear": {"jax_solver": {}}}}
+    from jax_fem_amd.multigrid import lattice
+    try:
+        lattice(prob)
+        use_mg = not args.no_mg
+    except NotImplementedError:
+        use_mg = False
+    main_opts = mgo if use_mg else mixed
 
     for w in range(args.warmup):
-        solve_case(jfa, prob, bc, mixed, nload, dmax)
+        solve_case(jfa, prob, bc, main_opts, nload, dmax)
         log("warmup %d done" % w)
 
     import ctypes
+    nnz = prob.pattern.nnz
     jfa._lib.call("hf_profile_enable", 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        sol_m, infos_m = solve_case(jfa, prob, bc, mixed, nload, dmax)
+        sol_m, infos_m = solve_case(jfa, prob, bc, main_opts, nload, dmax)
         log("timed step %d done" % s)
     torch.cuda.synchronize()
     wall_m = (time.perf_counter() - t0) / args.steps
     nl, ms = ctypes.c_int64(), ctypes.c_double()
     jfa._lib.call("hf_profile_read", ctypes.byref(nl), ctypes.byref(ms))
     jfa._lib.call("hf_profile_enable", 0)
-    spmv_us = 1e3 * ms.value / max(nl.value, 1)
-    nnz = prob.pattern.nnz
-    # algorithmic bytes of one PCG SpMV launch (w = A u): values (4 B/nnz), one
-    # block column per node-pair block, u read and w written (4 B per DOF each)
-    spmv_bytes = 4 * nnz + 4 * prob.pattern.nblocks + 8 * N
-    achieved = spmv_bytes / (spmv_us * 1e-6) / 1e9
+    kern_us = 1e3 * ms.value / max(nl.value, 1)
+    if use_mg:
+        # one pass of the fine-level smoother (mg_post_first): the 27-point node-block
+        # stencil (243 FP32 values per node), the 3x3 block-Jacobi inverse (9), and the
+        # node's r, e_in (read) and res, d, e_out (written): 3 FP32 each
+        kbytes = prob.N // 3 * (243 * 4 + 9 * 4 + 5 * 12)
+        kname = "mg_post_first (fine-level Chebyshev smoothing pass: 27-point node-block stencil, slot-major FP32)"
+        pmc = "mg_post_first_hex8_64_pmc.md"
+    else:
+        # one PCG SpMV (w = A u): values (4 B/nnz), one block column per node-pair
+        # block, u read and w written (4 B per DOF each)
+        kbytes = 4 * nnz + 4 * prob.pattern.nblocks + 8 * N
+        kname = "cg_spmv_kernel<float,3> (SELL column-form node-block SpMV w = A u fused with u.w)"
+        pmc = "cg_spmv_hex8_64_pmc.md"
+    achieved = kbytes / (kern_us * 1e-6) / 1e9
     rel_m = max(i["rel"] for i in infos_m)
-    log("mixed: %.3f s/step, newton %s, krylov %s, spmv %.1f us (%d samples) -> %.0f GB/s" % (
-        wall_m, [i["newton_iters"] for i in infos_m], [i["krylov_iters"] for i in infos_m], spmv_us, nl.value,
-        achieved))
+    log("%s: %.3f s/step, newton %s, krylov %s, kernel %.1f us (%d samples) -> %.0f GB/s" % (
+        "mg" if use_mg else "mixed", wall_m, [i["newton_iters"] for i in infos_m],
+        [i["krylov_iters"] for i in infos_m], kern_us, nl.value, achieved))
 
+    def leg(opts, name, note=None):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol, infos = solve_case(jfa, prob, bc, opts, nload, dmax)
+        wall = time.perf_counter() - t0
+        out = {"dof_per_s": N * nload / wall, "wall_s": wall,
+               "newton_iters": [i["newton_iters"] for i in infos],
+               "krylov_iters": [i["krylov_iters"] for i in infos],
+               "fallbacks": [i.get("fallbacks", 0) for i in infos],
+               "max_rel_residual": max(i["rel"] for i in infos)}
+        if note:
+            out["note"] = note
+        log("%s: %.3f s, newton %s, krylov %s, fallbacks %s" % (
+            name, wall, out["newton_iters"], out["krylov_iters"], out["fallbacks"]))
+        return sol, out
+
+    mixed_out = None
+    if use_mg and not args.no_pcg:
+        _, mixed_out = leg(mixed, "mixed jacobi-pcg", "single run after the timed steps")
     learned_out = None
     if not args.no_learned and wl != "elastic-hex8-20":
         from jax_fem_amd.learned_pc import LearnedPC
         model = LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_elastic.npz"))
         learned = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4,
                               "linear": {"learned": {"model": model}}}}
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sol_l, infos_l = solve_case(jfa, prob, bc, learned, nload, dmax)
-        wall_l = time.perf_counter() - t0
-        learned_out = {"dof_per_s": N * nload / wall_l, "wall_s": wall_l,
-                       "newton_iters": [i["newton_iters"] for i in infos_l],
-                       "krylov_iters": [i["krylov_iters"] for i in infos_l],
-                       "fallbacks": [i["fallbacks"] for i in infos_l],
-                       "max_rel_residual": max(i["rel"] for i in infos_l),
-                       "note": "weights trained on 32^3 heterogeneous linear elasticity; single run incl. "
-                               "first-call setup"}
-        log("learned: %.3f s, newton %s, krylov %s, fallbacks %s" % (
-            wall_l, learned_out["newton_iters"], learned_out["krylov_iters"], learned_out["fallbacks"]))
+        _, learned_out = leg(learned, "learned", "weights trained on 32^3 heterogeneous linear elasticity; single "
+                                                 "run incl. first-call setup")
 
     plain_out = None
     cpu = None
     rel_l2 = None
     if not args.no_plain:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sol_p, infos_p = solve_case(jfa, prob, bc, plain, nload, dmax)
-        wall_p = time.perf_counter() - t0
+        sol_p, plain_out = leg(plain, "plain")
+        plain_out["method"] = ("FP64 Jacobi-BiCGSTAB 1e-10 (jax_solver)" if wl == "elastic-hex8-20"
+                               else "FP64 Jacobi-PCG 1e-10")
         um = sol_m[0].reshape(-1)
         up = sol_p[0].reshape(-1)
         rel_l2 = float(torch.linalg.norm(um - up) / torch.linalg.norm(up))
-        plain_out = {"dof_per_s": N * nload / wall_p, "wall_s": wall_p,
-                     "method": "FP64 Jacobi-BiCGSTAB 1e-10 (jax_solver)" if wl == "elastic-hex8-20"
-                     else "FP64 Jacobi-PCG 1e-10",
-                     "newton_iters": [i["newton_iters"] for i in infos_p],
-                     "krylov_iters": [i["krylov_iters"] for i in infos_p],
-                     "max_rel_residual": max(i["rel"] for i in infos_p)}
-        log("plain: %.3f s, newton %s, krylov %s; mixed vs plain rel-L2 %.2e" % (
-            wall_p, plain_out["newton_iters"], plain_out["krylov_iters"], rel_l2))
+        log("value path vs plain rel-L2 %.2e" % rel_l2)
         if default and not args.no_cpu:
             t0 = time.perf_counter()
-            cpu = cpu_baseline(n, infos_p)
+            cpu = cpu_baseline(n, [{"newton_iters": a, "krylov_iters": b}
+                                   for a, b in zip(plain_out["newton_iters"], plain_out["krylov_iters"])])
             log("cpu baseline %.1f s of sampling" % (time.perf_counter() - t0))
     out = {
         "metric": "DOF/s + wall-to-1e-8 FP64 residual: learned-PC/FP32-refined vs plain PCG",
@@ -495,16 +521,20 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": desc, "dof": N,
-                   "nnz": nnz, "method": "value: FP32 Jacobi-PCG + FP64 Newton refinement (EW forcing); the "
-                                         "learned-PC variant is reported under 'learned_pc'",
+                   "nnz": nnz, "method": ("value: FP32 tangent, flexible CG preconditioned by geometric-multigrid "
+                                          "V-cycles (Galerkin levels, Chebyshev(2) smoothing), FP64 Newton refinement "
+                                          "(EW forcing); Jacobi-PCG under 'pcg_mixed', the learned CNN preconditioner "
+                                          "under 'learned_pc'") if use_mg else
+                   ("value: FP32 Jacobi-PCG + FP64 Newton refinement (EW forcing); the learned-PC variant is "
+                    "reported under 'learned_pc'"),
                    "wall_to_tol_s": wall_m, "max_rel_residual": rel_m,
                    "newton_iters": [i["newton_iters"] for i in infos_m],
                    "krylov_iters": [i["krylov_iters"] for i in infos_m]},
-        "roofline": {"bound": "hbm",
-                     "kernel": "cg_spmv_kernel<float,3> (SELL column-form node-block SpMV w = A u fused with u.w)",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-                     "traffic": _pmc_traffic() if (default and n == 64) else None, "avg_launch_us": spmv_us,
-                     "algorithmic_bytes": spmv_bytes},
+                     "traffic": _pmc_traffic(pmc) if (default and n == 64) else None, "avg_launch_us": kern_us,
+                     "algorithmic_bytes": kbytes},
+        "pcg_mixed": mixed_out,
         "learned_pc": learned_out,
         "plain": plain_out,
         "speedup_vs_plain": (plain_out["wall_s"] / wall_m) if plain_out else None,
@@ -516,7 +546,7 @@ def main():
         out["mfma"] = hex27_jacobian_mfma(jfa, prob)
         log("hex27 jacobian: %.3f ms, %.1f TFLOP/s" % (out["mfma"]["avg_ms"], out["mfma"]["achieved"]))
     if wl == "simp-hex8-96x48x48":
-        out["adjoint"] = simp_adjoint(jfa, prob, theta, args.fd_cells, mixed)
+        out["adjoint"] = simp_adjoint(jfa, prob, theta, args.fd_cells, main_opts)
         log("adjoint: %s" % json.dumps(out["adjoint"]))
     print(json.dumps(out))
 

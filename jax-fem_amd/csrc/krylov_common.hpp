@@ -237,6 +237,7 @@ extern bool g_prof;
 int prof_begin(hipStream_t s);
 int prof_end(hipStream_t s);
 int prof_collect();
+int prof_events(hipEvent_t* e0, hipEvent_t* e1);
 int poll_init();
 int poll_post(const Scalars* sc, int slot, hipStream_t s);
 int poll_wait(int slot, Scalars& out);

@@ -38,6 +38,8 @@
 #include "common.hpp"
 #include "hipfem.h"
 
+#include <hip/hip_ext.h>
+
 #include "krylov_common.hpp"
 
 #include <algorithm>
@@ -1225,7 +1227,11 @@ extern "C" int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels) {
 // Enqueue one V-cycle: level-0 input r0, result in *out (one of the two level-0
 // buffers z0 / z1 given by the caller).  Levels with more than kSmallNodes
 // nodes run as grid kernels; the rest (from level lb down) in mg_small_cycle.
-static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out, const Scalars* sc, hipStream_t s) {
+// timed: the level-0 post-smoother start (one pass over the fine stencil) is
+// launched between the two profiling events (HIP-event timing for roofline
+// reporting, see hf_profile_enable).
+static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out, const Scalars* sc, hipStream_t s,
+                  bool timed = false) {
   const int L = mg->nlev;
   const int k = mg->degree;
   float* cur[kMgMaxLevels];
@@ -1284,7 +1290,14 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const int gg = node_grid(V.g.n);
     mg_prolong<<<gg, kMgBlock, 0, s>>>(cur[l + 1], V.g, C.g, l == 0 ? mg->mask : nullptr, cur[l], sc);
     // post-smoothing: the same polynomial on the residual after the correction
-    mg_post_first<<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.s0, rin(l), cur[l], V.res, V.da, oth[l], sc);
+    if (timed && l == 0) {
+      hipEvent_t e0, e1;
+      if (prof_events(&e0, &e1)) return -2;
+      hipExtLaunchKernelGGL(mg_post_first, dim3(gg), dim3(kMgBlock), 0, s, e0, e1, 0, V.S, V.Dinv, V.g, V.s0, rin(l),
+                            (const float*)cur[l], V.res, V.da, oth[l], sc);
+    } else {
+      mg_post_first<<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.s0, rin(l), cur[l], V.res, V.da, oth[l], sc);
+    }
     float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
     float* din = V.da;
     float* dout = V.db;
@@ -1354,11 +1367,18 @@ extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   if (poll_init()) return -2;
   Scalars host{};
   int it = 0, batch = 0;
+  bool timed_pending = false;
+  int timed_batch = -1;
   for (;;) {
     for (int k = 0; k < poll; ++k, ++it) {
+      const bool timed = g_prof && mg->nlev > 1 && k == poll / 2 && !timed_pending;  // a steady-state launch
       mgcg_update_kernel<<<ge, kBlock, 0, s>>>(N, z, w, pv, q, x, r, sc, partA);
       mgcg_conv_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, ge);
-      if (vcycle(mg, r, z0, z1, &z, sc, s)) return -2;
+      if (vcycle(mg, r, z0, z1, &z, sc, s, timed)) return -2;
+      if (timed) {
+        timed_pending = true;
+        timed_batch = batch;
+      }
       mgcg_spmv_kernel<<<gr, kBlock, 0, s>>>(m, vals, z, r, q, w, sc, partB);
       mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(sc, partB, gr);
     }
@@ -1366,6 +1386,13 @@ extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals
     if (poll_post(sc, batch & 1, s)) return -2;
     if (batch > 0) {
       if (poll_wait((batch - 1) & 1, host)) return -2;
+      if (timed_pending && timed_batch == batch - 1) {
+        // the timed launch did work unless the solve had finished before it
+        if (!host.done || host.iter > (batch - 1) * poll + poll / 2) {
+          if (prof_collect()) return -2;
+        }
+        timed_pending = false;
+      }
       if (host.done) break;
     }
     ++batch;

@@ -170,7 +170,7 @@ class Multigrid:
         _lib.call("hf_mg_copy_stencil", self.handle, int(level), _lib.ptr(out), _lib.stream())
         return out.reshape(27, 3, 3, n)
 
-    def solve(self, A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=500, rel_r0=False, poll=4, stats=None,
+    def solve(self, A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=500, rel_r0=False, poll=1, stats=None,
               mask=None, problem=None, rebuild="always", token=None):
         """Flexible CG preconditioned by V-cycles; the operator is always the FP32
         tangent A, the hierarchy is rebuilt from A as `rebuild` says (needs_setup).
@@ -208,7 +208,7 @@ def _workspace(device, nbytes):
     return w
 
 
-def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=4, smoother="chebyshev", degree=2,
+def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=1, smoother="chebyshev", degree=2,
                         param=None, rebuild="solve", stats=None):
     """Multigrid FCG; on breakdown or non-finite values Jacobi-PCG solves to the same
     absolute target from the better of the FCG iterate and x0.  `stats` (the Newton

@@ -280,7 +280,7 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
             raise ValueError("the multigrid preconditioner runs on the FP32 tangent: use precision='mixed'")
         x, info, fb = solve_with_fallback(A, b, x0, A.problem, o.get("tol", 1e-8) if rtol is None else rtol,
                                           o.get("atol", 1e-10) if rtol is None else 0.0, o.get("maxiter", 1000),
-                                          rel_r0=rtol is not None, poll=o.get("poll", 4),
+                                          rel_r0=rtol is not None, poll=o.get("poll", 1),
                                           smoother=o.get("smoother", "chebyshev"), degree=o.get("degree", 2),
                                           param=o.get("param"), rebuild=o.get("rebuild", "solve"), stats=stats)
         if stats is not None:
