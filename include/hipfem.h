@@ -231,7 +231,14 @@ int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const float* dinv, 
  *      block Jacobi (param = omega). */
 typedef struct hf_mg hf_mg;
 enum { HF_MG_JACOBI = 0, HF_MG_CHEBYSHEV = 1 };
-int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, hf_mg** out);   /* nodes per dimension */
+/* HF_MG_FINE_BF16: the fine-level stencil the smoother and the first Galerkin
+ * product read is stored in BF16 (rounded to nearest even): half the bytes of
+ * the cycle's largest stream; coarser levels stay FP32.  The rounding (2^-8
+ * relative per entry) perturbs smooth modes by about 2^-8 n^2 relative on an
+ * n^3 lattice, so it only pays on small lattices (at 64^3 the iteration count
+ * triples). */
+enum { HF_MG_FINE_BF16 = 1 };
+int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out);   /* nodes per dimension */
 int hf_mg_destroy(hf_mg* mg);
 /* number of levels; dims (3 per level, up to max_levels) receives the node counts */
 int hf_mg_levels(const hf_mg* mg, int32_t* nlev, int32_t* dims, int32_t max_levels);
@@ -246,7 +253,7 @@ int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
                     hf_stream_t stream);
 /* z = V(r), one cycle */
 int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, hf_stream_t stream);
-/* copy level `level`'s stencil (243 * nnode floats) into dst */
+/* copy level `level`'s stencil (243 * nnode floats; a BF16 level widened) into dst */
 int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf_stream_t stream);
 /* flexible CG (one kept direction) preconditioned by V-cycles; the stop rule,
  * flags (HF_KRYLOV_REL_R0; no transpose) and info of hf_krylov_solve.

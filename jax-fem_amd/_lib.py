@@ -19,6 +19,7 @@ HF_MAT_POISSON, HF_MAT_LINEAR_ELASTIC, HF_MAT_NEO_HOOKEAN = 0, 1, 2
 HF_PARAM_NONE, HF_PARAM_SCALE_E, HF_PARAM_SIMP, HF_PARAM_MU, HF_PARAM_SCALE_K = 0, 1, 2, 3, 4
 HF_KRYLOV_PCG, HF_KRYLOV_BICGSTAB = 0, 1
 HF_MG_JACOBI, HF_MG_CHEBYSHEV = 0, 1
+HF_MG_FINE_BF16 = 1
 
 
 class HFMaterial(ctypes.Structure):
@@ -85,7 +86,7 @@ _SIGS = {
     "hf_lpc_workspace_size": (c_sz, [c_i64]),
     "hf_lpc_solve_f32": (c_i32, [vp, vp, vp, vp, vp, vp, vp, vp, c_i32, c_i32, c_i32, c_i32, c_f64, c_f64, c_i32,
                                  c_i32, c_i32, c_i32, vp, ctypes.POINTER(HFKrylovInfo), vp]),
-    "hf_mg_create": (c_i32, [c_i32, c_i32, c_i32, ctypes.POINTER(vp)]),
+    "hf_mg_create": (c_i32, [c_i32, c_i32, c_i32, c_i32, ctypes.POINTER(vp)]),
     "hf_mg_destroy": (c_i32, [vp]),
     "hf_mg_levels": (c_i32, [vp, ctypes.POINTER(c_i32), vp, c_i32]),
     "hf_mg_set_smoother": (c_i32, [vp, c_i32, c_i32, ctypes.c_float]),

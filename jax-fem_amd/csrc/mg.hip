@@ -44,6 +44,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace hf {
 const int32_t* pattern_blkpos(const hf_pattern* p);
@@ -79,6 +80,23 @@ __device__ __forceinline__ int64_t lat_id(const Lat& g, int x, int y, int z) {
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Stencil values are FP32, or on the fine level optionally BF16 held as raw
+// 16-bit patterns (the upper half of the FP32 word; stored rounded to nearest
+// even).  The smoother only needs the operator to about 3 digits, and the fine
+// stencil is the largest stream of every cycle.
+__device__ __forceinline__ float ldv(const float* p) { return *p; }
+__device__ __forceinline__ float ldv(const uint16_t* p) { return __uint_as_float((uint32_t)(*p) << 16); }
+__device__ __forceinline__ void stv(float* p, float v) { *p = v; }
+__device__ __forceinline__ void stv(uint16_t* p, float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7f800000u) == 0x7f800000u) {
+    *p = (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));   // inf stays inf, NaN stays NaN
+    return;
+  }
+  u += 0x7fffu + ((u >> 16) & 1u);
+  *p = (uint16_t)(u >> 16);
+}
+
 // 1-D transfer between nf fine intervals and their coarse nodes.
 __device__ __forceinline__ int cpos(int c, int nf) { return 2 * c < nf ? 2 * c : nf; }
 __device__ __forceinline__ bool is_cpos(int f, int nf) { return (f % 2 == 0) || f == nf; }
@@ -103,7 +121,8 @@ inline int node_grid(int64_t n) {
 }
 
 // out = sum over the 27 slots of S[n] v[neighbour]  (clamped neighbours)
-__device__ __forceinline__ void stencil_row(const float* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
+template <typename ST>
+__device__ __forceinline__ void stencil_row(const ST* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
                                             const float* __restrict__ v, float (&out)[3]) {
   out[0] = out[1] = out[2] = 0.f;
 #pragma unroll
@@ -118,10 +137,10 @@ __device__ __forceinline__ void stencil_row(const float* __restrict__ S, const L
         const int o = dx * 9 + dy * 3 + dz;
         const int64_t m = lat_id(g, xx, yy, zz) * 3;
         const float v0 = v[m], v1 = v[m + 1], v2 = v[m + 2];
-        const float* s = S + (int64_t)o * 9 * g.n + n;
+        const ST* s = S + (int64_t)o * 9 * g.n + n;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-          out[i] += s[(3 * i) * g.n] * v0 + s[(3 * i + 1) * g.n] * v1 + s[(3 * i + 2) * g.n] * v2;
+          out[i] += ldv(s + (3 * i) * g.n) * v0 + ldv(s + (3 * i + 1) * g.n) * v1 + ldv(s + (3 * i + 2) * g.n) * v2;
       }
     }
   }
@@ -194,12 +213,13 @@ __global__ void __launch_bounds__(kMgBlock) mg_check_lattice(const int32_t* __re
   if (colidx[r0 + ls.k] != (int32_t)ls.c || blkpos[r0 + ls.k] != pos) bad[0] = 1;
 }
 
+template <typename ST>
 __global__ void __launch_bounds__(kMgBlock) mg_fine_stencil(const int32_t* __restrict__ rowptr,
                                                             const int32_t* __restrict__ sell_pos,
                                                             const int32_t* __restrict__ slice_ptr, int R,
                                                             const float* __restrict__ vals,
                                                             const float* __restrict__ mask, Lat g,
-                                                            float* __restrict__ S) {
+                                                            ST* __restrict__ S) {
   const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
   const int o = blockIdx.y;
   if (n >= g.n) return;
@@ -218,19 +238,25 @@ __global__ void __launch_bounds__(kMgBlock) mg_fine_stencil(const int32_t* __res
       for (int j = 0; j < 3; ++j) out[3 * i + j] = v[3 * j + i] * mr[i] * mc[j];
   }
 #pragma unroll
-  for (int e = 0; e < 9; ++e) S[(int64_t)(o * 9 + e) * g.n + n] = out[e];
+  for (int e = 0; e < 9; ++e) stv(S + (int64_t)(o * 9 + e) * g.n + n, out[e]);
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_widen(const uint16_t* __restrict__ h, int64_t n, float* __restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (i < n) f[i] = ldv(h + i);
 }
 
 // Inverse of each node's diagonal block (slot 13); masked DOFs (mask 0) get a
 // unit diagonal.  A block that is not safely invertible falls back to the
 // inverse of its positive diagonal entries (0 elsewhere).
-__global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restrict__ S, const float* __restrict__ mask,
+template <typename ST>
+__global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const ST* __restrict__ S, const float* __restrict__ mask,
                                                           int64_t n, float* __restrict__ Dinv) {
   const int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
   if (node >= n) return;
   double a[9];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) a[e] = (double)S[(int64_t)(13 * 9 + e) * n + node];
+  for (int e = 0; e < 9; ++e) a[e] = (double)ldv(S + (int64_t)(13 * 9 + e) * n + node);
   if (mask) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -263,7 +289,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restric
 // Stage 1 of the Galerkin product: B = A P over fine rows i, with slot s of
 // row i holding the block of coarse column J = floor(i/2) + s - 1 (per
 // dimension).  One thread per (row, slot); the slot is blockIdx.y.
-__global__ void __launch_bounds__(kMgBlock) mg_rap_ap(const float* __restrict__ S, Lat f, Lat c,
+template <typename ST>
+__global__ void __launch_bounds__(kMgBlock) mg_rap_ap(const ST* __restrict__ S, Lat f, Lat c,
                                                       float* __restrict__ B) {
   const int64_t i = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
   const int slot = blockIdx.y;
@@ -294,9 +321,9 @@ __global__ void __launch_bounds__(kMgBlock) mg_rap_ap(const float* __restrict__ 
       for (int oz = 0; oz < 3; ++oz) {
         const float w = W[0][ox] * W[1][oy] * W[2][oz];
         if (w == 0.f) continue;
-        const float* sp = S + (int64_t)((ox * 9 + oy * 3 + oz) * 9) * f.n + i;
+        const ST* sp = S + (int64_t)((ox * 9 + oy * 3 + oz) * 9) * f.n + i;
 #pragma unroll
-        for (int e = 0; e < 9; ++e) acc[e] += w * sp[(int64_t)e * f.n];
+        for (int e = 0; e < 9; ++e) acc[e] += w * ldv(sp + (int64_t)e * f.n);
       }
 #pragma unroll
   for (int e = 0; e < 9; ++e) B[(int64_t)(slot * 9 + e) * f.n + i] = acc[e];
@@ -442,7 +469,8 @@ __device__ __forceinline__ void poly_first_node(const float* __restrict__ Dinv, 
 
 // res = res_in - A d_in;  d_out = c1 d_in + c2 Dinv res;  e += d_out
 // (res_in may alias res_out: each node reads and writes only its own entries)
-__device__ __forceinline__ void poly_step_node(const float* __restrict__ S, const float* __restrict__ Dinv,
+template <typename ST>
+__device__ __forceinline__ void poly_step_node(const ST* __restrict__ S, const float* __restrict__ Dinv,
                                                const Lat& g, float c1, float c2, const float* res_in,
                                                const float* __restrict__ d_in, float* res_out,
                                                float* __restrict__ d_out, float* __restrict__ e, int64_t node) {
@@ -463,7 +491,8 @@ __device__ __forceinline__ void poly_step_node(const float* __restrict__ S, cons
 }
 
 // t = res_in - A d_in (the residual after the last pre-smoothing step)
-__device__ __forceinline__ void poly_resid_node(const float* __restrict__ S, const Lat& g,
+template <typename ST>
+__device__ __forceinline__ void poly_resid_node(const ST* __restrict__ S, const Lat& g,
                                                 const float* __restrict__ res_in, const float* __restrict__ d_in,
                                                 float* __restrict__ t, int64_t node) {
   int x, y, z;
@@ -475,7 +504,8 @@ __device__ __forceinline__ void poly_resid_node(const float* __restrict__ S, con
 }
 
 // post-smoother start: res = r - A e_in;  d = s0 Dinv res;  e_out = e_in + d
-__device__ __forceinline__ void post_first_node(const float* __restrict__ S, const float* __restrict__ Dinv,
+template <typename ST>
+__device__ __forceinline__ void post_first_node(const ST* __restrict__ S, const float* __restrict__ Dinv,
                                                 const Lat& g, float s0, const float* __restrict__ r,
                                                 const float* __restrict__ e_in, float* __restrict__ res,
                                                 float* __restrict__ d, float* __restrict__ e_out, int64_t node) {
@@ -593,7 +623,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_first(const float* __restric
   MG_NODE_LOOP(n, poly_first_node(Dinv, n, s0, r, d, e, node));
 }
 
-__global__ void __launch_bounds__(kMgBlock) mg_poly_step(const float* __restrict__ S, const float* __restrict__ Dinv,
+template <typename ST>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_step(const ST* __restrict__ S, const float* __restrict__ Dinv,
                                                          Lat g, float c1, float c2, const float* res_in,
                                                          const float* __restrict__ d_in, float* res_out,
                                                          float* __restrict__ d_out, float* __restrict__ e,
@@ -602,7 +633,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_step(const float* __restrict
   MG_NODE_LOOP(g.n, poly_step_node(S, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
 }
 
-__global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const float* __restrict__ S, Lat g,
+template <typename ST>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const ST* __restrict__ S, Lat g,
                                                           const float* __restrict__ res_in,
                                                           const float* __restrict__ d_in, float* __restrict__ t,
                                                           const Scalars* sc) {
@@ -610,7 +642,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const float* __restric
   MG_NODE_LOOP(g.n, poly_resid_node(S, g, res_in, d_in, t, node));
 }
 
-__global__ void __launch_bounds__(kMgBlock) mg_post_first(const float* __restrict__ S, const float* __restrict__ Dinv,
+template <typename ST>
+__global__ void __launch_bounds__(kMgBlock) mg_post_first(const ST* __restrict__ S, const float* __restrict__ Dinv,
                                                           Lat g, float s0, const float* __restrict__ r,
                                                           const float* __restrict__ e_in, float* __restrict__ res,
                                                           float* __restrict__ d, float* __restrict__ e_out,
@@ -636,7 +669,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_prolong(const float* __restrict__
 
 // Gershgorin bound of Dinv A: per-workgroup max over nodes of the largest
 // absolute row sum of the node's block row of Dinv A.
-__global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S, const float* __restrict__ Dinv,
+template <typename ST>
+__global__ void __launch_bounds__(kMgBlock) mg_gersh(const ST* __restrict__ S, const float* __restrict__ Dinv,
                                                      int64_t n, float* __restrict__ part) {
   __shared__ float sm[kMgBlock / kWave];
   float best = 0.f;
@@ -648,7 +682,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S
     for (int o = 0; o < 27; ++o) {
       float B[9];
 #pragma unroll
-      for (int e = 0; e < 9; ++e) B[e] = S[(int64_t)(o * 9 + e) * n + node];
+      for (int e = 0; e < 9; ++e) B[e] = ldv(S + (int64_t)(o * 9 + e) * n + node);
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -1006,7 +1040,8 @@ using namespace hf;
 
 struct MgLevel {
   Lat g;
-  float* S = nullptr;     // 243 n
+  float* S = nullptr;     // 243 n (FP32 levels)
+  uint16_t* Sh = nullptr; // 243 n BF16 (the fine level with HF_MG_FINE_BF16)
   float* Dinv = nullptr;  // 9 n
   float* r = nullptr;     // 3 n (levels >= 1: the restricted residual)
   float* e = nullptr;     // 3 n
@@ -1028,6 +1063,7 @@ struct hf_mg {
   int nd = 0;
   int32_t* flags = nullptr;  // [0] pattern not on the lattice, [1] dropped coarse pivots
   float* part = nullptr;     // kMaxGrid per-workgroup maxima + kMgMaxLevels bounds
+  bool bf16 = false;         // fine-level stencil in BF16
   int kind = HF_MG_CHEBYSHEV;
   int degree = 2;
   float param = 8.f;         // Chebyshev: lam / lower end of the interval; Jacobi: omega
@@ -1043,6 +1079,7 @@ static int mg_free(hf_mg* mg) {
     float* bufs[] = {L.S, L.Dinv, L.r, L.e, L.e2, L.da, L.db, L.res};
     for (float* b : bufs)
       if (b) (void)hipFree(b);
+    if (L.Sh) (void)hipFree(L.Sh);
   }
   if (mg->B) (void)hipFree(mg->B);
   if (mg->Ainv) (void)hipFree(mg->Ainv);
@@ -1051,8 +1088,9 @@ static int mg_free(hf_mg* mg) {
   return 0;
 }
 
-extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, hf_mg** out) {
+extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out) {
   HF_REQUIRE(out, "hf_mg_create: null output");
+  HF_REQUIRE((flags & ~HF_MG_FINE_BF16) == 0, "hf_mg_create: unknown flags 0x%x", flags);
   HF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "hf_mg_create: lattice %dx%dx%d needs >= 2 nodes per dimension", nx,
              ny, nz);
   HF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 30, "hf_mg_create: lattice too large");
@@ -1073,6 +1111,7 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, hf_mg** out) {
   }
   hf_mg* mg = new hf_mg();
   mg->nlev = nlev;
+  mg->bf16 = (flags & HF_MG_FINE_BF16) && nlev > 1;   // a one-level hierarchy is the dense FP64 solve
   for (int l = 0; l < nlev; ++l) mg->lv[l].g = make_lat(dims[l][0], dims[l][1], dims[l][2]);
   auto alloc = [&](float** p, int64_t n) -> int {
     HF_CHECK_HIP(hipMalloc((void**)p, (size_t)n * sizeof(float)));
@@ -1082,7 +1121,14 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, hf_mg** out) {
   for (int l = 0; l < mg->nlev && !rc; ++l) {
     MgLevel& L = mg->lv[l];
     const int64_t n = L.g.n;
-    rc |= alloc(&L.S, kSt * n);
+    if (l == 0 && mg->bf16) {
+      if (hipMalloc((void**)&L.Sh, (size_t)kSt * n * sizeof(uint16_t)) != hipSuccess) {
+        set_error("hf_mg_create: hipMalloc failed");
+        rc = -2;
+      }
+    } else {
+      rc |= alloc(&L.S, kSt * n);
+    }
     if (l + 1 < mg->nlev) rc |= alloc(&L.Dinv, 9 * n) | alloc(&L.da, 3 * n) | alloc(&L.db, 3 * n) |
                                 alloc(&L.res, 3 * n);
     if (l > 0) rc |= alloc(&L.r, 3 * n);
@@ -1181,17 +1227,25 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
     mg_check_lattice<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p),
                                                         pattern_sell_pos(p), pattern_slice_ptr(p), pattern_R(p), g0,
                                                         mg->flags);
-  mg_fine_stencil<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p), pattern_slice_ptr(p),
-                                                     pattern_R(p), vals, mask, g0, mg->lv[0].S);
+  if (mg->bf16)
+    mg_fine_stencil<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p), pattern_slice_ptr(p),
+                                                       pattern_R(p), vals, mask, g0, mg->lv[0].Sh);
+  else
+    mg_fine_stencil<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p), pattern_slice_ptr(p),
+                                                       pattern_R(p), vals, mask, g0, mg->lv[0].S);
   float* lam_dev = mg->part + kMaxGrid;
   for (int l = 0; l + 1 < mg->nlev; ++l) {
     const Lat f = mg->lv[l].g, c = mg->lv[l + 1].g;
-    mg_block_dinv<<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(mg->lv[l].S, l == 0 ? mask : nullptr, f.n,
-                                                                         mg->lv[l].Dinv);
     const int gg = (int)std::min<int64_t>(ceil_div(f.n, kMgBlock), kMaxGrid);
-    mg_gersh<<<gg, kMgBlock, 0, s>>>(mg->lv[l].S, mg->lv[l].Dinv, f.n, mg->part);
+    auto level = [&](auto Sp) {
+      mg_block_dinv<<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(Sp, l == 0 ? mask : nullptr, f.n,
+                                                                           mg->lv[l].Dinv);
+      mg_gersh<<<gg, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f.n, mg->part);
+      mg_rap_ap<<<dim3((unsigned)ceil_div(f.n, kMgBlock), 27), kMgBlock, 0, s>>>(Sp, f, c, mg->B);
+    };
+    if (l == 0 && mg->bf16) level((const uint16_t*)mg->lv[0].Sh);
+    else level((const float*)mg->lv[l].S);
     mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gg, lam_dev + l);
-    mg_rap_ap<<<dim3((unsigned)ceil_div(f.n, kMgBlock), 27), kMgBlock, 0, s>>>(mg->lv[l].S, f, c, mg->B);
     mg_rap_ptap<<<dim3((unsigned)ceil_div(c.n, kMgBlock), 27), kMgBlock, 0, s>>>(mg->B, f, c, mg->lv[l + 1].S);
   }
   const Lat gl = mg->lv[mg->nlev - 1].g;
@@ -1254,12 +1308,16 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const float* res = rin(l);
     float* din = V.da;
     float* dout = V.db;
-    for (int j = 1; j < k; ++j) {
-      mg_poly_step<<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], res, din, V.res, dout, cur[l], sc);
-      res = V.res;
-      float* tmp = din; din = dout; dout = tmp;
-    }
-    mg_poly_resid<<<gg, kMgBlock, 0, s>>>(V.S, V.g, res, din, dout, sc);
+    auto down = [&](auto Sp) {
+      for (int j = 1; j < k; ++j) {
+        mg_poly_step<<<gg, kMgBlock, 0, s>>>(Sp, V.Dinv, V.g, V.c1[j], V.c2[j], res, din, V.res, dout, cur[l], sc);
+        res = V.res;
+        float* tmp = din; din = dout; dout = tmp;
+      }
+      mg_poly_resid<<<gg, kMgBlock, 0, s>>>(Sp, V.g, res, din, dout, sc);
+    };
+    if (l == 0 && mg->bf16) down((const uint16_t*)V.Sh);
+    else down((const float*)V.S);
     const MgLevel& C = mg->lv[l + 1];
     const bool smooth_c = (l + 2 < L);
     mg_restrict<<<node_grid(C.g.n), kMgBlock, 0, s>>>(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.s0,
@@ -1290,21 +1348,28 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const int gg = node_grid(V.g.n);
     mg_prolong<<<gg, kMgBlock, 0, s>>>(cur[l + 1], V.g, C.g, l == 0 ? mg->mask : nullptr, cur[l], sc);
     // post-smoothing: the same polynomial on the residual after the correction
-    if (timed && l == 0) {
-      hipEvent_t e0, e1;
-      if (prof_events(&e0, &e1)) return -2;
-      hipExtLaunchKernelGGL(mg_post_first, dim3(gg), dim3(kMgBlock), 0, s, e0, e1, 0, V.S, V.Dinv, V.g, V.s0, rin(l),
-                            (const float*)cur[l], V.res, V.da, oth[l], sc);
-    } else {
-      mg_post_first<<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.s0, rin(l), cur[l], V.res, V.da, oth[l], sc);
-    }
-    float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
-    float* din = V.da;
-    float* dout = V.db;
-    for (int j = 1; j < k; ++j) {
-      mg_poly_step<<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], V.res, din, V.res, dout, cur[l], sc);
-      float* t2 = din; din = dout; dout = t2;
-    }
+    int perr = 0;
+    auto up = [&](auto Sp) {
+      using ST = std::remove_cv_t<std::remove_pointer_t<decltype(Sp)>>;
+      if (timed && l == 0) {
+        hipEvent_t e0, e1;
+        if (prof_events(&e0, &e1)) { perr = -2; return; }
+        hipExtLaunchKernelGGL(mg_post_first<ST>, dim3(gg), dim3(kMgBlock), 0, s, e0, e1, 0, Sp, (const float*)V.Dinv,
+                              V.g, V.s0, rin(l), (const float*)cur[l], V.res, V.da, oth[l], sc);
+      } else {
+        mg_post_first<<<gg, kMgBlock, 0, s>>>(Sp, V.Dinv, V.g, V.s0, rin(l), cur[l], V.res, V.da, oth[l], sc);
+      }
+      float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
+      float* din = V.da;
+      float* dout = V.db;
+      for (int j = 1; j < k; ++j) {
+        mg_poly_step<<<gg, kMgBlock, 0, s>>>(Sp, V.Dinv, V.g, V.c1[j], V.c2[j], V.res, din, V.res, dout, cur[l], sc);
+        float* t2 = din; din = dout; dout = t2;
+      }
+    };
+    if (l == 0 && mg->bf16) up((const uint16_t*)V.Sh);
+    else up((const float*)V.S);
+    if (perr) return perr;
   }
   HF_LAUNCH_CHECK();
   *out = cur[0];
@@ -1326,8 +1391,14 @@ extern "C" int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf
   HF_REQUIRE(mg && dst, "hf_mg_copy_stencil: null argument");
   HF_REQUIRE(level >= 0 && level < mg->nlev, "hf_mg_copy_stencil: level %d outside 0..%d", level, mg->nlev - 1);
   HF_REQUIRE(mg->ready, "hf_mg_copy_stencil: hf_mg_setup_f32 has not run");
-  HF_CHECK_HIP(hipMemcpyAsync(dst, mg->lv[level].S, (size_t)kSt * mg->lv[level].g.n * sizeof(float),
-                              hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  const int64_t nv = (int64_t)kSt * mg->lv[level].g.n;
+  if (level == 0 && mg->bf16) {
+    mg_widen<<<(unsigned)ceil_div(nv, kMgBlock), kMgBlock, 0, (hipStream_t)stream>>>(mg->lv[0].Sh, nv, dst);
+    HF_LAUNCH_CHECK();
+  } else {
+    HF_CHECK_HIP(hipMemcpyAsync(dst, mg->lv[level].S, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice,
+                                (hipStream_t)stream));
+  }
   return 0;
 }
 

@@ -9,6 +9,10 @@ re-runs the set-up when the tangent changes and implements the fallback.
     level l+1    P^T A_l P, P trilinear interpolation from the lattice with
                  every dimension's interval count halved (rounded up)
     coarsest     <= 88 DOF, dense FP64 inverse
+    storage      FP32 (fine_bf16: the fine-level stencil in BF16, which halves the
+                 cycle's largest stream but rounds away the cancellation in the stencil's
+                 row sums that smooth error depends on: at 64^3 the Krylov count
+                 triples, so it is off by default)
     cycle        V-cycle, `degree` smoothing steps per side: Chebyshev in Dinv A on
                  [lam/ratio, lam], lam the Gershgorin bound of Dinv A (Dinv: inverse
                  3x3 diagonal blocks), or damped block Jacobi (omega)
@@ -47,12 +51,14 @@ def dirichlet_mask(problem):
 
 
 class Multigrid:
-    def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None):
+    def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None, fine_bf16=False):
         _lib.load()
         self.grid = tuple(int(v) for v in grid)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.fine_bf16 = bool(fine_bf16)
         h = ctypes.c_void_p()
-        _lib.call("hf_mg_create", self.grid[0], self.grid[1], self.grid[2], ctypes.byref(h))
+        _lib.call("hf_mg_create", self.grid[0], self.grid[1], self.grid[2],
+                  _lib.HF_MG_FINE_BF16 if self.fine_bf16 else 0, ctypes.byref(h))
         self.handle = h
         self.smoother_cfg = None
         self.set_smoother(smoother, degree, param)
@@ -75,12 +81,12 @@ class Multigrid:
             self.handle = None
 
     @classmethod
-    def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None):
+    def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None, fine_bf16=False):
         """The problem's cached hierarchy (one per lattice) with the given smoother."""
         grid = lattice(problem)
         mg = getattr(problem, "_mg", None)
-        if mg is None or mg.grid != grid:
-            mg = cls(grid, smoother, degree, param, device=problem.device)
+        if mg is None or mg.grid != grid or mg.fine_bf16 != bool(fine_bf16):
+            mg = cls(grid, smoother, degree, param, device=problem.device, fine_bf16=fine_bf16)
             problem._mg = mg
         else:
             mg.set_smoother(smoother, degree, param)
@@ -209,12 +215,12 @@ def _workspace(device, nbytes):
 
 
 def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=1, smoother="chebyshev", degree=2,
-                        param=None, rebuild="solve", stats=None):
+                        param=None, rebuild="solve", fine_bf16=False, stats=None):
     """Multigrid FCG; on breakdown or non-finite values Jacobi-PCG solves to the same
     absolute target from the better of the FCG iterate and x0.  `stats` (the Newton
     solve's KrylovStats) identifies the solve for rebuild='solve'.  Returns (x, info, fallback)."""
     from .solver import krylov
-    mg = Multigrid.for_problem(problem, smoother, degree, param)
+    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_bf16)
     x_start = (torch.zeros(A.shape[0], dtype=torch.float32, device=b.device) if x0 is None else
                x0.to(torch.float32).clone())
     x, info = mg.solve(A, b, x_start, rtol, atol, maxiter, rel_r0, poll, stats, problem=problem, rebuild=rebuild,
