@@ -301,7 +301,7 @@ def hex27_jacobian_mfma(jfa, prob):
                     "3,542,940 flop/cell"}
 
 
-def simp_adjoint(jfa, prob, theta, fd_cells, opts_fwd):
+def simp_adjoint(jfa, prob, theta, fd_cells, opts_fwd, adj_opts=None):
     """Compliance C = integral of t.u over the loaded face = -f_ext.u (the surface
     map is s = -t): gradient w.r.t. theta by ad_wrapper (transposed solve +
     parameter VJP), timed, and checked by central differences (h = 1e-4) on
@@ -309,7 +309,8 @@ def simp_adjoint(jfa, prob, theta, fd_cells, opts_fwd):
     import numpy as np
     import torch
     th = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
-    adj_opts = {"pcg": {"tol": 1e-12, "atol": 0.0}}
+    ref_opts = {"pcg": {"tol": 1e-12, "atol": 0.0}}
+    adj_opts = adj_opts or ref_opts
     fwd = jfa.ad_wrapper(prob, opts_fwd, adj_opts)
     fext = prob._external()
     torch.cuda.synchronize()
@@ -323,7 +324,20 @@ def simp_adjoint(jfa, prob, theta, fd_cells, opts_fwd):
     t2 = time.perf_counter()
     g = th.grad.detach().cpu().numpy()
     out = {"objective": "compliance", "value": float(obj.detach()), "forward_s": t1 - t0, "adjoint_s": t2 - t1,
-           "grad_norm": float(np.linalg.norm(g))}
+           "grad_norm": float(np.linalg.norm(g)), "adjoint_method": next(iter(adj_opts))}
+    if "mg" in adj_opts:
+        out["adjoint_refinement"] = getattr(prob, "adjoint_info", None)
+        # the same gradient with the FP64 Jacobi-PCG adjoint (tol 1e-12) beside it
+        th2 = th.detach().clone().requires_grad_(True)
+        fwd2 = jfa.ad_wrapper(prob, opts_fwd, ref_opts)
+        u2 = fwd2(th2)[0]
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        (-(fext * u2.reshape(-1)).sum()).backward()
+        torch.cuda.synchronize()
+        g2 = th2.grad.detach().cpu().numpy()
+        out["fp64_pcg_adjoint_s"] = time.perf_counter() - t3
+        out["rel_l2_vs_fp64_pcg_adjoint"] = float(np.linalg.norm(g - g2) / np.linalg.norm(g2))
     if fd_cells:
         rng = np.random.default_rng(0)
         cells = rng.choice(len(theta), fd_cells, replace=False)
@@ -546,7 +560,8 @@ def main():
         out["mfma"] = hex27_jacobian_mfma(jfa, prob)
         log("hex27 jacobian: %.3f ms, %.1f TFLOP/s" % (out["mfma"]["avg_ms"], out["mfma"]["achieved"]))
     if wl == "simp-hex8-96x48x48":
-        out["adjoint"] = simp_adjoint(jfa, prob, theta, args.fd_cells, main_opts)
+        out["adjoint"] = simp_adjoint(jfa, prob, theta, args.fd_cells, main_opts,
+                                      {"mg": {"tol": 1e-12}} if use_mg else None)
         log("adjoint: %s" % json.dumps(out["adjoint"]))
     print(json.dumps(out))
 

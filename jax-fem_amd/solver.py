@@ -444,7 +444,12 @@ def _unflatten(flat, spec):
 
 
 def adjoint_solve(problem, u, v, adjoint_solver_options=None):
-    """lam with A^T lam = v at u (solver.py:1358-1371); returns (lam, lam_interior)."""
+    """lam with A^T lam = v at u (solver.py:1358-1371); returns (lam, lam_interior).
+
+    Options: 'bicgstab' / 'jax_solver' (default: FP64 Jacobi-BiCGSTAB on A^T, as the
+    reference), 'pcg' (FP64 Jacobi-PCG on the symmetric A_II), 'mg' {tol, inner_tol,
+    max_refine, maxiter} (FP64 refinement around the FP32 multigrid flexible CG on
+    A_II; box_mesh HEX8 lattices)."""
     opts = adjoint_solver_options or {}
     A = TangentMatrix(problem, problem.jacobian_bsr(u, torch.float64))
     v = v.to(torch.float64).contiguous()
@@ -457,6 +462,33 @@ def adjoint_solve(problem, u, v, adjoint_solver_options=None):
         vI = v.clone()
         vI[dl] = 0.0
         lamI, _ = krylov(A, vI, None, "pcg", o.get("tol", 1e-10), o.get("atol", 1e-10), o.get("maxiter", 10000))
+        t = A.matvec(lamI, transpose=True)
+        lam = lamI.clone()
+        lam[dl] = v[dl] - t[dl]
+    elif "mg" in opts:
+        # A_II symmetric: A_II lam_I = v_I by FP64 iterative refinement around the FP32
+        # multigrid flexible CG (each correction solved to `inner_tol` relative to its
+        # right-hand side); stop when the FP64 residual ||v_I - A_II lam_I|| <= tol ||v_I||.
+        from .multigrid import solve_with_fallback
+        o = opts["mg"] or {}
+        A32 = TangentMatrix(problem, A.vals.to(torch.float32))
+        vI = v.clone()
+        vI[dl] = 0.0
+        nv = float(torch.linalg.norm(vI))
+        lamI = torch.zeros_like(vI)
+        r = vI.clone()
+        stats = KrylovStats()
+        for _ in range(o.get("max_refine", 20)):
+            if float(torch.linalg.norm(r)) <= o.get("tol", 1e-10) * nv:
+                break
+            c, _, _ = solve_with_fallback(A32, r, None, problem, o.get("inner_tol", 1e-4), 0.0, o.get("maxiter", 1000),
+                                          rel_r0=True, stats=stats)
+            lamI += c.to(torch.float64)
+            lamI[dl] = 0.0
+            r = vI - A.matvec(lamI)
+            r[dl] = 0.0
+        problem.adjoint_info = {"refinements": len(stats.calls), "krylov": [c["iterations"] for c in stats.calls],
+                                "rel": float(torch.linalg.norm(r)) / max(nv, 1e-300)}
         t = A.matvec(lamI, transpose=True)
         lam = lamI.clone()
         lam[dl] = v[dl] - t[dl]

@@ -271,6 +271,25 @@ def test_newton_with_mg_matches_fp64(jfa):
     assert rel <= 1e-6
 
 
+def test_mg_adjoint_matches_fp64_pcg(jfa):
+    say("mg adjoint")
+    from importlib import import_module
+    S = import_module("jax_fem_amd.solver")
+    lpc = import_module("jax_fem_amd.learned_pc")
+    cells = (9, 7, 8)
+    prob = box_problem(jfa, cells, material="lin", E_cell=lpc.heterogeneous_modulus(cells, 5))
+    u = jfa.solver(prob, {"newton": {"tol": 0.0, "rel_tol": 1e-10, "linear": {"pcg": {"tol": 1e-12}}}})[0].reshape(-1)
+    g = torch.Generator(device=DEV).manual_seed(3)
+    v = torch.randn(prob.N, generator=g, device=DEV, dtype=torch.float64)
+    lam_ref, _ = S.adjoint_solve(prob, u, v, {"pcg": {"tol": 1e-13, "atol": 0.0}})
+    lam, _ = S.adjoint_solve(prob, u, v, {"mg": {"tol": 1e-12}})
+    info = prob.adjoint_info
+    rel = (torch.linalg.norm(lam - lam_ref) / torch.linalg.norm(lam_ref)).item()
+    say("mg adjoint rel", rel, info)
+    assert info["rel"] <= 1e-12 and info["refinements"] <= 6
+    assert rel <= 1e-9
+
+
 def test_setup_rejects_non_lattice_pattern(jfa):
     say("mg non-lattice")
     MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
