@@ -463,7 +463,12 @@ def main():
     jfa._lib.call("hf_profile_read", ctypes.byref(nl), ctypes.byref(ms))
     jfa._lib.call("hf_profile_enable", 0)
     kern_us = 1e3 * ms.value / max(nl.value, 1)
-    if use_mg:
+    if use_mg and prob.fe.ele_type == "HEX27":
+        # the fine level is the SELL tangent: one smoothing product y = A v
+        kbytes = 4 * nnz + 4 * prob.pattern.nblocks + 8 * N
+        kname = "mg_sell_spmv (fine-level smoothing product y = A v, SELL column-form node blocks)"
+        pmc = "mg_sell_spmv_hex27_48_pmc.md"
+    elif use_mg:
         # one pass of the fine-level smoother (mg_post_first): the 27-point node-block
         # stencil (243 FP32 values per node), the 3x3 block-Jacobi inverse (9), and the
         # node's r, e_in (read) and res, d, e_out (written): 3 FP32 each

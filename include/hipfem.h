@@ -237,7 +237,13 @@ enum { HF_MG_JACOBI = 0, HF_MG_CHEBYSHEV = 1 };
  * relative per entry) perturbs smooth modes by about 2^-8 n^2 relative on an
  * n^3 lattice, so it only pays on small lattices (at 64^3 the iteration count
  * triples). */
-enum { HF_MG_FINE_BF16 = 1 };
+/* HF_MG_FINE_SELL: the fine level is the tangent's SELL matrix itself (elements
+ * whose nodes couple two lattice steps apart: HEX27 on its (2n+1)^3 node lattice).
+ * The smoother then runs SELL products, and level 1 = P^T A P is formed from the
+ * SELL rows; with P the interpolation from every second node it is the 27-point
+ * HEX8 operator of the same cells.  Couplings whose coarse parents are more than
+ * one coarse step apart are a set-up error. */
+enum { HF_MG_FINE_BF16 = 1, HF_MG_FINE_SELL = 2 };
 int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out);   /* nodes per dimension */
 int hf_mg_destroy(hf_mg* mg);
 /* number of levels; dims (3 per level, up to max_levels) receives the node counts */
@@ -251,8 +257,11 @@ int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels);
  * pattern couples nodes that are not lattice neighbours.  Synchronises. */
 int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* mask, int32_t* dropped,
                     hf_stream_t stream);
-/* z = V(r), one cycle */
-int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, hf_stream_t stream);
+/* z = V(r), one cycle; HF_MG_SKIP_FINE_POST stops after the fine level's coarse
+ * correction (no fine post-smoothing: the residual r - A z is then what a
+ * fine-level post-smoother sees) */
+enum { HF_MG_SKIP_FINE_POST = 1 };
+int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, int32_t flags, hf_stream_t stream);
 /* copy level `level`'s stencil (243 * nnode floats; a BF16 level widened) into dst */
 int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf_stream_t stream);
 /* flexible CG (one kept direction) preconditioned by V-cycles; the stop rule,

@@ -19,7 +19,8 @@ HF_MAT_POISSON, HF_MAT_LINEAR_ELASTIC, HF_MAT_NEO_HOOKEAN = 0, 1, 2
 HF_PARAM_NONE, HF_PARAM_SCALE_E, HF_PARAM_SIMP, HF_PARAM_MU, HF_PARAM_SCALE_K = 0, 1, 2, 3, 4
 HF_KRYLOV_PCG, HF_KRYLOV_BICGSTAB = 0, 1
 HF_MG_JACOBI, HF_MG_CHEBYSHEV = 0, 1
-HF_MG_FINE_BF16 = 1
+HF_MG_FINE_BF16, HF_MG_FINE_SELL = 1, 2
+HF_MG_SKIP_FINE_POST = 1
 
 
 class HFMaterial(ctypes.Structure):
@@ -92,7 +93,7 @@ _SIGS = {
     "hf_mg_set_smoother": (c_i32, [vp, c_i32, c_i32, ctypes.c_float]),
     "hf_mg_bounds": (c_i32, [vp, vp, c_i32]),
     "hf_mg_setup_f32": (c_i32, [vp, vp, vp, vp, ctypes.POINTER(c_i32), vp]),
-    "hf_mg_apply_f32": (c_i32, [vp, vp, vp, vp]),
+    "hf_mg_apply_f32": (c_i32, [vp, vp, vp, c_i32, vp]),
     "hf_mg_copy_stencil": (c_i32, [vp, c_i32, vp, vp]),
     "hf_mg_workspace_size": (c_sz, [vp]),
     "hf_mg_solve_f32": (c_i32, [vp, vp, vp, vp, vp, c_i32, c_i32, c_f64, c_f64, c_i32, vp,

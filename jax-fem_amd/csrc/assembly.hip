@@ -657,6 +657,7 @@ const int32_t* pattern_colidx(const hf_pattern* p) { return p->colidx; }
 const int32_t* pattern_trans(const hf_pattern* p) { return p->trans; }
 const int32_t* pattern_blkpos(const hf_pattern* p) { return p->blkpos; }
 const int32_t* pattern_sell_pos(const hf_pattern* p) { return p->sell_pos; }
+const int32_t* pattern_dpos(const hf_pattern* p) { return p->dpos; }
 int64_t pattern_nnode(const hf_pattern* p) { return p->nnode; }
 int64_t pattern_nblocks(const hf_pattern* p) { return p->nblocks; }
 int pattern_R(const hf_pattern* p) { return p->R; }

@@ -49,6 +49,7 @@
 namespace hf {
 const int32_t* pattern_blkpos(const hf_pattern* p);
 const int32_t* pattern_sell_pos(const hf_pattern* p);
+const int32_t* pattern_dpos(const hf_pattern* p);
 
 namespace {
 constexpr int kMgMaxLevels = 16;
@@ -609,11 +610,11 @@ __device__ __forceinline__ void prolong_node(const float* __restrict__ ec, const
   }
 }
 
-#define MG_NODE_LOOP(N, BODY)                                                   \
-  do {                                                                          \
-    int64_t i0_, i1_;                                                           \
-    node_range((N), i0_, i1_);                                                  \
-    for (int64_t node = i0_ + threadIdx.x; node < i1_; node += kMgBlock) BODY;  \
+#define MG_NODE_LOOP(N, ...)                                                           \
+  do {                                                                                 \
+    int64_t i0_, i1_;                                                                  \
+    node_range((N), i0_, i1_);                                                         \
+    for (int64_t node = i0_ + threadIdx.x; node < i1_; node += kMgBlock) __VA_ARGS__;  \
   } while (0)
 
 __global__ void __launch_bounds__(kMgBlock) mg_poly_first(const float* __restrict__ Dinv, int64_t n, float s0,
@@ -900,6 +901,239 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
 }
 
 // ---------------------------------------------------------------------------
+// Fine level kept as the tangent's SELL matrix (elements whose nodes couple more
+// than one lattice step apart, e.g. HEX27 on its (2n+1)^3 node lattice).  The
+// row-eliminated matrix acts on vectors that are zero on Dirichlet DOFs exactly
+// as M A M does, so the smoother multiplies by it directly; the 3x3 diagonal
+// blocks and the Gershgorin bound are formed from the masked blocks.  A smoothing
+// step is a SELL product y = A v (the Krylov sweep) followed by a node-wise update.
+
+// masked diagonal blocks (BC DOFs: unit diagonal) -> inverse, as mg_block_dinv
+__global__ void __launch_bounds__(kMgBlock) mg_sell_dinv(const int32_t* __restrict__ dpos,
+                                                         const float* __restrict__ vals, const float* __restrict__ mask,
+                                                         int64_t n, float* __restrict__ Dinv) {
+  const int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (node >= n) return;
+  double a[9];
+  const int32_t b = dpos[node];
+  const float m[3] = {mask[3 * node], mask[3 * node + 1], mask[3 * node + 2]};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      a[3 * i + j] = b >= 0 ? (double)vals[(int64_t)b + 3 * j + i] * m[i] * m[j] : 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (m[i] == 0.f) a[4 * i] += 1.0;
+  double c[9];
+  c[0] = a[4] * a[8] - a[5] * a[7];
+  c[1] = a[2] * a[7] - a[1] * a[8];
+  c[2] = a[1] * a[5] - a[2] * a[4];
+  c[3] = a[5] * a[6] - a[3] * a[8];
+  c[4] = a[0] * a[8] - a[2] * a[6];
+  c[5] = a[2] * a[3] - a[0] * a[5];
+  c[6] = a[3] * a[7] - a[4] * a[6];
+  c[7] = a[1] * a[6] - a[0] * a[7];
+  c[8] = a[0] * a[4] - a[1] * a[3];
+  const double det = a[0] * c[0] + a[1] * c[3] + a[2] * c[6];
+  const double sc = fmax(fabs(a[0]), fmax(fabs(a[4]), fabs(a[8])));
+  if (det > 1e-12 * sc * sc * sc && isfinite(det)) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Dinv[(int64_t)e * n + node] = (float)(c[e] / det);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      const bool d = (e % 4) == 0;
+      Dinv[(int64_t)e * n + node] = (d && a[e] > 0.0) ? (float)(1.0 / a[e]) : 0.f;
+    }
+  }
+}
+
+// Gershgorin bound of Dinv (M A M) over the SELL rows (per-workgroup maxima, as mg_gersh)
+__global__ void __launch_bounds__(kMgBlock) mg_sell_gersh(const int32_t* __restrict__ rowptr,
+                                                          const int32_t* __restrict__ colidx,
+                                                          const int32_t* __restrict__ blkpos,
+                                                          const float* __restrict__ vals,
+                                                          const float* __restrict__ mask,
+                                                          const float* __restrict__ Dinv, int64_t n,
+                                                          float* __restrict__ part) {
+  __shared__ float sm[kMgBlock / kWave];
+  float best = 0.f;
+  for (int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x; node < n; node += (int64_t)gridDim.x * kMgBlock) {
+    float M[9];
+    const float mr[3] = {mask[3 * node], mask[3 * node + 1], mask[3 * node + 2]};
+#pragma unroll
+    for (int e = 0; e < 9; ++e) M[e] = Dinv[(int64_t)e * n + node];
+    float rs[3] = {0.f, 0.f, 0.f};
+    for (int32_t b = rowptr[node]; b < rowptr[node + 1]; ++b) {
+      const int64_t c = colidx[b];
+      const float* v = vals + blkpos[b];
+      const float mc[3] = {mask[3 * c], mask[3 * c + 1], mask[3 * c + 2]};
+      float B[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) B[3 * i + j] = v[3 * j + i] * mr[i] * mc[j];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          rs[i] += fabsf(M[3 * i] * B[j] + M[3 * i + 1] * B[3 + j] + M[3 * i + 2] * B[6 + j]);
+    }
+    best = fmaxf(best, fmaxf(rs[0], fmaxf(rs[1], rs[2])));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o, 64));
+  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = sm[0];
+    for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
+    part[blockIdx.x] = m;
+  }
+}
+
+// Stage 1 of the Galerkin product from the SELL rows: B = (M A M) P in the 27
+// coarse-column slots of mg_rap_ap.  A thread per fine row accumulates its slots
+// in LDS (64 rows per workgroup, 243 floats each); a block whose coarse columns
+// fall outside the slots sets bad[0] (the pattern is not supported).
+constexpr int kSellRapRows = 64;
+__global__ void __launch_bounds__(kSellRapRows) mg_sell_rap_ap(const int32_t* __restrict__ rowptr,
+                                                               const int32_t* __restrict__ colidx,
+                                                               const int32_t* __restrict__ blkpos,
+                                                               const float* __restrict__ vals,
+                                                               const float* __restrict__ mask, Lat f, Lat c,
+                                                               float* __restrict__ B, int32_t* __restrict__ bad) {
+  __shared__ float acc[kSellRapRows * kSt];
+  const int64_t i = (int64_t)blockIdx.x * kSellRapRows + threadIdx.x;
+  float* my = acc + threadIdx.x * kSt;
+  for (int e = 0; e < kSt; ++e) my[e] = 0.f;
+  if (i < f.n) {
+    int fi[3];
+    lat_xyz(f, i, fi[0], fi[1], fi[2]);
+    const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
+    const int nc[3] = {c.nx - 1, c.ny - 1, c.nz - 1};
+    const float mr[3] = {mask[3 * i], mask[3 * i + 1], mask[3 * i + 2]};
+    bool broken = false;
+    for (int32_t b = rowptr[i]; b < rowptr[i + 1]; ++b) {
+      const int64_t col = colidx[b];
+      int fj[3];
+      lat_xyz(f, col, fj[0], fj[1], fj[2]);
+      const float* v = vals + blkpos[b];
+      const float mc[3] = {mask[3 * col], mask[3 * col + 1], mask[3 * col + 2]};
+      float blk[9];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) blk[3 * a + d] = v[3 * d + a] * mr[a] * mc[d];
+      // coarse columns J of the fine column j: per dimension the parent(s) of j
+      int pj[3][2], ps[3][2];
+      float pw[3][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int v2 = fj[d];
+        int c0, c1w;
+        float w0, w1;
+        if (v2 % 2 == 0) { c0 = v2 / 2; w0 = 1.f; c1w = c0; w1 = 0.f; }
+        else if (v2 == nf[d]) { c0 = nc[d]; w0 = 1.f; c1w = c0; w1 = 0.f; }
+        else { c0 = (v2 - 1) / 2; w0 = 0.5f; c1w = (v2 + 1) / 2; w1 = 0.5f; }
+        pj[d][0] = c0; pj[d][1] = c1w;
+        pw[d][0] = w0; pw[d][1] = w1;
+        ps[d][0] = c0 - fi[d] / 2 + 1;
+        ps[d][1] = c1w - fi[d] / 2 + 1;
+      }
+#pragma unroll
+      for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+        for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+          for (int az = 0; az < 2; ++az) {
+            const float w = pw[0][ax] * pw[1][ay] * pw[2][az];
+            if (w == 0.f) continue;
+            const int sx = ps[0][ax], sy = ps[1][ay], sz = ps[2][az];
+            if (sx < 0 || sx > 2 || sy < 0 || sy > 2 || sz < 0 || sz > 2) {
+              broken = true;
+              continue;
+            }
+            float* dst = my + (sx * 9 + sy * 3 + sz) * 9;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) dst[e] += w * blk[e];
+          }
+      (void)pj;
+    }
+    if (broken) bad[0] = 1;
+    for (int sl = 0; sl < 27; ++sl)
+#pragma unroll
+      for (int e = 0; e < 9; ++e) B[(int64_t)(sl * 9 + e) * f.n + i] = my[sl * 9 + e];
+  }
+}
+
+// y = A v over the SELL rows (the Krylov sweep), node-major output
+__global__ void __launch_bounds__(kBlock) mg_sell_spmv(MatArgs m, const float* __restrict__ vals,
+                                                       const float* __restrict__ v, float* __restrict__ y,
+                                                       const Scalars* sc) {
+  if (sc && sc->done) return;
+  int64_t r0, r1;
+  row_range(m.nslices, r0, r1);
+  auto xv = [&](int64_t c) { return v[c]; };
+  auto epi = [&](int64_t n, int i, float o) { y[n * 3 + i] = o; };
+  sweep<float, 3, false>(m, r0, r1, vals, xv, epi);
+}
+
+// node-wise halves of the fine-level smoothing passes, with y = A v from mg_sell_spmv:
+//   step:  res = res_in - y (v = d_in); d_out = c1 d_in + c2 Dinv res; e += d_out
+//   resid: t = res_in - y (v = d_in)
+//   post:  res = r - y (v = e_in); d = s0 Dinv res; e_out = e_in + d
+__global__ void __launch_bounds__(kMgBlock) mg_sell_step_update(const float* __restrict__ Dinv, int64_t n, float c1,
+                                                                float c2, const float* res_in,
+                                                                const float* __restrict__ y,
+                                                                const float* __restrict__ d_in, float* res_out,
+                                                                float* __restrict__ d_out, float* __restrict__ e,
+                                                                const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(n, {
+    const float res[3] = {res_in[3 * node] - y[3 * node], res_in[3 * node + 1] - y[3 * node + 1],
+                          res_in[3 * node + 2] - y[3 * node + 2]};
+    float dr[3];
+    block_apply(Dinv, n, node, res, dr);
+    for (int i = 0; i < 3; ++i) {
+      const float dn = c1 * d_in[3 * node + i] + c2 * dr[i];
+      res_out[3 * node + i] = res[i];
+      d_out[3 * node + i] = dn;
+      e[3 * node + i] += dn;
+    }
+  });
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_sell_resid_update(int64_t n, const float* __restrict__ res_in,
+                                                                 const float* __restrict__ y, float* __restrict__ t,
+                                                                 const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(n, {
+    for (int i = 0; i < 3; ++i) t[3 * node + i] = res_in[3 * node + i] - y[3 * node + i];
+  });
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_sell_post_update(const float* __restrict__ Dinv, int64_t n, float s0,
+                                                                const float* __restrict__ r,
+                                                                const float* __restrict__ y,
+                                                                const float* __restrict__ e_in, float* __restrict__ res,
+                                                                float* __restrict__ d, float* __restrict__ e_out,
+                                                                const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(n, {
+    const float rr[3] = {r[3 * node] - y[3 * node], r[3 * node + 1] - y[3 * node + 1], r[3 * node + 2] - y[3 * node + 2]};
+    float dr[3];
+    block_apply(Dinv, n, node, rr, dr);
+    for (int i = 0; i < 3; ++i) {
+      res[3 * node + i] = rr[i];
+      d[3 * node + i] = s0 * dr[i];
+      e_out[3 * node + i] = e_in[3 * node + i] + s0 * dr[i];
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
 // outer flexible CG kernels
 
 // workspace: scalars, partials A (<= 2 per workgroup) and B (<= 3), then the vectors
@@ -1064,6 +1298,10 @@ struct hf_mg {
   int32_t* flags = nullptr;  // [0] pattern not on the lattice, [1] dropped coarse pivots
   float* part = nullptr;     // kMaxGrid per-workgroup maxima + kMgMaxLevels bounds
   bool bf16 = false;         // fine-level stencil in BF16
+  bool sell = false;         // fine level is the tangent's SELL matrix (no fine stencil)
+  float* y = nullptr;        // 3 n0: SELL products of the fine level
+  const hf_pattern* sp = nullptr;  // the fine level's pattern and values (last set-up, caller's memory)
+  const float* svals = nullptr;
   int kind = HF_MG_CHEBYSHEV;
   int degree = 2;
   float param = 8.f;         // Chebyshev: lam / lower end of the interval; Jacobi: omega
@@ -1082,6 +1320,7 @@ static int mg_free(hf_mg* mg) {
     if (L.Sh) (void)hipFree(L.Sh);
   }
   if (mg->B) (void)hipFree(mg->B);
+  if (mg->y) (void)hipFree(mg->y);
   if (mg->Ainv) (void)hipFree(mg->Ainv);
   if (mg->flags) (void)hipFree(mg->flags);
   if (mg->part) (void)hipFree(mg->part);
@@ -1090,7 +1329,8 @@ static int mg_free(hf_mg* mg) {
 
 extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out) {
   HF_REQUIRE(out, "hf_mg_create: null output");
-  HF_REQUIRE((flags & ~HF_MG_FINE_BF16) == 0, "hf_mg_create: unknown flags 0x%x", flags);
+  HF_REQUIRE((flags & ~(HF_MG_FINE_BF16 | HF_MG_FINE_SELL)) == 0, "hf_mg_create: unknown flags 0x%x", flags);
+  HF_REQUIRE(!((flags & HF_MG_FINE_BF16) && (flags & HF_MG_FINE_SELL)), "hf_mg_create: BF16 needs the stencil fine level");
   HF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "hf_mg_create: lattice %dx%dx%d needs >= 2 nodes per dimension", nx,
              ny, nz);
   HF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 30, "hf_mg_create: lattice too large");
@@ -1112,6 +1352,11 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
   hf_mg* mg = new hf_mg();
   mg->nlev = nlev;
   mg->bf16 = (flags & HF_MG_FINE_BF16) && nlev > 1;   // a one-level hierarchy is the dense FP64 solve
+  mg->sell = (flags & HF_MG_FINE_SELL) != 0;
+  if (mg->sell && nlev < 2) {
+    delete mg;
+    HF_REQUIRE(false, "hf_mg_create: a SELL fine level needs at least one coarser level");
+  }
   for (int l = 0; l < nlev; ++l) mg->lv[l].g = make_lat(dims[l][0], dims[l][1], dims[l][2]);
   auto alloc = [&](float** p, int64_t n) -> int {
     HF_CHECK_HIP(hipMalloc((void**)p, (size_t)n * sizeof(float)));
@@ -1126,6 +1371,8 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
         set_error("hf_mg_create: hipMalloc failed");
         rc = -2;
       }
+    } else if (l == 0 && mg->sell) {
+      rc |= alloc(&mg->y, 3 * n);
     } else {
       rc |= alloc(&L.S, kSt * n);
     }
@@ -1222,7 +1469,22 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   mg->ready = false;
   HF_CHECK_HIP(hipMemsetAsync(mg->flags, 0, 4 * sizeof(int32_t), s));
   const unsigned gb0 = (unsigned)ceil_div(g0.n, kMgBlock);
-  const bool check = !(mg->checked == p && mg->checked_nblocks == pattern_nblocks(p));
+  float* lam_dev0 = mg->part + kMaxGrid;
+  if (mg->sell) {
+    // fine level = the SELL tangent: block inverses, bound and A P straight from its rows
+    const Lat c = mg->lv[1].g;
+    const int gg = (int)std::min<int64_t>(ceil_div(g0.n, kMgBlock), kMaxGrid);
+    mg_sell_dinv<<<gb0, kMgBlock, 0, s>>>(pattern_dpos(p), vals, mask, g0.n, mg->lv[0].Dinv);
+    mg_sell_gersh<<<gg, kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p), vals, mask,
+                                          mg->lv[0].Dinv, g0.n, mg->part);
+    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gg, lam_dev0);
+    mg_sell_rap_ap<<<(unsigned)ceil_div(g0.n, kSellRapRows), kSellRapRows, 0, s>>>(
+        pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p), vals, mask, g0, c, mg->B, mg->flags);
+    mg_rap_ptap<<<dim3((unsigned)ceil_div(c.n, kMgBlock), 27), kMgBlock, 0, s>>>(mg->B, g0, c, mg->lv[1].S);
+    mg->sp = p;
+    mg->svals = vals;
+  }
+  const bool check = !mg->sell && !(mg->checked == p && mg->checked_nblocks == pattern_nblocks(p));
   if (check)
     mg_check_lattice<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p),
                                                         pattern_sell_pos(p), pattern_slice_ptr(p), pattern_R(p), g0,
@@ -1230,11 +1492,11 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   if (mg->bf16)
     mg_fine_stencil<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p), pattern_slice_ptr(p),
                                                        pattern_R(p), vals, mask, g0, mg->lv[0].Sh);
-  else
+  else if (!mg->sell)
     mg_fine_stencil<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p), pattern_slice_ptr(p),
                                                        pattern_R(p), vals, mask, g0, mg->lv[0].S);
-  float* lam_dev = mg->part + kMaxGrid;
-  for (int l = 0; l + 1 < mg->nlev; ++l) {
+  float* lam_dev = lam_dev0;
+  for (int l = mg->sell ? 1 : 0; l + 1 < mg->nlev; ++l) {
     const Lat f = mg->lv[l].g, c = mg->lv[l + 1].g;
     const int gg = (int)std::min<int64_t>(ceil_div(f.n, kMgBlock), kMaxGrid);
     auto level = [&](auto Sp) {
@@ -1257,10 +1519,14 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   if (mg->nlev > 1)
     HF_CHECK_HIP(hipMemcpyAsync(lam, lam_dev, (mg->nlev - 1) * sizeof(float), hipMemcpyDeviceToHost, s));
   HF_CHECK_HIP(hipStreamSynchronize(s));
-  HF_REQUIRE(hflags[0] == 0, "hf_mg_setup_f32: the pattern couples nodes that are not lattice neighbours "
-                             "(the multigrid needs a HEX8 box lattice)");
-  mg->checked = p;
-  mg->checked_nblocks = pattern_nblocks(p);
+  HF_REQUIRE(hflags[0] == 0, mg->sell ? "hf_mg_setup_f32: the pattern couples lattice nodes whose coarse parents "
+                                        "are more than one coarse step apart (not a HEX8/HEX27 box lattice)"
+                                      : "hf_mg_setup_f32: the pattern couples nodes that are not lattice neighbours "
+                                        "(the multigrid needs a HEX8 box lattice)");
+  if (!mg->sell) {
+    mg->checked = p;
+    mg->checked_nblocks = pattern_nblocks(p);
+  }
   for (int l = 0; l + 1 < mg->nlev; ++l) {
     HF_REQUIRE(std::isfinite(lam[l]), "hf_mg_setup_f32: non-finite operator on level %d", l);
     mg->lv[l].lam = lam[l];
@@ -1285,7 +1551,7 @@ extern "C" int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels) {
 // launched between the two profiling events (HIP-event timing for roofline
 // reporting, see hf_profile_enable).
 static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out, const Scalars* sc, hipStream_t s,
-                  bool timed = false) {
+                  bool timed = false, bool skip_fine_post = false) {
   const int L = mg->nlev;
   const int k = mg->degree;
   float* cur[kMgMaxLevels];
@@ -1316,8 +1582,24 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       }
       mg_poly_resid<<<gg, kMgBlock, 0, s>>>(Sp, V.g, res, din, dout, sc);
     };
-    if (l == 0 && mg->bf16) down((const uint16_t*)V.Sh);
-    else down((const float*)V.S);
+    if (l == 0 && mg->sell) {
+      // SELL fine level: each pass is y = A v, then the node-wise update
+      const MatArgs m = mat_args(mg->sp);
+      const int gr = rows_grid(m.nslices);
+      for (int j = 1; j < k; ++j) {
+        mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
+        mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.c1[j], V.c2[j], res, mg->y, din, V.res, dout,
+                                                    cur[l], sc);
+        res = V.res;
+        float* tmp = din; din = dout; dout = tmp;
+      }
+      mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
+      mg_sell_resid_update<<<gg, kMgBlock, 0, s>>>(V.g.n, res, mg->y, dout, sc);
+    } else if (l == 0 && mg->bf16) {
+      down((const uint16_t*)V.Sh);
+    } else {
+      down((const float*)V.S);
+    }
     const MgLevel& C = mg->lv[l + 1];
     const bool smooth_c = (l + 2 < L);
     mg_restrict<<<node_grid(C.g.n), kMgBlock, 0, s>>>(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.s0,
@@ -1347,6 +1629,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const MgLevel& C = mg->lv[l + 1];
     const int gg = node_grid(V.g.n);
     mg_prolong<<<gg, kMgBlock, 0, s>>>(cur[l + 1], V.g, C.g, l == 0 ? mg->mask : nullptr, cur[l], sc);
+    if (l == 0 && skip_fine_post) break;
     // post-smoothing: the same polynomial on the residual after the correction
     int perr = 0;
     auto up = [&](auto Sp) {
@@ -1367,8 +1650,33 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
         float* t2 = din; din = dout; dout = t2;
       }
     };
-    if (l == 0 && mg->bf16) up((const uint16_t*)V.Sh);
-    else up((const float*)V.S);
+    if (l == 0 && mg->sell) {
+      const MatArgs m = mat_args(mg->sp);
+      const int gr = rows_grid(m.nslices);
+      if (timed) {
+        hipEvent_t e0, e1;
+        if (prof_events(&e0, &e1)) return -2;
+        hipExtLaunchKernelGGL(mg_sell_spmv, dim3(gr), dim3(kBlock), 0, s, e0, e1, 0, m, mg->svals,
+                              (const float*)cur[l], mg->y, sc);
+      } else {
+        mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, cur[l], mg->y, sc);
+      }
+      mg_sell_post_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.s0, rin(l), mg->y, cur[l], V.res, V.da, oth[l],
+                                                  sc);
+      float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
+      float* din = V.da;
+      float* dout = V.db;
+      for (int j = 1; j < k; ++j) {
+        mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
+        mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.c1[j], V.c2[j], V.res, mg->y, din, V.res, dout,
+                                                    cur[l], sc);
+        float* t2 = din; din = dout; dout = t2;
+      }
+    } else if (l == 0 && mg->bf16) {
+      up((const uint16_t*)V.Sh);
+    } else {
+      up((const float*)V.S);
+    }
     if (perr) return perr;
   }
   HF_LAUNCH_CHECK();
@@ -1376,12 +1684,13 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
   return 0;
 }
 
-extern "C" int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, hf_stream_t stream) {
+extern "C" int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, int32_t flags, hf_stream_t stream) {
   HF_REQUIRE(mg && r && z, "hf_mg_apply_f32: null argument");
   HF_REQUIRE(mg->ready, "hf_mg_apply_f32: hf_mg_setup_f32 has not run");
+  HF_REQUIRE((flags & ~HF_MG_SKIP_FINE_POST) == 0, "hf_mg_apply_f32: unknown flags 0x%x", flags);
   hipStream_t s = (hipStream_t)stream;
   float* res = nullptr;
-  if (vcycle(mg, r, z, mg->lv[0].e2, &res, nullptr, s)) return -2;
+  if (vcycle(mg, r, z, mg->lv[0].e2, &res, nullptr, s, false, (flags & HF_MG_SKIP_FINE_POST) != 0)) return -2;
   if (res != z) HF_CHECK_HIP(hipMemcpyAsync(z, res, (size_t)mg->lv[0].g.n * 3 * sizeof(float),
                                             hipMemcpyDeviceToDevice, s));
   return 0;
@@ -1391,6 +1700,7 @@ extern "C" int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf
   HF_REQUIRE(mg && dst, "hf_mg_copy_stencil: null argument");
   HF_REQUIRE(level >= 0 && level < mg->nlev, "hf_mg_copy_stencil: level %d outside 0..%d", level, mg->nlev - 1);
   HF_REQUIRE(mg->ready, "hf_mg_copy_stencil: hf_mg_setup_f32 has not run");
+  HF_REQUIRE(!(level == 0 && mg->sell), "hf_mg_copy_stencil: the fine level is the tangent itself (SELL)");
   const int64_t nv = (int64_t)kSt * mg->lv[level].g.n;
   if (level == 0 && mg->bf16) {
     mg_widen<<<(unsigned)ceil_div(nv, kMgBlock), kMgBlock, 0, (hipStream_t)stream>>>(mg->lv[0].Sh, nv, dst);

@@ -373,10 +373,12 @@ def fcg_residuals(model, A, problem, k_list, seed, gfield=None, kind="noise"):
 
 
 def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None, log_every=50, self_frac=0.5,
-          rhs_kinds=("noise",)):
+          rhs_kinds=("noise",), sampler=None):
     """self_frac: share of the steps whose residuals come from the model's own
     flexible CG (fcg_residuals) instead of Jacobi-PCG (residual_samples);
-    rhs_kinds: right-hand-side kinds (random_rhs) drawn uniformly per step."""
+    rhs_kinds: right-hand-side kinds (random_rhs) drawn uniformly per step;
+    sampler(model, A, problem, k_list, seed, gfield, kind) -> (B, N) residuals
+    replaces both sources when given."""
     """Adam on the energy loss over (A, problem) pairs; returns the loss history.
 
     matrices: callable i -> (TangentMatrix FP32, problem[, key]) or a list of such
@@ -399,8 +401,11 @@ def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None,
         ks = sorted(int(k) for k in rng.integers(0, k_max + 1, size=batch))
         sd = rng.integers(1 << 31)
         kind = rhs_kinds[int(rng.integers(len(rhs_kinds)))]
-        R = (fcg_residuals(model, A, prob, ks, sd, gfield=g, kind=kind) if rng.random() < self_frac else
-             residual_samples(A, prob, ks, seed=sd, kind=kind))
+        if sampler is not None:
+            R = sampler(model, A, prob, ks, sd, g, kind)
+        else:
+            R = (fcg_residuals(model, A, prob, ks, sd, gfield=g, kind=kind) if rng.random() < self_frac else
+                 residual_samples(A, prob, ks, seed=sd, kind=kind))
         z, cache = model.forward(A, R, g, mask, grid, keep=True)
         Az = torch.stack([A.matvec(zi) for zi in z])
         loss_per = 0.5 * (z * Az).sum(1) - (z * R).sum(1)

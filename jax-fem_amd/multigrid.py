@@ -5,7 +5,9 @@ No reference counterpart: the reference's Krylov solves use Jacobi only
 csrc/mg.hip; this module owns the device object, builds the Dirichlet mask,
 re-runs the set-up when the tangent changes and implements the fallback.
 
-    level 0      the FP32 tangent with Dirichlet DOFs masked out (M A M)
+    level 0      the FP32 tangent with Dirichlet DOFs masked out (M A M), as a 27-point
+                 stencil (HEX8) or the SELL matrix itself (HEX27, whose nodes couple two
+                 lattice steps apart; its level 1 is the HEX8 operator of the same cells)
     level l+1    P^T A_l P, P trilinear interpolation from the lattice with
                  every dimension's interval count halved (rounded up)
     coarsest     <= 88 DOF, dense FP64 inverse
@@ -34,11 +36,12 @@ from . import _lib
 
 
 def lattice(problem):
-    """Node counts (nx, ny, nz) of a structured box_mesh HEX8 problem with vec = 3."""
+    """Node counts (nx, ny, nz) of a structured box_mesh (HEX8) / box_mesh_hex27 problem with vec = 3."""
     g = getattr(problem.fe.mesh, "node_grid", None)
     if g is None or int(np.prod(g)) != problem.fe.num_total_nodes or problem.vec != 3 or \
-            problem.fe.ele_type != "HEX8":
-        raise NotImplementedError("the multigrid preconditioner needs a structured box_mesh HEX8 lattice, vec=3")
+            problem.fe.ele_type not in ("HEX8", "HEX27"):
+        raise NotImplementedError("the multigrid preconditioner needs a structured box_mesh HEX8 / HEX27 "
+                                  "lattice, vec=3")
     return tuple(int(x) for x in g)
 
 
@@ -51,14 +54,17 @@ def dirichlet_mask(problem):
 
 
 class Multigrid:
-    def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None, fine_bf16=False):
+    def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None, fine_bf16=False,
+                 fine_sell=False):
+        """fine_sell: the fine level is the tangent's SELL matrix (HEX27 lattices)."""
         _lib.load()
         self.grid = tuple(int(v) for v in grid)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
-        self.fine_bf16 = bool(fine_bf16)
+        self.fine_bf16 = bool(fine_bf16) and not fine_sell
+        self.fine_sell = bool(fine_sell)
         h = ctypes.c_void_p()
-        _lib.call("hf_mg_create", self.grid[0], self.grid[1], self.grid[2],
-                  _lib.HF_MG_FINE_BF16 if self.fine_bf16 else 0, ctypes.byref(h))
+        flags = (_lib.HF_MG_FINE_BF16 if self.fine_bf16 else 0) | (_lib.HF_MG_FINE_SELL if self.fine_sell else 0)
+        _lib.call("hf_mg_create", self.grid[0], self.grid[1], self.grid[2], flags, ctypes.byref(h))
         self.handle = h
         self.smoother_cfg = None
         self.set_smoother(smoother, degree, param)
@@ -84,9 +90,11 @@ class Multigrid:
     def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None, fine_bf16=False):
         """The problem's cached hierarchy (one per lattice) with the given smoother."""
         grid = lattice(problem)
+        sell = problem.fe.ele_type == "HEX27"
+        fine_bf16 = bool(fine_bf16) and not sell
         mg = getattr(problem, "_mg", None)
-        if mg is None or mg.grid != grid or mg.fine_bf16 != bool(fine_bf16):
-            mg = cls(grid, smoother, degree, param, device=problem.device, fine_bf16=fine_bf16)
+        if mg is None or mg.grid != grid or mg.fine_bf16 != fine_bf16 or mg.fine_sell != sell:
+            mg = cls(grid, smoother, degree, param, device=problem.device, fine_bf16=fine_bf16, fine_sell=sell)
             problem._mg = mg
         else:
             mg.set_smoother(smoother, degree, param)
@@ -160,11 +168,12 @@ class Multigrid:
         elif n > max(2 * self._iters_at_setup, self._iters_at_setup + 10):
             self._stale = True
 
-    def apply(self, r):
-        """z = V(r) (one cycle)."""
+    def apply(self, r, skip_fine_post=False):
+        """z = V(r) (one cycle); skip_fine_post stops after the fine level's coarse correction."""
         r = r.to(torch.float32).contiguous()
         z = torch.empty_like(r)
-        _lib.call("hf_mg_apply_f32", self.handle, _lib.ptr(r), _lib.ptr(z), _lib.stream())
+        _lib.call("hf_mg_apply_f32", self.handle, _lib.ptr(r), _lib.ptr(z),
+                  _lib.HF_MG_SKIP_FINE_POST if skip_fine_post else 0, _lib.stream())
         return z
 
     def stencil(self, level):

@@ -290,6 +290,87 @@ def test_mg_adjoint_matches_fp64_pcg(jfa):
     assert rel <= 1e-9
 
 
+def _hex27_problem(jfa, cells, material):
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-7)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-7)
+
+    class P(jfa.Problem):
+        def get_tensor_map(self):
+            return material
+
+    bc = [[left] * 3 + [right], [0, 1, 2, 0], [lambda p: 0.0] * 3 + [lambda p: 0.05]]
+    return P(jfa.box_mesh_hex27(*cells), vec=3, dim=3, ele_type="HEX27", quadrature_order=4, dirichlet_bc_info=bc)
+
+
+def test_hex27_level1_is_the_hex8_operator(jfa):
+    """P^T A P of the Q2 stiffness with P the interpolation from every second lattice
+    node (the cell vertices) is the Q1 (HEX8) stiffness of the same cells on a box:
+    Q1 functions are Q2 functions, and both Gauss rules integrate their form exactly."""
+    say("mg hex27 level 1 = hex8 operator")
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    cells = (4, 3, 5)
+    mat = jfa.LinearElastic(E=10.0, nu=0.3)
+    p27 = _hex27_problem(jfa, cells, mat)
+    A27 = tangent(jfa, p27, scale=0.0)
+    mg = MG.Multigrid(MG.lattice(p27), fine_sell=True)
+    mg.setup(A27, MG.dirichlet_mask(p27))
+    dims = mg.levels()
+    assert dims[0] == tuple(2 * c + 1 for c in cells) and dims[1] == tuple(c + 1 for c in cells)
+    G1 = stencil_to_csr(mg.stencil(1).cpu().numpy(), dims[1]).toarray()
+    p8 = box_problem(jfa, cells, material="lin")
+    p8.material = mat
+
+    class P8(jfa.Problem):
+        def get_tensor_map(self):
+            return mat
+
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-7)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-7)
+    p8 = P8(jfa.box_mesh(*cells), vec=3, dim=3,
+            dirichlet_bc_info=[[left] * 3 + [right], [0, 1, 2, 0], [lambda p: 0.0] * 3 + [lambda p: 0.05]])
+    A8 = tangent(jfa, p8, scale=0.0).to_scipy().toarray()
+    free = MG.dirichlet_mask(p8).cpu().numpy() > 0
+    ref = A8[np.ix_(free, free)]
+    np.testing.assert_allclose(G1[np.ix_(free, free)], ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+    # and level 1 is the Galerkin product of the masked SELL tangent (scipy)
+    m = sp.diags(MG.dirichlet_mask(p27).double().cpu().numpy())
+    As = galerkin_levels((m @ A27.to_scipy() @ m).tocsr(), dims)
+    for l in range(1, len(dims)):
+        G = stencil_to_csr(mg.stencil(l).cpu().numpy(), dims[l]).toarray()
+        r = As[l].toarray()
+        np.testing.assert_allclose(G, r, rtol=0, atol=2e-5 * np.abs(r).max(), err_msg="level %d" % l)
+
+
+def test_hex27_vcycle_and_newton(jfa):
+    say("mg hex27 vcycle + newton")
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    cells = (3, 2, 3)
+    p27 = _hex27_problem(jfa, cells, jfa.NeoHookean(E=10.0, nu=0.3))
+    A = tangent(jfa, p27, scale=0.01)
+    mg = MG.Multigrid(MG.lattice(p27), fine_sell=True)
+    mask = MG.dirichlet_mask(p27)
+    mg.setup(A, mask)
+    dims = mg.levels()
+    m = mask.double().cpu().numpy()
+    As = galerkin_levels((sp.diags(m) @ A.to_scipy() @ sp.diags(m)).tocsr(), dims)
+    Dinv = [block_dinv(As[l], m if l == 0 else None) for l in range(len(As) - 1)]
+    lams = [abs(Dinv[l] @ As[l]).sum(axis=1).max() for l in range(len(As) - 1)]
+    np.testing.assert_allclose(mg.bounds(), lams, rtol=1e-5)
+    g = torch.Generator(device=DEV).manual_seed(4)
+    r = torch.randn(p27.N, generator=g, device=DEV) * mask
+    z = mg.apply(r).double().cpu().numpy()
+    zr, _ = vcycle_ref(As, dims, m, r.double().cpu().numpy(), "chebyshev", 2, 8.0, lams)
+    np.testing.assert_allclose(z, zr, rtol=0, atol=1e-4 * np.abs(zr).max())
+    # Newton through the multigrid vs the FP64 path
+    u64 = jfa.solver(p27, {"newton": {"tol": 0.0, "rel_tol": 1e-10, "linear": {"pcg": {"tol": 1e-12, "atol": 1e-14}}}})[0]
+    p2 = _hex27_problem(jfa, cells, jfa.NeoHookean(E=10.0, nu=0.3))
+    u = jfa.solver(p2, {"newton": {"precision": "mixed", "tol": 0.0, "rel_tol": 1e-10, "linear": {"mg": {}}}})[0]
+    assert p2.solve_info["rel"] <= 1e-10 and p2.solve_info["fallbacks"] == 0
+    rel = (torch.linalg.norm(u - u64) / torch.linalg.norm(u64)).item()
+    say("hex27 newton mg rel-L2", rel, "krylov", p2.solve_info["krylov_iters"])
+    assert rel <= 1e-6
+
+
 def test_setup_rejects_non_lattice_pattern(jfa):
     say("mg non-lattice")
     MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
