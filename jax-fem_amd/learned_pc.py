@@ -372,6 +372,31 @@ def fcg_residuals(model, A, problem, k_list, seed, gfield=None, kind="noise"):
     return torch.stack(out)
 
 
+def mg_smoother_residuals(mg, A, problem, k_list, seed, kind="noise"):
+    """Residuals a fine-level multigrid post-smoother sees: for a random right-hand
+    side of the given kind, x_k after k multigrid flexible-CG iterations, r_k = b - A x_k,
+    then r_k - A e with e one V-cycle of r_k stopped after the fine level's coarse
+    correction (multigrid.Multigrid with its hierarchy set up for A).  Normalised
+    like residual_samples.  A sampler for train() (sampler=...)."""
+    gen = torch.Generator(device=A.vals.device).manual_seed(int(seed))
+    mask = LearnedPC.mask(problem)
+    b = random_rhs(A, problem, gen, kind)
+    out = []
+    for k in k_list:
+        if k == 0:
+            r = b.clone()
+        else:
+            x, _ = mg.solve(A, b, None, 0.0, 0.0, int(k))
+            r = (b - A.matvec(x)) * mask
+        e = mg.apply(r, skip_fine_post=True)
+        rp = (r - A.matvec(e)) * mask
+        if not bool(torch.isfinite(rp).all()):
+            rp = r
+        nrm = torch.sqrt((rp.double() ** 2 * A.dinv().double()).sum()).float()
+        out.append(rp / torch.clamp(nrm, min=1e-30))
+    return torch.stack(out)
+
+
 def train(model, matrices, steps, batch=4, lr=2e-3, k_max=150, seed=0, log=None, log_every=50, self_frac=0.5,
           rhs_kinds=("noise",), sampler=None):
     """self_frac: share of the steps whose residuals come from the model's own

@@ -48,6 +48,9 @@ def main():
     ap.add_argument("--domains", default="1.0", help="comma-separated field box sizes, cycled over instances")
     ap.add_argument("--self-frac", type=float, default=0.5)
     ap.add_argument("--rhs", default="noise", help="comma-separated right-hand-side kinds: noise, face, smooth")
+    ap.add_argument("--sampler", default="krylov", choices=("krylov", "mg"),
+                    help="krylov: Jacobi-PCG / own-FCG residuals (the preconditioner); mg: residuals after a "
+                         "multigrid coarse correction (a fine-level post-smoother)")
     ap.add_argument("--out", default=os.path.join(HERE, "weights", "lpc_elastic.npz"))
     args = ap.parse_args()
 
@@ -84,10 +87,19 @@ def main():
 
     model = L.LearnedPC.load(args.init) if args.init else L.LearnedPC(seed=0)
     steps = int(args.epochs * args.instances)
+    sampler = None
+    if args.sampler == "mg":
+        from jax_fem_amd.multigrid import Multigrid, dirichlet_mask
+        mg = Multigrid(L.LearnedPC.grid(prob))
+
+        def sampler(model, A, p, ks, sd, g, kind):
+            if not mg.is_current(A):
+                mg.setup(A, dirichlet_mask(p))
+            return L.mg_smoother_residuals(mg, A, p, ks, sd, kind)
     t0 = time.perf_counter()
     hist = L.train(model, matrix, steps, batch=args.batch, lr=args.lr, k_max=args.kmax,
                    log=lambda s: print("[train]", s, flush=True), log_every=max(1, steps // 40),
-                   self_frac=args.self_frac, rhs_kinds=tuple(args.rhs.split(",")))
+                   self_frac=args.self_frac, rhs_kinds=tuple(args.rhs.split(",")), sampler=sampler)
     t_train = time.perf_counter() - t0
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     model.save(args.out)
