@@ -104,3 +104,25 @@ def test_solver_options_resolution(jfa):
         s._resolve_solver_options({"newton": {}, "arc_length": {}})
     with pytest.raises(ValueError):
         s._resolve_solver_options({"newton": 3})
+
+
+def test_multigrid_lattice_detection(jfa):
+    from importlib import import_module
+    mgm = import_module("jax_fem_amd.multigrid")
+
+    class P(jfa.Problem):
+        def get_tensor_map(self):
+            return jfa.LinearElastic(E=1.0, nu=0.3)
+
+    assert mgm.lattice(P(jfa.box_mesh(3, 2, 4), vec=3, dim=3)) == (4, 3, 5)
+    assert mgm.lattice(P(jfa.box_mesh_hex27(2, 1, 2), vec=3, dim=3, ele_type="HEX27",
+                         quadrature_order=4)) == (5, 3, 5)
+    m = jfa.box_mesh(2, 2, 2)
+    with pytest.raises(NotImplementedError):   # the same cells without the lattice metadata
+        mgm.lattice(P(jfa.Mesh(m.points, m.cells, "HEX8"), vec=3, dim=3))
+    class Q(jfa.Problem):
+        def get_tensor_map(self):
+            return jfa.Poisson()
+
+    with pytest.raises(NotImplementedError):   # vec = 1
+        mgm.lattice(Q(jfa.box_mesh(2, 2, 2), vec=1, dim=3))
