@@ -507,7 +507,7 @@ def main():
     if use_mg and not args.no_pcg:
         _, mixed_out = leg(mixed, "mixed jacobi-pcg", "single run after the timed steps")
     learned_out = None
-    if not args.no_learned and wl != "elastic-hex8-20":
+    if not args.no_learned and wl not in ("elastic-hex8-20", "neohookean-hex27-48"):   # HEX8 grids only
         from jax_fem_amd.learned_pc import LearnedPC
         model = LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_elastic.npz"))
         learned = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4,

@@ -38,6 +38,7 @@ struct hf_pattern {
   int32_t* dpos;       // nnode value base of the diagonal block (-1: none)
   int32_t* sell_row;   // nslices*R node of each slice row (-1: padding row)
   int32_t* sell_pos;   // nnode slice row of each node
+  uint8_t* cslot;      // HEX8 (nn = 8): ncell*8*8 slot of block (cells[c,a], cells[c,b]) in row cells[c,a]
 };
 
 namespace hf {
@@ -216,6 +217,28 @@ __global__ void sell_tpos_fill(const int32_t* __restrict__ rowptr, const int32_t
   }
 }
 
+// For each (cell c, local row a, local column b): the index of the block
+// (cells[c,a], cells[c,b]) within its CSR row (255 if absent or beyond 254).
+__global__ void cslot_fill(const int32_t* __restrict__ cells, int64_t ncell, const int32_t* __restrict__ rowptr,
+                           const int32_t* __restrict__ colidx, uint8_t* __restrict__ cslot) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // c*8 + a
+  if (t >= ncell * 8) return;
+  const int64_t c = t >> 3;
+  const int32_t n = cells[t];
+  const int32_t b0 = rowptr[n], len = rowptr[n + 1] - b0;
+  for (int b = 0; b < 8; ++b) {
+    const int32_t m = cells[c * 8 + b];
+    int lo = 0, hi = len;   // columns of a row are sorted
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (colidx[b0 + mid] < m) lo = mid + 1;
+      else hi = mid;
+    }
+    const int j = (lo < len && colidx[b0 + lo] == m) ? lo : 255;
+    cslot[t * 8 + b] = (uint8_t)(j < 255 ? j : 255);
+  }
+}
+
 __global__ void dpos_fill(const int32_t* __restrict__ diag, const int32_t* __restrict__ blkpos, int64_t nnode,
                           int32_t* __restrict__ dpos) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -386,6 +409,7 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
   } while (0)
   p->rowptr = p->colidx = p->seg = p->contrib = p->diag = p->trans = p->brow = nullptr;
   p->slice_ptr = p->sell_col = p->sell_tpos = p->blkpos = p->dpos = p->sell_row = p->sell_pos = nullptr;
+  p->cslot = nullptr;
   p->R = (vec == 3) ? 21 : 64;
   p->nslices = (nnode + p->R - 1) / p->R;
   PB_CHECK(hipMalloc(&keys, n * sizeof(uint64_t)));
@@ -507,6 +531,11 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
     PB_CHECK(hipGetLastError());
     dpos_fill<<<grid_for(nnode), 256, 0, stream>>>(p->diag, p->blkpos, nnode, p->dpos);
     PB_CHECK(hipGetLastError());
+    if (nn == 8) {
+      PB_CHECK(hipMalloc(&p->cslot, ncell * 64));
+      cslot_fill<<<grid_for(ncell * 8), 256, 0, stream>>>(cells, ncell, p->rowptr, p->colidx, p->cslot);
+      PB_CHECK(hipGetLastError());
+    }
   }
   PB_CHECK(hipStreamSynchronize(stream));
   hipFree(keys_s); hipFree(flags); hipFree(scan); hipFree(rowcount); hipFree(tmp);
@@ -554,6 +583,7 @@ extern "C" int hf_pattern_destroy(hf_pattern* p) {
   hipFree(p->diag); hipFree(p->trans); hipFree(p->brow);
   hipFree(p->slice_ptr); hipFree(p->sell_col); hipFree(p->sell_tpos); hipFree(p->blkpos); hipFree(p->dpos);
   hipFree(p->sell_row); hipFree(p->sell_pos);
+  if (p->cslot) hipFree(p->cslot);
   delete p;
   return 0;
 }
@@ -658,6 +688,7 @@ const int32_t* pattern_trans(const hf_pattern* p) { return p->trans; }
 const int32_t* pattern_blkpos(const hf_pattern* p) { return p->blkpos; }
 const int32_t* pattern_sell_pos(const hf_pattern* p) { return p->sell_pos; }
 const int32_t* pattern_dpos(const hf_pattern* p) { return p->dpos; }
+const uint8_t* pattern_cslot(const hf_pattern* p) { return p->cslot; }
 int64_t pattern_nnode(const hf_pattern* p) { return p->nnode; }
 int64_t pattern_nblocks(const hf_pattern* p) { return p->nblocks; }
 int pattern_R(const hf_pattern* p) { return p->R; }

@@ -270,6 +270,7 @@ __global__ void __launch_bounds__(256) hex8_param_vjp_kernel(
 // of the two-stage path (hf_cell_jacobian + hf_assemble), which gives the same bits.
 struct FusedArgs {
   const int32_t *rowptr, *colidx, *seg, *contrib, *diag, *slice_ptr, *sell_row;
+  const uint8_t* cslot;  // per (cell, a): slots of the 8 column nodes in row cells[c,a]
   int64_t npos;  // slice rows (nslices * R)
   int R, max_row;
 };
@@ -310,14 +311,11 @@ __global__ void __launch_bounds__(256) hex8_assemble_fused(FusedArgs fa, const d
       const int a = (id / 8) % 8;
       T X[8][3], U[8][3];
       load_cell<T, 3>(pts, cells, u, c, X, U);
-      hex8_jacobian_row<T, MAT, 3>(X, U, params ? params + c * 8 : nullptr, mat, a, K);
+      // the slots of the 8 column nodes in this row, from the pattern's table
+      const uint2 sl = *reinterpret_cast<const uint2*>(fa.cslot + (c * 8 + a) * 8);
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const int32_t m = cells[c * 8 + b];
-        int j = 0;
-        while (j < len && fa.colidx[b0 + j] != m) ++j;
-        jslot[b] = j;
-      }
+      for (int b = 0; b < 8; ++b) jslot[b] = (int)(((b < 4 ? sl.x : sl.y) >> (8 * (b & 3))) & 0xffu);
+      hex8_jacobian_row<T, MAT, 3>(X, U, params ? params + c * 8 : nullptr, mat, a, K);
     }
     for (int round = 0; round < 8; ++round) {  // cells in ascending order
       if (mine && cl == round) {
@@ -357,6 +355,8 @@ static int launch_hex8_assemble_fused(const hf_pattern* p, const hf_material* ma
   FusedArgs fa;
   fa.rowptr = pattern_rowptr(p);
   fa.colidx = pattern_colidx(p);
+  fa.cslot = pattern_cslot(p);
+  HF_REQUIRE(fa.cslot != nullptr, "hf_assemble_hex8: the pattern has no HEX8 slot table");
   fa.seg = pattern_seg(p);
   fa.contrib = pattern_contrib(p);
   fa.diag = pattern_diag(p);

@@ -8,6 +8,7 @@
 namespace hf {
 const int32_t* pattern_rowptr(const hf_pattern* p);
 const int32_t* pattern_colidx(const hf_pattern* p);
+const uint8_t* pattern_cslot(const hf_pattern* p);
 const int32_t* pattern_trans(const hf_pattern* p);
 int64_t pattern_nnode(const hf_pattern* p);
 int64_t pattern_nblocks(const hf_pattern* p);

@@ -95,7 +95,8 @@ class LearnedPC:
     @staticmethod
     def grid(problem):
         g = getattr(problem.fe.mesh, "node_grid", None)
-        if g is None or int(np.prod(g)) != problem.fe.num_total_nodes or problem.vec != 3:
+        if g is None or int(np.prod(g)) != problem.fe.num_total_nodes or problem.vec != 3 or \
+                problem.fe.ele_type != "HEX8":
             raise NotImplementedError("the learned preconditioner needs a structured box_mesh HEX8 grid, vec=3")
         return tuple(int(x) for x in g)
 
