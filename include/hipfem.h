@@ -108,16 +108,14 @@ int64_t hf_matrix_values(const hf_pattern* p);
 int32_t hf_pattern_max_row_blocks(const hf_pattern* p);
 int hf_assemble_f64(const hf_pattern* p, const double* elem_jac, double* vals, hf_stream_t stream);
 int hf_assemble_f32(const hf_pattern* p, const float* elem_jac, float* vals, hf_stream_t stream);
-/* HEX8 elasticity (vec 3, linear elastic or Neo-Hookean): the element Jacobians
- * computed and assembled into the matrix values in one pass, without the
- * element-matrix array; same summation order and bits as hf_cell_jacobian_* +
- * hf_assemble_* (get_A over the jacfwd element matrices, problem.py:262-266,
+/* HEX8 patterns (vec 3): hf_assemble_* with the rows summed per node over its
+ * cells (8 lanes per row, the row's blocks in LDS) instead of per block over
+ * contribution segments; same summation order and bits as hf_assemble_*
+ * (get_A over the jacfwd element matrices, problem.py:262-266,
  * solver.py:466-515).  Rows must fit the LDS budget (max_row_blocks <= 42 in
- * FP64, 85 in FP32), else the call fails and the two-stage path applies. */
-int hf_assemble_hex8_f64(const hf_pattern* p, const hf_material* mat, const double* points, const int32_t* cells,
-                         const double* u, const double* params, double* vals, hf_stream_t stream);
-int hf_assemble_hex8_f32(const hf_pattern* p, const hf_material* mat, const double* points, const int32_t* cells,
-                         const double* u, const double* params, float* vals, hf_stream_t stream);
+ * FP64, 85 in FP32), else the call fails and hf_assemble_* applies. */
+int hf_assemble_hex8_f64(const hf_pattern* p, const double* elem, double* vals, hf_stream_t stream);
+int hf_assemble_hex8_f32(const hf_pattern* p, const float* elem, float* vals, hf_stream_t stream);
 /* scalar CSR values (the order of hf_pattern_csr) from matrix values */
 int hf_matrix_to_csr_f64(const hf_pattern* p, const double* vals, double* csr, hf_stream_t stream);
 

@@ -54,8 +54,8 @@ _SIGS = {
     "hf_matrix_values": (c_i64, [vp]),
     "hf_assemble_f64": (c_i32, [vp, vp, vp, vp]),
     "hf_pattern_max_row_blocks": (c_i32, [vp]),
-    "hf_assemble_hex8_f64": (c_i32, [vp, ctypes.POINTER(HFMaterial), vp, vp, vp, vp, vp, vp]),
-    "hf_assemble_hex8_f32": (c_i32, [vp, ctypes.POINTER(HFMaterial), vp, vp, vp, vp, vp, vp]),
+    "hf_assemble_hex8_f64": (c_i32, [vp, vp, vp, vp]),
+    "hf_assemble_hex8_f32": (c_i32, [vp, vp, vp, vp]),
     "hf_assemble_f32": (c_i32, [vp, vp, vp, vp]),
     "hf_matrix_to_csr_f64": (c_i32, [vp, vp, vp, vp]),
     "hf_bc_rows_f64": (c_i32, [vp, vp, c_i64, vp, vp]),

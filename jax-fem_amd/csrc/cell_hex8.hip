@@ -15,9 +15,10 @@
 //
 // Thread mapping (wave64):
 //   residual / param VJP: one thread per cell (24 DOF in registers).
-//   Jacobian: one thread per (cell, row node a): 8 consecutive lanes hold one
-//   cell, each writes its 8 x 3 x 3 row of blocks = 288 contiguous bytes (FP32);
-//   the 8 lanes of a cell write one contiguous 2304-byte element matrix.
+//   Jacobian: 8 consecutive lanes per cell; lane q first forms quadrature point
+//   q's shared data in LDS, then lane a contracts row a over the 8 points and
+//   writes its 8 x 3 x 3 row of blocks (288 contiguous bytes in FP32): the 8
+//   lanes of a cell write one contiguous 2304-byte element matrix.
 #include "common.hpp"
 #include "element_math.hpp"
 #include "hipfem.h"
@@ -130,92 +131,130 @@ __global__ void __launch_bounds__(256) hex8_residual_kernel(
     for (int i = 0; i < VEC; ++i) o[n * VEC + i] = r[n][i];
 }
 
-// Row a of the element Jacobian of one cell: K[b][i][k] = dR_(a,i) / du_(b,k).
-// Shared by the element-Jacobian kernel and the fused Jacobian + assembly kernel,
-// so both produce the same bits.
+// Element Jacobian K[a][b][i][k] = dR_(a,i) / du_(b,k), split into what a
+// quadrature point shares between the 8 rows of its cell (geometry, state,
+// F g_b and H g_b) and the contraction of one row.
+template <typename T>
+struct QpShared {
+  T g[8][3];    // physical shape gradients
+  T fg[8][3];   // F g_b (Neo-Hookean)
+  T hg[8][3];   // H g_b (Neo-Hookean)
+  T s1, s2, J, I1;  // NH: mu c w, kappa w, J, I1; linear elastic: mu w, lam w; Poisson: k w
+};
+
 template <typename T, int MAT, int VEC>
-__device__ __forceinline__ void hex8_jacobian_row(const T X[8][3], const T U[8][VEC],
-                                                  const double* __restrict__ prm, const hf_material& mat, int a,
-                                                  T K[8][VEC][VEC]) {
+__device__ __forceinline__ void qp_shared(const T X[8][3], const T U[8][VEC], const double* __restrict__ prm,
+                                          const hf_material& mat, int q, QpShared<T>& d) {
+  T g[8][3], w;
+  hex8_geometry<T>(q, X, g, w);
+  Moduli md = moduli(mat, prm ? prm[q] : 1.0, prm != nullptr);
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d.g[n][j] = g[n][j];
+  if constexpr (MAT == MAT_NEO_HOOKEAN) {
+    T ug[3][3];
+    grad_u<T, 8, 3>(U, g, ug);
+    NHState<T> s;
+    nh_state<T>(ug, s);
+    d.s1 = (T)md.m1 * s.c * w;
+    d.s2 = (T)md.m2 * w;
+    d.J = s.J;
+    d.I1 = s.I1;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      matvec3(s.F, g[n], d.fg[n]);
+      matvec3(s.H, g[n], d.hg[n]);
+    }
+  } else if constexpr (MAT == MAT_LINEAR_ELASTIC) {
+    d.s1 = (T)md.m1 * w;
+    d.s2 = (T)md.m2 * w;
+  } else {
+    d.s1 = (T)md.m1 * w;
+  }
+}
+
+template <typename T, int MAT, int VEC>
+__device__ __forceinline__ void k_row_accumulate(const QpShared<T>& d, int a, T K[8][VEC][VEC]) {
+  T ga[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) ga[j] = d.g[a][j];
+  if constexpr (MAT == MAT_NEO_HOOKEAN) {
+    const T s1 = d.s1, s2 = d.s2;
+    T Fga[3], Hga[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Fga[j] = d.fg[a][j];
+      Hga[j] = d.hg[a][j];
+    }
+    T al[3], be[3], ga_[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      al[i] = s1 * ((T)(-2.0 / 3.0) * Fga[i] + (T)(2.0 / 9.0) * d.I1 * Hga[i]) + s2 * ((T)2 * d.J - (T)1) * d.J * Hga[i];
+      be[i] = (T)(-2.0 / 3.0) * s1 * Hga[i];
+      ga_[i] = (s1 * d.I1 * (T)(1.0 / 3.0) - s2 * d.J * (d.J - (T)1)) * Hga[i];
+    }
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const T gab = s1 * (ga[0] * d.g[b][0] + ga[1] * d.g[b][1] + ga[2] * d.g[b][2]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          K[b][i][k] += al[i] * d.hg[b][k] + be[i] * d.fg[b][k] + d.hg[b][i] * ga_[k] + (i == k ? gab : (T)0);
+    }
+  } else if constexpr (MAT == MAT_LINEAR_ELASTIC) {
+    const T lw = d.s2, mw = d.s1;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const T gab = mw * (ga[0] * d.g[b][0] + ga[1] * d.g[b][1] + ga[2] * d.g[b][2]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          K[b][i][k] += lw * ga[i] * d.g[b][k] + mw * ga[k] * d.g[b][i] + (i == k ? gab : (T)0);
+    }
+  } else {  // Poisson, VEC == 1
+    const T kw = d.s1;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) K[b][0][0] += kw * (ga[0] * d.g[b][0] + ga[1] * d.g[b][1] + ga[2] * d.g[b][2]);
+  }
+}
+
+// Element Jacobians, 8 cells per wavefront: lane (cell, q) forms quadrature
+// point q's shared data in LDS, then lane (cell, a) contracts row a over the 8
+// points (each point's geometry and state computed once per cell, not once per
+// row).  Output block-major (cell, a, b, i, k): the 8 lanes of a cell write one
+// contiguous element matrix.
+template <typename T>
+constexpr int jac_waves() { return sizeof(T) == 4 ? 2 : 1; }   // LDS: 64 QpShared per wave
+
+template <typename T, int MAT, int VEC>
+__global__ void __launch_bounds__(64 * jac_waves<T>()) hex8_jacobian_kernel(
+    const double* __restrict__ pts, const int32_t* __restrict__ cells, int64_t ncell,
+    const double* __restrict__ u, const double* __restrict__ params, hf_material mat,
+    T* __restrict__ out) {
+  __shared__ QpShared<T> sh[jac_waves<T>()][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ci = lane >> 3, r = lane & 7;
+  const int64_t c = ((int64_t)blockIdx.x * jac_waves<T>() + wv) * 8 + ci;
+  if (c < ncell) {
+    T X[8][3], U[8][VEC];
+    load_cell<T, VEC>(pts, cells, u, c, X, U);
+    qp_shared<T, MAT, VEC>(X, U, params ? params + c * 8 : nullptr, mat, r, sh[wv][lane]);
+  }
+  wave_lds_sync();
+  if (c >= ncell) return;
+  T K[8][VEC][VEC];
 #pragma unroll
   for (int b = 0; b < 8; ++b)
 #pragma unroll
     for (int i = 0; i < VEC; ++i)
 #pragma unroll
       for (int k = 0; k < VEC; ++k) K[b][i][k] = 0;
-
 #pragma unroll 1
-  for (int q = 0; q < 8; ++q) {
-    T g[8][3], w;
-    hex8_geometry<T>(q, X, g, w);
-    Moduli md = moduli(mat, prm ? prm[q] : 1.0, prm != nullptr);
-    T ga[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) ga[j] = select8(g, a, j);
-    if constexpr (MAT == MAT_NEO_HOOKEAN) {
-      T ug[3][3];
-      grad_u<T, 8, 3>(U, g, ug);
-      NHState<T> s;
-      nh_state<T>(ug, s);
-      const T s1 = (T)md.m1 * s.c * w, s2 = (T)md.m2 * w;
-      T Fga[3], Hga[3];
-      matvec3(s.F, ga, Fga);
-      matvec3(s.H, ga, Hga);
-      T al[3], be[3], ga_[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        al[i] = s1 * ((T)(-2.0 / 3.0) * Fga[i] + (T)(2.0 / 9.0) * s.I1 * Hga[i]) +
-                s2 * ((T)2 * s.J - (T)1) * s.J * Hga[i];
-        be[i] = (T)(-2.0 / 3.0) * s1 * Hga[i];
-        ga_[i] = (s1 * s.I1 * (T)(1.0 / 3.0) - s2 * s.J * (s.J - (T)1)) * Hga[i];
-      }
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        T Fgb[3], Hgb[3];
-        matvec3(s.F, g[b], Fgb);
-        matvec3(s.H, g[b], Hgb);
-        const T gab = s1 * (ga[0] * g[b][0] + ga[1] * g[b][1] + ga[2] * g[b][2]);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-            K[b][i][k] += al[i] * Hgb[k] + be[i] * Fgb[k] + Hgb[i] * ga_[k] + (i == k ? gab : (T)0);
-      }
-    } else if constexpr (MAT == MAT_LINEAR_ELASTIC) {
-      const T lw = (T)md.m2 * w, mw = (T)md.m1 * w;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const T gab = mw * (ga[0] * g[b][0] + ga[1] * g[b][1] + ga[2] * g[b][2]);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-            K[b][i][k] += lw * ga[i] * g[b][k] + mw * ga[k] * g[b][i] + (i == k ? gab : (T)0);
-      }
-    } else {  // Poisson, VEC == 1
-      const T kw = (T)md.m1 * w;
-#pragma unroll
-      for (int b = 0; b < 8; ++b)
-        K[b][0][0] += kw * (ga[0] * g[b][0] + ga[1] * g[b][1] + ga[2] * g[b][2]);
-    }
-  }
-}
-
-// One thread per (cell, a).  Output block-major (cell, a, b, i, k).
-template <typename T, int MAT, int VEC>
-__global__ void __launch_bounds__(256) hex8_jacobian_kernel(
-    const double* __restrict__ pts, const int32_t* __restrict__ cells, int64_t ncell,
-    const double* __restrict__ u, const double* __restrict__ params, hf_material mat,
-    T* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t c = t >> 3;
-  const int a = (int)(t & 7);
-  if (c >= ncell) return;
-  T X[8][3], U[8][VEC];
-  load_cell<T, VEC>(pts, cells, u, c, X, U);
-  T K[8][VEC][VEC];
-  hex8_jacobian_row<T, MAT, VEC>(X, U, params ? params + c * 8 : nullptr, mat, a, K);
-  T* o = out + (c * 8 + a) * (8 * VEC * VEC);
+  for (int q = 0; q < 8; ++q) k_row_accumulate<T, MAT, VEC>(sh[wv][ci * 8 + q], r, K);
+  T* o = out + (c * 8 + r) * (8 * VEC * VEC);
 #pragma unroll
   for (int b = 0; b < 8; ++b)
 #pragma unroll
@@ -275,12 +314,15 @@ struct FusedArgs {
   int R, max_row;
 };
 
-template <typename T, int MAT>
-__global__ void __launch_bounds__(256) hex8_assemble_fused(FusedArgs fa, const double* __restrict__ pts,
-                                                           const int32_t* __restrict__ cells,
-                                                           const double* __restrict__ u,
-                                                           const double* __restrict__ params, hf_material mat,
-                                                           T* __restrict__ vals) {
+// HEX8 assembly of element matrices into the SELL values, 8 lanes per slice row
+// (one per cell of the node, the cells in ascending order as the pattern's
+// contribution segments list them): each lane reads its cell's row a of
+// blocks (288 contiguous bytes in FP32), the row's slots are accumulated in
+// LDS in 8 rounds of ascending cell index (the summation order of
+// hf_assemble_*), then written column-major to the row's SELL slots.
+template <typename T>
+__global__ void __launch_bounds__(256) hex8_assemble_rows(FusedArgs fa, const T* __restrict__ elem,
+                                                          T* __restrict__ vals) {
   extern __shared__ unsigned char hf_smem[];
   T* acc = reinterpret_cast<T*>(hf_smem);  // [32 rows][max_row][9], row-major blocks
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -288,10 +330,9 @@ __global__ void __launch_bounds__(256) hex8_assemble_fused(FusedArgs fa, const d
   const int64_t q = ((int64_t)blockIdx.x * 4 + wv) * 8 + ln;
   T* my = acc + (int64_t)(wv * 8 + ln) * fa.max_row * 9;
   const int32_t n = q < fa.npos ? fa.sell_row[q] : -1;
-  int32_t b0 = 0, len = 0, ncl = 0, s0 = 0;
+  int32_t len = 0, ncl = 0, s0 = 0;
   if (n >= 0) {
-    b0 = fa.rowptr[n];
-    len = fa.rowptr[n + 1] - b0;
+    len = fa.rowptr[n + 1] - fa.rowptr[n];
     const int32_t db = fa.diag[n];
     if (db >= 0) {
       s0 = fa.seg[db];
@@ -303,19 +344,20 @@ __global__ void __launch_bounds__(256) hex8_assemble_fused(FusedArgs fa, const d
   for (int cbase = 0; __ballot(ncl > cbase) != 0; cbase += 8) {
     const int ci = cbase + cl;
     const bool mine = ci < ncl;
-    T K[8][3][3];
+    T K[8][9];
     int jslot[8];
     if (mine) {
       const int32_t id = fa.contrib[s0 + ci];
       const int64_t c = id / 64;
       const int a = (id / 8) % 8;
-      T X[8][3], U[8][3];
-      load_cell<T, 3>(pts, cells, u, c, X, U);
-      // the slots of the 8 column nodes in this row, from the pattern's table
       const uint2 sl = *reinterpret_cast<const uint2*>(fa.cslot + (c * 8 + a) * 8);
 #pragma unroll
       for (int b = 0; b < 8; ++b) jslot[b] = (int)(((b < 4 ? sl.x : sl.y) >> (8 * (b & 3))) & 0xffu);
-      hex8_jacobian_row<T, MAT, 3>(X, U, params ? params + c * 8 : nullptr, mat, a, K);
+      const T* src = elem + (c * 8 + a) * 72;
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+#pragma unroll
+        for (int e = 0; e < 9; ++e) K[b][e] = src[b * 9 + e];
     }
     for (int round = 0; round < 8; ++round) {  // cells in ascending order
       if (mine && cl == round) {
@@ -324,9 +366,7 @@ __global__ void __launch_bounds__(256) hex8_assemble_fused(FusedArgs fa, const d
           if (jslot[b] < len) {
             T* dst = my + jslot[b] * 9;
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-              for (int k = 0; k < 3; ++k) dst[i * 3 + k] += K[b][i][k];
+            for (int e = 0; e < 9; ++e) dst[e] += K[b][e];
           }
         }
       }
@@ -346,12 +386,8 @@ __global__ void __launch_bounds__(256) hex8_assemble_fused(FusedArgs fa, const d
 }
 
 template <typename T>
-static int launch_hex8_assemble_fused(const hf_pattern* p, const hf_material* mat, const double* pts,
-                                      const int32_t* cells, const double* u, const double* params, T* vals,
-                                      hipStream_t s) {
+static int launch_hex8_assemble_rows(const hf_pattern* p, const T* elem, T* vals, hipStream_t s) {
   HF_REQUIRE(pattern_vec(p) == 3 && pattern_nn(p) == 8, "hf_assemble_hex8: needs a HEX8 vec-3 pattern");
-  HF_REQUIRE(mat->kind == MAT_LINEAR_ELASTIC || mat->kind == MAT_NEO_HOOKEAN,
-             "hf_assemble_hex8: material %d not supported", mat->kind);
   FusedArgs fa;
   fa.rowptr = pattern_rowptr(p);
   fa.colidx = pattern_colidx(p);
@@ -368,10 +404,7 @@ static int launch_hex8_assemble_fused(const hf_pattern* p, const hf_material* ma
   const size_t smem = (size_t)32 * fa.max_row * 9 * sizeof(T);
   HF_REQUIRE(smem <= 96 * 1024, "hf_assemble_hex8: rows of %d blocks exceed the LDS budget", fa.max_row);
   const unsigned grid = (unsigned)ceil_div(fa.npos, 32);
-  if (mat->kind == MAT_NEO_HOOKEAN)
-    hex8_assemble_fused<T, MAT_NEO_HOOKEAN><<<grid, 256, smem, s>>>(fa, pts, cells, u, params, *mat, vals);
-  else
-    hex8_assemble_fused<T, MAT_LINEAR_ELASTIC><<<grid, 256, smem, s>>>(fa, pts, cells, u, params, *mat, vals);
+  hex8_assemble_rows<T><<<grid, 256, smem, s>>>(fa, elem, vals);
   HF_LAUNCH_CHECK();
   return 0;
 }
@@ -406,7 +439,7 @@ static int launch_hex8_jacobian(const hf_material* mat, const double* pts, const
                                 int64_t ncell, const double* u, const double* params, T* out,
                                 hipStream_t s) {
   if (ncell == 0) return 0;
-  const dim3 blk(256), grd((unsigned)ceil_div(ncell * 8, 256));
+  const dim3 blk(64 * jac_waves<T>()), grd((unsigned)ceil_div(ncell, 8 * jac_waves<T>()));
   switch (mat->kind) {
     case MAT_POISSON:
       hex8_jacobian_kernel<T, MAT_POISSON, 1><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
@@ -465,15 +498,11 @@ int hex8_jacobian_f32(const hf_material* m, const double* p, const int32_t* c, i
 
 }  // namespace hf
 
-extern "C" int hf_assemble_hex8_f32(const hf_pattern* p, const hf_material* m, const double* pts, const int32_t* cells,
-                                    const double* u, const double* params, float* vals, hf_stream_t s) {
-  HF_REQUIRE(p && m && pts && cells && u && vals, "hf_assemble_hex8_f32: null argument");
-  HF_REQUIRE(m->quad1d == 0 || m->quad1d == 2, "hf_assemble_hex8_f32: HEX8 kernels use 2x2x2 Gauss points");
-  return hf::launch_hex8_assemble_fused<float>(p, m, pts, cells, u, params, vals, (hipStream_t)s);
+extern "C" int hf_assemble_hex8_f32(const hf_pattern* p, const float* elem, float* vals, hf_stream_t s) {
+  HF_REQUIRE(p && elem && vals, "hf_assemble_hex8_f32: null argument");
+  return hf::launch_hex8_assemble_rows<float>(p, elem, vals, (hipStream_t)s);
 }
-extern "C" int hf_assemble_hex8_f64(const hf_pattern* p, const hf_material* m, const double* pts, const int32_t* cells,
-                                    const double* u, const double* params, double* vals, hf_stream_t s) {
-  HF_REQUIRE(p && m && pts && cells && u && vals, "hf_assemble_hex8_f64: null argument");
-  HF_REQUIRE(m->quad1d == 0 || m->quad1d == 2, "hf_assemble_hex8_f64: HEX8 kernels use 2x2x2 Gauss points");
-  return hf::launch_hex8_assemble_fused<double>(p, m, pts, cells, u, params, vals, (hipStream_t)s);
+extern "C" int hf_assemble_hex8_f64(const hf_pattern* p, const double* elem, double* vals, hf_stream_t s) {
+  HF_REQUIRE(p && elem && vals, "hf_assemble_hex8_f64: null argument");
+  return hf::launch_hex8_assemble_rows<double>(p, elem, vals, (hipStream_t)s);
 }

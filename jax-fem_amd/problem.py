@@ -252,26 +252,23 @@ class Problem:
         prm = self._params()
         s = _lib.stream()
         vals = torch.empty(p.nvals, dtype=dtype, device=self.device)
+        ej = self._buf("elem_jac" + tag, self.num_cells * self.nn * self.nn * bb, dtype)
+        _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
         if self._fused_ok(dtype):
-            # element Jacobians assembled in the same pass (no element-matrix array)
-            _lib.call("hf_assemble_hex8_f" + tag, p.handle, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-                      _lib.ptr(u), _lib.ptr(prm), _lib.ptr(vals), s)
+            # HEX8: rows summed per node over its cells (same order and bits as hf_assemble)
+            _lib.call("hf_assemble_hex8_f" + tag, p.handle, _lib.ptr(ej), _lib.ptr(vals), s)
         else:
-            ej = self._buf("elem_jac" + tag, self.num_cells * self.nn * self.nn * bb, dtype)
-            _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-                      self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
             _lib.call("hf_assemble_f" + tag, p.handle, _lib.ptr(ej), _lib.ptr(vals), s)
         if bc:
             dofs, _ = self.bc_arrays()
             _lib.call("hf_bc_rows_f" + tag, p.handle, _lib.ptr(dofs), dofs.numel(), _lib.ptr(vals), s)
         return vals
 
-    fused_assembly = True   # HEX8 elasticity: hf_assemble_hex8 instead of element matrices + hf_assemble
+    fused_assembly = True   # HEX8 vec-3 patterns: hf_assemble_hex8 (per-node rows) instead of hf_assemble
 
     def _fused_ok(self, dtype):
         if not (self.fused_assembly and self.nn == 8 and self.vec == 3):
-            return False
-        if type(self.material).__name__ not in ("LinearElastic", "NeoHookean"):
             return False
         limit = 42 if dtype == torch.float64 else 85   # LDS: 32 rows x max_row x 9 values
         return self.pattern.max_row_blocks <= limit
