@@ -469,11 +469,14 @@ def main():
         kname = "mg_sell_spmv (fine-level smoothing product y = A v, SELL column-form node blocks)"
         pmc = "mg_sell_spmv_hex27_48_pmc.md"
     elif use_mg:
-        # one pass of the fine-level smoother (mg_post_first): the 27-point node-block
-        # stencil (243 FP32 values per node), the 3x3 block-Jacobi inverse (9), and the
-        # node's r, e_in (read) and res, d, e_out (written): 3 FP32 each
-        kbytes = prob.N // 3 * (243 * 4 + 9 * 4 + 5 * 12)
-        kname = "mg_post_first (fine-level Chebyshev smoothing pass: 27-point node-block stencil, slot-major FP32)"
+        # one pass of the fine-level smoother (mg_post_first): the symmetric 27-point
+        # node-block stencil stored as its diagonal and 13 upper blocks (126 FP32
+        # values per node; the lower blocks are the neighbours' upper ones, read
+        # once each in the ideal), the 3x3 block-Jacobi inverse (9), and the node's
+        # r, e_in (read) and res, d, e_out (written): 3 FP32 each
+        kbytes = prob.N // 3 * (126 * 4 + 9 * 4 + 5 * 12)
+        kname = ("mg_post_first<true> (fine-level Chebyshev smoothing pass: half-stored symmetric "
+                 "27-point node-block stencil, slot-major FP32)")
         pmc = "mg_post_first_hex8_64_pmc.md"
     else:
         # one PCG SpMV (w = A u): values (4 B/nnz), one block column per node-pair

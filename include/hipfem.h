@@ -229,19 +229,17 @@ int hf_lpc_solve_f32(const hf_pattern* p, const float* vals, const float* dinv, 
  *      block Jacobi (param = omega). */
 typedef struct hf_mg hf_mg;
 enum { HF_MG_JACOBI = 0, HF_MG_CHEBYSHEV = 1 };
-/* HF_MG_FINE_BF16: the fine-level stencil the smoother and the first Galerkin
- * product read is stored in BF16 (rounded to nearest even): half the bytes of
- * the cycle's largest stream; coarser levels stay FP32.  The rounding (2^-8
- * relative per entry) perturbs smooth modes by about 2^-8 n^2 relative on an
- * n^3 lattice, so it only pays on small lattices (at 64^3 the iteration count
- * triples). */
-/* HF_MG_FINE_SELL: the fine level is the tangent's SELL matrix itself (elements
+/* The fine-level stencil of a symmetric operator is stored half: the diagonal and
+ * the 13 upper offsets, block (n, o) of a lower offset read as the transpose of
+ * the neighbour's (half the bytes of the cycle's largest stream).
+ * HF_MG_FINE_FULL stores all 27 slots (a non-symmetric operator).
+ * HF_MG_FINE_SELL: the fine level is the tangent's SELL matrix itself (elements
  * whose nodes couple two lattice steps apart: HEX27 on its (2n+1)^3 node lattice).
  * The smoother then runs SELL products, and level 1 = P^T A P is formed from the
  * SELL rows; with P the interpolation from every second node it is the 27-point
  * HEX8 operator of the same cells.  Couplings whose coarse parents are more than
  * one coarse step apart are a set-up error. */
-enum { HF_MG_FINE_BF16 = 1, HF_MG_FINE_SELL = 2 };
+enum { HF_MG_FINE_FULL = 1, HF_MG_FINE_SELL = 2 };
 int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out);   /* nodes per dimension */
 int hf_mg_destroy(hf_mg* mg);
 /* number of levels; dims (3 per level, up to max_levels) receives the node counts */
@@ -260,7 +258,7 @@ int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
  * fine-level post-smoother sees) */
 enum { HF_MG_SKIP_FINE_POST = 1 };
 int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, int32_t flags, hf_stream_t stream);
-/* copy level `level`'s stencil (243 * nnode floats; a BF16 level widened) into dst */
+/* copy level `level`'s stencil (243 * nnode floats, all 27 slots; a half-stored level expanded) into dst */
 int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf_stream_t stream);
 /* flexible CG (one kept direction) preconditioned by V-cycles; the stop rule,
  * flags (HF_KRYLOV_REL_R0; no transpose) and info of hf_krylov_solve.

@@ -81,21 +81,36 @@ __device__ __forceinline__ int64_t lat_id(const Lat& g, int x, int y, int z) {
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Stencil values are FP32, or on the fine level optionally BF16 held as raw
-// 16-bit patterns (the upper half of the FP32 word; stored rounded to nearest
-// even).  The smoother only needs the operator to about 3 digits, and the fine
-// stencil is the largest stream of every cycle.
-__device__ __forceinline__ float ldv(const float* p) { return *p; }
-__device__ __forceinline__ float ldv(const uint16_t* p) { return __uint_as_float((uint32_t)(*p) << 16); }
-__device__ __forceinline__ void stv(float* p, float v) { *p = v; }
-__device__ __forceinline__ void stv(uint16_t* p, float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7f800000u) == 0x7f800000u) {
-    *p = (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));   // inf stays inf, NaN stays NaN
-    return;
+// The fine level of a symmetric operator stores only the diagonal and the 13
+// upper offsets (o >= 13) of its stencil: block (n, o) of a lower offset is the
+// transpose of block (n + off(o), 26 - o), the neighbour's upper slot 13 - o.
+// Half the bytes of the cycle's largest stream; stencil_row reads a neighbour's
+// slot at the step its owner reads it, so most of those second reads hit L2.
+constexpr int kHalfSlots = 14;
+__host__ __device__ constexpr int stored_slots(bool half) { return half ? kHalfSlots : 27; }
+__host__ __device__ constexpr int diag_slot(bool half) { return half ? 0 : 13; }
+
+// Block (n, o) of a stencil with N nodes into B (row-major); m is the (clamped)
+// neighbour index and valid whether the neighbour is inside the lattice.  Full
+// stencils hold zeros in slots that leave the lattice.
+template <bool HALF>
+__device__ __forceinline__ void load_block(const float* __restrict__ S, int64_t N, int64_t n, int o, int64_t m,
+                                           bool valid, float (&B)[9]) {
+  if constexpr (!HALF) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) B[e] = S[(int64_t)(o * 9 + e) * N + n];
+  } else {
+    if (o >= 13) {
+#pragma unroll
+      for (int e = 0; e < 9; ++e) B[e] = S[(int64_t)((o - 13) * 9 + e) * N + n];
+    } else {
+      const float vf = valid ? 1.f : 0.f;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) B[3 * i + j] = vf * S[(int64_t)((13 - o) * 9 + 3 * j + i) * N + m];
+    }
   }
-  u += 0x7fffu + ((u >> 16) & 1u);
-  *p = (uint16_t)(u >> 16);
 }
 
 // 1-D transfer between nf fine intervals and their coarse nodes.
@@ -121,29 +136,41 @@ inline int node_grid(int64_t n) {
   return (int)(g < 1 ? 1 : g);
 }
 
-// out = sum over the 27 slots of S[n] v[neighbour]  (clamped neighbours)
-template <typename ST>
-__device__ __forceinline__ void stencil_row(const ST* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
+// out = sum over the 27 slots of S[n] v[neighbour]  (clamped neighbours).
+// Offsets go in mirrored pairs (p, 26 - p).  With half storage both blocks of a
+// pair come from stored slot 13 - p (node n's own, and its neighbour
+// n + off(p)'s transposed), and every node reads that slot at the same step,
+// so the neighbour's copy is read while its owner's wave fetches it as well;
+// in offset order the second read came half a kernel later and mostly missed
+// L2 (fine-pass fetch 290 -> 210 MB at 65^3 nodes).  The pair loop is not
+// unrolled: unrolled, the compiler hoists all 27 block loads (256 VGPRs).
+template <bool HALF>
+__device__ __forceinline__ void stencil_row(const float* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
                                             const float* __restrict__ v, float (&out)[3]) {
-  out[0] = out[1] = out[2] = 0.f;
+  {
+    float B[9];
+    load_block<HALF>(S, g.n, n, 13, n, true, B);
+    const float v0 = v[3 * n], v1 = v[3 * n + 1], v2 = v[3 * n + 2];
 #pragma unroll
-  for (int dx = 0; dx < 3; ++dx) {
-    const int xx = clampi(x + dx - 1, 0, g.nx - 1);
+    for (int i = 0; i < 3; ++i) out[i] = B[3 * i] * v0 + B[3 * i + 1] * v1 + B[3 * i + 2] * v2;
+  }
+#pragma unroll 1
+  for (int p = 0; p < 13; ++p) {
+    const int ox = p / 9 - 1, oy = (p / 3) % 3 - 1, oz = p % 3 - 1;
+    const int xl = x + ox, yl = y + oy, zl = z + oz;
+    const int xu = x - ox, yu = y - oy, zu = z - oz;
+    const int64_t ml = lat_id(g, clampi(xl, 0, g.nx - 1), clampi(yl, 0, g.ny - 1), clampi(zl, 0, g.nz - 1));
+    const int64_t mu = lat_id(g, clampi(xu, 0, g.nx - 1), clampi(yu, 0, g.ny - 1), clampi(zu, 0, g.nz - 1));
+    const bool valid_l = xl >= 0 && xl < g.nx && yl >= 0 && yl < g.ny && zl >= 0 && zl < g.nz;
+    float Bl[9], Bu[9];
+    load_block<HALF>(S, g.n, n, p, ml, valid_l, Bl);
+    load_block<HALF>(S, g.n, n, 26 - p, mu, true, Bu);
+    const float l0 = v[3 * ml], l1 = v[3 * ml + 1], l2 = v[3 * ml + 2];
+    const float u0 = v[3 * mu], u1 = v[3 * mu + 1], u2 = v[3 * mu + 2];
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy) {
-      const int yy = clampi(y + dy - 1, 0, g.ny - 1);
-#pragma unroll
-      for (int dz = 0; dz < 3; ++dz) {
-        const int zz = clampi(z + dz - 1, 0, g.nz - 1);
-        const int o = dx * 9 + dy * 3 + dz;
-        const int64_t m = lat_id(g, xx, yy, zz) * 3;
-        const float v0 = v[m], v1 = v[m + 1], v2 = v[m + 2];
-        const ST* s = S + (int64_t)o * 9 * g.n + n;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-          out[i] += ldv(s + (3 * i) * g.n) * v0 + ldv(s + (3 * i + 1) * g.n) * v1 + ldv(s + (3 * i + 2) * g.n) * v2;
-      }
-    }
+    for (int i = 0; i < 3; ++i)
+      out[i] += (Bl[3 * i] * l0 + Bl[3 * i + 1] * l1 + Bl[3 * i + 2] * l2) +
+                (Bu[3 * i] * u0 + Bu[3 * i + 1] * u1 + Bu[3 * i + 2] * u2);
   }
 }
 
@@ -214,15 +241,16 @@ __global__ void __launch_bounds__(kMgBlock) mg_check_lattice(const int32_t* __re
   if (colidx[r0 + ls.k] != (int32_t)ls.c || blkpos[r0 + ls.k] != pos) bad[0] = 1;
 }
 
-template <typename ST>
+template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_fine_stencil(const int32_t* __restrict__ rowptr,
                                                             const int32_t* __restrict__ sell_pos,
                                                             const int32_t* __restrict__ slice_ptr, int R,
                                                             const float* __restrict__ vals,
                                                             const float* __restrict__ mask, Lat g,
-                                                            ST* __restrict__ S) {
+                                                            float* __restrict__ S) {
   const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
-  const int o = blockIdx.y;
+  const int so = blockIdx.y;                 // stored slot
+  const int o = HALF ? 13 + so : so;         // its offset
   if (n >= g.n) return;
   const LatSlot ls = lat_slot(g, n, o);
   float out[9];
@@ -239,25 +267,37 @@ __global__ void __launch_bounds__(kMgBlock) mg_fine_stencil(const int32_t* __res
       for (int j = 0; j < 3; ++j) out[3 * i + j] = v[3 * j + i] * mr[i] * mc[j];
   }
 #pragma unroll
-  for (int e = 0; e < 9; ++e) stv(S + (int64_t)(o * 9 + e) * g.n + n, out[e]);
+  for (int e = 0; e < 9; ++e) S[(int64_t)(so * 9 + e) * g.n + n] = out[e];
 }
 
-__global__ void __launch_bounds__(kMgBlock) mg_widen(const uint16_t* __restrict__ h, int64_t n, float* __restrict__ f) {
-  const int64_t i = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
-  if (i < n) f[i] = ldv(h + i);
+// the full 27-slot stencil of a half-stored one (read-back)
+__global__ void __launch_bounds__(kMgBlock) mg_expand_half(const float* __restrict__ Sh, Lat g,
+                                                           float* __restrict__ S) {
+  const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  const int o = blockIdx.y;
+  if (n >= g.n) return;
+  int x, y, z;
+  lat_xyz(g, n, x, y, z);
+  const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
+  const bool valid = xr >= 0 && xr < g.nx && yr >= 0 && yr < g.ny && zr >= 0 && zr < g.nz;
+  const int64_t m = lat_id(g, clampi(xr, 0, g.nx - 1), clampi(yr, 0, g.ny - 1), clampi(zr, 0, g.nz - 1));
+  float B[9];
+  load_block<true>(Sh, g.n, n, o, m, valid, B);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) S[(int64_t)(o * 9 + e) * g.n + n] = valid ? B[e] : 0.f;
 }
 
 // Inverse of each node's diagonal block (slot 13); masked DOFs (mask 0) get a
 // unit diagonal.  A block that is not safely invertible falls back to the
 // inverse of its positive diagonal entries (0 elsewhere).
-template <typename ST>
-__global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const ST* __restrict__ S, const float* __restrict__ mask,
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restrict__ S, const float* __restrict__ mask,
                                                           int64_t n, float* __restrict__ Dinv) {
   const int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
   if (node >= n) return;
   double a[9];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) a[e] = (double)ldv(S + (int64_t)(13 * 9 + e) * n + node);
+  for (int e = 0; e < 9; ++e) a[e] = (double)S[(int64_t)(diag_slot(HALF) * 9 + e) * n + node];
   if (mask) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -290,8 +330,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const ST* __restrict__
 // Stage 1 of the Galerkin product: B = A P over fine rows i, with slot s of
 // row i holding the block of coarse column J = floor(i/2) + s - 1 (per
 // dimension).  One thread per (row, slot); the slot is blockIdx.y.
-template <typename ST>
-__global__ void __launch_bounds__(kMgBlock) mg_rap_ap(const ST* __restrict__ S, Lat f, Lat c,
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_rap_ap(const float* __restrict__ S, Lat f, Lat c,
                                                       float* __restrict__ B) {
   const int64_t i = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
   const int slot = blockIdx.y;
@@ -322,9 +362,11 @@ __global__ void __launch_bounds__(kMgBlock) mg_rap_ap(const ST* __restrict__ S, 
       for (int oz = 0; oz < 3; ++oz) {
         const float w = W[0][ox] * W[1][oy] * W[2][oz];
         if (w == 0.f) continue;
-        const ST* sp = S + (int64_t)((ox * 9 + oy * 3 + oz) * 9) * f.n + i;
+        float B[9];
+        load_block<HALF>(S, f.n, i, ox * 9 + oy * 3 + oz,
+                         lat_id(f, fi[0] + ox - 1, fi[1] + oy - 1, fi[2] + oz - 1), true, B);
 #pragma unroll
-        for (int e = 0; e < 9; ++e) acc[e] += w * ldv(sp + (int64_t)e * f.n);
+        for (int e = 0; e < 9; ++e) acc[e] += w * B[e];
       }
 #pragma unroll
   for (int e = 0; e < 9; ++e) B[(int64_t)(slot * 9 + e) * f.n + i] = acc[e];
@@ -470,15 +512,15 @@ __device__ __forceinline__ void poly_first_node(const float* __restrict__ Dinv, 
 
 // res = res_in - A d_in;  d_out = c1 d_in + c2 Dinv res;  e += d_out
 // (res_in may alias res_out: each node reads and writes only its own entries)
-template <typename ST>
-__device__ __forceinline__ void poly_step_node(const ST* __restrict__ S, const float* __restrict__ Dinv,
+template <bool HALF>
+__device__ __forceinline__ void poly_step_node(const float* __restrict__ S, const float* __restrict__ Dinv,
                                                const Lat& g, float c1, float c2, const float* res_in,
                                                const float* __restrict__ d_in, float* res_out,
                                                float* __restrict__ d_out, float* __restrict__ e, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ad[3];
-  stencil_row(S, g, node, x, y, z, d_in, ad);
+  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad);
   const float res[3] = {res_in[3 * node] - ad[0], res_in[3 * node + 1] - ad[1], res_in[3 * node + 2] - ad[2]};
   float dr[3];
   block_apply(Dinv, g.n, node, res, dr);
@@ -492,28 +534,28 @@ __device__ __forceinline__ void poly_step_node(const ST* __restrict__ S, const f
 }
 
 // t = res_in - A d_in (the residual after the last pre-smoothing step)
-template <typename ST>
-__device__ __forceinline__ void poly_resid_node(const ST* __restrict__ S, const Lat& g,
+template <bool HALF>
+__device__ __forceinline__ void poly_resid_node(const float* __restrict__ S, const Lat& g,
                                                 const float* __restrict__ res_in, const float* __restrict__ d_in,
                                                 float* __restrict__ t, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ad[3];
-  stencil_row(S, g, node, x, y, z, d_in, ad);
+  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad);
 #pragma unroll
   for (int i = 0; i < 3; ++i) t[3 * node + i] = res_in[3 * node + i] - ad[i];
 }
 
 // post-smoother start: res = r - A e_in;  d = s0 Dinv res;  e_out = e_in + d
-template <typename ST>
-__device__ __forceinline__ void post_first_node(const ST* __restrict__ S, const float* __restrict__ Dinv,
+template <bool HALF>
+__device__ __forceinline__ void post_first_node(const float* __restrict__ S, const float* __restrict__ Dinv,
                                                 const Lat& g, float s0, const float* __restrict__ r,
                                                 const float* __restrict__ e_in, float* __restrict__ res,
                                                 float* __restrict__ d, float* __restrict__ e_out, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ae[3];
-  stencil_row(S, g, node, x, y, z, e_in, ae);
+  stencil_row<HALF>(S, g, node, x, y, z, e_in, ae);
   const float rr[3] = {r[3 * node] - ae[0], r[3 * node + 1] - ae[1], r[3 * node + 2] - ae[2]};
   float dr[3];
   block_apply(Dinv, g.n, node, rr, dr);
@@ -624,33 +666,33 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_first(const float* __restric
   MG_NODE_LOOP(n, poly_first_node(Dinv, n, s0, r, d, e, node));
 }
 
-template <typename ST>
-__global__ void __launch_bounds__(kMgBlock) mg_poly_step(const ST* __restrict__ S, const float* __restrict__ Dinv,
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_step(const float* __restrict__ S, const float* __restrict__ Dinv,
                                                          Lat g, float c1, float c2, const float* res_in,
                                                          const float* __restrict__ d_in, float* res_out,
                                                          float* __restrict__ d_out, float* __restrict__ e,
                                                          const Scalars* sc) {
   if (sc && sc->done) return;
-  MG_NODE_LOOP(g.n, poly_step_node(S, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
+  MG_NODE_LOOP(g.n, poly_step_node<HALF>(S, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
 }
 
-template <typename ST>
-__global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const ST* __restrict__ S, Lat g,
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const float* __restrict__ S, Lat g,
                                                           const float* __restrict__ res_in,
                                                           const float* __restrict__ d_in, float* __restrict__ t,
                                                           const Scalars* sc) {
   if (sc && sc->done) return;
-  MG_NODE_LOOP(g.n, poly_resid_node(S, g, res_in, d_in, t, node));
+  MG_NODE_LOOP(g.n, poly_resid_node<HALF>(S, g, res_in, d_in, t, node));
 }
 
-template <typename ST>
-__global__ void __launch_bounds__(kMgBlock) mg_post_first(const ST* __restrict__ S, const float* __restrict__ Dinv,
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_post_first(const float* __restrict__ S, const float* __restrict__ Dinv,
                                                           Lat g, float s0, const float* __restrict__ r,
                                                           const float* __restrict__ e_in, float* __restrict__ res,
                                                           float* __restrict__ d, float* __restrict__ e_out,
                                                           const Scalars* sc) {
   if (sc && sc->done) return;
-  MG_NODE_LOOP(g.n, post_first_node(S, Dinv, g, s0, r, e_in, res, d, e_out, node));
+  MG_NODE_LOOP(g.n, post_first_node<HALF>(S, Dinv, g, s0, r, e_in, res, d, e_out, node));
 }
 
 __global__ void __launch_bounds__(kMgBlock) mg_restrict(const float* __restrict__ t, Lat f, Lat c,
@@ -670,20 +712,25 @@ __global__ void __launch_bounds__(kMgBlock) mg_prolong(const float* __restrict__
 
 // Gershgorin bound of Dinv A: per-workgroup max over nodes of the largest
 // absolute row sum of the node's block row of Dinv A.
-template <typename ST>
-__global__ void __launch_bounds__(kMgBlock) mg_gersh(const ST* __restrict__ S, const float* __restrict__ Dinv,
-                                                     int64_t n, float* __restrict__ part) {
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S, const float* __restrict__ Dinv, Lat g,
+                                                     float* __restrict__ part) {
   __shared__ float sm[kMgBlock / kWave];
+  const int64_t n = g.n;
   float best = 0.f;
   for (int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x; node < n; node += (int64_t)gridDim.x * kMgBlock) {
     float M[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) M[e] = Dinv[(int64_t)e * n + node];
+    int x, y, z;
+    lat_xyz(g, node, x, y, z);
     float rs[3] = {0.f, 0.f, 0.f};
     for (int o = 0; o < 27; ++o) {
+      const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
+      const bool valid = xr >= 0 && xr < g.nx && yr >= 0 && yr < g.ny && zr >= 0 && zr < g.nz;
+      const int64_t m = lat_id(g, clampi(xr, 0, g.nx - 1), clampi(yr, 0, g.ny - 1), clampi(zr, 0, g.nz - 1));
       float B[9];
-#pragma unroll
-      for (int e = 0; e < 9; ++e) B[e] = ldv(S + (int64_t)(o * 9 + e) * n + node);
+      load_block<HALF>(S, n, node, o, m, valid, B);
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -763,22 +810,29 @@ struct SmallCycle {
   SmallLevel lv[kSmallMax];
 };
 
-// product of node n's stencil row with v, by the 8 lanes of a group (q = lane in group)
+// product of node n's stencil row with v, by the 8 lanes of a group (q = lane
+// in group); every lane of the group ends with the whole sum.  UNROLL issues a
+// lane's 3-4 block loads together (the grid kernels); in the 1024-thread small
+// cycle the unrolled form exceeds the 128 VGPRs a lane has there and spills.
+template <bool HALF, bool UNROLL>
 __device__ __forceinline__ void stencil_row_group(const float* __restrict__ S, const Lat& g, int64_t n,
                                                   const float* __restrict__ v, int q, float (&out)[3]) {
   int x, y, z;
   lat_xyz(g, n, x, y, z);
   out[0] = out[1] = out[2] = 0.f;
-  for (int o = q; o < 27; o += kGroup) {
-    const int xx = clampi(x + o / 9 - 1, 0, g.nx - 1);
-    const int yy = clampi(y + (o / 3) % 3 - 1, 0, g.ny - 1);
-    const int zz = clampi(z + o % 3 - 1, 0, g.nz - 1);
-    const int64_t m = lat_id(g, xx, yy, zz) * 3;
-    const float v0 = v[m], v1 = v[m + 1], v2 = v[m + 2];
-    const float* sp = S + (int64_t)o * 9 * g.n + n;
+  constexpr int kUnroll = UNROLL ? 4 : 1;
+#pragma unroll kUnroll
+  for (int t = 0; t < (27 + kGroup - 1) / kGroup; ++t) {
+    const int o = q + t * kGroup;
+    if (o >= 27) break;
+    const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
+    const int xx = clampi(xr, 0, g.nx - 1), yy = clampi(yr, 0, g.ny - 1), zz = clampi(zr, 0, g.nz - 1);
+    const int64_t m = lat_id(g, xx, yy, zz);
+    const float v0 = v[3 * m], v1 = v[3 * m + 1], v2 = v[3 * m + 2];
+    float B[9];
+    load_block<HALF>(S, g.n, n, o, m, xr == xx && yr == yy && zr == zz, B);
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      out[i] += sp[(3 * i) * g.n] * v0 + sp[(3 * i + 1) * g.n] * v1 + sp[(3 * i + 2) * g.n] * v2;
+    for (int i = 0; i < 3; ++i) out[i] += B[3 * i] * v0 + B[3 * i + 1] * v1 + B[3 * i + 2] * v2;
   }
 #pragma unroll
   for (int w = 1; w < kGroup; w <<= 1)
@@ -816,7 +870,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     for (int j = 1; j < k; ++j) {
       MG_GROUP_LOOP(V.g.n, {
         float ad[3];
-        stencil_row_group(V.S, V.g, node, din, q, ad);
+        stencil_row_group<false, false>(V.S, V.g, node, din, q, ad);
         if (q == 0) {
           const float rr[3] = {res[3 * node] - ad[0], res[3 * node + 1] - ad[1], res[3 * node + 2] - ad[2]};
           float dr[3];
@@ -834,7 +888,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     }
     MG_GROUP_LOOP(V.g.n, {
       float ad[3];
-      stencil_row_group(V.S, V.g, node, din, q, ad);
+      stencil_row_group<false, false>(V.S, V.g, node, din, q, ad);
       if (q == 0)
         for (int i = 0; i < 3; ++i) dout[3 * node + i] = res[3 * node + i] - ad[i];
     });
@@ -863,7 +917,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     float* eout = oth[l];
     MG_GROUP_LOOP(V.g.n, {
       float ae[3];
-      stencil_row_group(V.S, V.g, node, ein, q, ae);
+      stencil_row_group<false, false>(V.S, V.g, node, ein, q, ae);
       if (q == 0) {
         const float rr[3] = {V.r[3 * node] - ae[0], V.r[3 * node + 1] - ae[1], V.r[3 * node + 2] - ae[2]};
         float dr[3];
@@ -882,7 +936,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     for (int j = 1; j < k; ++j) {
       MG_GROUP_LOOP(V.g.n, {
         float ad[3];
-        stencil_row_group(V.S, V.g, node, din, q, ad);
+        stencil_row_group<false, false>(V.S, V.g, node, din, q, ad);
         if (q == 0) {
           const float rr[3] = {V.res[3 * node] - ad[0], V.res[3 * node + 1] - ad[1], V.res[3 * node + 2] - ad[2]};
           float dr[3];
@@ -898,6 +952,92 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
       float* t2 = din; din = dout; dout = t2;
     }
   }
+}
+
+// Levels of up to kLaneNodes nodes above the small cycle run the stencil passes
+// with kGroup lanes per node as well: a thread then issues 3-4 block loads
+// instead of 27 dependent ones, and a level of 33^3 nodes launches 8x the
+// workgroups (one lane per node leaves half the SIMDs of the chip idle there).
+// After the shuffle reduction lane i < 3 finishes component i of the node.
+constexpr int64_t kLaneNodes = 65536;
+
+#define MG_LANE_LOOP(N, ...)                                                                   \
+  do {                                                                                         \
+    const int q = threadIdx.x % kGroup;                                                        \
+    constexpr int per_ = kMgBlock / kGroup;                                                    \
+    for (int64_t node = (int64_t)logical_block() * per_ + threadIdx.x / kGroup; node < (N);   \
+         node += (int64_t)gridDim.x * per_)                                                    \
+      __VA_ARGS__;                                                                             \
+  } while (0)
+
+inline int lane_grid(int64_t n) {
+  int64_t g = ceil_div(n * kGroup, kMgBlock);
+  if (g > kMaxGrid) g = kMaxGrid;
+  if (g >= 64) g = (g / kNumXcd) * kNumXcd;
+  return (int)(g < 1 ? 1 : g);
+}
+
+// row q of the node's 3x3 block of D times v
+__device__ __forceinline__ float block_row(const float* __restrict__ D, int64_t n, int64_t node, int q,
+                                           const float (&v)[3]) {
+  return D[(3 * q) * n + node] * v[0] + D[(3 * q + 1) * n + node] * v[1] + D[(3 * q + 2) * n + node] * v[2];
+}
+
+// as mg_poly_step (res_in may alias res_out: a group reads all three entries of
+// its node before any of its lanes stores one)
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_step_lanes(const float* __restrict__ S,
+                                                               const float* __restrict__ Dinv, Lat g, float c1,
+                                                               float c2, const float* res_in,
+                                                               const float* __restrict__ d_in, float* res_out,
+                                                               float* __restrict__ d_out, float* __restrict__ e,
+                                                               const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_LANE_LOOP(g.n, {
+    float ad[3];
+    stencil_row_group<HALF, true>(S, g, node, d_in, q, ad);
+    const float rr[3] = {res_in[3 * node] - ad[0], res_in[3 * node + 1] - ad[1], res_in[3 * node + 2] - ad[2]};
+    if (q < 3) {
+      const float dn = c1 * d_in[3 * node + q] + c2 * block_row(Dinv, g.n, node, q, rr);
+      res_out[3 * node + q] = rr[q];
+      d_out[3 * node + q] = dn;
+      e[3 * node + q] += dn;
+    }
+  });
+}
+
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_resid_lanes(const float* __restrict__ S, Lat g,
+                                                                const float* __restrict__ res_in,
+                                                                const float* __restrict__ d_in,
+                                                                float* __restrict__ t, const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_LANE_LOOP(g.n, {
+    float ad[3];
+    stencil_row_group<HALF, true>(S, g, node, d_in, q, ad);
+    if (q < 3) t[3 * node + q] = res_in[3 * node + q] - ad[q];
+  });
+}
+
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_post_first_lanes(const float* __restrict__ S,
+                                                                const float* __restrict__ Dinv, Lat g, float s0,
+                                                                const float* __restrict__ r,
+                                                                const float* __restrict__ e_in,
+                                                                float* __restrict__ res, float* __restrict__ d,
+                                                                float* __restrict__ e_out, const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_LANE_LOOP(g.n, {
+    float ae[3];
+    stencil_row_group<HALF, true>(S, g, node, e_in, q, ae);
+    const float rr[3] = {r[3 * node] - ae[0], r[3 * node + 1] - ae[1], r[3 * node + 2] - ae[2]};
+    if (q < 3) {
+      const float dq = s0 * block_row(Dinv, g.n, node, q, rr);
+      res[3 * node + q] = rr[q];
+      d[3 * node + q] = dq;
+      e_out[3 * node + q] = e_in[3 * node + q] + dq;
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1274,8 +1414,7 @@ using namespace hf;
 
 struct MgLevel {
   Lat g;
-  float* S = nullptr;     // 243 n (FP32 levels)
-  uint16_t* Sh = nullptr; // 243 n BF16 (the fine level with HF_MG_FINE_BF16)
+  float* S = nullptr;     // 243 n (the fine level of a symmetric operator: 126 n, half-stored)
   float* Dinv = nullptr;  // 9 n
   float* r = nullptr;     // 3 n (levels >= 1: the restricted residual)
   float* e = nullptr;     // 3 n
@@ -1297,7 +1436,7 @@ struct hf_mg {
   int nd = 0;
   int32_t* flags = nullptr;  // [0] pattern not on the lattice, [1] dropped coarse pivots
   float* part = nullptr;     // kMaxGrid per-workgroup maxima + kMgMaxLevels bounds
-  bool bf16 = false;         // fine-level stencil in BF16
+  bool half = false;         // fine-level stencil half-stored (symmetric operator)
   bool sell = false;         // fine level is the tangent's SELL matrix (no fine stencil)
   float* y = nullptr;        // 3 n0: SELL products of the fine level
   const hf_pattern* sp = nullptr;  // the fine level's pattern and values (last set-up, caller's memory)
@@ -1317,7 +1456,6 @@ static int mg_free(hf_mg* mg) {
     float* bufs[] = {L.S, L.Dinv, L.r, L.e, L.e2, L.da, L.db, L.res};
     for (float* b : bufs)
       if (b) (void)hipFree(b);
-    if (L.Sh) (void)hipFree(L.Sh);
   }
   if (mg->B) (void)hipFree(mg->B);
   if (mg->y) (void)hipFree(mg->y);
@@ -1329,8 +1467,7 @@ static int mg_free(hf_mg* mg) {
 
 extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out) {
   HF_REQUIRE(out, "hf_mg_create: null output");
-  HF_REQUIRE((flags & ~(HF_MG_FINE_BF16 | HF_MG_FINE_SELL)) == 0, "hf_mg_create: unknown flags 0x%x", flags);
-  HF_REQUIRE(!((flags & HF_MG_FINE_BF16) && (flags & HF_MG_FINE_SELL)), "hf_mg_create: BF16 needs the stencil fine level");
+  HF_REQUIRE((flags & ~(HF_MG_FINE_FULL | HF_MG_FINE_SELL)) == 0, "hf_mg_create: unknown flags 0x%x", flags);
   HF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "hf_mg_create: lattice %dx%dx%d needs >= 2 nodes per dimension", nx,
              ny, nz);
   HF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 30, "hf_mg_create: lattice too large");
@@ -1351,8 +1488,9 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
   }
   hf_mg* mg = new hf_mg();
   mg->nlev = nlev;
-  mg->bf16 = (flags & HF_MG_FINE_BF16) && nlev > 1;   // a one-level hierarchy is the dense FP64 solve
   mg->sell = (flags & HF_MG_FINE_SELL) != 0;
+  // a one-level hierarchy is the dense FP64 solve, which reads the full stencil
+  mg->half = !(flags & HF_MG_FINE_FULL) && !mg->sell && nlev > 1;
   if (mg->sell && nlev < 2) {
     delete mg;
     HF_REQUIRE(false, "hf_mg_create: a SELL fine level needs at least one coarser level");
@@ -1366,11 +1504,8 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
   for (int l = 0; l < mg->nlev && !rc; ++l) {
     MgLevel& L = mg->lv[l];
     const int64_t n = L.g.n;
-    if (l == 0 && mg->bf16) {
-      if (hipMalloc((void**)&L.Sh, (size_t)kSt * n * sizeof(uint16_t)) != hipSuccess) {
-        set_error("hf_mg_create: hipMalloc failed");
-        rc = -2;
-      }
+    if (l == 0 && mg->half) {
+      rc |= alloc(&L.S, (int64_t)kHalfSlots * 9 * n);
     } else if (l == 0 && mg->sell) {
       rc |= alloc(&mg->y, 3 * n);
     } else {
@@ -1489,24 +1624,28 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
     mg_check_lattice<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p),
                                                         pattern_sell_pos(p), pattern_slice_ptr(p), pattern_R(p), g0,
                                                         mg->flags);
-  if (mg->bf16)
-    mg_fine_stencil<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p), pattern_slice_ptr(p),
-                                                       pattern_R(p), vals, mask, g0, mg->lv[0].Sh);
+  if (mg->half)
+    mg_fine_stencil<true><<<dim3(gb0, kHalfSlots), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p),
+                                                                     pattern_slice_ptr(p), pattern_R(p), vals, mask,
+                                                                     g0, mg->lv[0].S);
   else if (!mg->sell)
-    mg_fine_stencil<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p), pattern_slice_ptr(p),
-                                                       pattern_R(p), vals, mask, g0, mg->lv[0].S);
+    mg_fine_stencil<false><<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p),
+                                                              pattern_slice_ptr(p), pattern_R(p), vals, mask, g0,
+                                                              mg->lv[0].S);
   float* lam_dev = lam_dev0;
   for (int l = mg->sell ? 1 : 0; l + 1 < mg->nlev; ++l) {
     const Lat f = mg->lv[l].g, c = mg->lv[l + 1].g;
     const int gg = (int)std::min<int64_t>(ceil_div(f.n, kMgBlock), kMaxGrid);
-    auto level = [&](auto Sp) {
-      mg_block_dinv<<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(Sp, l == 0 ? mask : nullptr, f.n,
-                                                                           mg->lv[l].Dinv);
-      mg_gersh<<<gg, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f.n, mg->part);
-      mg_rap_ap<<<dim3((unsigned)ceil_div(f.n, kMgBlock), 27), kMgBlock, 0, s>>>(Sp, f, c, mg->B);
+    auto level = [&](auto half_tag) {
+      constexpr bool H = decltype(half_tag)::value;
+      const float* Sp = mg->lv[l].S;
+      mg_block_dinv<H><<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(Sp, l == 0 ? mask : nullptr, f.n,
+                                                                              mg->lv[l].Dinv);
+      mg_gersh<H><<<gg, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, mg->part);
+      mg_rap_ap<H><<<dim3((unsigned)ceil_div(f.n, kMgBlock), 27), kMgBlock, 0, s>>>(Sp, f, c, mg->B);
     };
-    if (l == 0 && mg->bf16) level((const uint16_t*)mg->lv[0].Sh);
-    else level((const float*)mg->lv[l].S);
+    if (l == 0 && mg->half) level(std::true_type{});
+    else level(std::false_type{});
     mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gg, lam_dev + l);
     mg_rap_ptap<<<dim3((unsigned)ceil_div(c.n, kMgBlock), 27), kMgBlock, 0, s>>>(mg->B, f, c, mg->lv[l + 1].S);
   }
@@ -1574,13 +1713,24 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const float* res = rin(l);
     float* din = V.da;
     float* dout = V.db;
-    auto down = [&](auto Sp) {
+    const bool lanes = V.g.n <= kLaneNodes;
+    const int gl = lane_grid(V.g.n);
+    auto down = [&](auto half_tag) {
+      constexpr bool H = decltype(half_tag)::value;
       for (int j = 1; j < k; ++j) {
-        mg_poly_step<<<gg, kMgBlock, 0, s>>>(Sp, V.Dinv, V.g, V.c1[j], V.c2[j], res, din, V.res, dout, cur[l], sc);
+        if (lanes)
+          mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], res, din, V.res, dout,
+                                                        cur[l], sc);
+        else
+          mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], res, din, V.res, dout,
+                                                  cur[l], sc);
         res = V.res;
         float* tmp = din; din = dout; dout = tmp;
       }
-      mg_poly_resid<<<gg, kMgBlock, 0, s>>>(Sp, V.g, res, din, dout, sc);
+      if (lanes)
+        mg_poly_resid_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.g, res, din, dout, sc);
+      else
+        mg_poly_resid<H><<<gg, kMgBlock, 0, s>>>(V.S, V.g, res, din, dout, sc);
     };
     if (l == 0 && mg->sell) {
       // SELL fine level: each pass is y = A v, then the node-wise update
@@ -1595,10 +1745,10 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       }
       mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
       mg_sell_resid_update<<<gg, kMgBlock, 0, s>>>(V.g.n, res, mg->y, dout, sc);
-    } else if (l == 0 && mg->bf16) {
-      down((const uint16_t*)V.Sh);
+    } else if (l == 0 && mg->half) {
+      down(std::true_type{});
     } else {
-      down((const float*)V.S);
+      down(std::false_type{});
     }
     const MgLevel& C = mg->lv[l + 1];
     const bool smooth_c = (l + 2 < L);
@@ -1632,21 +1782,30 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     if (l == 0 && skip_fine_post) break;
     // post-smoothing: the same polynomial on the residual after the correction
     int perr = 0;
-    auto up = [&](auto Sp) {
-      using ST = std::remove_cv_t<std::remove_pointer_t<decltype(Sp)>>;
+    const bool lanes = V.g.n <= kLaneNodes;
+    const int gl = lane_grid(V.g.n);
+    auto up = [&](auto half_tag) {
+      constexpr bool H = decltype(half_tag)::value;
+      auto post = lanes ? mg_post_first_lanes<H> : mg_post_first<H>;
+      const int gp = lanes ? gl : gg;
       if (timed && l == 0) {
         hipEvent_t e0, e1;
         if (prof_events(&e0, &e1)) { perr = -2; return; }
-        hipExtLaunchKernelGGL(mg_post_first<ST>, dim3(gg), dim3(kMgBlock), 0, s, e0, e1, 0, Sp, (const float*)V.Dinv,
-                              V.g, V.s0, rin(l), (const float*)cur[l], V.res, V.da, oth[l], sc);
+        hipExtLaunchKernelGGL(post, dim3(gp), dim3(kMgBlock), 0, s, e0, e1, 0, (const float*)V.S,
+                              (const float*)V.Dinv, V.g, V.s0, rin(l), (const float*)cur[l], V.res, V.da, oth[l], sc);
       } else {
-        mg_post_first<<<gg, kMgBlock, 0, s>>>(Sp, V.Dinv, V.g, V.s0, rin(l), cur[l], V.res, V.da, oth[l], sc);
+        post<<<gp, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.s0, rin(l), cur[l], V.res, V.da, oth[l], sc);
       }
       float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
       float* din = V.da;
       float* dout = V.db;
       for (int j = 1; j < k; ++j) {
-        mg_poly_step<<<gg, kMgBlock, 0, s>>>(Sp, V.Dinv, V.g, V.c1[j], V.c2[j], V.res, din, V.res, dout, cur[l], sc);
+        if (lanes)
+          mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], V.res, din, V.res, dout,
+                                                        cur[l], sc);
+        else
+          mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], V.res, din, V.res, dout,
+                                                  cur[l], sc);
         float* t2 = din; din = dout; dout = t2;
       }
     };
@@ -1672,10 +1831,10 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
                                                     cur[l], sc);
         float* t2 = din; din = dout; dout = t2;
       }
-    } else if (l == 0 && mg->bf16) {
-      up((const uint16_t*)V.Sh);
+    } else if (l == 0 && mg->half) {
+      up(std::true_type{});
     } else {
-      up((const float*)V.S);
+      up(std::false_type{});
     }
     if (perr) return perr;
   }
@@ -1702,8 +1861,10 @@ extern "C" int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf
   HF_REQUIRE(mg->ready, "hf_mg_copy_stencil: hf_mg_setup_f32 has not run");
   HF_REQUIRE(!(level == 0 && mg->sell), "hf_mg_copy_stencil: the fine level is the tangent itself (SELL)");
   const int64_t nv = (int64_t)kSt * mg->lv[level].g.n;
-  if (level == 0 && mg->bf16) {
-    mg_widen<<<(unsigned)ceil_div(nv, kMgBlock), kMgBlock, 0, (hipStream_t)stream>>>(mg->lv[0].Sh, nv, dst);
+  if (level == 0 && mg->half) {
+    const Lat g0 = mg->lv[0].g;
+    mg_expand_half<<<dim3((unsigned)ceil_div(g0.n, kMgBlock), 27), kMgBlock, 0, (hipStream_t)stream>>>(mg->lv[0].S, g0,
+                                                                                                     dst);
     HF_LAUNCH_CHECK();
   } else {
     HF_CHECK_HIP(hipMemcpyAsync(dst, mg->lv[level].S, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice,

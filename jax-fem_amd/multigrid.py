@@ -11,10 +11,9 @@ re-runs the set-up when the tangent changes and implements the fallback.
     level l+1    P^T A_l P, P trilinear interpolation from the lattice with
                  every dimension's interval count halved (rounded up)
     coarsest     <= 88 DOF, dense FP64 inverse
-    storage      FP32 (fine_bf16: the fine-level stencil in BF16, which halves the
-                 cycle's largest stream but rounds away the cancellation in the stencil's
-                 row sums that smooth error depends on: at 64^3 the Krylov count
-                 triples, so it is off by default)
+    storage      FP32; the fine-level stencil of a symmetric operator half-stored
+                 (diagonal + 13 upper offsets, the lower blocks read as the transposes
+                 of the neighbours'; fine_full=True keeps all 27 slots)
     cycle        V-cycle, `degree` smoothing steps per side: Chebyshev in Dinv A on
                  [lam/ratio, lam], lam the Gershgorin bound of Dinv A (Dinv: inverse
                  3x3 diagonal blocks), or damped block Jacobi (omega)
@@ -54,16 +53,17 @@ def dirichlet_mask(problem):
 
 
 class Multigrid:
-    def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None, fine_bf16=False,
+    def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None, fine_full=False,
                  fine_sell=False):
-        """fine_sell: the fine level is the tangent's SELL matrix (HEX27 lattices)."""
+        """fine_sell: the fine level is the tangent's SELL matrix (HEX27 lattices);
+        fine_full: store all 27 fine stencil slots (a non-symmetric operator)."""
         _lib.load()
         self.grid = tuple(int(v) for v in grid)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
-        self.fine_bf16 = bool(fine_bf16) and not fine_sell
+        self.fine_full = bool(fine_full)
         self.fine_sell = bool(fine_sell)
         h = ctypes.c_void_p()
-        flags = (_lib.HF_MG_FINE_BF16 if self.fine_bf16 else 0) | (_lib.HF_MG_FINE_SELL if self.fine_sell else 0)
+        flags = (_lib.HF_MG_FINE_FULL if self.fine_full else 0) | (_lib.HF_MG_FINE_SELL if self.fine_sell else 0)
         _lib.call("hf_mg_create", self.grid[0], self.grid[1], self.grid[2], flags, ctypes.byref(h))
         self.handle = h
         self.smoother_cfg = None
@@ -87,14 +87,13 @@ class Multigrid:
             self.handle = None
 
     @classmethod
-    def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None, fine_bf16=False):
+    def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None, fine_full=False):
         """The problem's cached hierarchy (one per lattice) with the given smoother."""
         grid = lattice(problem)
         sell = problem.fe.ele_type == "HEX27"
-        fine_bf16 = bool(fine_bf16) and not sell
         mg = getattr(problem, "_mg", None)
-        if mg is None or mg.grid != grid or mg.fine_bf16 != fine_bf16 or mg.fine_sell != sell:
-            mg = cls(grid, smoother, degree, param, device=problem.device, fine_bf16=fine_bf16, fine_sell=sell)
+        if mg is None or mg.grid != grid or mg.fine_full != bool(fine_full) or mg.fine_sell != sell:
+            mg = cls(grid, smoother, degree, param, device=problem.device, fine_full=fine_full, fine_sell=sell)
             problem._mg = mg
         else:
             mg.set_smoother(smoother, degree, param)
@@ -224,12 +223,12 @@ def _workspace(device, nbytes):
 
 
 def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=1, smoother="chebyshev", degree=2,
-                        param=None, rebuild="solve", fine_bf16=False, stats=None):
+                        param=None, rebuild="solve", fine_full=False, stats=None):
     """Multigrid FCG; on breakdown or non-finite values Jacobi-PCG solves to the same
     absolute target from the better of the FCG iterate and x0.  `stats` (the Newton
     solve's KrylovStats) identifies the solve for rebuild='solve'.  Returns (x, info, fallback)."""
     from .solver import krylov
-    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_bf16)
+    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_full)
     x_start = (torch.zeros(A.shape[0], dtype=torch.float32, device=b.device) if x0 is None else
                x0.to(torch.float32).clone())
     x, info = mg.solve(A, b, x_start, rtol, atol, maxiter, rel_r0, poll, stats, problem=problem, rebuild=rebuild,

@@ -18,7 +18,7 @@ a legacy flat dict, solver.py:1014-1068).  Linear keys:
   * ``learned`` {model: LearnedPC, tol, atol, maxiter, stall}: flexible CG with
     the learned CNN preconditioner on the FP32 tangent (precision 'mixed'),
     Jacobi-PCG as its fallback; the Newton residual stays the FP64 judge.
-  * ``mg`` {tol, atol, maxiter, smoother, degree, param, rebuild, fine_bf16, poll}: flexible CG
+  * ``mg`` {tol, atol, maxiter, smoother, degree, param, rebuild, fine_full, poll}: flexible CG
     preconditioned by geometric-multigrid V-cycles on the FP32 tangent (precision
     'mixed'; box_mesh HEX8 lattices, multigrid.py), Jacobi-PCG as its fallback;
     rebuild 'solve' (default: the hierarchy of a Newton solve's first tangent
@@ -283,7 +283,7 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
                                           rel_r0=rtol is not None, poll=o.get("poll", 1),
                                           smoother=o.get("smoother", "chebyshev"), degree=o.get("degree", 2),
                                           param=o.get("param"), rebuild=o.get("rebuild", "solve"),
-                                          fine_bf16=o.get("fine_bf16", False), stats=stats)
+                                          fine_full=o.get("fine_full", False), stats=stats)
         if stats is not None:
             stats.calls[-1]["fallback"] = fb
         return x

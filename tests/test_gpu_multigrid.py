@@ -8,8 +8,9 @@
   iteration count; a Newton solve with it vs the FP64 plain path;
 * set-up errors (a pattern that is not the lattice stencil).
 
-Tolerances: level 0 is an exact copy (FP32) or within 2^-8 relative per entry
-(BF16 storage); coarse levels, the Galerkin products of the stored level 0, 2e-5
+Tolerances: level 0 is an exact copy (27 slots stored) or, half-stored, within the
+tangent's FP32 asymmetry (2e-6 of its largest entry); coarse levels, the Galerkin
+products of the stored level 0, 2e-5
 of the operator's largest entry (FP32 accumulation); the V-cycle 1e-4 of its
 largest entry; solves by their FP64 residual; Newton rel-L2 1e-6."""
 import sys
@@ -106,13 +107,13 @@ def galerkin_levels(A0, dims_list):
     return As
 
 
-@pytest.mark.parametrize("cells,bf16", [((4, 3, 5), False), ((8, 6, 7), False), ((8, 6, 7), True)])
-def test_galerkin_levels_vs_scipy(jfa, cells, bf16):
-    say("mg galerkin levels", cells, "bf16" if bf16 else "fp32")
+@pytest.mark.parametrize("cells,full", [((4, 3, 5), True), ((8, 6, 7), True), ((4, 3, 5), False), ((8, 6, 7), False)])
+def test_galerkin_levels_vs_scipy(jfa, cells, full):
+    say("mg galerkin levels", cells, "full" if full else "half")
     MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
     prob = box_problem(jfa, cells)
     A = tangent(jfa, prob)
-    mg = MG.Multigrid(MG.lattice(prob), fine_bf16=bf16)
+    mg = MG.Multigrid(MG.lattice(prob), fine_full=full)
     mask = MG.dirichlet_mask(prob)
     mg.setup(A, mask)
     dims = mg.levels()
@@ -120,12 +121,11 @@ def test_galerkin_levels_vs_scipy(jfa, cells, bf16):
     m = sp.diags(mask.double().cpu().numpy())
     ref0 = (m @ A.to_scipy() @ m).toarray()
     G0 = stencil_to_csr(mg.stencil(0).cpu().numpy(), dims[0])
-    if bf16:   # rounded to nearest BF16 (7 fraction bits): relative error <= 2^-8 per entry
-        err = np.abs(G0.toarray() - ref0)
-        np.testing.assert_array_less(err, 2.0 ** -8 * np.abs(ref0) * (1 + 1e-6) + 1e-30)
-        assert err.max() > 0   # the stored level really is rounded
-    else:
+    if full:
         np.testing.assert_array_equal(G0.toarray(), ref0)
+    else:   # upper blocks copied, lower blocks the neighbours' transposed: the tangent's FP32 asymmetry
+        np.testing.assert_array_equal(np.triu(G0.toarray(), 1) != 0, np.triu(ref0, 1) != 0)
+        np.testing.assert_allclose(G0.toarray(), ref0, rtol=0, atol=2e-6 * np.abs(ref0).max())
     As = galerkin_levels(G0, dims)   # the coarse levels are Galerkin products of the stored fine level
     for l in range(1, len(dims)):
         G = stencil_to_csr(mg.stencil(l).cpu().numpy(), dims[l])
@@ -138,7 +138,7 @@ def test_coarse_operators_annihilate_translations(jfa):
     MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
     prob = box_problem(jfa, (6, 5, 7), material="lin", dirichlet=False)
     A = tangent(jfa, prob, scale=0.0)
-    mg = MG.Multigrid(MG.lattice(prob), fine_bf16=False)
+    mg = MG.Multigrid(MG.lattice(prob))
     mg.setup(A, torch.ones(prob.N, device=DEV))
     for l, d in enumerate(mg.levels()):
         G = stencil_to_csr(mg.stencil(l).cpu().numpy(), d)
@@ -206,14 +206,14 @@ def vcycle_ref(As, dims, mask, r, kind, degree, param, lams):
     return V(0, r), Dinv
 
 
-@pytest.mark.parametrize("kind,degree,param,bf16", [("chebyshev", 2, 8.0, True), ("chebyshev", 2, 8.0, False),
-                                                    ("chebyshev", 3, 4.0, True), ("jacobi", 2, 0.5, False)])
-def test_vcycle_vs_scipy(jfa, kind, degree, param, bf16):
-    say("mg vcycle", kind, degree, param, "bf16" if bf16 else "fp32")
+@pytest.mark.parametrize("kind,degree,param,full", [("chebyshev", 2, 8.0, False), ("chebyshev", 2, 8.0, True),
+                                                    ("chebyshev", 3, 4.0, False), ("jacobi", 2, 0.5, True)])
+def test_vcycle_vs_scipy(jfa, kind, degree, param, full):
+    say("mg vcycle", kind, degree, param, "full" if full else "half")
     MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
     prob = box_problem(jfa, (6, 5, 7))
     A = tangent(jfa, prob)
-    mg = MG.Multigrid(MG.lattice(prob), kind, degree, param, fine_bf16=bf16)
+    mg = MG.Multigrid(MG.lattice(prob), kind, degree, param, fine_full=full)
     mask = MG.dirichlet_mask(prob)
     mg.setup(A, mask)
     dims = mg.levels()
