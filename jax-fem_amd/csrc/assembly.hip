@@ -303,27 +303,53 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
 }
 
 // R[n*vec+i] = sum over the cells holding node n (diag-block contributions,
-// cell order) of elem_res[(cell*nn + a)*vec + i]  (+ f_ext)
+// cell order) of elem_res[(cell*nn + a)*vec + i]  (+ f_ext).  8 lanes per node:
+// lane l fetches contribution l of each chunk of 8, so a node's loads are in
+// flight together; the group then adds them in cell order (shuffles from lanes
+// 0..7), the same sequence of FP64 additions as one thread walking the list.
+constexpr int kGatherLanes = 8;
 template <int VEC>
-__global__ void gather_kernel(const int32_t* __restrict__ diag, const int32_t* __restrict__ seg,
-                              const int32_t* __restrict__ contrib, int64_t nnode, int nn,
-                              const double* __restrict__ elem, const double* __restrict__ fext,
-                              double* __restrict__ R) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nnode) return;
+__global__ void __launch_bounds__(256) gather_kernel(const int32_t* __restrict__ diag, const int32_t* __restrict__ seg,
+                                                     const int32_t* __restrict__ contrib, int64_t nnode, int nn,
+                                                     const double* __restrict__ elem, const double* __restrict__ fext,
+                                                     double* __restrict__ R) {
+  const int64_t n = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kGatherLanes;
+  const int cl = threadIdx.x % kGatherLanes;
+  if (n >= nnode) return;  // whole groups leave together
+  int32_t s0 = 0, ncl = 0;
+  const int32_t b = diag[n];
+  if (b >= 0) {
+    s0 = seg[b];
+    ncl = seg[b + 1] - s0;
+  }
   double acc[VEC];
 #pragma unroll
   for (int i = 0; i < VEC; ++i) acc[i] = fext ? fext[n * VEC + i] : 0.0;
-  const int32_t b = diag[n];
-  if (b >= 0) {
-    for (int32_t s = seg[b]; s < seg[b + 1]; ++s) {
-      const int64_t ca = contrib[s] / nn;  // = cell*nn + a  (contribution (cell,a,a))
+  for (int cbase = 0; cbase < ncl; cbase += kGatherLanes) {
+    double v[VEC];
+    if (cbase + cl < ncl) {
+      const int64_t ca = contrib[s0 + cbase + cl] / nn;  // = cell*nn + a  (contribution (cell,a,a))
 #pragma unroll
-      for (int i = 0; i < VEC; ++i) acc[i] += elem[ca * VEC + i];
+      for (int i = 0; i < VEC; ++i) v[i] = elem[ca * VEC + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) v[i] = 0.0;
+    }
+#pragma unroll
+    for (int l = 0; l < kGatherLanes; ++l) {
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) {
+        const double w = __shfl(v[i], l, kGatherLanes);
+        if (cbase + l < ncl) acc[i] += w;
+      }
     }
   }
+  if (cl < VEC) {
+    double out = acc[0];
 #pragma unroll
-  for (int i = 0; i < VEC; ++i) R[n * VEC + i] = acc[i];
+    for (int i = 1; i < VEC; ++i) out = (cl == i) ? acc[i] : out;
+    R[n * VEC + cl] = out;
+  }
 }
 
 template <typename T>
@@ -622,11 +648,11 @@ extern "C" int hf_gather_residual_f64(const hf_pattern* p, const double* elem, c
                                       hf_stream_t s) {
   HF_REQUIRE(p && elem && R, "hf_gather_residual_f64: null argument");
   if (p->vec == 3)
-    gather_kernel<3><<<grid_for(p->nnode), 256, 0, (hipStream_t)s>>>(p->diag, p->seg, p->contrib, p->nnode, p->nn,
-                                                                       elem, fext, R);
+    gather_kernel<3><<<grid_for(p->nnode * kGatherLanes), 256, 0, (hipStream_t)s>>>(
+        p->diag, p->seg, p->contrib, p->nnode, p->nn, elem, fext, R);
   else if (p->vec == 1)
-    gather_kernel<1><<<grid_for(p->nnode), 256, 0, (hipStream_t)s>>>(p->diag, p->seg, p->contrib, p->nnode, p->nn,
-                                                                       elem, fext, R);
+    gather_kernel<1><<<grid_for(p->nnode * kGatherLanes), 256, 0, (hipStream_t)s>>>(
+        p->diag, p->seg, p->contrib, p->nnode, p->nn, elem, fext, R);
   else
     HF_REQUIRE(false, "vec=%d not supported", p->vec);
   HF_LAUNCH_CHECK();

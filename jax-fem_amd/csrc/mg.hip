@@ -136,6 +136,12 @@ inline int node_grid(int64_t n) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// The 27 slots in mirrored-pair order: 13 (the diagonal), then p, 26 - p for
+// p = 0..12 (t = 2p + 1, 2p + 2); see stencil_row.
+__device__ __forceinline__ int pair_slot(int t) {
+  return t == 0 ? 13 : ((t & 1) ? (t - 1) / 2 : 26 - (t - 2) / 2);
+}
+
 // out = sum over the 27 slots of S[n] v[neighbour]  (clamped neighbours).
 // Offsets go in mirrored pairs (p, 26 - p).  With half storage both blocks of a
 // pair come from stored slot 13 - p (node n's own, and its neighbour
@@ -327,52 +333,90 @@ __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restric
   }
 }
 
-// Stage 1 of the Galerkin product: B = A P over fine rows i, with slot s of
-// row i holding the block of coarse column J = floor(i/2) + s - 1 (per
-// dimension).  One thread per (row, slot); the slot is blockIdx.y.
+// Galerkin product P^T A P, one dimension at a time: P = Px (x) Py (x) Pz, so it is
+// (Pz^T (Py^T (Px^T A Px) Py) Pz), and each factor keeps the operator 27-point
+// (linear interpolation of a 3-point coupling along one dimension stays 3-point).
+// One stage coarsens dimension `dim` of lattice f into lattice c (the other two
+// dimensions unchanged).  A thread per (output node, offsets q of the two other
+// dimensions) forms the node's three output slots along dim from the fine blocks
+// (row cpos(K) + a, offset d along dim), at most 3 x 3 of them; the 27 slots of
+// a node are 9 threads in 9 grid planes (blockIdx.y = q), each reading its own
+// blocks.  The intermediate lattices hold all 27 slots.
 template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_rap_ap(const float* __restrict__ S, Lat f, Lat c,
-                                                      float* __restrict__ B) {
-  const int64_t i = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
-  const int slot = blockIdx.y;
-  if (i >= f.n) return;
-  int fi[3];
-  lat_xyz(f, i, fi[0], fi[1], fi[2]);
-  const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
-  const int nc[3] = {c.nx - 1, c.ny - 1, c.nz - 1};
-  const int sd[3] = {slot / 9, (slot / 3) % 3, slot % 3};
-  float W[3][3];  // [dim][offset]: P(fine row + offset - 1, coarse column of this slot)
+__global__ void __launch_bounds__(kMgBlock) mg_rap_dim(const float* __restrict__ S, Lat f, Lat c, int dim,
+                                                       float* __restrict__ Out) {
+  const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (n >= c.n) return;
+  const int q = blockIdx.y;
+  int cc[3];
+  lat_xyz(c, n, cc[0], cc[1], cc[2]);
+  const int fdim[3] = {f.nx, f.ny, f.nz};
+  const int nf = fdim[dim] - 1;
+  const int nc = (dim == 0 ? c.nx : dim == 1 ? c.ny : c.nz) - 1;
+  const int d1 = dim == 0 ? 1 : 0, d2 = dim == 2 ? 1 : 2;  // the two other dimensions, in order
+  int off[3];                                               // slot offsets (0..2) per dimension
+  off[d1] = q / 3;
+  off[d2] = q % 3;
+  const int K = cc[dim];
+  const int pK = cpos(K, nf);
+  float acc[3][9];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int J = fi[d] / 2 + sd[d] - 1;
+  for (int k = 0; k < 3; ++k)
 #pragma unroll
-    for (int o = 0; o < 3; ++o) {
-      const int j = fi[d] + o - 1;
-      W[d][o] = (J >= 0 && J <= nc[d] && j >= 0 && j <= nf[d]) ? p1(j, J, nf[d]) : 0.f;
+    for (int e = 0; e < 9; ++e) acc[k][e] = 0.f;
+#pragma unroll
+  for (int a = -1; a <= 1; ++a) {
+    const int i = pK + a;
+    if (i < 0 || i > nf) continue;
+    const float wi = p1(i, K, nf);
+    if (wi == 0.f) continue;
+    int rc[3] = {cc[0], cc[1], cc[2]};
+    rc[dim] = i;
+    const int64_t row = lat_id(f, rc[0], rc[1], rc[2]);
+#pragma unroll
+    for (int d = -1; d <= 1; ++d) {
+      const int j = i + d;
+      if (j < 0 || j > nf) continue;
+      float w[3];
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int Kp = K + k - 1;
+        w[k] = (Kp >= 0 && Kp <= nc) ? wi * p1(j, Kp, nf) : 0.f;
+        any |= (w[k] != 0.f);
+      }
+      if (!any) continue;
+      off[dim] = d + 1;
+      const int o = off[0] * 9 + off[1] * 3 + off[2];
+      int nb[3] = {rc[0] + off[0] - 1, rc[1] + off[1] - 1, rc[2] + off[2] - 1};
+      const bool valid = nb[0] >= 0 && nb[0] < f.nx && nb[1] >= 0 && nb[1] < f.ny && nb[2] >= 0 && nb[2] < f.nz;
+      const int64_t m = lat_id(f, clampi(nb[0], 0, f.nx - 1), clampi(nb[1], 0, f.ny - 1), clampi(nb[2], 0, f.nz - 1));
+      float B[9];
+      load_block<HALF>(S, f.n, row, o, m, valid, B);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[k][e] += w[k] * B[e];
     }
   }
-  float acc[9];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) acc[e] = 0.f;
+  for (int k = 0; k < 3; ++k) {
+    off[dim] = k;
+    const int O = off[0] * 9 + off[1] * 3 + off[2];
 #pragma unroll
-  for (int ox = 0; ox < 3; ++ox)
-#pragma unroll
-    for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-      for (int oz = 0; oz < 3; ++oz) {
-        const float w = W[0][ox] * W[1][oy] * W[2][oz];
-        if (w == 0.f) continue;
-        float B[9];
-        load_block<HALF>(S, f.n, i, ox * 9 + oy * 3 + oz,
-                         lat_id(f, fi[0] + ox - 1, fi[1] + oy - 1, fi[2] + oz - 1), true, B);
-#pragma unroll
-        for (int e = 0; e < 9; ++e) acc[e] += w * B[e];
-      }
-#pragma unroll
-  for (int e = 0; e < 9; ++e) B[(int64_t)(slot * 9 + e) * f.n + i] = acc[e];
+    for (int e = 0; e < 9; ++e) Out[(int64_t)(O * 9 + e) * c.n + n] = acc[k][e];
+  }
 }
 
-// Stage 2: A_c = P^T B, one thread per (coarse node, output slot O = blockIdx.y).
+// Node count of the two intermediate lattices of the dimension-wise product
+// coarsening f into c (x coarsened, then x and y).
+inline int64_t rap_dim_scratch_nodes(const Lat& f, const Lat& c) {
+  return (int64_t)c.nx * f.ny * f.nz + (int64_t)c.nx * c.ny * f.nz;
+}
+
+// A_c = P^T B from B = A P held as 27 coarse-column slots per fine row (slot s
+// of row i: coarse column floor(i/2) + s - 1 per dimension; the SELL fine level's
+// first product), one thread per (coarse node, output slot O = blockIdx.y).
 __global__ void __launch_bounds__(kMgBlock) mg_rap_ptap(const float* __restrict__ B, Lat f, Lat c,
                                                         float* __restrict__ Sc) {
   const int64_t I = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
@@ -432,7 +476,8 @@ __global__ void __launch_bounds__(kDenseThreads) mg_dense_setup(const float* __r
                                                                 double* __restrict__ Ainv, int32_t* __restrict__ bad) {
   constexpr int kCols = kDenseThreads / kDenseMax;  // threads per row (11): row ti, columns tj, tj + kCols, ...
   __shared__ double A[kDenseMax * kDenseMax];
-  __shared__ double col[kDenseMax];
+  __shared__ double col[2 * kDenseMax];
+  __shared__ double prow[2 * kDenseMax];
   const int nd = (int)(3 * g.n);
   const int tid = threadIdx.x;
   const int ti = tid / kCols, tj = tid - (tid / kCols) * kCols;
@@ -463,24 +508,57 @@ __global__ void __launch_bounds__(kDenseThreads) mg_dense_setup(const float* __r
     for (int j = tj; j < nd; j += kCols, ++m) A[ti * nd + j] = sym[m];
   }
   __syncthreads();
+  // Gauss-Jordan with each owner's row segment in registers.  Step k publishes
+  // column k (fc) and the unscaled pivot row (pr) in LDS, double-buffered by the
+  // step's parity, so one barrier per step suffices: a buffer is rewritten two
+  // steps later, after every thread has passed the barrier between.  Columns
+  // past nd are zero padding up to kSeg * kCols and stay zero, so the step has
+  // no per-column branches (its 8 pivot-row reads issue together).
+  constexpr int kSeg = (kDenseMax + kCols - 1) / kCols;
+  static_assert(kSeg * kCols <= kDenseMax, "padded pivot row exceeds its buffer");
+  double R[kSeg];
+#pragma unroll
+  for (int m = 0; m < kSeg; ++m) {
+    const int j = tj + m * kCols;
+    R[m] = (owner && j < nd) ? A[ti * nd + j] : 0.0;
+  }
   int dropped = 0;
   for (int k = 0; k < nd; ++k) {
-    if (tid < nd) col[tid] = A[tid * nd + k];
-    __syncthreads();
-    const double piv = col[k];
-    const double pinv = (piv > 0.0 && isfinite(piv)) ? 1.0 / piv : 0.0;
-    dropped += (pinv == 0.0);
-    if (owner && ti == k)
-      for (int j = tj; j < nd; j += kCols) A[k * nd + j] = (j == k ? 1.0 : A[k * nd + j]) * pinv;
-    __syncthreads();
-    if (owner && ti != k) {
-      const double f = col[ti];
-      for (int j = tj; j < nd; j += kCols) A[ti * nd + j] = (j == k ? 0.0 : A[ti * nd + j]) - f * A[k * nd + j];
+    double* fc = col + (k & 1) * kDenseMax;
+    double* pr = prow + (k & 1) * kDenseMax;
+    const int kt = k % kCols, km = k / kCols;
+    if (owner && tj == kt) {
+      double v = R[0];
+#pragma unroll
+      for (int m = 1; m < kSeg; ++m) v = (m == km) ? R[m] : v;
+      fc[ti] = v;
+    }
+    if (ti == k) {
+#pragma unroll
+      for (int m = 0; m < kSeg; ++m) pr[tj + m * kCols] = R[m];
     }
     __syncthreads();
+    const double piv = fc[k];
+    const double pinv = (piv > 0.0 && isfinite(piv)) ? 1.0 / piv : 0.0;
+    dropped += (pinv == 0.0);
+    const bool pivot_row = (ti == k);
+    const double f = (owner && !pivot_row) ? fc[ti] * pinv : 0.0;
+#pragma unroll
+    for (int m = 0; m < kSeg; ++m) {
+      const bool diag = (tj == kt) && (m == km);
+      const double p = pr[tj + m * kCols];
+      const double other = diag ? -f : R[m] - f * p;
+      const double scaled = (diag ? 1.0 : R[m]) * pinv;
+      R[m] = pivot_row ? scaled : other;
+    }
   }
-  if (owner)
-    for (int j = tj; j < nd; j += kCols) Ainv[ti * nd + j] = A[ti * nd + j];
+  if (owner) {
+#pragma unroll
+    for (int m = 0; m < kSeg; ++m) {
+      const int j = tj + m * kCols;
+      if (j < nd) Ainv[j * nd + ti] = R[m];  // column-major: a row's reads in the solve coalesce
+    }
+  }
   if (tid == 0) bad[1] = dropped;
 }
 
@@ -725,7 +803,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S
     int x, y, z;
     lat_xyz(g, node, x, y, z);
     float rs[3] = {0.f, 0.f, 0.f};
-    for (int o = 0; o < 27; ++o) {
+    for (int t = 0; t < 27; ++t) {
+      const int o = pair_slot(t);
       const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
       const bool valid = xr >= 0 && xr < g.nx && yr >= 0 && yr < g.ny && zr >= 0 && zr < g.nz;
       const int64_t m = lat_id(g, clampi(xr, 0, g.nx - 1), clampi(yr, 0, g.ny - 1), clampi(zr, 0, g.nz - 1));
@@ -778,7 +857,7 @@ __global__ void __launch_bounds__(kDenseApplyThreads) mg_dense_apply(const doubl
   __syncthreads();
   if (tid < nd) {
     double acc = 0.0;
-    for (int j = 0; j < nd; ++j) acc += Ainv[(int64_t)tid * nd + j] * rs[j];
+    for (int j = 0; j < nd; ++j) acc += Ainv[(int64_t)j * nd + tid] * rs[j];
     e[tid] = (float)acc;
   }
 }
@@ -794,6 +873,7 @@ constexpr int kSmallThreads = 1024;
 constexpr int kGroup = 8;                             // lanes per node
 constexpr int kSmallNodes = kSmallThreads / kGroup;   // 128
 constexpr int kSmallMax = 4;
+constexpr int kSmallUnroll = 1;
 struct SmallLevel {
   Lat g;
   const float* S;
@@ -811,17 +891,17 @@ struct SmallCycle {
 };
 
 // product of node n's stencil row with v, by the 8 lanes of a group (q = lane
-// in group); every lane of the group ends with the whole sum.  UNROLL issues a
-// lane's 3-4 block loads together (the grid kernels); in the 1024-thread small
-// cycle the unrolled form exceeds the 128 VGPRs a lane has there and spills.
-template <bool HALF, bool UNROLL>
+// in group); every lane of the group ends with the whole sum.  UNROLL (1, 2 or
+// 4) is how many of a lane's 3-4 blocks have their loads in flight together:
+// 4 in the grid kernels; in the 1024-thread small cycle (128 VGPRs a lane) all
+// four spill.
+template <bool HALF, int UNROLL>
 __device__ __forceinline__ void stencil_row_group(const float* __restrict__ S, const Lat& g, int64_t n,
                                                   const float* __restrict__ v, int q, float (&out)[3]) {
   int x, y, z;
   lat_xyz(g, n, x, y, z);
   out[0] = out[1] = out[2] = 0.f;
-  constexpr int kUnroll = UNROLL ? 4 : 1;
-#pragma unroll kUnroll
+#pragma unroll UNROLL
   for (int t = 0; t < (27 + kGroup - 1) / kGroup; ++t) {
     const int o = q + t * kGroup;
     if (o >= 27) break;
@@ -870,7 +950,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     for (int j = 1; j < k; ++j) {
       MG_GROUP_LOOP(V.g.n, {
         float ad[3];
-        stencil_row_group<false, false>(V.S, V.g, node, din, q, ad);
+        stencil_row_group<false, kSmallUnroll>(V.S, V.g, node, din, q, ad);
         if (q == 0) {
           const float rr[3] = {res[3 * node] - ad[0], res[3 * node + 1] - ad[1], res[3 * node + 2] - ad[2]};
           float dr[3];
@@ -888,7 +968,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     }
     MG_GROUP_LOOP(V.g.n, {
       float ad[3];
-      stencil_row_group<false, false>(V.S, V.g, node, din, q, ad);
+      stencil_row_group<false, kSmallUnroll>(V.S, V.g, node, din, q, ad);
       if (q == 0)
         for (int i = 0; i < 3; ++i) dout[3 * node + i] = res[3 * node + i] - ad[i];
     });
@@ -904,7 +984,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     __syncthreads();
     if (tid < sc_.nd) {
       double acc = 0.0;
-      for (int j = 0; j < sc_.nd; ++j) acc += sc_.Ainv[(int64_t)tid * sc_.nd + j] * rs[j];
+      for (int j = 0; j < sc_.nd; ++j) acc += sc_.Ainv[(int64_t)j * sc_.nd + tid] * rs[j];
       cur[L - 1][tid] = (float)acc;
     }
     __syncthreads();
@@ -917,7 +997,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     float* eout = oth[l];
     MG_GROUP_LOOP(V.g.n, {
       float ae[3];
-      stencil_row_group<false, false>(V.S, V.g, node, ein, q, ae);
+      stencil_row_group<false, kSmallUnroll>(V.S, V.g, node, ein, q, ae);
       if (q == 0) {
         const float rr[3] = {V.r[3 * node] - ae[0], V.r[3 * node + 1] - ae[1], V.r[3 * node + 2] - ae[2]};
         float dr[3];
@@ -936,7 +1016,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     for (int j = 1; j < k; ++j) {
       MG_GROUP_LOOP(V.g.n, {
         float ad[3];
-        stencil_row_group<false, false>(V.S, V.g, node, din, q, ad);
+        stencil_row_group<false, kSmallUnroll>(V.S, V.g, node, din, q, ad);
         if (q == 0) {
           const float rr[3] = {V.res[3 * node] - ad[0], V.res[3 * node + 1] - ad[1], V.res[3 * node + 2] - ad[2]};
           float dr[3];
@@ -995,7 +1075,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_step_lanes(const float* __re
   if (sc && sc->done) return;
   MG_LANE_LOOP(g.n, {
     float ad[3];
-    stencil_row_group<HALF, true>(S, g, node, d_in, q, ad);
+    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad);
     const float rr[3] = {res_in[3 * node] - ad[0], res_in[3 * node + 1] - ad[1], res_in[3 * node + 2] - ad[2]};
     if (q < 3) {
       const float dn = c1 * d_in[3 * node + q] + c2 * block_row(Dinv, g.n, node, q, rr);
@@ -1014,7 +1094,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_resid_lanes(const float* __r
   if (sc && sc->done) return;
   MG_LANE_LOOP(g.n, {
     float ad[3];
-    stencil_row_group<HALF, true>(S, g, node, d_in, q, ad);
+    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad);
     if (q < 3) t[3 * node + q] = res_in[3 * node + q] - ad[q];
   });
 }
@@ -1029,7 +1109,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_post_first_lanes(const float* __r
   if (sc && sc->done) return;
   MG_LANE_LOOP(g.n, {
     float ae[3];
-    stencil_row_group<HALF, true>(S, g, node, e_in, q, ae);
+    stencil_row_group<HALF, 4>(S, g, node, e_in, q, ae);
     const float rr[3] = {r[3 * node] - ae[0], r[3 * node + 1] - ae[1], r[3 * node + 2] - ae[2]};
     if (q < 3) {
       const float dq = s0 * block_row(Dinv, g.n, node, q, rr);
@@ -1133,8 +1213,9 @@ __global__ void __launch_bounds__(kMgBlock) mg_sell_gersh(const int32_t* __restr
   }
 }
 
-// Stage 1 of the Galerkin product from the SELL rows: B = (M A M) P in the 27
-// coarse-column slots of mg_rap_ap.  A thread per fine row accumulates its slots
+// Stage 1 of the Galerkin product from the SELL rows: B = (M A M) P in 27
+// coarse-column slots per fine row (slot s: coarse column floor(i/2) + s - 1 per
+// dimension; mg_rap_ptap forms P^T B).  A thread per fine row accumulates its slots
 // in LDS (64 rows per workgroup, 243 floats each); a block whose coarse columns
 // fall outside the slots sets bad[0] (the pattern is not supported).
 constexpr int kSellRapRows = 64;
@@ -1431,7 +1512,7 @@ struct MgLevel {
 struct hf_mg {
   int nlev = 0;
   MgLevel lv[kMgMaxLevels];
-  float* B = nullptr;        // A P of the finest pair, reused by every level
+  float* B = nullptr;        // Galerkin-product scratch, reused by every level
   double* Ainv = nullptr;    // coarsest dense inverse
   int nd = 0;
   int32_t* flags = nullptr;  // [0] pattern not on the lattice, [1] dropped coarse pivots
@@ -1516,7 +1597,14 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
     if (l > 0) rc |= alloc(&L.r, 3 * n);
     rc |= alloc(&L.e, 3 * n) | alloc(&L.e2, 3 * n);
   }
-  if (!rc && mg->nlev > 1) rc |= alloc(&mg->B, kSt * mg->lv[0].g.n);
+  if (!rc && mg->nlev > 1) {
+    // B: the SELL fine level's A P (27 coarse-column slots per fine row), and the
+    // intermediate lattices of the dimension-wise Galerkin products
+    int64_t nb = mg->sell ? mg->lv[0].g.n : 0;
+    for (int l = mg->sell ? 1 : 0; l + 1 < mg->nlev; ++l)
+      nb = std::max(nb, rap_dim_scratch_nodes(mg->lv[l].g, mg->lv[l + 1].g));
+    rc |= alloc(&mg->B, kSt * nb);
+  }
   if (!rc) rc |= alloc(&mg->part, kMaxGrid + kMgMaxLevels);
   mg->nd = (int)(3 * mg->lv[mg->nlev - 1].g.n);
   if (!rc) {
@@ -1642,12 +1730,18 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
       mg_block_dinv<H><<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(Sp, l == 0 ? mask : nullptr, f.n,
                                                                               mg->lv[l].Dinv);
       mg_gersh<H><<<gg, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, mg->part);
-      mg_rap_ap<H><<<dim3((unsigned)ceil_div(f.n, kMgBlock), 27), kMgBlock, 0, s>>>(Sp, f, c, mg->B);
+      // P^T A P: x, then y, then z; the two intermediates share the B buffer
+      const Lat bx = make_lat(c.nx, f.ny, f.nz), bxy = make_lat(c.nx, c.ny, f.nz);
+      float* T1 = mg->B;
+      float* T2 = mg->B + (int64_t)kSt * bx.n;
+      mg_rap_dim<H><<<dim3((unsigned)ceil_div(bx.n, kMgBlock), 9), kMgBlock, 0, s>>>(Sp, f, bx, 0, T1);
+      mg_rap_dim<false><<<dim3((unsigned)ceil_div(bxy.n, kMgBlock), 9), kMgBlock, 0, s>>>(T1, bx, bxy, 1, T2);
+      mg_rap_dim<false><<<dim3((unsigned)ceil_div(c.n, kMgBlock), 9), kMgBlock, 0, s>>>(T2, bxy, c, 2,
+                                                                                      mg->lv[l + 1].S);
     };
     if (l == 0 && mg->half) level(std::true_type{});
     else level(std::false_type{});
     mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gg, lam_dev + l);
-    mg_rap_ptap<<<dim3((unsigned)ceil_div(c.n, kMgBlock), 27), kMgBlock, 0, s>>>(mg->B, f, c, mg->lv[l + 1].S);
   }
   const Lat gl = mg->lv[mg->nlev - 1].g;
   mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
