@@ -262,7 +262,13 @@ int hf_mg_apply_f32(hf_mg* mg, const float* r, float* z, int32_t flags, hf_strea
 int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf_stream_t stream);
 /* flexible CG (one kept direction) preconditioned by V-cycles; the stop rule,
  * flags (HF_KRYLOV_REL_R0; no transpose) and info of hf_krylov_solve.
- * status 2: p.Ap <= 0.  workspace: hf_mg_workspace_size. */
+ * status 2: p.Ap <= 0.  workspace: hf_mg_workspace_size.
+ * A poll batch (poll iterations) is captured once as a HIP graph and replayed;
+ * it is captured again when the hierarchy is set up again or the pattern,
+ * values pointer, workspace or poll change.  HF_MG_SOLVE_EAGER enqueues every
+ * launch instead (the same kernels in the same order: identical results).
+ * Calls that share one hf_mg must not overlap. */
+enum { HF_MG_SOLVE_EAGER = 4 };
 size_t hf_mg_workspace_size(const hf_mg* mg);
 int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* b, float* x, int32_t flags,
                     int32_t maxiter, double rtol, double atol, int32_t poll, void* workspace,

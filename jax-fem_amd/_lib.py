@@ -21,6 +21,7 @@ HF_KRYLOV_PCG, HF_KRYLOV_BICGSTAB = 0, 1
 HF_MG_JACOBI, HF_MG_CHEBYSHEV = 0, 1
 HF_MG_FINE_FULL, HF_MG_FINE_SELL = 1, 2
 HF_MG_SKIP_FINE_POST = 1
+HF_MG_SOLVE_EAGER = 4
 
 
 class HFMaterial(ctypes.Structure):

@@ -1529,6 +1529,12 @@ struct hf_mg {
   const hf_pattern* checked = nullptr;  // pattern whose lattice structure was verified
   int64_t checked_nblocks = -1;
   bool ready = false;
+  // hf_mg_solve_f32's poll batch as a captured graph, replayed while its key
+  // (set-up generation and every pointer and size the batch uses) matches
+  uint64_t gen = 0;                 // bumped whenever level data or coefficients change
+  hipStream_t cap = nullptr;        // capture stream (non-blocking, default priority)
+  hipGraphExec_t gexec = nullptr;
+  uintptr_t gkey[6] = {0, 0, 0, 0, 0, 0};
 };
 
 static int mg_free(hf_mg* mg) {
@@ -1540,6 +1546,8 @@ static int mg_free(hf_mg* mg) {
   }
   if (mg->B) (void)hipFree(mg->B);
   if (mg->y) (void)hipFree(mg->y);
+  if (mg->gexec) (void)hipGraphExecDestroy(mg->gexec);
+  if (mg->cap) (void)hipStreamDestroy(mg->cap);
   if (mg->Ainv) (void)hipFree(mg->Ainv);
   if (mg->flags) (void)hipFree(mg->flags);
   if (mg->part) (void)hipFree(mg->part);
@@ -1645,6 +1653,7 @@ extern "C" int hf_mg_levels(const hf_mg* mg, int32_t* nlev, int32_t* dims, int32
 
 // Polynomial coefficients of every level from its bound lam.
 static void mg_coefficients(hf_mg* mg) {
+  ++mg->gen;  // the cycle's kernels take the coefficients by value
   for (int l = 0; l + 1 < mg->nlev; ++l) {
     MgLevel& L = mg->lv[l];
     if (mg->kind == HF_MG_JACOBI) {
@@ -1969,7 +1978,60 @@ extern "C" int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf
 
 extern "C" size_t hf_mg_workspace_size(const hf_mg* mg) {
   if (!mg) return 0;
-  return kMgHeader + 6 * align256((size_t)mg->lv[0].g.n * 3 * sizeof(float));
+  return kMgHeader + 7 * align256((size_t)mg->lv[0].g.n * 3 * sizeof(float));
+}
+
+// Enqueue `poll` FCG iterations (update, convergence scalars, V-cycle, w = A z,
+// direction scalars) on stream s; timed marks one launch for HIP-event timing.
+struct MgBatch {
+  MatArgs m;
+  const float* vals;
+  int64_t N;
+  float *r, *z0, *z1, *w, *pv, *q, *x;
+  float* z;  // the V-cycle's output buffer (the same one every cycle)
+  Scalars* sc;
+  double *partA, *partB;
+  int gr, ge;
+};
+static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int timed_k) {
+  for (int k = 0; k < poll; ++k) {
+    mgcg_update_kernel<<<B.ge, kBlock, 0, s>>>(B.N, B.z, B.w, B.pv, B.q, B.x, B.r, B.sc, B.partA);
+    mgcg_conv_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partA, B.ge);
+    float* z = nullptr;
+    if (vcycle(mg, B.r, B.z0, B.z1, &z, B.sc, s, k == timed_k)) return -2;
+    B.z = z;
+    mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(B.m, B.vals, B.z, B.r, B.q, B.w, B.sc, B.partB);
+    mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partB, B.gr);
+  }
+  return 0;
+}
+
+// Capture a poll batch into mg->gexec unless the cached one matches.  The key
+// holds every value the captured launches depend on besides device memory
+// contents: the set-up generation (level coefficients are kernel arguments),
+// the tangent's values and pattern, the workspace and the batch length.
+static int mg_batch_graph(hf_mg* mg, const hf_pattern* p, MgBatch& B, void* ws, int poll) {
+  const uintptr_t key[6] = {(uintptr_t)mg->gen, (uintptr_t)p, (uintptr_t)B.vals, (uintptr_t)ws, (uintptr_t)poll,
+                            (uintptr_t)B.N};
+  bool same = mg->gexec != nullptr;
+  for (int i = 0; i < 6 && same; ++i) same = mg->gkey[i] == key[i];
+  if (same) return 0;
+  if (!mg->cap) HF_CHECK_HIP(hipStreamCreateWithFlags(&mg->cap, hipStreamNonBlocking));
+  if (mg->gexec) {
+    (void)hipGraphExecDestroy(mg->gexec);
+    mg->gexec = nullptr;
+  }
+  HF_CHECK_HIP(hipStreamBeginCapture(mg->cap, hipStreamCaptureModeThreadLocal));
+  const int rc = mg_enqueue_batch(mg, B, poll, mg->cap, -1);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(mg->cap, &g);
+  HF_REQUIRE(rc == 0 && e == hipSuccess && g, "hf_mg_solve_f32: capturing the iteration failed (%s)",
+             hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&mg->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HF_CHECK_HIP(ei);
+  for (int i = 0; i < 6; ++i) mg->gkey[i] = key[i];
+  return 0;
 }
 
 extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* b, float* x,
@@ -1990,15 +2052,27 @@ extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   double* partB = partA + kPartB;
   float* v = (float*)((char*)ws + kMgHeader);
   const size_t stride = align256(N * sizeof(float)) / sizeof(float);
-  float *r = v, *z0 = v + stride, *z1 = v + 2 * stride, *w = v + 3 * stride, *pv = v + 4 * stride,
-        *q = v + 5 * stride;
-  const int gr = rows_grid(m.nslices), ge = elem_grid(N);
-  mgcg_init_kernel<<<gr, kBlock, 0, s>>>(m, vals, b, x, r, pv, q, partA);
-  mgcg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
+  MgBatch B;
+  B.m = m;
+  B.vals = vals;
+  B.N = N;
+  B.r = v; B.z0 = v + stride; B.z1 = v + 2 * stride; B.w = v + 3 * stride; B.pv = v + 4 * stride;
+  B.q = v + 5 * stride;
+  B.x = v + 6 * stride;  // the iterate lives in the workspace, so a captured batch outlives the caller's x
+  B.sc = sc;
+  B.partA = partA;
+  B.partB = partB;
+  B.gr = rows_grid(m.nslices);
+  B.ge = elem_grid(N);
+  const bool graph = !(flags & HF_MG_SOLVE_EAGER);
+  HF_CHECK_HIP(hipMemcpyAsync(B.x, x, N * sizeof(float), hipMemcpyDeviceToDevice, s));
+  mgcg_init_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, b, B.x, B.r, B.pv, B.q, partA);
+  mgcg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, B.gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   float* z = nullptr;
-  if (vcycle(mg, r, z0, z1, &z, sc, s)) return -2;
-  mgcg_spmv_kernel<<<gr, kBlock, 0, s>>>(m, vals, z, r, q, w, sc, partB);
-  mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(sc, partB, gr);
+  if (vcycle(mg, B.r, B.z0, B.z1, &z, sc, s)) return -2;
+  B.z = z;
+  mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, B.z, B.r, B.q, B.w, sc, partB);
+  mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(sc, partB, B.gr);
   HF_LAUNCH_CHECK();
   if (poll_init()) return -2;
   Scalars host{};
@@ -2006,18 +2080,20 @@ extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   bool timed_pending = false;
   int timed_batch = -1;
   for (;;) {
-    for (int k = 0; k < poll; ++k, ++it) {
-      const bool timed = g_prof && mg->nlev > 1 && k == poll / 2 && !timed_pending;  // a steady-state launch
-      mgcg_update_kernel<<<ge, kBlock, 0, s>>>(N, z, w, pv, q, x, r, sc, partA);
-      mgcg_conv_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, ge);
-      if (vcycle(mg, r, z0, z1, &z, sc, s, timed)) return -2;
+    // the batch holding the HIP-event timed launch (a steady-state one) runs
+    // eagerly; the others replay the captured batch
+    const bool timed = g_prof && mg->nlev > 1 && !timed_pending;
+    if (timed || !graph) {
+      if (mg_enqueue_batch(mg, B, poll, s, timed ? poll / 2 : -1)) return -2;
       if (timed) {
         timed_pending = true;
         timed_batch = batch;
       }
-      mgcg_spmv_kernel<<<gr, kBlock, 0, s>>>(m, vals, z, r, q, w, sc, partB);
-      mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(sc, partB, gr);
+    } else {
+      if (mg_batch_graph(mg, p, B, ws, poll)) return -2;
+      HF_CHECK_HIP(hipGraphLaunch(mg->gexec, s));
     }
+    it += poll;
     HF_LAUNCH_CHECK();
     if (poll_post(sc, batch & 1, s)) return -2;
     if (batch > 0) {
@@ -2037,6 +2113,7 @@ extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals
       break;
     }
   }
+  HF_CHECK_HIP(hipMemcpyAsync(x, B.x, N * sizeof(float), hipMemcpyDeviceToDevice, s));
   HF_CHECK_HIP(hipStreamSynchronize(s));
   HF_CHECK_HIP(hipMemcpy(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
   if (info) {

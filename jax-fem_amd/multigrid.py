@@ -185,10 +185,11 @@ class Multigrid:
         return out.reshape(27, 3, 3, n)
 
     def solve(self, A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=500, rel_r0=False, poll=1, stats=None,
-              mask=None, problem=None, rebuild="always", token=None):
+              mask=None, problem=None, rebuild="always", token=None, eager=False):
         """Flexible CG preconditioned by V-cycles; the operator is always the FP32
         tangent A, the hierarchy is rebuilt from A as `rebuild` says (needs_setup).
-        Returns (x float32, info)."""
+        Each batch of `poll` iterations replays a captured HIP graph (eager=True
+        launches every kernel instead).  Returns (x float32, info)."""
         if self.needs_setup(A, rebuild, token):
             self.setup(A, dirichlet_mask(problem or A.problem) if mask is None else mask)
             self._token = token
@@ -200,7 +201,8 @@ class Multigrid:
         info = _lib.HFKrylovInfo()
         t0 = time.perf_counter()
         _lib.call("hf_mg_solve_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(b), _lib.ptr(x),
-                  2 if rel_r0 else 0, int(maxiter), float(rtol), float(atol), int(poll), _lib.ptr(ws),
+                  (2 if rel_r0 else 0) | (_lib.HF_MG_SOLVE_EAGER if eager else 0), int(maxiter), float(rtol),
+                  float(atol), int(poll), _lib.ptr(ws),
                   ctypes.byref(info), _lib.stream())
         self.last = (int(info.iterations), int(info.status))
         self.note_iterations(int(info.iterations))

@@ -255,6 +255,42 @@ def test_mg_fcg_converges(jfa):
     assert info.iterations <= 20 and 8 * info.iterations < jinfo.iterations
 
 
+def test_mg_solve_graph_replays_match_eager(jfa):
+    """The captured poll batch gives the eager loop's iterates bit for bit: on
+    capture, on replay after unrelated eager work, with a new right-hand side,
+    with another poll length, and after a set-up from another tangent."""
+    say("mg fcg graph replay vs eager")
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    prob = box_problem(jfa, (8, 6, 7))
+    mask = MG.dirichlet_mask(prob)
+    mg = MG.Multigrid(MG.lattice(prob))
+    g = torch.Generator(device=DEV).manual_seed(5)
+    A = tangent(jfa, prob, seed=1, scale=0.002)   # small strains: the Neo-Hookean tangent stays SPD
+    mg.setup(A, mask)
+
+    def both(A, b, poll):
+        xe, ie = mg.solve(A, b, rtol=1e-7, maxiter=200, poll=poll, mask=mask, rebuild="never", eager=True)
+        xg, ig = mg.solve(A, b, rtol=1e-7, maxiter=200, poll=poll, mask=mask, rebuild="never")
+        assert ie.status == 0 and ig.status == 0, (ie.status, ig.status)
+        assert ie.iterations == ig.iterations, (ie.iterations, ig.iterations)
+        assert torch.equal(xe, xg), float((xe - xg).abs().max())
+        return xg
+
+    b1 = torch.randn(prob.N, generator=g, device=DEV) * mask
+    x1 = both(A, b1, 2)
+    junk = torch.randn(1 << 20, device=DEV)   # unrelated eager work between replays
+    _ = (junk * 2.0).sum().item()
+    torch.cuda.synchronize()
+    xr, _ = mg.solve(A, b1, rtol=1e-7, maxiter=200, poll=2, mask=mask, rebuild="never")
+    assert torch.equal(xr, x1)
+    b2 = torch.randn(prob.N, generator=g, device=DEV) * mask
+    both(A, b2, 2)
+    both(A, b2, 3)
+    A2 = tangent(jfa, prob, seed=2, scale=0.004)
+    mg.setup(A2, mask)
+    both(A2, b1, 3)
+
+
 def test_newton_with_mg_matches_fp64(jfa):
     say("mg newton")
     cells = (10, 8, 8)
