@@ -274,6 +274,44 @@ int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
                     int32_t maxiter, double rtol, double atol, int32_t poll, void* workspace,
                     hf_krylov_info* info, hf_stream_t stream);
 
+/* ---- the multigrid's own form of a HEX8 lattice tangent, without the SELL
+ *      matrix.  No reference counterpart as a format: it replaces get_A's
+ *      COO -> CSR (solver.py:466-515) plus the row elimination of
+ *      apply_bc (solver.py:290-363) for the multigrid path on box lattices.
+ * The stencil holds, per node, the diagonal block and the blocks toward its 13
+ * upper lattice neighbours (offset o = (dx+1)*9 + (dy+1)*3 + (dz+1) > 13),
+ * slot-major: S[((o - 13) * 9 + 3i + j) * nnode + n] couples DOF i of node n to
+ * DOF j of the neighbour.  Dirichlet rows are kept; the operator is the
+ * row-eliminated A: (A v)_d = v_d on Dirichlet DOFs d, with the mask of the
+ * hierarchy's last set-up.  hf_mg_stencil_size: floats of one stencil. */
+size_t hf_mg_stencil_size(const hf_mg* mg);
+/* HEX8 element Jacobians (as hf_cell_jacobian_f32) restricted to the blocks
+ * (a, b) whose column node id is >= the row node's, cell-innermost:
+ * out[((a * 8 + b) * 9 + 3i + k) * ncell + c] (9 * 64 * ncell floats; the other
+ * blocks are left unwritten) */
+int hf_hex8_jacobian_upper_f32(const hf_material* mat, const double* points, const int32_t* cells, int64_t ncell,
+                               const double* u, const double* params, float* out, hf_stream_t stream);
+/* assemble S from hf_hex8_jacobian_upper_f32's blocks (HEX8, vec 3; cells: the
+ * pattern's cells, ncell x 8): each block sums its cell contributions in
+ * ascending cell order, as hf_assemble_hex8_f32 does.  corner (host, 8 ints, or
+ * NULL): the cells are the lattice's, cell (i, j, k) = (i * (ny-1) + j) * (nz-1) + k
+ * with its corner (dx, dy, dz) at local index corner[4 dx + 2 dy + dz] (a
+ * box_mesh); the contributions then follow from the coordinates (the caller
+ * vouches for the layout: a wrong table gives a wrong stencil).  NULL: the
+ * cells are read through the pattern's contribution lists. */
+int hf_mg_assemble_hex8_f32(const hf_mg* mg, const hf_pattern* p, const int32_t* cells, int64_t ncell,
+                            const int32_t* corner, const float* elem, float* S, hf_stream_t stream);
+/* hf_mg_setup_f32 with level 0 taken from S (a device copy); the pattern is
+ * still checked to be the lattice stencil */
+int hf_mg_setup_stencil_f32(hf_mg* mg, const hf_pattern* p, const float* S, const float* mask, int32_t* dropped,
+                            hf_stream_t stream);
+/* hf_mg_solve_f32 with S as the operator */
+int hf_mg_solve_stencil_f32(hf_mg* mg, const float* S, const float* b, float* x, int32_t flags, int32_t maxiter,
+                            double rtol, double atol, int32_t poll, void* workspace, hf_krylov_info* info,
+                            hf_stream_t stream);
+/* y = A x with S (the row-eliminated operator) */
+int hf_mg_stencil_matvec_f32(const hf_mg* mg, const float* S, const float* x, float* y, hf_stream_t stream);
+
 /* ---- measurement: HIP-event timing of the Krylov SpMV kernel (the first SpMV
  *      of every poll batch, on the solve's stream).  No reference counterpart:
  *      replaces the reference's wall-clock buckets (solver.py:22-57) at kernel

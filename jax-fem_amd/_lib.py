@@ -99,6 +99,13 @@ _SIGS = {
     "hf_mg_workspace_size": (c_sz, [vp]),
     "hf_mg_solve_f32": (c_i32, [vp, vp, vp, vp, vp, c_i32, c_i32, c_f64, c_f64, c_i32, vp,
                                 ctypes.POINTER(HFKrylovInfo), vp]),
+    "hf_mg_stencil_size": (c_sz, [vp]),
+    "hf_mg_assemble_hex8_f32": (c_i32, [vp, vp, vp, c_i64, vp, vp, vp, vp]),
+    "hf_hex8_jacobian_upper_f32": (c_i32, [vp, vp, vp, c_i64, vp, vp, vp, vp]),
+    "hf_mg_setup_stencil_f32": (c_i32, [vp, vp, vp, vp, ctypes.POINTER(c_i32), vp]),
+    "hf_mg_solve_stencil_f32": (c_i32, [vp, vp, vp, vp, c_i32, c_i32, c_f64, c_f64, c_i32, vp,
+                                        ctypes.POINTER(HFKrylovInfo), vp]),
+    "hf_mg_stencil_matvec_f32": (c_i32, [vp, vp, vp, vp, vp]),
     "hf_dot_workspace_size": (c_sz, []),
     "hf_dot_f64": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),
     "hf_dot_f32": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),

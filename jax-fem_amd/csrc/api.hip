@@ -20,6 +20,8 @@ int hex8_jacobian_f64(const hf_material*, const double*, const int32_t*, int64_t
                       double*, hipStream_t);
 int hex8_jacobian_f32(const hf_material*, const double*, const int32_t*, int64_t, const double*, const double*,
                       float*, hipStream_t);
+int hex8_jacobian_upper_f32(const hf_material*, const double*, const int32_t*, int64_t, const double*, const double*,
+                            float*, hipStream_t);
 int hex8_param_vjp(const hf_material*, const double*, const int32_t*, int64_t, const double*, const double*,
                    const double*, double*, hipStream_t);
 int hex27_residual_f64(const hf_material*, const double*, const int32_t*, int64_t, const double*, const double*,
@@ -85,6 +87,11 @@ extern "C" int hf_cell_jacobian_f32(int32_t ele, const hf_material* m, const dou
   if (int e = check_args(ele, m, p, c, u, o)) return e;
   return ele == 8 ? hex8_jacobian_f32(m, p, c, n, u, prm, o, (hipStream_t)s)
                   : hex27_jacobian_f32(m, p, c, n, u, prm, o, (hipStream_t)s);
+}
+extern "C" int hf_hex8_jacobian_upper_f32(const hf_material* m, const double* p, const int32_t* c, int64_t n,
+                                          const double* u, const double* prm, float* o, hf_stream_t s) {
+  if (int e = check_args(8, m, p, c, u, o)) return e;
+  return hex8_jacobian_upper_f32(m, p, c, n, u, prm, o, (hipStream_t)s);
 }
 extern "C" int hf_cell_param_vjp_f64(int32_t ele, const hf_material* m, const double* p, const int32_t* c,
                                      int64_t n, const double* u, const double* prm, const double* lam,

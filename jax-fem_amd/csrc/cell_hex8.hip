@@ -229,7 +229,11 @@ __device__ __forceinline__ void k_row_accumulate(const QpShared<T>& d, int a, T 
 template <typename T>
 constexpr int jac_waves() { return sizeof(T) == 4 ? 2 : 1; }   // LDS: 64 QpShared per wave
 
-template <typename T, int MAT, int VEC>
+// UPPER: only the blocks (a, b) whose column node id is >= the row node's (the
+// diagonal and upper blocks a lattice stencil keeps), cell-innermost:
+// out[((a * 8 + b) * VEC*VEC + VEC*i + k) * ncell + c], so that threads of
+// consecutive nodes read consecutive cells; the other blocks are not written.
+template <typename T, int MAT, int VEC, bool UPPER>
 __global__ void __launch_bounds__(64 * jac_waves<T>()) hex8_jacobian_kernel(
     const double* __restrict__ pts, const int32_t* __restrict__ cells, int64_t ncell,
     const double* __restrict__ u, const double* __restrict__ params, hf_material mat,
@@ -254,13 +258,27 @@ __global__ void __launch_bounds__(64 * jac_waves<T>()) hex8_jacobian_kernel(
       for (int k = 0; k < VEC; ++k) K[b][i][k] = 0;
 #pragma unroll 1
   for (int q = 0; q < 8; ++q) k_row_accumulate<T, MAT, VEC>(sh[wv][ci * 8 + q], r, K);
-  T* o = out + (c * 8 + r) * (8 * VEC * VEC);
+  if constexpr (UPPER) {
+    const int32_t* cp = cells + c * 8;
+    const int32_t nr = cp[r];
 #pragma unroll
-  for (int b = 0; b < 8; ++b)
+    for (int b = 0; b < 8; ++b) {
+      if (cp[b] < nr) continue;
+      T* o = out + (int64_t)((r * 8 + b) * VEC * VEC) * ncell + c;
 #pragma unroll
-    for (int i = 0; i < VEC; ++i)
+      for (int i = 0; i < VEC; ++i)
 #pragma unroll
-      for (int k = 0; k < VEC; ++k) o[(b * VEC + i) * VEC + k] = K[b][i][k];
+        for (int k = 0; k < VEC; ++k) o[(int64_t)(i * VEC + k) * ncell] = K[b][i][k];
+    }
+  } else {
+    T* o = out + (c * 8 + r) * (8 * VEC * VEC);
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+#pragma unroll
+      for (int i = 0; i < VEC; ++i)
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) o[(b * VEC + i) * VEC + k] = K[b][i][k];
+  }
 }
 
 // grad[c,q] = - w sum_ij (sum_a lam[a][i] g_a[j]) dP_ij/dp
@@ -434,7 +452,7 @@ static int launch_hex8_residual(const hf_material* mat, const double* pts, const
   return 0;
 }
 
-template <typename T>
+template <typename T, bool UPPER = false>
 static int launch_hex8_jacobian(const hf_material* mat, const double* pts, const int32_t* cells,
                                 int64_t ncell, const double* u, const double* params, T* out,
                                 hipStream_t s) {
@@ -442,13 +460,15 @@ static int launch_hex8_jacobian(const hf_material* mat, const double* pts, const
   const dim3 blk(64 * jac_waves<T>()), grd((unsigned)ceil_div(ncell, 8 * jac_waves<T>()));
   switch (mat->kind) {
     case MAT_POISSON:
-      hex8_jacobian_kernel<T, MAT_POISSON, 1><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      hex8_jacobian_kernel<T, MAT_POISSON, 1, UPPER><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
       break;
     case MAT_LINEAR_ELASTIC:
-      hex8_jacobian_kernel<T, MAT_LINEAR_ELASTIC, 3><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      hex8_jacobian_kernel<T, MAT_LINEAR_ELASTIC, 3, UPPER><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat,
+                                                                                out);
       break;
     case MAT_NEO_HOOKEAN:
-      hex8_jacobian_kernel<T, MAT_NEO_HOOKEAN, 3><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat, out);
+      hex8_jacobian_kernel<T, MAT_NEO_HOOKEAN, 3, UPPER><<<grd, blk, 0, s>>>(pts, cells, ncell, u, params, *mat,
+                                                                             out);
       break;
     default:
       HF_REQUIRE(false, "unknown material kind %d", mat->kind);
@@ -494,6 +514,10 @@ int hex8_jacobian_f64(const hf_material* m, const double* p, const int32_t* c, i
 int hex8_jacobian_f32(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
                       const double* prm, float* o, hipStream_t s) {
   return launch_hex8_jacobian<float>(m, p, c, n, u, prm, o, s);
+}
+int hex8_jacobian_upper_f32(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
+                            const double* prm, float* o, hipStream_t s) {
+  return launch_hex8_jacobian<float, true>(m, p, c, n, u, prm, o, s);
 }
 
 }  // namespace hf

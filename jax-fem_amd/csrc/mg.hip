@@ -276,6 +276,99 @@ __global__ void __launch_bounds__(kMgBlock) mg_fine_stencil(const int32_t* __res
   for (int e = 0; e < 9; ++e) S[(int64_t)(so * 9 + e) * g.n + n] = out[e];
 }
 
+// The same stencil on a box mesh whose cells are the lattice's (cell (i, j, k) =
+// (i * ncy + j) * ncz + k, corner (dx, dy, dz) of cell c at local index
+// corner[dx * 4 + dy * 2 + dz]): the contributing cells, their order and both
+// local indices follow from the coordinates, so a thread issues all its block
+// loads at once.  Cells are visited in ascending id, as the generic kernel does.
+struct CornerTable {
+  int8_t idx[8];
+};
+__global__ void __launch_bounds__(kMgBlock) mg_assemble_hex8_box(Lat g, int64_t ncell, CornerTable ct,
+                                                                 const float* __restrict__ elem,
+                                                                 float* __restrict__ S) {
+  const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (n >= g.n) return;
+  const int so = blockIdx.y, o = 13 + so;
+  int p[3];
+  lat_xyz(g, n, p[0], p[1], p[2]);
+  const int off[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+  const int nc[3] = {g.nx - 1, g.ny - 1, g.nz - 1};
+  // per dimension the candidate cells c_d in {p_d - 1, p_d} holding both p_d and p_d + off_d
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = max(max(p[d] - 1, p[d] + off[d] - 1), 0);
+    hi[d] = min(min(p[d], p[d] + off[d]), nc[d] - 1);
+  }
+  float acc[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) acc[e] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int cx = p[0] - 1 + i, cy = p[1] - 1 + j, cz = p[2] - 1 + k;
+        if (cx < lo[0] || cx > hi[0] || cy < lo[1] || cy > hi[1] || cz < lo[2] || cz > hi[2]) continue;
+        const int64_t c = ((int64_t)cx * nc[1] + cy) * nc[2] + cz;
+        const int a = ct.idx[(p[0] - cx) * 4 + (p[1] - cy) * 2 + (p[2] - cz)];
+        const int b = ct.idx[(p[0] + off[0] - cx) * 4 + (p[1] + off[1] - cy) * 2 + (p[2] + off[2] - cz)];
+        const float* src = elem + (int64_t)((a * 8 + b) * 9) * ncell + c;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] += src[(int64_t)e * ncell];
+      }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) S[(int64_t)(so * 9 + e) * g.n + n] = acc[e];
+}
+
+// Level-0 stencil of a HEX8 tangent straight from the element Jacobians (the
+// upper blocks, cell-innermost: elem[((a * 8 + b) * 9 + e) * ncell + c], see
+// hf_hex8_jacobian_upper_f32), without the SELL matrix: the diagonal and the 13
+// upper blocks of every node, Dirichlet rows included (the multigrid masks them
+// where it must).  A thread per (node, stored slot; the slot is blockIdx.y)
+// walks the node's cells in ascending order (the diagonal block's contribution
+// list), finds the neighbour among the cell's nodes and adds that block: the
+// same FP32 additions in the same order as the SELL assembly of the block.
+// Threads of consecutive nodes read consecutive cells.
+__global__ void __launch_bounds__(kMgBlock) mg_assemble_hex8(const int32_t* __restrict__ diag,
+                                                             const int32_t* __restrict__ seg,
+                                                             const int32_t* __restrict__ contrib,
+                                                             const int32_t* __restrict__ cells, Lat g,
+                                                             int64_t ncell, const float* __restrict__ elem,
+                                                             float* __restrict__ S) {
+  const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (n >= g.n) return;
+  const int so = blockIdx.y, o = 13 + so;
+  int x, y, z;
+  lat_xyz(g, n, x, y, z);
+  const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
+  float acc[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) acc[e] = 0.f;
+  const int32_t db = diag[n];
+  if (db >= 0 && xr < g.nx && yr < g.ny && zr < g.nz && xr >= 0 && yr >= 0 && zr >= 0) {
+    const int32_t m = (int32_t)lat_id(g, xr, yr, zr);
+    for (int32_t s = seg[db]; s < seg[db + 1]; ++s) {
+      const int32_t id = contrib[s];  // (cell * 8 + a) * 8 + a
+      const int64_t c = id / 64;
+      const int a = (id / 8) % 8;
+      const int4 c0 = *reinterpret_cast<const int4*>(cells + c * 8);
+      const int4 c1 = *reinterpret_cast<const int4*>(cells + c * 8 + 4);
+      const int b = c0.x == m ? 0 : c0.y == m ? 1 : c0.z == m ? 2 : c0.w == m ? 3 :
+                    c1.x == m ? 4 : c1.y == m ? 5 : c1.z == m ? 6 : c1.w == m ? 7 : -1;
+      if (b < 0) continue;  // the cell does not hold the neighbour
+      const float* src = elem + (int64_t)((a * 8 + b) * 9) * ncell + c;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) acc[e] += src[(int64_t)e * ncell];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) S[(int64_t)(so * 9 + e) * g.n + n] = acc[e];
+}
+
+
 // the full 27-slot stencil of a half-stored one (read-back)
 __global__ void __launch_bounds__(kMgBlock) mg_expand_half(const float* __restrict__ Sh, Lat g,
                                                            float* __restrict__ S) {
@@ -304,10 +397,18 @@ __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restric
   double a[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) a[e] = (double)S[(int64_t)(diag_slot(HALF) * 9 + e) * n + node];
+  float mk[3] = {1.f, 1.f, 1.f};
   if (mask) {
+    // the block of M A M (the stored block may hold the Dirichlet rows), with 1 on
+    // the masked diagonal so that it inverts; the masked rows and columns of the
+    // inverse are zeroed below
+#pragma unroll
+    for (int i = 0; i < 3; ++i) mk[i] = mask[3 * node + i];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a[e] *= (double)(mk[e / 3] * mk[e % 3]);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-      if (mask[3 * node + i] == 0.f) a[4 * i] += 1.0;
+      if (mk[i] == 0.f) a[4 * i] += 1.0;
   }
   double c[9];
   c[0] = a[4] * a[8] - a[5] * a[7];
@@ -323,12 +424,12 @@ __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restric
   const double sc = fmax(fabs(a[0]), fmax(fabs(a[4]), fabs(a[8])));
   if (det > 1e-12 * sc * sc * sc && isfinite(det)) {
 #pragma unroll
-    for (int e = 0; e < 9; ++e) Dinv[(int64_t)e * n + node] = (float)(c[e] / det);
+    for (int e = 0; e < 9; ++e) Dinv[(int64_t)e * n + node] = (float)(c[e] / det) * mk[e / 3] * mk[e % 3];
   } else {
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
       const bool d = (e % 4) == 0;
-      Dinv[(int64_t)e * n + node] = (d && a[e] > 0.0) ? (float)(1.0 / a[e]) : 0.f;
+      Dinv[(int64_t)e * n + node] = (d && a[e] > 0.0) ? (float)(1.0 / a[e]) * mk[e / 3] : 0.f;
     }
   }
 }
@@ -344,7 +445,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restric
 // blocks.  The intermediate lattices hold all 27 slots.
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_rap_dim(const float* __restrict__ S, Lat f, Lat c, int dim,
-                                                       float* __restrict__ Out) {
+                                                       const float* __restrict__ mask, float* __restrict__ Out) {
   const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
   if (n >= c.n) return;
   const int q = blockIdx.y;
@@ -393,6 +494,10 @@ __global__ void __launch_bounds__(kMgBlock) mg_rap_dim(const float* __restrict__
       const int64_t m = lat_id(f, clampi(nb[0], 0, f.nx - 1), clampi(nb[1], 0, f.ny - 1), clampi(nb[2], 0, f.nz - 1));
       float B[9];
       load_block<HALF>(S, f.n, row, o, m, valid, B);
+      if (mask) {  // the fine level's M A M
+#pragma unroll
+        for (int e = 0; e < 9; ++e) B[e] *= mask[3 * row + e / 3] * mask[3 * m + e % 3];
+      }
 #pragma unroll
       for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -611,17 +716,22 @@ __device__ __forceinline__ void poly_step_node(const float* __restrict__ S, cons
   }
 }
 
-// t = res_in - A d_in (the residual after the last pre-smoothing step)
+// t = res_in - A d_in (the residual after the last pre-smoothing step); with a
+// mask (the fine level, whose stencil may hold the Dirichlet rows) t = M (...):
+// the restriction must not see those rows
 template <bool HALF>
 __device__ __forceinline__ void poly_resid_node(const float* __restrict__ S, const Lat& g,
                                                 const float* __restrict__ res_in, const float* __restrict__ d_in,
-                                                float* __restrict__ t, int64_t node) {
+                                                const float* __restrict__ mask, float* __restrict__ t, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ad[3];
   stencil_row<HALF>(S, g, node, x, y, z, d_in, ad);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) t[3 * node + i] = res_in[3 * node + i] - ad[i];
+  for (int i = 0; i < 3; ++i) {
+    const float v = res_in[3 * node + i] - ad[i];
+    t[3 * node + i] = mask ? v * mask[3 * node + i] : v;
+  }
 }
 
 // post-smoother start: res = r - A e_in;  d = s0 Dinv res;  e_out = e_in + d
@@ -757,10 +867,11 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_step(const float* __restrict
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const float* __restrict__ S, Lat g,
                                                           const float* __restrict__ res_in,
-                                                          const float* __restrict__ d_in, float* __restrict__ t,
+                                                          const float* __restrict__ d_in,
+                                                          const float* __restrict__ mask, float* __restrict__ t,
                                                           const Scalars* sc) {
   if (sc && sc->done) return;
-  MG_NODE_LOOP(g.n, poly_resid_node<HALF>(S, g, res_in, d_in, t, node));
+  MG_NODE_LOOP(g.n, poly_resid_node<HALF>(S, g, res_in, d_in, mask, t, node));
 }
 
 template <bool HALF>
@@ -792,7 +903,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_prolong(const float* __restrict__
 // absolute row sum of the node's block row of Dinv A.
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S, const float* __restrict__ Dinv, Lat g,
-                                                     float* __restrict__ part) {
+                                                     const float* __restrict__ mask, float* __restrict__ part) {
   __shared__ float sm[kMgBlock / kWave];
   const int64_t n = g.n;
   float best = 0.f;
@@ -810,11 +921,17 @@ __global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S
       const int64_t m = lat_id(g, clampi(xr, 0, g.nx - 1), clampi(yr, 0, g.ny - 1), clampi(zr, 0, g.nz - 1));
       float B[9];
       load_block<HALF>(S, n, node, o, m, valid, B);
+      // columns of Dirichlet DOFs (M A M; the rows are zero in Dinv already)
+      float cm[3] = {1.f, 1.f, 1.f};
+      if (mask) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) cm[j] = mask[3 * m + j];
+      }
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          rs[i] += fabsf(M[3 * i] * B[j] + M[3 * i + 1] * B[3 + j] + M[3 * i + 2] * B[6 + j]);
+          rs[i] += fabsf(M[3 * i] * B[j] + M[3 * i + 1] * B[3 + j] + M[3 * i + 2] * B[6 + j]) * cm[j];
     }
     best = fmaxf(best, fmaxf(rs[0], fmaxf(rs[1], rs[2])));
   }
@@ -1090,12 +1207,16 @@ template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_poly_resid_lanes(const float* __restrict__ S, Lat g,
                                                                 const float* __restrict__ res_in,
                                                                 const float* __restrict__ d_in,
+                                                                const float* __restrict__ mask,
                                                                 float* __restrict__ t, const Scalars* sc) {
   if (sc && sc->done) return;
   MG_LANE_LOOP(g.n, {
     float ad[3];
     stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad);
-    if (q < 3) t[3 * node + q] = res_in[3 * node + q] - ad[q];
+    if (q < 3) {
+      const float v = res_in[3 * node + q] - ad[q];
+      t[3 * node + q] = mask ? v * mask[3 * node + q] : v;
+    }
   });
 }
 
@@ -1159,12 +1280,12 @@ __global__ void __launch_bounds__(kMgBlock) mg_sell_dinv(const int32_t* __restri
   const double sc = fmax(fabs(a[0]), fmax(fabs(a[4]), fabs(a[8])));
   if (det > 1e-12 * sc * sc * sc && isfinite(det)) {
 #pragma unroll
-    for (int e = 0; e < 9; ++e) Dinv[(int64_t)e * n + node] = (float)(c[e] / det);
+    for (int e = 0; e < 9; ++e) Dinv[(int64_t)e * n + node] = (float)(c[e] / det) * m[e / 3] * m[e % 3];
   } else {
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
       const bool d = (e % 4) == 0;
-      Dinv[(int64_t)e * n + node] = (d && a[e] > 0.0) ? (float)(1.0 / a[e]) : 0.f;
+      Dinv[(int64_t)e * n + node] = (d && a[e] > 0.0) ? (float)(1.0 / a[e]) * m[e / 3] : 0.f;
     }
   }
 }
@@ -1424,6 +1545,77 @@ __global__ void __launch_bounds__(kBlock) mgcg_spmv_kernel(MatArgs m, const floa
   };
   sweep<float, 3, false>(m, r0, r1, vals, xv, epi);
   block_partials<3>(loc, partials);
+}
+
+// The same two phases with the tangent as a half-stored lattice stencil (Dirichlet
+// rows included): A v = M (S v) + (1 - M) v, the row-eliminated operator.
+__global__ void __launch_bounds__(kMgBlock) mgcg_init_stencil(const float* __restrict__ S, Lat g,
+                                                              const float* __restrict__ mask,
+                                                              const float* __restrict__ b,
+                                                              const float* __restrict__ x, float* __restrict__ r,
+                                                              float* __restrict__ p, float* __restrict__ q,
+                                                              double* partials) {
+  double loc[2] = {0, 0};
+  MG_NODE_LOOP(g.n, {
+    int cx, cy, cz;
+    lat_xyz(g, node, cx, cy, cz);
+    float ax[3];
+    stencil_row<true>(S, g, node, cx, cy, cz, x, ax);
+    for (int i = 0; i < 3; ++i) {
+      const int64_t d = node * 3 + i;
+      const float mk = mask[d];
+      const float rr = b[d] - (mk * ax[i] + (1.f - mk) * x[d]);
+      r[d] = rr;
+      p[d] = 0.f;
+      q[d] = 0.f;
+      loc[0] += (double)rr * (double)rr;
+      loc[1] += (double)b[d] * (double)b[d];
+    }
+  });
+  block_partials<2>(loc, partials);
+}
+
+__global__ void __launch_bounds__(kMgBlock) mgcg_spmv_stencil(const float* __restrict__ S, Lat g,
+                                                              const float* __restrict__ mask,
+                                                              const float* __restrict__ z,
+                                                              const float* __restrict__ r,
+                                                              const float* __restrict__ q, float* __restrict__ w,
+                                                              const Scalars* sc, double* partials) {
+  if (sc->done) return;
+  double loc[3] = {0, 0, 0};
+  MG_NODE_LOOP(g.n, {
+    int cx, cy, cz;
+    lat_xyz(g, node, cx, cy, cz);
+    float az[3];
+    stencil_row<true>(S, g, node, cx, cy, cz, z, az);
+    for (int i = 0; i < 3; ++i) {
+      const int64_t d = node * 3 + i;
+      const float mk = mask[d];
+      const float zz = z[d];
+      const float v = mk * az[i] + (1.f - mk) * zz;
+      w[d] = v;
+      loc[0] += (double)r[d] * (double)zz;
+      loc[1] += (double)zz * (double)v;
+      loc[2] += (double)zz * (double)q[d];
+    }
+  });
+  block_partials<3>(loc, partials);
+}
+
+// y = A x for the stencil form (tests and the fallback)
+__global__ void __launch_bounds__(kMgBlock) mg_stencil_matvec(const float* __restrict__ S, Lat g,
+                                                              const float* __restrict__ mask,
+                                                              const float* __restrict__ x, float* __restrict__ y) {
+  MG_NODE_LOOP(g.n, {
+    int cx, cy, cz;
+    lat_xyz(g, node, cx, cy, cz);
+    float ax[3];
+    stencil_row<true>(S, g, node, cx, cy, cz, x, ax);
+    for (int i = 0; i < 3; ++i) {
+      const int64_t d = node * 3 + i;
+      y[d] = mask[d] * ax[i] + (1.f - mask[d]) * x[d];
+    }
+  });
 }
 
 // beta, p.q and alpha of the new direction
@@ -1690,9 +1882,12 @@ extern "C" int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float
   return 0;
 }
 
-extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* mask,
-                               int32_t* dropped, hf_stream_t stream) {
-  HF_REQUIRE(mg && p && vals && mask, "hf_mg_setup_f32: null argument");
+// Set-up from the SELL tangent (vals) or from a half-stored level-0 stencil (S0).
+static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const float* S0, const float* mask,
+                    int32_t* dropped, hf_stream_t stream) {
+  HF_REQUIRE(mg && p && (vals || S0) && mask, "hf_mg_setup_f32: null argument");
+  HF_REQUIRE(!S0 || (mg->half && !mg->sell), "hf_mg_setup_stencil_f32: the hierarchy does not keep a half-stored "
+                                              "fine stencil (HF_MG_FINE_FULL / HF_MG_FINE_SELL, or one level)");
   HF_REQUIRE(pattern_vec(p) == 3, "hf_mg_setup_f32: vec=%d, the multigrid is built for vec=3", pattern_vec(p));
   const Lat g0 = mg->lv[0].g;
   HF_REQUIRE(pattern_nnode(p) == g0.n, "hf_mg_setup_f32: pattern has %lld nodes, lattice %lld",
@@ -1721,7 +1916,10 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
     mg_check_lattice<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p),
                                                         pattern_sell_pos(p), pattern_slice_ptr(p), pattern_R(p), g0,
                                                         mg->flags);
-  if (mg->half)
+  if (S0)
+    HF_CHECK_HIP(hipMemcpyAsync(mg->lv[0].S, S0, (size_t)kHalfSlots * 9 * g0.n * sizeof(float),
+                                hipMemcpyDeviceToDevice, s));
+  else if (mg->half)
     mg_fine_stencil<true><<<dim3(gb0, kHalfSlots), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p),
                                                                      pattern_slice_ptr(p), pattern_R(p), vals, mask,
                                                                      g0, mg->lv[0].S);
@@ -1738,14 +1936,16 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
       const float* Sp = mg->lv[l].S;
       mg_block_dinv<H><<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(Sp, l == 0 ? mask : nullptr, f.n,
                                                                               mg->lv[l].Dinv);
-      mg_gersh<H><<<gg, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, mg->part);
+      mg_gersh<H><<<gg, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, l == 0 ? mask : nullptr, mg->part);
       // P^T A P: x, then y, then z; the two intermediates share the B buffer
       const Lat bx = make_lat(c.nx, f.ny, f.nz), bxy = make_lat(c.nx, c.ny, f.nz);
       float* T1 = mg->B;
       float* T2 = mg->B + (int64_t)kSt * bx.n;
-      mg_rap_dim<H><<<dim3((unsigned)ceil_div(bx.n, kMgBlock), 9), kMgBlock, 0, s>>>(Sp, f, bx, 0, T1);
-      mg_rap_dim<false><<<dim3((unsigned)ceil_div(bxy.n, kMgBlock), 9), kMgBlock, 0, s>>>(T1, bx, bxy, 1, T2);
-      mg_rap_dim<false><<<dim3((unsigned)ceil_div(c.n, kMgBlock), 9), kMgBlock, 0, s>>>(T2, bxy, c, 2,
+      mg_rap_dim<H><<<dim3((unsigned)ceil_div(bx.n, kMgBlock), 9), kMgBlock, 0, s>>>(Sp, f, bx, 0,
+                                                                                   l == 0 ? mask : nullptr, T1);
+      mg_rap_dim<false><<<dim3((unsigned)ceil_div(bxy.n, kMgBlock), 9), kMgBlock, 0, s>>>(T1, bx, bxy, 1, nullptr,
+                                                                                         T2);
+      mg_rap_dim<false><<<dim3((unsigned)ceil_div(c.n, kMgBlock), 9), kMgBlock, 0, s>>>(T2, bxy, c, 2, nullptr,
                                                                                       mg->lv[l + 1].S);
     };
     if (l == 0 && mg->half) level(std::true_type{});
@@ -1777,6 +1977,60 @@ extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   if (dropped) *dropped = hflags[1];
   mg->mask = mask;
   mg->ready = true;
+  return 0;
+}
+
+extern "C" int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* mask,
+                               int32_t* dropped, hf_stream_t stream) {
+  HF_REQUIRE(vals, "hf_mg_setup_f32: null argument");
+  return mg_setup(mg, p, vals, nullptr, mask, dropped, stream);
+}
+
+extern "C" int hf_mg_setup_stencil_f32(hf_mg* mg, const hf_pattern* p, const float* S, const float* mask,
+                                       int32_t* dropped, hf_stream_t stream) {
+  HF_REQUIRE(S, "hf_mg_setup_stencil_f32: null argument");
+  return mg_setup(mg, p, nullptr, S, mask, dropped, stream);
+}
+
+extern "C" size_t hf_mg_stencil_size(const hf_mg* mg) {
+  return mg ? (size_t)kHalfSlots * 9 * mg->lv[0].g.n : 0;
+}
+
+extern "C" int hf_mg_assemble_hex8_f32(const hf_mg* mg, const hf_pattern* p, const int32_t* cells, int64_t ncell,
+                                       const int32_t* corner, const float* elem, float* S, hf_stream_t stream) {
+  HF_REQUIRE(mg && p && cells && elem && S, "hf_mg_assemble_hex8_f32: null argument");
+  const Lat g = mg->lv[0].g;
+  HF_REQUIRE(pattern_vec(p) == 3 && pattern_nn(p) == 8 && pattern_nnode(p) == g.n,
+             "hf_mg_assemble_hex8_f32: needs the HEX8 vec-3 pattern of the hierarchy's lattice");
+  HF_REQUIRE(g.n < INT32_MAX, "hf_mg_assemble_hex8_f32: lattice too large");
+  const dim3 grid((unsigned)ceil_div(g.n, kMgBlock), kHalfSlots);
+  if (corner) {
+    HF_REQUIRE(ncell == (int64_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1),
+               "hf_mg_assemble_hex8_f32: %lld cells are not the lattice's", (long long)ncell);
+    CornerTable ct;
+    bool seen[8] = {false, false, false, false, false, false, false, false};
+    for (int q = 0; q < 8; ++q) {
+      HF_REQUIRE(corner[q] >= 0 && corner[q] < 8 && !seen[corner[q]],
+                 "hf_mg_assemble_hex8_f32: corner table is not a permutation of 0..7");
+      seen[corner[q]] = true;
+      ct.idx[q] = (int8_t)corner[q];
+    }
+    mg_assemble_hex8_box<<<grid, kMgBlock, 0, (hipStream_t)stream>>>(g, ncell, ct, elem, S);
+  } else {
+    mg_assemble_hex8<<<grid, kMgBlock, 0, (hipStream_t)stream>>>(pattern_diag(p), pattern_seg(p),
+                                                                pattern_contrib(p), cells, g, ncell, elem, S);
+  }
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int hf_mg_stencil_matvec_f32(const hf_mg* mg, const float* S, const float* x, float* y,
+                                        hf_stream_t stream) {
+  HF_REQUIRE(mg && S && x && y, "hf_mg_stencil_matvec_f32: null argument");
+  HF_REQUIRE(mg->mask, "hf_mg_stencil_matvec_f32: no set-up (the Dirichlet mask comes from it)");
+  const Lat g = mg->lv[0].g;
+  mg_stencil_matvec<<<node_grid(g.n), kMgBlock, 0, (hipStream_t)stream>>>(S, g, mg->mask, x, y);
+  HF_LAUNCH_CHECK();
   return 0;
 }
 
@@ -1830,10 +2084,11 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
         res = V.res;
         float* tmp = din; din = dout; dout = tmp;
       }
+      const float* rmask = l == 0 ? mg->mask : nullptr;
       if (lanes)
-        mg_poly_resid_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.g, res, din, dout, sc);
+        mg_poly_resid_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.g, res, din, rmask, dout, sc);
       else
-        mg_poly_resid<H><<<gg, kMgBlock, 0, s>>>(V.S, V.g, res, din, dout, sc);
+        mg_poly_resid<H><<<gg, kMgBlock, 0, s>>>(V.S, V.g, res, din, rmask, dout, sc);
     };
     if (l == 0 && mg->sell) {
       // SELL fine level: each pass is y = A v, then the node-wise update
@@ -1985,7 +2240,10 @@ extern "C" size_t hf_mg_workspace_size(const hf_mg* mg) {
 // direction scalars) on stream s; timed marks one launch for HIP-event timing.
 struct MgBatch {
   MatArgs m;
-  const float* vals;
+  const float* vals;  // the SELL tangent, or
+  const float* S;     // the half-stored lattice stencil (Dirichlet rows masked on the fly)
+  const float* mask;
+  Lat g;
   int64_t N;
   float *r, *z0, *z1, *w, *pv, *q, *x;
   float* z;  // the V-cycle's output buffer (the same one every cycle)
@@ -2000,7 +2258,10 @@ static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int 
     float* z = nullptr;
     if (vcycle(mg, B.r, B.z0, B.z1, &z, B.sc, s, k == timed_k)) return -2;
     B.z = z;
-    mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(B.m, B.vals, B.z, B.r, B.q, B.w, B.sc, B.partB);
+    if (B.S)
+      mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(B.S, B.g, B.mask, B.z, B.r, B.q, B.w, B.sc, B.partB);
+    else
+      mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(B.m, B.vals, B.z, B.r, B.q, B.w, B.sc, B.partB);
     mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partB, B.gr);
   }
   return 0;
@@ -2011,8 +2272,8 @@ static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int 
 // contents: the set-up generation (level coefficients are kernel arguments),
 // the tangent's values and pattern, the workspace and the batch length.
 static int mg_batch_graph(hf_mg* mg, const hf_pattern* p, MgBatch& B, void* ws, int poll) {
-  const uintptr_t key[6] = {(uintptr_t)mg->gen, (uintptr_t)p, (uintptr_t)B.vals, (uintptr_t)ws, (uintptr_t)poll,
-                            (uintptr_t)B.N};
+  const uintptr_t key[6] = {(uintptr_t)mg->gen, (uintptr_t)p, (uintptr_t)(B.S ? B.S : B.vals), (uintptr_t)ws,
+                            (uintptr_t)poll, (uintptr_t)B.N};
   bool same = mg->gexec != nullptr;
   for (int i = 0; i < 6 && same; ++i) same = mg->gkey[i] == key[i];
   if (same) return 0;
@@ -2034,19 +2295,17 @@ static int mg_batch_graph(hf_mg* mg, const hf_pattern* p, MgBatch& B, void* ws, 
   return 0;
 }
 
-extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* b, float* x,
-                               int32_t flags, int32_t maxiter, double rtol, double atol, int32_t poll, void* ws,
-                               hf_krylov_info* info, hf_stream_t stream) {
-  HF_REQUIRE(mg && p && vals && b && x && ws, "hf_mg_solve_f32: null argument");
+// Flexible CG with the SELL tangent (p, vals) or the half-stored stencil S as the operator.
+static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const float* S, const float* b, float* x,
+                    int32_t flags, int32_t maxiter, double rtol, double atol, int32_t poll, void* ws,
+                    hf_krylov_info* info, hipStream_t s) {
   HF_REQUIRE(mg->ready, "hf_mg_solve_f32: hf_mg_setup_f32 has not run");
-  HF_REQUIRE(pattern_vec(p) == 3 && pattern_nnode(p) == mg->lv[0].g.n, "hf_mg_solve_f32: pattern does not match");
   HF_REQUIRE(!(flags & HF_KRYLOV_TRANSPOSE), "hf_mg_solve_f32: transposed solves are not supported");
   HF_REQUIRE(maxiter >= 0, "hf_mg_solve_f32: maxiter < 0");
   if (poll < 1) poll = 1;
-  hipStream_t s = (hipStream_t)stream;
   const int rel_r0 = (flags & HF_KRYLOV_REL_R0) ? 1 : 0;
-  const MatArgs m = mat_args(p);
-  const int64_t N = m.nrows * 3;
+  const MatArgs m = S ? MatArgs{} : mat_args(p);
+  const int64_t N = mg->lv[0].g.n * 3;
   Scalars* sc = (Scalars*)ws;
   double* partA = (double*)((char*)ws + 4096);
   double* partB = partA + kPartB;
@@ -2055,6 +2314,9 @@ extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   MgBatch B;
   B.m = m;
   B.vals = vals;
+  B.S = S;
+  B.mask = mg->mask;
+  B.g = mg->lv[0].g;
   B.N = N;
   B.r = v; B.z0 = v + stride; B.z1 = v + 2 * stride; B.w = v + 3 * stride; B.pv = v + 4 * stride;
   B.q = v + 5 * stride;
@@ -2062,16 +2324,22 @@ extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals
   B.sc = sc;
   B.partA = partA;
   B.partB = partB;
-  B.gr = rows_grid(m.nslices);
+  B.gr = S ? node_grid(mg->lv[0].g.n) : rows_grid(m.nslices);  // the operator kernels' grid (= partial count)
   B.ge = elem_grid(N);
   const bool graph = !(flags & HF_MG_SOLVE_EAGER);
   HF_CHECK_HIP(hipMemcpyAsync(B.x, x, N * sizeof(float), hipMemcpyDeviceToDevice, s));
-  mgcg_init_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, b, B.x, B.r, B.pv, B.q, partA);
+  if (S)
+    mgcg_init_stencil<<<B.gr, kMgBlock, 0, s>>>(S, B.g, B.mask, b, B.x, B.r, B.pv, B.q, partA);
+  else
+    mgcg_init_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, b, B.x, B.r, B.pv, B.q, partA);
   mgcg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, B.gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   float* z = nullptr;
   if (vcycle(mg, B.r, B.z0, B.z1, &z, sc, s)) return -2;
   B.z = z;
-  mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, B.z, B.r, B.q, B.w, sc, partB);
+  if (S)
+    mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(S, B.g, B.mask, B.z, B.r, B.q, B.w, sc, partB);
+  else
+    mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, B.z, B.r, B.q, B.w, sc, partB);
   mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(sc, partB, B.gr);
   HF_LAUNCH_CHECK();
   if (poll_init()) return -2;
@@ -2123,4 +2391,20 @@ extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals
     info->b_norm = sqrt(host.bb);
   }
   return 0;
+}
+
+extern "C" int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* b, float* x,
+                               int32_t flags, int32_t maxiter, double rtol, double atol, int32_t poll, void* ws,
+                               hf_krylov_info* info, hf_stream_t stream) {
+  HF_REQUIRE(mg && p && vals && b && x && ws, "hf_mg_solve_f32: null argument");
+  HF_REQUIRE(pattern_vec(p) == 3 && pattern_nnode(p) == mg->lv[0].g.n, "hf_mg_solve_f32: pattern does not match");
+  return mg_solve(mg, p, vals, nullptr, b, x, flags, maxiter, rtol, atol, poll, ws, info, (hipStream_t)stream);
+}
+
+extern "C" int hf_mg_solve_stencil_f32(hf_mg* mg, const float* S, const float* b, float* x, int32_t flags,
+                                       int32_t maxiter, double rtol, double atol, int32_t poll, void* ws,
+                                       hf_krylov_info* info, hf_stream_t stream) {
+  HF_REQUIRE(mg && S && b && x && ws, "hf_mg_solve_stencil_f32: null argument");
+  HF_REQUIRE(mg->half && !mg->sell, "hf_mg_solve_stencil_f32: the hierarchy has no half-stored fine stencil");
+  return mg_solve(mg, nullptr, nullptr, S, b, x, flags, maxiter, rtol, atol, poll, ws, info, (hipStream_t)stream);
 }

@@ -52,6 +52,53 @@ def dirichlet_mask(problem):
     return m
 
 
+class LatticeTangent:
+    """FP32 tangent of a HEX8 box-lattice problem in the multigrid's stencil form
+    (Problem.jacobian_stencil): the operator is the row-eliminated tangent, as the
+    TangentMatrix the reference's get_A / apply_bc give (solver.py:466-515,
+    290-363), applied from each node's diagonal and upper node blocks.  It is valid
+    until the problem assembles its next stencil; to_sell() assembles the SELL
+    TangentMatrix at the same state for the paths that need one (the Jacobi-PCG
+    fallback)."""
+
+    def __init__(self, problem, u, mg):
+        self.problem = problem
+        self.pattern = problem.pattern
+        self.mg = mg
+        self.u = u
+        self.dtype = torch.float32
+        self.shape = (problem.N, problem.N)
+        self.stencil, self._version = problem.jacobian_stencil(u, mg)
+        self._sell = None
+
+    @property
+    def ident(self):
+        """Identity of this tangent for the hierarchy's rebuild policy (the stencil buffer is reused)."""
+        return self
+
+    def check_current(self):
+        if getattr(self.problem, "_stencil_version", None) != self._version:
+            raise RuntimeError("LatticeTangent: the problem has assembled a newer stencil over this one")
+
+    def to_sell(self):
+        if self._sell is None:
+            from .solver import TangentMatrix
+            self._sell = TangentMatrix(self.problem, self.problem.jacobian_bsr(self.u, torch.float32))
+        return self._sell
+
+    def matvec(self, x):
+        """A x with the stencil (needs a set-up of the hierarchy for the Dirichlet mask)."""
+        self.check_current()
+        x = x.to(torch.float32).contiguous()
+        y = torch.empty_like(x)
+        _lib.call("hf_mg_stencil_matvec_f32", self.mg.handle, _lib.ptr(self.stencil), _lib.ptr(x), _lib.ptr(y),
+                  _lib.stream())
+        return y
+
+    def __matmul__(self, x):
+        return self.matvec(x)
+
+
 class Multigrid:
     def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None, fine_full=False,
                  fine_sell=False):
@@ -125,17 +172,27 @@ class Multigrid:
         _lib.call("hf_mg_levels", self.handle, ctypes.byref(n), dims, 16)
         return [tuple(dims[3 * i:3 * i + 3]) for i in range(n.value)]
 
+    def stencil_size(self):
+        """Floats of one level-0 stencil (LatticeTangent)."""
+        return int(_lib.load().hf_mg_stencil_size(self.handle))
+
     def setup(self, A, mask):
-        """Build every level from the FP32 TangentMatrix A and the Dirichlet mask."""
+        """Build every level from the FP32 tangent A (a TangentMatrix, or a
+        LatticeTangent of this hierarchy's lattice) and the Dirichlet mask."""
         if A.dtype != torch.float32:
             raise ValueError("the multigrid preconditioner runs on the FP32 tangent")
         mask = mask.to(torch.float32).contiguous()
         dropped = ctypes.c_int32()
         t0 = time.perf_counter()
-        _lib.call("hf_mg_setup_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(mask),
-                  ctypes.byref(dropped), _lib.stream())
+        if isinstance(A, LatticeTangent):
+            A.check_current()
+            _lib.call("hf_mg_setup_stencil_f32", self.handle, A.pattern.handle, _lib.ptr(A.stencil),
+                      _lib.ptr(mask), ctypes.byref(dropped), _lib.stream())
+        else:
+            _lib.call("hf_mg_setup_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(mask),
+                      ctypes.byref(dropped), _lib.stream())
         self.setup_seconds += time.perf_counter() - t0
-        self._tangent = A.vals      # identity of the tangent the levels come from
+        self._tangent = _ident(A)   # identity of the tangent the levels come from
         self._mask = mask           # the cycle reads the mask by pointer
         self._pattern = A.pattern
         self.dropped = int(dropped.value)
@@ -144,7 +201,7 @@ class Multigrid:
         self._stale = False
 
     def is_current(self, A):
-        return self._tangent is A.vals
+        return self._tangent is _ident(A)
 
     def needs_setup(self, A, rebuild, token):
         """rebuild: 'always' (every tangent), 'solve' (once per Newton solve, identified
@@ -200,10 +257,16 @@ class Multigrid:
         ws = _workspace(b.device, int(_lib.load().hf_mg_workspace_size(self.handle)))
         info = _lib.HFKrylovInfo()
         t0 = time.perf_counter()
-        _lib.call("hf_mg_solve_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(b), _lib.ptr(x),
-                  (2 if rel_r0 else 0) | (_lib.HF_MG_SOLVE_EAGER if eager else 0), int(maxiter), float(rtol),
-                  float(atol), int(poll), _lib.ptr(ws),
-                  ctypes.byref(info), _lib.stream())
+        flags = (2 if rel_r0 else 0) | (_lib.HF_MG_SOLVE_EAGER if eager else 0)
+        if isinstance(A, LatticeTangent):
+            A.check_current()
+            _lib.call("hf_mg_solve_stencil_f32", self.handle, _lib.ptr(A.stencil), _lib.ptr(b), _lib.ptr(x), flags,
+                      int(maxiter), float(rtol), float(atol), int(poll), _lib.ptr(ws), ctypes.byref(info),
+                      _lib.stream())
+        else:
+            _lib.call("hf_mg_solve_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(b), _lib.ptr(x),
+                      flags, int(maxiter), float(rtol), float(atol), int(poll), _lib.ptr(ws), ctypes.byref(info),
+                      _lib.stream())
         self.last = (int(info.iterations), int(info.status))
         self.note_iterations(int(info.iterations))
         if info.status != 0:
@@ -211,6 +274,12 @@ class Multigrid:
         if stats is not None:
             stats.add("fcg-mg", info.iterations, info.status, info.res_norm, info.b_norm, time.perf_counter() - t0)
         return x, info
+
+
+def _ident(A):
+    """What identifies a tangent for the rebuild policy: the values tensor of a
+    TangentMatrix, the object itself for a LatticeTangent (its buffer is reused)."""
+    return A.ident if isinstance(A, LatticeTangent) else A.vals
 
 
 _WS = {}
@@ -237,6 +306,8 @@ def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=1, 
                        token=stats if stats is not None else object())
     if info.status in (0, 1):
         return x, info, False
+    if isinstance(A, LatticeTangent):
+        A = A.to_sell()
     b32 = b.to(torch.float32)
     r0 = torch.linalg.norm((b32 - A.matvec(x_start)).double()).item()
     target = max(rtol * (r0 if rel_r0 else info.b_norm), atol)

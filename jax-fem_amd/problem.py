@@ -265,6 +265,54 @@ class Problem:
             _lib.call("hf_bc_rows_f" + tag, p.handle, _lib.ptr(dofs), dofs.numel(), _lib.ptr(vals), s)
         return vals
 
+    def jacobian_stencil(self, u, mg):
+        """FP32 tangent of a HEX8 box lattice in the multigrid's stencil form: the
+        element Jacobians' upper blocks (hf_hex8_jacobian_upper_f32) summed into each
+        node's diagonal and 13 upper node blocks by hf_mg_assemble_hex8_f32, without
+        the SELL matrix.
+        Dirichlet rows stay in place (the multigrid applies the row elimination of
+        apply_bc, solver.py:290-363).  The stencil lives in a buffer of this problem
+        that the next call overwrites; returns (stencil, version)."""
+        pts, cells = self._dev()
+        u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
+        s = _lib.stream()
+        mat = self._hfmat()
+        prm = self._params()
+        ej = self._buf("elem_jac_upper32", self.num_cells * 64 * 9, torch.float32)
+        _lib.call("hf_hex8_jacobian_upper_f32", ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
+        S = self._buf("mg_stencil", mg.stencil_size(), torch.float32)
+        corner = self._box_corners(mg.grid)
+        _lib.call("hf_mg_assemble_hex8_f32", mg.handle, self.pattern.handle, _lib.ptr(cells), self.num_cells,
+                  corner, _lib.ptr(ej), _lib.ptr(S), s)
+        self._stencil_version = getattr(self, "_stencil_version", 0) + 1
+        return S, self._stencil_version
+
+    def _box_corners(self, grid):
+        """(ctypes int32[8]) local index of corner (dx, dy, dz) at 4dx + 2dy + dz when every
+        cell is the lattice cell (i, j, k) = (i*(ny-1) + j)*(nz-1) + k with one corner
+        order (a box_mesh), checked here once; None otherwise."""
+        key = ("box_corners", tuple(grid))
+        if getattr(self, "_corner_key", None) != key:
+            self._corner_key, self._corner = key, None
+            nx, ny, nz = grid
+            cells = np.asarray(self.fe.cells, dtype=np.int64)
+            nc = (nx - 1) * (ny - 1) * (nz - 1)
+            if cells.shape == (nc, 8) and nc > 0:
+                cid = np.arange(nc)
+                cx, cy, cz = cid // ((ny - 1) * (nz - 1)), (cid // (nz - 1)) % (ny - 1), cid % (nz - 1)
+                d = cells - ((cx * ny + cy) * nz + cz)[:, None]
+                step = {bx * ny * nz + by * nz + bz: 4 * bx + 2 * by + bz
+                        for bx in (0, 1) for by in (0, 1) for bz in (0, 1)}
+                if np.all(d == d[0][None, :]) and all(int(v) in step for v in d[0]):
+                    bits = [step[int(v)] for v in d[0]]
+                    if sorted(bits) == list(range(8)):
+                        table = [0] * 8
+                        for b, q in enumerate(bits):
+                            table[q] = b
+                        self._corner = (ctypes.c_int32 * 8)(*table)
+        return self._corner
+
     fused_assembly = True   # HEX8 vec-3 patterns: hf_assemble_hex8 (per-node rows) instead of hf_assemble
 
     def _fused_ok(self, dtype):

@@ -304,6 +304,25 @@ def _flat_initial_guess(problem, cfg):
     return torch.zeros(problem.N, dtype=torch.float64, device=problem.device)
 
 
+def _stencil_multigrid(problem, lin_opts, adt):
+    """The Multigrid whose lattice-stencil form the Newton tangent takes, or None:
+    the 'mg' linear method on the FP32 tangent of a HEX8 vec-3 box lattice with the
+    half-stored fine level ('stencil': False keeps the SELL tangent)."""
+    if "mg" not in lin_opts or adt != torch.float32:
+        return None
+    o = lin_opts["mg"] or {}
+    if not o.get("stencil", True) or o.get("fine_full", False):
+        return None
+    if problem.fe.ele_type != "HEX8" or problem.vec != 3:
+        return None
+    from .multigrid import Multigrid, lattice
+    try:
+        lattice(problem)
+    except NotImplementedError:
+        return None
+    return Multigrid.for_problem(problem, o.get("smoother", "chebyshev"), o.get("degree", 2), o.get("param"), False)
+
+
 def solver(problem, solver_options={}):
     """Newton solve (solver.py:1074-1318); returns sol_list = [u (nnode, vec) f64 device]."""
     method, cfg = _resolve_solver_options(solver_options)
@@ -341,6 +360,9 @@ def solver(problem, solver_options={}):
         timing["local_assembly"] += time.perf_counter() - t0
         return R, res
 
+    smg = _stencil_multigrid(problem, lin_opts, adt)
+    if smg is not None:
+        from .multigrid import LatticeTangent
     R, res = residual(u)
     res0 = res
     hist = [res]
@@ -349,7 +371,10 @@ def solver(problem, solver_options={}):
     while (res / res0 > rel_tol) and (res > tol) and n < max_iter:
         n += 1
         t0 = time.perf_counter()
-        A = TangentMatrix(problem, problem.jacobian_bsr(u, adt))
+        if smg is not None:
+            A = LatticeTangent(problem, u, smg)
+        else:
+            A = TangentMatrix(problem, problem.jacobian_bsr(u, adt))
         torch.cuda.synchronize()
         timing["global_matrix"] += time.perf_counter() - t0
         x0 = _initial_guess(problem, u)

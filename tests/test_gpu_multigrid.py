@@ -255,17 +255,28 @@ def test_mg_fcg_converges(jfa):
     assert info.iterations <= 20 and 8 * info.iterations < jinfo.iterations
 
 
-def test_mg_solve_graph_replays_match_eager(jfa):
+@pytest.mark.parametrize("stencil", [False, True])
+def test_mg_solve_graph_replays_match_eager(jfa, stencil):
     """The captured poll batch gives the eager loop's iterates bit for bit: on
     capture, on replay after unrelated eager work, with a new right-hand side,
-    with another poll length, and after a set-up from another tangent."""
-    say("mg fcg graph replay vs eager")
+    with another poll length, and after a set-up from another tangent (SELL
+    tangent or lattice stencil as the operator)."""
+    say("mg fcg graph replay vs eager", "stencil" if stencil else "sell")
     MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
     prob = box_problem(jfa, (8, 6, 7))
     mask = MG.dirichlet_mask(prob)
     mg = MG.Multigrid(MG.lattice(prob))
     g = torch.Generator(device=DEV).manual_seed(5)
-    A = tangent(jfa, prob, seed=1, scale=0.002)   # small strains: the Neo-Hookean tangent stays SPD
+
+    def tangent(jfa, prob, seed, scale):   # small strains: the Neo-Hookean tangent stays SPD
+        gu = torch.Generator(device=DEV).manual_seed(seed)
+        u = scale * torch.randn(prob.N, generator=gu, device=DEV, dtype=torch.float64)
+        if stencil:
+            return MG.LatticeTangent(prob, u, mg)
+        S = __import__("jax_fem_amd.solver", fromlist=["x"])
+        return S.TangentMatrix(prob, prob.jacobian_bsr(u, torch.float32))
+
+    A = tangent(jfa, prob, seed=1, scale=0.002)
     mg.setup(A, mask)
 
     def both(A, b, poll):
@@ -291,15 +302,88 @@ def test_mg_solve_graph_replays_match_eager(jfa):
     both(A2, b1, 3)
 
 
-def test_newton_with_mg_matches_fp64(jfa):
-    say("mg newton")
+def half_stencil_to_csr(S, dims):
+    """A from a half-stored stencil (14, 3, 3, n): slot k couples node n to its upper
+    neighbour at offset o = 13 + k; the lower blocks are the upper ones transposed."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    idx = np.arange(n)
+    x, y, z = idx // (ny * nz), (idx // nz) % ny, idx % nz
+    rows, cols, vals = [], [], []
+    for k in range(14):
+        o = 13 + k
+        dx, dy, dz = o // 9 - 1, (o // 3) % 3 - 1, o % 3 - 1
+        xx, yy, zz = x + dx, y + dy, z + dz
+        ok = (xx >= 0) & (xx < nx) & (yy >= 0) & (yy < ny) & (zz >= 0) & (zz < nz)
+        assert np.all(S[k][:, :, ~ok] == 0), "slot %d leaves the lattice but is not zero" % o
+        nb = (xx * ny + yy) * nz + zz
+        for i in range(3):
+            for j in range(3):
+                v = S[k, i, j, ok].astype(np.float64)
+                rows.append(3 * idx[ok] + i); cols.append(3 * nb[ok] + j); vals.append(v)
+                if k:
+                    rows.append(3 * nb[ok] + j); cols.append(3 * idx[ok] + i); vals.append(v)
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(3 * n, 3 * n))
+
+
+def test_lattice_stencil_assembly_and_operator(jfa):
+    """The stencil assembled from the element Jacobians holds the SELL tangent's
+    diagonal and upper blocks bit for bit (same cell-order sums, Dirichlet rows
+    kept), and its operator is the row-eliminated tangent."""
+    say("mg lattice stencil assembly")
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    S_ = __import__("jax_fem_amd.solver", fromlist=["x"])
+    cells = (6, 5, 7)
+    prob = box_problem(jfa, cells)
+    mask = MG.dirichlet_mask(prob)
+    mg = MG.Multigrid.for_problem(prob)
+    g = torch.Generator(device=DEV).manual_seed(3)
+    u = 0.002 * torch.randn(prob.N, generator=g, device=DEV, dtype=torch.float64)
+    L = MG.LatticeTangent(prob, u, mg)
+    assert prob._box_corners(mg.grid) is not None   # a box_mesh: the coordinate-driven kernel ran
+    # the kernel that reads the pattern's contribution lists gives the same stencil bit for bit
+    _lib = __import__("jax_fem_amd._lib", fromlist=["x"])
+    S_gen = torch.empty_like(L.stencil)
+    _lib.call("hf_mg_assemble_hex8_f32", mg.handle, prob.pattern.handle, _lib.ptr(prob._dev()[1]), prob.num_cells,
+              None, _lib.ptr(prob._bufs["elem_jac_upper32"]), _lib.ptr(S_gen), _lib.stream())
+    assert torch.equal(S_gen, L.stencil)
+    dims = tuple(c + 1 for c in cells)
+    n = int(np.prod(dims))
+    S = L.stencil.reshape(14, 3, 3, n).cpu().numpy()
+    raw = S_.TangentMatrix(prob, prob.jacobian_bsr(u, torch.float32, bc=False)).to_scipy().tocsr()
+    H = half_stencil_to_csr(S, dims)
+    upper = lambda M: sp.triu(sp.csr_matrix(M), k=0).toarray()
+    # block-upper part: node-pair blocks (n, m) with m >= n in lattice order
+    nodes = np.arange(3 * n) // 3
+    keep = nodes[:, None] <= nodes[None, :]
+    np.testing.assert_array_equal(np.where(keep, H.toarray(), 0), np.where(keep, raw.toarray(), 0))
+    mg.setup(L, mask)
+    A = S_.TangentMatrix(prob, prob.jacobian_bsr(u, torch.float32))
+    x = torch.randn(prob.N, generator=g, device=DEV)
+    x[mask == 0] = torch.randn(int((mask == 0).sum()), generator=g, device=DEV)
+    y_st, y_sell = L.matvec(x), A.matvec(x)
+    err = float((y_st - y_sell).abs().max() / y_sell.abs().max())
+    say("stencil operator vs row-eliminated SELL: max rel", err)
+    assert err <= 1e-5
+    # the hierarchy built from the stencil equals the one built from the SELL tangent
+    mg2 = MG.Multigrid(MG.lattice(prob))
+    mg2.setup(A, mask)
+    for l in range(1, len(mg.levels())):
+        a, b = mg.stencil(l), mg2.stencil(l)
+        np.testing.assert_allclose(a.cpu().numpy(), b.cpu().numpy(), rtol=0,
+                                   atol=1e-5 * float(b.abs().max()), err_msg="level %d" % l)
+
+
+@pytest.mark.parametrize("stencil", [True, False])
+def test_newton_with_mg_matches_fp64(jfa, stencil):
+    say("mg newton", "lattice stencil" if stencil else "SELL tangent")
     cells = (10, 8, 8)
     prob = box_problem(jfa, cells)
     u64 = jfa.solver(prob, {"newton": {"tol": 0.0, "rel_tol": 1e-10,
                                        "linear": {"pcg": {"tol": 1e-12, "atol": 1e-14}}}})[0]
     prob2 = box_problem(jfa, cells)
     u = jfa.solver(prob2, {"newton": {"precision": "mixed", "tol": 0.0, "rel_tol": 1e-10,
-                                      "linear": {"mg": {}}}})[0]
+                                      "linear": {"mg": {"stencil": stencil}}}})[0]
     assert prob2.solve_info["rel"] <= 1e-10
     assert prob2.solve_info["fallbacks"] == 0
     rel = (torch.linalg.norm(u - u64) / torch.linalg.norm(u64)).item()
