@@ -126,3 +126,33 @@ def test_multigrid_lattice_detection(jfa):
 
     with pytest.raises(NotImplementedError):   # vec = 1
         mgm.lattice(Q(jfa.box_mesh(2, 2, 2), vec=1, dim=3))
+
+
+def test_box_corner_table(jfa):
+    """The corner table that lets the stencil assembly take a node's cells from the
+    coordinates: corner (dx, dy, dz) of every cell sits at local index
+    table[4dx + 2dy + dz]; meshes whose cells are not in lattice order get none."""
+    class P(jfa.Problem):
+        def get_tensor_map(self):
+            return jfa.LinearElastic(E=1.0, nu=0.3)
+
+    nxc, nyc, nzc = 3, 2, 4
+    prob = P(jfa.box_mesh(nxc, nyc, nzc), vec=3, dim=3)
+    grid = (nxc + 1, nyc + 1, nzc + 1)
+    table = prob._box_corners(grid)
+    assert table is not None and sorted(table) == list(range(8))
+    cells = np.asarray(prob.fe.cells)
+    ny, nz = grid[1], grid[2]
+    for c in (0, 5, cells.shape[0] - 1):   # brute force: every corner's node id from the lattice
+        i, j, k = c // (nyc * nzc), (c // nzc) % nyc, c % nzc
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    assert cells[c, table[4 * dx + 2 * dy + dz]] == ((i + dx) * ny + (j + dy)) * nz + (k + dz)
+    prob.fe.cells = cells[::-1].copy()      # the same cells, not in lattice order
+    prob._corner_key = None
+    assert prob._box_corners(grid) is None
+    prob.fe.cells = cells[:, [1, 0, 2, 3, 4, 5, 6, 7]].copy()   # one consistent corner order: still a table
+    prob._corner_key = None
+    t2 = prob._box_corners(grid)
+    assert t2 is not None and list(t2) == [1 if t == 0 else 0 if t == 1 else t for t in table]
