@@ -211,14 +211,14 @@ constexpr size_t kHeader = 4096 + (size_t)kMaxGrid * 4 * sizeof(double);
 
 inline int rows_grid(int64_t nslices) {
   int64_t g = ceil_div(nslices, kWavesPerBlock);
+  if (g >= 64) g = ceil_div(g, kNumXcd) * kNumXcd;  // whole XCDs, rounded up: no workgroup gets a second round
   if (g > kMaxGrid) g = kMaxGrid;
-  if (g >= 64) g = (g / kNumXcd) * kNumXcd;
   return (int)(g < 1 ? 1 : g);
 }
 inline int elem_grid(int64_t n) {
   int64_t g = ceil_div(n, kBlock * 4);
+  if (g >= 64) g = ceil_div(g, kNumXcd) * kNumXcd;  // whole XCDs, rounded up: no workgroup gets a second round
   if (g > kMaxGrid) g = kMaxGrid;
-  if (g >= 64) g = (g / kNumXcd) * kNumXcd;
   return (int)(g < 1 ? 1 : g);
 }
 

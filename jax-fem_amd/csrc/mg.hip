@@ -131,8 +131,8 @@ __device__ __forceinline__ void node_range(int64_t n, int64_t& i0, int64_t& i1) 
 }
 inline int node_grid(int64_t n) {
   int64_t g = ceil_div(n, kMgBlock);
+  if (g >= 64) g = ceil_div(g, kNumXcd) * kNumXcd;  // whole XCDs, rounded up: no workgroup gets a second round
   if (g > kMaxGrid) g = kMaxGrid;
-  if (g >= 64) g = (g / kNumXcd) * kNumXcd;
   return (int)(g < 1 ? 1 : g);
 }
 
@@ -1169,8 +1169,8 @@ constexpr int64_t kLaneNodes = 65536;
 
 inline int lane_grid(int64_t n) {
   int64_t g = ceil_div(n * kGroup, kMgBlock);
+  if (g >= 64) g = ceil_div(g, kNumXcd) * kNumXcd;  // whole XCDs, rounded up: no workgroup gets a second round
   if (g > kMaxGrid) g = kMaxGrid;
-  if (g >= 64) g = (g / kNumXcd) * kNumXcd;
   return (int)(g < 1 ? 1 : g);
 }
 
