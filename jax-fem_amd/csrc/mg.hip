@@ -444,7 +444,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restric
 // blocks.  The intermediate lattices hold all 27 slots.
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_rap_dim(const float* __restrict__ S, Lat f, Lat c, int dim,
-                                                       const float* __restrict__ mask, float* __restrict__ Out) {
+                                                       const float* __restrict__ mask, float* __restrict__ Out,
+                                                       bool out_half) {
   const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
   if (n >= c.n) return;
   const int q = blockIdx.y;
@@ -507,8 +508,10 @@ __global__ void __launch_bounds__(kMgBlock) mg_rap_dim(const float* __restrict__
   for (int k = 0; k < 3; ++k) {
     off[dim] = k;
     const int O = off[0] * 9 + off[1] * 3 + off[2];
+    if (out_half && O < 13) continue;  // a half-stored level keeps the diagonal and upper slots
+    const int so = out_half ? O - 13 : O;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) Out[(int64_t)(O * 9 + e) * c.n + n] = acc[k][e];
+    for (int e = 0; e < 9; ++e) Out[(int64_t)(so * 9 + e) * c.n + n] = acc[k][e];
   }
 }
 
@@ -1686,7 +1689,8 @@ using namespace hf;
 
 struct MgLevel {
   Lat g;
-  float* S = nullptr;     // 243 n (the fine level of a symmetric operator: 126 n, half-stored)
+  bool half = false;      // stencil half-stored (diagonal + 13 upper slots: 126 n floats)
+  float* S = nullptr;     // 243 n, or 126 n when half
   float* Dinv = nullptr;  // 9 n
   float* r = nullptr;     // 3 n (levels >= 1: the restricted residual)
   float* e = nullptr;     // 3 n
@@ -1727,6 +1731,16 @@ struct hf_mg {
   hipGraphExec_t gexec = nullptr;
   uintptr_t gkey[6] = {0, 0, 0, 0, 0, 0};
 };
+
+// First level run inside mg_small_cycle (the levels from it down), or nlev - 1
+// (only the dense solve) when no level is small enough.
+static int small_cycle_start(const hf_mg* mg) {
+  const int L = mg->nlev;
+  if (L == 1) return 0;
+  for (int l = 1; l < L; ++l)
+    if (mg->lv[l].g.n <= kSmallNodes && L - l <= kSmallMax) return l;
+  return L - 1;
+}
 
 static int mg_free(hf_mg* mg) {
   for (int l = 0; l < mg->nlev; ++l) {
@@ -1776,6 +1790,13 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
     HF_REQUIRE(false, "hf_mg_create: a SELL fine level needs at least one coarser level");
   }
   for (int l = 0; l < nlev; ++l) mg->lv[l].g = make_lat(dims[l][0], dims[l][1], dims[l][2]);
+  // Coarse levels run by grid kernels are half-stored too (Galerkin products of a
+  // symmetric operator); the one-workgroup cycle's levels and the dense one keep
+  // 27 slots.  HF_MG_FINE_FULL (an operator that need not be symmetric) keeps all.
+  mg->lv[0].half = mg->half;
+  const int lb = small_cycle_start(mg);
+  // (a SELL fine level's level 1 comes from mg_rap_ptap, which writes 27 slots)
+  for (int l = mg->sell ? 2 : 1; l < lb; ++l) mg->lv[l].half = !(flags & HF_MG_FINE_FULL);
   auto alloc = [&](float** p, int64_t n) -> int {
     HF_CHECK_HIP(hipMalloc((void**)p, (size_t)n * sizeof(float)));
     return 0;
@@ -1784,7 +1805,7 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
   for (int l = 0; l < mg->nlev && !rc; ++l) {
     MgLevel& L = mg->lv[l];
     const int64_t n = L.g.n;
-    if (l == 0 && mg->half) {
+    if (L.half) {
       rc |= alloc(&L.S, (int64_t)kHalfSlots * 9 * n);
     } else if (l == 0 && mg->sell) {
       rc |= alloc(&mg->y, 3 * n);
@@ -1941,13 +1962,15 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
       float* T1 = mg->B;
       float* T2 = mg->B + (int64_t)kSt * bx.n;
       mg_rap_dim<H><<<dim3((unsigned)ceil_div(bx.n, kMgBlock), 9), kMgBlock, 0, s>>>(Sp, f, bx, 0,
-                                                                                   l == 0 ? mask : nullptr, T1);
+                                                                                   l == 0 ? mask : nullptr, T1,
+                                                                                   false);
       mg_rap_dim<false><<<dim3((unsigned)ceil_div(bxy.n, kMgBlock), 9), kMgBlock, 0, s>>>(T1, bx, bxy, 1, nullptr,
-                                                                                         T2);
+                                                                                         T2, false);
       mg_rap_dim<false><<<dim3((unsigned)ceil_div(c.n, kMgBlock), 9), kMgBlock, 0, s>>>(T2, bxy, c, 2, nullptr,
-                                                                                      mg->lv[l + 1].S);
+                                                                                      mg->lv[l + 1].S,
+                                                                                      mg->lv[l + 1].half);
     };
-    if (l == 0 && mg->half) level(std::true_type{});
+    if (mg->lv[l].half) level(std::true_type{});
     else level(std::false_type{});
     mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gg, lam_dev + l);
   }
@@ -2055,10 +2078,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     cur[l] = l == 0 ? z0 : mg->lv[l].e;
     oth[l] = l == 0 ? z1 : mg->lv[l].e2;
   }
-  int lb = L - 1;
-  for (int l = 1; l < L; ++l)
-    if (mg->lv[l].g.n <= kSmallNodes && L - l <= kSmallMax) { lb = l; break; }
-  if (L == 1) lb = 0;
+  const int lb = small_cycle_start(mg);
   auto rin = [&](int l) -> const float* { return l == 0 ? r0 : mg->lv[l].r; };
   for (int l = 0; l < lb; ++l) {
     const MgLevel& V = mg->lv[l];
@@ -2102,7 +2122,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       }
       mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
       mg_sell_resid_update<<<gg, kMgBlock, 0, s>>>(V.g.n, res, mg->y, dout, sc);
-    } else if (l == 0 && mg->half) {
+    } else if (V.half) {
       down(std::true_type{});
     } else {
       down(std::false_type{});
@@ -2188,7 +2208,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
                                                     cur[l], sc);
         float* t2 = din; din = dout; dout = t2;
       }
-    } else if (l == 0 && mg->half) {
+    } else if (V.half) {
       up(std::true_type{});
     } else {
       up(std::false_type{});
@@ -2218,10 +2238,10 @@ extern "C" int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf
   HF_REQUIRE(mg->ready, "hf_mg_copy_stencil: hf_mg_setup_f32 has not run");
   HF_REQUIRE(!(level == 0 && mg->sell), "hf_mg_copy_stencil: the fine level is the tangent itself (SELL)");
   const int64_t nv = (int64_t)kSt * mg->lv[level].g.n;
-  if (level == 0 && mg->half) {
-    const Lat g0 = mg->lv[0].g;
-    mg_expand_half<<<dim3((unsigned)ceil_div(g0.n, kMgBlock), 27), kMgBlock, 0, (hipStream_t)stream>>>(mg->lv[0].S, g0,
-                                                                                                     dst);
+  if (mg->lv[level].half) {
+    const Lat gl = mg->lv[level].g;
+    mg_expand_half<<<dim3((unsigned)ceil_div(gl.n, kMgBlock), 27), kMgBlock, 0, (hipStream_t)stream>>>(
+        mg->lv[level].S, gl, dst);
     HF_LAUNCH_CHECK();
   } else {
     HF_CHECK_HIP(hipMemcpyAsync(dst, mg->lv[level].S, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice,
