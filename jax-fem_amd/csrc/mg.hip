@@ -76,6 +76,14 @@ __device__ __forceinline__ void lat_xyz(const Lat& g, int64_t n, int& x, int& y,
   y = (int)(q % g.ny);
   x = (int)(q / g.ny);
 }
+// the same for node ids below 2^31, with 32-bit divisions (the 64-bit ones are
+// long software sequences; kernels that decode a node per matrix block use this)
+__device__ __forceinline__ void lat_xyz32(const Lat& g, uint32_t n, int& x, int& y, int& z) {
+  const uint32_t q = n / (uint32_t)g.nz;
+  z = (int)(n - q * (uint32_t)g.nz);
+  y = (int)(q % (uint32_t)g.ny);
+  x = (int)(q / (uint32_t)g.ny);
+}
 __device__ __forceinline__ int64_t lat_id(const Lat& g, int x, int y, int z) {
   return ((int64_t)x * g.ny + y) * g.nz + z;
 }
@@ -1293,123 +1301,140 @@ __global__ void __launch_bounds__(kMgBlock) mg_sell_dinv(const int32_t* __restri
 }
 
 // Gershgorin bound of Dinv (M A M) over the SELL rows (per-workgroup maxima, as mg_gersh)
-__global__ void __launch_bounds__(kMgBlock) mg_sell_gersh(const int32_t* __restrict__ rowptr,
-                                                          const int32_t* __restrict__ colidx,
-                                                          const int32_t* __restrict__ blkpos,
-                                                          const float* __restrict__ vals,
-                                                          const float* __restrict__ mask,
-                                                          const float* __restrict__ Dinv, int64_t n,
-                                                          float* __restrict__ part) {
-  __shared__ float sm[kMgBlock / kWave];
+// A wavefront per slice and a lane per (row, column component) as in the Krylov
+// sweep: lane (r, i) reads column i of its row's blocks (3 contiguous floats,
+// coalesced across the slice), masks it by the column DOF and adds
+// |Dinv col| into the row's three sums (Dinv is zero on masked rows); the row's
+// three lanes are summed through LDS.
+__global__ void __launch_bounds__(kBlock) mg_sell_gersh(MatArgs m, const float* __restrict__ vals,
+                                                        const float* __restrict__ mask,
+                                                        const float* __restrict__ Dinv, int64_t n,
+                                                        float* __restrict__ part) {
+  using Sh = SellShape<3>;
+  __shared__ float red[kWavesPerBlock][kWave * 3];
+  __shared__ float sm[kWavesPerBlock];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int r = lane / 3, i = lane - (lane / 3) * 3;
+  const bool in_slice = lane < Sh::L;
+  int64_t s0, s1;
+  row_range(m.nslices, s0, s1);
   float best = 0.f;
-  for (int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x; node < n; node += (int64_t)gridDim.x * kMgBlock) {
-    float M[9];
-    const float mr[3] = {mask[3 * node], mask[3 * node + 1], mask[3 * node + 2]};
-#pragma unroll
-    for (int e = 0; e < 9; ++e) M[e] = Dinv[(int64_t)e * n + node];
+  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    const int32_t node = in_slice ? m.sell_row[s * Sh::R + r] : -1;
     float rs[3] = {0.f, 0.f, 0.f};
-    for (int32_t b = rowptr[node]; b < rowptr[node + 1]; ++b) {
-      const int64_t c = colidx[b];
-      const float* v = vals + blkpos[b];
-      const float mc[3] = {mask[3 * c], mask[3 * c + 1], mask[3 * c + 2]};
-      float B[9];
+    if (node >= 0) {
+      float M[9];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+      for (int e = 0; e < 9; ++e) M[e] = Dinv[(int64_t)e * n + node];
+      const int32_t t0 = m.slice_ptr[s], t1 = m.slice_ptr[s + 1];
+#pragma unroll 4
+      for (int32_t t = t0; t < t1; ++t) {
+        const int64_t col = m.sell_col[(int64_t)t * Sh::R + r];
+        const float mc = mask[3 * col + i];
+        const float* vc = vals + ((int64_t)t * Sh::L + lane) * 3;  // entries (k, i), k = 0..2
+        const float b0 = vc[0] * mc, b1 = vc[1] * mc, b2 = vc[2] * mc;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) B[3 * i + j] = v[3 * j + i] * mr[i] * mc[j];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          rs[i] += fabsf(M[3 * i] * B[j] + M[3 * i + 1] * B[3 + j] + M[3 * i + 2] * B[6 + j]);
+        for (int p = 0; p < 3; ++p) rs[p] += fabsf(M[3 * p] * b0 + M[3 * p + 1] * b1 + M[3 * p + 2] * b2);
+      }
     }
-    best = fmaxf(best, fmaxf(rs[0], fmaxf(rs[1], rs[2])));
+#pragma unroll
+    for (int p = 0; p < 3; ++p) red[w][lane * 3 + p] = rs[p];
+    wave_lds_sync();
+    if (node >= 0 && i == 0) {
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const float tot = red[w][(r * 3) * 3 + p] + red[w][(r * 3 + 1) * 3 + p] + red[w][(r * 3 + 2) * 3 + p];
+        best = fmaxf(best, tot);
+      }
+    }
+    wave_lds_sync();  // the next slice rewrites red
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o, 64));
-  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = best;
+  if (lane == 0) sm[w] = best;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float m = sm[0];
-    for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
-    part[blockIdx.x] = m;
+    float mx = sm[0];
+    for (int k = 1; k < kWavesPerBlock; ++k) mx = fmaxf(mx, sm[k]);
+    part[blockIdx.x] = mx;
   }
 }
 
 // Stage 1 of the Galerkin product from the SELL rows: B = (M A M) P in 27
 // coarse-column slots per fine row (slot s: coarse column floor(i/2) + s - 1 per
-// dimension; mg_rap_ptap forms P^T B).  A thread per fine row accumulates its slots
-// in LDS (64 rows per workgroup, 243 floats each); a block whose coarse columns
-// fall outside the slots sets bad[0] (the pattern is not supported).
-constexpr int kSellRapRows = 64;
-__global__ void __launch_bounds__(kSellRapRows) mg_sell_rap_ap(const int32_t* __restrict__ rowptr,
-                                                               const int32_t* __restrict__ colidx,
-                                                               const int32_t* __restrict__ blkpos,
-                                                               const float* __restrict__ vals,
-                                                               const float* __restrict__ mask, Lat f, Lat c,
-                                                               float* __restrict__ B, int32_t* __restrict__ bad) {
-  __shared__ float acc[kSellRapRows * kSt];
-  const int64_t i = (int64_t)blockIdx.x * kSellRapRows + threadIdx.x;
-  float* my = acc + threadIdx.x * kSt;
-  for (int e = 0; e < kSt; ++e) my[e] = 0.f;
-  if (i < f.n) {
-    int fi[3];
-    lat_xyz(f, i, fi[0], fi[1], fi[2]);
-    const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
-    const int nc[3] = {c.nx - 1, c.ny - 1, c.nz - 1};
-    const float mr[3] = {mask[3 * i], mask[3 * i + 1], mask[3 * i + 2]};
-    bool broken = false;
-    for (int32_t b = rowptr[i]; b < rowptr[i + 1]; ++b) {
-      const int64_t col = colidx[b];
-      int fj[3];
-      lat_xyz(f, col, fj[0], fj[1], fj[2]);
-      const float* v = vals + blkpos[b];
-      const float mc[3] = {mask[3 * col], mask[3 * col + 1], mask[3 * col + 2]};
-      float blk[9];
+// dimension; mg_rap_ptap forms P^T B).  A wavefront per slice and a lane per
+// (row, column component i), as in the Krylov sweep: the lane adds column i of
+// each of its row's blocks (coalesced across the slice), weighted by the coarse
+// parents of the block's column node, into its slot accumulators in LDS.  The
+// slice's three wavefronts take the coarse x-planes sx = 0, 1, 2 of the 27 slots
+// (9 slots x 3 entries per lane), reading the same blocks, so that the
+// accumulators leave room for 8 waves per SIMD.  A parent outside the 27 slots
+// sets bad[0] (the pattern is not supported).
+__global__ void __launch_bounds__(3 * kWave) mg_sell_rap_ap(MatArgs m, const float* __restrict__ vals,
+                                                            const float* __restrict__ mask, Lat f, Lat c,
+                                                            float* __restrict__ B, int32_t* __restrict__ bad) {
+  using Sh = SellShape<3>;
+  __shared__ float acc[3 * kWave * 27];
+  const int lane = threadIdx.x & (kWave - 1), plane = threadIdx.x / kWave;
+  const int r = lane / 3, i = lane - (lane / 3) * 3;
+  const int64_t sl = blockIdx.x;
+  float* my = acc + (plane * kWave + lane) * 27;  // [slot in plane][k]: entry (k, i)
+  for (int e = 0; e < 27; ++e) my[e] = 0.f;
+  const int32_t node = lane < Sh::L ? m.sell_row[sl * Sh::R + r] : -1;
+  if (node < 0) return;
+  int fi[3];
+  lat_xyz(f, node, fi[0], fi[1], fi[2]);
+  const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
+  const int nc[3] = {c.nx - 1, c.ny - 1, c.nz - 1};
+  const float mr[3] = {mask[3 * (int64_t)node], mask[3 * (int64_t)node + 1], mask[3 * (int64_t)node + 2]};
+  bool broken = false;
+  const int32_t t0 = m.slice_ptr[sl], t1 = m.slice_ptr[sl + 1];
+#pragma unroll 4
+  for (int32_t t = t0; t < t1; ++t) {
+    const int64_t col = m.sell_col[(int64_t)t * Sh::R + r];
+    int fj[3];
+    lat_xyz32(f, (uint32_t)col, fj[0], fj[1], fj[2]);
+    const float mc = mask[3 * col + i];
+    const float* vc = vals + ((int64_t)t * Sh::L + lane) * 3;  // entries (k, i), k = 0..2
+    const float b[3] = {vc[0] * mr[0] * mc, vc[1] * mr[1] * mc, vc[2] * mr[2] * mc};
+    // coarse columns of the fine column j: per dimension the parent(s) of j
+    int ps[3][2];
+    float pw[3][2];
 #pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) blk[3 * a + d] = v[3 * d + a] * mr[a] * mc[d];
-      // coarse columns J of the fine column j: per dimension the parent(s) of j
-      int pj[3][2], ps[3][2];
-      float pw[3][2];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const int v2 = fj[d];
-        int c0, c1w;
-        float w0, w1;
-        if (v2 % 2 == 0) { c0 = v2 / 2; w0 = 1.f; c1w = c0; w1 = 0.f; }
-        else if (v2 == nf[d]) { c0 = nc[d]; w0 = 1.f; c1w = c0; w1 = 0.f; }
-        else { c0 = (v2 - 1) / 2; w0 = 0.5f; c1w = (v2 + 1) / 2; w1 = 0.5f; }
-        pj[d][0] = c0; pj[d][1] = c1w;
-        pw[d][0] = w0; pw[d][1] = w1;
-        ps[d][0] = c0 - fi[d] / 2 + 1;
-        ps[d][1] = c1w - fi[d] / 2 + 1;
-      }
-#pragma unroll
-      for (int ax = 0; ax < 2; ++ax)
-#pragma unroll
-        for (int ay = 0; ay < 2; ++ay)
-#pragma unroll
-          for (int az = 0; az < 2; ++az) {
-            const float w = pw[0][ax] * pw[1][ay] * pw[2][az];
-            if (w == 0.f) continue;
-            const int sx = ps[0][ax], sy = ps[1][ay], sz = ps[2][az];
-            if (sx < 0 || sx > 2 || sy < 0 || sy > 2 || sz < 0 || sz > 2) {
-              broken = true;
-              continue;
-            }
-            float* dst = my + (sx * 9 + sy * 3 + sz) * 9;
-#pragma unroll
-            for (int e = 0; e < 9; ++e) dst[e] += w * blk[e];
-          }
-      (void)pj;
+    for (int d = 0; d < 3; ++d) {
+      const int v2 = fj[d];
+      int c0, c1w;
+      float w0, w1;
+      if (v2 % 2 == 0) { c0 = v2 / 2; w0 = 1.f; c1w = c0; w1 = 0.f; }
+      else if (v2 == nf[d]) { c0 = nc[d]; w0 = 1.f; c1w = c0; w1 = 0.f; }
+      else { c0 = (v2 - 1) / 2; w0 = 0.5f; c1w = (v2 + 1) / 2; w1 = 0.5f; }
+      pw[d][0] = w0; pw[d][1] = w1;
+      ps[d][0] = c0 - fi[d] / 2 + 1;
+      ps[d][1] = c1w - fi[d] / 2 + 1;
     }
-    if (broken) bad[0] = 1;
-    for (int sl = 0; sl < 27; ++sl)
 #pragma unroll
-      for (int e = 0; e < 9; ++e) B[(int64_t)(sl * 9 + e) * f.n + i] = my[sl * 9 + e];
+    for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+      for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+        for (int az = 0; az < 2; ++az) {
+          const float w = pw[0][ax] * pw[1][ay] * pw[2][az];
+          if (w == 0.f) continue;
+          const int sx = ps[0][ax], sy = ps[1][ay], sz = ps[2][az];
+          if (sx < 0 || sx > 2 || sy < 0 || sy > 2 || sz < 0 || sz > 2) {
+            broken = true;
+            continue;
+          }
+          if (sx != plane) continue;
+          float* dst = my + (sy * 3 + sz) * 3;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) dst[k] += w * b[k];
+        }
   }
+  if (broken) bad[0] = 1;
+  for (int q = 0; q < 9; ++q)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) B[(int64_t)((plane * 9 + q) * 9 + 3 * k + i) * f.n + node] = my[q * 3 + k];
 }
 
 // y = A v over the SELL rows (the Krylov sweep), node-major output
@@ -1920,13 +1945,12 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   if (mg->sell) {
     // fine level = the SELL tangent: block inverses, bound and A P straight from its rows
     const Lat c = mg->lv[1].g;
-    const int gg = (int)std::min<int64_t>(ceil_div(g0.n, kMgBlock), kMaxGrid);
     mg_sell_dinv<<<gb0, kMgBlock, 0, s>>>(pattern_dpos(p), vals, mask, g0.n, mg->lv[0].Dinv);
-    mg_sell_gersh<<<gg, kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p), vals, mask,
-                                          mg->lv[0].Dinv, g0.n, mg->part);
-    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gg, lam_dev0);
-    mg_sell_rap_ap<<<(unsigned)ceil_div(g0.n, kSellRapRows), kSellRapRows, 0, s>>>(
-        pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p), vals, mask, g0, c, mg->B, mg->flags);
+    const MatArgs ma = mat_args(p);
+    const int gs = rows_grid(ma.nslices);
+    mg_sell_gersh<<<gs, kBlock, 0, s>>>(ma, vals, mask, mg->lv[0].Dinv, g0.n, mg->part);
+    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gs, lam_dev0);
+    mg_sell_rap_ap<<<(unsigned)ma.nslices, 3 * kWave, 0, s>>>(ma, vals, mask, g0, c, mg->B, mg->flags);
     mg_rap_ptap<<<dim3((unsigned)ceil_div(c.n, kMgBlock), 27), kMgBlock, 0, s>>>(mg->B, g0, c, mg->lv[1].S);
     mg->sp = p;
     mg->svals = vals;
