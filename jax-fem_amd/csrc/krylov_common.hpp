@@ -34,6 +34,7 @@ struct Scalars {
   double rho, rho_, alpha, omega, rr, bb, thresh, pq, ss, ts, tt;
   int32_t iter, done, status, exit_early;
   int32_t maxiter, pad;
+  double pqs[2];  // the multigrid FCG's p.q, by iteration parity
 };
 
 // Logical workgroup id, remapped so that each XCD owns a contiguous range.

@@ -973,20 +973,39 @@ __global__ void __launch_bounds__(kMgBlock) mg_max_final(const float* __restrict
 }
 
 // coarsest level: e = Ainv r (FP64 accumulation), one workgroup
-constexpr int kDenseApplyThreads = 128;
+// e = Ainv r for the dense coarsest level (Ainv column-major FP64, nd <= kDenseMax)
+// by one workgroup of T threads: thread t takes row t % nd and every ng-th
+// column from t / nd (ng = T / nd groups), so its few loads are in flight
+// together instead of nd dependent ones; the groups' partial sums meet in LDS
+// and are added in a fixed order.
+template <int T>
+__device__ __forceinline__ void dense_apply_wg(const double* __restrict__ Ainv, int nd, const float* __restrict__ r,
+                                               float* __restrict__ e) {
+  static_assert(T >= kDenseMax, "one row per thread at least");
+  __shared__ double rs[kDenseMax];
+  __shared__ double part[T];
+  const int tid = threadIdx.x;
+  if (tid < nd) rs[tid] = (double)r[tid];
+  __syncthreads();
+  const int ng = T / nd, i = tid % nd, jg = tid / nd;
+  double acc = 0.0;
+  if (jg < ng)
+    for (int j = jg; j < nd; j += ng) acc += Ainv[(int64_t)j * nd + i] * rs[j];
+  part[tid] = acc;
+  __syncthreads();
+  if (tid < nd) {
+    double v = 0.0;
+    for (int k = 0; k < ng; ++k) v += part[k * nd + tid];
+    e[tid] = (float)v;
+  }
+}
+
+constexpr int kDenseApplyThreads = 1024;
 __global__ void __launch_bounds__(kDenseApplyThreads) mg_dense_apply(const double* __restrict__ Ainv, int nd,
                                                                      const float* __restrict__ r,
                                                                      float* __restrict__ e, const Scalars* sc) {
   if (sc && sc->done) return;
-  __shared__ double rs[kDenseMax];
-  const int tid = threadIdx.x;
-  if (tid < nd) rs[tid] = (double)r[tid];
-  __syncthreads();
-  if (tid < nd) {
-    double acc = 0.0;
-    for (int j = 0; j < nd; ++j) acc += Ainv[(int64_t)j * nd + tid] * rs[j];
-    e[tid] = (float)acc;
-  }
+  dense_apply_wg<kDenseApplyThreads>(Ainv, nd, r, e);
 }
 
 // The levels from the first one with at most kSmallNodes nodes down, in one
@@ -998,7 +1017,11 @@ __global__ void __launch_bounds__(kDenseApplyThreads) mg_dense_apply(const doubl
 // post-smoother start swaps e and e2 once).
 constexpr int kSmallThreads = 1024;
 constexpr int kGroup = 8;                             // lanes per node
-constexpr int kSmallNodes = kSmallThreads / kGroup;   // 128
+constexpr int kSmallNodes = kSmallThreads / kGroup;   // 128 nodes a round
+#ifndef HF_SMALL_START
+#define HF_SMALL_START 128
+#endif
+constexpr int kSmallStart = HF_SMALL_START;
 constexpr int kSmallMax = 4;
 constexpr int kSmallUnroll = 1;  // 2 fits in 100 VGPRs but measured slower here (37 -> 42 us)
 struct SmallLevel {
@@ -1103,19 +1126,8 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     const bool smooth_c = (l + 2 < L);
     MG_WG_LOOP(C.g.n, restrict_node(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.s0, C.da, cur[l + 1], node));
   }
-  {
-    __shared__ double rs[kDenseMax];
-    const SmallLevel& B = sc_.lv[L - 1];
-    const int tid = threadIdx.x;
-    if (tid < sc_.nd) rs[tid] = (double)B.r[tid];
-    __syncthreads();
-    if (tid < sc_.nd) {
-      double acc = 0.0;
-      for (int j = 0; j < sc_.nd; ++j) acc += sc_.Ainv[(int64_t)j * sc_.nd + tid] * rs[j];
-      cur[L - 1][tid] = (float)acc;
-    }
-    __syncthreads();
-  }
+  dense_apply_wg<kSmallThreads>(sc_.Ainv, sc_.nd, sc_.lv[L - 1].r, cur[L - 1]);
+  __syncthreads();
   for (int l = L - 2; l >= 0; --l) {
     const SmallLevel& V = sc_.lv[l];
     const SmallLevel& C = sc_.lv[l + 1];
@@ -1510,6 +1522,33 @@ constexpr int kPartB = 2 * kMaxGrid;
 constexpr size_t kMgHeader = 4096 + (size_t)5 * kMaxGrid * sizeof(double);
 
 // r = b - A x, p = q = 0; partials (A) r.r, b.b
+// Sum of nparts partials in a fixed order by one kMgBlock workgroup; every
+// thread receives the totals, and every workgroup that runs it gets the same
+// bits (same data, same order).
+template <int NV>
+__device__ __forceinline__ void sum_partials_wg(const double* __restrict__ partials, int nparts, double (&tot)[NV]) {
+  __shared__ double sm[kMgBlock / kWave][NV];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  double acc[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) acc[j] = 0;
+  for (int b = threadIdx.x; b < nparts; b += kMgBlock)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] += partials[(int64_t)b * NV + j];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const double v = wave_sum(acc[j]);
+    if (lane == 0) sm[w][j] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double v = 0;
+    for (int k = 0; k < kMgBlock / kWave; ++k) v += sm[k][j];
+    tot[j] = v;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) mgcg_init_kernel(MatArgs m, const float* __restrict__ vals,
                                                            const float* __restrict__ b, const float* __restrict__ x,
                                                            float* __restrict__ r, float* __restrict__ p,
@@ -1645,51 +1684,73 @@ __global__ void __launch_bounds__(kMgBlock) mg_stencil_matvec(const float* __res
   });
 }
 
-// beta, p.q and alpha of the new direction
-__global__ void __launch_bounds__(kScalarBlock) mgcg_dir_scalars(Scalars* sc, const double* partB, int nB) {
+// One outer step, node by node: every workgroup first sums the product's
+// partials (B: gamma = r.z, delta = z.w, zeta = z.q) in the same fixed order
+// and forms beta = -zeta / p.q_old, p.q = delta + 2 beta zeta + beta^2 p.q_old,
+// alpha = gamma / p.q (beta = 0 on the first step); workgroup 0 records them.
+// p.q alternates between two slots by iteration parity, so no workgroup reads
+// a value another one has already replaced.  Then p = z + beta p, q = w + beta q,
+// x += alpha p, r -= alpha q, the fine level's pre-smoother start from e = 0 on
+// the new residual (d = e = s0 Dinv r; Dinv null: a one-level hierarchy, whose
+// cycle is the dense solve of r), and partials (A) r.r for the convergence test.
+__global__ void __launch_bounds__(kMgBlock) mgcg_update_first(int64_t n, const float* __restrict__ z,
+                                                              const float* __restrict__ w, float* __restrict__ p,
+                                                              float* __restrict__ q, float* __restrict__ x,
+                                                              float* __restrict__ r, const float* __restrict__ Dinv,
+                                                              float s0, float* __restrict__ d0,
+                                                              float* __restrict__ e0, Scalars* sc,
+                                                              const double* __restrict__ partB, int nB,
+                                                              double* partials) {
   if (sc->done) return;
-  double d[3];
-  sum_partials<3>(partB, nB, d);
-  if (threadIdx.x != 0) return;
-  const double gamma = d[0], delta = d[1], zeta = d[2];
+  double dv[3];
+  sum_partials_wg<3>(partB, nB, dv);
+  const double gamma = dv[0], delta = dv[1], zeta = dv[2];
+  const int it = sc->iter;
   double beta = 0.0, pq = delta;
-  if (sc->iter > 0) {
-    beta = -zeta / sc->pq;
-    pq = delta + 2.0 * beta * zeta + beta * beta * sc->pq;
+  if (it > 0) {
+    const double pq_old = sc->pqs[it & 1];
+    beta = -zeta / pq_old;
+    pq = delta + 2.0 * beta * zeta + beta * beta * pq_old;
   }
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   if (!(pq > 0.0) || !isfinite(pq) || !isfinite(gamma)) {
-    sc->done = 1;
-    sc->status = (isfinite(pq) && isfinite(gamma)) ? 2 : 3;
+    if (first) {
+      sc->status = (isfinite(pq) && isfinite(gamma)) ? 2 : 3;
+      sc->done = 1;
+    }
     return;
   }
-  sc->omega = beta;
-  sc->pq = pq;
-  sc->rho = gamma;
-  sc->alpha = gamma / pq;
-}
-
-// p = z + beta p, q = w + beta q, x += alpha p, r -= alpha q; partial (A) r.r
-__global__ void __launch_bounds__(kBlock) mgcg_update_kernel(int64_t N, const float* __restrict__ z,
-                                                             const float* __restrict__ w, float* __restrict__ p,
-                                                             float* __restrict__ q, float* __restrict__ x,
-                                                             float* __restrict__ r, const Scalars* sc,
-                                                             double* partials) {
-  if (sc->done) return;
-  const float alpha = (float)sc->alpha, beta = (float)sc->omega;
-  const int64_t per = ceil_div(N, gridDim.x);
-  const int64_t i0 = (int64_t)logical_block() * per;
-  const int64_t i1 = i0 + per < N ? i0 + per : N;
-  double loc[1] = {0};
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
-    const float pp = z[i] + beta * p[i];
-    const float qq = w[i] + beta * q[i];
-    p[i] = pp;
-    q[i] = qq;
-    x[i] += alpha * pp;
-    const float rr = r[i] - alpha * qq;
-    r[i] = rr;
-    loc[0] += (double)rr * (double)rr;
+  const double alpha_d = gamma / pq;
+  if (first) {
+    sc->pqs[(it + 1) & 1] = pq;
+    sc->omega = beta;
+    sc->rho = gamma;
+    sc->alpha = alpha_d;
   }
+  const float alpha = (float)alpha_d, betaf = (float)beta;
+  double loc[1] = {0};
+  MG_NODE_LOOP(n, {
+    float rn[3];
+    for (int i = 0; i < 3; ++i) {
+      const int64_t k = 3 * node + i;
+      const float pp = z[k] + betaf * p[k];
+      const float qq = w[k] + betaf * q[k];
+      p[k] = pp;
+      q[k] = qq;
+      x[k] += alpha * pp;
+      rn[i] = r[k] - alpha * qq;
+      r[k] = rn[i];
+      loc[0] += (double)rn[i] * (double)rn[i];
+    }
+    if (Dinv) {
+      float o[3];
+      block_apply(Dinv, n, node, rn, o);
+      for (int i = 0; i < 3; ++i) {
+        d0[3 * node + i] = s0 * o[i];
+        e0[3 * node + i] = s0 * o[i];
+      }
+    }
+  });
   block_partials<1>(loc, partials);
 }
 
@@ -1763,7 +1824,7 @@ static int small_cycle_start(const hf_mg* mg) {
   const int L = mg->nlev;
   if (L == 1) return 0;
   for (int l = 1; l < L; ++l)
-    if (mg->lv[l].g.n <= kSmallNodes && L - l <= kSmallMax) return l;
+    if (mg->lv[l].g.n <= kSmallStart && L - l <= kSmallMax) return l;
   return L - 1;
 }
 
@@ -2092,8 +2153,10 @@ extern "C" int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels) {
 // timed: the level-0 post-smoother start (one pass over the fine stencil) is
 // launched between the two profiling events (HIP-event timing for roofline
 // reporting, see hf_profile_enable).
+// first_done: the fine level's pre-smoother start (d = e = s0 Dinv r) was
+// written by the caller's kernel (the outer iteration's update)
 static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out, const Scalars* sc, hipStream_t s,
-                  bool timed = false, bool skip_fine_post = false) {
+                  bool timed = false, bool skip_fine_post = false, bool first_done = false) {
   const int L = mg->nlev;
   const int k = mg->degree;
   float* cur[kMgMaxLevels];
@@ -2109,7 +2172,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const int gg = node_grid(V.g.n);
     // pre-smoothing from e = 0 (its start on levels >= 1 is fused into the
     // restriction that produced rin(l)), then the residual for the coarse level
-    if (l == 0) mg_poly_first<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.s0, rin(l), V.da, cur[l], sc);
+    if (l == 0 && !first_done) mg_poly_first<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.s0, rin(l), V.da, cur[l], sc);
     const float* res = rin(l);
     float* din = V.da;
     float* dout = V.db;
@@ -2295,17 +2358,20 @@ struct MgBatch {
   int gr, ge;
 };
 static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int timed_k) {
+  const MgLevel& F = mg->lv[0];
+  // with one level the V-cycle is the dense solve of r: no smoother start to fuse
+  const bool fuse = mg->nlev > 1;
   for (int k = 0; k < poll; ++k) {
-    mgcg_update_kernel<<<B.ge, kBlock, 0, s>>>(B.N, B.z, B.w, B.pv, B.q, B.x, B.r, B.sc, B.partA);
+    mgcg_update_first<<<B.ge, kMgBlock, 0, s>>>(F.g.n, B.z, B.w, B.pv, B.q, B.x, B.r, fuse ? F.Dinv : nullptr,
+                                                F.s0, F.da, B.z0, B.sc, B.partB, B.gr, B.partA);
     mgcg_conv_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partA, B.ge);
     float* z = nullptr;
-    if (vcycle(mg, B.r, B.z0, B.z1, &z, B.sc, s, k == timed_k)) return -2;
+    if (vcycle(mg, B.r, B.z0, B.z1, &z, B.sc, s, k == timed_k, false, fuse)) return -2;
     B.z = z;
     if (B.S)
       mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(B.S, B.g, B.mask, B.z, B.r, B.q, B.w, B.sc, B.partB);
     else
       mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(B.m, B.vals, B.z, B.r, B.q, B.w, B.sc, B.partB);
-    mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partB, B.gr);
   }
   return 0;
 }
@@ -2368,7 +2434,7 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   B.partA = partA;
   B.partB = partB;
   B.gr = S ? node_grid(mg->lv[0].g.n) : rows_grid(m.nslices);  // the operator kernels' grid (= partial count)
-  B.ge = elem_grid(N);
+  B.ge = node_grid(mg->lv[0].g.n);  // the update's grid (= its partial count)
   const bool graph = !(flags & HF_MG_SOLVE_EAGER);
   HF_CHECK_HIP(hipMemcpyAsync(B.x, x, N * sizeof(float), hipMemcpyDeviceToDevice, s));
   if (S)
@@ -2383,7 +2449,6 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
     mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(S, B.g, B.mask, B.z, B.r, B.q, B.w, sc, partB);
   else
     mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, B.z, B.r, B.q, B.w, sc, partB);
-  mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(sc, partB, B.gr);
   HF_LAUNCH_CHECK();
   if (poll_init()) return -2;
   Scalars host{};
