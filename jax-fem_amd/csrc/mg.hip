@@ -1018,10 +1018,9 @@ __global__ void __launch_bounds__(kDenseApplyThreads) mg_dense_apply(const doubl
 constexpr int kSmallThreads = 1024;
 constexpr int kGroup = 8;                             // lanes per node
 constexpr int kSmallNodes = kSmallThreads / kGroup;   // 128 nodes a round
-#ifndef HF_SMALL_START
-#define HF_SMALL_START 128
-#endif
-constexpr int kSmallStart = HF_SMALL_START;
+// the cycle starts at the first level of at most kSmallStart nodes: one CU's
+// passes over a 6^3 level (216 nodes) cost more than the 6 launches they save
+constexpr int kSmallStart = 128;
 constexpr int kSmallMax = 4;
 constexpr int kSmallUnroll = 1;  // 2 fits in 100 VGPRs but measured slower here (37 -> 42 us)
 struct SmallLevel {
