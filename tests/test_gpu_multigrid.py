@@ -511,3 +511,41 @@ def test_setup_rejects_non_lattice_pattern(jfa):
         mg.setup(A, torch.ones(prob.N, device=DEV))
     with pytest.raises(jfa.HipFemError, match="2 nodes"):
         MG.Multigrid((1, 4, 4))
+
+
+@pytest.mark.parametrize("cells", [(2, 2, 2), (3, 3, 3), (5, 4, 6)])
+def test_mg_solve_edge_cases(jfa, cells):
+    """The solve loop at its edges: a one-level hierarchy (the dense FP64 solve
+    alone, 2x2x2 cells), a fine level over the dense one only (3x3x3), a zero
+    right-hand side (done at the start, x untouched), maxiter = 0 (status 1), and
+    poll batches longer than the solve needs: every launch after convergence
+    exits at once, so x and the iteration count equal those of poll = 1 bit for bit,
+    replayed or eager."""
+    say("mg edge cases", cells)
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    prob = box_problem(jfa, cells)
+    A = tangent(jfa, prob, scale=0.002)
+    mask = MG.dirichlet_mask(prob)
+    mg = MG.Multigrid(MG.lattice(prob))
+    nlev = len(mg.levels())
+    assert nlev == {(2, 2, 2): 1, (3, 3, 3): 2}.get(cells, nlev) and nlev >= 1
+    mg.setup(A, mask)
+    g = torch.Generator(device=DEV).manual_seed(5)
+    b = torch.randn(prob.N, generator=g, device=DEV) * mask
+    A64 = A.to_scipy()
+    bb = b.double().cpu().numpy()
+    x1, i1 = mg.solve(A, b, rtol=1e-6, atol=0.0, maxiter=100, rebuild="never")
+    assert i1.status == 0, (i1.status, i1.iterations)
+    rel = np.linalg.norm(bb - A64 @ x1.double().cpu().numpy()) / np.linalg.norm(bb)
+    assert rel <= 1e-5, rel
+    if nlev == 1:
+        assert i1.iterations <= 2, i1.iterations   # the preconditioner is the exact inverse
+    for poll, eager in ((3, False), (3, True), (7, False)):
+        xp, ip = mg.solve(A, b, rtol=1e-6, atol=0.0, maxiter=100, rebuild="never", poll=poll, eager=eager)
+        assert (ip.status, ip.iterations) == (i1.status, i1.iterations), (poll, eager, ip.iterations)
+        assert torch.equal(xp, x1), (poll, eager)
+    z = torch.zeros_like(b)
+    xz, iz = mg.solve(A, z, x0=torch.zeros_like(b), rtol=1e-6, atol=0.0, maxiter=100, rebuild="never")
+    assert iz.status == 0 and iz.iterations == 0 and bool((xz == 0).all())
+    xm, im = mg.solve(A, b, rtol=1e-6, atol=0.0, maxiter=0, rebuild="never")
+    assert im.status == 1 and im.iterations == 0 and bool((xm == 0).all())
