@@ -267,7 +267,9 @@ int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf_stream_t s
  * it is captured again when the hierarchy is set up again or the pattern,
  * values pointer, workspace or poll change.  HF_MG_SOLVE_EAGER enqueues every
  * launch instead (the same kernels in the same order: identical results).
- * Calls that share one hf_mg must not overlap. */
+ * Calls that share one hf_mg must not overlap.  info is final on return; when
+ * the solve converged, the copy of the iterate into x may still be queued on
+ * `stream` (work enqueued behind it on that stream sees x). */
 enum { HF_MG_SOLVE_EAGER = 4 };
 size_t hf_mg_workspace_size(const hf_mg* mg);
 int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* b, float* x, int32_t flags,

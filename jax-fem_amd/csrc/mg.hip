@@ -2489,8 +2489,13 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
     }
   }
   HF_CHECK_HIP(hipMemcpyAsync(x, B.x, N * sizeof(float), hipMemcpyDeviceToDevice, s));
-  HF_CHECK_HIP(hipStreamSynchronize(s));
-  HF_CHECK_HIP(hipMemcpy(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+  if (!host.done) {
+    // stopped by the iteration cap: the batches still queued may count further
+    HF_CHECK_HIP(hipStreamSynchronize(s));
+    HF_CHECK_HIP(hipMemcpy(&host, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+  }
+  // once done is set every later launch exits at its first line, so the polled
+  // scalars are final and the copy of x is left queued on the stream
   if (info) {
     info->iterations = host.iter;
     info->status = host.done ? host.status : 1;
