@@ -244,13 +244,18 @@ int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out)
 int hf_mg_destroy(hf_mg* mg);
 /* number of levels; dims (3 per level, up to max_levels) receives the node counts */
 int hf_mg_levels(const hf_mg* mg, int32_t* nlev, int32_t* dims, int32_t max_levels);
+/* after a set-up: recomputes the coefficients on the default stream and
+ * synchronises the device */
 int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float param);
-/* lam of every level but the coarsest from the last set-up */
+/* lam of every level but the coarsest from the last set-up (synchronises the device) */
 int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels);
 /* builds every level from the FP32 tangent (pattern + SELL values) and the
  * level-0 DOF mask (1 free, 0 Dirichlet; kept by reference until the next
  * set-up); *dropped (optional) = coarsest DOFs without support.  Fails if the
- * pattern couples nodes that are not lattice neighbours.  Synchronises. */
+ * pattern couples nodes that are not lattice neighbours.  Synchronises the
+ * stream on a pattern's first set-up, with a SELL fine level, or when dropped is
+ * given (the checks read device flags); otherwise it only enqueues, and the
+ * level data and coefficients are ready for work queued behind it on `stream`. */
 int hf_mg_setup_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* mask, int32_t* dropped,
                     hf_stream_t stream);
 /* z = V(r), one cycle; HF_MG_SKIP_FINE_POST stops after the fine level's coarse

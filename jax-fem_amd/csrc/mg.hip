@@ -57,6 +57,10 @@ constexpr int kDenseMax = 88;   // FP64 dense coarsest matrix in 62 KB of LDS
 constexpr int kMgBlock = 256;
 constexpr int kSt = 243;        // 27 slots x 9 values
 constexpr int kMaxDegree = 8;   // smoothing steps per side
+// A level's smoothing coefficients in device memory: s0, c1[kMaxDegree], c2[kMaxDegree]
+// (written on the set-up stream, read by the cycle's kernels, so a captured
+// cycle stays valid across set-ups).
+constexpr int kCoefStride = 1 + 2 * kMaxDegree;
 
 struct Lat {
   int nx, ny, nz;
@@ -857,20 +861,23 @@ __device__ __forceinline__ void prolong_node(const float* __restrict__ ec, const
     for (int64_t node = i0_ + threadIdx.x; node < i1_; node += kMgBlock) __VA_ARGS__;  \
   } while (0)
 
-__global__ void __launch_bounds__(kMgBlock) mg_poly_first(const float* __restrict__ Dinv, int64_t n, float s0,
-                                                          const float* __restrict__ r, float* __restrict__ d,
-                                                          float* __restrict__ e, const Scalars* sc) {
+__global__ void __launch_bounds__(kMgBlock) mg_poly_first(const float* __restrict__ Dinv, int64_t n,
+                                                          const float* __restrict__ cf, const float* __restrict__ r,
+                                                          float* __restrict__ d, float* __restrict__ e,
+                                                          const Scalars* sc) {
   if (sc && sc->done) return;
+  const float s0 = cf[0];
   MG_NODE_LOOP(n, poly_first_node(Dinv, n, s0, r, d, e, node));
 }
 
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_poly_step(const float* __restrict__ S, const float* __restrict__ Dinv,
-                                                         Lat g, float c1, float c2, const float* res_in,
-                                                         const float* __restrict__ d_in, float* res_out,
-                                                         float* __restrict__ d_out, float* __restrict__ e,
-                                                         const Scalars* sc) {
+                                                         Lat g, const float* __restrict__ cf, int j,
+                                                         const float* res_in, const float* __restrict__ d_in,
+                                                         float* res_out, float* __restrict__ d_out,
+                                                         float* __restrict__ e, const Scalars* sc) {
   if (sc && sc->done) return;
+  const float c1 = cf[1 + j], c2 = cf[1 + kMaxDegree + j];
   MG_NODE_LOOP(g.n, poly_step_node<HALF>(S, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
 }
 
@@ -886,19 +893,21 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const float* __restric
 
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_post_first(const float* __restrict__ S, const float* __restrict__ Dinv,
-                                                          Lat g, float s0, const float* __restrict__ r,
-                                                          const float* __restrict__ e_in, float* __restrict__ res,
-                                                          float* __restrict__ d, float* __restrict__ e_out,
-                                                          const Scalars* sc) {
+                                                          Lat g, const float* __restrict__ cf,
+                                                          const float* __restrict__ r, const float* __restrict__ e_in,
+                                                          float* __restrict__ res, float* __restrict__ d,
+                                                          float* __restrict__ e_out, const Scalars* sc) {
   if (sc && sc->done) return;
+  const float s0 = cf[0];
   MG_NODE_LOOP(g.n, post_first_node<HALF>(S, Dinv, g, s0, r, e_in, res, d, e_out, node));
 }
 
 __global__ void __launch_bounds__(kMgBlock) mg_restrict(const float* __restrict__ t, Lat f, Lat c,
                                                         float* __restrict__ rc, const float* __restrict__ Dc,
-                                                        float s0, float* __restrict__ dc, float* __restrict__ ec,
-                                                        const Scalars* sc) {
+                                                        const float* __restrict__ cf, float* __restrict__ dc,
+                                                        float* __restrict__ ec, const Scalars* sc) {
   if (sc && sc->done) return;
+  const float s0 = cf[0];
   MG_NODE_LOOP(c.n, restrict_node(t, f, c, rc, Dc, s0, dc, ec, node));
 }
 
@@ -1028,8 +1037,7 @@ struct SmallLevel {
   const float* S;
   const float* Dinv;
   float *r, *e, *e2, *da, *db, *res;
-  float s0;
-  float c1[kMaxDegree], c2[kMaxDegree];
+  const float* cf;  // the level's smoothing coefficients (device memory)
 };
 struct SmallCycle {
   int nlev;     // levels here, the last one dense
@@ -1105,7 +1113,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
           float dr[3];
           block_apply(V.Dinv, V.g.n, node, rr, dr);
           for (int i = 0; i < 3; ++i) {
-            const float dn = V.c1[j] * din[3 * node + i] + V.c2[j] * dr[i];
+            const float dn = V.cf[1 + j] * din[3 * node + i] + V.cf[1 + kMaxDegree + j] * dr[i];
             V.res[3 * node + i] = rr[i];
             dout[3 * node + i] = dn;
             cur[l][3 * node + i] += dn;
@@ -1123,7 +1131,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
     });
     const SmallLevel& C = sc_.lv[l + 1];
     const bool smooth_c = (l + 2 < L);
-    MG_WG_LOOP(C.g.n, restrict_node(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.s0, C.da, cur[l + 1], node));
+    MG_WG_LOOP(C.g.n, restrict_node(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.cf[0], C.da, cur[l + 1], node));
   }
   dense_apply_wg<kSmallThreads>(sc_.Ainv, sc_.nd, sc_.lv[L - 1].r, cur[L - 1]);
   __syncthreads();
@@ -1142,8 +1150,8 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
         block_apply(V.Dinv, V.g.n, node, rr, dr);
         for (int i = 0; i < 3; ++i) {
           V.res[3 * node + i] = rr[i];
-          V.da[3 * node + i] = V.s0 * dr[i];
-          eout[3 * node + i] = ein[3 * node + i] + V.s0 * dr[i];
+          V.da[3 * node + i] = V.cf[0] * dr[i];
+          eout[3 * node + i] = ein[3 * node + i] + V.cf[0] * dr[i];
         }
       }
     });
@@ -1160,7 +1168,7 @@ __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, 
           float dr[3];
           block_apply(V.Dinv, V.g.n, node, rr, dr);
           for (int i = 0; i < 3; ++i) {
-            const float dn = V.c1[j] * din[3 * node + i] + V.c2[j] * dr[i];
+            const float dn = V.cf[1 + j] * din[3 * node + i] + V.cf[1 + kMaxDegree + j] * dr[i];
             V.res[3 * node + i] = rr[i];
             dout[3 * node + i] = dn;
             eout[3 * node + i] += dn;
@@ -1205,12 +1213,13 @@ __device__ __forceinline__ float block_row(const float* __restrict__ D, int64_t 
 // its node before any of its lanes stores one)
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_poly_step_lanes(const float* __restrict__ S,
-                                                               const float* __restrict__ Dinv, Lat g, float c1,
-                                                               float c2, const float* res_in,
-                                                               const float* __restrict__ d_in, float* res_out,
-                                                               float* __restrict__ d_out, float* __restrict__ e,
-                                                               const Scalars* sc) {
+                                                               const float* __restrict__ Dinv, Lat g,
+                                                               const float* __restrict__ cf, int j,
+                                                               const float* res_in, const float* __restrict__ d_in,
+                                                               float* res_out, float* __restrict__ d_out,
+                                                               float* __restrict__ e, const Scalars* sc) {
   if (sc && sc->done) return;
+  const float c1 = cf[1 + j], c2 = cf[1 + kMaxDegree + j];
   MG_LANE_LOOP(g.n, {
     float ad[3];
     stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad);
@@ -1243,12 +1252,14 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_resid_lanes(const float* __r
 
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_post_first_lanes(const float* __restrict__ S,
-                                                                const float* __restrict__ Dinv, Lat g, float s0,
+                                                                const float* __restrict__ Dinv, Lat g,
+                                                                const float* __restrict__ cf,
                                                                 const float* __restrict__ r,
                                                                 const float* __restrict__ e_in,
                                                                 float* __restrict__ res, float* __restrict__ d,
                                                                 float* __restrict__ e_out, const Scalars* sc) {
   if (sc && sc->done) return;
+  const float s0 = cf[0];
   MG_LANE_LOOP(g.n, {
     float ae[3];
     stencil_row_group<HALF, 4>(S, g, node, e_in, q, ae);
@@ -1464,13 +1475,14 @@ __global__ void __launch_bounds__(kBlock) mg_sell_spmv(MatArgs m, const float* _
 //   step:  res = res_in - y (v = d_in); d_out = c1 d_in + c2 Dinv res; e += d_out
 //   resid: t = res_in - y (v = d_in)
 //   post:  res = r - y (v = e_in); d = s0 Dinv res; e_out = e_in + d
-__global__ void __launch_bounds__(kMgBlock) mg_sell_step_update(const float* __restrict__ Dinv, int64_t n, float c1,
-                                                                float c2, const float* res_in,
-                                                                const float* __restrict__ y,
+__global__ void __launch_bounds__(kMgBlock) mg_sell_step_update(const float* __restrict__ Dinv, int64_t n,
+                                                                const float* __restrict__ cf, int j,
+                                                                const float* res_in, const float* __restrict__ y,
                                                                 const float* __restrict__ d_in, float* res_out,
                                                                 float* __restrict__ d_out, float* __restrict__ e,
                                                                 const Scalars* sc) {
   if (sc && sc->done) return;
+  const float c1 = cf[1 + j], c2 = cf[1 + kMaxDegree + j];
   MG_NODE_LOOP(n, {
     const float res[3] = {res_in[3 * node] - y[3 * node], res_in[3 * node + 1] - y[3 * node + 1],
                           res_in[3 * node + 2] - y[3 * node + 2]};
@@ -1494,13 +1506,15 @@ __global__ void __launch_bounds__(kMgBlock) mg_sell_resid_update(int64_t n, cons
   });
 }
 
-__global__ void __launch_bounds__(kMgBlock) mg_sell_post_update(const float* __restrict__ Dinv, int64_t n, float s0,
+__global__ void __launch_bounds__(kMgBlock) mg_sell_post_update(const float* __restrict__ Dinv, int64_t n,
+                                                                const float* __restrict__ cf,
                                                                 const float* __restrict__ r,
                                                                 const float* __restrict__ y,
                                                                 const float* __restrict__ e_in, float* __restrict__ res,
                                                                 float* __restrict__ d, float* __restrict__ e_out,
                                                                 const Scalars* sc) {
   if (sc && sc->done) return;
+  const float s0 = cf[0];
   MG_NODE_LOOP(n, {
     const float rr[3] = {r[3 * node] - y[3 * node], r[3 * node + 1] - y[3 * node + 1], r[3 * node + 2] - y[3 * node + 2]};
     float dr[3];
@@ -1696,7 +1710,7 @@ __global__ void __launch_bounds__(kMgBlock) mgcg_update_first(int64_t n, const f
                                                               const float* __restrict__ w, float* __restrict__ p,
                                                               float* __restrict__ q, float* __restrict__ x,
                                                               float* __restrict__ r, const float* __restrict__ Dinv,
-                                                              float s0, float* __restrict__ d0,
+                                                              const float* __restrict__ cf, float* __restrict__ d0,
                                                               float* __restrict__ e0, Scalars* sc,
                                                               const double* __restrict__ partB, int nB,
                                                               double* partials) {
@@ -1727,6 +1741,7 @@ __global__ void __launch_bounds__(kMgBlock) mgcg_update_first(int64_t n, const f
     sc->alpha = alpha_d;
   }
   const float alpha = (float)alpha_d, betaf = (float)beta;
+  const float s0 = Dinv ? cf[0] : 0.f;
   double loc[1] = {0};
   MG_NODE_LOOP(n, {
     float rn[3];
@@ -1765,6 +1780,38 @@ __global__ void __launch_bounds__(kScalarBlock) mgcg_conv_scalars(Scalars* sc, c
   else if (sc->iter >= sc->maxiter) { sc->done = 1; sc->status = 1; }
 }
 
+// Smoothing coefficients of levels 0..nl-1 from their Gershgorin bounds, one
+// thread per level, in FP64 as the recurrence defines them (mg_poly_step):
+// Chebyshev on [lam / param, lam]: theta = (hi + lo) / 2, delta = (hi - lo) / 2,
+// sigma = theta / delta, s0 = 1 / theta, rho_1 = 1 / sigma, and for j >= 1
+// rho_{j+1} = 1 / (2 sigma - rho_j), c1_j = rho_{j+1} rho_j, c2_j = 2 rho_{j+1} / delta;
+// damped block Jacobi: s0 = c2_j = omega, c1_j = 0.
+__global__ void mg_coef(const float* __restrict__ lam, int nl, int kind, float param, float* __restrict__ coef) {
+  const int l = threadIdx.x;
+  if (l >= nl) return;
+  float* cf = coef + (int64_t)l * kCoefStride;
+  if (kind == HF_MG_JACOBI) {
+    cf[0] = param;
+    for (int j = 0; j < kMaxDegree; ++j) {
+      cf[1 + j] = 0.f;
+      cf[1 + kMaxDegree + j] = param;
+    }
+    return;
+  }
+  const double hi = lam[l] > 0.f ? (double)lam[l] : 1.0, lo = hi / (double)param;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  cf[0] = (float)(1.0 / theta);
+  cf[1] = 0.f;
+  cf[1 + kMaxDegree] = 0.f;
+  for (int j = 1; j < kMaxDegree; ++j) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    cf[1 + j] = (float)(rn * rho);
+    cf[1 + kMaxDegree + j] = (float)(2.0 * rn / delta);
+    rho = rn;
+  }
+}
+
 }  // namespace
 }  // namespace hf
 
@@ -1783,10 +1830,6 @@ struct MgLevel {
   float* da = nullptr;    // 3 n smoother directions (two, alternating)
   float* db = nullptr;    // 3 n
   float* res = nullptr;   // 3 n smoother residual / restriction input
-  float lam = 0.f;        // Gershgorin bound of Dinv A (last set-up)
-  float s0 = 0.f;         // polynomial coefficients (see mg_poly_step)
-  float c1[kMaxDegree] = {0};
-  float c2[kMaxDegree] = {0};
 };
 
 struct hf_mg {
@@ -1797,6 +1840,7 @@ struct hf_mg {
   int nd = 0;
   int32_t* flags = nullptr;  // [0] pattern not on the lattice, [1] dropped coarse pivots
   float* part = nullptr;     // kMaxGrid per-workgroup maxima + kMgMaxLevels bounds
+  float* coef = nullptr;     // kMgMaxLevels x kCoefStride smoothing coefficients (mg_coef)
   bool half = false;         // fine-level stencil half-stored (symmetric operator)
   bool sell = false;         // fine level is the tangent's SELL matrix (no fine stencil)
   float* y = nullptr;        // 3 n0: SELL products of the fine level
@@ -1810,12 +1854,16 @@ struct hf_mg {
   int64_t checked_nblocks = -1;
   bool ready = false;
   // hf_mg_solve_f32's poll batch as a captured graph, replayed while its key
-  // (set-up generation and every pointer and size the batch uses) matches
-  uint64_t gen = 0;                 // bumped whenever level data or coefficients change
+  // (the cycle's shape and every pointer and size the batch uses) matches; the
+  // level data and coefficients are read from device memory, so a new set-up
+  // reuses the graph
+  uint64_t gen = 0;                 // bumped when the smoother's kind or degree changes
   hipStream_t cap = nullptr;        // capture stream (non-blocking, default priority)
   hipGraphExec_t gexec = nullptr;
-  uintptr_t gkey[6] = {0, 0, 0, 0, 0, 0};
+  uintptr_t gkey[7] = {0, 0, 0, 0, 0, 0, 0};
 };
+
+static const float* coef_of(const hf_mg* mg, int l) { return mg->coef + (int64_t)l * kCoefStride; }
 
 // First level run inside mg_small_cycle (the levels from it down), or nlev - 1
 // (only the dense solve) when no level is small enough.
@@ -1841,6 +1889,7 @@ static int mg_free(hf_mg* mg) {
   if (mg->Ainv) (void)hipFree(mg->Ainv);
   if (mg->flags) (void)hipFree(mg->flags);
   if (mg->part) (void)hipFree(mg->part);
+  if (mg->coef) (void)hipFree(mg->coef);
   return 0;
 }
 
@@ -1911,6 +1960,7 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
     rc |= alloc(&mg->B, kSt * nb);
   }
   if (!rc) rc |= alloc(&mg->part, kMaxGrid + kMgMaxLevels);
+  if (!rc) rc |= alloc(&mg->coef, (int64_t)kMgMaxLevels * kCoefStride);
   mg->nd = (int)(3 * mg->lv[mg->nlev - 1].g.n);
   if (!rc) {
     hipError_t e1 = hipMalloc((void**)&mg->Ainv, (size_t)mg->nd * mg->nd * sizeof(double));
@@ -1949,27 +1999,12 @@ extern "C" int hf_mg_levels(const hf_mg* mg, int32_t* nlev, int32_t* dims, int32
 }
 
 // Polynomial coefficients of every level from its bound lam.
-static void mg_coefficients(hf_mg* mg) {
-  ++mg->gen;  // the cycle's kernels take the coefficients by value
-  for (int l = 0; l + 1 < mg->nlev; ++l) {
-    MgLevel& L = mg->lv[l];
-    if (mg->kind == HF_MG_JACOBI) {
-      L.s0 = mg->param;
-      for (int j = 0; j < kMaxDegree; ++j) { L.c1[j] = 0.f; L.c2[j] = mg->param; }
-      continue;
-    }
-    // Chebyshev on [lam / param, lam]
-    const double hi = L.lam > 0.f ? (double)L.lam : 1.0, lo = hi / mg->param;
-    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-    double rho = 1.0 / sigma;
-    L.s0 = (float)(1.0 / theta);
-    for (int j = 1; j < kMaxDegree; ++j) {
-      const double rn = 1.0 / (2.0 * sigma - rho);
-      L.c1[j] = (float)(rn * rho);
-      L.c2[j] = (float)(2.0 * rn / delta);
-      rho = rn;
-    }
-  }
+// the levels' smoothing coefficients from the bounds of the last set-up, on stream s
+static int mg_coefficients(hf_mg* mg, hipStream_t s) {
+  const float* lam_dev = mg->part + kMaxGrid;
+  if (mg->nlev > 1) mg_coef<<<1, kMgMaxLevels, 0, s>>>(lam_dev, mg->nlev - 1, mg->kind, mg->param, mg->coef);
+  HF_LAUNCH_CHECK();
+  return 0;
 }
 
 extern "C" int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float param) {
@@ -1983,7 +2018,13 @@ extern "C" int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float
   mg->kind = kind;
   mg->degree = degree;
   mg->param = param;
-  mg_coefficients(mg);
+  ++mg->gen;  // the cycle's launch sequence depends on the degree
+  if (mg->ready) {
+    // new coefficients for the current levels; the set-up ran on a caller's stream
+    HF_CHECK_HIP(hipDeviceSynchronize());
+    if (mg_coefficients(mg, nullptr)) return -2;
+    HF_CHECK_HIP(hipDeviceSynchronize());
+  }
   return 0;
 }
 
@@ -2060,7 +2101,17 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   }
   const Lat gl = mg->lv[mg->nlev - 1].g;
   mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
+  if (mg_coefficients(mg, s)) return -2;
   HF_LAUNCH_CHECK();
+  mg->mask = mask;
+  // The host waits for the set-up only to check it: a pattern's first set-up (its
+  // lattice structure), a SELL fine level (couplings beyond one coarse step), or a
+  // caller asking for the dropped pivot count.  Otherwise it returns at once and
+  // the solve's launches queue behind the set-up.
+  if (!(check || mg->sell || dropped)) {
+    mg->ready = true;
+    return 0;
+  }
   int32_t hflags[4] = {0, 0, 0, 0};
   float lam[kMgMaxLevels] = {0};
   HF_CHECK_HIP(hipMemcpyAsync(hflags, mg->flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
@@ -2075,13 +2126,9 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
     mg->checked = p;
     mg->checked_nblocks = pattern_nblocks(p);
   }
-  for (int l = 0; l + 1 < mg->nlev; ++l) {
+  for (int l = 0; l + 1 < mg->nlev; ++l)
     HF_REQUIRE(std::isfinite(lam[l]), "hf_mg_setup_f32: non-finite operator on level %d", l);
-    mg->lv[l].lam = lam[l];
-  }
-  mg_coefficients(mg);
   if (dropped) *dropped = hflags[1];
-  mg->mask = mask;
   mg->ready = true;
   return 0;
 }
@@ -2142,7 +2189,13 @@ extern "C" int hf_mg_stencil_matvec_f32(const hf_mg* mg, const float* S, const f
 
 extern "C" int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels) {
   HF_REQUIRE(mg && lam, "hf_mg_bounds: null argument");
-  for (int l = 0; l < mg->nlev && l < max_levels; ++l) lam[l] = mg->lv[l].lam;
+  HF_REQUIRE(mg->ready, "hf_mg_bounds: hf_mg_setup_f32 has not run");
+  float dev[kMgMaxLevels] = {0};
+  // the set-up may still run on the caller's stream
+  HF_CHECK_HIP(hipDeviceSynchronize());
+  if (mg->nlev > 1)
+    HF_CHECK_HIP(hipMemcpy(dev, mg->part + kMaxGrid, (mg->nlev - 1) * sizeof(float), hipMemcpyDeviceToHost));
+  for (int l = 0; l < mg->nlev && l < max_levels; ++l) lam[l] = dev[l];
   return 0;
 }
 
@@ -2171,7 +2224,8 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const int gg = node_grid(V.g.n);
     // pre-smoothing from e = 0 (its start on levels >= 1 is fused into the
     // restriction that produced rin(l)), then the residual for the coarse level
-    if (l == 0 && !first_done) mg_poly_first<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.s0, rin(l), V.da, cur[l], sc);
+    const float* cf = coef_of(mg, l);
+    if (l == 0 && !first_done) mg_poly_first<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, cf, rin(l), V.da, cur[l], sc);
     const float* res = rin(l);
     float* din = V.da;
     float* dout = V.db;
@@ -2181,10 +2235,10 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       constexpr bool H = decltype(half_tag)::value;
       for (int j = 1; j < k; ++j) {
         if (lanes)
-          mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], res, din, V.res, dout,
+          mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, res, din, V.res, dout,
                                                         cur[l], sc);
         else
-          mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], res, din, V.res, dout,
+          mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, res, din, V.res, dout,
                                                   cur[l], sc);
         res = V.res;
         float* tmp = din; din = dout; dout = tmp;
@@ -2201,7 +2255,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       const int gr = rows_grid(m.nslices);
       for (int j = 1; j < k; ++j) {
         mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
-        mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.c1[j], V.c2[j], res, mg->y, din, V.res, dout,
+        mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, cf, j, res, mg->y, din, V.res, dout,
                                                     cur[l], sc);
         res = V.res;
         float* tmp = din; din = dout; dout = tmp;
@@ -2215,7 +2269,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     }
     const MgLevel& C = mg->lv[l + 1];
     const bool smooth_c = (l + 2 < L);
-    mg_restrict<<<node_grid(C.g.n), kMgBlock, 0, s>>>(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.s0,
+    mg_restrict<<<node_grid(C.g.n), kMgBlock, 0, s>>>(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, coef_of(mg, l + 1),
                                                       C.da, cur[l + 1], sc);
   }
   if (L == 1) {
@@ -2231,8 +2285,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       SmallLevel& q = c.lv[l - lb];
       q.g = V.g; q.S = V.S; q.Dinv = V.Dinv;
       q.r = V.r; q.e = V.e; q.e2 = V.e2; q.da = V.da; q.db = V.db; q.res = V.res;
-      q.s0 = V.s0;
-      for (int j = 0; j < kMaxDegree; ++j) { q.c1[j] = V.c1[j]; q.c2[j] = V.c2[j]; }
+      q.cf = coef_of(mg, l);
     }
     mg_small_cycle<<<1, kSmallThreads, 0, s>>>(c, sc);
     cur[lb] = (L - lb >= 2) ? mg->lv[lb].e2 : mg->lv[lb].e;
@@ -2240,6 +2293,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
   for (int l = lb - 1; l >= 0; --l) {
     const MgLevel& V = mg->lv[l];
     const MgLevel& C = mg->lv[l + 1];
+    const float* cf = coef_of(mg, l);
     const int gg = node_grid(V.g.n);
     mg_prolong<<<gg, kMgBlock, 0, s>>>(cur[l + 1], V.g, C.g, l == 0 ? mg->mask : nullptr, cur[l], sc);
     if (l == 0 && skip_fine_post) break;
@@ -2255,19 +2309,19 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
         hipEvent_t e0, e1;
         if (prof_events(&e0, &e1)) { perr = -2; return; }
         hipExtLaunchKernelGGL(post, dim3(gp), dim3(kMgBlock), 0, s, e0, e1, 0, (const float*)V.S,
-                              (const float*)V.Dinv, V.g, V.s0, rin(l), (const float*)cur[l], V.res, V.da, oth[l], sc);
+                              (const float*)V.Dinv, V.g, cf, rin(l), (const float*)cur[l], V.res, V.da, oth[l], sc);
       } else {
-        post<<<gp, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.s0, rin(l), cur[l], V.res, V.da, oth[l], sc);
+        post<<<gp, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, rin(l), cur[l], V.res, V.da, oth[l], sc);
       }
       float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
       float* din = V.da;
       float* dout = V.db;
       for (int j = 1; j < k; ++j) {
         if (lanes)
-          mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], V.res, din, V.res, dout,
+          mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, V.res, din, V.res, dout,
                                                         cur[l], sc);
         else
-          mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, V.c1[j], V.c2[j], V.res, din, V.res, dout,
+          mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, V.res, din, V.res, dout,
                                                   cur[l], sc);
         float* t2 = din; din = dout; dout = t2;
       }
@@ -2283,14 +2337,14 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       } else {
         mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, cur[l], mg->y, sc);
       }
-      mg_sell_post_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.s0, rin(l), mg->y, cur[l], V.res, V.da, oth[l],
+      mg_sell_post_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, cf, rin(l), mg->y, cur[l], V.res, V.da, oth[l],
                                                   sc);
       float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
       float* din = V.da;
       float* dout = V.db;
       for (int j = 1; j < k; ++j) {
         mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
-        mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, V.c1[j], V.c2[j], V.res, mg->y, din, V.res, dout,
+        mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, cf, j, V.res, mg->y, din, V.res, dout,
                                                     cur[l], sc);
         float* t2 = din; din = dout; dout = t2;
       }
@@ -2362,7 +2416,7 @@ static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int 
   const bool fuse = mg->nlev > 1;
   for (int k = 0; k < poll; ++k) {
     mgcg_update_first<<<B.ge, kMgBlock, 0, s>>>(F.g.n, B.z, B.w, B.pv, B.q, B.x, B.r, fuse ? F.Dinv : nullptr,
-                                                F.s0, F.da, B.z0, B.sc, B.partB, B.gr, B.partA);
+                                                coef_of(mg, 0), F.da, B.z0, B.sc, B.partB, B.gr, B.partA);
     mgcg_conv_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partA, B.ge);
     float* z = nullptr;
     if (vcycle(mg, B.r, B.z0, B.z1, &z, B.sc, s, k == timed_k, false, fuse)) return -2;
@@ -2380,10 +2434,10 @@ static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int 
 // contents: the set-up generation (level coefficients are kernel arguments),
 // the tangent's values and pattern, the workspace and the batch length.
 static int mg_batch_graph(hf_mg* mg, const hf_pattern* p, MgBatch& B, void* ws, int poll) {
-  const uintptr_t key[6] = {(uintptr_t)mg->gen, (uintptr_t)p, (uintptr_t)(B.S ? B.S : B.vals), (uintptr_t)ws,
-                            (uintptr_t)poll, (uintptr_t)B.N};
+  const uintptr_t key[7] = {(uintptr_t)mg->gen, (uintptr_t)p, (uintptr_t)(B.S ? B.S : B.vals), (uintptr_t)ws,
+                            (uintptr_t)poll, (uintptr_t)B.N, (uintptr_t)mg->mask};
   bool same = mg->gexec != nullptr;
-  for (int i = 0; i < 6 && same; ++i) same = mg->gkey[i] == key[i];
+  for (int i = 0; i < 7 && same; ++i) same = mg->gkey[i] == key[i];
   if (same) return 0;
   if (!mg->cap) HF_CHECK_HIP(hipStreamCreateWithFlags(&mg->cap, hipStreamNonBlocking));
   if (mg->gexec) {
@@ -2399,7 +2453,7 @@ static int mg_batch_graph(hf_mg* mg, const hf_pattern* p, MgBatch& B, void* ws, 
   const hipError_t ei = hipGraphInstantiate(&mg->gexec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   HF_CHECK_HIP(ei);
-  for (int i = 0; i < 6; ++i) mg->gkey[i] = key[i];
+  for (int i = 0; i < 7; ++i) mg->gkey[i] = key[i];
   return 0;
 }
 

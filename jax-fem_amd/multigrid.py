@@ -118,7 +118,6 @@ class Multigrid:
         self._tangent = None     # the matrix values and mask the hierarchy was built from (kept alive)
         self._mask = None
         self._pattern = None
-        self.dropped = 0
         self.setup_seconds = 0.0
         self.setups = 0
         self.last = None
@@ -181,21 +180,25 @@ class Multigrid:
         LatticeTangent of this hierarchy's lattice) and the Dirichlet mask."""
         if A.dtype != torch.float32:
             raise ValueError("the multigrid preconditioner runs on the FP32 tangent")
-        mask = mask.to(torch.float32).contiguous()
-        dropped = ctypes.c_int32()
+        # one mask buffer per hierarchy: the cycle reads it by pointer, and a captured
+        # solve batch stays valid across set-ups while the pointer does not change
+        m = mask.to(device=self.device, dtype=torch.float32).reshape(-1)
+        if self._mask is None or self._mask.numel() != m.numel():
+            self._mask = torch.empty_like(m)
+        self._mask.copy_(m)
         t0 = time.perf_counter()
+        # no dropped-pivot count requested: after a pattern's first set-up the call
+        # only enqueues (the solve's launches queue behind it)
         if isinstance(A, LatticeTangent):
             A.check_current()
             _lib.call("hf_mg_setup_stencil_f32", self.handle, A.pattern.handle, _lib.ptr(A.stencil),
-                      _lib.ptr(mask), ctypes.byref(dropped), _lib.stream())
+                      _lib.ptr(self._mask), None, _lib.stream())
         else:
-            _lib.call("hf_mg_setup_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(mask),
-                      ctypes.byref(dropped), _lib.stream())
+            _lib.call("hf_mg_setup_f32", self.handle, A.pattern.handle, _lib.ptr(A.vals), _lib.ptr(self._mask),
+                      None, _lib.stream())
         self.setup_seconds += time.perf_counter() - t0
         self._tangent = _ident(A)   # identity of the tangent the levels come from
-        self._mask = mask           # the cycle reads the mask by pointer
         self._pattern = A.pattern
-        self.dropped = int(dropped.value)
         self.setups += 1
         self._iters_at_setup = None
         self._stale = False
