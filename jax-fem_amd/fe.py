@@ -90,8 +90,8 @@ def _eval_points(fn, pts, extra=()):
             out = np.broadcast_to(out, (n,) + ref0.shape)
         else:
             out = np.moveaxis(out.reshape(ref0.shape + (n,)), -1, 0) if out.ndim else np.full(n, float(out))
-        for i in np.unique(np.linspace(0, n - 1, min(n, 7)).astype(int)):
-            if not np.allclose(out[i], single(i), rtol=0, atol=0):
+        for i in sorted({(n - 1) * k // 6 for k in range(7)}):
+            if not np.array_equal(out[i], single(i)):   # exact, NaN unequal
                 raise ValueError
         return np.ascontiguousarray(out)
     except Exception:
@@ -151,12 +151,17 @@ class FiniteElement:
         function of the point (and index), and the mesh of an element is fixed, so
         the selection is evaluated once per function object; the cache holds the
         function itself, so its id cannot be reused by another one."""
+        return self._loc_entry(loc)[1]
+
+    def _loc_entry(self, loc):
+        """(loc, nodes, their coordinates), cached per location function."""
         hit = self._loc_cache.get(id(loc))
         if hit is not None and hit[0] is loc:
-            return hit[1]
+            return hit
         nodes = np.flatnonzero(_location_flags(loc, self.points))
-        self._loc_cache[id(loc)] = (loc, nodes)
-        return nodes
+        hit = (loc, nodes, np.ascontiguousarray(self.points[nodes]))
+        self._loc_cache[id(loc)] = hit
+        return hit
 
     # -- Dirichlet data (fe.py:220-269) --------------------------------------
     def Dirichlet_boundary_conditions(self, dirichlet_bc_info):
@@ -166,15 +171,15 @@ class FiniteElement:
         location_fns, vecs, value_fns = dirichlet_bc_info
         assert len(location_fns) == len(value_fns) and len(value_fns) == len(vecs)
         for loc, comp, val in zip(location_fns, vecs, value_fns):
-            nodes = self._loc_nodes(loc)
-            probe = val(self.points[nodes[0]]) if len(nodes) else 0.0
+            _, nodes, pts = self._loc_entry(loc)
+            probe = val(pts[0]) if len(nodes) else 0.0
             if _is_torch(probe):
                 # differentiable Dirichlet values (ad_wrapper through set_params)
                 import torch
                 values = torch.stack([torch.as_tensor(val(self.points[i]), dtype=torch.float64).reshape(())
                                       for i in nodes.tolist()]) if len(nodes) else torch.zeros(0)
             else:
-                values = _eval_points(val, self.points[nodes]).reshape(-1) if len(nodes) else np.zeros(0)
+                values = _eval_points(val, pts).reshape(-1) if len(nodes) else np.zeros(0)
                 values = np.asarray(values, dtype=np.float64)
             node_inds_list.append(nodes)
             vec_inds_list.append(np.full(len(nodes), comp, dtype=np.int32))

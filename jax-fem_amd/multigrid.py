@@ -45,10 +45,15 @@ def lattice(problem):
 
 
 def dirichlet_mask(problem):
-    """1 on free DOFs, 0 on Dirichlet DOFs (float32 device, N)."""
-    m = torch.ones(problem.N, dtype=torch.float32, device=problem.device)
+    """1 on free DOFs, 0 on Dirichlet DOFs (float32 device, N); rebuilt when the
+    problem's Dirichlet DOF set (the tensor bc_arrays returns) changes."""
     d, _ = problem.bc_arrays()
+    hit = getattr(problem, "_dirichlet_mask", None)
+    if hit is not None and hit[0] is d:
+        return hit[1]
+    m = torch.ones(problem.N, dtype=torch.float32, device=problem.device)
     m[d.long()] = 0.0
+    problem._dirichlet_mask = (d, m)
     return m
 
 

@@ -74,6 +74,7 @@ class Problem:
         self._bufs = {}
         self._ext_key = None
         self._bc_key = None
+        self._bc_struct = None     # node arrays + components the BC DOF set was built from
         self.custom_init(*self.additional_info)
         self._check_hooks()
 
@@ -171,17 +172,29 @@ class Problem:
         key = tuple((id(v), len(v), float(v.detach().sum()) if torch.is_tensor(v) else float(np.sum(v)))
                     for v in fe.vals_list) + tuple(id(n) for n in fe.node_inds_list)
         if self._bc_key != key:
-            ds, vs = [np.zeros(0, dtype=np.int64)], [np.zeros(0)]
-            for nodes, comps, vals in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list):
-                ds.append(np.asarray(nodes, dtype=np.int64) * self.vec + np.asarray(comps, dtype=np.int64))
+            # the DOF set depends on the node arrays and components only; a load step
+            # that changes the values alone reuses it (location functions are cached
+            # per function object, so their node arrays are the same objects)
+            struct = [(nodes, int(np.asarray(comps).reshape(-1)[0]) if len(comps) else -1, len(comps))
+                      for nodes, comps in zip(fe.node_inds_list, fe.vec_inds_list)]
+            same = self._bc_struct is not None and len(struct) == len(self._bc_struct) and all(
+                a[0] is b[0] and a[1:] == b[1:] and np.all(np.asarray(c) == a[1])
+                for a, b, c in zip(struct, self._bc_struct, fe.vec_inds_list))
+            if not same:
+                ds = [np.zeros(0, dtype=np.int64)]
+                for nodes, comps in zip(fe.node_inds_list, fe.vec_inds_list):
+                    ds.append(np.asarray(nodes, dtype=np.int64) * self.vec + np.asarray(comps, dtype=np.int64))
+                all_d = np.concatenate(ds)
+                # the last spec naming a DOF wins: first occurrence in reversed order; sorted DOFs
+                d, first = np.unique(all_d[::-1], return_index=True)
+                self._bc_first = first
+                self._bc_dofs = torch.as_tensor(d.astype(np.int32), device=self.device)
+                self._bc_struct = struct
+            vs = [np.zeros(0)]
+            for nodes, vals in zip(fe.node_inds_list, fe.vals_list):
                 vals = vals.detach().cpu().numpy() if torch.is_tensor(vals) else vals
-                vs.append(np.broadcast_to(np.asarray(vals, dtype=np.float64).reshape(-1), ds[-1].shape))
-            all_d, all_v = np.concatenate(ds), np.concatenate(vs)
-            # the last spec naming a DOF wins: first occurrence in reversed order; sorted DOFs
-            d, first = np.unique(all_d[::-1], return_index=True)
-            v = all_v[::-1][first]
-            d = d.astype(np.int32)
-            self._bc_dofs = torch.as_tensor(d, device=self.device)
+                vs.append(np.broadcast_to(np.asarray(vals, dtype=np.float64).reshape(-1), (len(nodes),)))
+            v = np.concatenate(vs)[::-1][self._bc_first]
             self._bc_vals = torch.as_tensor(v, device=self.device)
             self._bc_key = key
         return self._bc_dofs, self._bc_vals
