@@ -2508,6 +2508,7 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   int it = 0, batch = 0;
   bool timed_pending = false;
   int timed_batch = -1;
+  double rr_prev = -1.0;
   for (;;) {
     // the batch holding the HIP-event timed launch (a steady-state one) runs
     // eagerly; the others replay the captured batch
@@ -2535,6 +2536,16 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
         timed_pending = false;
       }
       if (host.done) break;
+      // When the last batch's reduction of r.r, carried forward, takes the batch just
+      // queued below the threshold, wait for that batch before queuing another: a
+      // batch queued behind a converged one only launches kernels that exit at once.
+      const double q = rr_prev > 0.0 ? host.rr / rr_prev : 1.0;
+      rr_prev = host.rr;
+      if (!timed_pending && q < 1.0 && host.rr * q <= host.thresh) {
+        if (poll_wait(batch & 1, host)) return -2;
+        if (host.done) break;
+        rr_prev = host.rr;
+      }
     }
     ++batch;
     if (it >= maxiter + 2 * poll) {
