@@ -23,6 +23,7 @@ from . import _lib
 class Material:
     kind = None
     vec = 3
+    linear = False   # stress linear in the displacement gradient: the tangent does not depend on u
 
     def __init__(self, E=1.0, nu=0.3, param=None, E0=1.0, Emin=1e-3, penal=3.0, kappa=None, k=1.0):
         modes = {None: _lib.HF_PARAM_NONE, "scale_E": _lib.HF_PARAM_SCALE_E, "simp": _lib.HF_PARAM_SIMP,
@@ -67,6 +68,7 @@ class Material:
 class LinearElastic(Material):
     """sigma = lam tr(eps) I + 2 mu eps (test_linear_elasticity_cube.py:15-25)."""
     kind = _lib.HF_MAT_LINEAR_ELASTIC
+    linear = True
 
     def __call__(self, u_grad, *iv):
         mu, lam = self.moduli(iv[0] if iv else None)
@@ -96,6 +98,7 @@ class Poisson(Material):
     """flux = k grad u, vec = 1 (tests/benchmarks/linear_poisson/test_linear_poisson.py:15-17)."""
     kind = _lib.HF_MAT_POISSON
     vec = 1
+    linear = True
 
     def __call__(self, u_grad, *iv):
         k = self.k * (iv[0] if (iv and self.param == "scale_k") else 1.0)

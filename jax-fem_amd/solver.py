@@ -369,13 +369,17 @@ def solver(problem, solver_options={}):
     hist = [res]
     n = 0
     eta_prev = None
+    A = None
+    linear_law = bool(getattr(problem.material, "linear", False))
     while (res / res0 > rel_tol) and (res > tol) and n < max_iter:
         n += 1
         # the tangent's time is read from events after the solve: no stall between
         # the assembly and the linear solve the host enqueues behind it
         ev0 = torch.cuda.Event(enable_timing=True)
         ev0.record()
-        if smg is not None:
+        if A is not None and linear_law:
+            pass   # a linear law's tangent does not depend on u: the first step's serves the solve
+        elif smg is not None:
             A = LatticeTangent(problem, u, smg)
         else:
             A = TangentMatrix(problem, problem.jacobian_bsr(u, adt))
