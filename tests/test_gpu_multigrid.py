@@ -391,6 +391,25 @@ def test_newton_with_mg_matches_fp64(jfa, stencil):
     assert rel <= 1e-6
 
 
+
+def test_linear_law_keeps_one_tangent_per_solve(jfa):
+    """A linear law's tangent does not depend on u: the mixed-precision Newton
+    refinement assembles it once per solve and still reaches the FP64 solution."""
+    say("linear law tangent reuse")
+    cells = (10, 8, 8)
+    E = np.exp(np.random.default_rng(4).normal(0.0, 1.0, int(np.prod(cells))))
+    prob = box_problem(jfa, cells, material="lin", E_cell=E)
+    u64 = jfa.solver(prob, {"newton": {"tol": 0.0, "rel_tol": 1e-10,
+                                       "linear": {"pcg": {"tol": 1e-13, "atol": 0.0}}}})[0]
+    prob2 = box_problem(jfa, cells, material="lin", E_cell=E)
+    v0 = getattr(prob2, "_stencil_version", 0)
+    u = jfa.solver(prob2, {"newton": {"precision": "mixed", "tol": 0.0, "rel_tol": 1e-10, "linear": {"mg": {}}}})[0]
+    assert prob2.solve_info["newton_iters"] >= 2      # FP32 solves refined in FP64
+    assert prob2._stencil_version - v0 == 1
+    assert prob2.solve_info["rel"] <= 1e-10
+    rel = (torch.linalg.norm(u - u64) / torch.linalg.norm(u64)).item()
+    assert rel <= 1e-8, rel
+
 def test_mg_adjoint_matches_fp64_pcg(jfa):
     say("mg adjoint")
     from importlib import import_module
