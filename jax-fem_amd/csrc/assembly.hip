@@ -265,6 +265,11 @@ __global__ void sell_to_csr_kernel(const int32_t* __restrict__ rowptr, const int
 // One thread per node row: every slot of the row's slice gets the sum of the
 // block's contributions (element blocks, block-major (cell, a, b, i, k), each
 // BB contiguous values) in cell order, or zeros for padding.
+// kAsmLanes lanes per slice row: lane l sums the row's slots t0 + l, t0 + l +
+// kAsmLanes, ..., so a HEX27 row's ~125 blocks take ~16 dependent rounds per lane
+// instead of 125 for one thread.  Each block is still summed by one lane over its
+// contributions in cell order: the same FP additions as one thread per row.
+constexpr int kAsmLanes = 8;
 template <typename T, int VEC>
 __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ seg,
@@ -273,29 +278,29 @@ __global__ void __launch_bounds__(256) assemble_kernel(const int32_t* __restrict
                                                        const int32_t* __restrict__ sell_row, int R, int64_t nslices,
                                                        const T* __restrict__ elem, T* __restrict__ vals) {
   constexpr int BB = VEC * VEC;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // slice row
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t q = gid / kAsmLanes;  // slice row
+  const int lane = (int)(gid % kAsmLanes);
   if (q >= nslices * R) return;
   const int64_t s = q / R;
   const int r = (int)(q - s * R);
-  const int64_t L = (int64_t)R * VEC;
   const int32_t n = sell_row[q];
   const int32_t t0 = slice_ptr[s], t1 = slice_ptr[s + 1];
   const int32_t b0 = n >= 0 ? rowptr[n] : 0, len = n >= 0 ? rowptr[n + 1] - b0 : 0;
-  for (int32_t t = t0; t < t1; ++t) {
+  for (int32_t t = t0 + lane; t < t1; t += kAsmLanes) {
     const int32_t j = t - t0;
     T acc[BB];
 #pragma unroll
     for (int e = 0; e < BB; ++e) acc[e] = 0;
     if (j < len) {
       const int32_t b = b0 + j;
-      for (int32_t q = seg[b]; q < seg[b + 1]; ++q) {
-        const T* src = elem + (int64_t)contrib[q] * BB;
+      for (int32_t c = seg[b]; c < seg[b + 1]; ++c) {
+        const T* src = elem + (int64_t)contrib[c] * BB;
 #pragma unroll
         for (int e = 0; e < BB; ++e) acc[e] += src[e];
       }
     }
     T* dst = vals + ((int64_t)t * R + r) * BB;
-    (void)L;
     // element blocks are row-major (i, k); matrix blocks are stored column-major
 #pragma unroll
     for (int e = 0; e < BB; ++e) dst[(e % VEC) * VEC + e / VEC] = acc[e];
@@ -627,11 +632,11 @@ static int assemble_impl(const hf_pattern* p, const T* elem, T* vals, hipStream_
   HF_REQUIRE(p && elem && vals, "hf_assemble: null argument");
   const int64_t n = p->nslices * p->R;
   if (p->vec == 3)
-    assemble_kernel<T, 3><<<grid_for(n), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr, p->sell_row,
-                                                       p->R, p->nslices, elem, vals);
+    assemble_kernel<T, 3><<<grid_for(n * kAsmLanes), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr,
+                                                                   p->sell_row, p->R, p->nslices, elem, vals);
   else if (p->vec == 1)
-    assemble_kernel<T, 1><<<grid_for(n), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr, p->sell_row,
-                                                       p->R, p->nslices, elem, vals);
+    assemble_kernel<T, 1><<<grid_for(n * kAsmLanes), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr,
+                                                                   p->sell_row, p->R, p->nslices, elem, vals);
   else
     HF_REQUIRE(false, "hf_assemble: vec=%d not supported", p->vec);
   HF_LAUNCH_CHECK();
