@@ -1697,15 +1697,60 @@ __global__ void __launch_bounds__(kMgBlock) mg_stencil_matvec(const float* __res
   });
 }
 
-// One outer step, node by node: every workgroup first sums the product's
-// partials (B: gamma = r.z, delta = z.w, zeta = z.q) in the same fixed order
-// and forms beta = -zeta / p.q_old, p.q = delta + 2 beta zeta + beta^2 p.q_old,
-// alpha = gamma / p.q (beta = 0 on the first step); workgroup 0 records them.
-// p.q alternates between two slots by iteration parity, so no workgroup reads
-// a value another one has already replaced.  Then p = z + beta p, q = w + beta q,
-// x += alpha p, r -= alpha q, the fine level's pre-smoother start from e = 0 on
-// the new residual (d = e = s0 Dinv r; Dinv null: a one-level hierarchy, whose
-// cycle is the dense solve of r), and partials (A) r.r for the convergence test.
+// beta = -zeta / p.q_old, p.q = delta + 2 beta zeta + beta^2 p.q_old, alpha =
+// gamma / p.q (beta = 0 on the first step) from d = (gamma, delta, zeta) =
+// (r.z, z.w, z.q); returns 0, or the breakdown status (2: p.q <= 0, 3: non-finite).
+__device__ __forceinline__ int dir_step(const double (&d)[3], int it, double pq_old, double& alpha, double& beta,
+                                        double& pq) {
+  const double gamma = d[0], delta = d[1], zeta = d[2];
+  beta = 0.0;
+  pq = delta;
+  if (it > 0) {
+    beta = -zeta / pq_old;
+    pq = delta + 2.0 * beta * zeta + beta * beta * pq_old;
+  }
+  alpha = gamma / pq;
+  if (!(pq > 0.0) || !isfinite(pq) || !isfinite(gamma)) return (isfinite(pq) && isfinite(gamma)) ? 2 : 3;
+  return 0;
+}
+// p.q alternates between two slots by iteration parity, so no reader of the old
+// value can see it replaced within the kernel that forms the new one
+__device__ __forceinline__ void dir_record(Scalars* sc, int st, int it, double gamma, double alpha, double beta,
+                                           double pq) {
+  if (st) {
+    sc->status = st;
+    sc->done = 1;
+    return;
+  }
+  sc->pqs[(it + 1) & 1] = pq;
+  sc->omega = beta;
+  sc->rho = gamma;
+  sc->alpha = alpha;
+}
+
+// The direction scalars by one workgroup, for products with many partials
+// (a grid of more than kFusedParts workgroups), where every workgroup of the
+// update summing them all would cost more than this launch.
+constexpr int kFusedParts = 2048;
+__global__ void __launch_bounds__(kScalarBlock) mgcg_dir_scalars(Scalars* sc, const double* partB, int nB) {
+  if (sc->done) return;
+  double d[3];
+  sum_partials<3>(partB, nB, d);
+  if (threadIdx.x != 0) return;
+  const int it = sc->iter;
+  double alpha, beta, pq;
+  const int st = dir_step(d, it, it > 0 ? sc->pqs[it & 1] : 0.0, alpha, beta, pq);
+  dir_record(sc, st, it, d[0], alpha, beta, pq);
+}
+
+// One outer step, node by node: with nB partials (at most kFusedParts) every
+// workgroup first sums the product's partials (B: gamma = r.z, delta = z.w,
+// zeta = z.q) in the same fixed order and forms the direction scalars itself
+// (dir_step; workgroup 0 records them); with nB = 0 mgcg_dir_scalars has formed
+// them.  Then p = z + beta p, q = w + beta q, x += alpha p, r -= alpha q, the fine
+// level's pre-smoother start from e = 0 on the new residual (d = e = s0 Dinv r;
+// Dinv null: a one-level hierarchy, whose cycle is the dense solve of r), and
+// partials (A) r.r for the convergence test.
 __global__ void __launch_bounds__(kMgBlock) mgcg_update_first(int64_t n, const float* __restrict__ z,
                                                               const float* __restrict__ w, float* __restrict__ p,
                                                               float* __restrict__ q, float* __restrict__ x,
@@ -1715,30 +1760,19 @@ __global__ void __launch_bounds__(kMgBlock) mgcg_update_first(int64_t n, const f
                                                               const double* __restrict__ partB, int nB,
                                                               double* partials) {
   if (sc->done) return;
-  double dv[3];
-  sum_partials_wg<3>(partB, nB, dv);
-  const double gamma = dv[0], delta = dv[1], zeta = dv[2];
-  const int it = sc->iter;
-  double beta = 0.0, pq = delta;
-  if (it > 0) {
-    const double pq_old = sc->pqs[it & 1];
-    beta = -zeta / pq_old;
-    pq = delta + 2.0 * beta * zeta + beta * beta * pq_old;
-  }
-  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-  if (!(pq > 0.0) || !isfinite(pq) || !isfinite(gamma)) {
-    if (first) {
-      sc->status = (isfinite(pq) && isfinite(gamma)) ? 2 : 3;
-      sc->done = 1;
-    }
-    return;
-  }
-  const double alpha_d = gamma / pq;
-  if (first) {
-    sc->pqs[(it + 1) & 1] = pq;
-    sc->omega = beta;
-    sc->rho = gamma;
-    sc->alpha = alpha_d;
+  double alpha_d, beta;
+  if (nB > 0) {
+    double dv[3];
+    sum_partials_wg<3>(partB, nB, dv);
+    double pq;
+    const int it = sc->iter;
+    const int st = dir_step(dv, it, it > 0 ? sc->pqs[it & 1] : 0.0, alpha_d, beta, pq);
+    if (blockIdx.x == 0 && threadIdx.x == 0) dir_record(sc, st, it, dv[0], alpha_d, beta, pq);
+    if (st) return;
+  } else {
+    // mgcg_dir_scalars formed them (many partials: one workgroup sums them once)
+    alpha_d = sc->alpha;
+    beta = sc->omega;
   }
   const float alpha = (float)alpha_d, betaf = (float)beta;
   const float s0 = Dinv ? cf[0] : 0.f;
@@ -2414,9 +2448,12 @@ static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int 
   const MgLevel& F = mg->lv[0];
   // with one level the V-cycle is the dense solve of r: no smoother start to fuse
   const bool fuse = mg->nlev > 1;
+  const bool dir_fused = B.gr <= kFusedParts;
   for (int k = 0; k < poll; ++k) {
+    if (!dir_fused) mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partB, B.gr);
     mgcg_update_first<<<B.ge, kMgBlock, 0, s>>>(F.g.n, B.z, B.w, B.pv, B.q, B.x, B.r, fuse ? F.Dinv : nullptr,
-                                                coef_of(mg, 0), F.da, B.z0, B.sc, B.partB, B.gr, B.partA);
+                                                coef_of(mg, 0), F.da, B.z0, B.sc, B.partB,
+                                                dir_fused ? B.gr : 0, B.partA);
     mgcg_conv_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partA, B.ge);
     float* z = nullptr;
     if (vcycle(mg, B.r, B.z0, B.z1, &z, B.sc, s, k == timed_k, false, fuse)) return -2;
