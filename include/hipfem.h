@@ -99,6 +99,14 @@ int hf_cell_param_vjp_f64(int32_t ele, const hf_material* mat, const double* poi
 /* R[node] = sum over cells of elem_res in cell order (+ f_ext if not NULL) */
 int hf_gather_residual_f64(const hf_pattern* p, const double* elem_res, const double* f_ext,
                            double* R, hf_stream_t stream);
+/* hf_gather_residual_f64 on a HEX8 vec-3 box lattice of nx*ny*nz nodes: the
+ * contributions follow from the coordinates (corner: host, 8 ints, local index of
+ * corner (dx, dy, dz) at 4 dx + 2 dy + dz, as hf_mg_assemble_hex8_f32's; the caller
+ * vouches that every cell is the lattice's), added in the same order: bit-identical.
+ * No reference counterpart beyond the scatter-add it shares with
+ * hf_gather_residual_f64 (problem.py:426-437). */
+int hf_gather_residual_box_f64(const hf_pattern* p, int32_t nx, int32_t ny, int32_t nz, const int32_t* corner,
+                               const double* elem, const double* fext, double* R, hf_stream_t stream);
 /* Matrix values live in a sliced-ELL layout over the pattern's node-pair blocks
  * (slices of 21 node rows for vec 3, 64 for vec 1; each slice padded to its
  * longest row): hf_matrix_values() gives the element count of a value buffer.

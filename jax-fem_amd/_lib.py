@@ -52,6 +52,7 @@ _SIGS = {
     "hf_cell_jacobian_f32": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
     "hf_cell_param_vjp_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp, vp]),
     "hf_gather_residual_f64": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_gather_residual_box_f64": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp, vp, vp, vp]),
     "hf_matrix_values": (c_i64, [vp]),
     "hf_assemble_f64": (c_i32, [vp, vp, vp, vp]),
     "hf_pattern_max_row_blocks": (c_i32, [vp]),

@@ -357,6 +357,43 @@ __global__ void __launch_bounds__(256) gather_kernel(const int32_t* __restrict__
   }
 }
 
+// The same sums on a HEX8 vec-3 box lattice, thread per node: the node's cells
+// and its local corner in each follow from its coordinates (cell (i, j, k) =
+// (i*(ny-1) + j)*(nz-1) + k, corner (dx, dy, dz) at local index ct[4dx+2dy+dz]),
+// so its 8 contributions load in one round; added in ascending cell order, the
+// order of the pattern's contribution lists, so the result is bit-identical.
+struct GatherCorners {
+  int8_t idx[8];
+};
+__global__ void __launch_bounds__(256) gather_box_kernel(int nx, int ny, int nz, GatherCorners ct,
+                                                         const double* __restrict__ elem,
+                                                         const double* __restrict__ fext, double* __restrict__ R) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nnode = (int64_t)nx * ny * nz;
+  if (n >= nnode) return;
+  const int z = (int)(n % nz), y = (int)((n / nz) % ny), x = (int)(n / ((int64_t)nz * ny));
+  const int ncx = nx - 1, ncy = ny - 1, ncz = nz - 1;
+  double acc[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) acc[i] = fext ? fext[3 * n + i] : 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int cx = x - 1 + a, cy = y - 1 + b, cz = z - 1 + c;
+        if (cx < 0 || cx >= ncx || cy < 0 || cy >= ncy || cz < 0 || cz >= ncz) continue;
+        const int64_t cell = ((int64_t)cx * ncy + cy) * ncz + cz;
+        const int loc = ct.idx[(x - cx) * 4 + (y - cy) * 2 + (z - cz)];
+        const double* src = elem + (cell * 8 + loc) * 3;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i] += src[i];
+      }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) R[3 * n + i] = acc[i];
+}
+
 template <typename T>
 __global__ void bc_rows_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ slice_ptr,
                                const int32_t* __restrict__ sell_pos, const int32_t* __restrict__ dpos,
@@ -660,6 +697,27 @@ extern "C" int hf_gather_residual_f64(const hf_pattern* p, const double* elem, c
         p->diag, p->seg, p->contrib, p->nnode, p->nn, elem, fext, R);
   else
     HF_REQUIRE(false, "vec=%d not supported", p->vec);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int hf_gather_residual_box_f64(const hf_pattern* p, int32_t nx, int32_t ny, int32_t nz,
+                                          const int32_t* corner, const double* elem, const double* fext, double* R,
+                                          hf_stream_t s) {
+  HF_REQUIRE(p && corner && elem && R, "hf_gather_residual_box_f64: null argument");
+  HF_REQUIRE(p->vec == 3 && p->nn == 8, "hf_gather_residual_box_f64: needs a HEX8 vec-3 pattern");
+  HF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2 && (int64_t)nx * ny * nz == p->nnode &&
+                 (int64_t)(nx - 1) * (ny - 1) * (nz - 1) == p->ncell,
+             "hf_gather_residual_box_f64: %dx%dx%d is not the pattern's lattice", nx, ny, nz);
+  GatherCorners ct;
+  bool seen[8] = {false, false, false, false, false, false, false, false};
+  for (int q = 0; q < 8; ++q) {
+    HF_REQUIRE(corner[q] >= 0 && corner[q] < 8 && !seen[corner[q]],
+               "hf_gather_residual_box_f64: corner table is not a permutation of 0..7");
+    seen[corner[q]] = true;
+    ct.idx[q] = (int8_t)corner[q];
+  }
+  gather_box_kernel<<<grid_for(p->nnode), 256, 0, (hipStream_t)s>>>(nx, ny, nz, ct, elem, fext, R);
   HF_LAUNCH_CHECK();
   return 0;
 }

@@ -251,8 +251,28 @@ class Problem:
                   self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(er), s)
         R = torch.empty(self.N, dtype=torch.float64, device=self.device)
         fext = self._external()
-        _lib.call("hf_gather_residual_f64", self.pattern.handle, _lib.ptr(er), _lib.ptr(fext), _lib.ptr(R), s)
+        box = self._gather_box()
+        if box is not None:   # a box_mesh: the cells of each node from its coordinates (same sums, same order)
+            (nx, ny, nz), corner = box
+            _lib.call("hf_gather_residual_box_f64", self.pattern.handle, nx, ny, nz, corner, _lib.ptr(er),
+                      _lib.ptr(fext), _lib.ptr(R), s)
+        else:
+            _lib.call("hf_gather_residual_f64", self.pattern.handle, _lib.ptr(er), _lib.ptr(fext), _lib.ptr(R), s)
         return R
+
+    def _gather_box(self):
+        """((nx, ny, nz), corner table) of a HEX8 vec-3 box_mesh, whose residual gathers by
+        lattice coordinates; None otherwise (checked once)."""
+        if not hasattr(self, "_gbox"):
+            self._gbox = None
+            g = getattr(self.fe.mesh, "node_grid", None)
+            if self.fe.ele_type == "HEX8" and self.vec == 3 and g is not None and \
+                    int(np.prod(g)) == self.fe.num_total_nodes:
+                grid = tuple(int(v) for v in g)
+                corner = self._box_corners(grid)
+                if corner is not None:
+                    self._gbox = (grid, corner)
+        return self._gbox
 
     def jacobian_bsr(self, u, dtype=torch.float64, bc=True):
         """Tangent values (the library's sliced-ELL layout of node-pair blocks); Dirichlet rows -> identity rows."""

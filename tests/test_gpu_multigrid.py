@@ -568,3 +568,33 @@ def test_mg_solve_edge_cases(jfa, cells):
     assert iz.status == 0 and iz.iterations == 0 and bool((xz == 0).all())
     xm, im = mg.solve(A, b, rtol=1e-6, atol=0.0, maxiter=0, rebuild="never")
     assert im.status == 1 and im.iterations == 0 and bool((xm == 0).all())
+
+
+@pytest.mark.parametrize("load", [False, True])
+def test_box_residual_gather_bit_exact(jfa, load):
+    """On a box_mesh the residual is gathered by lattice coordinates
+    (hf_gather_residual_box_f64); it equals the pattern-list gather bit for bit,
+    with and without an external load vector."""
+    say("box residual gather", "load" if load else "no load")
+    _lib = __import__("jax_fem_amd._lib", fromlist=["x"])
+    mat = jfa.NeoHookean(E=10.0, nu=0.3)
+
+    class P(jfa.Problem):
+        def get_tensor_map(self):
+            return mat
+
+        def get_surface_maps(self):
+            return [lambda u, x: np.array([0.0, 0.3, -1.0])] if load else []
+
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-7)
+    prob = P(jfa.box_mesh(5, 4, 6), vec=3, dim=3, location_fns=[right] if load else None)
+    assert prob._gather_box() is not None
+    g = torch.Generator(device=DEV).manual_seed(9)
+    u = 0.01 * torch.randn(prob.N, generator=g, device=DEV, dtype=torch.float64)
+    R_box = prob.residual_vec(u)
+    fext = prob._external()
+    assert (fext is not None) == load
+    R_gen = torch.empty_like(R_box)
+    _lib.call("hf_gather_residual_f64", prob.pattern.handle, _lib.ptr(prob._bufs["elem_res"]), _lib.ptr(fext),
+              _lib.ptr(R_gen), _lib.stream())
+    assert torch.equal(R_box, R_gen)
