@@ -280,10 +280,14 @@ int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf_stream_t s
  * it is captured again when the hierarchy is set up again or the pattern,
  * values pointer, workspace or poll change.  HF_MG_SOLVE_EAGER enqueues every
  * launch instead (the same kernels in the same order: identical results).
+ * The direction scalars are summed inside the update kernel while the operator
+ * grid has at most 2048 workgroups, by a one-workgroup kernel above that;
+ * HF_MG_SOLVE_SPLIT_SCALARS takes the second form at every size (same
+ * iterates up to the summation order of those scalars).
  * Calls that share one hf_mg must not overlap.  info is final on return; when
  * the solve converged, the copy of the iterate into x may still be queued on
  * `stream` (work enqueued behind it on that stream sees x). */
-enum { HF_MG_SOLVE_EAGER = 4 };
+enum { HF_MG_SOLVE_EAGER = 4, HF_MG_SOLVE_SPLIT_SCALARS = 8 };
 size_t hf_mg_workspace_size(const hf_mg* mg);
 int hf_mg_solve_f32(hf_mg* mg, const hf_pattern* p, const float* vals, const float* b, float* x, int32_t flags,
                     int32_t maxiter, double rtol, double atol, int32_t poll, void* workspace,

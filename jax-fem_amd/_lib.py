@@ -22,6 +22,7 @@ HF_MG_JACOBI, HF_MG_CHEBYSHEV = 0, 1
 HF_MG_FINE_FULL, HF_MG_FINE_SELL = 1, 2
 HF_MG_SKIP_FINE_POST = 1
 HF_MG_SOLVE_EAGER = 4
+HF_MG_SOLVE_SPLIT_SCALARS = 8
 
 
 class HFMaterial(ctypes.Structure):

@@ -1894,7 +1894,7 @@ struct hf_mg {
   uint64_t gen = 0;                 // bumped when the smoother's kind or degree changes
   hipStream_t cap = nullptr;        // capture stream (non-blocking, default priority)
   hipGraphExec_t gexec = nullptr;
-  uintptr_t gkey[7] = {0, 0, 0, 0, 0, 0, 0};
+  uintptr_t gkey[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 static const float* coef_of(const hf_mg* mg, int l) { return mg->coef + (int64_t)l * kCoefStride; }
@@ -2440,6 +2440,7 @@ struct MgBatch {
   int64_t N;
   float *r, *z0, *z1, *w, *pv, *q, *x;
   float* z;  // the V-cycle's output buffer (the same one every cycle)
+  bool split;  // direction scalars by their own one-workgroup kernel at every grid size
   Scalars* sc;
   double *partA, *partB;
   int gr, ge;
@@ -2448,7 +2449,7 @@ static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int 
   const MgLevel& F = mg->lv[0];
   // with one level the V-cycle is the dense solve of r: no smoother start to fuse
   const bool fuse = mg->nlev > 1;
-  const bool dir_fused = B.gr <= kFusedParts;
+  const bool dir_fused = B.gr <= kFusedParts && !B.split;
   for (int k = 0; k < poll; ++k) {
     if (!dir_fused) mgcg_dir_scalars<<<1, kScalarBlock, 0, s>>>(B.sc, B.partB, B.gr);
     mgcg_update_first<<<B.ge, kMgBlock, 0, s>>>(F.g.n, B.z, B.w, B.pv, B.q, B.x, B.r, fuse ? F.Dinv : nullptr,
@@ -2471,10 +2472,10 @@ static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int 
 // contents: the set-up generation (level coefficients are kernel arguments),
 // the tangent's values and pattern, the workspace and the batch length.
 static int mg_batch_graph(hf_mg* mg, const hf_pattern* p, MgBatch& B, void* ws, int poll) {
-  const uintptr_t key[7] = {(uintptr_t)mg->gen, (uintptr_t)p, (uintptr_t)(B.S ? B.S : B.vals), (uintptr_t)ws,
-                            (uintptr_t)poll, (uintptr_t)B.N, (uintptr_t)mg->mask};
+  const uintptr_t key[8] = {(uintptr_t)mg->gen, (uintptr_t)p, (uintptr_t)(B.S ? B.S : B.vals), (uintptr_t)ws,
+                            (uintptr_t)poll, (uintptr_t)B.N, (uintptr_t)mg->mask, (uintptr_t)B.split};
   bool same = mg->gexec != nullptr;
-  for (int i = 0; i < 7 && same; ++i) same = mg->gkey[i] == key[i];
+  for (int i = 0; i < 8 && same; ++i) same = mg->gkey[i] == key[i];
   if (same) return 0;
   if (!mg->cap) HF_CHECK_HIP(hipStreamCreateWithFlags(&mg->cap, hipStreamNonBlocking));
   if (mg->gexec) {
@@ -2490,7 +2491,7 @@ static int mg_batch_graph(hf_mg* mg, const hf_pattern* p, MgBatch& B, void* ws, 
   const hipError_t ei = hipGraphInstantiate(&mg->gexec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   HF_CHECK_HIP(ei);
-  for (int i = 0; i < 7; ++i) mg->gkey[i] = key[i];
+  for (int i = 0; i < 8; ++i) mg->gkey[i] = key[i];
   return 0;
 }
 
@@ -2520,6 +2521,7 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   B.r = v; B.z0 = v + stride; B.z1 = v + 2 * stride; B.w = v + 3 * stride; B.pv = v + 4 * stride;
   B.q = v + 5 * stride;
   B.x = v + 6 * stride;  // the iterate lives in the workspace, so a captured batch outlives the caller's x
+  B.split = (flags & HF_MG_SOLVE_SPLIT_SCALARS) != 0;
   B.sc = sc;
   B.partA = partA;
   B.partB = partB;

@@ -250,11 +250,13 @@ class Multigrid:
         return out.reshape(27, 3, 3, n)
 
     def solve(self, A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=500, rel_r0=False, poll=1, stats=None,
-              mask=None, problem=None, rebuild="always", token=None, eager=False):
+              mask=None, problem=None, rebuild="always", token=None, eager=False, split_scalars=False):
         """Flexible CG preconditioned by V-cycles; the operator is always the FP32
         tangent A, the hierarchy is rebuilt from A as `rebuild` says (needs_setup).
         Each batch of `poll` iterations replays a captured HIP graph (eager=True
-        launches every kernel instead).  Returns (x float32, info)."""
+        launches every kernel instead).  split_scalars: the direction scalars by
+        their own one-workgroup kernel at every size (the form large grids take).
+        Returns (x float32, info)."""
         if self.needs_setup(A, rebuild, token):
             self.setup(A, dirichlet_mask(problem or A.problem) if mask is None else mask)
             self._token = token
@@ -265,7 +267,8 @@ class Multigrid:
         ws = _workspace(b.device, int(_lib.load().hf_mg_workspace_size(self.handle)))
         info = _lib.HFKrylovInfo()
         t0 = time.perf_counter()
-        flags = (2 if rel_r0 else 0) | (_lib.HF_MG_SOLVE_EAGER if eager else 0)
+        flags = (2 if rel_r0 else 0) | (_lib.HF_MG_SOLVE_EAGER if eager else 0) | \
+            (_lib.HF_MG_SOLVE_SPLIT_SCALARS if split_scalars else 0)
         if isinstance(A, LatticeTangent):
             A.check_current()
             _lib.call("hf_mg_solve_stencil_f32", self.handle, _lib.ptr(A.stencil), _lib.ptr(b), _lib.ptr(x), flags,

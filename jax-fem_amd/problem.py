@@ -33,6 +33,23 @@ from .materials import Material
 from .mesh import Mesh
 
 
+def _fingerprint(obj):
+    """Content key of a nest of lists/tuples of arrays: torch tensors by (storage
+    pointer, version counter, shape) -- an in-place edit bumps the version --,
+    NumPy arrays and scalars by a digest of their bytes."""
+    import hashlib
+    if obj is None:
+        return None
+    if isinstance(obj, (list, tuple)):
+        return tuple(_fingerprint(o) for o in obj)
+    if torch.is_tensor(obj):
+        return ("t", obj.data_ptr(), obj._version, tuple(obj.shape), str(obj.dtype), str(obj.device))
+    a = np.ascontiguousarray(np.asarray(obj))
+    if a.dtype == object:
+        return ("o", id(obj))
+    return ("a", a.shape, a.dtype.str, hashlib.blake2b(a.view(np.uint8).reshape(-1), digest_size=16).digest())
+
+
 @dataclass
 class Problem:
     mesh: Mesh
@@ -169,8 +186,7 @@ class Problem:
     def bc_arrays(self):
         """Unique BC DOFs (int32, device) and values (f64, device); later specs win (solver.py:298-300)."""
         fe = self.fe
-        key = tuple((id(v), len(v), float(v.detach().sum()) if torch.is_tensor(v) else float(np.sum(v)))
-                    for v in fe.vals_list) + tuple(id(n) for n in fe.node_inds_list)
+        key = (_fingerprint(fe.vals_list), tuple(id(n) for n in fe.node_inds_list))
         if self._bc_key != key:
             # the DOF set depends on the node arrays and components only; a load step
             # that changes the values alone reuses it (location functions are cached
@@ -197,14 +213,20 @@ class Problem:
             v = np.concatenate(vs)[::-1][self._bc_first]
             self._bc_vals = torch.as_tensor(v, device=self.device)
             self._bc_key = key
+            self._bc_refs = (list(fe.vals_list), list(fe.node_inds_list))   # ids in the key stay unique
         return self._bc_dofs, self._bc_vals
 
     # ------------------------------------------------------------ load vector
     def _external(self):
         """u-independent mass and surface loads (problem.py:216-259), f64 device vector or None."""
-        key = (id(self.internal_vars_surfaces), id(self.internal_vars))
+        # the load depends on the surface maps' internal variables and, through a
+        # mass map, on the volume ones: keyed on their contents (torch tensors by
+        # storage and version counter, arrays by a digest of their bytes)
+        key = (_fingerprint(self.internal_vars_surfaces),
+               _fingerprint(self.internal_vars) if hasattr(self, "get_mass_map") else None)
         if self._ext_key == key:
             return self._fext
+        self._ext_refs = (self.internal_vars_surfaces, self.internal_vars)   # keeps ids in the key unique
         fe = self.fe
         f = np.zeros((fe.num_total_nodes, self.vec))
         have = False
@@ -212,6 +234,10 @@ class Problem:
             mmap = self.get_mass_map()
             x = fe.get_physical_quad_points().reshape(-1, self.dim)
             m = _eval_points(lambda xx: mmap(np.zeros(self.vec), xx), x).reshape(self.num_cells, fe.num_quads, -1)
+            u1 = np.random.default_rng(1).standard_normal((min(3, len(x)), self.vec))
+            m1 = np.stack([np.asarray(mmap(u1[i], x[i]), dtype=np.float64).reshape(-1) for i in range(len(u1))])
+            if not np.allclose(m1, m.reshape(len(x), -1)[:len(m1)]):
+                raise NotImplementedError("mass maps that depend on u have no HIP kernel yet")
             JxW = fe.JxW()
             r = np.einsum("cqi,qn,cq->cni", m, fe.shape_vals, JxW)
             np.add.at(f, fe.cells.reshape(-1), r.reshape(-1, self.vec))

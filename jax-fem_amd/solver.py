@@ -371,7 +371,12 @@ def solver(problem, solver_options={}):
     eta_prev = None
     A = None
     linear_law = bool(getattr(problem.material, "linear", False))
-    while (res / res0 > rel_tol) and (res > tol) and n < max_iter:
+    def rel_of(r):
+        # ||R||/||R0||; a zero initial residual gives NaN as the reference's NumPy
+        # division does, so the loop below does not start (solver.py:1283-1295)
+        return r / res0 if res0 > 0.0 else float("nan")
+
+    while (rel_of(res) > rel_tol) and (res > tol) and n < max_iter:
         n += 1
         # the tangent's time is read from events after the solve: no stall between
         # the assembly and the linear solve the host enqueues behind it
@@ -435,14 +440,14 @@ def solver(problem, solver_options={}):
             u, R, res = u_new, R_new, res_new
         hist.append(res)
         last = stats.calls[-1] if stats.calls else {}
-        logger.debug("newton %d: res %.3e rel %.3e | eta %s krylov %s status %s", n, res, res / res0,
+        logger.debug("newton %d: res %.3e rel %.3e | eta %s krylov %s status %s", n, res, rel_of(res),
                      "%.2e" % eta if eta is not None else "-", last.get("iterations"), last.get("status"))
     if not (np.isfinite(res) and bool(torch.isfinite(u).all())):
         raise FloatingPointError("Newton produced non-finite values")
     torch.cuda.synchronize()
     timing["global_matrix"] = sum(a.elapsed_time(b) for a, b in asm_events) * 1e-3
     timing["linear"] = sum(a.elapsed_time(b) for a, b in lin_events) * 1e-3
-    problem.solve_info = {"newton_iters": n, "res_hist": hist, "res0": res0, "rel": res / res0,
+    problem.solve_info = {"newton_iters": n, "res_hist": hist, "res0": res0, "rel": res / res0 if res0 > 0.0 else 0.0,
                           "krylov": stats.calls, "krylov_iters": stats.iterations,
                           "fallbacks": sum(1 for c in stats.calls if c.get("fallback")),
                           "timing": timing, "wall": time.perf_counter() - wall0, "precision": precision}

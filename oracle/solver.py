@@ -225,22 +225,48 @@ def linear_solve(A, b, x0, method="bicgstab", tol=1e-10, atol=1e-10, maxiter=100
     return x, k
 
 
+def line_search(prob, u, inc):
+    """line_search (solver.py:424-464): from alpha = 1, halve up to 3 times while
+    the BC-applied residual norm keeps decreasing; returns (alpha, new u)."""
+    def norm_at(alpha):
+        v = u + alpha * inc
+        return float(np.linalg.norm(prob.apply_bc_vec(prob.compute_residual(v), v)))
+    alpha = 1.0
+    res = norm_at(alpha)
+    for _ in range(3):
+        alpha *= 0.5
+        half = norm_at(alpha)
+        if half > res:
+            alpha *= 2.0
+            break
+        res = half
+    return alpha, u + alpha * inc
+
+
 def newton_solve(prob, u0=None, rel_tol=1e-8, tol=1e-6, method="bicgstab", lin_tol=1e-10,
-                 max_newton=50, log=None):
-    """solver() Newton loop (solver.py:1247-1318). Returns (u, info dict)."""
+                 max_newton=50, log=None, line_search_flag=False):
+    """solver() Newton loop (solver.py:1247-1318), with the optional line search of
+    newton_step (solver.py:416-417). Returns (u, info dict)."""
     u = np.zeros(prob.N) if u0 is None else np.asarray(u0, dtype=np.float64).copy()
     R = prob.apply_bc_vec(prob.compute_residual(u), u)
     A = prob.assemble(u)
     res0 = res = float(np.linalg.norm(R))
     hist = [res]
     lin_iters = []
+    alphas = []
     n = 0
-    while res / res0 > rel_tol and res > tol and n < max_newton:
+    with np.errstate(invalid="ignore", divide="ignore"):
+        first = np.float64(res) / np.float64(res0)    # NaN for a zero start: no step, as the reference
+    while (first if n == 0 else res / res0) > rel_tol and res > tol and n < max_newton:
         n += 1
         x0 = prob.assign_bc(np.zeros(prob.N)) - prob.copy_bc(u)
         inc, k = linear_solve(A, -R, x0, method, lin_tol, lin_tol)
         lin_iters.append(k)
-        u = u + inc
+        if line_search_flag:
+            alpha, u = line_search(prob, u, inc)
+            alphas.append(alpha)
+        else:
+            u = u + inc
         R = prob.apply_bc_vec(prob.compute_residual(u), u)
         A = prob.assemble(u)
         res = float(np.linalg.norm(R))
@@ -248,7 +274,7 @@ def newton_solve(prob, u0=None, rel_tol=1e-8, tol=1e-6, method="bicgstab", lin_t
         if log:
             log("newton %d res %.3e rel %.3e lin_iters %d" % (n, res, res / res0, k))
     assert np.isfinite(res) and np.all(np.isfinite(u))
-    return u, {"newton_iters": n, "res_hist": hist, "lin_iters": lin_iters}
+    return u, {"newton_iters": n, "res_hist": hist, "lin_iters": lin_iters, "alphas": alphas}
 
 
 def implicit_vjp(prob, u, v, method="bicgstab", dirichlet_param_grads=None):

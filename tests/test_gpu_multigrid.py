@@ -598,3 +598,63 @@ def test_box_residual_gather_bit_exact(jfa, load):
     _lib.call("hf_gather_residual_f64", prob.pattern.handle, _lib.ptr(prob._bufs["elem_res"]), _lib.ptr(fext),
               _lib.ptr(R_gen), _lib.stream())
     assert torch.equal(R_box, R_gen)
+
+
+@pytest.mark.parametrize("kind", ["stencil", "hex27"])
+def test_split_direction_scalars_match_fused(jfa, kind):
+    """The one-workgroup direction-scalar kernel (the form operators of more than
+    2048 workgroups take: c2, c4) against the fused form on a small lattice: same
+    status and iteration count, iterates equal to FP32 rounding, FP64 residual at
+    the target; a breakdown (status 2: the hierarchy of A preconditioning -A, so
+    p.q < 0) and a non-finite product (status 3: one value set to 3e38) are reported
+    by both forms."""
+    say("mg split vs fused direction scalars", kind)
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    S = __import__("jax_fem_amd.solver", fromlist=["x"])
+    if kind == "hex27":
+        prob = _hex27_problem(jfa, (3, 2, 3), jfa.NeoHookean(E=10.0, nu=0.3))
+        mg = MG.Multigrid(MG.lattice(prob), fine_sell=True)
+        A = tangent(jfa, prob, scale=0.005)
+    else:
+        prob = box_problem(jfa, (9, 6, 7))
+        mg = MG.Multigrid(MG.lattice(prob))
+        g0 = torch.Generator(device=DEV).manual_seed(3)
+        u = 0.002 * torch.randn(prob.N, generator=g0, device=DEV, dtype=torch.float64)
+        A = MG.LatticeTangent(prob, u, mg)
+    mask = MG.dirichlet_mask(prob)
+    mg.setup(A, mask)
+    g = torch.Generator(device=DEV).manual_seed(6)
+    b = torch.randn(prob.N, generator=g, device=DEV) * mask
+    out = {}
+    for split in (False, True):
+        for eager in (False, True):
+            x, info = mg.solve(A, b, rtol=1e-6, atol=0.0, maxiter=200, rebuild="never", eager=eager,
+                               split_scalars=split)
+            out[(split, eager)] = (x, info.status, info.iterations)
+    for key, (x, st, it) in out.items():
+        assert st == 0, (key, st)
+    xf, _, itf = out[(False, False)]
+    xs, _, its = out[(True, False)]
+    assert abs(itf - its) <= 1, (itf, its)
+    assert torch.equal(out[(True, True)][0], xs)          # replayed = eager, split form too
+    rel = (torch.linalg.norm(xs - xf) / torch.linalg.norm(xf)).item()
+    assert rel <= 1e-5, rel
+    A64 = (A.to_sell() if kind == "stencil" else A).to_scipy()
+    bb = b.double().cpu().numpy()
+    res = np.linalg.norm(bb - A64 @ xs.double().cpu().numpy()) / np.linalg.norm(bb)
+    assert res <= 1e-5, res
+    # breakdown and non-finite values, both forms (SELL operator)
+    As = A.to_sell() if kind == "stencil" else A
+    An = S.TangentMatrix(prob, -As.vals)
+    big = As.vals.clone()
+    big[torch.argmax(big.abs())] = 3e38
+    Ab = S.TangentMatrix(prob, big)
+    for split in (False, True):
+        # a failed solve marks the hierarchy stale: set it up from A again before each
+        mg.setup(As, mask)
+        _, i2 = mg.solve(An, b, rtol=1e-6, atol=0.0, maxiter=50, rebuild="never", split_scalars=split)
+        assert i2.status == 2, (split, i2.status, i2.iterations)
+        mg.setup(As, mask)
+        # right-hand side scaled so that A z overflows FP32 at that entry
+        _, i3 = mg.solve(Ab, 1e6 * b, rtol=1e-6, atol=0.0, maxiter=50, rebuild="never", split_scalars=split)
+        assert i3.status == 3, (split, i3.status, i3.iterations)

@@ -50,7 +50,7 @@ def make_problem(jfa, mesh, material, vec=3, dirichlet=None, location_fns=None, 
 
 MATS = [
     ("linear_elastic", None), ("neo_hookean", None), ("neo_hookean", "scale_E"),
-    ("linear_elastic", "simp"), ("neo_hookean", "mu"),
+    ("linear_elastic", "scale_E"), ("linear_elastic", "simp"), ("neo_hookean", "mu"),
 ]
 
 

@@ -176,3 +176,36 @@ def test_cnn_oracle_tap_convention():
     y = ocnn.conv(torch.as_tensor(x)[None], ocnn.to_torch_weight(W), torch.as_tensor(b), grid)[0].numpy()
     for n in (0, 7, 31, nn - 1):
         np.testing.assert_allclose(y[n], ocnn.conv_direct_node(x, W, b, grid, n), rtol=1e-12, atol=1e-12)
+
+
+def test_line_search_rule_and_root():
+    """The oracle's line search (solver.py:424-464) on a tip-loaded Neo-Hookean
+    cantilever: the step length is the first of 1, 1/2, 1/4, 1/8 after which the
+    next halving raises the residual norm (checked by evaluating all four norms),
+    and the damped Newton reaches the same root as the undamped one."""
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-5)
+    pts, cells = fe.box_mesh(3, 3, 3)
+    op = osv.OracleProblem(pts, cells, 3, kernels.Material("neo_hookean", E=10.0, nu=0.3),
+                           dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3], location_fns=[right],
+                           surface_maps=[lambda x: np.broadcast_to(np.array([0.0, 0.0, 1.0]), x.shape)])
+    u = np.zeros(op.N)
+    R = op.apply_bc_vec(op.compute_residual(u), u)
+    A = op.assemble(u)
+    inc, _ = osv.linear_solve(A, -R, op.assign_bc(np.zeros(op.N)) - op.copy_bc(u), "spsolve")
+    alpha, unew = osv.line_search(op, u, inc)
+    with np.errstate(invalid="ignore"):
+        norms = [np.linalg.norm(op.apply_bc_vec(op.compute_residual(u + a * inc), u + a * inc))
+                 for a in (1.0, 0.5, 0.25, 0.125)]
+    expect = 0.125
+    for k in range(3):
+        if norms[k + 1] > norms[k]:
+            expect = 0.5 ** k
+            break
+    assert alpha == expect and alpha < 1.0
+    np.testing.assert_array_equal(unew, u + alpha * inc)
+    with np.errstate(invalid="ignore"):
+        u1, i1 = osv.newton_solve(op, method="spsolve", line_search_flag=True)
+        u0, _ = osv.newton_solve(op, method="spsolve")
+    assert min(i1["alphas"]) < 1.0
+    assert np.linalg.norm(u1 - u0) / np.linalg.norm(u0) < 1e-8
