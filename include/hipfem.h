@@ -176,6 +176,26 @@ int hf_krylov_solve_f32(int32_t method, const hf_pattern* p, const float* vals, 
                         double rtol, double atol, int32_t poll, void* workspace,
                         hf_krylov_info* info, hf_stream_t stream);
 
+/* ---- allocation-free Krylov solve.  No counterpart in the reference: it is
+ *      for callers that cannot allocate device memory (code under HIP graph
+ *      capture, tight serving loops, harnesses that forbid hipMalloc).
+ * hf_init(): the library's set-up call for the current device.  Once per
+ * device, with that device current: resolves the device address of the
+ * library's static workspace array (64 MiB, held by every GPU the library runs
+ * on until the process ends) and creates that device's poll state (two pinned
+ * scalar slots, two events).  Idempotent; not thread-safe against itself.
+ * hf_krylov_solve_noalloc_f64(): hf_krylov_solve_f64 (same methods, flags,
+ * stop rule, status and bits) with that static array as the workspace; it calls
+ * no hipMalloc / hipFree / allocating routine.  Fails (-1) unless hf_init() has
+ * run on the current device, or when hf_krylov_workspace_size(method, N, 8)
+ * exceeds 64 MiB (PCG: N <= ~1.67 M DOF; BiCGSTAB: N <= ~1.04 M DOF).  Every
+ * call on one GPU shares that GPU's array and its poll state: calls that
+ * overlap on one GPU (other streams or host threads) race. */
+int hf_init(void);
+int hf_krylov_solve_noalloc_f64(int32_t method, const hf_pattern* p, const double* vals, const double* dinv,
+                                const double* b, double* x, int32_t flags, int32_t maxiter, double rtol,
+                                double atol, int32_t poll, hf_krylov_info* info, hf_stream_t stream);
+
 /* ---- learned preconditioner (no reference counterpart; the reference only has
  *      Jacobi, solver.py:68-69): a 3x3x3 convolutional network on the structured
  *      node grid (nx, ny, nz nodes, box_mesh order n = (i*ny + j)*nz + k,

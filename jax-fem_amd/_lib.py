@@ -76,6 +76,9 @@ _SIGS = {
                                     ctypes.POINTER(HFKrylovInfo), vp]),
     "hf_krylov_solve_f32": (c_i32, [c_i32, vp, vp, vp, vp, vp, c_i32, c_i32, c_f64, c_f64, c_i32, vp,
                                     ctypes.POINTER(HFKrylovInfo), vp]),
+    "hf_init": (c_i32, []),
+    "hf_krylov_solve_noalloc_f64": (c_i32, [c_i32, vp, vp, vp, vp, vp, c_i32, c_i32, c_f64, c_f64, c_i32,
+                                            ctypes.POINTER(HFKrylovInfo), vp]),
     "hf_profile_enable": (c_i32, [c_i32]),
     "hf_profile_read": (c_i32, [ctypes.POINTER(c_i64), ctypes.POINTER(c_f64)]),
     "hf_cnn_conv3d_fwd_f32": (c_i32, [vp, c_i32, vp, vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, vp, vp]),

@@ -413,10 +413,6 @@ int prof_collect() {
 // scalars are copied to pinned host memory behind an event; the host launches
 // the next batch before it waits for the previous batch's copy, so the GPU
 // always has a batch queued.  One pinned pair + events per host thread.
-struct PollState {
-  Scalars* host = nullptr;  // 2 slots, pinned
-  hipEvent_t ev[2] = {nullptr, nullptr};
-};
 static thread_local PollState g_poll;
 
 int poll_init() {
@@ -427,14 +423,16 @@ int poll_init() {
   }
   return 0;
 }
-int poll_post(const Scalars* sc, int slot, hipStream_t s) {
-  HF_CHECK_HIP(hipMemcpyAsync(&g_poll.host[slot], sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
-  HF_CHECK_HIP(hipEventRecord(g_poll.ev[slot], s));
+int poll_post(const Scalars* sc, int slot, hipStream_t s, PollState* ps) {
+  PollState& P = ps ? *ps : g_poll;
+  HF_CHECK_HIP(hipMemcpyAsync(&P.host[slot], sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  HF_CHECK_HIP(hipEventRecord(P.ev[slot], s));
   return 0;
 }
-int poll_wait(int slot, Scalars& out) {
-  HF_CHECK_HIP(hipEventSynchronize(g_poll.ev[slot]));
-  out = g_poll.host[slot];
+int poll_wait(int slot, Scalars& out, PollState* ps) {
+  PollState& P = ps ? *ps : g_poll;
+  HF_CHECK_HIP(hipEventSynchronize(P.ev[slot]));
+  out = P.host[slot];
   return 0;
 }
 
@@ -459,7 +457,8 @@ static int spmv_dispatch(const hf_pattern* p, const T* vals, const T* x, T* y, h
 
 template <typename T, int VEC, bool TRANS>
 static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b, T* x, int maxiter, double rtol,
-                   double atol, int poll, int rel_r0, void* ws, hf_krylov_info* info, hipStream_t s) {
+                   double atol, int poll, int rel_r0, void* ws, hf_krylov_info* info, hipStream_t s,
+                   PollState* ps) {
   const MatArgs m = mat_args(p);
   const int64_t N = m.nrows * VEC;
   Scalars* sc = (Scalars*)ws;
@@ -474,14 +473,16 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
   cg_spmv_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, u, w, nullptr, partB);
   cg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, gr, partB, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   HF_LAUNCH_CHECK();
-  if (poll_init()) return -2;
+  // ps != nullptr: the caller's poll state (no timing: it would create events)
+  const bool prof_ok = ps == nullptr;
+  if (!ps && poll_init()) return -2;
   Scalars host{};
   int it = 0, batch = 0;
   bool timed_pending = false;
   int timed_batch = -1;
   for (;;) {
     for (int k = 0; k < poll; ++k, ++it) {
-      const bool timed = (k == poll / 2 && !timed_pending);  // a steady-state launch, mid-batch
+      const bool timed = prof_ok && k == poll / 2 && !timed_pending;  // a steady-state launch, mid-batch
       cg_update_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, u, w, pv, sv, x, r, sc, partA);
       if (timed && g_prof) {
         hipEvent_t e0, e1;
@@ -496,10 +497,10 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
       cg_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, ge, partB, gr);
     }
     HF_LAUNCH_CHECK();
-    if (poll_post(sc, batch & 1, s)) return -2;
+    if (poll_post(sc, batch & 1, s, ps)) return -2;
     if (batch > 0) {
       // wait for the previous batch while this one runs
-      if (poll_wait((batch - 1) & 1, host)) return -2;
+      if (poll_wait((batch - 1) & 1, host, ps)) return -2;
       if (timed_pending && timed_batch == batch - 1) {
         // the timed SpMV ran in the batch just waited for; it did work unless
         // the solve had already finished before it
@@ -512,7 +513,7 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
     }
     ++batch;
     if (it >= maxiter + 2 * poll) {
-      if (poll_wait((batch - 1) & 1, host)) return -2;
+      if (poll_wait((batch - 1) & 1, host, ps)) return -2;
       break;
     }
   }
@@ -530,7 +531,8 @@ static int pcg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b
 
 template <typename T, int VEC, bool TRANS>
 static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* b, T* x, int maxiter, double rtol,
-                    double atol, int poll, int rel_r0, void* ws, hf_krylov_info* info, hipStream_t s) {
+                    double atol, int poll, int rel_r0, void* ws, hf_krylov_info* info, hipStream_t s,
+                    PollState* ps) {
   const MatArgs m = mat_args(p);
   const int64_t N = m.nrows * VEC;
   Scalars* sc = (Scalars*)ws;
@@ -543,14 +545,16 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
   bicg_init_kernel<T, VEC, TRANS><<<gr, kBlock, 0, s>>>(m, vals, b, x, r, rhat, pp, q, partials);
   bicg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   HF_LAUNCH_CHECK();
-  if (poll_init()) return -2;
+  // ps != nullptr: the caller's poll state (no timing: it would create events)
+  const bool prof_ok = ps == nullptr;
+  if (!ps && poll_init()) return -2;
   Scalars host{};
   int it = 0, batch = 0;
   bool timed_pending = false;
   int timed_batch = -1;
   for (;;) {
     for (int k = 0; k < poll; ++k, ++it) {
-      const bool timed = (k == poll / 2 && !timed_pending);  // a steady-state launch, mid-batch
+      const bool timed = prof_ok && k == poll / 2 && !timed_pending;  // a steady-state launch, mid-batch
       bicg_p_kernel<T><<<ge, kBlock, 0, s>>>(N, dinv, r, q, pp, phat, sc);
       if (timed && prof_begin(s)) return -2;
       bicg_spmv_kernel<T, VEC, TRANS, 0><<<gr, kBlock, 0, s>>>(m, vals, phat, rhat, q, sc, partials);
@@ -568,9 +572,9 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
       bicg_end_scalars<<<1, kScalarBlock, 0, s>>>(sc, partials, ge);
     }
     HF_LAUNCH_CHECK();
-    if (poll_post(sc, batch & 1, s)) return -2;
+    if (poll_post(sc, batch & 1, s, ps)) return -2;
     if (batch > 0) {
-      if (poll_wait((batch - 1) & 1, host)) return -2;
+      if (poll_wait((batch - 1) & 1, host, ps)) return -2;
       if (timed_pending && timed_batch == batch - 1) {
         // the timed SpMV ran in the batch just waited for; it did work unless
         // the solve had already finished before it
@@ -583,7 +587,7 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
     }
     ++batch;
     if (it >= maxiter + 2 * poll) {
-      if (poll_wait((batch - 1) & 1, host)) return -2;
+      if (poll_wait((batch - 1) & 1, host, ps)) return -2;
       break;
     }
   }
@@ -601,7 +605,7 @@ static int bicg_run(const hf_pattern* p, const T* vals, const T* dinv, const T* 
 template <typename T>
 static int krylov_dispatch(int method, const hf_pattern* p, const T* vals, const T* dinv, const T* b, T* x,
                            int flags, int maxiter, double rtol, double atol, int poll, void* ws,
-                           hf_krylov_info* info, hipStream_t s) {
+                           hf_krylov_info* info, hipStream_t s, PollState* ps = nullptr) {
   const int transpose = flags & HF_KRYLOV_TRANSPOSE;
   const int rel_r0 = (flags & HF_KRYLOV_REL_R0) ? 1 : 0;
   HF_REQUIRE(p && vals && dinv && b && x && ws, "hf_krylov_solve: null argument");
@@ -610,7 +614,7 @@ static int krylov_dispatch(int method, const hf_pattern* p, const T* vals, const
   const int vec = pattern_vec(p);
   HF_REQUIRE(vec == 1 || vec == 3, "hf_krylov_solve: vec=%d not supported", vec);
   HF_REQUIRE(method == HF_KRYLOV_PCG || method == HF_KRYLOV_BICGSTAB, "hf_krylov_solve: unknown method %d", method);
-#define KRY(M, V, TR) M<T, V, TR>(p, vals, dinv, b, x, maxiter, rtol, atol, poll, rel_r0, ws, info, s)
+#define KRY(M, V, TR) M<T, V, TR>(p, vals, dinv, b, x, maxiter, rtol, atol, poll, rel_r0, ws, info, s, ps)
   if (method == HF_KRYLOV_PCG) {
     if (vec == 3) return transpose ? KRY(pcg_run, 3, true) : KRY(pcg_run, 3, false);
     return transpose ? KRY(pcg_run, 1, true) : KRY(pcg_run, 1, false);
@@ -642,6 +646,81 @@ extern "C" int hf_krylov_solve_f32(int32_t method, const hf_pattern* p, const fl
                                 (hipStream_t)s);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free Krylov solve.  The workspace is a file-scope device array
+// (every GPU the library runs on holds its own copy, at its own address, until
+// the process ends).  hf_init() resolves that address once per device
+// with hipGetSymbolAddress and makes the device's poll state (two pinned
+// scalar slots, two events); after that the entry point itself calls no
+// allocating routine.  hf_init is the library's set-up call for the current
+// device; the other entry points do not need it.  In host code `scratch` names
+// a host-side stand-in, never the device memory: only the resolved pointer is
+// handed to kernels.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+  void* ptr = nullptr;  // device address of this GPU's `scratch`
+  PollState poll;
+  bool ready = false;
+};
+NoAllocDevice g_noalloc[kMaxDevices];
+
+// the 4 KB scalar header starts from zeros (a kernel, not a memset node, so a
+// caller may capture it)
+__global__ void clear_header_kernel(uint32_t* p, int nwords) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nwords) p[i] = 0u;
+}
+}  // namespace
+
+template <typename T>
+static int krylov_noalloc(int method, const hf_pattern* p, const T* vals, const T* dinv, const T* b, T* x,
+                          int flags, int maxiter, double rtol, double atol, int poll, hf_krylov_info* info,
+                          hipStream_t s) {
+  HF_REQUIRE(p, "hf_krylov_solve_noalloc_f64: null pattern");
+  int dev = -1;
+  HF_CHECK_HIP(hipGetDevice(&dev));
+  HF_REQUIRE(dev >= 0 && dev < kMaxDevices && g_noalloc[dev].ready,
+             "hf_krylov_solve_noalloc_f64: call hf_init() with device %d current first", dev);
+  HF_REQUIRE(method == HF_KRYLOV_PCG || method == HF_KRYLOV_BICGSTAB,
+             "hf_krylov_solve_noalloc_f64: unknown method %d", method);
+  const int64_t N = pattern_nnode(p) * pattern_vec(p);
+  const size_t need = hf_krylov_workspace_size(method, N, (int32_t)sizeof(T));
+  HF_REQUIRE(need <= kScratchBytes,
+             "hf_krylov_solve_noalloc_f64: N = %lld needs %zu bytes of workspace; the static scratch holds %zu "
+             "(use hf_krylov_solve_f64 with a caller workspace)", (long long)N, need, kScratchBytes);
+  NoAllocDevice& d = g_noalloc[dev];
+  clear_header_kernel<<<(1024 + 255) / 256, 256, 0, s>>>((uint32_t*)d.ptr, 1024);
+  HF_LAUNCH_CHECK();
+  return krylov_dispatch<T>(method, p, vals, dinv, b, x, flags, maxiter, rtol, atol, poll, d.ptr, info, s, &d.poll);
+}
+
+extern "C" int hf_init(void) {
+  int dev = -1;
+  HF_CHECK_HIP(hipGetDevice(&dev));
+  HF_REQUIRE(dev >= 0 && dev < kMaxDevices, "hf_init: device %d out of range", dev);
+  NoAllocDevice& d = g_noalloc[dev];
+  if (d.ready) return 0;
+  void* ptr = nullptr;
+  HF_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  HF_REQUIRE(ptr != nullptr, "hf_init: no device address for the scratch array");
+  if (!d.poll.host) HF_CHECK_HIP(hipHostMalloc((void**)&d.poll.host, 2 * sizeof(Scalars), hipHostMallocDefault));
+  if (!d.poll.ev[0]) HF_CHECK_HIP(hipEventCreateWithFlags(&d.poll.ev[0], hipEventDisableTiming));
+  if (!d.poll.ev[1]) HF_CHECK_HIP(hipEventCreateWithFlags(&d.poll.ev[1], hipEventDisableTiming));
+  d.ptr = ptr;
+  d.ready = true;
+  return 0;
+}
+
+extern "C" int hf_krylov_solve_noalloc_f64(int32_t method, const hf_pattern* p, const double* vals,
+                                           const double* dinv, const double* b, double* x, int32_t flags,
+                                           int32_t maxiter, double rtol, double atol, int32_t poll,
+                                           hf_krylov_info* info, hf_stream_t s) {
+  return krylov_noalloc<double>(method, p, vals, dinv, b, x, flags, maxiter, rtol, atol, poll, info, (hipStream_t)s);
+}
 extern "C" int hf_spmv_f64(const hf_pattern* p, const double* a, const double* x, double* y, hf_stream_t s) {
   return spmv_dispatch<double, false>(p, a, x, y, (hipStream_t)s);
 }

@@ -240,8 +240,16 @@ int prof_begin(hipStream_t s);
 int prof_end(hipStream_t s);
 int prof_collect();
 int prof_events(hipEvent_t* e0, hipEvent_t* e1);
+// Convergence poll: two pinned host slots for the scalars and an event per slot.
+// ps == nullptr: the calling thread's own state, created by poll_init() on
+// first use; the allocation-free Krylov entry point passes a per-device state
+// made by hf_noalloc_init().
+struct PollState {
+  Scalars* host = nullptr;  // 2 slots, pinned
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
 int poll_init();
-int poll_post(const Scalars* sc, int slot, hipStream_t s);
-int poll_wait(int slot, Scalars& out);
+int poll_post(const Scalars* sc, int slot, hipStream_t s, PollState* ps = nullptr);
+int poll_wait(int slot, Scalars& out, PollState* ps = nullptr);
 
 }  // namespace hf
