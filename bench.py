@@ -1,25 +1,32 @@
 """Benchmark: DOF/s and wall-to-1e-8 FP64 residual on one MI355X.
 
-Workload (default, 1 GPU): compressible Neo-Hookean, unit cube, HEX8 64^3
-(262,144 cells, 823,875 DOF), E=10, nu=0.3, u=0 on x=0, u_x = 0.1*k/5 on x=1
-for load steps k = 1..5 (u_y, u_z free there), each load step warm-started
-from the previous one and solved to ||R||/||R0|| <= 1e-8 (FP64 residual).
+Workload (default, 1 GPU): heterogeneous linear elasticity, HEX8 128x128x64 on
+[0,1]x[0,1]x[0,0.5] (1,048,576 cells, 3,244,995 DOF), E per cell a clipped
+log-normal Gaussian random field (seed 0, sigma 0.5, correlation length 0.1,
+clipped to [0.05, 20]), nu = 0.3, u = 0 on x = 0, traction (0,0,-10) on x = 1;
+solved to ||R||/||R0|| <= 1e-8 (FP64 residual).  `--workload` selects the
+others: neohookean-hex8-64 (5 load steps), neohookean-hex27-48 (10 load
+steps), simp-hex8-96x48x48 (+ adjoint), elastic-hex8-20.
 
-A "step" is the whole 5-load-step solve.  Methods (all FP32 tangent + FP64
+A "step" is the whole solve (every load step).  Methods (FP32 tangent + FP64
 Newton residual/update, i.e. FP64 iterative refinement, Eisenstat-Walker
 forcing, unless stated):
-  mg      : flexible CG preconditioned by geometric-multigrid V-cycles (Galerkin
-            levels, Chebyshev smoothing; box lattices).  This is `value` where
-            the mesh is a HEX8 box lattice.
-  mixed   : Jacobi-PCG with FP64 dot products (`value` on other meshes).
-  learned : the learned CNN preconditioner (flexible CG, Jacobi-PCG fallback),
-            timed once and reported beside it.
-  plain   : the drop-in: everything FP64, Jacobi-PCG to 1e-10 per Newton step
-            (the reference's jax_solve tolerances).
-roofline: the dominant kernel of the `value` path, timed live with HIP events
-(the fine-level smoother pass of the multigrid, or the Krylov SpMV); traffic is
-its HBM bytes per launch from rocprofv3 PMC counters committed under profiles/
-(null if absent).
+  mg        : flexible CG preconditioned by geometric-multigrid V-cycles (Galerkin
+              levels, Chebyshev smoothing; box lattices).  This is `value` where
+              the mesh is a box lattice.
+  mixed     : Jacobi-PCG with FP64 dot products (`value` on other meshes).
+  learned   : the learned CNN preconditioner, timed once and reported beside it.
+  plain     : the drop-in: everything FP64, Jacobi-PCG to 1e-10 per Newton step
+              (the reference's jax_solve tolerances).
+  reference : the reference's defaults, FP64 Jacobi-BiCGSTAB tol = atol = 1e-10;
+              its counts drive the CPU baseline's extrapolation.
+roofline: the dominant kernel of the `value` path (the fine-level Chebyshev step
+of the multigrid; the SELL product on HEX27, the Krylov SpMV without multigrid),
+timed live with HIP events; traffic is its HBM bytes per launch from rocprofv3
+PMC counters committed under profiles/ (null if absent).
+cpu_baseline: the FP64 oracle on this host, in full for elastic-hex8-20,
+extrapolated from a bounded sample otherwise (`--cpu-only full` times a whole
+workload on the CPU alone and prints the sampled estimate beside it).
 `--calibrate-copy` runs only a 256 MiB device copy (the PMC calibration kernel).
 Prints one JSON line.
 """
@@ -177,44 +184,175 @@ def run_sequence(jfa, prob, bc, opts, nload=5, dmax=0.1):
     return sol, infos
 
 
-def cpu_baseline(n, plain_infos, budget_cells=16384):
-    """Oracle (FP64 NumPy/SciPy restatement of the reference algorithm) timed on a
-    bounded sample: element kernels + COO->CSR on `budget_cells` cells and Krylov
-    iterations on that sub-matrix, scaled to the full mesh by cell / nnz counts and
-    to the full solve by the Newton and Krylov counts the FP64 GPU run measured."""
+def _plane_x(x1):
+    """Location function of the plane x = x1 (one argument: the point)."""
+    return lambda p: abs(p[0] - x1) < 1e-9
+
+
+def _oracle_case(wl, cells=None):
+    """The workload's problem as an oracle.solver.OracleProblem (the FP64 NumPy/SciPy
+    restatement of the reference algorithm: the CPU baseline), at its full size or on
+    a box of `cells` cells with the same cell size.  Returns (problem, load steps:
+    a list of Dirichlet specs, or [None] for one solve)."""
     import numpy as np
     from oracle import fe as ofe, kernels as ok, solver as osv
+    left = lambda p: abs(p[0]) < 1e-9
+    if wl == "hetero-elastic-hex8-128x128x64":
+        from jax_fem_amd.learned_pc import heterogeneous_modulus
+        full = (128, 128, 64)
+        c = cells or full
+        L = tuple(n / 128.0 for n in c)
+        E = heterogeneous_modulus(c, 0, domain=L)
+        pts, cl = ofe.box_mesh(*c, *L)
+        right = _plane_x(L[0])
+        op = osv.OracleProblem(pts, cl, 3, ok.Material("linear_elastic", E=1.0, nu=0.3, param="scale_E"),
+                               dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3], location_fns=[right],
+                               surface_maps=[lambda x: np.broadcast_to(np.array([0.0, 0.0, 10.0]), x.shape)],
+                               params=np.repeat(E[:, None], 8, axis=1))
+        return op, [None]
+    if wl == "elastic-hex8-20":
+        c = cells or (20, 20, 20)
+        L = tuple(n / 20.0 for n in c)
+        pts, cl = ofe.box_mesh(*c, *L)
+        right = _plane_x(L[0])
+        op = osv.OracleProblem(pts, cl, 3, ok.Material("linear_elastic", E=70e3, nu=0.3),
+                               dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3], location_fns=[right],
+                               surface_maps=[lambda x: np.broadcast_to(np.array([0.0, 0.0, 10.0]), x.shape)])
+        return op, [None]
+    if wl == "neohookean-hex8-64":
+        c = cells or (64, 64, 64)
+        L = tuple(n / 64.0 for n in c)
+        pts, cl = ofe.box_mesh(*c, *L)
+        right = _plane_x(L[0])
 
-    pts, cells = ofe.box_mesh(n, n, n)
-    nc_total = len(cells)
-    m = int(round(budget_cells ** (1 / 3)))
-    spts, scells = ofe.box_mesh(m, m, m)
-    sub = osv.OracleProblem(spts, scells, 3, ok.Material("neo_hookean", E=10.0, nu=0.3),
-                            dirichlet_bc_info=[[lambda p: abs(p[0]) < 1e-9] * 3, [0, 1, 2], [lambda p: 0.0] * 3])
-    u = 1e-3 * np.random.default_rng(0).standard_normal(sub.N)   # small enough that no element inverts
+        def bc(d):
+            return [[left] * 3 + [right], [0, 1, 2, 0], [lambda p: 0.0] * 3 + [lambda p, d=d: d]]
+        op = osv.OracleProblem(pts, cl, 3, ok.Material("neo_hookean", E=10.0, nu=0.3), dirichlet_bc_info=bc(0.0))
+        return op, [bc(0.1 * k / 5) for k in range(1, 6)]
+    raise ValueError("no CPU baseline for workload %s" % wl)
+
+
+def _oracle_solve(op, steps):
+    """The reference's solve of a workload: for each load step a Newton solve with
+    Jacobi-BiCGSTAB (tol = atol = 1e-10, the jax_solve defaults, solver.py:63-92),
+    warm-started from the last.  Returns (seconds, residual+tangent updates, BiCGSTAB iterations)."""
+    import numpy as np
+    from oracle import solver as osv
+    u = None
     t0 = time.perf_counter()
-    R = sub.apply_bc_vec(sub.compute_residual(u), u)
-    A = sub.assemble(u)
-    t_asm = time.perf_counter() - t0
+    upd = its = 0
+    for spec in steps:
+        if spec is not None:
+            op.update_bcs(spec)
+        u, info = osv.newton_solve(op, u0=u, method="bicgstab")
+        upd += info["newton_iters"] + 1
+        its += sum(info["lin_iters"])
+    return time.perf_counter() - t0, upd, its
+
+
+def _oracle_unit_times(op):
+    """One residual + tangent update (apply_bc_vec(compute_residual) + get_A, what
+    newton_update_helper does per Newton iterate, solver.py:1259-1276), one SpMV and
+    one Jacobi-BiCGSTAB iteration of the oracle on `op`, in seconds."""
+    import numpy as np
+    from oracle import solver as osv
+    u = 1e-4 * np.random.default_rng(0).standard_normal(op.N)   # small: no element inverts
+    op.apply_bc_vec(op.compute_residual(u), u)                    # warm caches / lazily built data
+    t0 = time.perf_counter()
+    R = op.apply_bc_vec(op.compute_residual(u), u)
+    A = op.assemble(u)
+    t_upd = time.perf_counter() - t0
+    x = np.random.default_rng(1).standard_normal(op.N)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        A @ x
+    t_spmv = (time.perf_counter() - t0) / 10
     d = A.diagonal()
-    b = -R
+    k = 40
     t0 = time.perf_counter()
-    _, it, _ = osv.bicgstab(A, b, np.zeros_like(b), lambda v: v / d, 1e-30, 0.0, 60)
-    t_it = (time.perf_counter() - t0) / max(it, 1) / 2.0   # one PCG iteration ~ half a BiCGSTAB one
-    scale_cells = nc_total / len(scells)
-    N_full = 3 * (n + 1) ** 3
-    nnz_full = 9 * (3 * n + 1) ** 3
-    scale_iter = (nnz_full + 10 * N_full) / (A.nnz + 10 * sub.N)
-    newton = sum(i["newton_iters"] + 1 for i in plain_infos)
-    kry = sum(i["krylov_iters"] for i in plain_infos)
-    wall = newton * t_asm * scale_cells + kry * t_it * scale_iter
-    return {"value": N_full * len(plain_infos) / wall, "unit": "DOF/s", "cores": 1, "kind": "port",
-            "sample": ("oracle (FP64 NumPy/SciPy restatement, 1 thread) timed on a %d^3-cell sub-mesh: "
-                       "residual+element Jacobians+COO->CSR %.3fs, Jacobi-PCG iteration %.4fs; scaled by "
-                       "cells (x%.0f) and nnz (x%.1f) and by the FP64 GPU run's %d Newton assemblies and %d "
-                       "PCG iterations -> %.1f s estimated wall" % (m, t_asm, t_it, scale_cells, scale_iter,
-                                                                     newton, kry, wall)),
-            "cpu_model": _cpu_model(), "host_cores_available": len(os.sched_getaffinity(0))}
+    _, it, _ = osv.bicgstab(A, -R, np.zeros(op.N), lambda v: v / d, 0.0, 0.0, k)
+    t_it = (time.perf_counter() - t0) / max(it, 1)
+    return t_upd, t_spmv, t_it, A.nnz
+
+
+SAMPLE_CELLS = {"hetero-elastic-hex8-128x128x64": (40, 40, 20), "neohookean-hex8-64": (32, 32, 32),
+                "elastic-hex8-20": (10, 10, 10)}
+FULL_CELLS = {"hetero-elastic-hex8-128x128x64": (128, 128, 64), "neohookean-hex8-64": (64, 64, 64),
+              "elastic-hex8-20": (20, 20, 20)}
+
+
+def cpu_estimate(wl, updates, iters):
+    """Oracle wall time of the whole workload, extrapolated from a bounded sample:
+    residual + tangent updates scale with the cell count, a BiCGSTAB iteration is
+    2 SpMVs (scaled by nnz) plus vector work (scaled by N); `updates` and `iters`
+    are the counts of the reference's method (Jacobi-BiCGSTAB, jax stop rule)."""
+    import numpy as np
+    sub = SAMPLE_CELLS[wl]
+    op, _ = _oracle_case(wl, sub)
+    t_upd, t_spmv, t_it, nnz_s = _oracle_unit_times(op)
+    nx, ny, nz = FULL_CELLS[wl]
+    cells_full, cells_s = nx * ny * nz, int(np.prod(sub))
+    N_full, N_s = 3 * (nx + 1) * (ny + 1) * (nz + 1), op.N
+    nnz_full = 9 * (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
+    t_vec = max(t_it - 2 * t_spmv, 0.0)
+    wall = updates * t_upd * cells_full / cells_s + iters * (2 * t_spmv * nnz_full / nnz_s + t_vec * N_full / N_s)
+    return wall, {"sample_cells": list(sub), "t_update_s": t_upd, "t_spmv_s": t_spmv, "t_bicgstab_iter_s": t_it,
+                  "updates": int(updates), "bicgstab_iters": int(iters)}
+
+
+def cpu_baseline(wl, dof, nload, ref_counts):
+    """The reported CPU baseline: the oracle timed on this host.  The 20^3 workload
+    runs in full; larger ones are extrapolated from a bounded sample (cpu_estimate)
+    with the residual+tangent update and BiCGSTAB iteration counts of the reference's
+    method as measured by the FP64 Jacobi-BiCGSTAB run on the GPU (ref_counts)."""
+    cores = len(os.sched_getaffinity(0))
+    common = {"unit": "DOF/s", "cores": cores, "kind": "port", "cpu_model": _cpu_model(),
+              "threads_note": "BLAS-backed NumPy parts may use every core of the process's affinity; SciPy "
+                              "sparse products (the Krylov iterations) run on one thread"}
+    if wl == "elastic-hex8-20":
+        op, steps = _oracle_case(wl)
+        _oracle_solve(op, steps)                           # warm-up
+        runs = sorted(_oracle_solve(_oracle_case(wl)[0], steps)[0] for _ in range(3))
+        wall = runs[1]
+        return dict(common, value=dof * nload / wall, wall_s=wall, runs_s=runs,
+                    sample="full oracle solve (FP64 NumPy/SciPy restatement: one Newton step, Jacobi-BiCGSTAB "
+                           "1e-10), one warm-up, median of 3")
+    wall, parts = cpu_estimate(wl, ref_counts["updates"], ref_counts["iters"])
+    return dict(common, value=dof * nload / wall, wall_s=wall, parts=parts,
+                sample=("oracle timed on a %s-cell box of the same problem (one residual+tangent update %.2f s, "
+                        "SpMV %.4f s, BiCGSTAB iteration %.4f s), scaled by cells / nnz / DOF to the full mesh "
+                        "and by the GPU FP64 Jacobi-BiCGSTAB run's %d updates and %d iterations -> %.0f s "
+                        "estimated wall" % ("x".join(map(str, parts["sample_cells"])), parts["t_update_s"],
+                                            parts["t_spmv_s"], parts["t_bicgstab_iter_s"], parts["updates"],
+                                            parts["bicgstab_iters"], wall)))
+
+
+def cpu_only(wl, mode):
+    """--cpu-only: the oracle alone (no GPU).  'full' times the whole workload (one
+    warm-up unless --no-warmup, median of 3 unless --runs) and the sampled estimate
+    for the same counts beside it; 'sample' prints the unit times."""
+    nx, ny, nz = FULL_CELLS[wl]
+    out = {"workload": wl, "dof": 3 * (nx + 1) * (ny + 1) * (nz + 1), "cpu_model": _cpu_model(),
+           "host_cores_available": len(os.sched_getaffinity(0))}
+    if mode == "full":
+        op, steps = _oracle_case(wl)
+        runs = []
+        nrun = int(os.environ.get("HF_CPU_RUNS", "3"))
+        if os.environ.get("HF_CPU_WARMUP", "1") == "1":
+            _oracle_solve(op, steps)
+        for _ in range(nrun):
+            op, steps = _oracle_case(wl)
+            t, upd, its = _oracle_solve(op, steps)
+            runs.append(t)
+            log("full oracle run: %.1f s, %d updates, %d BiCGSTAB iterations" % (t, upd, its))
+        runs.sort()
+        out.update(full_runs_s=runs, full_median_s=runs[len(runs) // 2], updates=upd, bicgstab_iters=its)
+        est, parts = cpu_estimate(wl, upd, its)
+        out.update(estimate_s=est, estimate_parts=parts, estimate_over_full=est / out["full_median_s"])
+    else:
+        est, parts = cpu_estimate(wl, 1, 1)
+        out.update(unit_times=parts)
+    print(json.dumps(out))
 
 
 def _pmc_traffic(name):
@@ -364,19 +502,31 @@ def simp_adjoint(jfa, prob, theta, fd_cells, opts_fwd, adj_opts=None):
     return out
 
 
+def fine_model_bytes(n0):
+    """HBM bytes of one flexible-CG iteration's fine-level work per the kernels'
+    algorithmic traffic (FP32, half-stored stencil 126 values + block inverse 9 per
+    node): update 180, pre-smoothing step 612, residual 552, restriction read 12,
+    prolongation 36, post-smoothing start 600 and step 612, outer product 564 bytes
+    per node.  The coarse levels (about a sixth more) are not counted."""
+    return n0 * (180 + 612 + 552 + 12 + 36 + 600 + 612 + 564)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="neohookean-hex8-64", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="hetero-elastic-hex8-128x128x64", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=None, help="cells per direction (cube workloads)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-ref", action="store_true", help="skip the FP64 Jacobi-BiCGSTAB (reference defaults) leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-mg", action="store_true", help="value path: Jacobi-PCG even on box lattices")
     ap.add_argument("--no-pcg", action="store_true", help="skip the mixed Jacobi-PCG leg beside the multigrid")
     ap.add_argument("--fd-cells", type=int, default=0, help="SIMP workload: cells checked by central FD")
+    ap.add_argument("--cpu-only", choices=["full", "sample"], default=None,
+                    help="time only the CPU oracle of the workload (no GPU)")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--debug", action="store_true")
     args = ap.parse_args()
@@ -384,7 +534,10 @@ def main():
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
               "got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
-
+    if args.cpu_only:
+        load_pkg()
+        cpu_only(args.workload, args.cpu_only)
+        return
     if args.calibrate_copy:
         _calibrate_copy()
         return
@@ -397,9 +550,8 @@ def main():
         logging.getLogger("jax_fem_amd").setLevel(logging.DEBUG)
     torch.cuda.init()
     wl = args.workload
-    default = wl == "neohookean-hex8-64"
     bc, nload, dmax, theta = None, 1, 0.0, None
-    if default:
+    if wl == "neohookean-hex8-64":
         n = args.n or 64
         prob, bc = build_problem(jfa, n)
         nload, dmax = 5, 0.1
@@ -419,6 +571,7 @@ def main():
         n = args.n or 20
         prob = build_elastic20(jfa, n)
         desc = WORKLOADS[wl].replace("20^3", "%d^3" % n)
+    full_size = args.n is None
     N = prob.N
     log("problem: %s, %d DOF" % (desc, N))
     t0 = time.perf_counter()
@@ -429,13 +582,14 @@ def main():
     # tol=0: stop on the relative FP64 residual 1e-8 only (the reference also stops at ||R|| <= 1e-6)
     mixed = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4, "linear": {"pcg": {}}}}
     mgo = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4, "linear": {"mg": {}}}}
-    # plain: the reference's jax_solve tolerances (tol = atol = 1e-10, solver.py:78-84).
-    # The SIMP load is small (||R0|| ~ 1e-2), where the absolute 1e-10 would stop the
-    # Krylov solves above the 1e-8 relative Newton target: there the tolerance is relative only.
+    # plain: the reference's jax_solve tolerances (tol = atol = 1e-10, solver.py:78-84) with
+    # Jacobi-PCG.  The SIMP load is small (||R0|| ~ 1e-2), where the absolute 1e-10 would stop
+    # the Krylov solves above the 1e-8 relative Newton target: there the tolerance is relative only.
     plain = {"newton": {"precision": "fp64", "tol": 0.0, "forcing": "fixed",
                         "linear": {"pcg": {"tol": 1e-10, "atol": 0.0 if wl == "simp-hex8-96x48x48" else 1e-10}}}}
+    # the reference's defaults: FP64 Jacobi-BiCGSTAB, tol = atol = 1e-10, Newton rel_tol 1e-8 / tol 1e-6
+    refdef = {"newton": {"linear": {"jax_solver": {}}}}
     if wl == "elastic-hex8-20":
-        # the reference's own configuration: one Newton step, FP64 Jacobi-BiCGSTAB to 1e-10
         plain = {"newton": {"tol": 0.0, "linear": {"jax_solver": {}}}}
     from jax_fem_amd.multigrid import lattice
     try:
@@ -463,30 +617,34 @@ def main():
     jfa._lib.call("hf_profile_read", ctypes.byref(nl), ctypes.byref(ms))
     jfa._lib.call("hf_profile_enable", 0)
     kern_us = 1e3 * ms.value / max(nl.value, 1)
+    tag = {"hetero-elastic-hex8-128x128x64": "hetero_hex8_128x128x64", "neohookean-hex8-64": "hex8_64",
+           "neohookean-hex27-48": "hex27_48", "simp-hex8-96x48x48": "simp_hex8_96x48x48",
+           "elastic-hex8-20": "elastic_hex8_20"}[wl]
     if use_mg and prob.fe.ele_type == "HEX27":
         # the fine level is the SELL tangent: one smoothing product y = A v
         kbytes = 4 * nnz + 4 * prob.pattern.nblocks + 8 * N
         kname = "mg_sell_spmv (fine-level smoothing product y = A v, SELL column-form node blocks)"
-        pmc = "mg_sell_spmv_hex27_48_pmc.md"
+        pmc = "mg_sell_spmv_%s_pmc.md" % tag
     elif use_mg:
-        # one pass of the fine-level smoother (mg_post_first): the symmetric 27-point
-        # node-block stencil stored as its diagonal and 13 upper blocks (126 FP32
-        # values per node; the lower blocks are the neighbours' upper ones, read
-        # once each in the ideal), the 3x3 block-Jacobi inverse (9), and the node's
-        # r, e_in (read) and res, d, e_out (written): 3 FP32 each
-        kbytes = prob.N // 3 * (126 * 4 + 9 * 4 + 5 * 12)
-        kname = ("mg_post_first<true> (fine-level Chebyshev smoothing pass: half-stored symmetric "
+        # one fine-level Chebyshev smoothing step (mg_poly_step<true>): the symmetric
+        # 27-point node-block stencil stored as its diagonal and 13 upper blocks (126
+        # FP32 values per node; the lower blocks are the neighbours' upper ones, read
+        # once each in the ideal), the 3x3 block-Jacobi inverse (9), and per node the
+        # residual, direction and correction read (3 FP32 each) and written (3 each)
+        kbytes = prob.N // 3 * (126 * 4 + 9 * 4 + 6 * 12)
+        kname = ("mg_poly_step<true> (fine-level Chebyshev smoothing step: half-stored symmetric "
                  "27-point node-block stencil, slot-major FP32)")
-        pmc = "mg_post_first_hex8_64_pmc.md"
+        pmc = "mg_poly_step_%s_pmc.md" % tag
     else:
         # one PCG SpMV (w = A u): values (4 B/nnz), one block column per node-pair
         # block, u read and w written (4 B per DOF each)
         kbytes = 4 * nnz + 4 * prob.pattern.nblocks + 8 * N
         kname = "cg_spmv_kernel<float,3> (SELL column-form node-block SpMV w = A u fused with u.w)"
-        pmc = "cg_spmv_hex8_64_pmc.md"
+        pmc = "cg_spmv_%s_pmc.md" % tag
     achieved = kbytes / (kern_us * 1e-6) / 1e9
     rel_m = max(i["rel"] for i in infos_m)
-    log("%s: %.3f s/step, newton %s, krylov %s, kernel %.1f us (%d samples) -> %.0f GB/s" % (
+    fcg_iters = sum(i["krylov_iters"] for i in infos_m)
+    log("%s: %.4f s/step, newton %s, krylov %s, kernel %.1f us (%d samples) -> %.0f GB/s" % (
         "mg" if use_mg else "mixed", wall_m, [i["newton_iters"] for i in infos_m],
         [i["krylov_iters"] for i in infos_m], kern_us, nl.value, achieved))
 
@@ -506,17 +664,21 @@ def main():
             name, wall, out["newton_iters"], out["krylov_iters"], out["fallbacks"]))
         return sol, out
 
+    def rel_to(sol, ref):
+        a, b = sol[0].reshape(-1), ref[0].reshape(-1)
+        return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
     mixed_out = None
     if use_mg and not args.no_pcg:
         _, mixed_out = leg(mixed, "mixed jacobi-pcg", "single run after the timed steps")
-    learned_out = None
+    learned_out, sol_l = None, None
     if not args.no_learned and wl not in ("elastic-hex8-20", "neohookean-hex27-48"):   # HEX8 grids only
         from jax_fem_amd.learned_pc import LearnedPC
         model = LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_elastic.npz"))
         learned = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4,
                               "linear": {"learned": {"model": model}}}}
-        _, learned_out = leg(learned, "learned", "weights trained on 32^3 heterogeneous linear elasticity; single "
-                                                 "run incl. first-call setup")
+        sol_l, learned_out = leg(learned, "learned", "weights trained on 32^3 heterogeneous linear elasticity; "
+                                                     "single run incl. first-call setup")
 
     plain_out = None
     cpu = None
@@ -525,15 +687,30 @@ def main():
         sol_p, plain_out = leg(plain, "plain")
         plain_out["method"] = ("FP64 Jacobi-BiCGSTAB 1e-10 (jax_solver)" if wl == "elastic-hex8-20"
                                else "FP64 Jacobi-PCG 1e-10")
-        um = sol_m[0].reshape(-1)
-        up = sol_p[0].reshape(-1)
-        rel_l2 = float(torch.linalg.norm(um - up) / torch.linalg.norm(up))
+        rel_l2 = rel_to(sol_m, sol_p)
         log("value path vs plain rel-L2 %.2e" % rel_l2)
-        if default and not args.no_cpu:
-            t0 = time.perf_counter()
-            cpu = cpu_baseline(n, [{"newton_iters": a, "krylov_iters": b}
-                                   for a, b in zip(plain_out["newton_iters"], plain_out["krylov_iters"])])
-            log("cpu baseline %.1f s of sampling" % (time.perf_counter() - t0))
+        if learned_out is not None:
+            learned_out["rel_l2_vs_plain"] = rel_to(sol_l, sol_p)
+            learned_out["speedup_vs_plain"] = plain_out["wall_s"] / learned_out["wall_s"]
+    ref_out = None
+    if not args.no_ref and wl != "elastic-hex8-20":
+        _, ref_out = leg(refdef, "reference defaults (FP64 Jacobi-BiCGSTAB)")
+        ref_out["method"] = "FP64 Jacobi-BiCGSTAB, tol = atol = 1e-10, Newton rel_tol 1e-8 / tol 1e-6 (jax_solve)"
+    elif wl == "elastic-hex8-20" and plain_out is not None:
+        ref_out = plain_out
+    if not args.no_cpu and full_size and wl in SAMPLE_CELLS and ref_out is not None:
+        t0 = time.perf_counter()
+        counts = {"updates": sum(k + 1 for k in ref_out["newton_iters"]), "iters": sum(ref_out["krylov_iters"])}
+        cpu = cpu_baseline(wl, N, nload, counts)
+        log("cpu baseline %.1f s of sampling: %s" % (time.perf_counter() - t0, cpu["sample"]))
+    n0 = prob.N // 3
+    whole = None
+    if use_mg and prob.fe.ele_type == "HEX8":
+        wb = fine_model_bytes(n0) * fcg_iters
+        whole = {"model": "fine-level flexible-CG traffic only (3168 B per fine node per iteration; coarse levels, "
+                          "tangent assembly and set-up not counted: a lower bound)",
+                 "bytes": wb, "wall_s": wall_m, "achieved": wb / wall_m / 1e9, "unit": "GB/s",
+                 "frac": wb / wall_m / 1e9 / 8000.0}
     out = {
         "metric": "DOF/s + wall-to-1e-8 FP64 residual: learned-PC/FP32-refined vs plain PCG",
         "value": N * nload / wall_m,
@@ -542,7 +719,7 @@ def main():
         "ms_per_step": wall_m * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": desc, "dof": N,
+        "config": {"workload": wl, "description": desc, "dof": N,
                    "nnz": nnz, "method": ("value: FP32 tangent, flexible CG preconditioned by geometric-multigrid "
                                           "V-cycles (Galerkin levels, Chebyshev(2) smoothing), FP64 Newton refinement "
                                           "(EW forcing); Jacobi-PCG under 'pcg_mixed', the learned CNN preconditioner "
@@ -554,11 +731,13 @@ def main():
                    "krylov_iters": [i["krylov_iters"] for i in infos_m]},
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-                     "traffic": _pmc_traffic(pmc) if (default and n == 64) else None, "avg_launch_us": kern_us,
-                     "algorithmic_bytes": kbytes},
+                     "traffic": _pmc_traffic(pmc) if full_size else None, "avg_launch_us": kern_us,
+                     "launches_timed": int(nl.value), "algorithmic_bytes": kbytes},
+        "whole_solve": whole,
         "pcg_mixed": mixed_out,
         "learned_pc": learned_out,
         "plain": plain_out,
+        "reference_defaults": ref_out,
         "speedup_vs_plain": (plain_out["wall_s"] / wall_m) if plain_out else None,
         "cpu_baseline": cpu,
     }

@@ -2236,9 +2236,10 @@ extern "C" int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels) {
 // Enqueue one V-cycle: level-0 input r0, result in *out (one of the two level-0
 // buffers z0 / z1 given by the caller).  Levels with more than kSmallNodes
 // nodes run as grid kernels; the rest (from level lb down) in mg_small_cycle.
-// timed: the level-0 post-smoother start (one pass over the fine stencil) is
-// launched between the two profiling events (HIP-event timing for roofline
-// reporting, see hf_profile_enable).
+// timed: the level-0 pre-smoother's first stencil step (mg_poly_step, one pass
+// over the fine stencil; the SELL product on a SELL fine level) is launched
+// between the two profiling events (HIP-event timing for roofline reporting,
+// see hf_profile_enable).
 // first_done: the fine level's pre-smoother start (d = e = s0 Dinv r) was
 // written by the caller's kernel (the outer iteration's update)
 static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out, const Scalars* sc, hipStream_t s,
@@ -2265,15 +2266,23 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     float* dout = V.db;
     const bool lanes = V.g.n <= kLaneNodes;
     const int gl = lane_grid(V.g.n);
+    int derr = 0;
     auto down = [&](auto half_tag) {
       constexpr bool H = decltype(half_tag)::value;
       for (int j = 1; j < k; ++j) {
-        if (lanes)
+        if (timed && l == 0 && j == 1) {
+          hipEvent_t e0, e1;
+          if (prof_events(&e0, &e1)) { derr = -2; return; }
+          auto step = lanes ? mg_poly_step_lanes<H> : mg_poly_step<H>;
+          hipExtLaunchKernelGGL(step, dim3(lanes ? gl : gg), dim3(kMgBlock), 0, s, e0, e1, 0, (const float*)V.S,
+                                (const float*)V.Dinv, V.g, cf, j, res, (const float*)din, V.res, dout, cur[l], sc);
+        } else if (lanes) {
           mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, res, din, V.res, dout,
                                                         cur[l], sc);
-        else
+        } else {
           mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, res, din, V.res, dout,
                                                   cur[l], sc);
+        }
         res = V.res;
         float* tmp = din; din = dout; dout = tmp;
       }
@@ -2301,6 +2310,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     } else {
       down(std::false_type{});
     }
+    if (derr) return derr;
     const MgLevel& C = mg->lv[l + 1];
     const bool smooth_c = (l + 2 < L);
     mg_restrict<<<node_grid(C.g.n), kMgBlock, 0, s>>>(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, coef_of(mg, l + 1),
@@ -2339,14 +2349,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       constexpr bool H = decltype(half_tag)::value;
       auto post = lanes ? mg_post_first_lanes<H> : mg_post_first<H>;
       const int gp = lanes ? gl : gg;
-      if (timed && l == 0) {
-        hipEvent_t e0, e1;
-        if (prof_events(&e0, &e1)) { perr = -2; return; }
-        hipExtLaunchKernelGGL(post, dim3(gp), dim3(kMgBlock), 0, s, e0, e1, 0, (const float*)V.S,
-                              (const float*)V.Dinv, V.g, cf, rin(l), (const float*)cur[l], V.res, V.da, oth[l], sc);
-      } else {
-        post<<<gp, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, rin(l), cur[l], V.res, V.da, oth[l], sc);
-      }
+      post<<<gp, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, rin(l), cur[l], V.res, V.da, oth[l], sc);
       float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
       float* din = V.da;
       float* dout = V.db;
@@ -2551,7 +2554,8 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   for (;;) {
     // the batch holding the HIP-event timed launch (a steady-state one) runs
     // eagerly; the others replay the captured batch
-    const bool timed = g_prof && mg->nlev > 1 && !timed_pending;
+    // (a timed launch exists when the fine level is SELL or smooths with degree >= 2)
+    const bool timed = g_prof && mg->nlev > 1 && (mg->sell || mg->degree >= 2) && !timed_pending;
     if (timed || !graph) {
       if (mg_enqueue_batch(mg, B, poll, s, timed ? poll / 2 : -1)) return -2;
       if (timed) {

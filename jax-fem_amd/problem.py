@@ -161,7 +161,12 @@ class Problem:
         return mat
 
     def _params(self):
-        """Per-quadrature parameter (ncell, nq) f64 on the device, or None."""
+        """Per-quadrature parameter (ncell, nq) f64 on the device, or None.  Inside a
+        solve (hold_params) the tensor made at its start serves every evaluation: a
+        NumPy field is copied to the device once per solve, not per residual."""
+        held = getattr(self, "_held_params", None)
+        if held is not None:
+            return held[0]
         iv = self.internal_vars
         if iv is None or len(iv) == 0:
             return None
@@ -174,6 +179,13 @@ class Problem:
         if tuple(t.shape) != (self.num_cells, self.fe.num_quads):
             raise ValueError("internal_vars[0] must have shape (num_cells, num_quads)")
         return t.contiguous()
+
+    def hold_params(self, on):
+        """Freeze (on=True) or release the device copy of the parameter field for the
+        duration of a solve; internal_vars must not change while it is held."""
+        self._held_params = None
+        if on:
+            self._held_params = (self._params(),)
 
     def _buf(self, name, n, dtype):
         b = self._bufs.get(name)

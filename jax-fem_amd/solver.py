@@ -342,11 +342,18 @@ def solver(problem, solver_options={}):
         if key in lin_opts and precision != "mixed":
             raise ValueError("the %s preconditioner runs on the FP32 tangent: use precision='mixed'" % key)
     adt = torch.float32 if precision == "mixed" else torch.float64
+    problem.hold_params(True)
+    try:
+        return _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, adt)
+    finally:
+        problem.hold_params(False)
+
+
+def _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, adt):
     eta_max = cfg.get("eta_max", 0.1)
     eta_min = cfg.get("eta_min", 1e-6 if precision == "mixed" else 1e-10)
     eta_first = cfg.get("eta_first", 1e-3)
     backtrack = cfg.get("backtrack", forcing == "ew")
-
     timing = {"local_assembly": 0.0, "global_matrix": 0.0, "linear": 0.0}
     stats = KrylovStats()
     torch.cuda.synchronize()
