@@ -15,7 +15,9 @@ forcing, unless stated):
               levels, Chebyshev smoothing; box lattices).  This is `value` where
               the mesh is a box lattice.
   mixed     : Jacobi-PCG with FP64 dot products (`value` on other meshes).
-  learned   : the learned CNN preconditioner, timed once and reported beside it.
+  learned   : the multigrid with the learned CNN as the fine level's post-smoother
+              (reported as learned_pc); --learned-fcg also times the CNN as the
+              whole preconditioner of a flexible CG.
   plain     : the drop-in: everything FP64, Jacobi-PCG to 1e-10 per Newton step
               (the reference's jax_solve tolerances).
   reference : the reference's defaults, FP64 Jacobi-BiCGSTAB tol = atol = 1e-10;
@@ -522,6 +524,8 @@ def main():
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 Jacobi-BiCGSTAB (reference defaults) leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-fcg", action="store_true",
+                    help="also time the learned network as the whole preconditioner of a flexible CG")
     ap.add_argument("--no-mg", action="store_true", help="value path: Jacobi-PCG even on box lattices")
     ap.add_argument("--no-pcg", action="store_true", help="skip the mixed Jacobi-PCG leg beside the multigrid")
     ap.add_argument("--fd-cells", type=int, default=0, help="SIMP workload: cells checked by central FD")
@@ -671,14 +675,27 @@ def main():
     mixed_out = None
     if use_mg and not args.no_pcg:
         _, mixed_out = leg(mixed, "mixed jacobi-pcg", "single run after the timed steps")
-    learned_out, sol_l = None, None
-    if not args.no_learned and wl not in ("elastic-hex8-20", "neohookean-hex27-48"):   # HEX8 grids only
+    learned_out, sol_l, fcg_out = None, None, None
+    if not args.no_learned and use_mg and prob.fe.ele_type == "HEX8":
+        # the learned network as the multigrid's fine-level post-smoother (weights trained
+        # on post-coarse-correction residuals of 32^3 heterogeneous elasticity)
+        from jax_fem_amd.learned_pc import LearnedPC
+        smoother = LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_mg_smoother.npz"))
+        learned = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4,
+                              "linear": {"mg": {"smoother": "learned", "model": smoother}}}}
+        solve_case(jfa, prob, bc, learned, nload, dmax)        # first use: set-up and capture
+        sol_l, learned_out = leg(learned, "learned smoother",
+                                 "multigrid flexible CG with the learned network as the fine-level post-smoother "
+                                 "(weights/lpc_mg_smoother.npz); second run")
+        learned_out["method"] = ("FP32 tangent + FP64 Newton refinement; multigrid V-cycles with Chebyshev(2) "
+                                 "pre-smoothing and one learned-network step as the fine post-smoother")
+    if args.learned_fcg and wl not in ("elastic-hex8-20", "neohookean-hex27-48"):   # HEX8 grids only
         from jax_fem_amd.learned_pc import LearnedPC
         model = LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_elastic.npz"))
-        learned = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4,
-                              "linear": {"learned": {"model": model}}}}
-        sol_l, learned_out = leg(learned, "learned", "weights trained on 32^3 heterogeneous linear elasticity; "
-                                                     "single run incl. first-call setup")
+        lfcg = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4,
+                           "linear": {"learned": {"model": model}}}}
+        _, fcg_out = leg(lfcg, "learned fcg", "the learned network as the whole preconditioner of a flexible CG "
+                                              "(weights/lpc_elastic.npz); single run incl. first-call setup")
 
     plain_out = None
     cpu = None
@@ -722,8 +739,8 @@ def main():
         "config": {"workload": wl, "description": desc, "dof": N,
                    "nnz": nnz, "method": ("value: FP32 tangent, flexible CG preconditioned by geometric-multigrid "
                                           "V-cycles (Galerkin levels, Chebyshev(2) smoothing), FP64 Newton refinement "
-                                          "(EW forcing); Jacobi-PCG under 'pcg_mixed', the learned CNN preconditioner "
-                                          "under 'learned_pc'") if use_mg else
+                                          "(EW forcing); Jacobi-PCG under 'pcg_mixed', the multigrid with the learned "
+                                          "network as fine-level post-smoother under 'learned_pc'") if use_mg else
                    ("value: FP32 Jacobi-PCG + FP64 Newton refinement (EW forcing); the learned-PC variant is "
                     "reported under 'learned_pc'"),
                    "wall_to_tol_s": wall_m, "max_rel_residual": rel_m,
@@ -736,6 +753,7 @@ def main():
         "whole_solve": whole,
         "pcg_mixed": mixed_out,
         "learned_pc": learned_out,
+        "learned_fcg": fcg_out,
         "plain": plain_out,
         "reference_defaults": ref_out,
         "speedup_vs_plain": (plain_out["wall_s"] / wall_m) if plain_out else None,

@@ -275,6 +275,18 @@ int hf_mg_levels(const hf_mg* mg, int32_t* nlev, int32_t* dims, int32_t max_leve
 /* after a set-up: recomputes the coefficients on the default stream and
  * synchronises the device */
 int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float param);
+/* The learned network (the layers of hf_lpc_apply_f32: weights w1,b1,w2,b2,w3,b3 as
+ * device pointers, gfield the nnode input field, kept by reference) as the fine
+ * level's post-smoother: after the coarse correction one step
+ *   e += D^-1 res + s D^-1/2 mask N([D^-1/2 res / s, g]),  res = M (r - A e),
+ *   s = rms(D^-1/2 res), D = diag of the fine operator,
+ * replaces the Chebyshev post-smoothing (mode 1) or follows it (mode 2); mode 0
+ * switches it off.  The weights are packed on `stream` into the hierarchy's own
+ * buffer (allocated on the first call); a lattice-stencil fine level (not
+ * HF_MG_FINE_SELL) and at least two levels are required.  No reference
+ * counterpart (the reference preconditions with Jacobi, solver.py:68-69). */
+int hf_mg_set_learned(hf_mg* mg, int32_t mode, const float* const* weights, const float* gfield,
+                      hf_stream_t stream);
 /* lam of every level but the coarsest from the last set-up (synchronises the device) */
 int hf_mg_bounds(const hf_mg* mg, float* lam, int32_t max_levels);
 /* builds every level from the FP32 tangent (pattern + SELL values) and the

@@ -250,6 +250,18 @@ struct PollState {
 };
 int poll_init();
 int poll_post(const Scalars* sc, int slot, hipStream_t s, PollState* ps = nullptr);
+// the learned network as a multigrid fine-level smoother (lpc.hip, used by mg.hip):
+// e += D^-1 res + s D^-1/2 mask N([D^-1/2 res / s, g]), s = rms(D^-1/2 res), on the
+// nx*ny*nz node grid; its kernels exit at once while *skip != 0
+struct LpcSmoother {
+  void* buf = nullptr;            // lpc_smoother_bytes(nnode), packed by lpc_smoother_load
+  const float* gfield = nullptr;  // nnode input feature (caller's memory)
+  int nx = 0, ny = 0, nz = 0;
+};
+size_t lpc_smoother_bytes(int64_t nnode);
+int lpc_smoother_load(const LpcSmoother& L, const float* const* weights, hipStream_t s);
+int lpc_smoother_apply(const LpcSmoother& L, const float* res, const float* dinv, const float* mask, float* e,
+                       const int32_t* skip, hipStream_t s);
 int poll_wait(int slot, Scalars& out, PollState* ps = nullptr);
 
 }  // namespace hf

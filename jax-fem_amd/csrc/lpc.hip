@@ -452,9 +452,48 @@ __global__ void __launch_bounds__(256) conv3_apply_tile(const h4* __restrict__ h
   }
 }
 
+// Layer 3 (16 -> 3) as a smoothing step: e += D^-1 r + s D^-1/2 mask y
+__global__ void __launch_bounds__(256) conv3_accum_tile(const h4* __restrict__ h2, PackedNet net, CnnGrid g,
+                                                        const float* __restrict__ r, const float* __restrict__ dinv,
+                                                        const float* __restrict__ mask,
+                                                        const float* __restrict__ scale, float* __restrict__ e,
+                                                        const int32_t* skip) {
+  __shared__ h4 sh[HN * 4];
+  if (skip && *skip) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, kk = lane & 15, grp = lane >> 4;
+  h4 wa[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) wa[t] = net.w3[t * 64 + lane];
+  const f4 b0 = bias4(net.b3, grp, 3);
+  const float s = scale[0];
+  const TileGeom tg = tile_geom(g);
+  for (int64_t tile = blockIdx.x; tile < tg.ntiles; tile += gridDim.x) {
+    int i0, j0, k0;
+    tile_origin(tg, tile, i0, j0, k0);
+    stage16(h2, g, i0, j0, k0, sh);
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const int line = wv * 2 + x, li = line / TJ, lj = line % TJ;
+      const f4 acc = conv16_lds(wa, sh, li, lj, kk, grp, b0);
+      const int ii = i0 + li, jj = j0 + lj, kn = k0 + kk;
+      if (grp == 0 && ii < g.nx && jj < g.ny && kn < g.nz) {
+        const int64_t n = ((int64_t)ii * g.ny + jj) * g.nz + kn;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const int64_t d = n * 3 + i;
+          e[d] += dinv[d] * r[d] + s * sqrtf(dinv[d]) * mask[d] * acc[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // scale = rms(D^-1/2 r): partials then one workgroup
 __global__ void __launch_bounds__(kBlock) rdr_partials(const float* __restrict__ r, const float* __restrict__ dinv,
-                                                       int64_t N, double* partials) {
+                                                       int64_t N, double* partials, const int32_t* skip) {
+  if (skip && *skip) return;
   double loc[1] = {0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
     loc[0] += (double)r[i] * (double)r[i] * (double)dinv[i];
@@ -462,7 +501,8 @@ __global__ void __launch_bounds__(kBlock) rdr_partials(const float* __restrict__
 }
 
 __global__ void __launch_bounds__(kScalarBlock) rms_scalar(const double* partials, int nparts, int64_t N,
-                                                           float* scale) {
+                                                           float* scale, const int32_t* skip) {
+  if (skip && *skip) return;
   double tot[1];
   sum_partials<1>(partials, nparts, tot);
   if (threadIdx.x == 0) scale[0] = (float)sqrt(fmax(tot[0], 1e-300) / (double)N);
@@ -937,6 +977,68 @@ static inline size_t net_bytes(int64_t nnode) {
          2 * align256(nnode * 32);
 }
 
+namespace hf {
+// ---- the network as a multigrid fine-level smoother (mg.hip) -----------------
+// buffer: packed weights | biases (48 floats) | scale | partials | h1 | h2
+namespace {
+constexpr size_t kSmPacked = (size_t)(kPackW1 + kPackW2 + kPackW3) * sizeof(h4);
+inline size_t sm_off_bias() { return align256(kSmPacked); }
+inline size_t sm_off_scale() { return sm_off_bias() + 256; }
+inline size_t sm_off_part() { return sm_off_scale() + 256; }
+inline size_t sm_off_h1() { return sm_off_part() + align256((size_t)kMaxGrid * sizeof(double)); }
+__global__ void copy_bias(const float* __restrict__ b1, const float* __restrict__ b2, const float* __restrict__ b3,
+                          float* __restrict__ out) {
+  const int i = threadIdx.x;
+  if (i < 16) out[i] = b1[i];
+  else if (i < 32) out[i] = b2[i - 16];
+  else if (i < 35) out[i] = b3[i - 32];
+  else if (i < 48) out[i] = 0.f;
+}
+}  // namespace
+
+size_t lpc_smoother_bytes(int64_t nnode) { return sm_off_h1() + 2 * align256((size_t)nnode * 32); }
+
+int lpc_smoother_load(const LpcSmoother& L, const float* const* weights, hipStream_t s) {
+  for (int i = 0; i < 6; ++i) HF_REQUIRE(weights[i], "learned smoother: null weight %d", i);
+  char* buf = (char*)L.buf;
+  h4* packed = (h4*)buf;
+  const int tot = kPackW1 + kPackW2 + kPackW3;
+  pack_weights<<<(tot + 255) / 256, 256, 0, s>>>(weights[0], weights[2], weights[4], packed);
+  copy_bias<<<1, 64, 0, s>>>(weights[1], weights[3], weights[5], (float*)(buf + sm_off_bias()));
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+int lpc_smoother_apply(const LpcSmoother& L, const float* res, const float* dinv, const float* mask, float* e,
+                       const int32_t* skip, hipStream_t s) {
+  char* buf = (char*)L.buf;
+  const h4* packed = (const h4*)buf;
+  const float* bias = (const float*)(buf + sm_off_bias());
+  PackedNet net;
+  net.w1 = packed;
+  net.w2 = packed + kPackW1;
+  net.w3 = packed + kPackW1 + kPackW2;
+  net.b1 = bias;
+  net.b2 = bias + 16;
+  net.b3 = bias + 32;
+  float* scale = (float*)(buf + sm_off_scale());
+  double* part = (double*)(buf + sm_off_part());
+  const int64_t nnode = (int64_t)L.nx * L.ny * L.nz, N = 3 * nnode;
+  h4* h1 = (h4*)(buf + sm_off_h1());
+  h4* h2 = (h4*)(buf + sm_off_h1() + align256((size_t)nnode * 32));
+  const CnnGrid g = mkgrid(L.nx, L.ny, L.nz, 1);
+  const int ge = elem_grid(N);
+  rdr_partials<<<ge, kBlock, 0, s>>>(res, dinv, N, part, skip);
+  rms_scalar<<<1, kScalarBlock, 0, s>>>(part, ge, N, scale, skip);
+  const unsigned gt = tile_grid(g);
+  conv1_tile<<<gt, 256, 0, s>>>(res, dinv, L.gfield, scale, net, g, h1, skip);
+  conv2_tile<<<gt, 256, 0, s>>>(h1, net, g, h2, skip);
+  conv3_accum_tile<<<gt, 256, 0, s>>>(h2, net, g, res, dinv, mask, scale, e, skip);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+}  // namespace hf
+
 extern "C" size_t hf_lpc_apply_workspace_size(int64_t nnode) { return kHeader + 256 + net_bytes(nnode); }
 
 extern "C" int hf_lpc_apply_f32(const float* r, const float* dinv, const float* gfield, const float* mask,
@@ -961,8 +1063,8 @@ extern "C" int hf_lpc_apply_f32(const float* r, const float* dinv, const float* 
   const CnnGrid g = mkgrid(nx, ny, nz, 1);
   const PackedNet net = pack_net(weights, packed, s);
   const int ge = elem_grid(N);
-  rdr_partials<<<ge, kBlock, 0, s>>>(r, dinv, N, partials);
-  rms_scalar<<<1, kScalarBlock, 0, s>>>(partials, ge, N, scale);
+  rdr_partials<<<ge, kBlock, 0, s>>>(r, dinv, N, partials, nullptr);
+  rms_scalar<<<1, kScalarBlock, 0, s>>>(partials, ge, N, scale, nullptr);
   const unsigned gt = tile_grid(g);
   (void)f;
   conv1_tile<<<gt, 256, 0, s>>>(r, dinv, gfield, scale, net, g, h1, nullptr);

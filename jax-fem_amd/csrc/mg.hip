@@ -400,6 +400,21 @@ __global__ void __launch_bounds__(kMgBlock) mg_expand_half(const float* __restri
 // Inverse of each node's diagonal block (slot 13); masked DOFs (mask 0) get a
 // unit diagonal.  A block that is not safely invertible falls back to the
 // inverse of its positive diagonal entries (0 elsewhere).
+// 1 / diag(A) of the fine level (the scalar Jacobi the learned smoother's network
+// was trained with); 1 on Dirichlet DOFs, whose residual entries are masked to 0
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_scalar_dinv(const float* __restrict__ S, const float* __restrict__ mask,
+                                                           int64_t n, float* __restrict__ dinv) {
+  const int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (node >= n) return;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float a = S[(int64_t)(diag_slot(HALF) * 9 + 4 * i) * n + node];
+    const bool free = !mask || mask[3 * node + i] != 0.f;
+    dinv[3 * node + i] = (free && a != 0.f) ? 1.f / a : 1.f;
+  }
+}
+
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restrict__ S, const float* __restrict__ mask,
                                                           int64_t n, float* __restrict__ Dinv) {
@@ -1891,6 +1906,11 @@ struct hf_mg {
   // (the cycle's shape and every pointer and size the batch uses) matches; the
   // level data and coefficients are read from device memory, so a new set-up
   // reuses the graph
+  // learned fine-level smoother (hf_mg_set_learned): mode 1 replaces the fine
+  // post-smoothing by one network step, mode 2 adds the step after it
+  int lmode = 0;
+  LpcSmoother lsm;
+  float* sdinv = nullptr;           // 3 n0: 1 / diag of the fine operator (each set-up)
   uint64_t gen = 0;                 // bumped when the smoother's kind or degree changes
   hipStream_t cap = nullptr;        // capture stream (non-blocking, default priority)
   hipGraphExec_t gexec = nullptr;
@@ -1924,6 +1944,8 @@ static int mg_free(hf_mg* mg) {
   if (mg->flags) (void)hipFree(mg->flags);
   if (mg->part) (void)hipFree(mg->part);
   if (mg->coef) (void)hipFree(mg->coef);
+  if (mg->lsm.buf) (void)hipFree(mg->lsm.buf);
+  if (mg->sdinv) (void)hipFree(mg->sdinv);
   return 0;
 }
 
@@ -2136,6 +2158,11 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   const Lat gl = mg->lv[mg->nlev - 1].g;
   mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
   if (mg_coefficients(mg, s)) return -2;
+  if (mg->lmode) {
+    const MgLevel& F = mg->lv[0];
+    auto sd = F.half ? mg_scalar_dinv<true> : mg_scalar_dinv<false>;
+    sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(F.S, mask, F.g.n, mg->sdinv);
+  }
   HF_LAUNCH_CHECK();
   mg->mask = mask;
   // The host waits for the set-up only to check it: a pattern's first set-up (its
@@ -2341,6 +2368,16 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const int gg = node_grid(V.g.n);
     mg_prolong<<<gg, kMgBlock, 0, s>>>(cur[l + 1], V.g, C.g, l == 0 ? mg->mask : nullptr, cur[l], sc);
     if (l == 0 && skip_fine_post) break;
+    // the learned step: res = M (r - A e), then e += D^-1 res + s D^-1/2 mask N(res)
+    auto learned_step = [&]() -> int {
+      auto resid = V.half ? mg_poly_resid<true> : mg_poly_resid<false>;
+      resid<<<gg, kMgBlock, 0, s>>>(V.S, V.g, rin(0), cur[0], mg->mask, V.res, sc);
+      return lpc_smoother_apply(mg->lsm, V.res, mg->sdinv, mg->mask, cur[0], sc ? &sc->done : nullptr, s);
+    };
+    if (l == 0 && mg->lmode == 1) {
+      if (learned_step()) return -2;
+      continue;
+    }
     // post-smoothing: the same polynomial on the residual after the correction
     int perr = 0;
     const bool lanes = V.g.n <= kLaneNodes;
@@ -2391,9 +2428,42 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       up(std::false_type{});
     }
     if (perr) return perr;
+    if (l == 0 && mg->lmode == 2 && learned_step()) return -2;
   }
   HF_LAUNCH_CHECK();
   *out = cur[0];
+  return 0;
+}
+
+extern "C" int hf_mg_set_learned(hf_mg* mg, int32_t mode, const float* const* weights, const float* gfield,
+                                 hf_stream_t stream) {
+  HF_REQUIRE(mg, "hf_mg_set_learned: null hierarchy");
+  HF_REQUIRE(mode >= 0 && mode <= 2, "hf_mg_set_learned: mode %d (0 off, 1 replace, 2 after)", mode);
+  hipStream_t s = (hipStream_t)stream;
+  if (mode == 0) {
+    if (mg->lmode) ++mg->gen;
+    mg->lmode = 0;
+    return 0;
+  }
+  HF_REQUIRE(weights && gfield, "hf_mg_set_learned: null argument");
+  HF_REQUIRE(!mg->sell && mg->nlev > 1, "hf_mg_set_learned: needs a lattice-stencil fine level and a coarser level");
+  const MgLevel& F = mg->lv[0];
+  if (!mg->lsm.buf) {
+    HF_CHECK_HIP(hipMalloc(&mg->lsm.buf, lpc_smoother_bytes(F.g.n)));
+    HF_CHECK_HIP(hipMalloc((void**)&mg->sdinv, (size_t)F.g.n * 3 * sizeof(float)));
+  }
+  mg->lsm.nx = F.g.nx;
+  mg->lsm.ny = F.g.ny;
+  mg->lsm.nz = F.g.nz;
+  mg->lsm.gfield = gfield;
+  if (lpc_smoother_load(mg->lsm, weights, s)) return -2;
+  if (mg->ready) {
+    auto sd = F.half ? mg_scalar_dinv<true> : mg_scalar_dinv<false>;
+    sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(F.S, mg->mask, F.g.n, mg->sdinv);
+    HF_LAUNCH_CHECK();
+  }
+  mg->lmode = mode;
+  ++mg->gen;   // the captured solve batch has another shape
   return 0;
 }
 

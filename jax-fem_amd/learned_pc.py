@@ -120,6 +120,38 @@ class LearnedPC:
         return torch.as_tensor(acc / np.maximum(cnt, 1) / 3.0, dtype=torch.float32, device=problem.device)
 
     @staticmethod
+    def gfield_device(problem):
+        """gfield computed on the device from the problem's parameter tensor (the same
+        field: log E averaged from cells to nodes, / 3), cached per parameter tensor."""
+        mat = problem.material
+        nnode = problem.fe.num_total_nodes
+        prm = problem._params()
+        key = (id(prm), prm._version if prm is not None else None)
+        hit = getattr(problem, "_gfield_dev", None)
+        if hit is not None and hit[0] == key and hit[1] is prm:
+            return hit[2]
+        if prm is None or mat.param is None:
+            g = torch.zeros(nnode, dtype=torch.float32, device=problem.device)
+        else:
+            pc = prm.mean(dim=1)
+            if mat.param == "scale_E":
+                E = mat.E * pc / max(mat.E, 1e-30)
+            elif mat.param == "simp":
+                E = mat.Emin + (mat.E0 - mat.Emin) * pc ** mat.penal
+            else:
+                E = pc
+            logE = torch.log(torch.clamp(E, min=1e-30))
+            _, cells = problem._dev()
+            idx = cells.long().reshape(-1)
+            acc = torch.zeros(nnode, dtype=torch.float64, device=problem.device)
+            cnt = torch.zeros(nnode, dtype=torch.float64, device=problem.device)
+            acc.index_add_(0, idx, logE.repeat_interleave(cells.shape[1]))
+            cnt.index_add_(0, idx, torch.ones_like(idx, dtype=torch.float64))
+            g = (acc / torch.clamp(cnt, min=1.0) / 3.0).float().contiguous()
+        problem._gfield_dev = (key, prm, g)
+        return g
+
+    @staticmethod
     def mask(problem):
         m = torch.ones(problem.N, dtype=torch.float32, device=problem.device)
         d, _ = problem.bc_arrays()

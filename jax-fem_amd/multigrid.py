@@ -138,17 +138,50 @@ class Multigrid:
             self.handle = None
 
     @classmethod
-    def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None, fine_full=False):
-        """The problem's cached hierarchy (one per lattice) with the given smoother."""
+    def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None, fine_full=False, model=None,
+                    learned_mode="replace"):
+        """The problem's cached hierarchy (one per lattice) with the given smoother.
+        smoother 'learned': Chebyshev everywhere, and on the fine level the learned
+        network `model` (a LearnedPC) as the post-smoother (learned_mode 'replace') or
+        one network step after the Chebyshev post-smoothing ('after')."""
         grid = lattice(problem)
         sell = problem.fe.ele_type == "HEX27"
+        poly = "chebyshev" if smoother == "learned" else smoother
         mg = getattr(problem, "_mg", None)
         if mg is None or mg.grid != grid or mg.fine_full != bool(fine_full) or mg.fine_sell != sell:
-            mg = cls(grid, smoother, degree, param, device=problem.device, fine_full=fine_full, fine_sell=sell)
+            mg = cls(grid, poly, degree, param, device=problem.device, fine_full=fine_full, fine_sell=sell)
             problem._mg = mg
         else:
-            mg.set_smoother(smoother, degree, param)
+            mg.set_smoother(poly, degree, param)
+        if smoother == "learned":
+            if model is None:
+                raise ValueError("the 'learned' smoother needs linear option 'model' (a LearnedPC)")
+            mg.set_learned(model, problem, learned_mode)
+        else:
+            mg.set_learned(None)
         return mg
+
+    def set_learned(self, model, problem=None, mode="replace"):
+        """The fine level's post-smoothing by the learned network (hf_mg_set_learned):
+        one step e += M(r - A e), M(v) = D^-1 v + s D^-1/2 mask N([D^-1/2 v / s, g]),
+        in place of the Chebyshev post-smoothing ('replace') or after it ('after');
+        model None switches it off.  The network's input field g is the problem's
+        log-modulus at the nodes (LearnedPC.gfield), refreshed when its parameters change."""
+        if model is None:
+            if getattr(self, "_learned", None) is not None:
+                _lib.call("hf_mg_set_learned", self.handle, 0, None, None, _lib.stream())
+                self._learned = None
+            return
+        from .learned_pc import LearnedPC
+        modes = {"replace": 1, "after": 2}
+        if mode not in modes:
+            raise ValueError("learned_mode must be 'replace' or 'after'")
+        g = LearnedPC.gfield_device(problem)
+        key = (id(model), modes[mode], g.data_ptr(), getattr(model, "version", 0))
+        if getattr(self, "_learned", None) is not None and self._learned[0] == key:
+            return
+        _lib.call("hf_mg_set_learned", self.handle, modes[mode], model.weight_ptrs(), _lib.ptr(g), _lib.stream())
+        self._learned = (key, model, g)     # keeps the weights and the field alive
 
     def set_smoother(self, smoother="chebyshev", degree=2, param=None):
         """smoother 'chebyshev' (param: interval ratio, default 8) or 'jacobi' (param: omega,
@@ -305,12 +338,12 @@ def _workspace(device, nbytes):
 
 
 def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=1, smoother="chebyshev", degree=2,
-                        param=None, rebuild="solve", fine_full=False, stats=None):
+                        param=None, rebuild="solve", fine_full=False, stats=None, model=None, learned_mode="replace"):
     """Multigrid FCG; on breakdown or non-finite values Jacobi-PCG solves to the same
     absolute target from the better of the FCG iterate and x0.  `stats` (the Newton
     solve's KrylovStats) identifies the solve for rebuild='solve'.  Returns (x, info, fallback)."""
     from .solver import krylov
-    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_full)
+    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_full, model, learned_mode)
     x_start = (torch.zeros(A.shape[0], dtype=torch.float32, device=b.device) if x0 is None else
                x0.to(torch.float32).clone())
     x, info = mg.solve(A, b, x_start, rtol, atol, maxiter, rel_r0, poll, stats, problem=problem, rebuild=rebuild,

@@ -198,7 +198,11 @@ class Problem:
     def bc_arrays(self):
         """Unique BC DOFs (int32, device) and values (f64, device); later specs win (solver.py:298-300)."""
         fe = self.fe
-        key = (_fingerprint(fe.vals_list), tuple(id(n) for n in fe.node_inds_list))
+        # the value arrays are made by FiniteElement (immutable in the reference, JAX
+        # arrays): keyed by identity, with references kept so no id is reused;
+        # torch tensors also by their version counter (an in-place edit bumps it)
+        key = (tuple((id(v), v._version if torch.is_tensor(v) else -1) for v in fe.vals_list),
+               tuple(id(n) for n in fe.node_inds_list))
         if self._bc_key != key:
             # the DOF set depends on the node arrays and components only; a load step
             # that changes the values alone reuses it (location functions are cached

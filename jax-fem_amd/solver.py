@@ -18,9 +18,11 @@ a legacy flat dict, solver.py:1014-1068).  Linear keys:
   * ``learned`` {model: LearnedPC, tol, atol, maxiter, stall}: flexible CG with
     the learned CNN preconditioner on the FP32 tangent (precision 'mixed'),
     Jacobi-PCG as its fallback; the Newton residual stays the FP64 judge.
-  * ``mg`` {tol, atol, maxiter, smoother, degree, param, rebuild, fine_full, poll}: flexible CG
-    preconditioned by geometric-multigrid V-cycles on the FP32 tangent (precision
-    'mixed'; box_mesh HEX8 lattices, multigrid.py), Jacobi-PCG as its fallback;
+  * ``mg`` {tol, atol, maxiter, smoother, degree, param, rebuild, fine_full, poll, model,
+    learned_mode}: flexible CG preconditioned by geometric-multigrid V-cycles on the
+    FP32 tangent (precision 'mixed'; box_mesh HEX8 lattices, multigrid.py), Jacobi-PCG
+    as its fallback; smoother 'learned' with model=LearnedPC makes the learned
+    network the fine level's post-smoother;
     rebuild 'solve' (default: the hierarchy of a Newton solve's first tangent
     preconditions its later steps), 'always' or 'never'.
   * ``custom_solver``: callable(A, b, x0, linear_options) -> x (solver.py:278-280);
@@ -283,7 +285,8 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
                                           rel_r0=rtol is not None, poll=o.get("poll", 1),
                                           smoother=o.get("smoother", "chebyshev"), degree=o.get("degree", 2),
                                           param=o.get("param"), rebuild=o.get("rebuild", "solve"),
-                                          fine_full=o.get("fine_full", False), stats=stats)
+                                          fine_full=o.get("fine_full", False), stats=stats, model=o.get("model"),
+                                          learned_mode=o.get("learned_mode", "replace"))
         if stats is not None:
             stats.calls[-1]["fallback"] = fb
         return x
@@ -320,7 +323,8 @@ def _stencil_multigrid(problem, lin_opts, adt):
         lattice(problem)
     except NotImplementedError:
         return None
-    return Multigrid.for_problem(problem, o.get("smoother", "chebyshev"), o.get("degree", 2), o.get("param"), False)
+    return Multigrid.for_problem(problem, o.get("smoother", "chebyshev"), o.get("degree", 2), o.get("param"), False,
+                                 o.get("model"), o.get("learned_mode", "replace"))
 
 
 def solver(problem, solver_options={}):

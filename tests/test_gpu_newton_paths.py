@@ -3,7 +3,8 @@
 [0,1]x[0,1]x[0,0.5]) and the reference's Newton options.
 
 * the heterogeneous problem on a 16x16x8 lattice through the FP64 path, the
-  mixed multigrid path and the learned-preconditioner path, each against
+  mixed multigrid path (Chebyshev and learned fine-level smoothing) and the
+  learned-preconditioner path, each against
   ``oracle.solver.newton_solve`` (direct FP64 solve): rel-L2 <= 1e-8 (FP64),
   <= 1e-6 (mixed / learned);
 * the same problem at 128x128x64 (3,244,995 DOF) through the multigrid path:
@@ -80,6 +81,10 @@ def test_hetero_small_lattice_vs_oracle(jfa):
     model = lpc.LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_elastic.npz"))
     runs.append(("mixed learned", {"newton": {"precision": "mixed", "tol": 0.0,
                                               "linear": {"learned": {"model": model}}}}, 1e-6))
+    smoother = lpc.LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_mg_smoother.npz"))
+    runs.append(("mixed multigrid, learned smoother",
+                 {"newton": {"precision": "mixed", "tol": 0.0,
+                             "linear": {"mg": {"smoother": "learned", "model": smoother}}}}, 1e-6))
     for name, opts, bar in runs:
         u = jfa.solver(prob, opts)[0].reshape(-1).cpu().numpy()
         rel = np.linalg.norm(u - uo) / nrm
