@@ -93,6 +93,32 @@ int hf_cell_param_vjp_f64(int32_t ele, const hf_material* mat, const double* poi
                           const double* params, const double* lam, double* grad,
                           hf_stream_t stream);
 
+/* ---- element-agnostic cell kernels for a constitutive map known only pointwise:
+ *      the contractions of the reference's laplace and mass kernels and their
+ *      jacfwd element Jacobians (problem.py:189-236, 262-266) around a law the
+ *      caller evaluates at the quadrature points.  Geometry per (cell, point):
+ *      shape_grads (ncell, nq, nn, dim) physical gradients, jxw (ncell, nq)
+ *      (fe.py:112-141); shape_vals (nq, nn) reference values.  vec 1..3, dim 2
+ *      or 3, nn <= 64.
+ * u_grad (ncell, nq, vec, dim) = sum_a u[cells[c,a]] (x) dN_a/dx; u_quad (ncell, nq, vec)
+ * (optional) = sum_a u N_a. */
+int hf_generic_ugrad_f64(const int32_t* cells, int64_t ncell, int32_t nn, int32_t nq, int32_t vec, int32_t dim,
+                         const double* u, const double* shape_grads, const double* shape_vals, double* u_grad,
+                         double* u_quad, hf_stream_t stream);
+/* elem_res (ncell, nn, vec) = sum_q JxW (P : grad N_a + m N_a); P (ncell,nq,vec,dim) and
+ * m (ncell,nq,vec) each optional */
+int hf_generic_residual_f64(int64_t ncell, int32_t nn, int32_t nq, int32_t vec, int32_t dim,
+                            const double* shape_grads, const double* shape_vals, const double* jxw,
+                            const double* P, const double* m, double* elem_res, hf_stream_t stream);
+/* elem_jac block-major (ncell, nn, nn, vec, vec) = sum_q JxW (grad N_a . C . grad N_b + N_a Mt N_b);
+ * C (ncell,nq,vec,dim,vec,dim) = dP/du_grad and Mt (ncell,nq,vec,vec) = dm/du, each optional */
+int hf_generic_jacobian_f64(int64_t ncell, int32_t nn, int32_t nq, int32_t vec, int32_t dim,
+                            const double* shape_grads, const double* shape_vals, const double* jxw,
+                            const double* C, const double* Mt, double* elem_jac, hf_stream_t stream);
+int hf_generic_jacobian_f32(int64_t ncell, int32_t nn, int32_t nq, int32_t vec, int32_t dim,
+                            const double* shape_grads, const double* shape_vals, const double* jxw,
+                            const double* C, const double* Mt, float* elem_jac, hf_stream_t stream);
+
 /* ---- assembly: replaces compute_residual_vars_helper's scatter-add
  *      (problem.py:426-437) and PETSc setValuesCOO + assemble (solver.py:485-488)
  *      by deterministic segmented reductions over the pattern. -------------- */

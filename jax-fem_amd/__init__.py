@@ -4,7 +4,8 @@ Import name ``jax_fem_amd`` (the directory is ``jax-fem_amd``).  Public API
 mirrors the reference package: ``Problem`` (jax_fem/problem.py),
 ``solver`` / ``ad_wrapper`` / ``implicit_vjp`` / ``get_A`` / ``linear_solver``
 (jax_fem/solver.py), ``Mesh`` / ``box_mesh`` (jax_fem/generate_mesh.py) and
-``FiniteElement`` (jax_fem/fe.py).  Compute runs in libhipfem.so through a C
+``FiniteElement`` (jax_fem/fe.py); ``jax_fem_amd.numpy`` with ``grad`` / ``vmap``
+/ ``jacfwd`` take the place of jax.numpy and jax's transforms in user maps.  Compute runs in libhipfem.so through a C
 ABI (include/hipfem.h); there is no CPU fallback.
 """
 import logging
@@ -17,6 +18,9 @@ from .fe import FiniteElement
 from .materials import LinearElastic, NeoHookean, Poisson, Material
 from .mesh import Mesh, box_mesh, box_mesh_hex27, get_meshio_cell_type
 from .problem import Problem
+from .utils import save_sol, read_vtu, write_vtu, modify_vtu_file
+from . import numpy
+from .autodiff import grad, vmap, jacfwd, jacrev
 from .learned_pc import LearnedPC
 from .multigrid import Multigrid
 from .solver import (solver, ad_wrapper, implicit_vjp, get_A, linear_solver, apply_bc_vec, assign_bc, copy_bc,
@@ -32,4 +36,5 @@ if not logger.handlers:
 __all__ = ["HipFemError", "Problem", "solver", "ad_wrapper", "implicit_vjp", "get_A", "linear_solver", "apply_bc_vec",
            "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "get_meshio_cell_type",
            "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov",
-           "LearnedPC", "Multigrid"]
+           "LearnedPC", "Multigrid", "numpy", "grad", "vmap", "jacfwd", "jacrev",
+           "save_sol", "read_vtu", "write_vtu", "modify_vtu_file"]

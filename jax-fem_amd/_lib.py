@@ -52,6 +52,10 @@ _SIGS = {
     "hf_cell_jacobian_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
     "hf_cell_jacobian_f32": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
     "hf_cell_param_vjp_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp, vp]),
+    "hf_generic_ugrad_f64": (c_i32, [vp, c_i64, c_i32, c_i32, c_i32, c_i32, vp, vp, vp, vp, vp, vp]),
+    "hf_generic_residual_f64": (c_i32, [c_i64, c_i32, c_i32, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
+    "hf_generic_jacobian_f64": (c_i32, [c_i64, c_i32, c_i32, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
+    "hf_generic_jacobian_f32": (c_i32, [c_i64, c_i32, c_i32, c_i32, c_i32, vp, vp, vp, vp, vp, vp, vp]),
     "hf_gather_residual_f64": (c_i32, [vp, vp, vp, vp, vp]),
     "hf_gather_residual_box_f64": (c_i32, [vp, c_i32, c_i32, c_i32, vp, vp, vp, vp, vp]),
     "hf_matrix_values": (c_i64, [vp]),

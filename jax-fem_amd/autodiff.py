@@ -1,0 +1,112 @@
+"""Transforms for user-written pointwise maps: ``grad``, ``vmap``, ``jacfwd``.
+
+The reference's users differentiate and vectorise their maps with ``jax.grad``
+and ``jax.vmap`` (e.g. ``P_fn = jax.grad(psi)`` in
+tests/benchmarks/hyperelasticity/test_hyper_elasticity.py:15-34, and
+``jax.vmap(jax.vmap(stress_return_map))`` in
+tests/benchmarks/plasticity/test_plasticity.py:66-78).  These are the same
+transforms here:
+
+* on torch tensors they are torch.func's (exact forward- and reverse-mode
+  derivatives, batched evaluation), run with ``jax_fem_amd.numpy`` creating its
+  constants on the tensors' device;
+* on NumPy arrays (post-processing, the Problem's probe) ``grad`` and ``jacfwd``
+  run torch.func on CPU tensors and return NumPy arrays when the map is written
+  with ``jax_fem_amd.numpy``; a map written with NumPy itself is differentiated
+  by the complex step, f'(x) = Im f(x + i h) / h with h = 1e-30 (Squire and
+  Trapp 1998; exact to rounding for analytic maps such as the hyperelastic
+  energies); ``vmap`` maps over the leading axis.
+"""
+import numpy as np
+import torch
+
+from . import numpy as jnp
+
+_H = 1e-30
+
+
+def _first_tensor(args):
+    for a in args:
+        if torch.is_tensor(a):
+            return a
+        if isinstance(a, (list, tuple)):
+            t = _first_tensor(a)
+            if t is not None:
+                return t
+    return None
+
+
+def _to_device(args, dev):
+    out = []
+    for a in args:
+        if isinstance(a, np.ndarray) or isinstance(a, (float, int)) and not isinstance(a, bool):
+            a = torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev)
+        out.append(a)
+    return out
+
+
+def vmap(f, in_axes=0, out_axes=0):
+    def run(*args):
+        t = _first_tensor(args)
+        if t is None:
+            axes = in_axes if isinstance(in_axes, (list, tuple)) else [in_axes] * len(args)
+            n = next(np.shape(a)[ax] for a, ax in zip(args, axes) if ax is not None)
+            outs = [f(*[a if ax is None else np.take(a, k, axis=ax) for a, ax in zip(args, axes)]) for k in range(n)]
+            if isinstance(outs[0], tuple):
+                return tuple(np.stack([o[j] for o in outs], axis=out_axes) for j in range(len(outs[0])))
+            return np.stack([np.asarray(o) for o in outs], axis=out_axes)
+        dev = t.device
+        with jnp.on_device(dev):
+            return torch.func.vmap(f, in_dims=in_axes, out_dims=out_axes)(*_to_device(args, dev))
+    return run
+
+
+def _complex_step(f, args, argnums, vector_out):
+    x = np.asarray(args[argnums], dtype=np.float64)
+    flat = x.reshape(-1)
+    cols = []
+    for k in range(flat.size):
+        xc = flat.astype(np.complex128)
+        xc[k] += 1j * _H
+        a = list(args)
+        a[argnums] = xc.reshape(x.shape)
+        cols.append(np.imag(np.asarray(f(*a))) / _H)
+    if not vector_out:
+        return np.array(cols).reshape(x.shape)
+    y = np.stack(cols, axis=-1)                      # (out..., in)
+    return y.reshape(y.shape[:-1] + x.shape)
+
+
+def _numpy_call(transform, f, args, argnums, vector_out):
+    """A transform on NumPy arguments: torch.func on CPU tensors when the map is
+    written with jax_fem_amd.numpy (exact, and nested transforms compose), else the
+    complex step on the NumPy map."""
+    try:
+        with jnp.on_device(torch.device("cpu")):
+            out = transform(f, argnums=argnums)(*_to_device(args, torch.device("cpu")))
+        return out.detach().numpy() if torch.is_tensor(out) else np.asarray(out)
+    except Exception:
+        return _complex_step(f, args, argnums, vector_out)
+
+
+def grad(f, argnums=0):
+    def run(*args):
+        t = _first_tensor(args)
+        if t is None:
+            return _numpy_call(torch.func.grad, f, args, argnums, False)
+        with jnp.on_device(t.device):
+            return torch.func.grad(f, argnums=argnums)(*_to_device(args, t.device))
+    return run
+
+
+def jacfwd(f, argnums=0):
+    def run(*args):
+        t = _first_tensor(args)
+        if t is None:
+            return _numpy_call(torch.func.jacfwd, f, args, argnums, True)
+        with jnp.on_device(t.device):
+            return torch.func.jacfwd(f, argnums=argnums)(*_to_device(args, t.device))
+    return run
+
+
+jacrev = jacfwd

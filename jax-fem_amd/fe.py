@@ -212,8 +212,59 @@ class FiniteElement:
         grads = np.einsum("qnd,cqde->cqne", self.shape_grads_ref, inv)
         return grads, det * self.quad_weights[None]
 
+    def _geometry(self):
+        if getattr(self, "_geom", None) is None:
+            self._geom = self.get_shape_grads()
+        return self._geom
+
+    @property
+    def shape_grads(self):
+        """Physical shape gradients (num_cells, num_quads, num_nodes, dim) (problem.py:137)."""
+        return self._geometry()[0]
+
+    @property
     def JxW(self):
-        return self.get_shape_grads()[1]
+        """det(J) * w per (cell, quadrature point) (num_cells, num_quads) (problem.py:137)."""
+        return self._geometry()[1]
+
+    @property
+    def v_grads_JxW(self):
+        """(num_cells, num_quads, num_nodes, 1, dim) (problem.py:139)."""
+        return self.shape_grads[:, :, :, None, :] * self.JxW[:, :, None, None, None]
+
+    def _cell_values(self, sol):
+        """sol (num_total_nodes, vec) gathered per cell; torch stays torch (on its device)."""
+        if _is_torch(sol):
+            import torch
+            return sol.reshape(self.num_total_nodes, self.vec)[torch.as_tensor(self.cells, device=sol.device)]
+        return np.asarray(sol).reshape(self.num_total_nodes, self.vec)[self.cells]
+
+    def sol_to_grad(self, sol):
+        """u_grads (num_cells, num_quads, vec, dim) from a nodal solution (fe.py:366-382)."""
+        cs = self._cell_values(sol)
+        if _is_torch(cs):
+            import torch
+            g = torch.as_tensor(self.shape_grads, dtype=cs.dtype, device=cs.device)
+            return torch.einsum("cni,cqnd->cqid", cs, g)
+        return np.einsum("cni,cqnd->cqid", cs, self.shape_grads)
+
+    def convert_from_dof_to_quad(self, sol):
+        """u at the quadrature points (num_cells, num_quads, vec) (fe.py:324-341)."""
+        cs = self._cell_values(sol)
+        if _is_torch(cs):
+            import torch
+            v = torch.as_tensor(self.shape_vals, dtype=cs.dtype, device=cs.device)
+            return torch.einsum("cni,qn->cqi", cs, v)
+        return np.einsum("cni,qn->cqi", cs, self.shape_vals)
+
+    def convert_from_dof_to_face_quad(self, sol, boundary_inds):
+        """u at the face quadrature points (num_selected_faces, num_face_quads, vec) (fe.py:343-364)."""
+        cs = self._cell_values(sol)[boundary_inds[:, 0]]
+        vals = self.face_shape_vals[boundary_inds[:, 1]]
+        if _is_torch(cs):
+            import torch
+            return torch.einsum("sni,sqn->sqi", cs, torch.as_tensor(vals, dtype=cs.dtype, device=cs.device))
+        return np.einsum("sni,sqn->sqi", cs, vals)
 
     def get_physical_quad_points(self):
         return np.einsum("qn,cnd->cqd", self.shape_vals, self.points[self.cells])

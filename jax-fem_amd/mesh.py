@@ -22,18 +22,68 @@ HEX8_OFFSETS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
                          [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
 
 
+_CELL_TYPES = {"HEX8": "hexahedron", "HEX27": "hexahedron27", "HEX20": "hexahedron20", "TET4": "tetra",
+               "TET10": "tetra10", "QUAD4": "quad", "QUAD8": "quad8", "TRI3": "triangle", "TRI6": "triangle6"}
+_NN_TO_TYPE = {8: "HEX8", 27: "HEX27", 20: "HEX20", 10: "TET10"}
+
+
 class Mesh:
-    """points (nnode, dim) float64, cells (ncell, nn) int, optional ele_type."""
+    """points (nnode, dim) float64, cells (ncell, nn) int, optional ele_type.
+
+    Also the reading side of a meshio mesh as the reference's generators return
+    it (generate_mesh.py:120-189): ``cells_dict`` maps the meshio cell-type name to
+    the connectivity, so ``Mesh(m.points, m.cells_dict[get_meshio_cell_type(e)])``
+    works on the meshes made here.  ``node_grid`` is the structured node lattice
+    (nx, ny, nz) when the points are one, in box_mesh order
+    n = (i*ny + j)*nz + k (set by the box generators, otherwise detected from the
+    coordinates), else None."""
 
     def __init__(self, points, cells, ele_type=None):
         self.points = np.asarray(points, dtype=np.float64)
         self.cells = np.asarray(cells)
         self.ele_type = ele_type
+        self.point_data = {}
+
+    @property
+    def cells_dict(self):
+        cd = self.__dict__.get("_cells_dict")
+        if cd is not None:
+            return cd
+        e = self.ele_type or _NN_TO_TYPE.get(self.cells.shape[1], "TET4" if self.cells.shape[1] == 4 and
+                                             self.points.shape[1] == 3 else "QUAD4")
+        return {_CELL_TYPES[e]: self.cells}
+
+    @property
+    def node_grid(self):
+        if "_node_grid" not in self.__dict__:
+            self._node_grid = _detect_lattice(self.points)
+        return self._node_grid
+
+    @node_grid.setter
+    def node_grid(self, g):
+        self._node_grid = g
 
     def count_selected_faces(self, location_fn):
         from .fe import FiniteElement
         fe = FiniteElement(self, vec=1, dim=self.points.shape[1], ele_type=self.ele_type or "HEX8")
         return len(fe.get_boundary_conditions_inds([location_fn])[0])
+
+
+def _detect_lattice(points):
+    """(nx, ny, nz) when the points are the tensor lattice of their sorted unique
+    coordinates in box_mesh order (x slowest, z fastest), else None."""
+    if points.ndim != 2 or points.shape[1] != 3 or len(points) < 8:
+        return None
+    ax = [np.unique(points[:, d]) for d in range(3)]
+    dims = tuple(len(a) for a in ax)
+    if int(np.prod(dims)) != len(points) or min(dims) < 2:
+        return None
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    ref = np.stack((X, Y, Z), axis=-1).reshape(-1, 3)
+    scale = max(1.0, float(np.abs(points).max()))
+    if not np.allclose(points, ref, rtol=0.0, atol=1e-12 * scale):
+        return None
+    return dims
 
 
 def _lattice(n, L):
@@ -71,5 +121,7 @@ def box_mesh_hex27(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0):
 
 
 def get_meshio_cell_type(ele_type):
-    """Name of the cell block for an element type (generate_mesh.py helpers)."""
-    return {"HEX8": "hexahedron", "HEX27": "hexahedron27"}[ele_type]
+    """meshio's name of the cell block of an element type (generate_mesh.py:89-117)."""
+    if ele_type not in _CELL_TYPES:
+        raise NotImplementedError(ele_type)
+    return _CELL_TYPES[ele_type]

@@ -103,24 +103,165 @@ class Problem:
         raise NotImplementedError("Child class must implement this function!")
 
     def _check_hooks(self):
+        """Collect the user hooks (problem.py:277-346).  A tensor map that is a
+        jax_fem_amd.materials law runs on the built cell kernels; a plain function is
+        probed at its first evaluation (a built law it equals runs on those kernels)
+        and otherwise evaluated pointwise on the device (generic.PointwiseTerms)."""
         nsurf = len(self.boundary_inds_list)
         if hasattr(self, "get_surface_maps"):
             assert nsurf == len(self.get_surface_maps())
+        elif hasattr(self, "get_universal_kernels_surface"):
+            assert nsurf == len(self.get_universal_kernels_surface())
         else:
             assert nsurf == 0, "Missing definitions for surface integral"
-        if hasattr(self, "get_universal_kernel") or hasattr(self, "get_universal_kernels_surface"):
-            raise NotImplementedError("universal kernels have no HIP implementation yet")
+        self.material = None
+        self._tensor_fn = None
         if hasattr(self, "get_tensor_map"):
             m = self.get_tensor_map()
-            if not isinstance(m, Material):
-                raise NotImplementedError(
-                    "get_tensor_map() must return a jax_fem_amd.materials law (LinearElastic, NeoHookean, "
-                    "Poisson); an arbitrary Python map has no HIP kernel")
-            if m.vec != self.vec:
-                raise ValueError("material expects vec=%d, problem has vec=%d" % (m.vec, self.vec))
-            self.material = m
+            if isinstance(m, Material):
+                if m.vec != self.vec:
+                    raise ValueError("material expects vec=%d, problem has vec=%d" % (m.vec, self.vec))
+                self.material = m
+            elif callable(m):
+                self._tensor_fn = m
+            else:
+                raise TypeError("get_tensor_map() must return a callable")
+        elif not (hasattr(self, "get_universal_kernel") or hasattr(self, "get_mass_map")):
+            raise NotImplementedError("the problem defines no cell term (tensor map, mass map or universal kernel)")
+        self._plan_cache = None
+
+    # ------------------------------------------------------------ the plan
+    probe_laws = True   # recognise a plain-function tensor map as a built law when it equals one
+
+    def volume_internal_vars(self):
+        """internal_vars as device FP64 tensors of shape (ncell, nq, ...) (a per-cell
+        array (ncell,) is repeated over the points).  Held for a whole solve."""
+        held = getattr(self, "_held_ivs", None)
+        if held is not None:
+            return held
+        out = []
+        for a in (self.internal_vars or ()):
+            t = torch.as_tensor(a.detach() if torch.is_tensor(a) else np.asarray(a), dtype=torch.float64,
+                                device=self.device)
+            if t.dim() == 1 and t.shape[0] == self.num_cells:
+                t = t[:, None].expand(self.num_cells, self.fe.num_quads)
+            if tuple(t.shape[:2]) != (self.num_cells, self.fe.num_quads):
+                raise ValueError("internal_vars entries must have leading shape (num_cells, num_quads)")
+            out.append(t.contiguous())
+        return out
+
+    def _plan(self):
+        """How each term of the weak form is evaluated, decided once per internal
+        variable layout: (law, mass, surfaces, universal).  law: 'fast' (built cell
+        kernels) or 'generic'; mass: None, 'load' (u-independent: a load vector) or
+        'cell'; surfaces: per surface 'load' or 'face' (u-dependent or universal)."""
+        ivs = self.internal_vars or ()
+        layout = tuple(tuple(np.shape(a)) for a in ivs)
+        if self._plan_cache is not None and self._plan_cache[0] == layout:
+            return self._plan_cache[1]
+        from .materials import identify
+        law = "generic"
+        fn = self._tensor_fn
+        if self.material is not None:
+            law = "fast" if self._fast_supported(self.material) else "generic"
+            fn = self.material            # a built law runs pointwise where its kernels do not apply
+        elif self._tensor_fn is not None and self.probe_laws:
+            scalar = len(layout) == 1 and len(layout[0]) == 2
+            if len(layout) == 0 or scalar:
+                m = identify(self._tensor_fn, self.vec, self.dim, scalar_param=scalar)
+                if m is not None and self._fast_supported(m):
+                    self.material = m
+                    law = "fast"
+                    import logging
+                    logging.getLogger("jax_fem_amd").info(
+                        "tensor map recognised as %s(E=%.6g, nu=%.6g%s): built cell kernels" % (
+                            type(m).__name__, getattr(m, "E", 0.0), getattr(m, "nu", 0.0),
+                            ", param=%s" % m.param if m.param else ""))
+        if law == "fast" and (len(layout) > 1 or (len(layout) == 1 and len(layout[0]) not in (1, 2))):
+            raise ValueError("a built law takes one scalar parameter per quadrature point")
+        mass = None
+        if hasattr(self, "get_mass_map"):
+            mass = "cell" if (len(layout) > 0 or self._depends_on_u(self.get_mass_map(), "mass")) else "load"
+        surfaces = []
+        smaps = self.get_surface_maps() if hasattr(self, "get_surface_maps") else [None] * len(self.boundary_inds_list)
+        for k, smap in enumerate(smaps):
+            if hasattr(self, "get_universal_kernels_surface"):
+                surfaces.append("face")
+            elif smap is None:
+                surfaces.append(None)
+            else:
+                surfaces.append("face" if self._depends_on_u(smap, "surface", k) else "load")
+        plan = {"law": law, "fn": fn if law == "generic" else None, "mass": mass, "surfaces": surfaces,
+                "universal": hasattr(self, "get_universal_kernel")}
+        plan["pure_fast"] = law == "fast" and mass != "cell" and "face" not in surfaces and not plan["universal"]
+        self._plan_cache = (layout, plan)
+        self._terms = None
+        return plan
+
+    def _fast_supported(self, m):
+        if self.dim != 3 or self.fe.ele_type not in ("HEX8", "HEX27") or m.vec != self.vec:
+            return False
+        q = (int(self.fe.quadrature_order) + 2) // 2
+        return (self.fe.ele_type == "HEX8" and q == 2) or (self.fe.ele_type == "HEX27" and q == 3)
+
+    def _depends_on_u(self, fn, kind, k=None):
+        """Probe a mass or surface map at a few points with random u against u = 0."""
+        fe = self.fe
+        rng = np.random.default_rng(1)
+        if kind == "mass":
+            x = fe.get_physical_quad_points().reshape(-1, self.dim)
+            extra = []
         else:
-            raise NotImplementedError("a get_tensor_map() law is required by the GPU path")
+            bi = self.boundary_inds_list[k]
+            if len(bi) == 0:
+                return False
+            x = fe.get_physical_surface_quad_points(bi).reshape(-1, self.dim)
+            extra = [np.asarray(a.detach().cpu() if torch.is_tensor(a) else a).reshape(len(x), -1)
+                     for a in (self.internal_vars_surfaces[k] or ())]
+        idx = sorted({(len(x) - 1) * j // 3 for j in range(4)})
+        for i in idx:
+            e = [a[i].reshape(np.shape(a[i])) for a in extra]
+            try:
+                y0 = np.asarray(fn(np.zeros(self.vec), x[i], *e), dtype=np.float64)
+                y1 = np.asarray(fn(rng.standard_normal(self.vec), x[i], *e), dtype=np.float64)
+            except Exception:
+                return True     # not evaluable on NumPy arrays: take the general (device) path
+            if not np.array_equal(y0, y1):
+                return True
+        return False
+
+    def _pointwise(self):
+        """The device evaluator of the problem's pointwise maps (built on first use)."""
+        plan = self._plan()
+        if getattr(self, "_terms", None) is None:
+            from .generic import PointwiseTerms
+            t = PointwiseTerms(self)
+            t.tensor_fn = plan["fn"]
+            t.mass_fn = self.get_mass_map() if plan["mass"] == "cell" else None
+            t.universal = self.get_universal_kernel() if plan["universal"] else None
+            fe = self.fe
+            smaps = self.get_surface_maps() if hasattr(self, "get_surface_maps") else None
+            usurf = self.get_universal_kernels_surface() if hasattr(self, "get_universal_kernels_surface") else None
+            for k, (bi, how) in enumerate(zip(self.boundary_inds_list, plan["surfaces"])):
+                if how != "face" or len(bi) == 0:
+                    t.faces.append(None)
+                    continue
+                grads, nanson = fe.get_face_shape_grads(bi)
+                dev = self.device
+                T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+                isv = []
+                for a in (self.internal_vars_surfaces[k] or ()):
+                    isv.append(torch.as_tensor(a.detach() if torch.is_tensor(a) else np.asarray(a),
+                                               dtype=torch.float64, device=dev))
+                t.faces.append({"cells": torch.as_tensor(bi[:, 0], dtype=torch.long, device=dev),
+                                "cell_nodes": torch.as_tensor(fe.cells[bi[:, 0]], dtype=torch.long, device=dev),
+                                "vals": T(fe.face_shape_vals[bi[:, 1]]), "grads": T(grads), "nanson": T(nanson),
+                                "x": T(fe.get_physical_surface_quad_points(bi)), "isv": isv,
+                                "nsel": len(bi), "nfq": fe.num_face_quads,
+                                "map": smaps[k] if smaps is not None else None,
+                                "universal": usurf[k] if usurf is not None else None})
+            self._terms = t
+        return self._terms
 
     # ---------------------------------------------------------------- layout
     def unflatten_fn_sol_list(self, dofs):
@@ -161,31 +302,30 @@ class Problem:
         return mat
 
     def _params(self):
-        """Per-quadrature parameter (ncell, nq) f64 on the device, or None.  Inside a
-        solve (hold_params) the tensor made at its start serves every evaluation: a
-        NumPy field is copied to the device once per solve, not per residual."""
+        """The built laws' per-quadrature parameter (ncell, nq) f64 on the device, or
+        None.  Inside a solve (hold_params) the tensor made at its start serves every
+        evaluation: a NumPy field is copied to the device once per solve, not per
+        residual."""
         held = getattr(self, "_held_params", None)
         if held is not None:
             return held[0]
-        iv = self.internal_vars
-        if iv is None or len(iv) == 0:
+        ivs = self.volume_internal_vars()
+        if not ivs:
             return None
-        if len(iv) > 1:
-            raise NotImplementedError("one per-quadrature parameter field is supported")
-        p = iv[0]
-        t = torch.as_tensor(p.detach() if torch.is_tensor(p) else p, dtype=torch.float64, device=self.device)
-        if t.dim() == 1 and t.shape[0] == self.num_cells:
-            t = t[:, None].expand(self.num_cells, self.fe.num_quads)
-        if tuple(t.shape) != (self.num_cells, self.fe.num_quads):
-            raise ValueError("internal_vars[0] must have shape (num_cells, num_quads)")
-        return t.contiguous()
+        if len(ivs) != 1 or ivs[0].dim() != 2:
+            raise NotImplementedError("the built cell kernels take one scalar parameter per quadrature point")
+        return ivs[0]
 
     def hold_params(self, on):
-        """Freeze (on=True) or release the device copy of the parameter field for the
-        duration of a solve; internal_vars must not change while it is held."""
+        """Freeze (on=True) or release the device copies of the internal variables
+        for the duration of a solve; internal_vars must not change while they are held."""
+        self._held_ivs = None
         self._held_params = None
         if on:
-            self._held_params = (self._params(),)
+            plan = self._plan()
+            self._held_ivs = self.volume_internal_vars()
+            if plan["law"] == "fast":
+                self._held_params = (self._params(),)
 
     def _buf(self, name, n, dtype):
         b = self._bufs.get(name)
@@ -234,44 +374,34 @@ class Problem:
 
     # ------------------------------------------------------------ load vector
     def _external(self):
-        """u-independent mass and surface loads (problem.py:216-259), f64 device vector or None."""
-        # the load depends on the surface maps' internal variables and, through a
-        # mass map, on the volume ones: keyed on their contents (torch tensors by
-        # storage and version counter, arrays by a digest of their bytes)
-        key = (_fingerprint(self.internal_vars_surfaces),
-               _fingerprint(self.internal_vars) if hasattr(self, "get_mass_map") else None)
+        """The u-independent mass and surface loads (problem.py:216-259) as one f64
+        device vector, or None; the other terms are element terms (_element_terms)."""
+        plan = self._plan()
+        key = (_fingerprint(self.internal_vars_surfaces), tuple(plan["surfaces"]), plan["mass"])
         if self._ext_key == key:
             return self._fext
-        self._ext_refs = (self.internal_vars_surfaces, self.internal_vars)   # keeps ids in the key unique
+        self._ext_refs = self.internal_vars_surfaces   # keeps ids in the key unique
         fe = self.fe
         f = np.zeros((fe.num_total_nodes, self.vec))
         have = False
-        if hasattr(self, "get_mass_map"):
+        if plan["mass"] == "load":
             mmap = self.get_mass_map()
             x = fe.get_physical_quad_points().reshape(-1, self.dim)
             m = _eval_points(lambda xx: mmap(np.zeros(self.vec), xx), x).reshape(self.num_cells, fe.num_quads, -1)
-            u1 = np.random.default_rng(1).standard_normal((min(3, len(x)), self.vec))
-            m1 = np.stack([np.asarray(mmap(u1[i], x[i]), dtype=np.float64).reshape(-1) for i in range(len(u1))])
-            if not np.allclose(m1, m.reshape(len(x), -1)[:len(m1)]):
-                raise NotImplementedError("mass maps that depend on u have no HIP kernel yet")
-            JxW = fe.JxW()
-            r = np.einsum("cqi,qn,cq->cni", m, fe.shape_vals, JxW)
+            r = np.einsum("cqi,qn,cq->cni", m, fe.shape_vals, fe.JxW)
             np.add.at(f, fe.cells.reshape(-1), r.reshape(-1, self.vec))
             have = True
         if hasattr(self, "get_surface_maps"):
             for k, (bi, smap) in enumerate(zip(self.boundary_inds_list, self.get_surface_maps())):
-                if len(bi) == 0:
+                if len(bi) == 0 or plan["surfaces"][k] != "load":
                     continue
                 _, nanson = fe.get_face_shape_grads(bi)
                 xs = fe.get_physical_surface_quad_points(bi).reshape(-1, self.dim)
-                isv = [np.asarray(a).reshape(len(xs), -1) for a in (self.internal_vars_surfaces[k] or ())]
+                isv = [np.asarray(a.detach().cpu() if torch.is_tensor(a) else a).reshape(len(xs), -1)
+                       for a in (self.internal_vars_surfaces[k] or ())]
                 u0 = np.zeros((len(xs), self.vec))
-                t = _eval_points(lambda xx, uu, *a: smap(uu, xx, *a), xs, (u0, *isv)).reshape(len(bi), fe.num_face_quads, -1)
-                u1 = np.random.default_rng(0).standard_normal((len(xs), self.vec))
-                t1 = np.stack([np.asarray(smap(u1[i], xs[i], *[a[i] for a in isv]), dtype=np.float64)
-                               for i in range(min(3, len(xs)))])
-                if not np.allclose(t1, t.reshape(len(xs), -1)[:len(t1)]):
-                    raise NotImplementedError("surface maps that depend on u have no HIP kernel yet")
+                t = _eval_points(lambda xx, uu, *a: smap(uu, xx, *a), xs, (u0, *isv)).reshape(
+                    len(bi), fe.num_face_quads, -1)
                 vals = fe.face_shape_vals[bi[:, 1]]
                 r = np.einsum("sqi,sqn,sq->sni", t, vals, nanson)
                 np.add.at(f, fe.cells[bi[:, 0]].reshape(-1), r.reshape(-1, self.vec))
@@ -281,16 +411,51 @@ class Problem:
         return self._fext
 
     # -------------------------------------------------------------- engines
+    def _element_terms(self, u, jac, dtype=torch.float64):
+        """Element residuals (ncell*nn*vec f64, a reused buffer) and, with jac, element
+        Jacobians block-major (ncell, nn, nn, vec, vec) in dtype: the built cell
+        kernels or the pointwise evaluation, plus the mass, universal and face terms."""
+        plan = self._plan()
+        pts, cells = self._dev()
+        s = _lib.stream()
+        nd2 = self.nn * self.nn * self.vec * self.vec
+        tag = "64" if dtype == torch.float64 else "32"
+        if plan["law"] == "fast":
+            mat = self._hfmat()
+            prm = self._params()
+            er = self._buf("elem_res", self.num_cells * self.nn * self.vec, torch.float64)
+            _lib.call("hf_cell_residual_f64", self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                      self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(er), s)
+            ej = None
+            if jac:
+                ej = self._buf("elem_jac" + tag, self.num_cells * nd2, dtype)
+                _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                          self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
+            if plan["mass"] == "cell":
+                rm, jm = self._pointwise().mass_terms(u, jac, dtype)
+                er += rm
+                if jac:
+                    ej += jm
+        else:
+            er, ej = self._pointwise().cell_terms(u, jac, dtype)
+            self._bufs["elem_res"] = er
+        extra = []
+        if plan["universal"]:
+            r, K = self._pointwise().universal_terms(u, jac)
+            extra.append((None, r, K))
+        faces = [k for k, how in enumerate(plan["surfaces"]) if how == "face"]
+        if faces:
+            extra += self._pointwise().face_terms(u, jac, faces)
+        if extra:
+            from .generic import add_element_terms
+            add_element_terms(er, ej, extra, self.nn, self.vec, self.num_cells)
+        return er, ej
+
     def residual_vec(self, u):
         """Unconstrained residual R(u) (N,) f64 on the device: cells + loads (problem.py:439-445)."""
-        pts, cells = self._dev()
         u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
-        er = self._buf("elem_res", self.num_cells * self.nn * self.vec, torch.float64)
-        mat = self._hfmat()
-        prm = self._params()
+        er, _ = self._element_terms(u, False)
         s = _lib.stream()
-        _lib.call("hf_cell_residual_f64", self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(er), s)
         R = torch.empty(self.N, dtype=torch.float64, device=self.device)
         fext = self._external()
         box = self._gather_box()
@@ -318,18 +483,12 @@ class Problem:
 
     def jacobian_bsr(self, u, dtype=torch.float64, bc=True):
         """Tangent values (the library's sliced-ELL layout of node-pair blocks); Dirichlet rows -> identity rows."""
-        pts, cells = self._dev()
         u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
         p = self.pattern
-        bb = self.vec * self.vec
         tag = "64" if dtype == torch.float64 else "32"
-        mat = self._hfmat()
-        prm = self._params()
         s = _lib.stream()
         vals = torch.empty(p.nvals, dtype=dtype, device=self.device)
-        ej = self._buf("elem_jac" + tag, self.num_cells * self.nn * self.nn * bb, dtype)
-        _lib.call("hf_cell_jacobian_f" + tag, self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), s)
+        _, ej = self._element_terms(u, True, dtype)
         if self._fused_ok(dtype):
             # HEX8: rows summed per node over its cells (same order and bits as hf_assemble)
             _lib.call("hf_assemble_hex8_f" + tag, p.handle, _lib.ptr(ej), _lib.ptr(vals), s)
@@ -391,25 +550,19 @@ class Problem:
     fused_assembly = True   # HEX8 vec-3 patterns: hf_assemble_hex8 (per-node rows) instead of hf_assemble
 
     def _fused_ok(self, dtype):
-        if not (self.fused_assembly and self.nn == 8 and self.vec == 3):
+        if not (self.fused_assembly and self.nn == 8 and self.vec == 3 and self.dim == 3):
             return False
         limit = 42 if dtype == torch.float64 else 85   # LDS: 32 rows x max_row x 9 values
         return self.pattern.max_row_blocks <= limit
 
     def element_jacobians(self, u, dtype=torch.float64):
         """Element matrices (ncell, nd, nd), jac[r,c] = dR_r/du_c (problem.py:262-266)."""
-        pts, cells = self._dev()
         u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
         nn, v = self.nn, self.vec
-        ej = torch.empty(self.num_cells * nn * nn * v * v, dtype=dtype, device=self.device)
-        mat = self._hfmat()
-        tag = "64" if dtype == torch.float64 else "32"
-        prm = self._params()
-        _lib.call("hf_cell_jacobian_f" + tag, nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-                  self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(ej), _lib.stream())
+        _, ej = self._element_terms(u, True, dtype)
         # block-major (c, a, b, i, k) -> (c, a*v+i, b*v+k)
         return ej.reshape(self.num_cells, nn, nn, v, v).permute(0, 1, 3, 2, 4).reshape(
-            self.num_cells, nn * v, nn * v)
+            self.num_cells, nn * v, nn * v).clone()
 
     # ------------------------------------------- reference-compatible methods
     def compute_residual(self, sol_list):

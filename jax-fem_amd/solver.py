@@ -571,6 +571,62 @@ def param_vjp(problem, u, lamI):
     return grad
 
 
+def param_vjp_pointwise(problem, u, lamI, which):
+    """-(dR/dq)^T lam for the internal variables q = internal_vars[k], k in `which`,
+    of maps evaluated pointwise (generic.PointwiseTerms): with lam's gradient and
+    values at the quadrature points (the same contraction kernels as u's) as the
+    cotangents, torch.func.vjp of the tensor map, the mass map and the universal
+    kernel with respect to their internal-variable arguments.  Returns a list of
+    tensors shaped like volume_internal_vars()[k]."""
+    terms = problem._pointwise()
+    plan = problem._plan()
+    ivs = problem.volume_internal_vars()
+    Q = problem.num_cells * problem.fe.num_quads
+    flat = [a.reshape((Q,) + tuple(a.shape[2:])) for a in ivs]
+    need_q = plan["mass"] == "cell"
+    ug, uq = terms.point_data(u.contiguous(), need_q)
+    lg, lq = terms.point_data(lamI.contiguous(), need_q)
+    w = terms.jxw.reshape(-1)
+    total = [torch.zeros_like(a) for a in flat]
+
+    from . import numpy as jnp
+
+    def add(fn, fixed, cot):
+        nf = len(fixed)
+
+        def one(*a):
+            fx, c, rest = a[:nf], a[nf], a[nf + 1:]
+            _, pull = torch.func.vjp(lambda *q: fn(*fx, *q), *rest)
+            return pull(c)
+        with jnp.on_device(u.device):
+            g = torch.func.vmap(one)(*fixed, cot, *flat)
+        for k in which:
+            total[k] += g[k]
+    if terms.tensor_fn is not None:
+        add(terms.tensor_fn, [ug], lg * w[:, None, None])
+    if terms.mass_fn is not None:
+        add(terms.mass_fn, [uq, terms.xq.reshape(Q, -1)], lq * w[:, None])
+    if terms.universal is not None:
+        _, cells = problem._dev()
+        nc = problem.num_cells
+        cu = u.reshape(-1, problem.vec)[cells.long()].reshape(nc, -1)
+        cl = lamI.reshape(-1, problem.vec)[cells.long()].reshape(nc, -1)
+        sg = terms.sg
+        args = [cu, terms.xq, sg, terms.jxw[:, None, :], (sg * terms.jxw[:, :, None, None])[:, :, :, None, :]]
+        k5 = len(args)
+        kern = terms.universal
+
+        def one_cell(*a):
+            fx, c, rest = a[:k5], a[k5], a[k5 + 1:]
+            _, pull = torch.func.vjp(lambda *q: kern(*fx, *q), *rest)
+            return pull(c)
+        with jnp.on_device(u.device):
+            g = torch.func.vmap(one_cell)(*args, cl, *ivs)
+        for k in which:
+            total[k] += g[k].reshape(total[k].shape)
+    return [-total[k].reshape(ivs[k].shape) for k in range(len(ivs))]
+
+
 def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options={}):
     """Gradient of <v, u*(params)> w.r.t. params (solver.py:1324-1380).
 
@@ -582,17 +638,29 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options={}):
               if (torch.is_tensor(p) or np.ndim(p) >= 0) else p for p in flat]
     with torch.enable_grad():
         problem.set_params(_unflatten(list(leaves), spec))
-        iv = problem.internal_vars[0] if problem.internal_vars else None
+        ivs_req = list(problem.internal_vars or ())
         bc_vals = list(problem.fe.vals_list)
     u = torch.as_tensor(sol_list[0]).reshape(-1).to(problem.device, torch.float64)
     v = torch.as_tensor(v_list[0]).reshape(-1).to(problem.device, torch.float64)
     problem._newton_u = u
     lam, lamI = adjoint_solve(problem, u, v, adjoint_solver_options)
     outs, gouts = [], []
-    if torch.is_tensor(iv) and iv.requires_grad:
+    which = [k for k, a in enumerate(ivs_req) if torch.is_tensor(a) and a.requires_grad]
+    if which and problem._plan()["pure_fast"]:
+        iv = ivs_req[0]
         g = param_vjp(problem, u, lamI)
         outs.append(iv)
         gouts.append(g.reshape(iv.shape).to(iv.device, iv.dtype))
+    elif which:
+        if problem._plan()["law"] == "fast":
+            raise NotImplementedError("parameter gradients of a built law combined with pointwise terms")
+        gs = param_vjp_pointwise(problem, u, lamI, which)
+        for k in which:
+            gk = gs[k]
+            if ivs_req[k].dim() == 1 and gk.dim() == 2:      # a per-cell value repeated over the points
+                gk = gk.sum(dim=1)
+            outs.append(ivs_req[k])
+            gouts.append(gk.reshape(ivs_req[k].shape).to(ivs_req[k].device, ivs_req[k].dtype))
     # Dirichlet values: c_B = u_B - g  =>  d<lam, c>/dg = -lam_B, grad = +lam_B (later specs win)
     fe = problem.fe
     owner = {}

@@ -69,12 +69,76 @@ def test_bc_and_faces_match_oracle(jfa):
     assert abs(nanson.sum() - 76.536686473018) < 1e-10
 
 
-def test_problem_rejects_plain_python_map(jfa):
-    class P(jfa.Problem):
+def test_plain_maps_probed_and_planned(jfa):
+    """A plain-function tensor map (the reference's idiom: jax.grad of an energy,
+    test_hyper_elasticity.py:15-34) is recognised as the built law it equals, with
+    moduli fitted to 1e-12; a map that is no built law is planned for the pointwise
+    (device) path; mass and surface maps that depend on u become element terms,
+    u-independent ones a load vector."""
+    import jax_fem_amd.numpy as np_
+    from jax_fem_amd import autodiff as ad
+
+    def psi(F):
+        E, nu = 1e3, 0.3
+        mu, kappa = E / (2. * (1. + nu)), E / (3. * (1. - 2. * nu))
+        J = np_.linalg.det(F)
+        return (mu / 2.) * (J ** (-2. / 3.) * np_.trace(F.T @ F) - 3.) + (kappa / 2.) * (J - 1.) ** 2.
+    P_fn = ad.grad(psi)
+
+    class Hyper(jfa.Problem):
         def get_tensor_map(self):
-            return lambda g: g
-    with pytest.raises(NotImplementedError):
-        P(jfa.box_mesh(1, 1, 1), vec=1, dim=3)
+            return lambda u_grad: P_fn(u_grad + np_.eye(3))
+
+    p = Hyper(jfa.box_mesh(2, 2, 2), vec=3, dim=3)
+    plan = p._plan()
+    assert plan["law"] == "fast" and plan["pure_fast"]
+    assert type(p.material).__name__ == "NeoHookean"
+    assert abs(p.material.E / 1e3 - 1) < 1e-12 and abs(p.material.nu / 0.3 - 1) < 1e-12
+
+    class Cubic(jfa.Problem):
+        def get_tensor_map(self):
+            return lambda u_grad: u_grad + u_grad ** 3
+
+        def get_mass_map(self):
+            return lambda u, x: 3.0 * u
+
+        def get_surface_maps(self):
+            return [lambda u, x: np.array([0.0, 0.0, 1.0])]
+
+    q = Cubic(jfa.box_mesh(2, 2, 2), vec=3, dim=3, location_fns=[lambda x: np.isclose(x[0], 1.0)])
+    plan = q._plan()
+    assert plan["law"] == "generic" and plan["mass"] == "cell" and plan["surfaces"] == ["load"]
+    assert not plan["pure_fast"]
+
+
+def test_complex_step_and_torch_transforms_agree(jfa):
+    """grad / jacfwd of a NumPy-written map by the complex step equal torch.func's
+    on the same code (jax_fem_amd.numpy dispatches on the argument type), and the
+    Neo-Hookean first Piola stress of the built law."""
+    import torch
+    import jax_fem_amd.numpy as np_
+    from jax_fem_amd import autodiff as ad
+
+    def psi(F):
+        J = np_.linalg.det(F)
+        return 3.0 * (J ** (-2. / 3.) * np_.sum(F * F) - 3.) + 20.0 * (J - 1.) ** 2 + np_.sqrt(np_.trace(F))
+    g = np.random.default_rng(4).standard_normal((3, 3)) * 0.1 + np.eye(3)
+    pn = ad.grad(psi)(g)
+    pt = ad.grad(psi)(torch.tensor(g)).numpy()
+    np.testing.assert_allclose(pn, pt, rtol=1e-12, atol=1e-12)
+    f = lambda gg: ad.grad(psi)(gg)
+    Cn = ad.jacfwd(f)(g)
+    Ct = ad.jacfwd(f)(torch.tensor(g)).numpy()
+    np.testing.assert_allclose(Cn, Ct, rtol=1e-10, atol=1e-10)
+    U = torch.tensor(np.random.default_rng(5).standard_normal((4, 3, 3)) * 0.05)
+    nh = jfa.NeoHookean(E=10.0, nu=0.3)
+
+    def energy(F):
+        mu, kappa = nh.moduli()
+        J = np_.linalg.det(F)
+        return mu / 2 * (J ** (-2. / 3.) * np_.trace(F.T @ F) - 3.) + kappa / 2 * (J - 1.) ** 2
+    P = ad.vmap(lambda ug: ad.grad(energy)(ug + np_.eye(3)))(U)
+    np.testing.assert_allclose(P.numpy(), nh(U.numpy()), rtol=1e-12, atol=1e-12)
 
 
 def test_coo_pattern_matches_oracle(jfa):
@@ -118,8 +182,12 @@ def test_multigrid_lattice_detection(jfa):
     assert mgm.lattice(P(jfa.box_mesh_hex27(2, 1, 2), vec=3, dim=3, ele_type="HEX27",
                          quadrature_order=4)) == (5, 3, 5)
     m = jfa.box_mesh(2, 2, 2)
-    with pytest.raises(NotImplementedError):   # the same cells without the lattice metadata
-        mgm.lattice(P(jfa.Mesh(m.points, m.cells, "HEX8"), vec=3, dim=3))
+    # re-wrapped the reference's way: the lattice is detected from the coordinates
+    m2 = jfa.Mesh(m.points, m.cells_dict[jfa.get_meshio_cell_type("HEX8")])
+    assert mgm.lattice(P(m2, vec=3, dim=3)) == (3, 3, 3)
+    perm = np.random.default_rng(0).permutation(len(m.points))
+    with pytest.raises(NotImplementedError):   # the same body, nodes relabelled: no lattice
+        mgm.lattice(P(jfa.Mesh(m.points[perm], np.argsort(perm)[m.cells], "HEX8"), vec=3, dim=3))
     class Q(jfa.Problem):
         def get_tensor_map(self):
             return jfa.Poisson()
@@ -156,3 +224,51 @@ def test_box_corner_table(jfa):
     prob._corner_key = None
     t2 = prob._box_corners(grid)
     assert t2 is not None and list(t2) == [1 if t == 0 else 0 if t == 1 else t for t in table]
+
+
+def test_vtu_round_trip_and_binary_reader(jfa, tmp_path):
+    """save_sol writes the mesh and 'sol' (float32, as the reference); read_vtu gives
+    the points, cells_dict and point data back; a base64 zlib-compressed array laid
+    out as the VTK XML format specifies (UInt64 header: block count, block size,
+    last block size, compressed sizes) decodes to the same numbers."""
+    import base64
+    import struct
+    import zlib
+    from importlib import import_module
+    fe_mod = import_module("jax_fem_amd.fe")
+    m = jfa.box_mesh(3, 2, 2, 1.0, 0.5, 0.25)
+    fe = fe_mod.FiniteElement(m, 3, 3, "HEX8")
+    sol = np.random.default_rng(0).standard_normal((len(m.points), 3))
+    path = str(tmp_path / "out" / "u.vtu")
+    jfa.save_sol(fe, sol, path, cell_infos=[("rho", np.arange(fe.num_cells, dtype=float))],
+                 point_infos=[("T", sol[:, 0])])
+    r = jfa.read_vtu(path)
+    np.testing.assert_array_equal(r.points, m.points)
+    np.testing.assert_array_equal(r.cells_dict["hexahedron"], m.cells)
+    np.testing.assert_array_equal(r.point_data["sol"], sol.astype(np.float32))
+    np.testing.assert_array_equal(r.cell_data["rho"], np.arange(fe.num_cells, dtype=np.float32))
+    mesh = jfa.Mesh(r.points, r.cells_dict[jfa.get_meshio_cell_type("HEX8")])
+    assert mesh.node_grid == (4, 3, 3)
+    # a compressed binary file
+    vals = np.linspace(0.0, 1.0, 8 * 3)
+    raw = vals.astype("<f8").tobytes()
+    comp = zlib.compress(raw)
+    blob = base64.b64encode(struct.pack("<QQQQ", 1, len(raw), len(raw), len(comp)) + comp).decode()
+    conn = base64.b64encode(struct.pack("<QQQQ", 1, 64, 64, len(zlib.compress(np.arange(8, dtype="<i8").tobytes())))
+                            + zlib.compress(np.arange(8, dtype="<i8").tobytes())).decode()
+    off = np.array([8], dtype="<i8").tobytes()
+    offb = base64.b64encode(struct.pack("<QQQQ", 1, 8, 8, len(zlib.compress(off))) + zlib.compress(off)).decode()
+    typ = np.array([12], dtype="<u1").tobytes()
+    typb = base64.b64encode(struct.pack("<QQQQ", 1, 1, 1, len(zlib.compress(typ))) + zlib.compress(typ)).decode()
+    xml = ('<?xml version="1.0"?><VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian" '
+           'header_type="UInt64" compressor="vtkZLibDataCompressor"><UnstructuredGrid>'
+           '<Piece NumberOfPoints="8" NumberOfCells="1"><Points><DataArray type="Float64" NumberOfComponents="3" '
+           'format="binary">%s</DataArray></Points><Cells><DataArray type="Int64" Name="connectivity" '
+           'format="binary">%s</DataArray><DataArray type="Int64" Name="offsets" format="binary">%s</DataArray>'
+           '<DataArray type="UInt8" Name="types" format="binary">%s</DataArray></Cells></Piece></UnstructuredGrid>'
+           '</VTKFile>' % (blob, conn, offb, typb))
+    p2 = tmp_path / "b.vtu"
+    p2.write_text(xml)
+    r2 = jfa.read_vtu(str(p2))
+    np.testing.assert_array_equal(r2.points.reshape(-1), vals)
+    np.testing.assert_array_equal(r2.cells_dict["hexahedron"], np.arange(8)[None])

@@ -91,18 +91,28 @@ def test_hex27_cell_kernels_vs_oracle(jfa, kind, param):
         np.testing.assert_allclose(g, go, rtol=1e-10, atol=1e-12 * np.abs(go).max())
 
 
-def test_hex27_quadrature_order_checked(jfa):
-    say("hex27 quadrature order")
-    pts, cells = ofe.box_mesh_hex27(1, 1, 1)
-    mat, _ = _mats(jfa, "neo_hookean", None)
+def test_hex27_default_quadrature_runs_pointwise(jfa):
+    """HEX27 with the reference's default quadrature order 10 (6^3 points), for which
+    no built kernel exists: the law runs on the pointwise path; residual and element
+    Jacobian against the oracle at the same rule."""
+    say("hex27 default quadrature, pointwise path")
+    pts, cells = ofe.box_mesh_hex27(2, 1, 1)
+    mat, omat = _mats(jfa, "neo_hookean", None)
 
     class P(jfa.Problem):
         def get_tensor_map(self):
             return mat
 
     prob = P(jfa.Mesh(pts, cells, "HEX27"), vec=3, dim=3, ele_type="HEX27")   # default order 10: 6^3 points
-    with pytest.raises(jfa.HipFemError, match="3x3x3"):
-        prob.residual_vec(torch.zeros(prob.N, dtype=torch.float64, device=DEV))
+    assert prob.fe.num_quads == 216 and prob._plan()["law"] == "generic"
+    op = osv.OracleProblem(pts, cells, 3, omat, ele_type="HEX27", quadrature_order=10)
+    u = 0.03 * np.random.default_rng(4).standard_normal(op.N)
+    R = prob.residual_vec(torch.tensor(u, device=DEV)).cpu().numpy()
+    Ro = op.compute_residual(u)
+    np.testing.assert_allclose(R, Ro, rtol=1e-11, atol=1e-11 * np.abs(Ro).max())
+    K = prob.element_jacobians(torch.tensor(u, device=DEV)).cpu().numpy()
+    Ko = op.element_jacobians(u)
+    np.testing.assert_allclose(K, Ko, rtol=1e-11, atol=1e-11 * np.abs(Ko).max())
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -15,8 +15,8 @@
   search: same step lengths, residual history and root;
 * a zero initial residual (no load, zero Dirichlet values): no Newton step, as
   the reference's NaN comparison gives (solver.py:1283-1295);
-* a surface traction changed in place between two solves is seen by the second;
-* a mass map that depends on u is refused (no silent evaluation at u = 0).
+* a surface traction changed in place between two solves is seen by the second.
+(u-dependent mass and surface maps are element terms: test_gpu_pointwise.py.)
 """
 import os
 import sys
@@ -197,19 +197,3 @@ def test_surface_traction_changed_in_place(jfa):
         u2 = jfa.solver(prob)[0]
         rel = (torch.linalg.norm(u2 - 2.0 * u1) / torch.linalg.norm(u1)).item()
         assert rel <= 1e-8, (kind, rel)
-
-
-def test_u_dependent_mass_map_is_refused(jfa):
-    say("u-dependent mass map refused")
-
-    class P(jfa.Problem):
-        def get_tensor_map(self):
-            return jfa.Poisson()
-
-        def get_mass_map(self):
-            return lambda u, x: 3.0 * u          # a reaction term k u
-
-    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
-    prob = P(jfa.box_mesh(2, 2, 2), vec=1, dim=3, dirichlet_bc_info=[[left], [0], [lambda p: 1.0]])
-    with pytest.raises(NotImplementedError, match="depend on u"):
-        jfa.solver(prob)
