@@ -55,6 +55,7 @@ const char* hf_version(void);
  *      radix sort of node-pair keys; the scalar CSR view is bit-identical to
  *      the sorted-unique (I, J) pattern. ---------------------------------- */
 typedef struct hf_pattern hf_pattern;
+/* vec: components per node, 1..3 */
 int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nodes_per_cell, int64_t nnode,
                      int32_t vec, hf_stream_t stream, hf_pattern** out);
 /* host outputs: number of blocks (node pairs), scalar nnz, contributions */
@@ -134,7 +135,7 @@ int hf_gather_residual_f64(const hf_pattern* p, const double* elem_res, const do
 int hf_gather_residual_box_f64(const hf_pattern* p, int32_t nx, int32_t ny, int32_t nz, const int32_t* corner,
                                const double* elem, const double* fext, double* R, hf_stream_t stream);
 /* Matrix values live in a sliced-ELL layout over the pattern's node-pair blocks
- * (slices of 21 node rows for vec 3, 64 for vec 1; each slice padded to its
+ * (slices of 64/vec node rows: 21 for vec 3, 32 for vec 2, 64 for vec 1; each slice padded to its
  * longest row): hf_matrix_values() gives the element count of a value buffer.
  * assemble: every block = sum of its element blocks in cell order. */
 int64_t hf_matrix_values(const hf_pattern* p);

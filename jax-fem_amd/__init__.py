@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import HipFemError
 from .fe import FiniteElement
 from .materials import LinearElastic, NeoHookean, Poisson, Material
-from .mesh import Mesh, box_mesh, box_mesh_hex27, get_meshio_cell_type
+from .mesh import Mesh, box_mesh, box_mesh_hex27, rectangle_mesh, get_meshio_cell_type
 from .problem import Problem
 from .utils import save_sol, read_vtu, write_vtu, modify_vtu_file
 from . import numpy
@@ -34,7 +34,7 @@ if not logger.handlers:
     logger.setLevel(logging.INFO)
 
 __all__ = ["HipFemError", "Problem", "solver", "ad_wrapper", "implicit_vjp", "get_A", "linear_solver", "apply_bc_vec",
-           "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "get_meshio_cell_type",
+           "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "rectangle_mesh", "get_meshio_cell_type",
            "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov",
            "LearnedPC", "Multigrid", "numpy", "grad", "vmap", "jacfwd", "jacrev",
            "save_sol", "read_vtu", "write_vtu", "modify_vtu_file"]

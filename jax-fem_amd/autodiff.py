@@ -10,8 +10,13 @@ transforms here:
 * on torch tensors they are torch.func's (exact forward- and reverse-mode
   derivatives, batched evaluation), run with ``jax_fem_amd.numpy`` creating its
   constants on the tensors' device;
+* a function that torch.func cannot transform but torch autograd can
+  differentiate -- an objective through ``ad_wrapper``'s implicit solve, as
+  ``jax.grad(J)`` in the reference's inverse-problem demos -- is differentiated
+  by reverse-mode autograd on the eager graph (``grad``) or row by row
+  (``jacfwd``);
 * on NumPy arrays (post-processing, the Problem's probe) ``grad`` and ``jacfwd``
-  run torch.func on CPU tensors and return NumPy arrays when the map is written
+  run the above on CPU tensors and return NumPy arrays when the map is written
   with ``jax_fem_amd.numpy``; a map written with NumPy itself is differentiated
   by the complex step, f'(x) = Im f(x + i h) / h with h = 1e-30 (Squire and
   Trapp 1998; exact to rounding for analytic maps such as the hyperelastic
@@ -77,14 +82,50 @@ def _complex_step(f, args, argnums, vector_out):
     return y.reshape(y.shape[:-1] + x.shape)
 
 
+def _eager(f, args, argnums, vector_out):
+    """Derivative of f with respect to args[argnums] by torch autograd on the eager
+    graph (reverse mode; a Jacobian row by row).  Raises TypeError when f does not
+    produce a tensor connected to its argument."""
+    a = list(args)
+    x = a[argnums].detach().clone().requires_grad_(True)
+    a[argnums] = x
+    with torch.enable_grad():
+        y = f(*a)
+    if not torch.is_tensor(y) or y.grad_fn is None:
+        raise TypeError("the function's output is not a differentiable tensor of its argument")
+    if not vector_out:
+        if y.numel() != 1:
+            raise TypeError("grad needs a scalar-valued function")
+        return torch.autograd.grad(y, x)[0]
+    flat = y.reshape(-1)
+    rows = [torch.autograd.grad(flat[k], x, retain_graph=k + 1 < flat.numel(), allow_unused=True)[0]
+            for k in range(flat.numel())]
+    rows = [torch.zeros_like(x) if r is None else r for r in rows]
+    return torch.stack(rows).reshape(tuple(y.shape) + tuple(x.shape))
+
+
+def _torch_call(transform, f, args, argnums, vector_out):
+    """torch.func's transform; where it refuses the function (an autograd.Function
+    without functorch support, as the implicit solve), eager autograd."""
+    try:
+        return transform(f, argnums=argnums)(*args)
+    except Exception as err:
+        if not isinstance(argnums, int):
+            raise
+        try:
+            return _eager(f, args, argnums, vector_out)
+        except Exception:
+            raise err
+
+
 def _numpy_call(transform, f, args, argnums, vector_out):
-    """A transform on NumPy arguments: torch.func on CPU tensors when the map is
-    written with jax_fem_amd.numpy (exact, and nested transforms compose), else the
-    complex step on the NumPy map."""
+    """A transform on NumPy arguments: torch (torch.func, else eager autograd) on CPU
+    tensors when the function runs on them (maps written with jax_fem_amd.numpy,
+    objectives through ad_wrapper), else the complex step on the NumPy map."""
     try:
         with jnp.on_device(torch.device("cpu")):
-            out = transform(f, argnums=argnums)(*_to_device(args, torch.device("cpu")))
-        return out.detach().numpy() if torch.is_tensor(out) else np.asarray(out)
+            out = _torch_call(transform, f, _to_device(args, torch.device("cpu")), argnums, vector_out)
+        return out.detach().cpu().numpy() if torch.is_tensor(out) else np.asarray(out)
     except Exception:
         return _complex_step(f, args, argnums, vector_out)
 
@@ -95,7 +136,7 @@ def grad(f, argnums=0):
         if t is None:
             return _numpy_call(torch.func.grad, f, args, argnums, False)
         with jnp.on_device(t.device):
-            return torch.func.grad(f, argnums=argnums)(*_to_device(args, t.device))
+            return _torch_call(torch.func.grad, f, _to_device(args, t.device), argnums, False)
     return run
 
 
@@ -105,7 +146,7 @@ def jacfwd(f, argnums=0):
         if t is None:
             return _numpy_call(torch.func.jacfwd, f, args, argnums, True)
         with jnp.on_device(t.device):
-            return torch.func.jacfwd(f, argnums=argnums)(*_to_device(args, t.device))
+            return _torch_call(torch.func.jacfwd, f, _to_device(args, t.device), argnums, True)
     return run
 
 

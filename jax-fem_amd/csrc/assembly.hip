@@ -29,7 +29,7 @@ struct hf_pattern {
   int32_t* brow;     // nblocks block row
   int32_t max_row_blocks;
   // sliced-ELL value layout (see below)
-  int R;             // node rows per slice (21 for vec 3, 64 for vec 1)
+  int R;             // node rows per slice: 64 / vec (64, 32, 21 for vec 1, 2, 3)
   int64_t nslices, nslots;
   int32_t* slice_ptr;  // nslices+1 first slot of each slice
   int32_t* sell_col;   // nslots*R block column per (slot, row); padding -> own row
@@ -121,7 +121,7 @@ __global__ void csr_view(const int32_t* __restrict__ rowptr, const int32_t* __re
 
 // Matrix values: sliced ELL ("SELL-C-sigma").  Node rows are grouped in slices of
 // R rows (R = 21 for vec = 3, so a slice is 63 scalar rows = one wavefront lane
-// each; R = 64 for vec = 1).  Within windows of 8 slices the rows are ordered by
+// each; R = 32 for vec = 2 and 64 for vec = 1).  Within windows of 8 slices the rows are ordered by
 // decreasing length (stable), so rows of similar length share a slice and the
 // padding stays small when row lengths vary (the vertex / edge / face / interior
 // nodes of HEX27 meshes have 125 / 75 / 45 / 27 blocks); sell_row[s*R + r] is the
@@ -453,6 +453,7 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
   hipStream_t stream = (hipStream_t)stream_;
   HF_REQUIRE(out != nullptr && cells != nullptr, "hf_pattern_build: null argument");
   HF_REQUIRE(ncell > 0 && nnode > 0 && nn > 0 && vec > 0, "hf_pattern_build: empty mesh");
+  HF_REQUIRE(vec <= 3, "hf_pattern_build: vec=%d (1, 2 or 3 components per node)", vec);
   const int64_t n = ncell * (int64_t)nn * nn;
   HF_REQUIRE(n < (int64_t)INT32_MAX, "hf_pattern_build: %lld contributions exceed int32", (long long)n);
   HF_REQUIRE(nnode * vec < (int64_t)INT32_MAX, "hf_pattern_build: too many DOFs");
@@ -478,7 +479,7 @@ extern "C" int hf_pattern_build(const int32_t* cells, int64_t ncell, int32_t nn,
   p->rowptr = p->colidx = p->seg = p->contrib = p->diag = p->trans = p->brow = nullptr;
   p->slice_ptr = p->sell_col = p->sell_tpos = p->blkpos = p->dpos = p->sell_row = p->sell_pos = nullptr;
   p->cslot = nullptr;
-  p->R = (vec == 3) ? 21 : 64;
+  p->R = kWave / vec;   // node rows per slice: one wavefront holds R*vec <= 64 scalar rows
   p->nslices = (nnode + p->R - 1) / p->R;
   PB_CHECK(hipMalloc(&keys, n * sizeof(uint64_t)));
   PB_CHECK(hipMalloc(&keys_s, n * sizeof(uint64_t)));
@@ -668,14 +669,18 @@ template <typename T>
 static int assemble_impl(const hf_pattern* p, const T* elem, T* vals, hipStream_t s) {
   HF_REQUIRE(p && elem && vals, "hf_assemble: null argument");
   const int64_t n = p->nslices * p->R;
+#define ASM_LAUNCH(V)                                                                                        \
+  assemble_kernel<T, V><<<grid_for(n * kAsmLanes), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr,   \
+                                                                 p->sell_row, p->R, p->nslices, elem, vals)
   if (p->vec == 3)
-    assemble_kernel<T, 3><<<grid_for(n * kAsmLanes), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr,
-                                                                   p->sell_row, p->R, p->nslices, elem, vals);
+    ASM_LAUNCH(3);
+  else if (p->vec == 2)
+    ASM_LAUNCH(2);
   else if (p->vec == 1)
-    assemble_kernel<T, 1><<<grid_for(n * kAsmLanes), 256, 0, s>>>(p->rowptr, p->seg, p->contrib, p->slice_ptr,
-                                                                   p->sell_row, p->R, p->nslices, elem, vals);
+    ASM_LAUNCH(1);
   else
     HF_REQUIRE(false, "hf_assemble: vec=%d not supported", p->vec);
+#undef ASM_LAUNCH
   HF_LAUNCH_CHECK();
   return 0;
 }
@@ -689,14 +694,18 @@ extern "C" int hf_assemble_f32(const hf_pattern* p, const float* e, float* v, hf
 extern "C" int hf_gather_residual_f64(const hf_pattern* p, const double* elem, const double* fext, double* R,
                                       hf_stream_t s) {
   HF_REQUIRE(p && elem && R, "hf_gather_residual_f64: null argument");
+#define GATHER_LAUNCH(V)                                                        \
+  gather_kernel<V><<<grid_for(p->nnode * kGatherLanes), 256, 0, (hipStream_t)s>>>( \
+      p->diag, p->seg, p->contrib, p->nnode, p->nn, elem, fext, R)
   if (p->vec == 3)
-    gather_kernel<3><<<grid_for(p->nnode * kGatherLanes), 256, 0, (hipStream_t)s>>>(
-        p->diag, p->seg, p->contrib, p->nnode, p->nn, elem, fext, R);
+    GATHER_LAUNCH(3);
+  else if (p->vec == 2)
+    GATHER_LAUNCH(2);
   else if (p->vec == 1)
-    gather_kernel<1><<<grid_for(p->nnode * kGatherLanes), 256, 0, (hipStream_t)s>>>(
-        p->diag, p->seg, p->contrib, p->nnode, p->nn, elem, fext, R);
+    GATHER_LAUNCH(1);
   else
     HF_REQUIRE(false, "vec=%d not supported", p->vec);
+#undef GATHER_LAUNCH
   HF_LAUNCH_CHECK();
   return 0;
 }

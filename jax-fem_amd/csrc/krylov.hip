@@ -450,6 +450,7 @@ static int spmv_dispatch(const hf_pattern* p, const T* vals, const T* x, T* y, h
   HF_REQUIRE(p && vals && x && y, "hf_spmv: null argument");
   switch (pattern_vec(p)) {
     case 1: return spmv_launch<T, 1, TRANS>(p, vals, x, y, s);
+    case 2: return spmv_launch<T, 2, TRANS>(p, vals, x, y, s);
     case 3: return spmv_launch<T, 3, TRANS>(p, vals, x, y, s);
     default: HF_REQUIRE(false, "hf_spmv: vec=%d not supported", pattern_vec(p));
   }
@@ -612,14 +613,16 @@ static int krylov_dispatch(int method, const hf_pattern* p, const T* vals, const
   HF_REQUIRE(maxiter >= 0, "hf_krylov_solve: maxiter < 0");
   if (poll < 1) poll = 1;
   const int vec = pattern_vec(p);
-  HF_REQUIRE(vec == 1 || vec == 3, "hf_krylov_solve: vec=%d not supported", vec);
+  HF_REQUIRE(vec >= 1 && vec <= 3, "hf_krylov_solve: vec=%d not supported", vec);
   HF_REQUIRE(method == HF_KRYLOV_PCG || method == HF_KRYLOV_BICGSTAB, "hf_krylov_solve: unknown method %d", method);
 #define KRY(M, V, TR) M<T, V, TR>(p, vals, dinv, b, x, maxiter, rtol, atol, poll, rel_r0, ws, info, s, ps)
   if (method == HF_KRYLOV_PCG) {
     if (vec == 3) return transpose ? KRY(pcg_run, 3, true) : KRY(pcg_run, 3, false);
+    if (vec == 2) return transpose ? KRY(pcg_run, 2, true) : KRY(pcg_run, 2, false);
     return transpose ? KRY(pcg_run, 1, true) : KRY(pcg_run, 1, false);
   }
   if (vec == 3) return transpose ? KRY(bicg_run, 3, true) : KRY(bicg_run, 3, false);
+  if (vec == 2) return transpose ? KRY(bicg_run, 2, true) : KRY(bicg_run, 2, false);
   return transpose ? KRY(bicg_run, 1, true) : KRY(bicg_run, 1, false);
 #undef KRY
 }

@@ -114,7 +114,7 @@ struct MatArgs {
 
 template <int VEC>
 struct SellShape {
-  static constexpr int R = (VEC == 3) ? 21 : 64;
+  static constexpr int R = kWave / VEC;   // node rows per slice: 64, 32, 21
   static constexpr int L = R * VEC;
   static constexpr int BB = VEC * VEC;
 };

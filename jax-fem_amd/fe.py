@@ -1,26 +1,40 @@
 """Set-up side of a finite-element variable (host, once per mesh / load step).
 
-Mirrors ``jax_fem.fe.FiniteElement`` (fe.py:20-322) for HEX8 and HEX27:
-reference tables on the [0,1]^3 cell (basis.py:141-250), Dirichlet index/value
-extraction (fe.py:220-269), boundary-face selection (fe.py:271-322), and the
-face / volume load vectors of u-independent surface and mass maps
+Mirrors ``jax_fem.fe.FiniteElement`` (fe.py:20-322) for the tensor-product
+Lagrange elements HEX8, HEX27 and QUAD4: reference tables on the unit cell
+[0,1]^dim (basis.py:141-250), Dirichlet index/value extraction
+(fe.py:220-269), boundary-face selection (fe.py:271-322), and the face /
+volume load vectors of u-independent surface and mass maps
 (problem.py:216-259).  None of this is on the solve path: the per-iteration
 work (cell kernels, assembly, Krylov) runs in libhipfem.
 """
 import numpy as np
 
-from .mesh import HEX8_OFFSETS, HEX27_OFFSETS
+from .mesh import HEX8_OFFSETS, HEX27_OFFSETS, QUAD4_OFFSETS
 
-_RE_ORDER_NOTE = "mesh (VTK) node order; basis.py:53 / :60-61 map it onto basix"
+# Nodes are in the mesh's (VTK / meshio) order; the reference maps that order
+# onto basix's with re_order (basis.py:53, :60-61, :86-91), which is the same as
+# tabulating in mesh order directly.
 ELEMENTS = {
-    # nodes, degree, default quadrature order (basis.py:56, :64)
+    # reference coordinates of the nodes, degree, default quadrature order (basis.py:56, :64, :89)
     "HEX8": (HEX8_OFFSETS / 1.0, 1, 2),
     "HEX27": (HEX27_OFFSETS / 2.0, 2, 10),
+    "QUAD4": (QUAD4_OFFSETS / 1.0, 1, 2),
 }
-# basix hexahedron facets as mesh-order corner nodes and their outward normals
-# (facet order: z=0, y=0, x=0, x=1, y=1, z=1)
-FACES = np.array([[0, 1, 3, 2], [0, 1, 4, 5], [0, 3, 4, 7], [1, 2, 5, 6], [3, 2, 7, 6], [4, 5, 7, 6]])
-FACE_NORMALS = np.array([[0, 0, -1], [0, -1, 0], [-1, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
+# basix facets as mesh-order corner nodes (in basix's vertex order) and their
+# outward normals.  Hexahedron: z=0, y=0, x=0, x=1, y=1, z=1; quadrilateral:
+# y=0, x=0, x=1, y=1.
+FACETS = {
+    3: (np.array([[0, 1, 3, 2], [0, 1, 4, 5], [0, 3, 4, 7], [1, 2, 5, 6], [3, 2, 7, 6], [4, 5, 7, 6]]),
+        np.array([[0, 0, -1], [0, -1, 0], [-1, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)),
+    2: (np.array([[0, 1], [0, 3], [1, 2], [3, 2]]),
+        np.array([[0, -1], [-1, 0], [1, 0], [0, 1]], dtype=np.float64)),
+}
+FACES, FACE_NORMALS = FACETS[3]
+
+
+def element_dim(ele_type):
+    return ELEMENTS[ele_type][0].shape[1]
 
 
 def gauss_points(order):
@@ -41,33 +55,49 @@ def lagrange(deg, t):
 
 
 def tabulate(ele_type, X):
-    """Shape values (np, nn) and reference gradients (np, nn, 3) at points X (np, 3)."""
+    """Shape values (np, nn) and reference gradients (np, nn, dim) at points X (np, dim):
+    products of 1-D Lagrange factors, one per coordinate."""
     nodes, deg, _ = ELEMENTS[ele_type]
+    d = nodes.shape[1]
     ijk = np.rint(nodes * deg).astype(int)
-    V, D = zip(*[lagrange(deg, X[:, c]) for c in range(3)])
-    vx, vy, vz = V[0][:, ijk[:, 0]], V[1][:, ijk[:, 1]], V[2][:, ijk[:, 2]]
-    dx, dy, dz = D[0][:, ijk[:, 0]], D[1][:, ijk[:, 1]], D[2][:, ijk[:, 2]]
-    return vx * vy * vz, np.stack([dx * vy * vz, vx * dy * vz, vx * vy * dz], axis=-1)
+    V, D = zip(*[lagrange(deg, X[:, c]) for c in range(d)])
+    v = [V[c][:, ijk[:, c]] for c in range(d)]
+    g = [D[c][:, ijk[:, c]] for c in range(d)]
+    vals = np.prod(np.stack(v), axis=0)
+    grads = np.stack([np.prod(np.stack([g[c] if c == e else v[c] for c in range(d)]), axis=0)
+                      for e in range(d)], axis=-1)
+    return vals, grads
 
 
-def volume_rule(order):
+def volume_rule(order, dim=3):
+    """Tensor Gauss rule on [0,1]^dim, first coordinate slowest (basix's order)."""
     t, w = gauss_points(order)
-    P = np.stack(np.meshgrid(t, t, t, indexing="ij"), -1).reshape(-1, 3)
-    W = np.einsum("i,j,k->ijk", w, w, w).reshape(-1)
+    P = np.stack(np.meshgrid(*([t] * dim), indexing="ij"), -1).reshape(-1, dim)
+    W = np.prod(np.stack(np.meshgrid(*([w] * dim), indexing="ij"), -1).reshape(-1, dim), axis=-1)
     return P, W
 
 
-def face_rule(order):
-    """Face quadrature points (6, nfq, 3) and weights (6, nfq) on the unit-cube facets."""
+def face_rule(order, dim=3):
+    """Facet quadrature points (nfacet, nfq, dim) and weights (nfacet, nfq): the
+    (dim-1)-dimensional Gauss rule mapped onto each facet from its first vertex
+    along the edges to its second (and third) vertex."""
     t, w = gauss_points(order)
-    s2 = np.stack(np.meshgrid(t, t, indexing="ij"), -1).reshape(-1, 2)
-    w2 = np.outer(w, w).reshape(-1)
-    corners = HEX8_OFFSETS.astype(np.float64)
+    if dim == 3:
+        s = np.stack(np.meshgrid(t, t, indexing="ij"), -1).reshape(-1, 2)
+        ws = np.outer(w, w).reshape(-1)
+        corners = HEX8_OFFSETS.astype(np.float64)
+    else:
+        s, ws = t[:, None], w
+        corners = QUAD4_OFFSETS.astype(np.float64)
+    facets = FACETS[dim][0]
     pts = []
-    for f in FACES:
-        v0, v1, v2 = corners[f[0]], corners[f[1]], corners[f[2]]
-        pts.append(v0[None] + s2[:, :1] * (v1 - v0)[None] + s2[:, 1:] * (v2 - v0)[None])
-    return np.stack(pts), np.tile(w2, (6, 1))
+    for f in facets:
+        v0 = corners[f[0]]
+        p = np.repeat(v0[None], len(s), axis=0)
+        for a in range(dim - 1):
+            p = p + s[:, a:a + 1] * (corners[f[a + 1]] - v0)[None]
+        pts.append(p)
+    return np.stack(pts), np.tile(ws, (len(facets), 1))
 
 
 def _eval_points(fn, pts, extra=()):
@@ -102,6 +132,43 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
 
+def _tensor_device(obj):
+    if _is_torch(obj) and hasattr(obj, "device"):
+        return obj.device
+    if isinstance(obj, (list, tuple)):
+        for o in obj:
+            d = _tensor_device(o)
+            if d is not None:
+                return d
+    return None
+
+
+class HostField(np.ndarray):
+    """A NumPy array of set-up data (JxW, physical shape gradients, quadrature
+    points) that meets torch tensors on their device: in an operation with a
+    tensor it takes part as a tensor on that tensor's device, so reference-style
+    post-processing such as ``np.sum((u_q - v_q)**2 * fe.JxW[:, :, None])`` runs
+    on the GPU and stays differentiable.  On its own it is an ordinary array."""
+
+    @classmethod
+    def __torch_function__(cls, func, types, args=(), kwargs=None):
+        import torch
+        kwargs = kwargs or {}
+        dev = _tensor_device(list(args) + list(kwargs.values())) or torch.device("cpu")
+
+        def conv(a):
+            if isinstance(a, HostField):
+                return torch.as_tensor(np.asarray(a), device=dev)
+            if isinstance(a, (list, tuple)):
+                return type(a)(conv(b) for b in a)
+            return a
+        return func(*conv(args), **{k: conv(v) for k, v in kwargs.items()})
+
+
+def _field(a):
+    return np.asarray(a).view(HostField)
+
+
 def _location_flags(fn, points):
     nargs = fn.__code__.co_argcount
     if nargs == 1:
@@ -119,7 +186,9 @@ class FiniteElement:
     def __init__(self, mesh, vec, dim, ele_type="HEX8", quadrature_rule=None, quadrature_order=None,
                  dirichlet_bc_info=None):
         if ele_type not in ELEMENTS:
-            raise NotImplementedError("element %s (HEX8 and HEX27 are supported)" % ele_type)
+            raise NotImplementedError("element %s (HEX8, HEX27 and QUAD4 are supported)" % ele_type)
+        if element_dim(ele_type) != dim:
+            raise ValueError("element %s is %d-dimensional, dim=%d" % (ele_type, element_dim(ele_type), dim))
         if quadrature_rule not in (None, "default", "GLL") and quadrature_rule is not None:
             raise NotImplementedError("quadrature_rule %r" % (quadrature_rule,))
         self.mesh, self.vec, self.dim, self.ele_type = mesh, vec, dim, ele_type
@@ -130,15 +199,14 @@ class FiniteElement:
         self.num_total_dofs = self.num_total_nodes * vec
         order = ELEMENTS[ele_type][2] if quadrature_order is None else quadrature_order
         self.quadrature_order = order
-        qp, self.quad_weights = volume_rule(order)
+        qp, self.quad_weights = volume_rule(order, dim)
         self.shape_vals, self.shape_grads_ref = tabulate(ele_type, qp)
-        fp, self.face_quad_weights = face_rule(order)
+        fp, self.face_quad_weights = face_rule(order, dim)
         nf, nfq, _ = fp.shape
-        fv, fg = tabulate(ele_type, fp.reshape(-1, 3))
+        fv, fg = tabulate(ele_type, fp.reshape(-1, dim))
         self.face_shape_vals = fv.reshape(nf, nfq, -1)
-        self.face_shape_grads_ref = fg.reshape(nf, nfq, -1, 3)
-        self.face_normals = FACE_NORMALS
-        self.face_inds = FACES
+        self.face_shape_grads_ref = fg.reshape(nf, nfq, -1, dim)
+        self.face_inds, self.face_normals = FACETS[dim]
         self.num_quads, self.num_nodes = self.shape_vals.shape
         self.num_faces, self.num_face_quads = nf, nfq
         self.dirichlet_bc_info = dirichlet_bc_info
@@ -196,7 +264,7 @@ class FiniteElement:
         out = []
         for loc in (location_fns or []):
             flags = _location_flags(loc, self.points)
-            on = np.all(flags[self.cells[:, self.face_inds]], axis=-1)       # (ncell, 6)
+            on = np.all(flags[self.cells[:, self.face_inds]], axis=-1)       # (ncell, nfacet)
             out.append(np.argwhere(on))
         return out
 
@@ -207,7 +275,7 @@ class FiniteElement:
 
     def get_shape_grads(self, points=None):
         pts = self.points if points is None else points
-        X = pts[self.cells][:, None]                                         # (c,1,n,3)
+        X = pts[self.cells][:, None]                                         # (c,1,n,dim)
         _, det, inv = self._jac(X, self.shape_grads_ref[None])
         grads = np.einsum("qnd,cqde->cqne", self.shape_grads_ref, inv)
         return grads, det * self.quad_weights[None]
@@ -220,17 +288,18 @@ class FiniteElement:
     @property
     def shape_grads(self):
         """Physical shape gradients (num_cells, num_quads, num_nodes, dim) (problem.py:137)."""
-        return self._geometry()[0]
+        return _field(self._geometry()[0])
 
     @property
     def JxW(self):
         """det(J) * w per (cell, quadrature point) (num_cells, num_quads) (problem.py:137)."""
-        return self._geometry()[1]
+        return _field(self._geometry()[1])
 
     @property
     def v_grads_JxW(self):
         """(num_cells, num_quads, num_nodes, 1, dim) (problem.py:139)."""
-        return self.shape_grads[:, :, :, None, :] * self.JxW[:, :, None, None, None]
+        g, jxw = self._geometry()
+        return _field(g[:, :, :, None, :] * jxw[:, :, None, None, None])
 
     def _cell_values(self, sol):
         """sol (num_total_nodes, vec) gathered per cell; torch stays torch (on its device)."""
@@ -267,7 +336,7 @@ class FiniteElement:
         return np.einsum("sni,sqn->sqi", cs, vals)
 
     def get_physical_quad_points(self):
-        return np.einsum("qn,cnd->cqd", self.shape_vals, self.points[self.cells])
+        return _field(np.einsum("qn,cnd->cqd", self.shape_vals, self.points[self.cells]))
 
     def get_face_shape_grads(self, boundary_inds):
         X = self.points[self.cells[boundary_inds[:, 0]]][:, None]
@@ -279,5 +348,5 @@ class FiniteElement:
         return grads, nanson
 
     def get_physical_surface_quad_points(self, boundary_inds):
-        return np.einsum("sqn,snd->sqd", self.face_shape_vals[boundary_inds[:, 1]],
-                         self.points[self.cells[boundary_inds[:, 0]]])
+        return _field(np.einsum("sqn,snd->sqd", self.face_shape_vals[boundary_inds[:, 1]],
+                                self.points[self.cells[boundary_inds[:, 0]]]))

@@ -1,7 +1,8 @@
 """Mesh container and structured box generators.
 
 ``Mesh`` mirrors jax_fem/generate_mesh.py:13-27 (points (nnode, dim), cells
-(ncell, nn)).  ``box_mesh`` reproduces generate_mesh.py:151-189 (node index
+(ncell, nn)).  ``rectangle_mesh`` reproduces generate_mesh.py:120-148 (node
+index i*(Ny+1)+j, QUAD4 corners counter-clockwise).  ``box_mesh`` reproduces generate_mesh.py:151-189 (node index
 (i*(Ny+1)+j)*(Nz+1)+k, HEX8 corners in VTK order) and returns a ``Mesh``
 directly (the reference wraps it in a meshio object).  ``box_mesh_hex27`` is
 the structured HEX27 counterpart (the reference only has a gmsh path,
@@ -20,6 +21,8 @@ HEX27_OFFSETS = np.array([
     [0, 1, 1], [2, 1, 1], [1, 0, 1], [1, 2, 1], [1, 1, 0], [1, 1, 2], [1, 1, 1]])
 HEX8_OFFSETS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
                          [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+# VTK quadrilateral: counter-clockwise corners
+QUAD4_OFFSETS = np.array([[0, 0], [1, 0], [1, 1], [0, 1]])
 
 
 _CELL_TYPES = {"HEX8": "hexahedron", "HEX27": "hexahedron27", "HEX20": "hexahedron20", "TET4": "tetra",
@@ -88,6 +91,20 @@ def _detect_lattice(points):
 
 def _lattice(n, L):
     return np.linspace(0.0, L, n + 1)
+
+
+def rectangle_mesh(Nx, Ny, domain_x=1.0, domain_y=1.0):
+    """QUAD4 rectangle with Nx x Ny cells (generate_mesh.py:120-148); like the
+    reference's meshio object it has ``points`` and ``cells_dict['quad']``."""
+    X, Y = np.meshgrid(_lattice(Nx, domain_x), _lattice(Ny, domain_y), indexing="ij")
+    points = np.stack((X, Y), axis=-1).reshape(-1, 2)
+    idx = np.arange(len(points)).reshape(Nx + 1, Ny + 1)
+    i, j = [a.reshape(-1) for a in np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")]
+    o = QUAD4_OFFSETS
+    cells = idx[i[:, None] + o[None, :, 0], j[:, None] + o[None, :, 1]]
+    mesh = Mesh(points, cells.astype(np.int64), "QUAD4")
+    mesh.cell_grid = (Nx, Ny)
+    return mesh
 
 
 def box_mesh(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0):

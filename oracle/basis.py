@@ -1,8 +1,9 @@
-"""Reference-cell basis and quadrature for HEX8 / HEX27 (oracle).
+"""Reference-cell basis and quadrature for HEX8 / HEX27 / QUAD4 (oracle).
 
 Restates ``jax_fem/basis.py``:
   * ``get_elements`` (basis.py:19-114): HEX8 re_order [0,1,3,2,4,5,7,6] (:53),
-    default quadrature order 2 (:56); HEX27 re_order (:60-61), default order 10 (:64).
+    default quadrature order 2 (:56); HEX27 re_order (:60-61), default order 10 (:64);
+    QUAD4 re_order [0,1,3,2] (:87), default order 2 (:89), interval facets.
   * ``get_shape_vals_and_grads`` (basis.py:141-175): Lagrange basis of degree 1/2
     on the [0,1]^3 reference cell at Gauss-Jacobi (= Gauss-Legendre for the hex)
     points, columns permuted into mesh (VTK) node order.
@@ -15,11 +16,13 @@ restatement works from the element definition instead: the basis function of a
 node is the tensor product of 1-D Lagrange polynomials through the node's
 reference coordinates, and the node coordinates in mesh order follow from
 basix's hexahedron numbering (vertices lexicographic with x fastest, then the 12
-edges, 6 facets and the interior) composed with ``re_order``.
+edges, 6 facets and the interior) or quadrilateral numbering (vertices
+lexicographic with x fastest) composed with ``re_order``.
 
 Pinned by: partition of unity, sum of gradients = 0, exact reproduction of
-linear (HEX8) and quadratic (HEX27) fields, weights summing to 1, exactness of
-the Gauss rule on monomials, and end-to-end by the FEniCSx fixtures.
+linear (HEX8, QUAD4) and quadratic (HEX27) fields, weights summing to 1,
+exactness of the Gauss rule on monomials, and end-to-end by the FEniCSx fixtures
+and the source-field notebook's gradient (QUAD4).
 """
 import numpy as np
 
@@ -35,13 +38,20 @@ _BASIX_FACETS = [[0, 1, 2, 3], [0, 1, 4, 5], [0, 2, 4, 6],
 _BASIX_FACET_NORMALS = np.array([[0, 0, -1], [0, -1, 0], [-1, 0, 0],
                                  [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
 
+# basix quadrilateral vertices (x fastest), facets (intervals) and outward normals.
+_BASIX_QUAD_VERTS = np.array([[0, 0], [1, 0], [0, 1], [1, 1]], dtype=np.float64)
+_BASIX_QUAD_FACETS = [[0, 1], [0, 2], [1, 3], [2, 3]]
+_BASIX_QUAD_FACET_NORMALS = np.array([[0, -1], [-1, 0], [1, 0], [0, 1]], dtype=np.float64)
+
 RE_ORDER = {
     "HEX8": [0, 1, 3, 2, 4, 5, 7, 6],                                    # basis.py:53
     "HEX27": [0, 1, 3, 2, 4, 5, 7, 6, 8, 11, 13, 9, 16, 18, 19,
               17, 10, 12, 15, 14, 22, 23, 21, 24, 20, 25, 26],           # basis.py:60-61
+    "QUAD4": [0, 1, 3, 2],                                               # basis.py:87
 }
-DEFAULT_ORDER = {"HEX8": 2, "HEX27": 10}                                 # basis.py:56,64
-DEGREE = {"HEX8": 1, "HEX27": 2}
+DEFAULT_ORDER = {"HEX8": 2, "HEX27": 10, "QUAD4": 2}                     # basis.py:56,64,89
+DEGREE = {"HEX8": 1, "HEX27": 2, "QUAD4": 1}
+DIM = {"HEX8": 3, "HEX27": 3, "QUAD4": 2}
 
 
 def basix_node_coords(degree):
@@ -56,6 +66,8 @@ def basix_node_coords(degree):
 
 def node_coords(ele_type):
     """Reference coordinates of the element nodes in mesh order (basis.py:171)."""
+    if DIM[ele_type] == 2:
+        return _BASIX_QUAD_VERTS[RE_ORDER[ele_type]]
     return basix_node_coords(DEGREE[ele_type])[RE_ORDER[ele_type]]
 
 
@@ -109,28 +121,34 @@ def _lagrange_1d(nodes, x):
 
 
 def tabulate(ele_type, X):
-    """Shape values (np, nn) and reference gradients (np, nn, 3) at points X."""
+    """Shape values (np, nn) and reference gradients (np, nn, dim) at points X."""
     deg = DEGREE[ele_type]
+    dim = DIM[ele_type]
     nodes_1d = np.linspace(0.0, 1.0, deg + 1)
     nc = node_coords(ele_type)
     idx = np.rint(nc * deg).astype(int)           # 1-D node index per direction
     v = []
     d = []
-    for c in range(3):
+    for c in range(dim):
         vc, dc = _lagrange_1d(nodes_1d, X[:, c])
-        v.append(vc)
-        d.append(dc)
-    vals = v[0][:, idx[:, 0]] * v[1][:, idx[:, 1]] * v[2][:, idx[:, 2]]
-    gx = d[0][:, idx[:, 0]] * v[1][:, idx[:, 1]] * v[2][:, idx[:, 2]]
-    gy = v[0][:, idx[:, 0]] * d[1][:, idx[:, 1]] * v[2][:, idx[:, 2]]
-    gz = v[0][:, idx[:, 0]] * v[1][:, idx[:, 1]] * d[2][:, idx[:, 2]]
-    return vals, np.stack([gx, gy, gz], axis=-1)
+        v.append(vc[:, idx[:, c]])
+        d.append(dc[:, idx[:, c]])
+    vals = np.ones_like(v[0])
+    for c in range(dim):
+        vals = vals * v[c]
+    grads = []
+    for e in range(dim):
+        g = np.ones_like(v[0])
+        for c in range(dim):
+            g = g * (d[c] if c == e else v[c])
+        grads.append(g)
+    return vals, np.stack(grads, axis=-1)
 
 
 def get_shape_vals_and_grads(ele_type, quadrature_order=None):
-    """Restates basis.py:141-175 -> (shape_vals (nq,nn), shape_grads_ref (nq,nn,3), weights (nq))."""
+    """Restates basis.py:141-175 -> (shape_vals (nq,nn), shape_grads_ref (nq,nn,dim), weights (nq))."""
     order = DEFAULT_ORDER[ele_type] if quadrature_order is None else quadrature_order
-    X, W = hex_quadrature(order)
+    X, W = hex_quadrature(order) if DIM[ele_type] == 3 else quad_quadrature(order)
     vals, grads = tabulate(ele_type, X)
     return vals, grads, W
 
@@ -142,6 +160,8 @@ def get_face_shape_vals_and_grads(ele_type, quadrature_order=None):
     face_weights (6,nfq), face_normals (6,3), face_inds (6,4) in mesh numbering.
     """
     order = DEFAULT_ORDER[ele_type] if quadrature_order is None else quadrature_order
+    if DIM[ele_type] == 2:
+        return _quad_facets(ele_type, order)
     P2, W2 = quad_quadrature(order)
     # P1 map of the reference quadrilateral (basix vertex order (0,0),(1,0),(0,1),(1,1)).
     s, t = P2[:, 0], P2[:, 1]
@@ -162,3 +182,22 @@ def get_face_shape_vals_and_grads(ele_type, quadrature_order=None):
     nn = vals.shape[1]
     return (vals.reshape(nf, nfq, nn), grads.reshape(nf, nfq, nn, 3),
             np.stack(face_w), _BASIX_FACET_NORMALS.copy(), np.array(face_inds))
+
+
+def _quad_facets(ele_type, order):
+    """Interval facets of the quadrilateral: Gauss points on [0,1] mapped from the
+    facet's first basix vertex to its second, weights scaled by the edge length."""
+    x1, w1 = gauss_legendre_01(points_per_direction(order))
+    re = np.array(RE_ORDER[ele_type])
+    face_pts, face_w, face_inds = [], [], []
+    for facet in _BASIX_QUAD_FACETS:
+        V = _BASIX_QUAD_VERTS[facet]
+        face_pts.append(V[0][None] + x1[:, None] * (V[1] - V[0])[None])
+        face_w.append(w1 * np.linalg.norm(V[1] - V[0]))
+        face_inds.append([int(np.argwhere(re == v)[0, 0]) for v in facet])
+    face_pts = np.stack(face_pts)
+    nf, nfq, _ = face_pts.shape
+    vals, grads = tabulate(ele_type, face_pts.reshape(-1, 2))
+    nn = vals.shape[1]
+    return (vals.reshape(nf, nfq, nn), grads.reshape(nf, nfq, nn, 2),
+            np.stack(face_w), _BASIX_QUAD_FACET_NORMALS.copy(), np.array(face_inds))

@@ -2,7 +2,8 @@
 
 Restates:
   * ``box_mesh`` (generate_mesh.py:151-189): structured HEX8 grid, node index
-    (i*(Ny+1) + j)*(Nz+1) + k, cell corners in VTK order.
+    (i*(Ny+1) + j)*(Nz+1) + k, cell corners in VTK order; ``rectangle_mesh``
+    (generate_mesh.py:120-148), its QUAD4 counterpart.
   * a structured HEX27 grid in the same style (no reference counterpart: the
     reference builds HEX27 boxes with gmsh, generate_mesh.py:192-262); nodes on
     the doubled lattice, cell nodes in the VTK/mesh order of ``basis.node_coords``.
@@ -35,6 +36,18 @@ def box_mesh(Nx, Ny, Nz, Lx=1.0, Ly=1.0, Lz=1.0):
          ids[:-1, :-1, 1:], ids[1:, :-1, 1:], ids[1:, 1:, 1:], ids[:-1, 1:, 1:]]
     cells = np.stack(c, axis=3).reshape(-1, 8)
     return points, cells
+
+
+def rectangle_mesh(Nx, Ny, Lx=1.0, Ly=1.0):
+    """QUAD4 rectangle (generate_mesh.py:120-148): node index i*(Ny+1) + j,
+    corners (i,j), (i+1,j), (i+1,j+1), (i,j+1)."""
+    x = np.linspace(0, Lx, Nx + 1)
+    y = np.linspace(0, Ly, Ny + 1)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    points = np.stack((X, Y), axis=2).reshape(-1, 2)
+    ids = np.arange(len(points)).reshape(Nx + 1, Ny + 1)
+    c = [ids[:-1, :-1], ids[1:, :-1], ids[1:, 1:], ids[:-1, 1:]]
+    return points, np.stack(c, axis=2).reshape(-1, 4)
 
 
 def box_mesh_hex27(Nx, Ny, Nz, Lx=1.0, Ly=1.0, Lz=1.0):

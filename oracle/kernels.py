@@ -101,8 +101,9 @@ class Material:
         vec = ug.shape[-2]
         shp = ug.shape[:-2]
         if self.kind == "poisson":
-            A = np.zeros(shp + (1, 3, 1, 3))
-            A[..., 0, :, 0, :] = np.eye(3)
+            d = ug.shape[-1]
+            A = np.zeros(shp + (1, d, 1, d))
+            A[..., 0, :, 0, :] = np.eye(d)
             return _bc4(m1, A) * A
         I = np.eye(3)
         if self.kind == "linear_elastic":
@@ -212,9 +213,22 @@ def surface_residual(surface_fn, x_face, face_vals, nanson):
     return r.reshape(r.shape[0], -1)
 
 
-def mass_residual(mass_fn, x_quad, shape_vals, JxW):
-    """Mass kernel (problem.py:216-236) for a map m(x) independent of u."""
-    m = np.asarray(mass_fn(x_quad), dtype=np.float64)
+def mass_residual(mass_fn, x_quad, shape_vals, JxW, theta=None):
+    """Mass kernel (problem.py:216-236) for a map m(x) independent of u, or
+    m(x, theta) with a per-quadrature internal variable theta (c,q) (the
+    reference passes cell_internal_vars to the mass map, problem.py:225-229)."""
+    m = mass_fn(x_quad) if theta is None else mass_fn(x_quad, theta)
+    m = np.asarray(m)
     m = np.broadcast_to(m, x_quad.shape[:2] + (m.shape[-1],))
     r = np.einsum("cqi,qn,cq->cni", m, shape_vals, JxW)
     return r.reshape(r.shape[0], -1)
+
+
+def mass_theta_vjp(mass_fn, x_quad, shape_vals, JxW, theta, cell_lam):
+    """d(lam . r_mass)/d theta (c,q): sum_a,i lam[c,a,i] N_a(q) JxW dm_i/dtheta,
+    with dm/dtheta by the complex step (exact to rounding for a map analytic in theta)."""
+    h = 1e-30
+    dm = np.asarray(mass_fn(x_quad, theta + 1j * h)).imag / h
+    dm = np.broadcast_to(dm, x_quad.shape[:2] + (dm.shape[-1],))
+    lam_q = np.einsum("cni,qn->cqi", cell_lam, shape_vals)
+    return np.einsum("cqi,cqi,cq->cq", lam_q, dm, JxW)

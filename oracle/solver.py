@@ -37,7 +37,7 @@ class OracleProblem:
 
     def __init__(self, points, cells, vec, material, ele_type="HEX8", quadrature_order=None,
                  dirichlet_bc_info=None, location_fns=None, surface_maps=None, mass_map=None,
-                 params=None):
+                 params=None, mass_params=None):
         self.points = np.asarray(points, dtype=np.float64)
         self.cells = np.asarray(cells, dtype=np.int64)
         self.vec, self.mat, self.ele_type = vec, material, ele_type
@@ -60,6 +60,7 @@ class OracleProblem:
             xs = fe.physical_surface_quad_points(self.points, self.cells, bi, fsv)
             self.face_data.append((bi, fsv[bi[:, 1]], nanson, xs))
         self.params = params
+        self.mass_params = mass_params      # internal variable of the mass map, (ncell, nq)
         self._csr = None
 
     # -- Dirichlet data (fe.py:220-269) --------------------------------------
@@ -91,7 +92,8 @@ class OracleProblem:
         """Element vectors of the u-independent surface and mass maps, scattered."""
         f = np.zeros((self.num_nodes, self.vec))
         if self.mass_map is not None:
-            r = kernels.mass_residual(self.mass_map, self.quad_points, self.shape_vals, self.JxW)
+            r = kernels.mass_residual(self.mass_map, self.quad_points, self.shape_vals, self.JxW,
+                                      self.mass_params)
             np.add.at(f, self.cells.reshape(-1), r.reshape(-1, self.vec))
         for (bi, fvals, nanson, xs), smap in zip(self.face_data, self.surface_maps):
             r = kernels.surface_residual(smap, xs, fvals, nanson)
@@ -277,8 +279,9 @@ def newton_solve(prob, u0=None, rel_tol=1e-8, tol=1e-6, method="bicgstab", lin_t
     return u, {"newton_iters": n, "res_hist": hist, "lin_iters": lin_iters, "alphas": alphas}
 
 
-def implicit_vjp(prob, u, v, method="bicgstab", dirichlet_param_grads=None):
-    """implicit_vjp (solver.py:1324-1380) for per-quad material parameters.
+def implicit_vjp(prob, u, v, method="bicgstab", dirichlet_param_grads=None, wrt="material"):
+    """implicit_vjp (solver.py:1324-1380) for per-quad material parameters
+    (wrt="material") or the mass map's internal variable (wrt="mass").
 
     Returns (grad_p (ncell,nq) or None, lam).  ``dirichlet_param_grads`` is an
     optional list of (rows, dg_dp) pairs: the derivative of the Dirichlet values
@@ -291,7 +294,11 @@ def implicit_vjp(prob, u, v, method="bicgstab", dirichlet_param_grads=None):
     bc_all = np.concatenate(prob.bc_rows()) if prob.bc_rows() else np.zeros(0, dtype=np.int64)
     lam_I[bc_all] = 0.0
     grad_p = None
-    if prob.params is not None:
+    if wrt == "mass":
+        cl = lam_I.reshape(prob.num_nodes, prob.vec)[prob.cells]
+        grad_p = -kernels.mass_theta_vjp(prob.mass_map, prob.quad_points, prob.shape_vals, prob.JxW,
+                                         prob.mass_params, cl)
+    elif prob.params is not None:
         cl = lam_I.reshape(prob.num_nodes, prob.vec)[prob.cells]
         grad_p = -kernels.cell_param_vjp(prob.mat, prob.cell_u(u), prob.grads, prob.JxW,
                                          prob.params, cl)

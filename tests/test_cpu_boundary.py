@@ -45,6 +45,35 @@ def test_tables_match_oracle(jfa):
         np.testing.assert_array_equal(F.face_normals, fn)
 
 
+def test_quad4_tables_and_rectangle_match_oracle(jfa):
+    from importlib import import_module
+    fe = import_module("jax_fem_amd.fe")
+    m = jfa.rectangle_mesh(5, 3, 2.0, 1.5)
+    pts, cells = ofe.rectangle_mesh(5, 3, 2.0, 1.5)
+    np.testing.assert_array_equal(m.points, pts)
+    np.testing.assert_array_equal(m.cells_dict[jfa.get_meshio_cell_type("QUAD4")], cells)
+    top = lambda p: np.isclose(p[1], 1.5, atol=1e-5)
+    for order in (None, 1, 3):
+        F = fe.FiniteElement(jfa.Mesh(m.points, m.cells), 1, 2, "QUAD4", quadrature_order=order)
+        v, g, w = obasis.get_shape_vals_and_grads("QUAD4", order)
+        np.testing.assert_allclose(F.shape_vals, v, atol=1e-15)
+        np.testing.assert_allclose(F.shape_grads_ref, g, atol=1e-14)
+        np.testing.assert_allclose(F.quad_weights, w, atol=1e-15)
+        fv, fg, fw, fn, fi = obasis.get_face_shape_vals_and_grads("QUAD4", order)
+        np.testing.assert_allclose(F.face_shape_vals, fv, atol=1e-15)
+        np.testing.assert_allclose(F.face_quad_weights, fw, atol=1e-15)
+        np.testing.assert_array_equal(F.face_inds, fi)
+        np.testing.assert_array_equal(F.face_normals, fn)
+        sg, jxw = ofe.shape_grads(pts, cells, g, w)
+        np.testing.assert_allclose(F.shape_grads, sg, atol=1e-13)
+        np.testing.assert_allclose(F.JxW, jxw, atol=1e-15)
+        bi = F.get_boundary_conditions_inds([top])[0]
+        np.testing.assert_array_equal(bi, ofe.boundary_inds(pts, cells, [top], fi)[0])
+        assert abs(F.get_face_shape_grads(bi)[1].sum() - 2.0) < 1e-13
+    with pytest.raises(ValueError):
+        fe.FiniteElement(m, 1, 3, "QUAD4")
+
+
 def _iscl(a, b):
     return np.abs(a - b) < 1e-5
 

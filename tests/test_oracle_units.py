@@ -24,6 +24,63 @@ def test_basis_invariants(ele):
     np.testing.assert_allclose(vn, np.eye(len(nc)), atol=1e-12)
 
 
+def test_quad4_basis_invariants_and_edges():
+    for order in (0, 2, 3, 5):
+        v, g, w = basis.get_shape_vals_and_grads("QUAD4", order)
+        assert abs(w.sum() - 1) < 1e-14 and v.shape[1] == 4 and g.shape[2] == 2
+        np.testing.assert_allclose(v.sum(1), 1, atol=1e-14)
+        np.testing.assert_allclose(g.sum(1), 0, atol=1e-13)
+        X, _ = basis.quad_quadrature(order)
+        nc = basis.node_coords("QUAD4")
+        f = lambda x: 1 + 2 * x[:, 0] - x[:, 1] + 0.5 * x[:, 0] * x[:, 1]       # bilinear: reproduced
+        np.testing.assert_allclose(v @ f(nc), f(X), atol=1e-13)
+    # counter-clockwise corners (re_order [0,1,3,2] of basix's lexicographic vertices)
+    np.testing.assert_array_equal(basis.node_coords("QUAD4"), [[0, 0], [1, 0], [1, 1], [0, 1]])
+    vn, _ = basis.tabulate("QUAD4", basis.node_coords("QUAD4"))
+    np.testing.assert_allclose(vn, np.eye(4), atol=1e-14)
+    pts, cells = fe.rectangle_mesh(4, 3, 2.0, 1.5)
+    assert np.allclose(pts[cells[5]], [[0.5, 1.0], [1.0, 1.0], [1.0, 1.5], [0.5, 1.5]])
+    v, g, w = basis.get_shape_vals_and_grads("QUAD4")
+    _, JxW = fe.shape_grads(pts, cells, g, w)
+    assert abs(JxW.sum() - 3.0) < 1e-13 and np.all(JxW > 0)
+    fsv, fsg, fw, fn, finds = basis.get_face_shape_vals_and_grads("QUAD4")
+    for loc, length, nface in ((lambda p: abs(p[1] - 1.5) < 1e-9, 2.0, 4), (lambda p: abs(p[0]) < 1e-9, 1.5, 3)):
+        bi = fe.boundary_inds(pts, cells, [loc], finds)[0]
+        assert len(bi) == nface
+        _, nanson = fe.face_shape_grads(pts, cells, bi, fsg, fn, fw)
+        assert abs(nanson.sum() - length) < 1e-13
+        xs = fe.physical_surface_quad_points(pts, cells, bi, fsv)
+        assert all(loc(x) for x in xs.reshape(-1, 2))
+
+
+def test_source_field_notebook_gradient():
+    """docs/source/learn/source_field_identification/example.ipynb: Poisson on a
+    64x64 QUAD4 square, mass map m = theta, u = 0 at x = 0 and x = 1; the
+    gradient of J = sum (u_pred - u_true)^2 JxW at theta_ini = 1e5 is
+    0.006816018916280742 at [10, 3] (finite differences: 0.0068160189166665075)."""
+    pts, cells = fe.rectangle_mesh(64, 64)
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-5)
+    info = [[left, right], [0, 0], [lambda p: 0.0, lambda p: 0.0]]
+    mass = lambda x, th: th[..., None]
+
+    def make(theta):
+        return osv.OracleProblem(pts, cells, 1, kernels.Material("poisson", k=1.0), ele_type="QUAD4",
+                                 dirichlet_bc_info=info, mass_map=mass, mass_params=theta)
+    xq = make(np.zeros((len(cells), 4))).quad_points
+    th_true = -1e5 * np.exp(-((xq[..., 0] - 0.5) ** 2 + (xq[..., 1] - 0.5) ** 2) / 0.02)
+    u_true, _ = osv.newton_solve(make(th_true))
+    pp = make(1e5 * np.ones_like(th_true))
+    u_pred, _ = osv.newton_solve(pp)
+    uq = lambda u: np.einsum("cn,qn->cq", u[cells], pp.shape_vals)
+    d = uq(u_pred) - uq(u_true)
+    v = np.zeros(pp.N)
+    np.add.at(v, cells.reshape(-1), np.einsum("cq,qn,cq->cn", 2 * d, pp.shape_vals, pp.JxW).reshape(-1))
+    g, _, _ = osv.implicit_vjp(pp, u_pred, v, wrt="mass")
+    assert g.shape == (64 * 64, 4)
+    assert abs(g[10, 3] - 0.006816018916280742) < 1e-9 * 0.0068
+
+
 def test_gauss_exactness_and_hex27_order():
     x, w = basis.gauss_legendre_01(3)
     for k in range(6):
