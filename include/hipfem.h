@@ -397,6 +397,38 @@ int hf_mg_stencil_matvec_f32(const hf_mg* mg, const float* S, const float* x, fl
 int hf_profile_enable(int32_t on);
 int hf_profile_read(int64_t* launches, double* total_ms);
 
+/* ---- topology optimisation: filters and the MMA subproblem (jax_fem/mma.py) ----
+ * Filters replace applySensitivityFilter / applyDensityFilter (mma.py:58-66)
+ * over compute_filter_kd_tree's H (mma.py:27-56), held as a k-wide neighbour
+ * table: row i of H has entries w[i*k+t] at columns nbr[i*k+t]; hs = row sums.
+ *   sensitivity: out[c*n+i] = sum_t w[i*k+t] rho[j] g[c*n+j] / max(1e-3, rho[j]) / hs[j],
+ *                j = nbr[i*k+t], for the ncols gradients g (objective, constraints)
+ *   density:     out[i] = sum_t w[i*k+t] rho[nbr[i*k+t]] / hs[i]                    */
+int hf_mma_sensitivity_filter_f64(const int32_t* nbr, const double* w, const double* hs, int64_t n, int32_t k,
+                                  const double* rho, const double* g, int32_t ncols, double* out,
+                                  hf_stream_t stream);
+int hf_mma_density_filter_f64(const int32_t* nbr, const double* w, const double* hs, int64_t n, int32_t k,
+                              const double* rho, double* out, hf_stream_t stream);
+/* One MMA subproblem: replaces MMA.mmasub + subsolv (mma.py:114-410, the
+ * m < n branch).  n design variables, m = 1..4 constraints.  Device arrays
+ * (FP64): xval, xold1, xold2, xmin, xmax, df0dx (n); fval (m); dfdx (m, n)
+ * row-major; low, upp (n) in/out (the asymptotes, read when iter > 2);
+ * xmma (n) out.  Host arrays: a, c, d (m).  iter counts from 1 as MMA.epoch.
+ * Synchronises the stream (the backtracking line search reads residual norms). */
+typedef struct hf_mma hf_mma;
+typedef struct {
+  int32_t newton_steps;   /* primal-dual Newton steps over all epsi levels */
+  int32_t trials;         /* residual evaluations in the backtracking */
+  double res_max;         /* max-norm of the KKT residual at the returned point */
+  double res_norm;
+} hf_mma_info;
+int hf_mma_create(int64_t n, int32_t m, hf_mma** out);
+int hf_mma_destroy(hf_mma* mma);
+int hf_mma_update_f64(hf_mma* mma, int32_t iter, const double* xval, const double* xold1, const double* xold2,
+                      const double* xmin, const double* xmax, double* low, double* upp, const double* df0dx,
+                      const double* fval, const double* dfdx, double a0, const double* a, const double* c,
+                      const double* d, double move, double* xmma, hf_mma_info* info, hf_stream_t stream);
+
 /* ---- small vector helpers (FP64 accumulate) ---- */
 size_t hf_dot_workspace_size(void);
 /* *result (host) = sum x*y accumulated in FP64, deterministic order */

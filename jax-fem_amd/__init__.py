@@ -20,7 +20,8 @@ from .mesh import Mesh, box_mesh, box_mesh_hex27, rectangle_mesh, get_meshio_cel
 from .problem import Problem
 from .utils import save_sol, read_vtu, write_vtu, modify_vtu_file
 from . import numpy
-from .autodiff import grad, vmap, jacfwd, jacrev
+from .autodiff import grad, vmap, jacfwd, jacrev, value_and_grad
+from . import mma
 from .learned_pc import LearnedPC
 from .multigrid import Multigrid
 from .solver import (solver, ad_wrapper, implicit_vjp, get_A, linear_solver, apply_bc_vec, assign_bc, copy_bc,
@@ -36,5 +37,5 @@ if not logger.handlers:
 __all__ = ["HipFemError", "Problem", "solver", "ad_wrapper", "implicit_vjp", "get_A", "linear_solver", "apply_bc_vec",
            "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "rectangle_mesh", "get_meshio_cell_type",
            "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov",
-           "LearnedPC", "Multigrid", "numpy", "grad", "vmap", "jacfwd", "jacrev",
+           "LearnedPC", "Multigrid", "numpy", "grad", "vmap", "jacfwd", "jacrev", "value_and_grad", "mma",
            "save_sol", "read_vtu", "write_vtu", "modify_vtu_file"]

@@ -35,6 +35,10 @@ class HFKrylovInfo(ctypes.Structure):
     _fields_ = [("iterations", c_i32), ("status", c_i32), ("res_norm", c_f64), ("b_norm", c_f64)]
 
 
+class HFMmaInfo(ctypes.Structure):
+    _fields_ = [("newton_steps", c_i32), ("trials", c_i32), ("res_max", c_f64), ("res_norm", c_f64)]
+
+
 class HipFemError(RuntimeError):
     pass
 
@@ -116,6 +120,13 @@ _SIGS = {
     "hf_mg_solve_stencil_f32": (c_i32, [vp, vp, vp, vp, c_i32, c_i32, c_f64, c_f64, c_i32, vp,
                                         ctypes.POINTER(HFKrylovInfo), vp]),
     "hf_mg_stencil_matvec_f32": (c_i32, [vp, vp, vp, vp, vp]),
+    "hf_mma_sensitivity_filter_f64": (c_i32, [vp, vp, vp, c_i64, c_i32, vp, vp, c_i32, vp, vp]),
+    "hf_mma_density_filter_f64": (c_i32, [vp, vp, vp, c_i64, c_i32, vp, vp, vp]),
+    "hf_mma_create": (c_i32, [c_i64, c_i32, ctypes.POINTER(vp)]),
+    "hf_mma_destroy": (c_i32, [vp]),
+    "hf_mma_update_f64": (c_i32, [vp, c_i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_f64,
+                                  ctypes.POINTER(c_f64), ctypes.POINTER(c_f64), ctypes.POINTER(c_f64), c_f64, vp,
+                                  ctypes.POINTER(HFMmaInfo), vp]),
     "hf_dot_workspace_size": (c_sz, []),
     "hf_dot_f64": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),
     "hf_dot_f32": (c_i32, [vp, vp, c_i64, vp, ctypes.POINTER(c_f64), vp]),

@@ -82,7 +82,7 @@ def _complex_step(f, args, argnums, vector_out):
     return y.reshape(y.shape[:-1] + x.shape)
 
 
-def _eager(f, args, argnums, vector_out):
+def _eager(f, args, argnums, vector_out, with_value=False):
     """Derivative of f with respect to args[argnums] by torch autograd on the eager
     graph (reverse mode; a Jacobian row by row).  Raises TypeError when f does not
     produce a tensor connected to its argument."""
@@ -96,7 +96,8 @@ def _eager(f, args, argnums, vector_out):
     if not vector_out:
         if y.numel() != 1:
             raise TypeError("grad needs a scalar-valued function")
-        return torch.autograd.grad(y, x)[0]
+        g = torch.autograd.grad(y, x)[0]
+        return (y.detach(), g) if with_value else g
     flat = y.reshape(-1)
     rows = [torch.autograd.grad(flat[k], x, retain_graph=k + 1 < flat.numel(), allow_unused=True)[0]
             for k in range(flat.numel())]
@@ -116,6 +117,32 @@ def _torch_call(transform, f, args, argnums, vector_out):
             return _eager(f, args, argnums, vector_out)
         except Exception:
             raise err
+
+
+def value_and_grad(f, argnums=0):
+    """(f(x), grad f(x)) in one evaluation, as jax.value_and_grad."""
+    def both(*args):
+        try:
+            g, v = torch.func.grad_and_value(f, argnums=argnums)(*args)
+            return v, g
+        except Exception as err:
+            try:
+                return _eager(f, args, argnums, False, with_value=True)
+            except Exception:
+                raise err
+
+    def run(*args):
+        t = _first_tensor(args)
+        if t is None:
+            try:
+                with jnp.on_device(torch.device("cpu")):
+                    v, g = both(*_to_device(args, torch.device("cpu")))
+                return v.detach().cpu().numpy(), g.detach().cpu().numpy()
+            except Exception:
+                return np.asarray(f(*args)), _complex_step(f, args, argnums, False)
+        with jnp.on_device(t.device):
+            return both(*_to_device(args, t.device))
+    return run
 
 
 def _numpy_call(transform, f, args, argnums, vector_out):

@@ -199,14 +199,17 @@ class FiniteElement:
         self.num_total_dofs = self.num_total_nodes * vec
         order = ELEMENTS[ele_type][2] if quadrature_order is None else quadrature_order
         self.quadrature_order = order
-        qp, self.quad_weights = volume_rule(order, dim)
-        self.shape_vals, self.shape_grads_ref = tabulate(ele_type, qp)
-        fp, self.face_quad_weights = face_rule(order, dim)
+        qp, qw = volume_rule(order, dim)
+        sv, sg = tabulate(ele_type, qp)
+        fp, fw = face_rule(order, dim)
         nf, nfq, _ = fp.shape
         fv, fg = tabulate(ele_type, fp.reshape(-1, dim))
-        self.face_shape_vals = fv.reshape(nf, nfq, -1)
-        self.face_shape_grads_ref = fg.reshape(nf, nfq, -1, dim)
-        self.face_inds, self.face_normals = FACETS[dim]
+        self.quad_weights, self.shape_vals, self.shape_grads_ref = _field(qw), _field(sv), _field(sg)
+        self.face_quad_weights = _field(fw)
+        self.face_shape_vals = _field(fv.reshape(nf, nfq, -1))
+        self.face_shape_grads_ref = _field(fg.reshape(nf, nfq, -1, dim))
+        self.face_inds, normals = FACETS[dim]
+        self.face_normals = _field(normals)
         self.num_quads, self.num_nodes = self.shape_vals.shape
         self.num_faces, self.num_face_quads = nf, nfq
         self.dirichlet_bc_info = dirichlet_bc_info
@@ -344,8 +347,8 @@ class FiniteElement:
         _, det, inv = self._jac(X, g)
         grads = np.einsum("sqnd,sqde->sqne", g, inv)
         nJ = np.einsum("sd,sqde->sqe", self.face_normals[boundary_inds[:, 1]], inv)
-        nanson = np.linalg.norm(nJ, axis=-1) * det * self.face_quad_weights[boundary_inds[:, 1]]
-        return grads, nanson
+        nanson = np.linalg.norm(nJ, axis=-1) * det * np.asarray(self.face_quad_weights)[boundary_inds[:, 1]]
+        return _field(grads), _field(nanson)
 
     def get_physical_surface_quad_points(self, boundary_inds):
         return _field(np.einsum("sqn,snd->sqd", self.face_shape_vals[boundary_inds[:, 1]],

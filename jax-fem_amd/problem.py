@@ -263,6 +263,61 @@ class Problem:
             self._terms = t
         return self._terms
 
+    # ------------------------------------------- geometric quantities (problem.py:130-182)
+    def _geometric(self):
+        g = getattr(self, "_geom_q", None)
+        if g is None:
+            from .fe import _field
+            fe = self.fe
+            g = {"physical_quad_points": fe.get_physical_quad_points(),
+                 "JxW": _field(np.asarray(fe.JxW)[:, None, :]), "shape_grads": fe.shape_grads,
+                 "v_grads_JxW": fe.v_grads_JxW, "selected_face_shape_grads": [], "nanson_scale": [],
+                 "selected_face_shape_vals": [], "physical_surface_quad_points": []}
+            for bi in self.boundary_inds_list:
+                grads, nanson = fe.get_face_shape_grads(bi)
+                g["selected_face_shape_grads"].append(_field(grads))
+                g["nanson_scale"].append(_field(np.asarray(nanson)[:, None, :]))
+                g["selected_face_shape_vals"].append(_field(np.asarray(fe.face_shape_vals)[bi[:, 1]]))
+                g["physical_surface_quad_points"].append(fe.get_physical_surface_quad_points(bi))
+            self._geom_q = g
+        return g
+
+    @property
+    def physical_quad_points(self):
+        """(num_cells, num_quads, dim)"""
+        return self._geometric()["physical_quad_points"]
+
+    @property
+    def physical_surface_quad_points(self):
+        """per surface: (num_selected_faces, num_face_quads, dim)"""
+        return self._geometric()["physical_surface_quad_points"]
+
+    @property
+    def selected_face_shape_grads(self):
+        return self._geometric()["selected_face_shape_grads"]
+
+    @property
+    def selected_face_shape_vals(self):
+        return self._geometric()["selected_face_shape_vals"]
+
+    @property
+    def nanson_scale(self):
+        """per surface: (num_selected_faces, num_vars, num_face_quads)"""
+        return self._geometric()["nanson_scale"]
+
+    @property
+    def JxW(self):
+        """(num_cells, num_vars, num_quads)"""
+        return self._geometric()["JxW"]
+
+    @property
+    def shape_grads(self):
+        return self._geometric()["shape_grads"]
+
+    @property
+    def v_grads_JxW(self):
+        return self._geometric()["v_grads_JxW"]
+
     # ---------------------------------------------------------------- layout
     def unflatten_fn_sol_list(self, dofs):
         return [dofs.reshape(self.fe.num_total_nodes, self.vec)]
