@@ -234,7 +234,7 @@ def _oracle_case(wl, cells=None):
     raise ValueError("no CPU baseline for workload %s" % wl)
 
 
-def _oracle_solve(op, steps):
+def _oracle_solve(op, steps, progress=None):
     """The reference's solve of a workload: for each load step a Newton solve with
     Jacobi-BiCGSTAB (tol = atol = 1e-10, the jax_solve defaults, solver.py:63-92),
     warm-started from the last.  Returns (seconds, residual+tangent updates, BiCGSTAB iterations)."""
@@ -246,7 +246,7 @@ def _oracle_solve(op, steps):
     for spec in steps:
         if spec is not None:
             op.update_bcs(spec)
-        u, info = osv.newton_solve(op, u0=u, method="bicgstab")
+        u, info = osv.newton_solve(op, u0=u, method="bicgstab", log=progress)
         upd += info["newton_iters"] + 1
         its += sum(info["lin_iters"])
     return time.perf_counter() - t0, upd, its
@@ -344,7 +344,7 @@ def cpu_only(wl, mode):
             _oracle_solve(op, steps)
         for _ in range(nrun):
             op, steps = _oracle_case(wl)
-            t, upd, its = _oracle_solve(op, steps)
+            t, upd, its = _oracle_solve(op, steps, progress=log)
             runs.append(t)
             log("full oracle run: %.1f s, %d updates, %d BiCGSTAB iterations" % (t, upd, its))
         runs.sort()
