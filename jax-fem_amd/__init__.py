@@ -16,11 +16,12 @@ from . import _lib
 from ._lib import HipFemError
 from .fe import FiniteElement
 from .materials import LinearElastic, NeoHookean, Poisson, Material
-from .mesh import Mesh, box_mesh, box_mesh_hex27, rectangle_mesh, get_meshio_cell_type
+from .mesh import (Mesh, box_mesh, box_mesh_hex27, box_mesh_tet, box_mesh_gmsh, rectangle_mesh, rectangle_mesh_tri,
+                   get_meshio_cell_type)
 from .problem import Problem
 from .utils import save_sol, read_vtu, write_vtu, modify_vtu_file
 from . import numpy
-from .autodiff import grad, vmap, jacfwd, jacrev, value_and_grad
+from .autodiff import grad, vmap, jacfwd, jacrev, value_and_grad, hessian
 from . import mma
 from .learned_pc import LearnedPC
 from .multigrid import Multigrid
@@ -35,7 +36,8 @@ if not logger.handlers:
     logger.setLevel(logging.INFO)
 
 __all__ = ["HipFemError", "Problem", "solver", "ad_wrapper", "implicit_vjp", "get_A", "linear_solver", "apply_bc_vec",
-           "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "rectangle_mesh", "get_meshio_cell_type",
+           "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "box_mesh_tet", "box_mesh_gmsh", "rectangle_mesh",
+           "rectangle_mesh_tri", "get_meshio_cell_type",
            "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov",
-           "LearnedPC", "Multigrid", "numpy", "grad", "vmap", "jacfwd", "jacrev", "value_and_grad", "mma",
+           "LearnedPC", "Multigrid", "numpy", "grad", "vmap", "jacfwd", "jacrev", "value_and_grad", "hessian", "mma",
            "save_sol", "read_vtu", "write_vtu", "modify_vtu_file"]

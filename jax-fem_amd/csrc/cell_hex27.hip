@@ -11,21 +11,29 @@
 // dN_n/dxi, grad u = (sum_n u_n (x) dN_n/dxi) J^-1, P and S_q = w P J^-T;
 // lane (n, i) then sums r_n[i] = sum_q sum_e dN_n/dxi_e(q) S_q[i][e].
 //
-// Jacobian (one 256-thread workgroup per cell), K[(a,i),(b,k)] =
-// sum_q sum_{J,L} G_a[q][J] C_q[iJkL] G_b[q][L] with C_q the full tangent (w
-// included) and G the physical gradients, all staged in LDS:
-//   M_k[(a,i)][(q,L)] = sum_J G_a[q][J] C_q[iJkL]            (vector ALU)
-//   K_k[(a,i)][b]     = sum_{(q,L)} M_k[(a,i)][(q,L)] G_b[q][L]  (81 x 81 x 27 GEMM)
-// The GEMM runs on MFMA in FP32 (v_mfma_f32_16x16x4f32: 6 x 2 output tiles of
-// 16 x 16, 21 K-steps of 4; 81 padded to 96 rows / 84 inner, 27 to 32 columns)
-// and on the vector ALU in FP64.  2 * 3 * 81 * 81 * 27 = 1.06 MFLOP per cell in
-// the GEMM, 0.12 MFLOP in forming M.  Output block-major (cell, a, b, i, k).
+// Jacobian, K[(a,i),(b,k)] = sum_q sum_{J,L} G_a[q][J] C_q[iJkL] G_b[q][L] with C_q
+// the full tangent (w included) and G the physical gradients.
+//   FP64 (one 256-thread workgroup per cell, everything in LDS):
+//     M_k[(a,i)][(q,L)] = sum_J G_a[q][J] C_q[iJkL], K_k = M_k G on the vector ALU.
+//   FP32 on MFMA (hex27_jacobian_mfma: one wavefront per cell, 4 cells per
+//   workgroup, no workgroup barrier): for each (i,k) the 27 x 27 block
+//     K^{ik}[a][b] = sum_r A[a][r] V^{ik}[r][b],  r = (q, J),
+//     A[a][r] = G[q][a][J],  V^{ik}[r][b] = sum_L C_q[iJkL] G[q][b][L],
+//   is one GEMM on v_mfma_f32_32x32x2f32 (27 padded to 32, K = 84 in 42 steps).
+//   Lane l stands for node n = l % 32 as both the row a of A and the column b
+//   of V, and half-wave h = l / 32 owns the quadrature points q = t + 14h
+//   (t = 0..13): step s feeds r = (t = s / 3, J = s % 3), so every operand is a
+//   compile-time register of the lane's own gradients Gl[t][.] (42 floats),
+//   and only the 3 tangent entries C_q[iJk.] come from LDS, a broadcast per
+//   half-wave.  No index division or bound test in the K loop.
+// 2 * 3 * 81 * 81 * 27 = 1.06 MFLOP per cell of algorithmic GEMM work (1.39
+// MFLOP executed on MFMA with the padding), 0.12 MFLOP forming V.  Output
+// block-major (cell, a, b, i, k).
 #include "common.hpp"
 #include "element_math.hpp"
 
 namespace hf {
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Lattice position (0, 1, 2 = reference coordinate 0, 1/2, 1) of each VTK node.
 __constant__ int8_t c_h27_node[27][3] = {
@@ -145,7 +153,7 @@ __global__ void __launch_bounds__(64) hex27_residual_kernel(const double* __rest
 // ---------------------------------------------------------------------------
 constexpr int kMStride = 85;  // row stride of M_k in LDS (81 columns + padding)
 
-template <typename T, int MAT, bool MFMA>
+template <typename T, int MAT>
 __global__ void __launch_bounds__(256) hex27_jacobian_kernel(const double* __restrict__ pts,
                                                              const int32_t* __restrict__ cells, int64_t ncell,
                                                              const double* __restrict__ u,
@@ -191,30 +199,7 @@ __global__ void __launch_bounds__(256) hex27_jacobian_kernel(const double* __res
                                  sG[q][a][2] * Cq[((i * 3 + 2) * 3 + k) * 3 + l];
     }
     __syncthreads();
-    if constexpr (MFMA) {
-      const int lane = t & 63, wv = t >> 6;
-      for (int tile = wv; tile < 12; tile += 4) {  // wave-uniform
-        const int rt = tile >> 1, ct = tile & 1;
-        const int row = rt * 16 + (lane & 15), b = ct * 16 + (lane & 15);
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 3
-        for (int ks = 0; ks < 21; ++ks) {
-          const int kk = ks * 4 + (lane >> 4);
-          const float av = (row < 81 && kk < 81) ? (float)sM[row * kMStride + kk] : 0.f;
-          const float bv = (b < 27 && kk < 81) ? (float)sG[kk / 3][b][kk - 3 * (kk / 3)] : 0.f;
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
-        }
-        const int bo = ct * 16 + (lane & 15);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int ro = rt * 16 + 4 * (lane >> 4) + r;
-          if (ro < 81 && bo < 27) {
-            const int a = ro / 3, i = ro - 3 * (ro / 3);
-            o[((a * 27 + bo) * 3 + i) * 3 + k] = (T)acc[r];
-          }
-        }
-      }
-    } else {
+    {
       for (int p = t; p < 81 * 27; p += 256) {
         const int row = p / 27, b = p - 27 * (p / 27);
         const T* mr = sM + row * kMStride;
@@ -226,6 +211,185 @@ __global__ void __launch_bounds__(256) hex27_jacobian_kernel(const double* __res
       }
     }
     __syncthreads();  // sM is rewritten for the next k
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 element Jacobian on MFMA, one wavefront per cell.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kH27Waves = 4;   // cells (wavefronts) per workgroup
+constexpr int kH27T = 14;      // quadrature points per half-wave
+
+// VTK node lattice positions (c_h27_node) as a constexpr function: folds to a
+// constant in unrolled loops, so the 1-D factor tables are indexed at compile time.
+__host__ __device__ constexpr int h27_pos(int m, int d) {
+  constexpr int8_t tbl[27][3] = {{0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2},
+                                 {0, 2, 2}, {1, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {1, 0, 2}, {2, 1, 2},
+                                 {1, 2, 2}, {0, 1, 2}, {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1}, {0, 1, 1},
+                                 {2, 1, 1}, {1, 0, 1}, {1, 2, 1}, {1, 1, 0}, {1, 1, 2}, {1, 1, 1}};
+  return tbl[m][d];
+}
+
+struct H27Wave {          // one wavefront's LDS
+  float X[27][3], U[27][3];
+  float C[2 * kH27T][81];  // tangent per point (w included); point 27 is zero padding
+  float Inv[27][9];
+};
+
+__device__ __forceinline__ void h27_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int MAT>
+__global__ void __launch_bounds__(64 * kH27Waves) hex27_jacobian_mfma(const double* __restrict__ pts,
+                                                                      const int32_t* __restrict__ cells,
+                                                                      int64_t ncell, const double* __restrict__ u,
+                                                                      const double* __restrict__ params,
+                                                                      hf_material mat, float* __restrict__ out) {
+  __shared__ H27Wave sw[kH27Waves];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t c = (int64_t)blockIdx.x * kH27Waves + wv;
+  if (c >= ncell) return;  // wave-uniform; the kernel has no workgroup barrier
+  H27Wave& S = sw[wv];
+  const int h = lane >> 5, n = lane & 31;
+
+  // nodes of the cell
+  if (lane < 27) {
+    const int64_t nid = cells[c * 27 + lane];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      S.X[lane][d] = (float)pts[3 * nid + d];
+      S.U[lane][d] = (float)u[3 * nid + d];
+    }
+  }
+  S.C[27][lane] = 0.f;                       // the padding point: entries 0..63
+  if (lane < 81 - 64) S.C[27][64 + lane] = 0.f;  // and 64..80
+  h27_wave_sync();
+
+  // geometry and tangent at point q = n: half 0 sums x (x) dN, half 1 u (x) dN
+  {
+    const int q = n < 27 ? n : 26;
+    const float tq[3] = {(float)g3_point(q / 9), (float)g3_point((q / 3) % 3), (float)g3_point(q % 3)};
+    float P1[3][3], D1[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        P1[d][a] = lag2<float>(a, tq[d]);
+        D1[d][a] = dlag2<float>(a, tq[d]);
+      }
+    const float* src = h ? &S.U[0][0] : &S.X[0][0];
+    float acc[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) acc[d][e] = 0.f;
+#pragma unroll
+    for (int m = 0; m < 27; ++m) {
+      const int a = h27_pos(m, 0), b = h27_pos(m, 1), cc = h27_pos(m, 2);
+      const float dN0 = D1[0][a] * P1[1][b] * P1[2][cc];
+      const float dN1 = P1[0][a] * D1[1][b] * P1[2][cc];
+      const float dN2 = P1[0][a] * P1[1][b] * D1[2][cc];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float v = src[m * 3 + d];
+        acc[d][0] += v * dN0;
+        acc[d][1] += v * dN1;
+        acc[d][2] += v * dN2;
+      }
+    }
+    float Jm[3][3], Hr[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const float other = __shfl_xor(acc[d][e], 32, 64);
+        Jm[d][e] = h ? other : acc[d][e];
+        Hr[d][e] = h ? acc[d][e] : other;
+      }
+    float inv[3][3], ug[3][3];
+    const float det = inv3(Jm, inv);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ug[i][j] = Hr[i][0] * inv[0][j] + Hr[i][1] * inv[1][j] + Hr[i][2] * inv[2][j];
+    if (n < 27) {
+      if (h == 0) {
+        const float w = det * (float)g3_qweight(q);
+        const Moduli md = moduli(mat, params ? params[c * 27 + q] : 1.0, params != nullptr);
+        material_tangent<float, MAT>(ug, (float)md.m1, (float)md.m2, w, &S.C[q][0]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) S.Inv[q][e * 3 + l] = inv[e][l];
+      }
+    }
+  }
+  h27_wave_sync();
+
+  // this lane's node's physical gradients at its half's points: Gl[t][L] = G[t + 14h][n][L]
+  float Gl[kH27T][3];
+  {
+    const int nn = n < 27 ? n : 0;
+    float lv[3][3], ld[3][3];  // [direction][Gauss point] for the node's lattice position
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int a = c_h27_node[nn][d];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        lv[d][p] = lag2<float>(a, (float)g3_point(p));
+        ld[d][p] = dlag2<float>(a, (float)g3_point(p));
+      }
+    }
+    const bool node_ok = n < 27;
+#pragma unroll
+    for (int t = 0; t < kH27T; ++t) {
+      // the point index of each half is a compile-time constant: select between them
+      const int q0 = t, q1 = t + kH27T < 27 ? t + kH27T : 26;
+      const float fx = h ? lv[0][q1 / 9] : lv[0][q0 / 9], gx = h ? ld[0][q1 / 9] : ld[0][q0 / 9];
+      const float fy = h ? lv[1][(q1 / 3) % 3] : lv[1][(q0 / 3) % 3];
+      const float gy = h ? ld[1][(q1 / 3) % 3] : ld[1][(q0 / 3) % 3];
+      const float fz = h ? lv[2][q1 % 3] : lv[2][q0 % 3], gz = h ? ld[2][q1 % 3] : ld[2][q0 % 3];
+      const float dN0 = gx * fy * fz, dN1 = fx * gy * fz, dN2 = fx * fy * gz;
+      const int q = h ? q1 : q0;
+      const bool ok = node_ok && (t + kH27T * h) < 27;
+      const float* iv = &S.Inv[q][0];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        const float g = dN0 * iv[l] + dN1 * iv[3 + l] + dN2 * iv[6 + l];
+        Gl[t][l] = ok ? g : 0.f;
+      }
+    }
+  }
+
+  float* o = out + c * (27 * 27 * 9);
+  const float* Cbase = &S.C[kH27T * h][0];
+#pragma unroll 1
+  for (int ik = 0; ik < 9; ++ik) {
+    const int i = ik / 3, k = ik - 3 * (ik / 3);
+    const float* Cik = Cbase + i * 27 + k * 3;   // + t * 81 + J * 9 + L
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+#pragma unroll
+    for (int s2 = 0; s2 < 3 * kH27T; ++s2) {
+      const int t = s2 / 3, J = s2 - 3 * (s2 / 3);
+      const float* cq = Cik + t * 81 + J * 9;
+      const float bv = cq[0] * Gl[t][0] + cq[1] * Gl[t][1] + cq[2] * Gl[t][2];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Gl[t][J], bv, acc, 0, 0, 0);
+    }
+    // D[m][n]: lane holds column n, rows m = 8 (r / 4) + 4 h + r % 4
+    if (n < 27) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = 8 * (r / 4) + 4 * h + (r % 4);
+        if (m < 27) o[((m * 27 + n) * 3 + i) * 3 + k] = acc[r];
+      }
+    }
   }
 }
 
@@ -281,10 +445,18 @@ template <typename T, bool MFMA>
 static int launch_hex27_jacobian(const hf_material* m, const double* p, const int32_t* c, int64_t n,
                                  const double* u, const double* prm, T* o, hipStream_t s) {
   if (n == 0) return 0;
-  if (m->kind == MAT_LINEAR_ELASTIC)
-    hex27_jacobian_kernel<T, MAT_LINEAR_ELASTIC, MFMA><<<(unsigned)n, 256, 0, s>>>(p, c, n, u, prm, *m, o);
-  else
-    hex27_jacobian_kernel<T, MAT_NEO_HOOKEAN, MFMA><<<(unsigned)n, 256, 0, s>>>(p, c, n, u, prm, *m, o);
+  if constexpr (MFMA) {
+    const unsigned grid = (unsigned)ceil_div(n, kH27Waves);
+    if (m->kind == MAT_LINEAR_ELASTIC)
+      hex27_jacobian_mfma<MAT_LINEAR_ELASTIC><<<grid, 64 * kH27Waves, 0, s>>>(p, c, n, u, prm, *m, o);
+    else
+      hex27_jacobian_mfma<MAT_NEO_HOOKEAN><<<grid, 64 * kH27Waves, 0, s>>>(p, c, n, u, prm, *m, o);
+  } else {
+    if (m->kind == MAT_LINEAR_ELASTIC)
+      hex27_jacobian_kernel<T, MAT_LINEAR_ELASTIC><<<(unsigned)n, 256, 0, s>>>(p, c, n, u, prm, *m, o);
+    else
+      hex27_jacobian_kernel<T, MAT_NEO_HOOKEAN><<<(unsigned)n, 256, 0, s>>>(p, c, n, u, prm, *m, o);
+  }
   HF_LAUNCH_CHECK();
   return 0;
 }

@@ -1,40 +1,76 @@
 """Set-up side of a finite-element variable (host, once per mesh / load step).
 
-Mirrors ``jax_fem.fe.FiniteElement`` (fe.py:20-322) for the tensor-product
-Lagrange elements HEX8, HEX27 and QUAD4: reference tables on the unit cell
-[0,1]^dim (basis.py:141-250), Dirichlet index/value extraction
-(fe.py:220-269), boundary-face selection (fe.py:271-322), and the face /
-volume load vectors of u-independent surface and mass maps
-(problem.py:216-259).  None of this is on the solve path: the per-iteration
-work (cell kernels, assembly, Krylov) runs in libhipfem.
+Mirrors ``jax_fem.fe.FiniteElement`` (fe.py:20-322) for the Lagrange elements
+HEX8, HEX27, QUAD4 (tensor products on [0,1]^dim) and TET4, TET10, TRI3, TRI6
+(barycentric, on the unit simplex): reference tables (basis.py:141-250),
+Dirichlet index/value extraction (fe.py:220-269), boundary-face selection
+(fe.py:271-322), and the face / volume load vectors of u-independent surface
+and mass maps (problem.py:216-259).  None of this is on the solve path: the
+per-iteration work (cell kernels, assembly, Krylov) runs in libhipfem.
+
+Quadrature as basix's default rule for each cell (basis.py:154-160): Gauss
+(Gauss-Jacobi = Gauss-Legendre) with (order+2)//2 points per direction on
+intervals, quadrilaterals and hexahedra, first coordinate slowest; on simplices
+the symmetric rules basix takes from Xiao-Gimbutas for degrees <= 2 (1 point,
+3 / 4 points at the closed-form positions, equal weights) and, for higher
+orders, collapsed Gauss-Jacobi (Stroud) rules of the same polynomial exactness
+(basix uses Xiao-Gimbutas there too: different points, the same integrals of
+polynomials up to the order).
 """
+from collections import namedtuple
+
 import numpy as np
 
 from .mesh import HEX8_OFFSETS, HEX27_OFFSETS, QUAD4_OFFSETS
 
+Element = namedtuple("Element", "shape nodes degree order")
+
+_TET_VERTS = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
+_TRI_VERTS = np.array([[0, 0], [1, 0], [0, 1]], dtype=np.float64)
+# mid-edge nodes in VTK / meshio order (the reference's re_order maps them onto
+# basix's edge order, basis.py:80, :104)
+_TET10_EDGES = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
+_TRI6_EDGES = [(0, 1), (1, 2), (0, 2)]
+
+
+def _with_edges(verts, edges):
+    return np.vstack([verts] + [0.5 * (verts[a] + verts[b])[None] for a, b in edges])
+
+
 # Nodes are in the mesh's (VTK / meshio) order; the reference maps that order
-# onto basix's with re_order (basis.py:53, :60-61, :86-91), which is the same as
+# onto basix's with re_order (basis.py:53, :60-61, :74-110), which is the same as
 # tabulating in mesh order directly.
 ELEMENTS = {
-    # reference coordinates of the nodes, degree, default quadrature order (basis.py:56, :64, :89)
-    "HEX8": (HEX8_OFFSETS / 1.0, 1, 2),
-    "HEX27": (HEX27_OFFSETS / 2.0, 2, 10),
-    "QUAD4": (QUAD4_OFFSETS / 1.0, 1, 2),
+    # shape, reference coordinates of the nodes, degree, default quadrature order
+    # (basis.py:56, :64, :77, :83, :89, :101, :107)
+    "HEX8": Element("cube", HEX8_OFFSETS / 1.0, 1, 2),
+    "HEX27": Element("cube", HEX27_OFFSETS / 2.0, 2, 10),
+    "QUAD4": Element("cube", QUAD4_OFFSETS / 1.0, 1, 2),
+    "TET4": Element("simplex", _TET_VERTS, 1, 0),
+    "TET10": Element("simplex", _with_edges(_TET_VERTS, _TET10_EDGES), 2, 2),
+    "TRI3": Element("simplex", _TRI_VERTS, 1, 0),
+    "TRI6": Element("simplex", _with_edges(_TRI_VERTS, _TRI6_EDGES), 2, 2),
 }
-# basix facets as mesh-order corner nodes (in basix's vertex order) and their
+# basix facets as mesh-order vertex lists (in basix's vertex order) and their
 # outward normals.  Hexahedron: z=0, y=0, x=0, x=1, y=1, z=1; quadrilateral:
-# y=0, x=0, x=1, y=1.
+# y=0, x=0, x=1, y=1; simplices: facet i opposite vertex i.
+_S3 = 1.0 / np.sqrt(3.0)
+_S2 = 1.0 / np.sqrt(2.0)
 FACETS = {
-    3: (np.array([[0, 1, 3, 2], [0, 1, 4, 5], [0, 3, 4, 7], [1, 2, 5, 6], [3, 2, 7, 6], [4, 5, 7, 6]]),
-        np.array([[0, 0, -1], [0, -1, 0], [-1, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)),
-    2: (np.array([[0, 1], [0, 3], [1, 2], [3, 2]]),
-        np.array([[0, -1], [-1, 0], [1, 0], [0, 1]], dtype=np.float64)),
+    ("cube", 3): (np.array([[0, 1, 3, 2], [0, 1, 4, 5], [0, 3, 4, 7], [1, 2, 5, 6], [3, 2, 7, 6], [4, 5, 7, 6]]),
+                  np.array([[0, 0, -1], [0, -1, 0], [-1, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)),
+    ("cube", 2): (np.array([[0, 1], [0, 3], [1, 2], [3, 2]]),
+                  np.array([[0, -1], [-1, 0], [1, 0], [0, 1]], dtype=np.float64)),
+    ("simplex", 3): (np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]]),
+                     np.array([[_S3, _S3, _S3], [-1, 0, 0], [0, -1, 0], [0, 0, -1]], dtype=np.float64)),
+    ("simplex", 2): (np.array([[1, 2], [0, 2], [0, 1]]),
+                     np.array([[_S2, _S2], [-1, 0], [0, -1]], dtype=np.float64)),
 }
-FACES, FACE_NORMALS = FACETS[3]
+FACES, FACE_NORMALS = FACETS[("cube", 3)]
 
 
 def element_dim(ele_type):
-    return ELEMENTS[ele_type][0].shape[1]
+    return ELEMENTS[ele_type].nodes.shape[1]
 
 
 def gauss_points(order):
@@ -42,6 +78,39 @@ def gauss_points(order):
     m = (order + 2) // 2
     t, w = np.polynomial.legendre.leggauss(m)
     return (t + 1.0) / 2.0, w / 2.0
+
+
+def _gauss_jacobi01(n, alpha):
+    """n-point Gauss-Jacobi rule on [0,1] for the weight (1-t)^alpha."""
+    from scipy.special import roots_jacobi
+    x, w = roots_jacobi(n, alpha, 0.0)
+    return (x + 1.0) / 2.0, w / 2.0 ** (alpha + 1)
+
+
+def simplex_rule(order, dim):
+    """Quadrature on the unit triangle / tetrahedron (points (n, dim), weights summing
+    to the volume 1/2 or 1/6)."""
+    if order <= 1:
+        return np.full((1, dim), 1.0 / (dim + 1)), np.array([1.0 / (2 if dim == 2 else 6)])
+    if order == 2 and dim == 2:
+        a, b = 1.0 / 6.0, 2.0 / 3.0
+        return np.array([[a, a], [a, b], [b, a]]), np.full(3, 1.0 / 6.0)
+    if order == 2 and dim == 3:
+        a, b = (5.0 - np.sqrt(5.0)) / 20.0, (5.0 + 3.0 * np.sqrt(5.0)) / 20.0
+        return np.array([[b, a, a], [a, a, a], [a, a, b], [a, b, a]]), np.full(4, 1.0 / 24.0)
+    n = (order + 2) // 2
+    if dim == 2:
+        x, wx = _gauss_jacobi01(n, 1.0)
+        y, wy = _gauss_jacobi01(n, 0.0)
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        P = np.stack([X, Y * (1 - X)], -1).reshape(-1, 2)
+        return P, np.outer(wx, wy).reshape(-1)
+    x, wx = _gauss_jacobi01(n, 2.0)
+    y, wy = _gauss_jacobi01(n, 1.0)
+    z, wz = _gauss_jacobi01(n, 0.0)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    P = np.stack([X, Y * (1 - X), Z * (1 - X) * (1 - Y)], -1).reshape(-1, 3)
+    return P, np.einsum("i,j,k->ijk", wx, wy, wz).reshape(-1)
 
 
 def lagrange(deg, t):
@@ -54,10 +123,31 @@ def lagrange(deg, t):
     return v, d
 
 
+def _tabulate_simplex(el, X):
+    d = el.nodes.shape[1]
+    lam = np.hstack([1.0 - X.sum(axis=1, keepdims=True), X])                 # (np, d+1)
+    dlam = np.vstack([-np.ones((1, d)), np.eye(d)])                          # (d+1, d)
+    nv = d + 1
+    if el.degree == 1:
+        return lam, np.broadcast_to(dlam[None], (len(X), nv, d)).copy()
+    edges = _TET10_EDGES if d == 3 else _TRI6_EDGES
+    vals = [lam[:, i] * (2 * lam[:, i] - 1) for i in range(nv)]
+    grads = [(4 * lam[:, i] - 1)[:, None] * dlam[i][None] for i in range(nv)]
+    for a, b in edges:
+        vals.append(4 * lam[:, a] * lam[:, b])
+        grads.append(4 * (lam[:, b][:, None] * dlam[a][None] + lam[:, a][:, None] * dlam[b][None]))
+    return np.stack(vals, axis=1), np.stack(grads, axis=1)
+
+
 def tabulate(ele_type, X):
     """Shape values (np, nn) and reference gradients (np, nn, dim) at points X (np, dim):
-    products of 1-D Lagrange factors, one per coordinate."""
-    nodes, deg, _ = ELEMENTS[ele_type]
+    products of 1-D Lagrange factors (cube cells) or barycentric Lagrange
+    polynomials (simplices)."""
+    el = ELEMENTS[ele_type]
+    X = np.asarray(X, dtype=np.float64)
+    if el.shape == "simplex":
+        return _tabulate_simplex(el, X)
+    nodes, deg = el.nodes, el.degree
     d = nodes.shape[1]
     ijk = np.rint(nodes * deg).astype(int)
     V, D = zip(*[lagrange(deg, X[:, c]) for c in range(d)])
@@ -69,35 +159,44 @@ def tabulate(ele_type, X):
     return vals, grads
 
 
-def volume_rule(order, dim=3):
-    """Tensor Gauss rule on [0,1]^dim, first coordinate slowest (basix's order)."""
+def volume_rule(order, dim=3, shape="cube"):
+    """The cell's quadrature rule: tensor Gauss on [0,1]^dim, first coordinate slowest
+    (basix's order), or the simplex rule."""
+    if shape == "simplex":
+        return simplex_rule(order, dim)
     t, w = gauss_points(order)
     P = np.stack(np.meshgrid(*([t] * dim), indexing="ij"), -1).reshape(-1, dim)
     W = np.prod(np.stack(np.meshgrid(*([w] * dim), indexing="ij"), -1).reshape(-1, dim), axis=-1)
     return P, W
 
 
-def face_rule(order, dim=3):
+def face_rule(order, dim=3, shape="cube"):
     """Facet quadrature points (nfacet, nfq, dim) and weights (nfacet, nfq): the
-    (dim-1)-dimensional Gauss rule mapped onto each facet from its first vertex
-    along the edges to its second (and third) vertex."""
-    t, w = gauss_points(order)
-    if dim == 3:
+    facet cell's rule (interval, quadrilateral or triangle) mapped affinely from the
+    facet's first vertex along its edges to the second (and third) vertex, the
+    weights scaled by the map's length / area factor (basis.py:205-231)."""
+    if dim == 2:
+        t, w = gauss_points(order)
+        s, ws = t[:, None], w
+    elif shape == "cube":
+        t, w = gauss_points(order)
         s = np.stack(np.meshgrid(t, t, indexing="ij"), -1).reshape(-1, 2)
         ws = np.outer(w, w).reshape(-1)
-        corners = HEX8_OFFSETS.astype(np.float64)
     else:
-        s, ws = t[:, None], w
-        corners = QUAD4_OFFSETS.astype(np.float64)
-    facets = FACETS[dim][0]
-    pts = []
+        s, ws = simplex_rule(order, 2)
+    if shape == "cube":
+        corners = (HEX8_OFFSETS if dim == 3 else QUAD4_OFFSETS).astype(np.float64)
+    else:
+        corners = _TET_VERTS if dim == 3 else _TRI_VERTS
+    facets = FACETS[(shape, dim)][0]
+    pts, wts = [], []
     for f in facets:
         v0 = corners[f[0]]
-        p = np.repeat(v0[None], len(s), axis=0)
-        for a in range(dim - 1):
-            p = p + s[:, a:a + 1] * (corners[f[a + 1]] - v0)[None]
-        pts.append(p)
-    return np.stack(pts), np.tile(ws, (len(facets), 1))
+        E = np.stack([corners[f[a + 1]] - v0 for a in range(dim - 1)], axis=1)   # (dim, dim-1)
+        pts.append(v0[None] + s @ E.T)
+        scale = np.linalg.norm(E[:, 0]) if dim == 2 else np.linalg.norm(np.cross(E[:, 0], E[:, 1]))
+        wts.append(ws * scale)
+    return np.stack(pts), np.stack(wts)
 
 
 def _eval_points(fn, pts, extra=()):
@@ -186,29 +285,30 @@ class FiniteElement:
     def __init__(self, mesh, vec, dim, ele_type="HEX8", quadrature_rule=None, quadrature_order=None,
                  dirichlet_bc_info=None):
         if ele_type not in ELEMENTS:
-            raise NotImplementedError("element %s (HEX8, HEX27 and QUAD4 are supported)" % ele_type)
+            raise NotImplementedError("element %s (supported: %s)" % (ele_type, ", ".join(ELEMENTS)))
         if element_dim(ele_type) != dim:
             raise ValueError("element %s is %d-dimensional, dim=%d" % (ele_type, element_dim(ele_type), dim))
-        if quadrature_rule not in (None, "default", "GLL") and quadrature_rule is not None:
-            raise NotImplementedError("quadrature_rule %r" % (quadrature_rule,))
+        if quadrature_rule not in (None, "default"):
+            raise NotImplementedError("quadrature_rule %r: only basix's default rule is built" % (quadrature_rule,))
         self.mesh, self.vec, self.dim, self.ele_type = mesh, vec, dim, ele_type
         self.points = np.asarray(mesh.points, dtype=np.float64)
         self.cells = np.asarray(mesh.cells, dtype=np.int64)
         self.num_cells = len(self.cells)
         self.num_total_nodes = len(self.points)
         self.num_total_dofs = self.num_total_nodes * vec
-        order = ELEMENTS[ele_type][2] if quadrature_order is None else quadrature_order
+        order = ELEMENTS[ele_type].order if quadrature_order is None else quadrature_order
         self.quadrature_order = order
-        qp, qw = volume_rule(order, dim)
+        shape = ELEMENTS[ele_type].shape
+        qp, qw = volume_rule(order, dim, shape)
         sv, sg = tabulate(ele_type, qp)
-        fp, fw = face_rule(order, dim)
+        fp, fw = face_rule(order, dim, shape)
         nf, nfq, _ = fp.shape
         fv, fg = tabulate(ele_type, fp.reshape(-1, dim))
         self.quad_weights, self.shape_vals, self.shape_grads_ref = _field(qw), _field(sv), _field(sg)
         self.face_quad_weights = _field(fw)
         self.face_shape_vals = _field(fv.reshape(nf, nfq, -1))
         self.face_shape_grads_ref = _field(fg.reshape(nf, nfq, -1, dim))
-        self.face_inds, normals = FACETS[dim]
+        self.face_inds, normals = FACETS[(shape, dim)]
         self.face_normals = _field(normals)
         self.num_quads, self.num_nodes = self.shape_vals.shape
         self.num_faces, self.num_face_quads = nf, nfq

@@ -4,7 +4,9 @@
 (ncell, nn)).  ``rectangle_mesh`` reproduces generate_mesh.py:120-148 (node
 index i*(Ny+1)+j, QUAD4 corners counter-clockwise).  ``box_mesh`` reproduces generate_mesh.py:151-189 (node index
 (i*(Ny+1)+j)*(Nz+1)+k, HEX8 corners in VTK order) and returns a ``Mesh``
-directly (the reference wraps it in a meshio object).  ``box_mesh_hex27`` is
+directly (the reference wraps it in a meshio object).  ``box_mesh_tet`` /
+``rectangle_mesh_tri`` / ``box_mesh_gmsh`` are structured simplex and gmsh-free
+counterparts of the reference's gmsh boxes; ``box_mesh_hex27`` is
 the structured HEX27 counterpart (the reference only has a gmsh path,
 generate_mesh.py:192-262): nodes on the doubled lattice, cell nodes in the
 VTK triquadratic order that the reference's HEX27 re_order maps to basix
@@ -27,7 +29,7 @@ QUAD4_OFFSETS = np.array([[0, 0], [1, 0], [1, 1], [0, 1]])
 
 _CELL_TYPES = {"HEX8": "hexahedron", "HEX27": "hexahedron27", "HEX20": "hexahedron20", "TET4": "tetra",
                "TET10": "tetra10", "QUAD4": "quad", "QUAD8": "quad8", "TRI3": "triangle", "TRI6": "triangle6"}
-_NN_TO_TYPE = {8: "HEX8", 27: "HEX27", 20: "HEX20", 10: "TET10"}
+_NN_TO_TYPE = {8: "HEX8", 27: "HEX27", 20: "HEX20", 10: "TET10", 3: "TRI3"}
 
 
 class Mesh:
@@ -52,8 +54,8 @@ class Mesh:
         cd = self.__dict__.get("_cells_dict")
         if cd is not None:
             return cd
-        e = self.ele_type or _NN_TO_TYPE.get(self.cells.shape[1], "TET4" if self.cells.shape[1] == 4 and
-                                             self.points.shape[1] == 3 else "QUAD4")
+        nn, d = self.cells.shape[1], self.points.shape[1]
+        e = self.ele_type or {4: "TET4" if d == 3 else "QUAD4", 6: "TRI6"}.get(nn) or _NN_TO_TYPE[nn]
         return {_CELL_TYPES[e]: self.cells}
 
     @property
@@ -135,6 +137,86 @@ def box_mesh_hex27(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0):
     mesh.node_grid = (2 * Nx + 1, 2 * Ny + 1, 2 * Nz + 1)   # node n = (i*(2Ny+1)+j)*(2Nz+1)+k
     mesh.cell_grid = (Nx, Ny, Nz)
     return mesh
+
+
+def _freudenthal():
+    """The 6 tetrahedra of a unit cube along its (0,0,0)-(1,1,1) diagonal, one per
+    axis order, each positively oriented, as lists of 4 corner offsets."""
+    import itertools
+    tets = []
+    for perm in itertools.permutations(range(3)):
+        p = [np.zeros(3, dtype=int)]
+        for ax in perm:
+            q = p[-1].copy()
+            q[ax] = 1
+            p.append(q)
+        if np.linalg.det(np.stack([p[1] - p[0], p[2] - p[0], p[3] - p[0]])) < 0:
+            p[1], p[2] = p[2], p[1]
+        tets.append(p)
+    return tets
+
+
+def box_mesh_tet(Nx, Ny, Nz, domain_x=1.0, domain_y=1.0, domain_z=1.0, ele_type="TET4"):
+    """Structured TET4 / TET10 box: every cell of the Nx x Ny x Nz lattice split into 6
+    tetrahedra along its main diagonal (a conforming Freudenthal triangulation),
+    vertices and, for TET10, mid-edge nodes (VTK order) on the doubled lattice.
+    The reference builds these boxes with gmsh (generate_mesh.py:192-262); this is a
+    structured equivalent (same domain and element type, different triangulation)."""
+    if ele_type not in ("TET4", "TET10"):
+        raise ValueError(ele_type)
+    f = 1 if ele_type == "TET4" else 2
+    X, Y, Z = np.meshgrid(_lattice(f * Nx, domain_x), _lattice(f * Ny, domain_y), _lattice(f * Nz, domain_z),
+                          indexing="ij")
+    points = np.stack((X, Y, Z), axis=-1).reshape(-1, 3)
+    idx = np.arange(len(points)).reshape(f * Nx + 1, f * Ny + 1, f * Nz + 1)
+    i, j, k = [a.reshape(-1) for a in np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")]
+    cells = []
+    for p in _freudenthal():
+        verts = [f * np.stack([i + v[0], j + v[1], k + v[2]], -1) for v in p]     # doubled-lattice coords
+        nodes = list(verts)
+        if f == 2:
+            for a, b in ((0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)):
+                nodes.append((verts[a] + verts[b]) // 2)
+        cells.append(np.stack([idx[n[:, 0], n[:, 1], n[:, 2]] for n in nodes], axis=1))
+    cells = np.stack(cells, axis=1).reshape(-1, 4 if f == 1 else 10)
+    return Mesh(points, cells.astype(np.int64), ele_type)
+
+
+def rectangle_mesh_tri(Nx, Ny, domain_x=1.0, domain_y=1.0, ele_type="TRI3"):
+    """Structured TRI3 / TRI6 rectangle: every cell split along its (0,0)-(1,1)
+    diagonal into two counter-clockwise triangles; TRI6 mid-edge nodes (VTK order)
+    on the doubled lattice."""
+    if ele_type not in ("TRI3", "TRI6"):
+        raise ValueError(ele_type)
+    f = 1 if ele_type == "TRI3" else 2
+    X, Y = np.meshgrid(_lattice(f * Nx, domain_x), _lattice(f * Ny, domain_y), indexing="ij")
+    points = np.stack((X, Y), axis=-1).reshape(-1, 2)
+    idx = np.arange(len(points)).reshape(f * Nx + 1, f * Ny + 1)
+    i, j = [a.reshape(-1) for a in np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")]
+    cells = []
+    for tri in (((0, 0), (1, 0), (1, 1)), ((0, 0), (1, 1), (0, 1))):
+        verts = [f * np.stack([i + v[0], j + v[1]], -1) for v in tri]
+        nodes = list(verts)
+        if f == 2:
+            for a, b in ((0, 1), (1, 2), (0, 2)):
+                nodes.append((verts[a] + verts[b]) // 2)
+        cells.append(np.stack([idx[n[:, 0], n[:, 1]] for n in nodes], axis=1))
+    cells = np.stack(cells, axis=1).reshape(-1, 3 if f == 1 else 6)
+    return Mesh(points, cells.astype(np.int64), ele_type)
+
+
+def box_mesh_gmsh(Nx, Ny, Nz, domain_x, domain_y, domain_z, data_dir=None, ele_type="HEX8"):
+    """The reference's box_mesh_gmsh (generate_mesh.py:192-262) without gmsh: the
+    structured box of the element type (HEX8 box_mesh, HEX27 box_mesh_hex27,
+    TET4/TET10 box_mesh_tet), returned as a Mesh with meshio's cells_dict; nothing
+    is written to data_dir."""
+    if ele_type == "HEX8":
+        return box_mesh(Nx, Ny, Nz, domain_x, domain_y, domain_z)
+    if ele_type == "HEX27":
+        return box_mesh_hex27(Nx, Ny, Nz, domain_x, domain_y, domain_z)
+    if ele_type in ("TET4", "TET10"):
+        return box_mesh_tet(Nx, Ny, Nz, domain_x, domain_y, domain_z, ele_type)
+    raise NotImplementedError("box_mesh_gmsh: element %s" % ele_type)
 
 
 def get_meshio_cell_type(ele_type):

@@ -441,8 +441,12 @@ class Problem:
         have = False
         if plan["mass"] == "load":
             mmap = self.get_mass_map()
-            x = fe.get_physical_quad_points().reshape(-1, self.dim)
-            m = _eval_points(lambda xx: mmap(np.zeros(self.vec), xx), x).reshape(self.num_cells, fe.num_quads, -1)
+            x = np.asarray(fe.get_physical_quad_points()).reshape(-1, self.dim)
+            m = self._eval_device(lambda xx: mmap(torch.zeros(self.vec, dtype=torch.float64, device=xx.device), xx),
+                                  lambda xx: mmap(np.zeros(self.vec), xx), x)
+            if m is None:
+                m = _eval_points(lambda xx: mmap(np.zeros(self.vec), xx), x)
+            m = m.reshape(self.num_cells, fe.num_quads, -1)
             r = np.einsum("cqi,qn,cq->cni", m, fe.shape_vals, fe.JxW)
             np.add.at(f, fe.cells.reshape(-1), r.reshape(-1, self.vec))
             have = True
@@ -464,6 +468,29 @@ class Problem:
         self._fext = torch.as_tensor(f.reshape(-1), dtype=torch.float64, device=self.device) if have else None
         self._ext_key = key
         return self._fext
+
+    def _eval_device(self, fn_t, fn_h, x, extra=()):
+        """fn at every point, vmapped on the device (maps written with
+        jax_fem_amd.numpy and its transforms), checked against host evaluations at a
+        sample of points (1e-12 relative); None when the map does not run on tensors."""
+        if self.device.type != "cuda" or len(x) == 0:
+            return None
+        from . import numpy as jnp
+        from .generic import _vmap_chunks
+        try:
+            args = [torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=self.device)
+                    for a in [x] + list(extra)]
+            with jnp.on_device(self.device):
+                out = _vmap_chunks(fn_t, args)
+            out = out.detach().cpu().numpy()
+            for i in sorted({(len(x) - 1) * k // 6 for k in range(7)}):
+                ref = np.asarray(fn_h(x[i], *[np.asarray(e)[i] for e in extra]), dtype=np.float64)
+                if out[i].shape != ref.shape or not np.allclose(out[i], ref, rtol=1e-12,
+                                                                atol=1e-12 * max(1.0, float(np.abs(ref).max()))):
+                    return None
+            return np.ascontiguousarray(out)
+        except Exception:
+            return None
 
     # -------------------------------------------------------------- engines
     def _element_terms(self, u, jac, dtype=torch.float64):
