@@ -410,7 +410,8 @@ def solve_case(jfa, prob, bc, opts, nload, dmax):
 
 
 def hex27_jacobian_mfma(jfa, prob):
-    """HIP-event time of one FP32 HEX27 element-Jacobian launch (MFMA) at u = 0."""
+    """HIP-event time of one FP32 HEX27 element-Jacobian launch (MFMA) at u = 0,
+    averaged over 5 launches after a warm one."""
     import ctypes
     import torch
     pts, cells = prob._dev()
@@ -434,10 +435,12 @@ def hex27_jacobian_mfma(jfa, prob):
     ms = e0.elapsed_time(e1) / reps
     flop_cell = 2 * 3 * 81 * 81 * 27 + 2 * 3 * 81 * 81 * 3   # GEMM + forming M_k
     tf = prob.num_cells * flop_cell / (ms * 1e-3) / 1e12
-    return {"kernel": "hex27_jacobian_kernel<float, NEO_HOOKEAN, MFMA>", "avg_ms": ms, "flop_per_cell": flop_cell,
+    return {"kernel": "hex27_jacobian_mfma<NEO_HOOKEAN> (wavefront per cell, 9 GEMMs 27x27x81 on "
+                      "v_mfma_f32_32x32x2f32)", "avg_ms": ms, "flop_per_cell": flop_cell,
             "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
             "output_GBps": prob.num_cells * 6561 * 4 / (ms * 1e-3) / 1e9,
-            "note": "flop count of the factored form used (K_k = M_k G per k); the dense B^T D B form counts "
+            "note": "flop count of the algorithmic factored form (per (i,k): V = C G, K = G^T V; 1,548,288 flop "
+                    "per cell executed on MFMA with the 27 -> 32 padding); the dense B^T D B form counts "
                     "3,542,940 flop/cell"}
 
 
@@ -609,6 +612,8 @@ def main():
     ap.add_argument("--fd-cells", type=int, default=0,
                     help="SIMP and HEX27 workloads: cells whose adjoint gradient is checked by central FD")
     ap.add_argument("--no-adjoint", action="store_true", help="HEX27 workload: skip the adjoint legs")
+    ap.add_argument("--jacobian-only", action="store_true",
+                    help="HEX27 workload: time the FP32 element-Jacobian kernel alone (for profiling) and exit")
     ap.add_argument("--cpu-only", choices=["full", "sample"], default=None,
                     help="time only the CPU oracle of the workload (no GPU)")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -658,6 +663,11 @@ def main():
     full_size = args.n is None
     N = prob.N
     log("problem: %s, %d DOF" % (desc, N))
+    if args.jacobian_only:
+        if wl != "neohookean-hex27-48":
+            raise SystemExit("--jacobian-only applies to the HEX27 workload")
+        print(json.dumps(hex27_jacobian_mfma(jfa, prob)))
+        return
     t0 = time.perf_counter()
     _ = prob.pattern
     torch.cuda.synchronize()

@@ -244,7 +244,7 @@ __device__ __forceinline__ void h27_wave_sync() {
 }
 
 template <int MAT>
-__global__ void __launch_bounds__(64 * kH27Waves) hex27_jacobian_mfma(const double* __restrict__ pts,
+__global__ void __launch_bounds__(64 * kH27Waves, 2) hex27_jacobian_mfma(const double* __restrict__ pts,
                                                                       const int32_t* __restrict__ cells,
                                                                       int64_t ncell, const double* __restrict__ u,
                                                                       const double* __restrict__ params,
@@ -366,28 +366,35 @@ __global__ void __launch_bounds__(64 * kH27Waves) hex27_jacobian_mfma(const doub
     }
   }
 
-  float* o = out + c * (27 * 27 * 9);
+  // all nine (i, k) blocks stay in accumulators, so that each 3x3 block (a, b) of
+  // the output is written at once: 9 consecutive words per lane, 27 lanes -> 972
+  // contiguous bytes per row a (full cache lines while they are hot in L2)
   const float* Cbase = &S.C[kH27T * h][0];
-#pragma unroll 1
+  f32x16 acc[9];
+#pragma unroll
   for (int ik = 0; ik < 9; ++ik) {
     const int i = ik / 3, k = ik - 3 * (ik / 3);
     const float* Cik = Cbase + i * 27 + k * 3;   // + t * 81 + J * 9 + L
-    f32x16 acc;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    for (int r = 0; r < 16; ++r) acc[ik][r] = 0.f;
 #pragma unroll
     for (int s2 = 0; s2 < 3 * kH27T; ++s2) {
       const int t = s2 / 3, J = s2 - 3 * (s2 / 3);
       const float* cq = Cik + t * 81 + J * 9;
       const float bv = cq[0] * Gl[t][0] + cq[1] * Gl[t][1] + cq[2] * Gl[t][2];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Gl[t][J], bv, acc, 0, 0, 0);
+      acc[ik] = __builtin_amdgcn_mfma_f32_32x32x2f32(Gl[t][J], bv, acc[ik], 0, 0, 0);
     }
-    // D[m][n]: lane holds column n, rows m = 8 (r / 4) + 4 h + r % 4
-    if (n < 27) {
+  }
+  // D[m][n]: lane holds column n, rows m = 8 (r / 4) + 4 h + r % 4
+  if (n < 27) {
+    float* o = out + c * (27 * 27 * 9) + (int64_t)n * 9;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int m = 8 * (r / 4) + 4 * h + (r % 4);
-        if (m < 27) o[((m * 27 + n) * 3 + i) * 3 + k] = acc[r];
+    for (int r = 0; r < 16; ++r) {
+      const int m = 8 * (r / 4) + 4 * h + (r % 4);
+      if (m < 27) {
+        float* blk = o + m * (27 * 9);
+#pragma unroll
+        for (int ik = 0; ik < 9; ++ik) blk[ik] = acc[ik][r];
       }
     }
   }
