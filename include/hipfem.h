@@ -172,7 +172,9 @@ int hf_spmv_f32(const hf_pattern* p, const float* vals, const float* x, float* y
 /* y = A^T x through the transposed-block map (PETSc A.transpose, solver.py:1369-1370) */
 int hf_spmv_t_f64(const hf_pattern* p, const double* vals, const double* x, double* y, hf_stream_t stream);
 int hf_spmv_t_f32(const hf_pattern* p, const float* vals, const float* x, float* y, hf_stream_t stream);
-/* inverse of the scalar diagonal (Jacobi M = 1/diag(A), solver.py:68-69) */
+/* inverse of the scalar diagonal (Jacobi M = 1/diag(A), solver.py:68-69); a zero
+ * diagonal entry (the pressure block of a saddle-point system) gives 1 (unscaled),
+ * where the reference's 1/0 would make the preconditioner infinite */
 int hf_jacobi_f64(const hf_pattern* p, const double* vals, double* dinv, hf_stream_t stream);
 int hf_jacobi_f32(const hf_pattern* p, const float* vals, float* dinv, hf_stream_t stream);
 

@@ -23,6 +23,7 @@ from .utils import save_sol, read_vtu, write_vtu, modify_vtu_file
 from . import numpy
 from .autodiff import grad, vmap, jacfwd, jacrev, value_and_grad, hessian
 from . import mma
+from . import flatten_util
 from .learned_pc import LearnedPC
 from .multigrid import Multigrid
 from .solver import (solver, ad_wrapper, implicit_vjp, get_A, linear_solver, apply_bc_vec, assign_bc, copy_bc,
@@ -39,5 +40,5 @@ __all__ = ["HipFemError", "Problem", "solver", "ad_wrapper", "implicit_vjp", "ge
            "assign_bc", "copy_bc", "Mesh", "box_mesh", "box_mesh_hex27", "box_mesh_tet", "box_mesh_gmsh", "rectangle_mesh",
            "rectangle_mesh_tri", "get_meshio_cell_type",
            "FiniteElement", "LinearElastic", "NeoHookean", "Poisson", "Material", "TangentMatrix", "krylov",
-           "LearnedPC", "Multigrid", "numpy", "grad", "vmap", "jacfwd", "jacrev", "value_and_grad", "hessian", "mma",
+           "LearnedPC", "Multigrid", "numpy", "grad", "vmap", "jacfwd", "jacrev", "value_and_grad", "hessian", "mma", "flatten_util",
            "save_sol", "read_vtu", "write_vtu", "modify_vtu_file"]

@@ -439,7 +439,8 @@ __global__ void jacobi_kernel(const int32_t* __restrict__ dpos, int64_t nnode, i
   const int i = (int)(rr - n * vec);
   const int32_t b = dpos[n];
   (void)R;
-  dinv[rr] = b >= 0 ? (T)1 / vals[(int64_t)b + (int64_t)i * vec + i] : (T)1;
+  const T d = b >= 0 ? vals[(int64_t)b + (int64_t)i * vec + i] : (T)0;
+  dinv[rr] = d != (T)0 ? (T)1 / d : (T)1;   // a zero diagonal (saddle-point block) is left unscaled
 }
 
 }  // namespace hf

@@ -43,10 +43,11 @@ class PointwiseTerms:
 
     def __init__(self, problem):
         self.p = problem
-        fe = problem.fe
+        fe = problem.fe          # tensor, mass and surface maps act on the first variable
         dev = problem.device
-        self.nn, self.nq, self.vec, self.dim = fe.num_nodes, fe.num_quads, problem.vec, problem.dim
+        self.nn, self.nq, self.vec, self.dim = fe.num_nodes, fe.num_quads, fe.vec, problem.dim
         self.ncell = problem.num_cells
+        self.ndof0 = fe.num_total_dofs
         grads, jxw = fe.get_shape_grads()
         self.sg = torch.as_tensor(np.ascontiguousarray(grads), dtype=torch.float64, device=dev)
         self.jxw = torch.as_tensor(np.ascontiguousarray(jxw), dtype=torch.float64, device=dev)
@@ -60,8 +61,9 @@ class PointwiseTerms:
 
     # -- quadrature-point data ----------------------------------------------------
     def point_data(self, u, need_uq):
-        """u_grad (ncell*nq, vec, dim) and u at the points (or None)."""
-        _, cells = self.p._dev()
+        """u_grad (ncell*nq, vec, dim) and u at the points (or None) of the first variable."""
+        cells = self.p._var0_cells()
+        u = u[:self.ndof0].contiguous()
         Q = self.ncell * self.nq
         ug = torch.empty(Q * self.vec * self.dim, dtype=torch.float64, device=u.device)
         uq = torch.empty(Q * self.vec, dtype=torch.float64, device=u.device) if need_uq else None
@@ -81,7 +83,7 @@ class PointwiseTerms:
         """Element residuals (ncell, nn*vec) f64 and, with jac, element Jacobians
         block-major (ncell, nn, nn, vec, vec) in dtype from the tensor and mass maps."""
         dev = u.device
-        ivs = self._ivs()
+        ivs = self._ivs() if (self.tensor_fn is not None or self.mass_fn is not None) else []
         need_uq = self.mass_fn is not None
         ug, uq = self.point_data(u, need_uq)
         P = C = M = Mt = None
@@ -124,19 +126,39 @@ class PointwiseTerms:
         finally:
             self.tensor_fn = tf
 
+    def _all_vars_geometry(self):
+        """Shape gradients (ncell, nq, sum nn, dim), JxW (ncell, nvars, nq) and v_grads_JxW
+        (ncell, nq, sum nn, 1, dim) of every variable, concatenated (problem.py:130-147)."""
+        if getattr(self, "_geo_all", None) is None:
+            dev = self.sg.device
+            if self.p.num_vars == 1:
+                sg, jxw = self.sg, self.jxw[:, None, :]
+            else:
+                gs, js = zip(*[fe.get_shape_grads() for fe in self.p.fes])
+                sg = torch.as_tensor(np.ascontiguousarray(np.concatenate(gs, axis=2)), dtype=torch.float64,
+                                     device=dev)
+                jxw = torch.as_tensor(np.ascontiguousarray(np.stack(js, axis=1)), dtype=torch.float64, device=dev)
+            nn_of = np.diff(self.p.num_nodes_cumsum)
+            w = torch.cat([jxw[:, k, :, None, None].expand(-1, -1, int(n), 1) for k, n in enumerate(nn_of)], dim=2)
+            vgj = (sg * w)[:, :, :, None, :]
+            self._geo_all = (sg, jxw, vgj)
+        return self._geo_all
+
+    def _cell_sol_flat(self, u):
+        """(ncell, nd): each cell's DOFs of every variable, in the reference's flat order."""
+        _, dcells = self.p._dev()
+        if self.p.num_vars == 1:
+            return u.reshape(-1, self.vec)[dcells.long()].reshape(self.ncell, -1)
+        return u[dcells.long()]
+
     def universal_terms(self, u, jac):
         """get_universal_kernel's element residuals (ncell, nd) and Jacobians (ncell, nd, nd)."""
         p = self.p
-        _, cells = p._dev()
-        dev = u.device
-        cu = u.reshape(-1, self.vec)[cells.long()].reshape(self.ncell, -1)
-        sgc = self.sg
-        jxw = self.jxw[:, None, :]                          # (ncell, 1, nq): one variable
-        vgj = (sgc * self.jxw[:, :, None, None])[:, :, :, None, :]     # (ncell, nq, nn, 1, dim)
-        ivs = [a for a in p.volume_internal_vars()]
-        args = [cu, self.xq, sgc, jxw, vgj] + ivs
+        cu = self._cell_sol_flat(u)
+        sgc, jxw, vgj = self._all_vars_geometry()
+        args = [cu, self.xq, sgc, jxw, vgj] + p.cell_internal_vars()
         k = self.universal
-        with jnp.on_device(dev):
+        with jnp.on_device(u.device):
             r = torch.func.vmap(k)(*args)
             K = torch.func.vmap(torch.func.jacfwd(k, argnums=0))(*args) if jac else None
         return r, K
@@ -153,12 +175,16 @@ class PointwiseTerms:
             fd = self.faces[k]
             if fd is None:
                 continue
-            cu = u.reshape(-1, self.vec)[fd["cell_nodes"]]          # (nsel, nn, vec)
+            cu = u[:self.ndof0].reshape(-1, self.vec)[fd["cell_nodes"]]          # (nsel, nn, vec)
             isv = fd["isv"]
             with jnp.on_device(dev):
                 if fd["universal"] is not None:
                     kern = fd["universal"]
-                    args = [cu.reshape(cu.shape[0], -1), fd["x"], fd["vals"], fd["grads"], fd["nanson"][:, None, :]] + isv
+                    if "sol_idx" in fd:       # several variables: the cells' flat DOFs, geometry of all
+                        args = [u[fd["sol_idx"]], fd["x"], fd["vals_all"], fd["grads_all"], fd["nanson_all"]] + isv
+                    else:
+                        args = [cu.reshape(cu.shape[0], -1), fd["x"], fd["vals"], fd["grads"],
+                                fd["nanson"][:, None, :]] + isv
                     r = torch.func.vmap(kern)(*args)
                     K = torch.func.vmap(torch.func.jacfwd(kern, argnums=0))(*args) if jac else None
                 else:
@@ -179,6 +205,47 @@ class PointwiseTerms:
                             fd["nsel"], self.nn * self.vec, self.nn * self.vec)
             out.append((fd["cells"], r, K))
         return out
+
+
+    def multi_terms(self, u, jac, dtype):
+        """Element residuals (ncell*nd f64) and Jacobians (ncell*nd*nd, dense per cell:
+        the layout of a DOF-level pattern) of a problem with several variables: the
+        first variable's tensor / mass / surface maps placed in its DOFs of each cell
+        (problem.py:189-259 use cell_sol_list[0]), plus the universal kernels."""
+        p = self.p
+        nd = p._enn
+        nd0 = self.nn * self.vec
+        dev = u.device
+        er = torch.zeros(self.ncell, nd, dtype=torch.float64, device=dev)
+        ej = torch.zeros(self.ncell, nd, nd, dtype=dtype, device=dev) if jac else None
+        if self.tensor_fn is not None or self.mass_fn is not None:
+            r0, k0 = self.cell_terms(u, jac, dtype)
+            er[:, :nd0] += r0.view(self.ncell, nd0)
+            if jac:
+                k0 = k0.view(self.ncell, self.nn, self.nn, self.vec, self.vec).permute(0, 1, 3, 2, 4)
+                ej[:, :nd0, :nd0] += k0.reshape(self.ncell, nd0, nd0)
+        terms = []
+        if self.universal is not None:
+            r, K = self.universal_terms(u, jac)
+            terms.append((None, r, K))
+        faces = [k for k, how in enumerate(p._plan()["surfaces"]) if how == "face"]
+        if faces:
+            for cells, r, K in self.face_terms(u, jac, faces):
+                if r.shape[1] != nd:          # a first-variable surface map: its DOFs
+                    r = torch.nn.functional.pad(r, (0, nd - r.shape[1]))
+                    if K is not None:
+                        K = torch.nn.functional.pad(K, (0, nd - K.shape[2], 0, nd - K.shape[1]))
+                terms.append((cells, r, K))
+        for cells, r, K in terms:
+            if cells is None:
+                er += r.to(er.dtype)
+                if jac and K is not None:
+                    ej += K.to(ej.dtype)
+            else:
+                er.index_add_(0, cells, r.to(er.dtype))
+                if jac and K is not None:
+                    ej.index_add_(0, cells, K.to(ej.dtype))
+        return er.reshape(-1), (ej.reshape(-1) if jac else None)
 
 
 def add_element_terms(er, ej, terms, nn, vec, ncell):

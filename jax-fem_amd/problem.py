@@ -64,9 +64,8 @@ class Problem:
     device: Any = None
 
     def __post_init__(self):
-        if isinstance(self.mesh, list):
-            if len(self.mesh) != 1:
-                raise NotImplementedError("multi-variable problems are not supported by the GPU path")
+        multi = isinstance(self.mesh, list) and len(self.mesh) > 1
+        if isinstance(self.mesh, list) and not multi:
             self.mesh, self.vec, self.ele_type = self.mesh[0], self.vec[0], self.ele_type[0]
             qo = self.quadrature_order
             self.quadrature_order = qo[0] if isinstance(qo, list) else qo
@@ -76,14 +75,31 @@ class Problem:
         if self.device is None:
             self.device = torch.device("cuda", torch.cuda.current_device()) if torch.cuda.is_available() \
                 else torch.device("cpu")
-        self.fes = [FiniteElement(self.mesh, self.vec, self.dim, self.ele_type, self.quadrature_rule,
-                                  self.quadrature_order, self.dirichlet_bc_info)]
+        if multi:
+            # several variables (problem.py:55-128): one FiniteElement each on the same cells
+            nv = len(self.mesh)
+            per = lambda a: a[:] if isinstance(a, list) else [a] * nv
+            self.vec, self.ele_type = per(self.vec), per(self.ele_type)
+            qos, dbcs = per(self.quadrature_order), per(self.dirichlet_bc_info)
+            self.fes = [FiniteElement(self.mesh[i], self.vec[i], self.dim, self.ele_type[i], self.quadrature_rule,
+                                      qos[i], dbcs[i]) for i in range(nv)]
+            if len({fe.num_cells for fe in self.fes}) != 1 or len({fe.num_quads for fe in self.fes}) != 1:
+                raise ValueError("the variables of a problem must share the cells and the quadrature points")
+        else:
+            self.fes = [FiniteElement(self.mesh, self.vec, self.dim, self.ele_type, self.quadrature_rule,
+                                      self.quadrature_order, self.dirichlet_bc_info)]
         self.fe = self.fes[0]
-        self.num_vars = 1
-        self.cells_list = [self.fe.cells]
+        self.num_vars = len(self.fes)
+        self.cells_list = [fe.cells for fe in self.fes]
         self.num_cells = self.fe.num_cells
         self.offset = [0]
-        self.num_total_dofs_all_vars = self.fe.num_total_dofs
+        for fe in self.fes[:-1]:
+            self.offset.append(self.offset[-1] + fe.num_total_dofs)
+        self.num_total_dofs_all_vars = sum(fe.num_total_dofs for fe in self.fes)
+        self.num_nodes_cumsum = np.cumsum([0] + [fe.num_nodes for fe in self.fes])
+        # each cell's DOFs, variable by variable, node-major (the reference's ravel_pytree order)
+        self._dof_cells = np.hstack([(o + fe.vec * fe.cells[:, :, None] + np.arange(fe.vec)[None, None, :])
+                                     .reshape(fe.num_cells, -1) for o, fe in zip(self.offset, self.fes)])
         self.boundary_inds_list = self.fe.get_boundary_conditions_inds(self.location_fns)
         self.internal_vars = ()
         self.internal_vars_surfaces = [() for _ in self.boundary_inds_list]
@@ -94,6 +110,31 @@ class Problem:
         self._bc_struct = None     # node arrays + components the BC DOF set was built from
         self.custom_init(*self.additional_info)
         self._check_hooks()
+
+    # -------------------------------------------------- layout of the unknowns
+    @property
+    def _multi(self):
+        return self.num_vars > 1
+
+    @property
+    def _enn(self):
+        """Entities per cell in the device pattern: nodes, or DOFs for several variables."""
+        return self._dof_cells.shape[1] if self._multi else self.fe.num_nodes
+
+    @property
+    def _evec(self):
+        """Components per pattern entity: vec, or 1 (DOF-level pattern) for several variables."""
+        return 1 if self._multi else self.vec
+
+    def unflatten_fn_dof(self, cell_flat):
+        """A cell's flat DOF vector -> [(num_nodes_i, vec_i), ...] (problem.py:117-118);
+        torch or NumPy, also under vmap."""
+        out, k = [], 0
+        for fe in self.fes:
+            n = fe.num_nodes * fe.vec
+            out.append(cell_flat[..., k:k + n].reshape(tuple(cell_flat.shape[:-1]) + (fe.num_nodes, fe.vec)))
+            k += n
+        return out
 
     # ------------------------------------------------------------------ hooks
     def custom_init(self, *args):
@@ -119,8 +160,8 @@ class Problem:
         if hasattr(self, "get_tensor_map"):
             m = self.get_tensor_map()
             if isinstance(m, Material):
-                if m.vec != self.vec:
-                    raise ValueError("material expects vec=%d, problem has vec=%d" % (m.vec, self.vec))
+                if m.vec != self.fe.vec:
+                    raise ValueError("material expects vec=%d, problem has vec=%d" % (m.vec, self.fe.vec))
                 self.material = m
             elif callable(m):
                 self._tensor_fn = m
@@ -133,20 +174,37 @@ class Problem:
     # ------------------------------------------------------------ the plan
     probe_laws = True   # recognise a plain-function tensor map as a built law when it equals one
 
-    def volume_internal_vars(self):
-        """internal_vars as device FP64 tensors of shape (ncell, nq, ...) (a per-cell
-        array (ncell,) is repeated over the points).  Held for a whole solve."""
-        held = getattr(self, "_held_ivs", None)
+    def cell_internal_vars(self):
+        """internal_vars as device FP64 tensors in the shapes given, each with leading
+        dimension num_cells: what a universal kernel receives per cell (problem.py:
+        335-357 vmap it over the cells unchanged: per-cell node values (ncell, nn, vec),
+        per-cell scalars (ncell,), per-point fields (ncell, nq, ...)).  Held for a whole solve."""
+        held = getattr(self, "_held_raw", None)
         if held is not None:
             return held
         out = []
         for a in (self.internal_vars or ()):
             t = torch.as_tensor(a.detach() if torch.is_tensor(a) else np.asarray(a), dtype=torch.float64,
                                 device=self.device)
-            if t.dim() == 1 and t.shape[0] == self.num_cells:
+            if t.dim() == 0 or t.shape[0] != self.num_cells:
+                raise ValueError("internal_vars entries must have leading dimension num_cells")
+            out.append(t.contiguous())
+        return out
+
+    def volume_internal_vars(self):
+        """internal_vars as device FP64 tensors of shape (ncell, nq, ...) (a per-cell
+        array (ncell,) is repeated over the points): what tensor and mass maps receive
+        per quadrature point.  Held for a whole solve."""
+        held = getattr(self, "_held_ivs", None)
+        if held is not None:
+            return held
+        out = []
+        for t in self.cell_internal_vars():
+            if t.dim() == 1:
                 t = t[:, None].expand(self.num_cells, self.fe.num_quads)
             if tuple(t.shape[:2]) != (self.num_cells, self.fe.num_quads):
-                raise ValueError("internal_vars entries must have leading shape (num_cells, num_quads)")
+                raise ValueError("internal_vars entries of tensor and mass maps must have leading shape "
+                                 "(num_cells, num_quads)")
             out.append(t.contiguous())
         return out
 
@@ -165,7 +223,7 @@ class Problem:
         if self.material is not None:
             law = "fast" if self._fast_supported(self.material) else "generic"
             fn = self.material            # a built law runs pointwise where its kernels do not apply
-        elif self._tensor_fn is not None and self.probe_laws:
+        elif self._tensor_fn is not None and self.probe_laws and not self._multi:
             scalar = len(layout) == 1 and len(layout[0]) == 2
             if len(layout) == 0 or scalar:
                 m = identify(self._tensor_fn, self.vec, self.dim, scalar_param=scalar)
@@ -199,7 +257,7 @@ class Problem:
         return plan
 
     def _fast_supported(self, m):
-        if self.dim != 3 or self.fe.ele_type not in ("HEX8", "HEX27") or m.vec != self.vec:
+        if self._multi or self.dim != 3 or self.fe.ele_type not in ("HEX8", "HEX27") or m.vec != self.vec:
             return False
         q = (int(self.fe.quadrature_order) + 2) // 2
         return (self.fe.ele_type == "HEX8" and q == 2) or (self.fe.ele_type == "HEX27" and q == 3)
@@ -222,8 +280,8 @@ class Problem:
         for i in idx:
             e = [a[i].reshape(np.shape(a[i])) for a in extra]
             try:
-                y0 = np.asarray(fn(np.zeros(self.vec), x[i], *e), dtype=np.float64)
-                y1 = np.asarray(fn(rng.standard_normal(self.vec), x[i], *e), dtype=np.float64)
+                y0 = np.asarray(fn(np.zeros(fe.vec), x[i], *e), dtype=np.float64)
+                y1 = np.asarray(fn(rng.standard_normal(fe.vec), x[i], *e), dtype=np.float64)
             except Exception:
                 return True     # not evaluable on NumPy arrays: take the general (device) path
             if not np.array_equal(y0, y1):
@@ -253,13 +311,22 @@ class Problem:
                 for a in (self.internal_vars_surfaces[k] or ()):
                     isv.append(torch.as_tensor(a.detach() if torch.is_tensor(a) else np.asarray(a),
                                                dtype=torch.float64, device=dev))
-                t.faces.append({"cells": torch.as_tensor(bi[:, 0], dtype=torch.long, device=dev),
-                                "cell_nodes": torch.as_tensor(fe.cells[bi[:, 0]], dtype=torch.long, device=dev),
-                                "vals": T(fe.face_shape_vals[bi[:, 1]]), "grads": T(grads), "nanson": T(nanson),
-                                "x": T(fe.get_physical_surface_quad_points(bi)), "isv": isv,
-                                "nsel": len(bi), "nfq": fe.num_face_quads,
-                                "map": smaps[k] if smaps is not None else None,
-                                "universal": usurf[k] if usurf is not None else None})
+                fd = {"cells": torch.as_tensor(bi[:, 0], dtype=torch.long, device=dev),
+                      "cell_nodes": torch.as_tensor(fe.cells[bi[:, 0]], dtype=torch.long, device=dev),
+                      "vals": T(fe.face_shape_vals[bi[:, 1]]), "grads": T(grads), "nanson": T(nanson),
+                      "x": T(fe.get_physical_surface_quad_points(bi)), "isv": isv,
+                      "nsel": len(bi), "nfq": fe.num_face_quads,
+                      "map": smaps[k] if smaps is not None else None,
+                      "universal": usurf[k] if usurf is not None else None}
+                if self._multi:
+                    # universal surface kernels see every variable (problem.py:153-182)
+                    gn = [f.get_face_shape_grads(bi) for f in self.fes]
+                    fd["sol_idx"] = torch.as_tensor(self._dof_cells[bi[:, 0]], dtype=torch.long, device=dev)
+                    fd["vals_all"] = T(np.concatenate([np.asarray(f.face_shape_vals)[bi[:, 1]] for f in self.fes],
+                                                      axis=2))
+                    fd["grads_all"] = T(np.concatenate([np.asarray(g) for g, _ in gn], axis=2))
+                    fd["nanson_all"] = T(np.stack([np.asarray(n) for _, n in gn], axis=1))
+                t.faces.append(fd)
             self._terms = t
         return self._terms
 
@@ -320,11 +387,21 @@ class Problem:
 
     # ---------------------------------------------------------------- layout
     def unflatten_fn_sol_list(self, dofs):
-        return [dofs.reshape(self.fe.num_total_nodes, self.vec)]
+        """Global DOF vector -> [(num_total_nodes_i, vec_i), ...] (problem.py:120-122)."""
+        out = []
+        for o, fe in zip(self.offset, self.fes):
+            out.append(dofs[o:o + fe.num_total_dofs].reshape(fe.num_total_nodes, fe.vec))
+        return out
+
+    def flatten_sol_list(self, sol_list):
+        """[(num_total_nodes_i, vec_i), ...] (torch or NumPy) -> one device f64 DOF vector."""
+        if torch.is_tensor(sol_list) or not isinstance(sol_list, (list, tuple)):
+            return torch.as_tensor(sol_list, dtype=torch.float64).reshape(-1).to(self.device)
+        return torch.cat([torch.as_tensor(s, dtype=torch.float64).reshape(-1).to(self.device) for s in sol_list])
 
     @property
     def nn(self):
-        return self.fe.num_nodes
+        return self._enn
 
     @property
     def N(self):
@@ -337,16 +414,25 @@ class Problem:
                 raise RuntimeError("the jax_fem_amd solve path needs a GPU (torch.cuda is unavailable)")
             _lib.load()
             self._bufs["points"] = torch.as_tensor(self.fe.points, dtype=torch.float64, device=self.device).contiguous()
-            self._bufs["cells"] = torch.as_tensor(self.fe.cells, dtype=torch.int32, device=self.device).contiguous()
+            pc = self._dof_cells if self._multi else self.fe.cells
+            self._bufs["cells"] = torch.as_tensor(pc, dtype=torch.int32, device=self.device).contiguous()
+            self._bufs["cells_var0"] = torch.as_tensor(self.fe.cells, dtype=torch.int32,
+                                                       device=self.device).contiguous()
         return self._bufs["points"], self._bufs["cells"]
+
+    def _var0_cells(self):
+        """Node cells of the first variable on the device (its pointwise maps act on it)."""
+        self._dev()
+        return self._bufs["cells_var0"]
 
     @property
     def pattern(self):
         if self._pattern is None:
             _, cells = self._dev()
             h = ctypes.c_void_p()
-            _lib.call("hf_pattern_build", _lib.ptr(cells), self.num_cells, self.nn, self.fe.num_total_nodes,
-                      self.vec, _lib.stream(), ctypes.byref(h))
+            nent = self.N if self._multi else self.fe.num_total_nodes
+            _lib.call("hf_pattern_build", _lib.ptr(cells), self.num_cells, self._enn, nent,
+                      self._evec, _lib.stream(), ctypes.byref(h))
             self._pattern = Pattern(h, self)
         return self._pattern
 
@@ -375,10 +461,13 @@ class Problem:
         """Freeze (on=True) or release the device copies of the internal variables
         for the duration of a solve; internal_vars must not change while they are held."""
         self._held_ivs = None
+        self._held_raw = None
         self._held_params = None
         if on:
             plan = self._plan()
-            self._held_ivs = self.volume_internal_vars()
+            self._held_raw = self.cell_internal_vars()
+            if plan["law"] == "fast" or plan["fn"] is not None or plan["mass"] == "cell":
+                self._held_ivs = self.volume_internal_vars()
             if plan["law"] == "fast":
                 self._held_params = (self._params(),)
 
@@ -390,27 +479,32 @@ class Problem:
         return b[:n]
 
     # ------------------------------------------------------- Dirichlet data
+    def bc_specs(self):
+        """Every Dirichlet spec of every variable: (DOF offset, vec, node_inds, vec_inds, vals)."""
+        return [(o, fe.vec, nodes, comps, vals) for o, fe in zip(self.offset, self.fes)
+                for nodes, comps, vals in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list)]
+
     def bc_arrays(self):
         """Unique BC DOFs (int32, device) and values (f64, device); later specs win (solver.py:298-300)."""
-        fe = self.fe
+        specs = self.bc_specs()
         # the value arrays are made by FiniteElement (immutable in the reference, JAX
         # arrays): keyed by identity, with references kept so no id is reused;
         # torch tensors also by their version counter (an in-place edit bumps it)
-        key = (tuple((id(v), v._version if torch.is_tensor(v) else -1) for v in fe.vals_list),
-               tuple(id(n) for n in fe.node_inds_list))
+        key = (tuple((id(v), v._version if torch.is_tensor(v) else -1) for _, _, _, _, v in specs),
+               tuple(id(n) for _, _, n, _, _ in specs))
         if self._bc_key != key:
             # the DOF set depends on the node arrays and components only; a load step
             # that changes the values alone reuses it (location functions are cached
             # per function object, so their node arrays are the same objects)
-            struct = [(nodes, int(np.asarray(comps).reshape(-1)[0]) if len(comps) else -1, len(comps))
-                      for nodes, comps in zip(fe.node_inds_list, fe.vec_inds_list)]
+            struct = [(nodes, int(np.asarray(comps).reshape(-1)[0]) if len(comps) else -1, len(comps), o)
+                      for o, _, nodes, comps, _ in specs]
             same = self._bc_struct is not None and len(struct) == len(self._bc_struct) and all(
-                a[0] is b[0] and a[1:] == b[1:] and np.all(np.asarray(c) == a[1])
-                for a, b, c in zip(struct, self._bc_struct, fe.vec_inds_list))
+                a[0] is b[0] and a[1:] == b[1:] and np.all(np.asarray(sp[3]) == a[1])
+                for a, b, sp in zip(struct, self._bc_struct, specs))
             if not same:
                 ds = [np.zeros(0, dtype=np.int64)]
-                for nodes, comps in zip(fe.node_inds_list, fe.vec_inds_list):
-                    ds.append(np.asarray(nodes, dtype=np.int64) * self.vec + np.asarray(comps, dtype=np.int64))
+                for o, vec, nodes, comps, _ in specs:
+                    ds.append(o + np.asarray(nodes, dtype=np.int64) * vec + np.asarray(comps, dtype=np.int64))
                 all_d = np.concatenate(ds)
                 # the last spec naming a DOF wins: first occurrence in reversed order; sorted DOFs
                 d, first = np.unique(all_d[::-1], return_index=True)
@@ -418,13 +512,13 @@ class Problem:
                 self._bc_dofs = torch.as_tensor(d.astype(np.int32), device=self.device)
                 self._bc_struct = struct
             vs = [np.zeros(0)]
-            for nodes, vals in zip(fe.node_inds_list, fe.vals_list):
+            for _, _, nodes, _, vals in specs:
                 vals = vals.detach().cpu().numpy() if torch.is_tensor(vals) else vals
                 vs.append(np.broadcast_to(np.asarray(vals, dtype=np.float64).reshape(-1), (len(nodes),)))
             v = np.concatenate(vs)[::-1][self._bc_first]
             self._bc_vals = torch.as_tensor(v, device=self.device)
             self._bc_key = key
-            self._bc_refs = (list(fe.vals_list), list(fe.node_inds_list))   # ids in the key stay unique
+            self._bc_refs = ([sp[4] for sp in specs], [sp[2] for sp in specs])   # ids in the key stay unique
         return self._bc_dofs, self._bc_vals
 
     # ------------------------------------------------------------ load vector
@@ -436,19 +530,20 @@ class Problem:
         if self._ext_key == key:
             return self._fext
         self._ext_refs = self.internal_vars_surfaces   # keeps ids in the key unique
-        fe = self.fe
-        f = np.zeros((fe.num_total_nodes, self.vec))
+        fe = self.fe          # mass and surface maps act on the first variable (problem.py:216-259)
+        vec = fe.vec
+        f = np.zeros((fe.num_total_nodes, vec))
         have = False
         if plan["mass"] == "load":
             mmap = self.get_mass_map()
             x = np.asarray(fe.get_physical_quad_points()).reshape(-1, self.dim)
-            m = self._eval_device(lambda xx: mmap(torch.zeros(self.vec, dtype=torch.float64, device=xx.device), xx),
-                                  lambda xx: mmap(np.zeros(self.vec), xx), x)
+            m = self._eval_device(lambda xx: mmap(torch.zeros(vec, dtype=torch.float64, device=xx.device), xx),
+                                  lambda xx: mmap(np.zeros(vec), xx), x)
             if m is None:
-                m = _eval_points(lambda xx: mmap(np.zeros(self.vec), xx), x)
+                m = _eval_points(lambda xx: mmap(np.zeros(vec), xx), x)
             m = m.reshape(self.num_cells, fe.num_quads, -1)
             r = np.einsum("cqi,qn,cq->cni", m, fe.shape_vals, fe.JxW)
-            np.add.at(f, fe.cells.reshape(-1), r.reshape(-1, self.vec))
+            np.add.at(f, fe.cells.reshape(-1), r.reshape(-1, vec))
             have = True
         if hasattr(self, "get_surface_maps"):
             for k, (bi, smap) in enumerate(zip(self.boundary_inds_list, self.get_surface_maps())):
@@ -458,13 +553,15 @@ class Problem:
                 xs = fe.get_physical_surface_quad_points(bi).reshape(-1, self.dim)
                 isv = [np.asarray(a.detach().cpu() if torch.is_tensor(a) else a).reshape(len(xs), -1)
                        for a in (self.internal_vars_surfaces[k] or ())]
-                u0 = np.zeros((len(xs), self.vec))
+                u0 = np.zeros((len(xs), vec))
                 t = _eval_points(lambda xx, uu, *a: smap(uu, xx, *a), xs, (u0, *isv)).reshape(
                     len(bi), fe.num_face_quads, -1)
                 vals = fe.face_shape_vals[bi[:, 1]]
                 r = np.einsum("sqi,sqn,sq->sni", t, vals, nanson)
-                np.add.at(f, fe.cells[bi[:, 0]].reshape(-1), r.reshape(-1, self.vec))
+                np.add.at(f, fe.cells[bi[:, 0]].reshape(-1), r.reshape(-1, vec))
                 have = True
+        if have and self._multi:
+            f = np.concatenate([f.reshape(-1), np.zeros(self.N - f.size)])
         self._fext = torch.as_tensor(f.reshape(-1), dtype=torch.float64, device=self.device) if have else None
         self._ext_key = key
         return self._fext
@@ -500,9 +597,9 @@ class Problem:
         plan = self._plan()
         pts, cells = self._dev()
         s = _lib.stream()
-        nd2 = self.nn * self.nn * self.vec * self.vec
         tag = "64" if dtype == torch.float64 else "32"
         if plan["law"] == "fast":
+            nd2 = self.nn * self.nn * self.vec * self.vec
             mat = self._hfmat()
             prm = self._params()
             er = self._buf("elem_res", self.num_cells * self.nn * self.vec, torch.float64)
@@ -518,6 +615,8 @@ class Problem:
                 er += rm
                 if jac:
                     ej += jm
+        elif self._multi:
+            return self._pointwise().multi_terms(u, jac, dtype)
         else:
             er, ej = self._pointwise().cell_terms(u, jac, dtype)
             self._bufs["elem_res"] = er
@@ -640,7 +739,7 @@ class Problem:
     def element_jacobians(self, u, dtype=torch.float64):
         """Element matrices (ncell, nd, nd), jac[r,c] = dR_r/du_c (problem.py:262-266)."""
         u = u.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
-        nn, v = self.nn, self.vec
+        nn, v = self._enn, self._evec
         _, ej = self._element_terms(u, True, dtype)
         # block-major (c, a, b, i, k) -> (c, a*v+i, b*v+k)
         return ej.reshape(self.num_cells, nn, nn, v, v).permute(0, 1, 3, 2, 4).reshape(
@@ -648,36 +747,33 @@ class Problem:
 
     # ------------------------------------------- reference-compatible methods
     def compute_residual(self, sol_list):
-        u = sol_list[0] if isinstance(sol_list, (list, tuple)) else sol_list
-        return self.unflatten_fn_sol_list(self.residual_vec(torch.as_tensor(u)))
+        return self.unflatten_fn_sol_list(self.residual_vec(self.flatten_sol_list(sol_list)))
 
     def newton_update(self, sol_list):
-        u = sol_list[0] if isinstance(sol_list, (list, tuple)) else sol_list
-        u = torch.as_tensor(u).reshape(-1)
-        self._newton_u = u.to(device=self.device, dtype=torch.float64).clone()
+        u = self.flatten_sol_list(sol_list)
+        self._newton_u = u.clone()
         self._A_cache = None
-        return self.compute_residual([u])
+        return self.compute_residual(u)
 
     @property
     def V(self):
         """Host COO values of the last newton_update: cells, then zero face blocks (problem.py:453-458)."""
         K = self.element_jacobians(self._newton_u).reshape(-1).cpu().numpy()
-        nd = self.nn * self.vec
+        nd = self._enn * self._evec
         nface = sum(len(b) for b in self.boundary_inds_list) * nd * nd
         return np.concatenate([K, np.zeros(nface)])
 
-    def _dof_inds(self, cells):
-        return (self.vec * cells[:, :, None] + np.arange(self.vec)[None, None, :]).reshape(len(cells), -1)
+    def _coo_inds(self):
+        d = self._dof_cells
+        return [d] + [d[b[:, 0]] for b in self.boundary_inds_list]
 
     @property
     def I(self):
-        inds = [self._dof_inds(self.fe.cells)] + [self._dof_inds(self.fe.cells[b[:, 0]]) for b in self.boundary_inds_list]
-        return np.concatenate([np.repeat(x[:, :, None], x.shape[1], axis=2).reshape(-1) for x in inds])
+        return np.concatenate([np.repeat(x[:, :, None], x.shape[1], axis=2).reshape(-1) for x in self._coo_inds()])
 
     @property
     def J(self):
-        inds = [self._dof_inds(self.fe.cells)] + [self._dof_inds(self.fe.cells[b[:, 0]]) for b in self.boundary_inds_list]
-        return np.concatenate([np.repeat(x[:, None, :], x.shape[1], axis=1).reshape(-1) for x in inds])
+        return np.concatenate([np.repeat(x[:, None, :], x.shape[1], axis=1).reshape(-1) for x in self._coo_inds()])
 
 
 class Pattern:
@@ -691,7 +787,7 @@ class Pattern:
         self.nvals = int(_lib.load().hf_matrix_values(handle))
         self.max_row_blocks = int(_lib.load().hf_pattern_max_row_blocks(handle))
         self.N = problem.N
-        self.vec = problem.vec
+        self.vec = problem._evec
         self.device = problem.device
 
     def csr(self):

@@ -28,11 +28,12 @@ a legacy flat dict, solver.py:1014-1068).  Linear keys:
   * ``custom_solver``: callable(A, b, x0, linear_options) -> x (solver.py:278-280);
     A is the device ``TangentMatrix``.
   * ``spsolve_solver`` (a host direct solve in the reference, solver.py:93-106)
-    runs as FP64 Jacobi-BiCGSTAB on the device to a relative residual of 1e-13,
-    i.e. to the accuracy a direct solve delivers in FP64;
-  * ``petsc_solver`` {ksp_type, pc_type} (solver.py:109-138) runs as FP64
-    Jacobi-BiCGSTAB on the device to PETSc's default relative tolerance 1e-5,
-    with the reference's ||Ax - b|| < 0.1 check; ``amgx_solver`` (NVIDIA AmgX)
+    is a dense LU on the device up to 32,768 DOFs and, above, FP64 Jacobi-BiCGSTAB
+    to a relative residual of 1e-13 (the accuracy a direct solve delivers in FP64);
+  * ``petsc_solver`` {ksp_type, pc_type} (solver.py:109-138): with pc_type 'lu'
+    the same dense LU up to 32,768 DOFs; otherwise FP64 Jacobi-BiCGSTAB on the
+    device to PETSc's default relative tolerance 1e-5, with the reference's
+    ||Ax - b|| < 0.1 check; ``amgx_solver`` (NVIDIA AmgX)
     raises ``NotImplementedError``.
 Added Newton keys:
   * ``precision``: 'fp64' (default: everything FP64, as the reference) or
@@ -297,6 +298,8 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
         return x
     if "custom_solver" in opts:
         return opts["custom_solver"](A, b, x0, opts)
+    if _direct(opts) and A.shape[0] <= _DENSE_MAX and A.dtype == torch.float64:
+        return dense_solve(A, b)
     for key, tol in (("spsolve_solver", 1e-13), ("petsc_solver", 1e-5)):
         if key in opts:
             x, info = krylov(A, b, x0, "bicgstab", tol if rtol is None else rtol, 0.0, 50000, True, stats=stats,
@@ -313,12 +316,41 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
     raise NotImplementedError("Unknown linear solver.")
 
 
+_DENSE_MAX = 32768     # direct solves: a dense LU on the device up to this many DOFs (8 GiB)
+
+
+def _direct(opts):
+    """A direct solve was asked for: spsolve_solver, or petsc_solver with an LU
+    preconditioner (one application of which solves the system)."""
+    if "spsolve_solver" in opts:
+        return True
+    o = opts.get("petsc_solver")
+    return o is not None and (o or {}).get("pc_type") == "lu"
+
+
+def dense_solve(A, b, transpose=False):
+    """x = A^-1 b (or A^-T b) by LU on the device (rocSOLVER through torch), A formed
+    densely from its CSR view: the direct solve of the reference's spsolve_solver
+    (solver.py:93-106) for systems that fit (N <= 32768: <= 8 GiB)."""
+    rp, ci, v = A.csr()
+    n = A.shape[0]
+    rows = torch.repeat_interleave(torch.arange(n, device=v.device), (rp[1:] - rp[:-1]).long())
+    D = torch.zeros(n, n, dtype=torch.float64, device=v.device)
+    D[rows, ci.long()] = v.to(torch.float64)
+    if transpose:
+        D = D.T.contiguous()
+    return torch.linalg.solve(D, b.to(torch.float64).reshape(-1, 1)).reshape(-1)
+
+
 # ---------------------------------------------------------------------------
 def _flat_initial_guess(problem, cfg):
     if "initial_guess" in cfg and cfg["initial_guess"] is not None:
         g = cfg["initial_guess"]
-        g = g[0] if isinstance(g, (list, tuple)) else g
-        return torch.as_tensor(g, dtype=torch.float64).detach().reshape(-1).to(problem.device).clone()
+        if isinstance(g, (list, tuple)):
+            g = g if problem.num_vars > 1 else g[0]
+        return problem.flatten_sol_list([t.detach() if torch.is_tensor(t) else t for t in g]
+                                        if isinstance(g, (list, tuple)) else
+                                        (g.detach() if torch.is_tensor(g) else g)).clone()
     return torch.zeros(problem.N, dtype=torch.float64, device=problem.device)
 
 
@@ -566,6 +598,11 @@ def adjoint_solve(problem, u, v, adjoint_solver_options=None):
     else:
         o = opts.get("bicgstab", opts.get("jax_solver", {})) or {}
         tol, atol, maxiter = o.get("tol", 1e-10), o.get("atol", 1e-10), o.get("maxiter", 10000)
+        if _direct(opts) and A.shape[0] <= _DENSE_MAX:     # a direct solve, as linear_solver
+            lam = dense_solve(A, v, transpose=True)
+            lamI = lam.clone()
+            lamI[dl] = 0.0
+            return lam, lamI
         if "spsolve_solver" in opts:         # direct-solve accuracy (see linear_solver)
             tol, atol, maxiter = 1e-13, 0.0, 50000
         elif "petsc_solver" in opts:
@@ -595,44 +632,50 @@ def param_vjp_pointwise(problem, u, lamI, which):
     """-(dR/dq)^T lam for the internal variables q = internal_vars[k], k in `which`,
     of maps evaluated pointwise (generic.PointwiseTerms): with lam's gradient and
     values at the quadrature points (the same contraction kernels as u's) as the
-    cotangents, torch.func.vjp of the tensor map, the mass map and the universal
-    kernel with respect to their internal-variable arguments.  Returns a list of
-    tensors shaped like volume_internal_vars()[k]."""
+    cotangents, torch.func.vjp of the tensor map and the mass map with respect to
+    their per-point internal-variable arguments, and of the universal kernel with
+    respect to its per-cell ones.  Returns a list of tensors shaped like
+    cell_internal_vars()[k] (a per-cell value repeated over the points receives the
+    sum over its points)."""
     terms = problem._pointwise()
     plan = problem._plan()
-    ivs = problem.volume_internal_vars()
-    Q = problem.num_cells * problem.fe.num_quads
-    flat = [a.reshape((Q,) + tuple(a.shape[2:])) for a in ivs]
-    need_q = plan["mass"] == "cell"
-    ug, uq = terms.point_data(u.contiguous(), need_q)
-    lg, lq = terms.point_data(lamI.contiguous(), need_q)
-    w = terms.jxw.reshape(-1)
-    total = [torch.zeros_like(a) for a in flat]
+    raw = problem.cell_internal_vars()
+    total = [torch.zeros_like(a) for a in raw]
 
     from . import numpy as jnp
 
-    def add(fn, fixed, cot):
-        nf = len(fixed)
+    if terms.tensor_fn is not None or terms.mass_fn is not None:
+        ivs = problem.volume_internal_vars()
+        Q = problem.num_cells * problem.fe.num_quads
+        flat = [a.reshape((Q,) + tuple(a.shape[2:])) for a in ivs]
+        need_q = plan["mass"] == "cell"
+        ug, uq = terms.point_data(u.contiguous(), need_q)
+        lg, lq = terms.point_data(lamI.contiguous(), need_q)
+        w = terms.jxw.reshape(-1)
 
-        def one(*a):
-            fx, c, rest = a[:nf], a[nf], a[nf + 1:]
-            _, pull = torch.func.vjp(lambda *q: fn(*fx, *q), *rest)
-            return pull(c)
-        with jnp.on_device(u.device):
-            g = torch.func.vmap(one)(*fixed, cot, *flat)
-        for k in which:
-            total[k] += g[k]
-    if terms.tensor_fn is not None:
-        add(terms.tensor_fn, [ug], lg * w[:, None, None])
-    if terms.mass_fn is not None:
-        add(terms.mass_fn, [uq, terms.xq.reshape(Q, -1)], lq * w[:, None])
+        def add(fn, fixed, cot):
+            nf = len(fixed)
+
+            def one(*a):
+                fx, c, rest = a[:nf], a[nf], a[nf + 1:]
+                _, pull = torch.func.vjp(lambda *q: fn(*fx, *q), *rest)
+                return pull(c)
+            with jnp.on_device(u.device):
+                g = torch.func.vmap(one)(*fixed, cot, *flat)
+            for k in which:
+                gk = g[k].reshape(ivs[k].shape)
+                if raw[k].dim() == 1:
+                    gk = gk.sum(dim=1)
+                total[k] += gk
+        if terms.tensor_fn is not None:
+            add(terms.tensor_fn, [ug], lg * w[:, None, None])
+        if terms.mass_fn is not None:
+            add(terms.mass_fn, [uq, terms.xq.reshape(Q, -1)], lq * w[:, None])
     if terms.universal is not None:
-        _, cells = problem._dev()
-        nc = problem.num_cells
-        cu = u.reshape(-1, problem.vec)[cells.long()].reshape(nc, -1)
-        cl = lamI.reshape(-1, problem.vec)[cells.long()].reshape(nc, -1)
-        sg = terms.sg
-        args = [cu, terms.xq, sg, terms.jxw[:, None, :], (sg * terms.jxw[:, :, None, None])[:, :, :, None, :]]
+        cu = terms._cell_sol_flat(u)
+        cl = terms._cell_sol_flat(lamI)
+        sgc, jxw_all, vgj = terms._all_vars_geometry()
+        args = [cu, terms.xq, sgc, jxw_all, vgj]
         k5 = len(args)
         kern = terms.universal
 
@@ -641,10 +684,10 @@ def param_vjp_pointwise(problem, u, lamI, which):
             _, pull = torch.func.vjp(lambda *q: kern(*fx, *q), *rest)
             return pull(c)
         with jnp.on_device(u.device):
-            g = torch.func.vmap(one_cell)(*args, cl, *ivs)
+            g = torch.func.vmap(one_cell)(*args, cl, *raw)
         for k in which:
             total[k] += g[k].reshape(total[k].shape)
-    return [-total[k].reshape(ivs[k].shape) for k in range(len(ivs))]
+    return [-t for t in total]
 
 
 def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options={}):
@@ -659,9 +702,9 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options={}):
     with torch.enable_grad():
         problem.set_params(_unflatten(list(leaves), spec))
         ivs_req = list(problem.internal_vars or ())
-        bc_vals = list(problem.fe.vals_list)
-    u = torch.as_tensor(sol_list[0]).reshape(-1).to(problem.device, torch.float64)
-    v = torch.as_tensor(v_list[0]).reshape(-1).to(problem.device, torch.float64)
+        bc_vals = [sp[4] for sp in problem.bc_specs()]
+    u = problem.flatten_sol_list(sol_list)
+    v = problem.flatten_sol_list(v_list)
     problem._newton_u = u
     lam, lamI = adjoint_solve(problem, u, v, adjoint_solver_options)
     outs, gouts = [], []
@@ -677,20 +720,19 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options={}):
         gs = param_vjp_pointwise(problem, u, lamI, which)
         for k in which:
             gk = gs[k]
-            if ivs_req[k].dim() == 1 and gk.dim() == 2:      # a per-cell value repeated over the points
-                gk = gk.sum(dim=1)
             outs.append(ivs_req[k])
             gouts.append(gk.reshape(ivs_req[k].shape).to(ivs_req[k].device, ivs_req[k].dtype))
     # Dirichlet values: c_B = u_B - g  =>  d<lam, c>/dg = -lam_B, grad = +lam_B (later specs win)
-    fe = problem.fe
+    specs = problem.bc_specs()
     owner = {}
-    for s, (nodes, comps) in enumerate(zip(fe.node_inds_list, fe.vec_inds_list)):
-        for j, dof in enumerate((np.asarray(nodes) * problem.vec + np.asarray(comps)).tolist()):
+    for s, (o, vec, nodes, comps, _) in enumerate(specs):
+        for j, dof in enumerate((o + np.asarray(nodes) * vec + np.asarray(comps)).tolist()):
             owner[dof] = (s, j)
     lam_h = lam.detach().cpu().numpy()
     for s, vals in enumerate(bc_vals):
         if torch.is_tensor(vals) and vals.requires_grad:
-            dofs = (np.asarray(fe.node_inds_list[s]) * problem.vec + np.asarray(fe.vec_inds_list[s])).tolist()
+            o, vec, nodes, comps, _ = specs[s]
+            dofs = (o + np.asarray(nodes) * vec + np.asarray(comps)).tolist()
             gv = np.array([lam_h[d] if owner[d] == (s, j) else 0.0 for j, d in enumerate(dofs)])
             outs.append(vals)
             gouts.append(torch.as_tensor(gv, dtype=vals.dtype, device=vals.device))
@@ -712,7 +754,7 @@ def ad_wrapper(problem, solver_options={}, adjoint_solver_options={}):
         def forward(ctx, spec_holder, *flat):
             params = _unflatten([f.detach() for f in flat], spec_holder[0])
             problem.set_params(params)
-            sol = solver(problem, solver_options)[0]
+            sol = problem.flatten_sol_list(solver(problem, solver_options))   # every variable's DOFs
             ctx.spec = spec_holder[0]
             ctx.save_for_backward(*flat)
             ctx.sol = sol.detach()
@@ -729,6 +771,6 @@ def ad_wrapper(problem, solver_options={}, adjoint_solver_options={}):
     def fwd_pred(params):
         flat, spec = _flatten(params)
         flat = [f if torch.is_tensor(f) else torch.as_tensor(f, dtype=torch.float64) for f in flat]
-        return [_ImplicitSolve.apply([spec], *flat)]
+        return problem.unflatten_fn_sol_list(_ImplicitSolve.apply([spec], *flat))
 
     return fwd_pred
