@@ -587,11 +587,16 @@ def simp_adjoint(jfa, prob, theta, fd_cells, opts_fwd, adj_opts=None):
 
 def fine_model_bytes(n0):
     """HBM bytes of one flexible-CG iteration's fine-level work per the kernels'
-    algorithmic traffic (FP32, half-stored stencil 126 values + block inverse 9 per
-    node): update 180, pre-smoothing step 612, residual 552, restriction read 12,
-    prolongation 36, post-smoothing start 600 and step 612, outer product 564 bytes
-    per node.  The coarse levels (about a sixth more) are not counted."""
-    return n0 * (180 + 612 + 552 + 12 + 36 + 600 + 612 + 564)
+    algorithmic traffic: the smoothing passes read the FP16 copy of the half-stored
+    stencil (14 blocks x 5 half2 words = 280 B per node) and the FP32 block inverse
+    (36 B), the outer product the FP32 stencil (126 x 4 B): update 180, pre-smoothing
+    step 388, residual 328, restriction read 12, prolongation 36, post-smoothing start
+    376 and step 388, outer product 564 bytes per node.  The coarse levels (about a
+    sixth more) are not counted."""
+    return n0 * FINE_NODE_BYTES
+
+
+FINE_NODE_BYTES = 180 + 388 + 328 + 12 + 36 + 376 + 388 + 564
 
 
 def main():
@@ -720,15 +725,16 @@ def main():
         kname = "mg_sell_spmv (fine-level smoothing product y = A v, SELL column-form node blocks)"
         pmc = "mg_sell_spmv_%s_pmc.md" % tag
     elif use_mg:
-        # one fine-level Chebyshev smoothing step (mg_poly_step<true>): the symmetric
-        # 27-point node-block stencil stored as its diagonal and 13 upper blocks (126
-        # FP32 values per node; the lower blocks are the neighbours' upper ones, read
-        # once each in the ideal), the 3x3 block-Jacobi inverse (9), and per node the
-        # residual, direction and correction read (3 FP32 each) and written (3 each)
-        kbytes = prob.N // 3 * (126 * 4 + 9 * 4 + 6 * 12)
-        kname = ("mg_poly_step<true> (fine-level Chebyshev smoothing step: half-stored symmetric "
-                 "27-point node-block stencil, slot-major FP32)")
-        pmc = "mg_poly_step_%s_pmc.md" % tag
+        # one fine-level Chebyshev smoothing step (mg_poly_step<true, __half2>): the
+        # symmetric 27-point node-block stencil stored as its diagonal and 13 upper
+        # blocks, each block's 9 entries as 5 FP16 pairs (280 B per node; the lower
+        # blocks are the neighbours' upper ones, read once each in the ideal), the FP32
+        # 3x3 block-Jacobi inverse (36 B), and per node the residual, direction and
+        # correction read (3 FP32 each) and written (3 each)
+        kbytes = prob.N // 3 * (14 * 5 * 4 + 9 * 4 + 6 * 12)
+        kname = ("mg_poly_step<true, __half2> (fine-level Chebyshev smoothing step: half-stored symmetric "
+                 "27-point node-block stencil, slot-major FP16 pairs)")
+        pmc = "mg_poly_step_f16_%s_pmc.md" % tag
     else:
         # one PCG SpMV (w = A u): values (4 B/nnz), one block column per node-pair
         # block, u read and w written (4 B per DOF each)
@@ -814,8 +820,8 @@ def main():
     whole = None
     if use_mg and prob.fe.ele_type == "HEX8":
         wb = fine_model_bytes(n0) * fcg_iters
-        whole = {"model": "fine-level flexible-CG traffic only (3168 B per fine node per iteration; coarse levels, "
-                          "tangent assembly and set-up not counted: a lower bound)",
+        whole = {"model": "fine-level flexible-CG traffic only (%d B per fine node per iteration; coarse levels, "
+                          "tangent assembly and set-up not counted: a lower bound)" % FINE_NODE_BYTES,
                  "bytes": wb, "wall_s": wall_m, "achieved": wb / wall_m / 1e9, "unit": "GB/s",
                  "frac": wb / wall_m / 1e9 / 8000.0}
     out = {

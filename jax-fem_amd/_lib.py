@@ -19,7 +19,7 @@ HF_MAT_POISSON, HF_MAT_LINEAR_ELASTIC, HF_MAT_NEO_HOOKEAN = 0, 1, 2
 HF_PARAM_NONE, HF_PARAM_SCALE_E, HF_PARAM_SIMP, HF_PARAM_MU, HF_PARAM_SCALE_K = 0, 1, 2, 3, 4
 HF_KRYLOV_PCG, HF_KRYLOV_BICGSTAB = 0, 1
 HF_MG_JACOBI, HF_MG_CHEBYSHEV = 0, 1
-HF_MG_FINE_FULL, HF_MG_FINE_SELL = 1, 2
+HF_MG_FINE_FULL, HF_MG_FINE_SELL, HF_MG_FINE_F16 = 1, 2, 4
 HF_MG_SKIP_FINE_POST = 1
 HF_MG_SOLVE_EAGER = 4
 HF_MG_SOLVE_SPLIT_SCALARS = 8

@@ -39,6 +39,7 @@
 #include "hipfem.h"
 
 #include <hip/hip_ext.h>
+#include <hip/hip_fp16.h>
 
 #include "krylov_common.hpp"
 
@@ -124,6 +125,41 @@ __device__ __forceinline__ void load_block(const float* __restrict__ S, int64_t 
   }
 }
 
+// The fine level's stencil in FP16 for the smoothing passes (HF_MG_FINE_F16):
+// the half-stored slots with each block's 9 entries packed in 5 half2 words
+// (entries 2k, 2k + 1; the fifth word's upper half is zero), word-major over
+// the nodes: W[(s*5 + k)*nnode + n].  The entries are scaled by a power of two
+// that puts the largest magnitude in [2^14, 2^15) (exact: FP16's 11-bit
+// significand is the only rounding), and the products are scaled back by its
+// inverse.  280 bytes a node instead of 504.
+constexpr int kF16Words = 5;
+template <bool HALF>
+__device__ __forceinline__ void load_block(const __half2* __restrict__ W, int64_t N, int64_t n, int o, int64_t m,
+                                           bool valid, float (&B)[9]) {
+  static_assert(HALF, "FP16 stencils are half-stored");
+  const bool own = o >= 13;
+  const __half2* p = W + (int64_t)((own ? o - 13 : 13 - o) * kF16Words) * N + (own ? n : m);
+  float T[2 * kF16Words];
+#pragma unroll
+  for (int k = 0; k < kF16Words; ++k) {
+    const float2 f = __half22float2(p[(int64_t)k * N]);
+    T[2 * k] = f.x;
+    T[2 * k + 1] = f.y;
+  }
+  if (own) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) B[e] = T[e];
+  } else {
+    const float vf = valid ? 1.f : 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) B[3 * i + j] = vf * T[3 * j + i];
+  }
+}
+template <typename ST>
+constexpr bool is_f16() { return std::is_same<ST, __half2>::value; }
+
 // 1-D transfer between nf fine intervals and their coarse nodes.
 __device__ __forceinline__ int cpos(int c, int nf) { return 2 * c < nf ? 2 * c : nf; }
 __device__ __forceinline__ bool is_cpos(int f, int nf) { return (f % 2 == 0) || f == nf; }
@@ -161,9 +197,9 @@ __device__ __forceinline__ int pair_slot(int t) {
 // in offset order the second read came half a kernel later and mostly missed
 // L2 (fine-pass fetch 290 -> 210 MB at 65^3 nodes).  The pair loop is not
 // unrolled: unrolled, the compiler hoists all 27 block loads (256 VGPRs).
-template <bool HALF>
-__device__ __forceinline__ void stencil_row(const float* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
-                                            const float* __restrict__ v, float (&out)[3]) {
+template <bool HALF, typename ST = float>
+__device__ __forceinline__ void stencil_row(const ST* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
+                                            const float* __restrict__ v, float (&out)[3], float sinv = 1.f) {
   {
     float B[9];
     load_block<HALF>(S, g.n, n, 13, n, true, B);
@@ -188,6 +224,10 @@ __device__ __forceinline__ void stencil_row(const float* __restrict__ S, const L
     for (int i = 0; i < 3; ++i)
       out[i] += (Bl[3 * i] * l0 + Bl[3 * i + 1] * l1 + Bl[3 * i + 2] * l2) +
                 (Bu[3 * i] * u0 + Bu[3 * i + 1] * u1 + Bu[3 * i + 2] * u2);
+  }
+  if constexpr (is_f16<ST>()) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] *= sinv;
   }
 }
 
@@ -724,15 +764,15 @@ __device__ __forceinline__ void poly_first_node(const float* __restrict__ Dinv, 
 
 // res = res_in - A d_in;  d_out = c1 d_in + c2 Dinv res;  e += d_out
 // (res_in may alias res_out: each node reads and writes only its own entries)
-template <bool HALF>
-__device__ __forceinline__ void poly_step_node(const float* __restrict__ S, const float* __restrict__ Dinv,
+template <bool HALF, typename ST>
+__device__ __forceinline__ void poly_step_node(const ST* __restrict__ S, float sinv, const float* __restrict__ Dinv,
                                                const Lat& g, float c1, float c2, const float* res_in,
                                                const float* __restrict__ d_in, float* res_out,
                                                float* __restrict__ d_out, float* __restrict__ e, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ad[3];
-  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad);
+  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad, sinv);
   const float res[3] = {res_in[3 * node] - ad[0], res_in[3 * node + 1] - ad[1], res_in[3 * node + 2] - ad[2]};
   float dr[3];
   block_apply(Dinv, g.n, node, res, dr);
@@ -748,14 +788,14 @@ __device__ __forceinline__ void poly_step_node(const float* __restrict__ S, cons
 // t = res_in - A d_in (the residual after the last pre-smoothing step); with a
 // mask (the fine level, whose stencil may hold the Dirichlet rows) t = M (...):
 // the restriction must not see those rows
-template <bool HALF>
-__device__ __forceinline__ void poly_resid_node(const float* __restrict__ S, const Lat& g,
+template <bool HALF, typename ST>
+__device__ __forceinline__ void poly_resid_node(const ST* __restrict__ S, float sinv, const Lat& g,
                                                 const float* __restrict__ res_in, const float* __restrict__ d_in,
                                                 const float* __restrict__ mask, float* __restrict__ t, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ad[3];
-  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad);
+  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad, sinv);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float v = res_in[3 * node + i] - ad[i];
@@ -764,15 +804,15 @@ __device__ __forceinline__ void poly_resid_node(const float* __restrict__ S, con
 }
 
 // post-smoother start: res = r - A e_in;  d = s0 Dinv res;  e_out = e_in + d
-template <bool HALF>
-__device__ __forceinline__ void post_first_node(const float* __restrict__ S, const float* __restrict__ Dinv,
+template <bool HALF, typename ST>
+__device__ __forceinline__ void post_first_node(const ST* __restrict__ S, float sinv, const float* __restrict__ Dinv,
                                                 const Lat& g, float s0, const float* __restrict__ r,
                                                 const float* __restrict__ e_in, float* __restrict__ res,
                                                 float* __restrict__ d, float* __restrict__ e_out, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ae[3];
-  stencil_row<HALF>(S, g, node, x, y, z, e_in, ae);
+  stencil_row<HALF>(S, g, node, x, y, z, e_in, ae, sinv);
   const float rr[3] = {r[3 * node] - ae[0], r[3 * node + 1] - ae[1], r[3 * node + 2] - ae[2]};
   float dr[3];
   block_apply(Dinv, g.n, node, rr, dr);
@@ -885,36 +925,50 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_first(const float* __restric
   MG_NODE_LOOP(n, poly_first_node(Dinv, n, s0, r, d, e, node));
 }
 
-template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_poly_step(const float* __restrict__ S, const float* __restrict__ Dinv,
+// The smoothing passes take the level's stencil as FP32 (ST = float, scl null)
+// or, on the fine level with HF_MG_FINE_F16, as FP16 words (ST = __half2) with
+// scl[1] the inverse of their scale.
+template <typename ST>
+__device__ __forceinline__ float stencil_sinv(const float* scl) {
+  if constexpr (is_f16<ST>()) return scl[1];
+  return 1.f;
+}
+
+template <bool HALF, typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_step(const ST* __restrict__ S, const float* __restrict__ scl,
+                                                         const float* __restrict__ Dinv,
                                                          Lat g, const float* __restrict__ cf, int j,
                                                          const float* res_in, const float* __restrict__ d_in,
                                                          float* res_out, float* __restrict__ d_out,
                                                          float* __restrict__ e, const Scalars* sc) {
   if (sc && sc->done) return;
   const float c1 = cf[1 + j], c2 = cf[1 + kMaxDegree + j];
-  MG_NODE_LOOP(g.n, poly_step_node<HALF>(S, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
+  const float sinv = stencil_sinv<ST>(scl);
+  MG_NODE_LOOP(g.n, poly_step_node<HALF>(S, sinv, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
 }
 
-template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const float* __restrict__ S, Lat g,
-                                                          const float* __restrict__ res_in,
+template <bool HALF, typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const ST* __restrict__ S, const float* __restrict__ scl,
+                                                          Lat g, const float* __restrict__ res_in,
                                                           const float* __restrict__ d_in,
                                                           const float* __restrict__ mask, float* __restrict__ t,
                                                           const Scalars* sc) {
   if (sc && sc->done) return;
-  MG_NODE_LOOP(g.n, poly_resid_node<HALF>(S, g, res_in, d_in, mask, t, node));
+  const float sinv = stencil_sinv<ST>(scl);
+  MG_NODE_LOOP(g.n, poly_resid_node<HALF>(S, sinv, g, res_in, d_in, mask, t, node));
 }
 
-template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_post_first(const float* __restrict__ S, const float* __restrict__ Dinv,
+template <bool HALF, typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mg_post_first(const ST* __restrict__ S, const float* __restrict__ scl,
+                                                          const float* __restrict__ Dinv,
                                                           Lat g, const float* __restrict__ cf,
                                                           const float* __restrict__ r, const float* __restrict__ e_in,
                                                           float* __restrict__ res, float* __restrict__ d,
                                                           float* __restrict__ e_out, const Scalars* sc) {
   if (sc && sc->done) return;
   const float s0 = cf[0];
-  MG_NODE_LOOP(g.n, post_first_node<HALF>(S, Dinv, g, s0, r, e_in, res, d, e_out, node));
+  const float sinv = stencil_sinv<ST>(scl);
+  MG_NODE_LOOP(g.n, post_first_node<HALF>(S, sinv, Dinv, g, s0, r, e_in, res, d, e_out, node));
 }
 
 __global__ void __launch_bounds__(kMgBlock) mg_restrict(const float* __restrict__ t, Lat f, Lat c,
@@ -996,6 +1050,60 @@ __global__ void __launch_bounds__(kMgBlock) mg_max_final(const float* __restrict
   }
 }
 
+// FP16 copy of the fine stencil (HF_MG_FINE_F16), in three launches on the
+// set-up stream: per-workgroup maxima of |S|, then (mg_max_final into
+// scl[2]) the scale, then the conversion.
+__global__ void __launch_bounds__(kMgBlock) mg_absmax(const float* __restrict__ S, int64_t count,
+                                                      float* __restrict__ part) {
+  __shared__ float sm[kMgBlock / kWave];
+  float best = 0.f;
+  const int64_t n4 = count / 4;
+  const float4* S4 = reinterpret_cast<const float4*>(S);
+  for (int64_t i = (int64_t)blockIdx.x * kMgBlock + threadIdx.x; i < n4; i += (int64_t)gridDim.x * kMgBlock) {
+    const float4 v = S4[i];
+    best = fmaxf(best, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+  }
+  if (blockIdx.x == 0)
+    for (int64_t i = 4 * n4 + threadIdx.x; i < count; i += kMgBlock) best = fmaxf(best, fabsf(S[i]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o, 64));
+  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = sm[0];
+    for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
+    part[blockIdx.x] = m;
+  }
+}
+
+// scl[0] = 2^(14 - floor(log2 max|S|)) (1 for a zero or non-finite maximum),
+// scl[1] = 1 / scl[0]; scl[2] holds the maximum
+__global__ void mg_f16_scale(float* __restrict__ scl) {
+  if (threadIdx.x != 0) return;
+  const float m = scl[2];
+  float sc = 1.f;
+  if (m > 0.f && isfinite(m)) sc = ldexpf(1.f, 14 - ilogbf(m));
+  scl[0] = sc;
+  scl[1] = 1.f / sc;
+}
+
+// W[(s*5 + k)*n + node] = half2(scl[0] * S[(s*9 + 2k)*n + node], scl[0] * S[(s*9 + 2k + 1)*n + node]),
+// one thread per (node, slot = blockIdx.y)
+__global__ void __launch_bounds__(kMgBlock) mg_to_f16(const float* __restrict__ S, int64_t n,
+                                                      const float* __restrict__ scl, __half2* __restrict__ W) {
+  const int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (node >= n) return;
+  const int sl = blockIdx.y;
+  const float sc = scl[0];
+  float v[2 * kF16Words];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) v[e] = sc * S[(int64_t)(sl * 9 + e) * n + node];
+  v[9] = 0.f;
+#pragma unroll
+  for (int k = 0; k < kF16Words; ++k)
+    W[(int64_t)(sl * kF16Words + k) * n + node] = __floats2half2_rn(v[2 * k], v[2 * k + 1]);
+}
+
 // coarsest level: e = Ainv r (FP64 accumulation), one workgroup
 // e = Ainv r for the dense coarsest level (Ainv column-major FP64, nd <= kDenseMax)
 // by one workgroup of T threads: thread t takes row t % nd and every ng-th
@@ -1067,9 +1175,10 @@ struct SmallCycle {
 // 4) is how many of a lane's 3-4 blocks have their loads in flight together:
 // 4 in the grid kernels; in the 1024-thread small cycle (128 VGPRs a lane) all
 // four spill.
-template <bool HALF, int UNROLL>
-__device__ __forceinline__ void stencil_row_group(const float* __restrict__ S, const Lat& g, int64_t n,
-                                                  const float* __restrict__ v, int q, float (&out)[3]) {
+template <bool HALF, int UNROLL, typename ST = float>
+__device__ __forceinline__ void stencil_row_group(const ST* __restrict__ S, const Lat& g, int64_t n,
+                                                  const float* __restrict__ v, int q, float (&out)[3],
+                                                  float sinv = 1.f) {
   int x, y, z;
   lat_xyz(g, n, x, y, z);
   out[0] = out[1] = out[2] = 0.f;
@@ -1090,6 +1199,10 @@ __device__ __forceinline__ void stencil_row_group(const float* __restrict__ S, c
   for (int w = 1; w < kGroup; w <<= 1)
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[i] += __shfl_xor(out[i], w, kWave);
+  if constexpr (is_f16<ST>()) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] *= sinv;
+  }
 }
 
 #define MG_GROUP_LOOP(N, ...)                                                                           \
@@ -1226,8 +1339,9 @@ __device__ __forceinline__ float block_row(const float* __restrict__ D, int64_t 
 
 // as mg_poly_step (res_in may alias res_out: a group reads all three entries of
 // its node before any of its lanes stores one)
-template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_poly_step_lanes(const float* __restrict__ S,
+template <bool HALF, typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_step_lanes(const ST* __restrict__ S,
+                                                               const float* __restrict__ scl,
                                                                const float* __restrict__ Dinv, Lat g,
                                                                const float* __restrict__ cf, int j,
                                                                const float* res_in, const float* __restrict__ d_in,
@@ -1235,9 +1349,10 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_step_lanes(const float* __re
                                                                float* __restrict__ e, const Scalars* sc) {
   if (sc && sc->done) return;
   const float c1 = cf[1 + j], c2 = cf[1 + kMaxDegree + j];
+  const float sinv = stencil_sinv<ST>(scl);
   MG_LANE_LOOP(g.n, {
     float ad[3];
-    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad);
+    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad, sinv);
     const float rr[3] = {res_in[3 * node] - ad[0], res_in[3 * node + 1] - ad[1], res_in[3 * node + 2] - ad[2]};
     if (q < 3) {
       const float dn = c1 * d_in[3 * node + q] + c2 * block_row(Dinv, g.n, node, q, rr);
@@ -1248,16 +1363,18 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_step_lanes(const float* __re
   });
 }
 
-template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_poly_resid_lanes(const float* __restrict__ S, Lat g,
+template <bool HALF, typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mg_poly_resid_lanes(const ST* __restrict__ S,
+                                                                const float* __restrict__ scl, Lat g,
                                                                 const float* __restrict__ res_in,
                                                                 const float* __restrict__ d_in,
                                                                 const float* __restrict__ mask,
                                                                 float* __restrict__ t, const Scalars* sc) {
   if (sc && sc->done) return;
+  const float sinv = stencil_sinv<ST>(scl);
   MG_LANE_LOOP(g.n, {
     float ad[3];
-    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad);
+    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad, sinv);
     if (q < 3) {
       const float v = res_in[3 * node + q] - ad[q];
       t[3 * node + q] = mask ? v * mask[3 * node + q] : v;
@@ -1265,8 +1382,9 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_resid_lanes(const float* __r
   });
 }
 
-template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_post_first_lanes(const float* __restrict__ S,
+template <bool HALF, typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mg_post_first_lanes(const ST* __restrict__ S,
+                                                                const float* __restrict__ scl,
                                                                 const float* __restrict__ Dinv, Lat g,
                                                                 const float* __restrict__ cf,
                                                                 const float* __restrict__ r,
@@ -1275,9 +1393,10 @@ __global__ void __launch_bounds__(kMgBlock) mg_post_first_lanes(const float* __r
                                                                 float* __restrict__ e_out, const Scalars* sc) {
   if (sc && sc->done) return;
   const float s0 = cf[0];
+  const float sinv = stencil_sinv<ST>(scl);
   MG_LANE_LOOP(g.n, {
     float ae[3];
-    stencil_row_group<HALF, 4>(S, g, node, e_in, q, ae);
+    stencil_row_group<HALF, 4>(S, g, node, e_in, q, ae, sinv);
     const float rr[3] = {r[3 * node] - ae[0], r[3 * node + 1] - ae[1], r[3 * node + 2] - ae[2]};
     if (q < 3) {
       const float dq = s0 * block_row(Dinv, g.n, node, q, rr);
@@ -1879,6 +1998,7 @@ struct MgLevel {
   float* da = nullptr;    // 3 n smoother directions (two, alternating)
   float* db = nullptr;    // 3 n
   float* res = nullptr;   // 3 n smoother residual / restriction input
+  __half2* S16 = nullptr; // fine level with HF_MG_FINE_F16: 70 n words (mg_to_f16)
 };
 
 struct hf_mg {
@@ -1891,6 +2011,9 @@ struct hf_mg {
   float* part = nullptr;     // kMaxGrid per-workgroup maxima + kMgMaxLevels bounds
   float* coef = nullptr;     // kMgMaxLevels x kCoefStride smoothing coefficients (mg_coef)
   bool half = false;         // fine-level stencil half-stored (symmetric operator)
+  bool f16 = false;          // fine-level smoothing passes read an FP16 copy of the stencil
+  float* scl = nullptr;      // its scale, inverse scale and |S| maximum (mg_f16_scale)
+  const float* s0 = nullptr; // the FP32 fine stencil of the last set-up (the caller's, or lv[0].S)
   bool sell = false;         // fine level is the tangent's SELL matrix (no fine stencil)
   float* y = nullptr;        // 3 n0: SELL products of the fine level
   const hf_pattern* sp = nullptr;  // the fine level's pattern and values (last set-up, caller's memory)
@@ -1946,12 +2069,15 @@ static int mg_free(hf_mg* mg) {
   if (mg->coef) (void)hipFree(mg->coef);
   if (mg->lsm.buf) (void)hipFree(mg->lsm.buf);
   if (mg->sdinv) (void)hipFree(mg->sdinv);
+  if (mg->lv[0].S16) (void)hipFree(mg->lv[0].S16);
+  if (mg->scl) (void)hipFree(mg->scl);
   return 0;
 }
 
 extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out) {
   HF_REQUIRE(out, "hf_mg_create: null output");
-  HF_REQUIRE((flags & ~(HF_MG_FINE_FULL | HF_MG_FINE_SELL)) == 0, "hf_mg_create: unknown flags 0x%x", flags);
+  HF_REQUIRE((flags & ~(HF_MG_FINE_FULL | HF_MG_FINE_SELL | HF_MG_FINE_F16)) == 0,
+             "hf_mg_create: unknown flags 0x%x", flags);
   HF_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "hf_mg_create: lattice %dx%dx%d needs >= 2 nodes per dimension", nx,
              ny, nz);
   HF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 30, "hf_mg_create: lattice too large");
@@ -1975,6 +2101,8 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
   mg->sell = (flags & HF_MG_FINE_SELL) != 0;
   // a one-level hierarchy is the dense FP64 solve, which reads the full stencil
   mg->half = !(flags & HF_MG_FINE_FULL) && !mg->sell && nlev > 1;
+  // the FP16 copy serves the half-stored fine stencil only (ignored otherwise)
+  mg->f16 = (flags & HF_MG_FINE_F16) && mg->half;
   if (mg->sell && nlev < 2) {
     delete mg;
     HF_REQUIRE(false, "hf_mg_create: a SELL fine level needs at least one coarser level");
@@ -2016,6 +2144,14 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
     rc |= alloc(&mg->B, kSt * nb);
   }
   if (!rc) rc |= alloc(&mg->part, kMaxGrid + kMgMaxLevels);
+  if (!rc && mg->f16) {
+    rc |= alloc(&mg->scl, 4);
+    if (!rc && hipMalloc((void**)&mg->lv[0].S16,
+                         (size_t)kHalfSlots * kF16Words * mg->lv[0].g.n * sizeof(__half2)) != hipSuccess) {
+      set_error("hf_mg_create: hipMalloc failed");
+      rc = -2;
+    }
+  }
   if (!rc) rc |= alloc(&mg->coef, (int64_t)kMgMaxLevels * kCoefStride);
   mg->nd = (int)(3 * mg->lv[mg->nlev - 1].g.n);
   if (!rc) {
@@ -2117,9 +2253,14 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
     mg_check_lattice<<<dim3(gb0, 27), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_colidx(p), pattern_blkpos(p),
                                                         pattern_sell_pos(p), pattern_slice_ptr(p), pattern_R(p), g0,
                                                         mg->flags);
-  if (S0)
+  // the FP32 fine stencil the set-up reads: with the FP16 copy serving the
+  // smoother, the caller's own (no copy); otherwise the hierarchy's copy
+  const float* S0p = (S0 && mg->f16) ? S0 : mg->lv[0].S;
+  if (S0 && !mg->f16)
     HF_CHECK_HIP(hipMemcpyAsync(mg->lv[0].S, S0, (size_t)kHalfSlots * 9 * g0.n * sizeof(float),
                                 hipMemcpyDeviceToDevice, s));
+  else if (S0)
+    HF_REQUIRE(((uintptr_t)S0 & 15) == 0, "hf_mg_setup_stencil_f32: stencil not 16-byte aligned");
   else if (mg->half)
     mg_fine_stencil<true><<<dim3(gb0, kHalfSlots), kMgBlock, 0, s>>>(pattern_rowptr(p), pattern_sell_pos(p),
                                                                      pattern_slice_ptr(p), pattern_R(p), vals, mask,
@@ -2134,7 +2275,7 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
     const int gg = (int)std::min<int64_t>(ceil_div(f.n, kMgBlock), kMaxGrid);
     auto level = [&](auto half_tag) {
       constexpr bool H = decltype(half_tag)::value;
-      const float* Sp = mg->lv[l].S;
+      const float* Sp = l == 0 ? S0p : mg->lv[l].S;
       mg_block_dinv<H><<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(Sp, l == 0 ? mask : nullptr, f.n,
                                                                               mg->lv[l].Dinv);
       mg_gersh<H><<<gg, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, l == 0 ? mask : nullptr, mg->part);
@@ -2158,11 +2299,20 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   const Lat gl = mg->lv[mg->nlev - 1].g;
   mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
   if (mg_coefficients(mg, s)) return -2;
+  if (mg->f16) {
+    const int64_t cnt = (int64_t)kHalfSlots * 9 * g0.n;
+    const int ga = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(cnt / 4, kMgBlock), 1), kMaxGrid);
+    mg_absmax<<<ga, kMgBlock, 0, s>>>(S0p, cnt, mg->part);
+    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, ga, mg->scl + 2);
+    mg_f16_scale<<<1, kWave, 0, s>>>(mg->scl);
+    mg_to_f16<<<dim3(gb0, kHalfSlots), kMgBlock, 0, s>>>(S0p, g0.n, mg->scl, mg->lv[0].S16);
+  }
   if (mg->lmode) {
     const MgLevel& F = mg->lv[0];
     auto sd = F.half ? mg_scalar_dinv<true> : mg_scalar_dinv<false>;
-    sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(F.S, mask, F.g.n, mg->sdinv);
+    sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(S0p, mask, F.g.n, mg->sdinv);
   }
+  mg->s0 = mg->sell ? nullptr : S0p;
   HF_LAUNCH_CHECK();
   mg->mask = mask;
   // The host waits for the set-up only to check it: a pattern's first set-up (its
@@ -2294,30 +2444,33 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     const bool lanes = V.g.n <= kLaneNodes;
     const int gl = lane_grid(V.g.n);
     int derr = 0;
-    auto down = [&](auto half_tag) {
+    // Sx: the level's stencil, FP32 or (the fine level with HF_MG_FINE_F16) FP16
+    // words with scale scl
+    auto down = [&](auto half_tag, const auto* Sx, const float* scl) {
       constexpr bool H = decltype(half_tag)::value;
+      using ST = std::remove_cv_t<std::remove_pointer_t<decltype(Sx)>>;
       for (int j = 1; j < k; ++j) {
         if (timed && l == 0 && j == 1) {
           hipEvent_t e0, e1;
           if (prof_events(&e0, &e1)) { derr = -2; return; }
-          auto step = lanes ? mg_poly_step_lanes<H> : mg_poly_step<H>;
-          hipExtLaunchKernelGGL(step, dim3(lanes ? gl : gg), dim3(kMgBlock), 0, s, e0, e1, 0, (const float*)V.S,
+          auto step = lanes ? mg_poly_step_lanes<H, ST> : mg_poly_step<H, ST>;
+          hipExtLaunchKernelGGL(step, dim3(lanes ? gl : gg), dim3(kMgBlock), 0, s, e0, e1, 0, Sx, scl,
                                 (const float*)V.Dinv, V.g, cf, j, res, (const float*)din, V.res, dout, cur[l], sc);
         } else if (lanes) {
-          mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, res, din, V.res, dout,
-                                                        cur[l], sc);
+          mg_poly_step_lanes<H, ST><<<gl, kMgBlock, 0, s>>>(Sx, scl, V.Dinv, V.g, cf, j, res, din, V.res, dout,
+                                                            cur[l], sc);
         } else {
-          mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, res, din, V.res, dout,
-                                                  cur[l], sc);
+          mg_poly_step<H, ST><<<gg, kMgBlock, 0, s>>>(Sx, scl, V.Dinv, V.g, cf, j, res, din, V.res, dout,
+                                                      cur[l], sc);
         }
         res = V.res;
         float* tmp = din; din = dout; dout = tmp;
       }
       const float* rmask = l == 0 ? mg->mask : nullptr;
       if (lanes)
-        mg_poly_resid_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.g, res, din, rmask, dout, sc);
+        mg_poly_resid_lanes<H, ST><<<gl, kMgBlock, 0, s>>>(Sx, scl, V.g, res, din, rmask, dout, sc);
       else
-        mg_poly_resid<H><<<gg, kMgBlock, 0, s>>>(V.S, V.g, res, din, rmask, dout, sc);
+        mg_poly_resid<H, ST><<<gg, kMgBlock, 0, s>>>(Sx, scl, V.g, res, din, rmask, dout, sc);
     };
     if (l == 0 && mg->sell) {
       // SELL fine level: each pass is y = A v, then the node-wise update
@@ -2332,10 +2485,12 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       }
       mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
       mg_sell_resid_update<<<gg, kMgBlock, 0, s>>>(V.g.n, res, mg->y, dout, sc);
+    } else if (l == 0 && mg->f16) {
+      down(std::true_type{}, (const __half2*)V.S16, (const float*)mg->scl);
     } else if (V.half) {
-      down(std::true_type{});
+      down(std::true_type{}, (const float*)V.S, (const float*)nullptr);
     } else {
-      down(std::false_type{});
+      down(std::false_type{}, (const float*)V.S, (const float*)nullptr);
     }
     if (derr) return derr;
     const MgLevel& C = mg->lv[l + 1];
@@ -2370,8 +2525,13 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     if (l == 0 && skip_fine_post) break;
     // the learned step: res = M (r - A e), then e += D^-1 res + s D^-1/2 mask N(res)
     auto learned_step = [&]() -> int {
-      auto resid = V.half ? mg_poly_resid<true> : mg_poly_resid<false>;
-      resid<<<gg, kMgBlock, 0, s>>>(V.S, V.g, rin(0), cur[0], mg->mask, V.res, sc);
+      if (mg->f16) {
+        mg_poly_resid<true, __half2><<<gg, kMgBlock, 0, s>>>(V.S16, mg->scl, V.g, rin(0), cur[0], mg->mask, V.res,
+                                                             sc);
+      } else {
+        auto resid = V.half ? mg_poly_resid<true> : mg_poly_resid<false>;
+        resid<<<gg, kMgBlock, 0, s>>>(V.S, nullptr, V.g, rin(0), cur[0], mg->mask, V.res, sc);
+      }
       return lpc_smoother_apply(mg->lsm, V.res, mg->sdinv, mg->mask, cur[0], sc ? &sc->done : nullptr, s);
     };
     if (l == 0 && mg->lmode == 1) {
@@ -2382,21 +2542,22 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     int perr = 0;
     const bool lanes = V.g.n <= kLaneNodes;
     const int gl = lane_grid(V.g.n);
-    auto up = [&](auto half_tag) {
+    auto up = [&](auto half_tag, const auto* Sx, const float* scl) {
       constexpr bool H = decltype(half_tag)::value;
-      auto post = lanes ? mg_post_first_lanes<H> : mg_post_first<H>;
+      using ST = std::remove_cv_t<std::remove_pointer_t<decltype(Sx)>>;
+      auto post = lanes ? mg_post_first_lanes<H, ST> : mg_post_first<H, ST>;
       const int gp = lanes ? gl : gg;
-      post<<<gp, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, rin(l), cur[l], V.res, V.da, oth[l], sc);
+      post<<<gp, kMgBlock, 0, s>>>(Sx, scl, V.Dinv, V.g, cf, rin(l), cur[l], V.res, V.da, oth[l], sc);
       float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
       float* din = V.da;
       float* dout = V.db;
       for (int j = 1; j < k; ++j) {
         if (lanes)
-          mg_poly_step_lanes<H><<<gl, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, V.res, din, V.res, dout,
-                                                        cur[l], sc);
+          mg_poly_step_lanes<H, ST><<<gl, kMgBlock, 0, s>>>(Sx, scl, V.Dinv, V.g, cf, j, V.res, din, V.res, dout,
+                                                            cur[l], sc);
         else
-          mg_poly_step<H><<<gg, kMgBlock, 0, s>>>(V.S, V.Dinv, V.g, cf, j, V.res, din, V.res, dout,
-                                                  cur[l], sc);
+          mg_poly_step<H, ST><<<gg, kMgBlock, 0, s>>>(Sx, scl, V.Dinv, V.g, cf, j, V.res, din, V.res, dout,
+                                                      cur[l], sc);
         float* t2 = din; din = dout; dout = t2;
       }
     };
@@ -2422,10 +2583,12 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
                                                     cur[l], sc);
         float* t2 = din; din = dout; dout = t2;
       }
+    } else if (l == 0 && mg->f16) {
+      up(std::true_type{}, (const __half2*)V.S16, (const float*)mg->scl);
     } else if (V.half) {
-      up(std::true_type{});
+      up(std::true_type{}, (const float*)V.S, (const float*)nullptr);
     } else {
-      up(std::false_type{});
+      up(std::false_type{}, (const float*)V.S, (const float*)nullptr);
     }
     if (perr) return perr;
     if (l == 0 && mg->lmode == 2 && learned_step()) return -2;
@@ -2459,7 +2622,7 @@ extern "C" int hf_mg_set_learned(hf_mg* mg, int32_t mode, const float* const* we
   if (lpc_smoother_load(mg->lsm, weights, s)) return -2;
   if (mg->ready) {
     auto sd = F.half ? mg_scalar_dinv<true> : mg_scalar_dinv<false>;
-    sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(F.S, mg->mask, F.g.n, mg->sdinv);
+    sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(mg->s0, mg->mask, F.g.n, mg->sdinv);
     HF_LAUNCH_CHECK();
   }
   mg->lmode = mode;
@@ -2485,13 +2648,15 @@ extern "C" int hf_mg_copy_stencil(const hf_mg* mg, int32_t level, float* dst, hf
   HF_REQUIRE(mg->ready, "hf_mg_copy_stencil: hf_mg_setup_f32 has not run");
   HF_REQUIRE(!(level == 0 && mg->sell), "hf_mg_copy_stencil: the fine level is the tangent itself (SELL)");
   const int64_t nv = (int64_t)kSt * mg->lv[level].g.n;
+  // the fine level: the FP32 stencil of the last set-up (the caller's buffer when it gave one)
+  const float* src = level == 0 ? mg->s0 : mg->lv[level].S;
   if (mg->lv[level].half) {
     const Lat gl = mg->lv[level].g;
     mg_expand_half<<<dim3((unsigned)ceil_div(gl.n, kMgBlock), 27), kMgBlock, 0, (hipStream_t)stream>>>(
-        mg->lv[level].S, gl, dst);
+        src, gl, dst);
     HF_LAUNCH_CHECK();
   } else {
-    HF_CHECK_HIP(hipMemcpyAsync(dst, mg->lv[level].S, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice,
+    HF_CHECK_HIP(hipMemcpyAsync(dst, src, (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice,
                                 (hipStream_t)stream));
   }
   return 0;

@@ -13,7 +13,9 @@ re-runs the set-up when the tangent changes and implements the fallback.
     coarsest     <= 88 DOF, dense FP64 inverse
     storage      FP32; the fine-level stencil of a symmetric operator half-stored
                  (diagonal + 13 upper offsets, the lower blocks read as the transposes
-                 of the neighbours'; fine_full=True keeps all 27 slots)
+                 of the neighbours'; fine_full=True keeps all 27 slots); with
+                 fine_f16 the fine level's smoothing passes read an FP16 copy of it
+                 (scaled by a power of two; the outer CG keeps FP32)
     cycle        V-cycle, `degree` smoothing steps per side: Chebyshev in Dinv A on
                  [lam/ratio, lam], lam the Gershgorin bound of Dinv A (Dinv: inverse
                  3x3 diagonal blocks), or damped block Jacobi (omega)
@@ -106,16 +108,20 @@ class LatticeTangent:
 
 class Multigrid:
     def __init__(self, grid, smoother="chebyshev", degree=2, param=None, device=None, fine_full=False,
-                 fine_sell=False):
+                 fine_sell=False, fine_f16=False):
         """fine_sell: the fine level is the tangent's SELL matrix (HEX27 lattices);
-        fine_full: store all 27 fine stencil slots (a non-symmetric operator)."""
+        fine_full: store all 27 fine stencil slots (a non-symmetric operator);
+        fine_f16: the fine level's smoothing passes read an FP16 copy of its
+        half-stored stencil (HF_MG_FINE_F16; ignored with fine_full or fine_sell)."""
         _lib.load()
         self.grid = tuple(int(v) for v in grid)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.fine_full = bool(fine_full)
         self.fine_sell = bool(fine_sell)
+        self.fine_f16 = bool(fine_f16) and not (self.fine_full or self.fine_sell)
         h = ctypes.c_void_p()
-        flags = (_lib.HF_MG_FINE_FULL if self.fine_full else 0) | (_lib.HF_MG_FINE_SELL if self.fine_sell else 0)
+        flags = (_lib.HF_MG_FINE_FULL if self.fine_full else 0) | (_lib.HF_MG_FINE_SELL if self.fine_sell else 0) | \
+            (_lib.HF_MG_FINE_F16 if self.fine_f16 else 0)
         _lib.call("hf_mg_create", self.grid[0], self.grid[1], self.grid[2], flags, ctypes.byref(h))
         self.handle = h
         self.smoother_cfg = None
@@ -139,7 +145,7 @@ class Multigrid:
 
     @classmethod
     def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None, fine_full=False, model=None,
-                    learned_mode="replace"):
+                    learned_mode="replace", fine_f16=True):
         """The problem's cached hierarchy (one per lattice) with the given smoother.
         smoother 'learned': Chebyshev everywhere, and on the fine level the learned
         network `model` (a LearnedPC) as the post-smoother (learned_mode 'replace') or
@@ -148,8 +154,11 @@ class Multigrid:
         sell = problem.fe.ele_type == "HEX27"
         poly = "chebyshev" if smoother == "learned" else smoother
         mg = getattr(problem, "_mg", None)
-        if mg is None or mg.grid != grid or mg.fine_full != bool(fine_full) or mg.fine_sell != sell:
-            mg = cls(grid, poly, degree, param, device=problem.device, fine_full=fine_full, fine_sell=sell)
+        f16 = bool(fine_f16) and not (fine_full or sell)
+        if mg is None or mg.grid != grid or mg.fine_full != bool(fine_full) or mg.fine_sell != sell or \
+                mg.fine_f16 != f16:
+            mg = cls(grid, poly, degree, param, device=problem.device, fine_full=fine_full, fine_sell=sell,
+                     fine_f16=f16)
             problem._mg = mg
         else:
             mg.set_smoother(poly, degree, param)
@@ -338,12 +347,13 @@ def _workspace(device, nbytes):
 
 
 def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=1, smoother="chebyshev", degree=2,
-                        param=None, rebuild="solve", fine_full=False, stats=None, model=None, learned_mode="replace"):
+                        param=None, rebuild="solve", fine_full=False, stats=None, model=None, learned_mode="replace",
+                        fine_f16=True):
     """Multigrid FCG; on breakdown or non-finite values Jacobi-PCG solves to the same
     absolute target from the better of the FCG iterate and x0.  `stats` (the Newton
     solve's KrylovStats) identifies the solve for rebuild='solve'.  Returns (x, info, fallback)."""
     from .solver import krylov
-    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_full, model, learned_mode)
+    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_full, model, learned_mode, fine_f16)
     x_start = (torch.zeros(A.shape[0], dtype=torch.float32, device=b.device) if x0 is None else
                x0.to(torch.float32).clone())
     x, info = mg.solve(A, b, x_start, rtol, atol, maxiter, rel_r0, poll, stats, problem=problem, rebuild=rebuild,

@@ -18,10 +18,11 @@ a legacy flat dict, solver.py:1014-1068).  Linear keys:
   * ``learned`` {model: LearnedPC, tol, atol, maxiter, stall}: flexible CG with
     the learned CNN preconditioner on the FP32 tangent (precision 'mixed'),
     Jacobi-PCG as its fallback; the Newton residual stays the FP64 judge.
-  * ``mg`` {tol, atol, maxiter, smoother, degree, param, rebuild, fine_full, poll, model,
-    learned_mode}: flexible CG preconditioned by geometric-multigrid V-cycles on the
-    FP32 tangent (precision 'mixed'; box_mesh HEX8 lattices, multigrid.py), Jacobi-PCG
-    as its fallback; smoother 'learned' with model=LearnedPC makes the learned
+  * ``mg`` {tol, atol, maxiter, smoother, degree, param, rebuild, fine_full, fine_f16,
+    poll, model, learned_mode}: flexible CG preconditioned by geometric-multigrid V-cycles
+    on the FP32 tangent (precision 'mixed'; box_mesh HEX8 lattices, multigrid.py), Jacobi-PCG
+    as its fallback; fine_f16 (default True) smooths the fine level with an FP16 copy of
+    its stencil; smoother 'learned' with model=LearnedPC makes the learned
     network the fine level's post-smoother;
     rebuild 'solve' (default: the hierarchy of a Newton solve's first tangent
     preconditions its later steps), 'always' or 'never'.
@@ -292,6 +293,7 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
                                           smoother=o.get("smoother", "chebyshev"), degree=o.get("degree", 2),
                                           param=o.get("param"), rebuild=o.get("rebuild", "solve"),
                                           fine_full=o.get("fine_full", False), stats=stats, model=o.get("model"),
+                                          fine_f16=o.get("fine_f16", True),
                                           learned_mode=o.get("learned_mode", "replace"))
         if stats is not None:
             stats.calls[-1]["fallback"] = fb
@@ -371,7 +373,7 @@ def _stencil_multigrid(problem, lin_opts, adt):
     except NotImplementedError:
         return None
     return Multigrid.for_problem(problem, o.get("smoother", "chebyshev"), o.get("degree", 2), o.get("param"), False,
-                                 o.get("model"), o.get("learned_mode", "replace"))
+                                 o.get("model"), o.get("learned_mode", "replace"), o.get("fine_f16", True))
 
 
 def solver(problem, solver_options={}):

@@ -187,7 +187,10 @@ def smoother_ref(A, D, r, e, kind, degree, param, lam):
     return e
 
 
-def vcycle_ref(As, dims, mask, r, kind, degree, param, lams):
+def vcycle_ref(As, dims, mask, r, kind, degree, param, lams, A0_smooth=None):
+    """One V-cycle with scipy matrices; A0_smooth (optional) is the operator the fine
+    level's smoothing and restricted residual use (its block inverses, bound and the
+    coarse levels still come from As[0])."""
     L = len(As)
     Dinv = [block_dinv(As[l], mask if l == 0 else None) for l in range(L - 1)]
     Ps = [interp_3d(dims[l]) for l in range(L - 1)]
@@ -197,7 +200,7 @@ def vcycle_ref(As, dims, mask, r, kind, degree, param, lams):
     def V(l, rl):
         if l == L - 1:
             return Ainv @ rl
-        A = As[l]
+        A = A0_smooth if (l == 0 and A0_smooth is not None) else As[l]
         e = smoother_ref(A, Dinv[l], rl, np.zeros_like(rl), kind, degree, param, lams[l])
         ec = V(l + 1, Ps[l].T @ (rl - A @ e))
         e = e + (mask if l == 0 else 1.0) * (Ps[l] @ ec)
@@ -228,6 +231,72 @@ def test_vcycle_vs_scipy(jfa, kind, degree, param, full):
     z = mg.apply(r).double().cpu().numpy()
     zr, _ = vcycle_ref(As, dims, m, r.double().cpu().numpy(), kind, degree, param, lams)
     np.testing.assert_allclose(z, zr, rtol=0, atol=1e-4 * np.abs(zr).max())
+
+
+def round_f16(A):
+    """The fine stencil as the FP16 smoothing passes see it: scaled by the power of two
+    that puts max |A| in [2^14, 2^15), rounded to FP16 (nearest even), scaled back."""
+    a = A.tocsr().copy()
+    _, e = np.frexp(np.abs(a.data).max())
+    sc = 2.0 ** (14 - (int(e) - 1))
+    a.data = (a.data * sc).astype(np.float32).astype(np.float16).astype(np.float64) / sc
+    return a
+
+
+@pytest.mark.parametrize("cells", [(6, 5, 7), (8, 6, 7)])
+def test_vcycle_f16_vs_scipy(jfa, cells):
+    """HF_MG_FINE_F16: the fine level's smoothing passes and the residual restricted
+    from it use the FP16-rounded stencil (packed 5 half2 words per block, lower
+    blocks read transposed from the neighbour, power-of-two scale), the block
+    inverses, bounds and coarse levels the FP32 one."""
+    say("mg vcycle fp16 fine level", cells)
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    prob = box_problem(jfa, cells)
+    A = tangent(jfa, prob)
+    mask = MG.dirichlet_mask(prob)
+    mg16 = MG.Multigrid(MG.lattice(prob), fine_f16=True)
+    mg16.setup(A, mask)
+    dims = mg16.levels()
+    m = mask.double().cpu().numpy()
+    A0 = stencil_to_csr(mg16.stencil(0).cpu().numpy(), dims[0])
+    As = galerkin_levels(A0, dims)
+    Dinv = [block_dinv(As[l], m if l == 0 else None) for l in range(len(As) - 1)]
+    lams = [abs(Dinv[l] @ As[l]).sum(axis=1).max() for l in range(len(As) - 1)]
+    np.testing.assert_allclose(mg16.bounds(), lams, rtol=1e-5)
+    g = torch.Generator(device=DEV).manual_seed(11)
+    r = torch.randn(prob.N, generator=g, device=DEV) * mask
+    rr = r.double().cpu().numpy()
+    z = mg16.apply(r).double().cpu().numpy()
+    z16, _ = vcycle_ref(As, dims, m, rr, "chebyshev", 2, 8.0, lams, A0_smooth=round_f16(A0))
+    z32, _ = vcycle_ref(As, dims, m, rr, "chebyshev", 2, 8.0, lams)
+    e16 = np.abs(z - z16).max() / np.abs(z16).max()
+    e32 = np.abs(z - z32).max() / np.abs(z32).max()
+    say("  vs the FP16-stencil cycle %.2e, vs the FP32 one %.2e" % (e16, e32))
+    assert e16 <= 1e-5 and e16 < 0.2 * e32
+
+
+def test_mg_f16_node_kernels_newton(jfa):
+    """A lattice above the lane-kernel size (74,333 nodes): the FP16 fine level in
+    the thread-per-node passes.  Mixed Newton with the multigrid reaches the FP64
+    residual target with and without it, to the same solution."""
+    say("mg fp16 fine level, node kernels")
+    lpc = __import__("jax_fem_amd.learned_pc", fromlist=["x"])
+    cells = (48, 40, 36)
+    E = lpc.heterogeneous_modulus(cells, 6)
+    out = {}
+    for f16 in (False, True):
+        prob = box_problem(jfa, cells, material="lin", E_cell=E)
+        u = jfa.solver(prob, {"newton": {"precision": "mixed", "tol": 0.0, "rel_tol": 1e-9,
+                                         "linear": {"mg": {"fine_f16": f16}}}})[0]
+        info = prob.solve_info
+        say("  fine_f16 %s: krylov %s, rel %.2e, fallbacks %d" % (f16, info["krylov_iters"], info["rel"],
+                                                                   info["fallbacks"]))
+        assert info["rel"] <= 1e-9 and info["fallbacks"] == 0
+        out[f16] = (u, int(info["krylov_iters"]))
+    rel = (torch.linalg.norm(out[True][0] - out[False][0]) / torch.linalg.norm(out[False][0])).item()
+    say("  rel-L2 between the two", rel)
+    assert rel <= 1e-7
+    assert out[True][1] <= out[False][1] + 3
 
 
 def test_mg_fcg_converges(jfa):
