@@ -597,6 +597,14 @@ def fine_model_bytes(n0):
 
 
 FINE_NODE_BYTES = 180 + 388 + 328 + 12 + 36 + 376 + 388 + 564
+# the same with matrix-free fine-level rows (Problem.cell_operator): no stencil bytes, a
+# 4-byte cell modulus per node instead: update 180, pre-smoothing step 112, residual 52,
+# restriction 12, prolongation 36, post-smoothing start 100 and step 112, outer product 64
+FINE_NODE_BYTES_MF = 180 + 112 + 52 + 12 + 36 + 100 + 112 + 64
+# flop of one matrix-free smoothing step per node: the row (8 cells x 7 blocks x 9 FMA,
+# 8 x 3 scale FMA: 1,056 flop, and 26 x 3 neighbour differences) and the update (33)
+MF_STEP_FLOP = 2 * (8 * 7 * 9 + 8 * 3) + 26 * 3 + 33
+MF_STEP_BYTES = 112
 
 
 def main():
@@ -719,11 +727,27 @@ def main():
     tag = {"hetero-elastic-hex8-128x128x64": "hetero_hex8_128x128x64", "neohookean-hex8-64": "hex8_64",
            "neohookean-hex27-48": "hex27_48", "simp-hex8-96x48x48": "simp_hex8_96x48x48",
            "elastic-hex8-20": "elastic_hex8_20"}[wl]
+    n0 = prob.N // 3
+    kflop = None
+    # the fine level ran matrix-free when the hierarchy holds a cell operator
+    mf_on = use_mg and getattr(getattr(prob, "_mg", None), "_cell_op", None) is not None
     if use_mg and prob.fe.ele_type == "HEX27":
         # the fine level is the SELL tangent: one smoothing product y = A v
         kbytes = 4 * nnz + 4 * prob.pattern.nblocks + 8 * N
         kname = "mg_sell_spmv (fine-level smoothing product y = A v, SELL column-form node blocks)"
         pmc = "mg_sell_spmv_%s_pmc.md" % tag
+    elif use_mg and mf_on:
+        # one fine-level Chebyshev smoothing step with matrix-free rows
+        # (mg_poly_step<true, MfOp>): per node the 8 cells' 7 off-diagonal blocks of the
+        # unit cell matrix (48 values in scalar registers) applied to the 26 neighbour
+        # differences, scaled by each cell's modulus; bytes: the vectors (res_in, d_in
+        # read, e read and written, res_out, d_out written: 6 x 12 B), the block inverse
+        # (36 B) and the cell modulus (4 B)
+        kbytes = n0 * MF_STEP_BYTES
+        kflop = n0 * MF_STEP_FLOP
+        kname = ("mg_poly_step<true, MfOp> (fine-level Chebyshev smoothing step, matrix-free rows: one "
+                 "reflection-symmetric cell matrix applied to neighbour differences, per-cell modulus; FP32 FMA)")
+        pmc = "mg_poly_step_mf_%s_pmc.md" % tag
     elif use_mg:
         # one fine-level Chebyshev smoothing step (mg_poly_step<true, __half2>): the
         # symmetric 27-point node-block stencil stored as its diagonal and 13 upper
@@ -742,6 +766,8 @@ def main():
         kname = "cg_spmv_kernel<float,3> (SELL column-form node-block SpMV w = A u fused with u.w)"
         pmc = "cg_spmv_%s_pmc.md" % tag
     achieved = kbytes / (kern_us * 1e-6) / 1e9
+    if kflop is not None:
+        achieved_tf = kflop / (kern_us * 1e-6) / 1e12
     rel_m = max(i["rel"] for i in infos_m)
     fcg_iters = sum(i["krylov_iters"] for i in infos_m)
     log("%s: %.4f s/step, newton %s, krylov %s, kernel %.1f us (%d samples) -> %.0f GB/s" % (
@@ -816,12 +842,13 @@ def main():
         counts = {"updates": sum(k + 1 for k in ref_out["newton_iters"]), "iters": sum(ref_out["krylov_iters"])}
         cpu = cpu_baseline(wl, N, nload, counts)
         log("cpu baseline %.1f s of sampling: %s" % (time.perf_counter() - t0, cpu["sample"]))
-    n0 = prob.N // 3
     whole = None
     if use_mg and prob.fe.ele_type == "HEX8":
-        wb = fine_model_bytes(n0) * fcg_iters
-        whole = {"model": "fine-level flexible-CG traffic only (%d B per fine node per iteration; coarse levels, "
-                          "tangent assembly and set-up not counted: a lower bound)" % FINE_NODE_BYTES,
+        nb = FINE_NODE_BYTES_MF if mf_on else FINE_NODE_BYTES
+        wb = n0 * nb * fcg_iters
+        whole = {"model": "fine-level flexible-CG traffic only (%d B per fine node per iteration%s; coarse levels, "
+                          "tangent assembly and set-up not counted: a lower bound)" % (
+                              nb, ", matrix-free rows" if mf_on else ""),
                  "bytes": wb, "wall_s": wall_m, "achieved": wb / wall_m / 1e9, "unit": "GB/s",
                  "frac": wb / wall_m / 1e9 / 8000.0}
     out = {
@@ -842,10 +869,18 @@ def main():
                    "wall_to_tol_s": wall_m, "max_rel_residual": rel_m,
                    "newton_iters": [i["newton_iters"] for i in infos_m],
                    "krylov_iters": [i["krylov_iters"] for i in infos_m]},
-        "roofline": {"bound": "hbm", "kernel": kname,
-                     "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-                     "traffic": _pmc_traffic(pmc) if full_size else None, "avg_launch_us": kern_us,
-                     "launches_timed": int(nl.value), "algorithmic_bytes": kbytes},
+        "roofline": ({"bound": "hbm", "kernel": kname,
+                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                      "traffic": _pmc_traffic(pmc) if full_size else None, "avg_launch_us": kern_us,
+                      "launches_timed": int(nl.value), "algorithmic_bytes": kbytes} if kflop is None else
+                     {"bound": "mfma", "kernel": kname,
+                      "achieved": achieved_tf, "peak": 157.3, "unit": "TFLOP/s", "frac": achieved_tf / 157.3,
+                      "traffic": _pmc_traffic(pmc) if full_size else None, "avg_launch_us": kern_us,
+                      "launches_timed": int(nl.value), "algorithmic_flop": kflop, "algorithmic_bytes": kbytes,
+                      "hbm_achieved_GBps": achieved, "hbm_frac": achieved / 8000.0,
+                      "note": "FP32 FMA on the vector ALU (no matrix cores): the compute bound against MI355X's "
+                              "FP32 peak; without packed-FP32 instructions (not generated here) the vector ALU "
+                              "issues at most half of it, 78.6 TFLOP/s"}),
         "whole_solve": whole,
         "pcg_mixed": mixed_out,
         "learned_pc": learned_out,

@@ -310,6 +310,32 @@ int hf_mg_levels(const hf_mg* mg, int32_t* nlev, int32_t* dims, int32_t max_leve
 /* after a set-up: recomputes the coefficients on the default stream and
  * synchronises the device */
 int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float param);
+/* post-smoothing steps on the levels run by grid kernels, 1..degree (0: the
+ * smoother's degree, the default): the first steps of the same recurrence, i.e.
+ * the lower-degree Chebyshev polynomial on the same interval.  The flexible CG
+ * accepts the then non-symmetric cycle. */
+int hf_mg_set_post_degree(hf_mg* mg, int32_t degree);
+/* Matrix-free fine level: for a box lattice whose cells share one geometry and one
+ * linear law up to a per-cell factor, the operator is A = sum_c cell_scale[c] K,
+ * K the 24x24 cell matrix at unit scale.  k (host, 576 floats): 3x3 block (q, q')
+ * of K, row-major, at k[(8 q + q') * 9], corners q = 4 dx + 2 dy + dz (dx, dy, dz in
+ * {0, 1}); cell_scale (device, (nx-1)(ny-1)(nz-1) in box cell order
+ * (cx (ny-1) + cy)(nz-1) + cz) is kept by reference.  Once set, the fine level's
+ * smoothing passes, the flexible CG's products and hf_mg_stencil_matvec_f32 with a
+ * null stencil compute each row from the node's 8 cells, applied to neighbour
+ * differences (504 FMAs) instead of reading a stencil.  K must be symmetric under
+ * the cell's mid-plane reflections (block (a, b) = R_a block (0, a^b) R_a) and its
+ * block rows must sum to zero (translations carry no force); both are checked.
+ * The set-up still reads the assembled stencil (block inverses, bound, coarse
+ * levels), and the caller vouches that both are the same operator.  A new k
+ * synchronises the device once; cell_scale NULL switches it off.  No reference
+ * counterpart (the reference assembles every tangent, solver.py:466-515). */
+int hf_mg_set_cell_operator(hf_mg* mg, const float* k, const float* cell_scale);
+/* the half-stored level-0 stencil (hf_mg_stencil_size floats) of the cell operator,
+ * for hf_mg_setup_stencil_f32: replaces the element Jacobians + hf_mg_assemble_hex8_f32
+ * when the tangent has that form.  A set-up with a cell operator set makes no FP16
+ * copy (HF_MG_FINE_F16): switching it off then needs a new set-up. */
+int hf_mg_assemble_cellop_f32(const hf_mg* mg, float* S, hf_stream_t stream);
 /* The learned network (the layers of hf_lpc_apply_f32: weights w1,b1,w2,b2,w3,b3 as
  * device pointers, gfield the nnode input field, kept by reference) as the fine
  * level's post-smoother: after the coarse correction one step
@@ -395,7 +421,8 @@ int hf_mg_setup_stencil_f32(hf_mg* mg, const hf_pattern* p, const float* S, cons
 int hf_mg_solve_stencil_f32(hf_mg* mg, const float* S, const float* b, float* x, int32_t flags, int32_t maxiter,
                             double rtol, double atol, int32_t poll, void* workspace, hf_krylov_info* info,
                             hf_stream_t stream);
-/* y = A x with S (the row-eliminated operator) */
+/* y = A x with S (the row-eliminated operator), or with the cell operator of
+ * hf_mg_set_cell_operator when S is NULL */
 int hf_mg_stencil_matvec_f32(const hf_mg* mg, const float* S, const float* x, float* y, hf_stream_t stream);
 
 /* ---- measurement: HIP-event timing of the Krylov SpMV kernel (the first SpMV

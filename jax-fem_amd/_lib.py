@@ -105,6 +105,9 @@ _SIGS = {
     "hf_mg_destroy": (c_i32, [vp]),
     "hf_mg_levels": (c_i32, [vp, ctypes.POINTER(c_i32), vp, c_i32]),
     "hf_mg_set_smoother": (c_i32, [vp, c_i32, c_i32, ctypes.c_float]),
+    "hf_mg_set_post_degree": (c_i32, [vp, c_i32]),
+    "hf_mg_set_cell_operator": (c_i32, [vp, vp, vp]),
+    "hf_mg_assemble_cellop_f32": (c_i32, [vp, vp, vp]),
     "hf_mg_set_learned": (c_i32, [vp, c_i32, vp, vp, vp]),
     "hf_mg_bounds": (c_i32, [vp, vp, c_i32]),
     "hf_mg_setup_f32": (c_i32, [vp, vp, vp, vp, ctypes.POINTER(c_i32), vp]),

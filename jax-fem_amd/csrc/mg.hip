@@ -45,6 +45,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 
 namespace hf {
@@ -197,9 +198,10 @@ __device__ __forceinline__ int pair_slot(int t) {
 // in offset order the second read came half a kernel later and mostly missed
 // L2 (fine-pass fetch 290 -> 210 MB at 65^3 nodes).  The pair loop is not
 // unrolled: unrolled, the compiler hoists all 27 block loads (256 VGPRs).
-template <bool HALF, typename ST = float>
-__device__ __forceinline__ void stencil_row(const ST* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
-                                            const float* __restrict__ v, float (&out)[3], float sinv = 1.f) {
+template <bool HALF, typename ST>
+__device__ __forceinline__ void stencil_row_stored(const ST* __restrict__ S, const Lat& g, int64_t n, int x, int y,
+                                                   int z, const float* __restrict__ v, float (&out)[3],
+                                                   const float* __restrict__ scl) {
   {
     float B[9];
     load_block<HALF>(S, g.n, n, 13, n, true, B);
@@ -226,8 +228,121 @@ __device__ __forceinline__ void stencil_row(const ST* __restrict__ S, const Lat&
                 (Bu[3 * i] * u0 + Bu[3 * i + 1] * u1 + Bu[3 * i + 2] * u2);
   }
   if constexpr (is_f16<ST>()) {
+    const float sinv = scl[1];
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[i] *= sinv;
+  }
+}
+
+// Matrix-free fine level (hf_mg_set_cell_operator): A = sum_c scale[c] K over the
+// lattice's cells, K the 24x24 cell matrix at unit scale in 3x3 blocks (q, q') of
+// corners q = 4 dx + 2 dy + dz (dx, dy, dz in {0, 1}).  A box cell and its law are
+// symmetric under the reflections through the cell's mid-planes, so block (a, b) =
+// R_a B_{a^b} R_a with B_q = block (0, q) and R_a the reflection of the axes where
+// corner a has a 1, and B_q^T = R_q B_q R_q: the 8 blocks' upper triangles (48
+// values, held in scalar registers) give all 576 entries.  The host checks this
+// before it accepts a matrix.
+struct MfOp {
+  float b[8 * 6];  // B_q entries (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+};
+__host__ __device__ constexpr int mf_tri(int r, int c) {
+  const int lo = r < c ? r : c, hi = r < c ? c : r;
+  return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+}
+__host__ __device__ constexpr float mf_refl(int q, int axis) { return ((q >> (2 - axis)) & 1) ? -1.f : 1.f; }
+// sign of entry (r, c) of block (a, b) relative to B_{a^b}'s stored upper-triangle value
+__host__ __device__ constexpr float mf_sign(int a, int b, int r, int c) {
+  return mf_refl(a, r) * mf_refl(a, c) * (r > c ? mf_refl(a ^ b, r) * mf_refl(a ^ b, c) : 1.f);
+}
+
+// Row of node (x, y, z): over its cells c = (x - 1 + i, y - 1 + j, z - 1 + k), in
+// which it is corner a = 7 - (4i + 2j + k), scale[c] sum_{q' != a} K(a, q')
+// (v[corner q'] - v[node]).  K's block rows sum to zero (a rigid translation
+// carries no force), so this is the row of A, with the diagonal block implied:
+// the differences of neighbouring values keep their digits where a smooth v
+// would cancel them in sum_q' K(a, q') v[q'] (the FP32 error then grows like
+// 1/h), and the operator annihilates translations exactly.  The 26 neighbours'
+// differences are formed once; a clamped neighbour meets only cells outside the
+// lattice, whose scale is taken as 0.
+__device__ __forceinline__ void mf_row(const MfOp* __restrict__ op, const float* __restrict__ cs, const Lat& g,
+                                       int x, int y, int z, const float* __restrict__ v, float (&out)[3]) {
+  // 32-bit offsets (3 n < 2^32: hf_mg_create keeps n below 2^30): per axis, the
+  // clamped neighbour coordinates' share of 3 * (lattice index) and of the cell index
+  const uint32_t nz = (uint32_t)g.nz, nyz = (uint32_t)g.ny * nz;
+  const uint32_t ncz = nz - 1, ncyz = (uint32_t)(g.ny - 1) * ncz;
+  uint32_t X[3], Y[3], Z[3], CX[2], CY[2], CZ[2];
+  bool inx[2], iny[2], inz[2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    X[d] = 3u * (uint32_t)clampi(x + d - 1, 0, g.nx - 1) * nyz;
+    Y[d] = 3u * (uint32_t)clampi(y + d - 1, 0, g.ny - 1) * nz;
+    Z[d] = 3u * (uint32_t)clampi(z + d - 1, 0, g.nz - 1);
+  }
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    inx[d] = x - 1 + d >= 0 && x - 1 + d < g.nx - 1;
+    iny[d] = y - 1 + d >= 0 && y - 1 + d < g.ny - 1;
+    inz[d] = z - 1 + d >= 0 && z - 1 + d < g.nz - 1;
+    CX[d] = (uint32_t)clampi(x - 1 + d, 0, g.nx - 2) * ncyz;
+    CY[d] = (uint32_t)clampi(y - 1 + d, 0, g.ny - 2) * ncz;
+    CZ[d] = (uint32_t)clampi(z - 1 + d, 0, g.nz - 2);
+  }
+  const float* pn = v + (X[1] + Y[1] + Z[1]);
+  const float v0 = pn[0], v1 = pn[1], v2 = pn[2];
+  float u[27][3];
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    if (o == 13) {
+      u[o][0] = u[o][1] = u[o][2] = 0.f;
+      continue;
+    }
+    const float* pm = v + (X[o / 9] + Y[(o / 3) % 3] + Z[o % 3]);   // one 64-bit address, 3 loads
+    u[o][0] = pm[0] - v0;
+    u[o][1] = pm[1] - v1;
+    u[o][2] = pm[2] - v2;
+  }
+  out[0] = out[1] = out[2] = 0.f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int i = c >> 2, j = (c >> 1) & 1, k = c & 1;
+    // the clamped index is always a valid cell: load, then zero outside the lattice
+    const float scl = cs[CX[i] + CY[j] + CZ[k]];
+    const float sc = (inx[i] && iny[j] && inz[k]) ? scl : 0.f;
+    const int qa = 7 - c;
+    float t[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int qb = 0; qb < 8; ++qb) {
+      if (qb == qa) continue;   // the node itself: difference 0
+      const int o = (i + (qb >> 2)) * 9 + (j + ((qb >> 1) & 1)) * 3 + (k + (qb & 1));
+      const float* B = op->b + (qa ^ qb) * 6;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          // the sign is a compile-time constant here: it folds into the FMA
+          const float bv = B[mf_tri(r, cc)];
+          if (mf_sign(qa, qb, r, cc) > 0.f)
+            t[r] += bv * u[o][cc];
+          else
+            t[r] -= bv * u[o][cc];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) out[r] += sc * t[r];
+  }
+}
+
+// out = row n of the level's operator times v: a stored stencil (FP32, or FP16
+// words with scl[1] the inverse scale) or the matrix-free cell operator (scl the
+// per-cell scales)
+template <bool HALF, typename ST = float>
+__device__ __forceinline__ void stencil_row(const ST* __restrict__ S, const Lat& g, int64_t n, int x, int y, int z,
+                                            const float* __restrict__ v, float (&out)[3],
+                                            const float* __restrict__ scl = nullptr) {
+  if constexpr (std::is_same<ST, MfOp>::value) {
+    mf_row(S, scl, g, x, y, z, v, out);
+  } else {
+    stencil_row_stored<HALF>(S, g, n, x, y, z, v, out, scl);
   }
 }
 
@@ -325,6 +440,57 @@ __global__ void __launch_bounds__(kMgBlock) mg_fine_stencil(const int32_t* __res
   }
 #pragma unroll
   for (int e = 0; e < 9; ++e) S[(int64_t)(so * 9 + e) * g.n + n] = out[e];
+}
+
+// The level-0 half-stored stencil from the cell operator (hf_mg_set_cell_operator),
+// thread per (node, slot = blockIdx.y): block (n, 13 + slot) = sum over the cells
+// holding n (corner a) and its neighbour at that offset (corner b) of scale[c]
+// K(a, b); the diagonal block is minus the sum of the row's others, as the
+// matrix-free rows imply.  Slots that leave the lattice hold zeros.
+__global__ void __launch_bounds__(kMgBlock) mg_assemble_cellop(const MfOp* __restrict__ op,
+                                                               const float* __restrict__ cs, Lat g,
+                                                               float* __restrict__ S) {
+  const int64_t n = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (n >= g.n) return;
+  const int slot = blockIdx.y;
+  const int o = 13 + slot;
+  const int off[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+  int p[3];
+  lat_xyz(g, n, p[0], p[1], p[2]);
+  const int np[3] = {g.nx, g.ny, g.nz};
+  bool inside = true;
+  for (int d = 0; d < 3; ++d) inside = inside && p[d] + off[d] >= 0 && p[d] + off[d] < np[d];
+  float B[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (inside) {
+    for (int c = 0; c < 8; ++c) {
+      const int cb[3] = {c >> 2, (c >> 1) & 1, c & 1};
+      bool cin = true;
+      int ab[3];
+      for (int d = 0; d < 3; ++d) {
+        const int pc = p[d] - 1 + cb[d];
+        cin = cin && pc >= 0 && pc < np[d] - 1;
+        ab[d] = 1 - cb[d];   // the node's corner in this cell
+      }
+      if (!cin) continue;
+      const float sc = cs[((int64_t)(p[0] - 1 + cb[0]) * (g.ny - 1) + (p[1] - 1 + cb[1])) * (g.nz - 1) +
+                          (p[2] - 1 + cb[2])];
+      const int a = 4 * ab[0] + 2 * ab[1] + ab[2];
+      for (int b = 0; b < 8; ++b) {
+        if (b == a) continue;
+        const int bb[3] = {b >> 2, (b >> 1) & 1, b & 1};
+        // slot 0 (the diagonal): minus every other block of the row; else the block
+        // whose corner is the neighbour
+        const bool take = slot == 0 || (bb[0] - ab[0] == off[0] && bb[1] - ab[1] == off[1] &&
+                                        bb[2] - ab[2] == off[2]);
+        if (!take) continue;
+        const float w = slot == 0 ? -sc : sc;
+        for (int r = 0; r < 3; ++r)
+          for (int cc = 0; cc < 3; ++cc) B[3 * r + cc] += w * mf_sign(a, b, r, cc) * op->b[(a ^ b) * 6 + mf_tri(r, cc)];
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) S[(int64_t)(slot * 9 + e) * g.n + n] = B[e];
 }
 
 // The same stencil on a box mesh whose cells are the lattice's (cell (i, j, k) =
@@ -765,14 +931,15 @@ __device__ __forceinline__ void poly_first_node(const float* __restrict__ Dinv, 
 // res = res_in - A d_in;  d_out = c1 d_in + c2 Dinv res;  e += d_out
 // (res_in may alias res_out: each node reads and writes only its own entries)
 template <bool HALF, typename ST>
-__device__ __forceinline__ void poly_step_node(const ST* __restrict__ S, float sinv, const float* __restrict__ Dinv,
+__device__ __forceinline__ void poly_step_node(const ST* __restrict__ S, const float* __restrict__ scl,
+                                               const float* __restrict__ Dinv,
                                                const Lat& g, float c1, float c2, const float* res_in,
                                                const float* __restrict__ d_in, float* res_out,
                                                float* __restrict__ d_out, float* __restrict__ e, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ad[3];
-  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad, sinv);
+  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad, scl);
   const float res[3] = {res_in[3 * node] - ad[0], res_in[3 * node + 1] - ad[1], res_in[3 * node + 2] - ad[2]};
   float dr[3];
   block_apply(Dinv, g.n, node, res, dr);
@@ -789,13 +956,14 @@ __device__ __forceinline__ void poly_step_node(const ST* __restrict__ S, float s
 // mask (the fine level, whose stencil may hold the Dirichlet rows) t = M (...):
 // the restriction must not see those rows
 template <bool HALF, typename ST>
-__device__ __forceinline__ void poly_resid_node(const ST* __restrict__ S, float sinv, const Lat& g,
+__device__ __forceinline__ void poly_resid_node(const ST* __restrict__ S, const float* __restrict__ scl,
+                                                const Lat& g,
                                                 const float* __restrict__ res_in, const float* __restrict__ d_in,
                                                 const float* __restrict__ mask, float* __restrict__ t, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ad[3];
-  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad, sinv);
+  stencil_row<HALF>(S, g, node, x, y, z, d_in, ad, scl);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float v = res_in[3 * node + i] - ad[i];
@@ -805,14 +973,15 @@ __device__ __forceinline__ void poly_resid_node(const ST* __restrict__ S, float 
 
 // post-smoother start: res = r - A e_in;  d = s0 Dinv res;  e_out = e_in + d
 template <bool HALF, typename ST>
-__device__ __forceinline__ void post_first_node(const ST* __restrict__ S, float sinv, const float* __restrict__ Dinv,
+__device__ __forceinline__ void post_first_node(const ST* __restrict__ S, const float* __restrict__ scl,
+                                                const float* __restrict__ Dinv,
                                                 const Lat& g, float s0, const float* __restrict__ r,
                                                 const float* __restrict__ e_in, float* __restrict__ res,
                                                 float* __restrict__ d, float* __restrict__ e_out, int64_t node) {
   int x, y, z;
   lat_xyz(g, node, x, y, z);
   float ae[3];
-  stencil_row<HALF>(S, g, node, x, y, z, e_in, ae, sinv);
+  stencil_row<HALF>(S, g, node, x, y, z, e_in, ae, scl);
   const float rr[3] = {r[3 * node] - ae[0], r[3 * node + 1] - ae[1], r[3 * node + 2] - ae[2]};
   float dr[3];
   block_apply(Dinv, g.n, node, rr, dr);
@@ -925,14 +1094,10 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_first(const float* __restric
   MG_NODE_LOOP(n, poly_first_node(Dinv, n, s0, r, d, e, node));
 }
 
-// The smoothing passes take the level's stencil as FP32 (ST = float, scl null)
-// or, on the fine level with HF_MG_FINE_F16, as FP16 words (ST = __half2) with
-// scl[1] the inverse of their scale.
-template <typename ST>
-__device__ __forceinline__ float stencil_sinv(const float* scl) {
-  if constexpr (is_f16<ST>()) return scl[1];
-  return 1.f;
-}
+// The smoothing passes take the level's operator as an FP32 stencil (ST = float,
+// scl null), on the fine level with HF_MG_FINE_F16 as FP16 words (ST = __half2,
+// scl[1] the inverse of their scale), or as the matrix-free cell operator (ST =
+// MfOp, scl the per-cell scales).
 
 template <bool HALF, typename ST = float>
 __global__ void __launch_bounds__(kMgBlock) mg_poly_step(const ST* __restrict__ S, const float* __restrict__ scl,
@@ -943,8 +1108,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_step(const ST* __restrict__ 
                                                          float* __restrict__ e, const Scalars* sc) {
   if (sc && sc->done) return;
   const float c1 = cf[1 + j], c2 = cf[1 + kMaxDegree + j];
-  const float sinv = stencil_sinv<ST>(scl);
-  MG_NODE_LOOP(g.n, poly_step_node<HALF>(S, sinv, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
+  MG_NODE_LOOP(g.n, poly_step_node<HALF>(S, scl, Dinv, g, c1, c2, res_in, d_in, res_out, d_out, e, node));
 }
 
 template <bool HALF, typename ST = float>
@@ -954,8 +1118,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_resid(const ST* __restrict__
                                                           const float* __restrict__ mask, float* __restrict__ t,
                                                           const Scalars* sc) {
   if (sc && sc->done) return;
-  const float sinv = stencil_sinv<ST>(scl);
-  MG_NODE_LOOP(g.n, poly_resid_node<HALF>(S, sinv, g, res_in, d_in, mask, t, node));
+  MG_NODE_LOOP(g.n, poly_resid_node<HALF>(S, scl, g, res_in, d_in, mask, t, node));
 }
 
 template <bool HALF, typename ST = float>
@@ -967,8 +1130,7 @@ __global__ void __launch_bounds__(kMgBlock) mg_post_first(const ST* __restrict__
                                                           float* __restrict__ e_out, const Scalars* sc) {
   if (sc && sc->done) return;
   const float s0 = cf[0];
-  const float sinv = stencil_sinv<ST>(scl);
-  MG_NODE_LOOP(g.n, post_first_node<HALF>(S, sinv, Dinv, g, s0, r, e_in, res, d, e_out, node));
+  MG_NODE_LOOP(g.n, post_first_node<HALF>(S, scl, Dinv, g, s0, r, e_in, res, d, e_out, node));
 }
 
 __global__ void __launch_bounds__(kMgBlock) mg_restrict(const float* __restrict__ t, Lat f, Lat c,
@@ -1178,28 +1340,55 @@ struct SmallCycle {
 template <bool HALF, int UNROLL, typename ST = float>
 __device__ __forceinline__ void stencil_row_group(const ST* __restrict__ S, const Lat& g, int64_t n,
                                                   const float* __restrict__ v, int q, float (&out)[3],
-                                                  float sinv = 1.f) {
+                                                  const float* __restrict__ scl = nullptr) {
   int x, y, z;
   lat_xyz(g, n, x, y, z);
-  out[0] = out[1] = out[2] = 0.f;
-#pragma unroll UNROLL
-  for (int t = 0; t < (27 + kGroup - 1) / kGroup; ++t) {
-    const int o = q + t * kGroup;
-    if (o >= 27) break;
-    const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
-    const int xx = clampi(xr, 0, g.nx - 1), yy = clampi(yr, 0, g.ny - 1), zz = clampi(zr, 0, g.nz - 1);
-    const int64_t m = lat_id(g, xx, yy, zz);
-    const float v0 = v[3 * m], v1 = v[3 * m + 1], v2 = v[3 * m + 2];
-    float B[9];
-    load_block<HALF>(S, g.n, n, o, m, xr == xx && yr == yy && zr == zz, B);
+  if constexpr (std::is_same<ST, MfOp>::value) {
+    // lane q takes cell q of the node (its 8 corners' values), as mf_row
+    const int i = q >> 2, j = (q >> 1) & 1, k = q & 1;
+    const int ncx = g.nx - 1, ncy = g.ny - 1, ncz = g.nz - 1;
+    const int px = x - 1 + i, py = y - 1 + j, pz = z - 1 + k;
+    const bool in = q < 8 && px >= 0 && px < ncx && py >= 0 && py < ncy && pz >= 0 && pz < ncz;
+    const float sc = in ? scl[((int64_t)px * ncy + py) * ncz + pz] : 0.f;
+    const int qa = 7 - (q & 7);
+    float t[3] = {0.f, 0.f, 0.f};
+    const float w0 = v[3 * n], w1 = v[3 * n + 1], w2 = v[3 * n + 2];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) out[i] += B[3 * i] * v0 + B[3 * i + 1] * v1 + B[3 * i + 2] * v2;
+    for (int qb = 0; qb < 8; ++qb) {
+      const int64_t m = lat_id(g, clampi(px + (qb >> 2), 0, g.nx - 1), clampi(py + ((qb >> 1) & 1), 0, g.ny - 1),
+                               clampi(pz + (qb & 1), 0, g.nz - 1));
+      // differences from the node's own value (the diagonal block implied, as mf_row)
+      const float uu[3] = {v[3 * m] - w0, v[3 * m + 1] - w1, v[3 * m + 2] - w2};
+      const float* B = S->b + (qa ^ qb) * 6;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) t[r] += (mf_sign(qa, qb, r, cc) * B[mf_tri(r, cc)]) * uu[cc];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) out[r] = sc * t[r];
+  } else {
+    out[0] = out[1] = out[2] = 0.f;
+#pragma unroll UNROLL
+    for (int t = 0; t < (27 + kGroup - 1) / kGroup; ++t) {
+      const int o = q + t * kGroup;
+      if (o >= 27) break;
+      const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
+      const int xx = clampi(xr, 0, g.nx - 1), yy = clampi(yr, 0, g.ny - 1), zz = clampi(zr, 0, g.nz - 1);
+      const int64_t m = lat_id(g, xx, yy, zz);
+      const float v0 = v[3 * m], v1 = v[3 * m + 1], v2 = v[3 * m + 2];
+      float B[9];
+      load_block<HALF>(S, g.n, n, o, m, xr == xx && yr == yy && zr == zz, B);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) out[i] += B[3 * i] * v0 + B[3 * i + 1] * v1 + B[3 * i + 2] * v2;
+    }
   }
 #pragma unroll
   for (int w = 1; w < kGroup; w <<= 1)
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[i] += __shfl_xor(out[i], w, kWave);
   if constexpr (is_f16<ST>()) {
+    const float sinv = scl[1];
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[i] *= sinv;
   }
@@ -1349,10 +1538,9 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_step_lanes(const ST* __restr
                                                                float* __restrict__ e, const Scalars* sc) {
   if (sc && sc->done) return;
   const float c1 = cf[1 + j], c2 = cf[1 + kMaxDegree + j];
-  const float sinv = stencil_sinv<ST>(scl);
   MG_LANE_LOOP(g.n, {
     float ad[3];
-    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad, sinv);
+    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad, scl);
     const float rr[3] = {res_in[3 * node] - ad[0], res_in[3 * node + 1] - ad[1], res_in[3 * node + 2] - ad[2]};
     if (q < 3) {
       const float dn = c1 * d_in[3 * node + q] + c2 * block_row(Dinv, g.n, node, q, rr);
@@ -1371,10 +1559,9 @@ __global__ void __launch_bounds__(kMgBlock) mg_poly_resid_lanes(const ST* __rest
                                                                 const float* __restrict__ mask,
                                                                 float* __restrict__ t, const Scalars* sc) {
   if (sc && sc->done) return;
-  const float sinv = stencil_sinv<ST>(scl);
   MG_LANE_LOOP(g.n, {
     float ad[3];
-    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad, sinv);
+    stencil_row_group<HALF, 4>(S, g, node, d_in, q, ad, scl);
     if (q < 3) {
       const float v = res_in[3 * node + q] - ad[q];
       t[3 * node + q] = mask ? v * mask[3 * node + q] : v;
@@ -1393,10 +1580,9 @@ __global__ void __launch_bounds__(kMgBlock) mg_post_first_lanes(const ST* __rest
                                                                 float* __restrict__ e_out, const Scalars* sc) {
   if (sc && sc->done) return;
   const float s0 = cf[0];
-  const float sinv = stencil_sinv<ST>(scl);
   MG_LANE_LOOP(g.n, {
     float ae[3];
-    stencil_row_group<HALF, 4>(S, g, node, e_in, q, ae, sinv);
+    stencil_row_group<HALF, 4>(S, g, node, e_in, q, ae, scl);
     const float rr[3] = {r[3 * node] - ae[0], r[3 * node + 1] - ae[1], r[3 * node + 2] - ae[2]};
     if (q < 3) {
       const float dq = s0 * block_row(Dinv, g.n, node, q, rr);
@@ -1762,7 +1948,9 @@ __global__ void __launch_bounds__(kBlock) mgcg_spmv_kernel(MatArgs m, const floa
 
 // The same two phases with the tangent as a half-stored lattice stencil (Dirichlet
 // rows included): A v = M (S v) + (1 - M) v, the row-eliminated operator.
-__global__ void __launch_bounds__(kMgBlock) mgcg_init_stencil(const float* __restrict__ S, Lat g,
+template <typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mgcg_init_stencil(const ST* __restrict__ S, const float* __restrict__ scl,
+                                                              Lat g,
                                                               const float* __restrict__ mask,
                                                               const float* __restrict__ b,
                                                               const float* __restrict__ x, float* __restrict__ r,
@@ -1773,7 +1961,7 @@ __global__ void __launch_bounds__(kMgBlock) mgcg_init_stencil(const float* __res
     int cx, cy, cz;
     lat_xyz(g, node, cx, cy, cz);
     float ax[3];
-    stencil_row<true>(S, g, node, cx, cy, cz, x, ax);
+    stencil_row<true>(S, g, node, cx, cy, cz, x, ax, scl);
     for (int i = 0; i < 3; ++i) {
       const int64_t d = node * 3 + i;
       const float mk = mask[d];
@@ -1788,7 +1976,9 @@ __global__ void __launch_bounds__(kMgBlock) mgcg_init_stencil(const float* __res
   block_partials<2>(loc, partials);
 }
 
-__global__ void __launch_bounds__(kMgBlock) mgcg_spmv_stencil(const float* __restrict__ S, Lat g,
+template <typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mgcg_spmv_stencil(const ST* __restrict__ S, const float* __restrict__ scl,
+                                                              Lat g,
                                                               const float* __restrict__ mask,
                                                               const float* __restrict__ z,
                                                               const float* __restrict__ r,
@@ -1800,7 +1990,7 @@ __global__ void __launch_bounds__(kMgBlock) mgcg_spmv_stencil(const float* __res
     int cx, cy, cz;
     lat_xyz(g, node, cx, cy, cz);
     float az[3];
-    stencil_row<true>(S, g, node, cx, cy, cz, z, az);
+    stencil_row<true>(S, g, node, cx, cy, cz, z, az, scl);
     for (int i = 0; i < 3; ++i) {
       const int64_t d = node * 3 + i;
       const float mk = mask[d];
@@ -1816,14 +2006,16 @@ __global__ void __launch_bounds__(kMgBlock) mgcg_spmv_stencil(const float* __res
 }
 
 // y = A x for the stencil form (tests and the fallback)
-__global__ void __launch_bounds__(kMgBlock) mg_stencil_matvec(const float* __restrict__ S, Lat g,
+template <typename ST = float>
+__global__ void __launch_bounds__(kMgBlock) mg_stencil_matvec(const ST* __restrict__ S, const float* __restrict__ scl,
+                                                              Lat g,
                                                               const float* __restrict__ mask,
                                                               const float* __restrict__ x, float* __restrict__ y) {
   MG_NODE_LOOP(g.n, {
     int cx, cy, cz;
     lat_xyz(g, node, cx, cy, cz);
     float ax[3];
-    stencil_row<true>(S, g, node, cx, cy, cz, x, ax);
+    stencil_row<true>(S, g, node, cx, cy, cz, x, ax, scl);
     for (int i = 0; i < 3; ++i) {
       const int64_t d = node * 3 + i;
       y[d] = mask[d] * ax[i] + (1.f - mask[d]) * x[d];
@@ -2012,14 +2204,22 @@ struct hf_mg {
   float* coef = nullptr;     // kMgMaxLevels x kCoefStride smoothing coefficients (mg_coef)
   bool half = false;         // fine-level stencil half-stored (symmetric operator)
   bool f16 = false;          // fine-level smoothing passes read an FP16 copy of the stencil
+  bool s16_valid = false;    // the copy was made by the last set-up (skipped while a cell operator is set)
   float* scl = nullptr;      // its scale, inverse scale and |S| maximum (mg_f16_scale)
   const float* s0 = nullptr; // the FP32 fine stencil of the last set-up (the caller's, or lv[0].S)
+  // matrix-free fine level (hf_mg_set_cell_operator): the cell matrix on the device,
+  // its host copy, and the per-cell scales (the caller's; null: off)
+  MfOp* mf = nullptr;
+  float mf_host[8 * 6] = {0};
+  bool mf_loaded = false;
+  const float* mf_scale = nullptr;
   bool sell = false;         // fine level is the tangent's SELL matrix (no fine stencil)
   float* y = nullptr;        // 3 n0: SELL products of the fine level
   const hf_pattern* sp = nullptr;  // the fine level's pattern and values (last set-up, caller's memory)
   const float* svals = nullptr;
   int kind = HF_MG_CHEBYSHEV;
   int degree = 2;
+  int degree_post = 0;       // post-smoothing steps on the grid levels (0: degree)
   float param = 8.f;         // Chebyshev: lam / lower end of the interval; Jacobi: omega
   const float* mask = nullptr;  // level-0 Dirichlet mask of the last set-up (caller's memory)
   const hf_pattern* checked = nullptr;  // pattern whose lattice structure was verified
@@ -2071,6 +2271,7 @@ static int mg_free(hf_mg* mg) {
   if (mg->sdinv) (void)hipFree(mg->sdinv);
   if (mg->lv[0].S16) (void)hipFree(mg->lv[0].S16);
   if (mg->scl) (void)hipFree(mg->scl);
+  if (mg->mf) (void)hipFree(mg->mf);
   return 0;
 }
 
@@ -2210,6 +2411,7 @@ extern "C" int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float
   mg->kind = kind;
   mg->degree = degree;
   mg->param = param;
+  if (mg->degree_post > degree) mg->degree_post = 0;
   ++mg->gen;  // the cycle's launch sequence depends on the degree
   if (mg->ready) {
     // new coefficients for the current levels; the set-up ran on a caller's stream
@@ -2217,6 +2419,71 @@ extern "C" int hf_mg_set_smoother(hf_mg* mg, int32_t kind, int32_t degree, float
     if (mg_coefficients(mg, nullptr)) return -2;
     HF_CHECK_HIP(hipDeviceSynchronize());
   }
+  return 0;
+}
+
+extern "C" int hf_mg_set_post_degree(hf_mg* mg, int32_t degree) {
+  HF_REQUIRE(mg, "hf_mg_set_post_degree: null handle");
+  HF_REQUIRE(degree >= 0 && degree <= mg->degree, "hf_mg_set_post_degree: %d outside 0..%d (0: the pre-smoothing "
+             "degree)", degree, mg->degree);
+  if (degree != mg->degree_post) ++mg->gen;
+  mg->degree_post = degree;
+  return 0;
+}
+
+extern "C" int hf_mg_set_cell_operator(hf_mg* mg, const float* k, const float* cell_scale) {
+  HF_REQUIRE(mg, "hf_mg_set_cell_operator: null handle");
+  if (!cell_scale) {
+    if (mg->mf_scale) {
+      ++mg->gen;   // the cycle's kernels change
+      // a set-up with the cell operator made no FP16 copy: the smoother has none until the next
+      if (mg->f16 && !mg->s16_valid) mg->ready = false;
+    }
+    mg->mf_scale = nullptr;
+    return 0;
+  }
+  HF_REQUIRE(k, "hf_mg_set_cell_operator: null cell matrix");
+  HF_REQUIRE(!mg->sell && mg->nlev > 1, "hf_mg_set_cell_operator: needs a lattice-stencil fine level and a coarser "
+                                        "level");
+  float kmax = 0.f;
+  for (int i = 0; i < 8 * 8 * 9; ++i) {
+    HF_REQUIRE(std::isfinite(k[i]), "hf_mg_set_cell_operator: non-finite cell matrix entry %d", i);
+    kmax = std::max(kmax, std::fabs(k[i]));
+  }
+  MfOp h;
+  for (int q = 0; q < 8; ++q)
+    for (int r = 0; r < 3; ++r)
+      for (int c = r; c < 3; ++c) h.b[q * 6 + mf_tri(r, c)] = k[q * 9 + 3 * r + c];   // block (0, q)
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          const float want = k[(8 * a + b) * 9 + 3 * r + c];
+          const float got = mf_sign(a, b, r, c) * h.b[(a ^ b) * 6 + mf_tri(r, c)];
+          HF_REQUIRE(std::fabs(want - got) <= 1e-5f * kmax,
+                     "hf_mg_set_cell_operator: the cell matrix is not symmetric under the cell's mid-plane "
+                     "reflections (block (%d, %d) entry (%d, %d): %g, expected %g)", a, b, r, c, (double)want,
+                     (double)got);
+        }
+  // block rows sum to zero (the rows are applied to neighbour differences)
+  for (int a = 0; a < 8; ++a)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double sum = 0.0;
+        for (int b = 0; b < 8; ++b) sum += k[(8 * a + b) * 9 + 3 * r + c];
+        HF_REQUIRE(std::fabs(sum) <= 1e-5 * kmax, "hf_mg_set_cell_operator: block row %d of the cell matrix does "
+                   "not sum to zero (entry (%d, %d): %g): not a translation-invariant operator", a, r, c, sum);
+      }
+  if (!mg->mf) HF_CHECK_HIP(hipMalloc((void**)&mg->mf, sizeof(MfOp)));
+  if (!mg->mf_loaded || std::memcmp(mg->mf_host, h.b, sizeof(h.b)) != 0) {
+    // a new cell matrix (once per geometry and law): no cycle may be reading the old one
+    HF_CHECK_HIP(hipDeviceSynchronize());
+    std::memcpy(mg->mf_host, h.b, sizeof(h.b));
+    HF_CHECK_HIP(hipMemcpy(mg->mf, &h, sizeof(MfOp), hipMemcpyHostToDevice));
+    mg->mf_loaded = true;
+  }
+  if (mg->mf_scale != cell_scale) ++mg->gen;   // a kernel argument of the captured batch
+  mg->mf_scale = cell_scale;
   return 0;
 }
 
@@ -2299,7 +2566,9 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   const Lat gl = mg->lv[mg->nlev - 1].g;
   mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
   if (mg_coefficients(mg, s)) return -2;
-  if (mg->f16) {
+  mg->s16_valid = false;
+  if (mg->f16 && !mg->mf_scale) {
+    mg->s16_valid = true;
     const int64_t cnt = (int64_t)kHalfSlots * 9 * g0.n;
     const int ga = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(cnt / 4, kMgBlock), 1), kMaxGrid);
     mg_absmax<<<ga, kMgBlock, 0, s>>>(S0p, cnt, mg->part);
@@ -2388,12 +2657,27 @@ extern "C" int hf_mg_assemble_hex8_f32(const hf_mg* mg, const hf_pattern* p, con
   return 0;
 }
 
+extern "C" int hf_mg_assemble_cellop_f32(const hf_mg* mg, float* S, hf_stream_t stream) {
+  HF_REQUIRE(mg && S, "hf_mg_assemble_cellop_f32: null argument");
+  HF_REQUIRE(mg->mf_scale, "hf_mg_assemble_cellop_f32: no cell operator (hf_mg_set_cell_operator)");
+  const Lat g = mg->lv[0].g;
+  mg_assemble_cellop<<<dim3((unsigned)ceil_div(g.n, kMgBlock), kHalfSlots), kMgBlock, 0, (hipStream_t)stream>>>(
+      mg->mf, mg->mf_scale, g, S);
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
 extern "C" int hf_mg_stencil_matvec_f32(const hf_mg* mg, const float* S, const float* x, float* y,
                                         hf_stream_t stream) {
-  HF_REQUIRE(mg && S && x && y, "hf_mg_stencil_matvec_f32: null argument");
+  HF_REQUIRE(mg && x && y, "hf_mg_stencil_matvec_f32: null argument");
+  HF_REQUIRE(S || mg->mf_scale, "hf_mg_stencil_matvec_f32: no stencil and no cell operator");
   HF_REQUIRE(mg->mask, "hf_mg_stencil_matvec_f32: no set-up (the Dirichlet mask comes from it)");
   const Lat g = mg->lv[0].g;
-  mg_stencil_matvec<<<node_grid(g.n), kMgBlock, 0, (hipStream_t)stream>>>(S, g, mg->mask, x, y);
+  if (S)
+    mg_stencil_matvec<<<node_grid(g.n), kMgBlock, 0, (hipStream_t)stream>>>(S, nullptr, g, mg->mask, x, y);
+  else
+    mg_stencil_matvec<<<node_grid(g.n), kMgBlock, 0, (hipStream_t)stream>>>(mg->mf, mg->mf_scale, g, mg->mask, x,
+                                                                             y);
   HF_LAUNCH_CHECK();
   return 0;
 }
@@ -2423,6 +2707,10 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
                   bool timed = false, bool skip_fine_post = false, bool first_done = false) {
   const int L = mg->nlev;
   const int k = mg->degree;
+  // post-smoothing: the first kp steps of the same recurrence (the Chebyshev
+  // polynomial of degree kp on the same interval: its coefficients depend only on
+  // the step, not on the degree)
+  const int kp = mg->degree_post > 0 ? mg->degree_post : k;
   float* cur[kMgMaxLevels];
   float* oth[kMgMaxLevels];
   for (int l = 0; l < L; ++l) {
@@ -2485,6 +2773,8 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       }
       mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
       mg_sell_resid_update<<<gg, kMgBlock, 0, s>>>(V.g.n, res, mg->y, dout, sc);
+    } else if (l == 0 && mg->mf_scale) {
+      down(std::true_type{}, (const MfOp*)mg->mf, mg->mf_scale);
     } else if (l == 0 && mg->f16) {
       down(std::true_type{}, (const __half2*)V.S16, (const float*)mg->scl);
     } else if (V.half) {
@@ -2525,7 +2815,10 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     if (l == 0 && skip_fine_post) break;
     // the learned step: res = M (r - A e), then e += D^-1 res + s D^-1/2 mask N(res)
     auto learned_step = [&]() -> int {
-      if (mg->f16) {
+      if (mg->mf_scale) {
+        mg_poly_resid<true, MfOp><<<gg, kMgBlock, 0, s>>>(mg->mf, mg->mf_scale, V.g, rin(0), cur[0], mg->mask,
+                                                          V.res, sc);
+      } else if (mg->f16) {
         mg_poly_resid<true, __half2><<<gg, kMgBlock, 0, s>>>(V.S16, mg->scl, V.g, rin(0), cur[0], mg->mask, V.res,
                                                              sc);
       } else {
@@ -2551,7 +2844,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
       float* din = V.da;
       float* dout = V.db;
-      for (int j = 1; j < k; ++j) {
+      for (int j = 1; j < kp; ++j) {
         if (lanes)
           mg_poly_step_lanes<H, ST><<<gl, kMgBlock, 0, s>>>(Sx, scl, V.Dinv, V.g, cf, j, V.res, din, V.res, dout,
                                                             cur[l], sc);
@@ -2577,12 +2870,14 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       float* tmp = cur[l]; cur[l] = oth[l]; oth[l] = tmp;
       float* din = V.da;
       float* dout = V.db;
-      for (int j = 1; j < k; ++j) {
+      for (int j = 1; j < kp; ++j) {
         mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
         mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, cf, j, V.res, mg->y, din, V.res, dout,
                                                     cur[l], sc);
         float* t2 = din; din = dout; dout = t2;
       }
+    } else if (l == 0 && mg->mf_scale) {
+      up(std::true_type{}, (const MfOp*)mg->mf, mg->mf_scale);
     } else if (l == 0 && mg->f16) {
       up(std::true_type{}, (const __half2*)V.S16, (const float*)mg->scl);
     } else if (V.half) {
@@ -2673,6 +2968,8 @@ struct MgBatch {
   MatArgs m;
   const float* vals;  // the SELL tangent, or
   const float* S;     // the half-stored lattice stencil (Dirichlet rows masked on the fly)
+  const MfOp* mf;     // or the matrix-free cell operator with per-cell scales mfs
+  const float* mfs;
   const float* mask;
   Lat g;
   int64_t N;
@@ -2697,8 +2994,10 @@ static int mg_enqueue_batch(hf_mg* mg, MgBatch& B, int poll, hipStream_t s, int 
     float* z = nullptr;
     if (vcycle(mg, B.r, B.z0, B.z1, &z, B.sc, s, k == timed_k, false, fuse)) return -2;
     B.z = z;
-    if (B.S)
-      mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(B.S, B.g, B.mask, B.z, B.r, B.q, B.w, B.sc, B.partB);
+    if (B.mf)
+      mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(B.mf, B.mfs, B.g, B.mask, B.z, B.r, B.q, B.w, B.sc, B.partB);
+    else if (B.S)
+      mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(B.S, nullptr, B.g, B.mask, B.z, B.r, B.q, B.w, B.sc, B.partB);
     else
       mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(B.m, B.vals, B.z, B.r, B.q, B.w, B.sc, B.partB);
   }
@@ -2753,6 +3052,10 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   B.m = m;
   B.vals = vals;
   B.S = S;
+  // the cell operator replaces the lattice stencil's products (a SELL fine level keeps its own)
+  const bool mf = mg->mf_scale && !mg->sell;
+  B.mf = mf ? mg->mf : nullptr;
+  B.mfs = mf ? mg->mf_scale : nullptr;
   B.mask = mg->mask;
   B.g = mg->lv[0].g;
   B.N = N;
@@ -2763,20 +3066,24 @@ static int mg_solve(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   B.sc = sc;
   B.partA = partA;
   B.partB = partB;
-  B.gr = S ? node_grid(mg->lv[0].g.n) : rows_grid(m.nslices);  // the operator kernels' grid (= partial count)
+  B.gr = (S || mf) ? node_grid(mg->lv[0].g.n) : rows_grid(m.nslices);  // the operator kernels' grid (= partials)
   B.ge = node_grid(mg->lv[0].g.n);  // the update's grid (= its partial count)
   const bool graph = !(flags & HF_MG_SOLVE_EAGER);
   HF_CHECK_HIP(hipMemcpyAsync(B.x, x, N * sizeof(float), hipMemcpyDeviceToDevice, s));
-  if (S)
-    mgcg_init_stencil<<<B.gr, kMgBlock, 0, s>>>(S, B.g, B.mask, b, B.x, B.r, B.pv, B.q, partA);
+  if (mf)
+    mgcg_init_stencil<<<B.gr, kMgBlock, 0, s>>>(B.mf, B.mfs, B.g, B.mask, b, B.x, B.r, B.pv, B.q, partA);
+  else if (S)
+    mgcg_init_stencil<<<B.gr, kMgBlock, 0, s>>>(S, nullptr, B.g, B.mask, b, B.x, B.r, B.pv, B.q, partA);
   else
     mgcg_init_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, b, B.x, B.r, B.pv, B.q, partA);
   mgcg_init_scalars<<<1, kScalarBlock, 0, s>>>(sc, partA, B.gr, rtol * rtol, atol * atol, rel_r0, maxiter);
   float* z = nullptr;
   if (vcycle(mg, B.r, B.z0, B.z1, &z, sc, s)) return -2;
   B.z = z;
-  if (S)
-    mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(S, B.g, B.mask, B.z, B.r, B.q, B.w, sc, partB);
+  if (mf)
+    mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(B.mf, B.mfs, B.g, B.mask, B.z, B.r, B.q, B.w, sc, partB);
+  else if (S)
+    mgcg_spmv_stencil<<<B.gr, kMgBlock, 0, s>>>(S, nullptr, B.g, B.mask, B.z, B.r, B.q, B.w, sc, partB);
   else
     mgcg_spmv_kernel<<<B.gr, kBlock, 0, s>>>(m, vals, B.z, B.r, B.q, B.w, sc, partB);
   HF_LAUNCH_CHECK();

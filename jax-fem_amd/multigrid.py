@@ -68,14 +68,22 @@ class LatticeTangent:
     TangentMatrix at the same state for the paths that need one (the Jacobi-PCG
     fallback)."""
 
-    def __init__(self, problem, u, mg):
+    def __init__(self, problem, u, mg, matrix_free=True):
         self.problem = problem
         self.pattern = problem.pattern
         self.mg = mg
         self.u = u
         self.dtype = torch.float32
         self.shape = (problem.N, problem.N)
-        self.stencil, self._version = problem.jacobian_stencil(u, mg)
+        # the operator as sum_c scale[c] K, when it has that form (Problem.cell_operator):
+        # the multigrid's fine-level products then run matrix-free, and the stencil its
+        # set-up reads comes from the cell matrix instead of the element Jacobians
+        self.cell_op = problem.cell_operator() if matrix_free else None
+        if self.cell_op is not None:
+            mg.set_cell_operator(self.cell_op)
+            self.stencil, self._version = problem.cell_stencil(mg)
+        else:
+            self.stencil, self._version = problem.jacobian_stencil(u, mg)
         self._sell = None
 
     @property
@@ -145,7 +153,7 @@ class Multigrid:
 
     @classmethod
     def for_problem(cls, problem, smoother="chebyshev", degree=2, param=None, fine_full=False, model=None,
-                    learned_mode="replace", fine_f16=True):
+                    learned_mode="replace", fine_f16=True, post_degree=None):
         """The problem's cached hierarchy (one per lattice) with the given smoother.
         smoother 'learned': Chebyshev everywhere, and on the fine level the learned
         network `model` (a LearnedPC) as the post-smoother (learned_mode 'replace') or
@@ -160,8 +168,7 @@ class Multigrid:
             mg = cls(grid, poly, degree, param, device=problem.device, fine_full=fine_full, fine_sell=sell,
                      fine_f16=f16)
             problem._mg = mg
-        else:
-            mg.set_smoother(poly, degree, param)
+        mg.set_smoother(poly, degree, param, post_degree)
         if smoother == "learned":
             if model is None:
                 raise ValueError("the 'learned' smoother needs linear option 'model' (a LearnedPC)")
@@ -192,17 +199,20 @@ class Multigrid:
         _lib.call("hf_mg_set_learned", self.handle, modes[mode], model.weight_ptrs(), _lib.ptr(g), _lib.stream())
         self._learned = (key, model, g)     # keeps the weights and the field alive
 
-    def set_smoother(self, smoother="chebyshev", degree=2, param=None):
+    def set_smoother(self, smoother="chebyshev", degree=2, param=None, post_degree=None):
         """smoother 'chebyshev' (param: interval ratio, default 8) or 'jacobi' (param: omega,
-        default 0.5); degree: steps per side."""
+        default 0.5); degree: pre-smoothing steps, post_degree: post-smoothing steps
+        (1..degree, default degree; the first steps of the same recurrence)."""
         kinds = {"jacobi": _lib.HF_MG_JACOBI, "chebyshev": _lib.HF_MG_CHEBYSHEV}
         if smoother not in kinds:
             raise ValueError("smoother must be 'chebyshev' or 'jacobi'")
         if param is None:
             param = 8.0 if smoother == "chebyshev" else 0.5
-        cfg = (smoother, int(degree), float(param))
+        post = 0 if post_degree is None or int(post_degree) == int(degree) else int(post_degree)
+        cfg = (smoother, int(degree), float(param), post)
         if cfg != self.smoother_cfg:
             _lib.call("hf_mg_set_smoother", self.handle, kinds[smoother], int(degree), ctypes.c_float(param))
+            _lib.call("hf_mg_set_post_degree", self.handle, post)
             self.smoother_cfg = cfg
 
     def bounds(self):
@@ -274,6 +284,32 @@ class Multigrid:
         elif n > max(2 * self._iters_at_setup, self._iters_at_setup + 10):
             self._stale = True
 
+    def set_cell_operator(self, op):
+        """op = (K, scale) from Problem.cell_operator: the fine level's smoothing passes
+        and the flexible CG's products run matrix-free (hf_mg_set_cell_operator) with
+        it; None returns them to the stencil.  The scale tensor is kept alive here."""
+        cur = getattr(self, "_cell_op", None)
+        if op is None:
+            if cur is not None:
+                _lib.call("hf_mg_set_cell_operator", self.handle, None, None)
+                self._cell_op = None
+                self._stale = True    # a set-up with the cell operator made no FP16 copy of the stencil
+            return
+        K, scale = op
+        if cur is not None and cur[1] is scale and cur[0] is K:
+            return
+        k = np.ascontiguousarray(K, dtype=np.float32)
+        _lib.call("hf_mg_set_cell_operator", self.handle, k.ctypes.data_as(ctypes.c_void_p), _lib.ptr(scale))
+        self._cell_op = (K, scale, k)
+
+    def cell_matvec(self, x):
+        """y = A x with the cell operator set by set_cell_operator (the row-eliminated
+        operator, as LatticeTangent.matvec)."""
+        x = x.to(torch.float32).contiguous()
+        y = torch.empty_like(x)
+        _lib.call("hf_mg_stencil_matvec_f32", self.handle, None, _lib.ptr(x), _lib.ptr(y), _lib.stream())
+        return y
+
     def apply(self, r, skip_fine_post=False):
         """z = V(r) (one cycle); skip_fine_post stops after the fine level's coarse correction."""
         r = r.to(torch.float32).contiguous()
@@ -299,6 +335,7 @@ class Multigrid:
         launches every kernel instead).  split_scalars: the direction scalars by
         their own one-workgroup kernel at every size (the form large grids take).
         Returns (x float32, info)."""
+        self.set_cell_operator(A.cell_op if isinstance(A, LatticeTangent) else None)
         if self.needs_setup(A, rebuild, token):
             self.setup(A, dirichlet_mask(problem or A.problem) if mask is None else mask)
             self._token = token
@@ -348,12 +385,13 @@ def _workspace(device, nbytes):
 
 def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=1, smoother="chebyshev", degree=2,
                         param=None, rebuild="solve", fine_full=False, stats=None, model=None, learned_mode="replace",
-                        fine_f16=True):
+                        fine_f16=True, post_degree=None):
     """Multigrid FCG; on breakdown or non-finite values Jacobi-PCG solves to the same
     absolute target from the better of the FCG iterate and x0.  `stats` (the Newton
     solve's KrylovStats) identifies the solve for rebuild='solve'.  Returns (x, info, fallback)."""
     from .solver import krylov
-    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_full, model, learned_mode, fine_f16)
+    mg = Multigrid.for_problem(problem, smoother, degree, param, fine_full, model, learned_mode, fine_f16,
+                               post_degree)
     x_start = (torch.zeros(A.shape[0], dtype=torch.float32, device=b.device) if x0 is None else
                x0.to(torch.float32).clone())
     x, info = mg.solve(A, b, x_start, rtol, atol, maxiter, rel_r0, poll, stats, problem=problem, rebuild=rebuild,

@@ -680,6 +680,91 @@ class Problem:
             _lib.call("hf_bc_rows_f" + tag, p.handle, _lib.ptr(dofs), dofs.numel(), _lib.ptr(vals), s)
         return vals
 
+    def cell_operator(self):
+        """The tangent as sum_c scale[c] K when that form holds exactly: a built
+        LinearElastic law alone (no other term adds to the tangent) on a box_mesh
+        HEX8 lattice of identical cells, its parameter constant over each cell's
+        points.  Returns (K, scale): K (576 float32, host) the 3x3 blocks (q, q') of
+        the cell matrix at unit modulus, corners q = 4 dx + 2 dy + dz; scale (ncell
+        float32, device) each cell's modulus (Material._E).  None otherwise, or when
+        K lacks the mid-plane reflection symmetry the multigrid's matrix-free kernels
+        use.  K is computed once per law and geometry by the FP64 cell kernel on one
+        cell (hf_cell_jacobian_f64)."""
+        from .materials import LinearElastic
+        plan = self._plan()
+        mat = self.material
+        if not (isinstance(mat, LinearElastic) and plan["law"] == "fast" and plan["pure_fast"]) or \
+                self.fe.ele_type != "HEX8" or self.vec != 3 or self.dim != 3:
+            return None
+        grid = getattr(self.fe.mesh, "node_grid", None)
+        if grid is None or int(np.prod(grid)) != self.fe.num_total_nodes:
+            return None
+        grid = tuple(int(v) for v in grid)
+        corner = self._box_corners(grid)
+        if corner is None:
+            return None
+        key = ("cell_op", grid, float(mat.nu), int(self.fe.quadrature_order))
+        hit = getattr(self, "_cell_op_K", None)
+        if hit is None or hit[0] != key:
+            self._cell_op_K = (key, self._cell_matrix(grid, corner, mat.nu))
+        K = self._cell_op_K[1]
+        if K is None:
+            return None
+        prm = self._params()
+        if prm is None:
+            scale = torch.full((self.num_cells,), float(mat._E(None)), dtype=torch.float32, device=self.device)
+        else:
+            if prm.dim() != 2 or not bool((prm == prm[:, :1]).all()):
+                return None
+            scale = mat._E(prm[:, 0]).to(torch.float32).contiguous()
+        return K, scale
+
+    def cell_stencil(self, mg):
+        """The multigrid's level-0 stencil (as jacobian_stencil) from the cell operator
+        set on `mg` (hf_mg_assemble_cellop_f32): no element Jacobians.  Returns
+        (stencil, version)."""
+        S = self._buf("mg_stencil", mg.stencil_size(), torch.float32)
+        _lib.call("hf_mg_assemble_cellop_f32", mg.handle, _lib.ptr(S), _lib.stream())
+        self._stencil_version = getattr(self, "_stencil_version", 0) + 1
+        return S, self._stencil_version
+
+    def _cell_matrix(self, grid, corner, nu):
+        """The unit-modulus linear-elastic cell matrix in corner order (see
+        cell_operator), or None if the lattice's cells differ, its block rows do not
+        sum to zero or it is not reflection-symmetric (relative 1e-10)."""
+        from .materials import LinearElastic
+        nx, ny, nz = grid
+        pts = np.asarray(self.fe.points, dtype=np.float64).reshape(nx, ny, nz, 3)
+        axes = [pts[:, 0, 0, 0], pts[0, :, 0, 1], pts[0, 0, :, 2]]
+        for d, a in enumerate(axes):
+            h = np.diff(a)
+            if not np.allclose(h, h[0], rtol=1e-12, atol=0.0):
+                return None
+        lat = np.stack(np.meshgrid(*axes, indexing="ij"), axis=3)
+        if np.abs(lat - pts).max() > 1e-12 * max(1.0, np.abs(pts).max()):
+            return None
+        m1 = LinearElastic(E=1.0, nu=nu).hf()
+        m1.quad1d = (int(self.fe.quadrature_order) + 2) // 2
+        pts_d, cells_d = self._dev()
+        u0 = torch.zeros(self.N, dtype=torch.float64, device=self.device)
+        ej = torch.empty(64 * 9, dtype=torch.float64, device=self.device)
+        _lib.call("hf_cell_jacobian_f64", 8, ctypes.byref(m1), _lib.ptr(pts_d), _lib.ptr(cells_d), 1,
+                  _lib.ptr(u0), None, _lib.ptr(ej), _lib.stream())
+        Kb = ej.cpu().numpy().reshape(8, 8, 3, 3)          # block-major (a, b, i, k), local node order
+        t = [int(corner[q]) for q in range(8)]             # corner q -> local node
+        Kq = Kb[np.ix_(t, t)]                               # (q, q', i, k)
+        refl = lambda q: np.array([-1.0 if (q >> (2 - ax)) & 1 else 1.0 for ax in range(3)])
+        scale = np.abs(Kq).max()
+        if np.abs(Kq.sum(axis=1)).max() > 1e-10 * scale:      # translations carry no force
+            return None
+        for a in range(8):
+            for b in range(8):
+                Ra = refl(a)
+                want = Ra[:, None] * Kq[0, a ^ b] * Ra[None, :]
+                if np.abs(Kq[a, b] - want).max() > 1e-10 * scale:
+                    return None
+        return np.ascontiguousarray(Kq.reshape(-1).astype(np.float32))
+
     def jacobian_stencil(self, u, mg):
         """FP32 tangent of a HEX8 box lattice in the multigrid's stencil form: the
         element Jacobians' upper blocks (hf_hex8_jacobian_upper_f32) summed into each

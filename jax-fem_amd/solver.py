@@ -18,11 +18,13 @@ a legacy flat dict, solver.py:1014-1068).  Linear keys:
   * ``learned`` {model: LearnedPC, tol, atol, maxiter, stall}: flexible CG with
     the learned CNN preconditioner on the FP32 tangent (precision 'mixed'),
     Jacobi-PCG as its fallback; the Newton residual stays the FP64 judge.
-  * ``mg`` {tol, atol, maxiter, smoother, degree, param, rebuild, fine_full, fine_f16,
-    poll, model, learned_mode}: flexible CG preconditioned by geometric-multigrid V-cycles
+  * ``mg`` {tol, atol, maxiter, smoother, degree, post_degree, param, rebuild, fine_full,
+    fine_f16, matrix_free, poll, model, learned_mode}: flexible CG preconditioned by geometric-multigrid V-cycles
     on the FP32 tangent (precision 'mixed'; box_mesh HEX8 lattices, multigrid.py), Jacobi-PCG
     as its fallback; fine_f16 (default True) smooths the fine level with an FP16 copy of
-    its stencil; smoother 'learned' with model=LearnedPC makes the learned
+    its stencil; matrix_free (default True) computes the fine level's products from one
+    cell matrix and per-cell moduli when the tangent has that form (a built linear-elastic
+    law on a box lattice, Problem.cell_operator); smoother 'learned' with model=LearnedPC makes the learned
     network the fine level's post-smoother;
     rebuild 'solve' (default: the hierarchy of a Newton solve's first tangent
     preconditions its later steps), 'always' or 'never'.
@@ -293,7 +295,7 @@ def linear_solver(A, b, x0, linear_options, stats=None, rtol=None):
                                           smoother=o.get("smoother", "chebyshev"), degree=o.get("degree", 2),
                                           param=o.get("param"), rebuild=o.get("rebuild", "solve"),
                                           fine_full=o.get("fine_full", False), stats=stats, model=o.get("model"),
-                                          fine_f16=o.get("fine_f16", True),
+                                          fine_f16=o.get("fine_f16", True), post_degree=o.get("post_degree"),
                                           learned_mode=o.get("learned_mode", "replace"))
         if stats is not None:
             stats.calls[-1]["fallback"] = fb
@@ -373,7 +375,8 @@ def _stencil_multigrid(problem, lin_opts, adt):
     except NotImplementedError:
         return None
     return Multigrid.for_problem(problem, o.get("smoother", "chebyshev"), o.get("degree", 2), o.get("param"), False,
-                                 o.get("model"), o.get("learned_mode", "replace"), o.get("fine_f16", True))
+                                 o.get("model"), o.get("learned_mode", "replace"), o.get("fine_f16", True),
+                                 o.get("post_degree"))
 
 
 def solver(problem, solver_options={}):
@@ -445,7 +448,7 @@ def _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, 
         if A is not None and linear_law:
             pass   # a linear law's tangent does not depend on u: the first step's serves the solve
         elif smg is not None:
-            A = LatticeTangent(problem, u, smg)
+            A = LatticeTangent(problem, u, smg, matrix_free=(lin_opts["mg"] or {}).get("matrix_free", True))
         else:
             A = TangentMatrix(problem, problem.jacobian_bsr(u, adt))
         ev1 = torch.cuda.Event(enable_timing=True)

@@ -287,7 +287,7 @@ def test_mg_f16_node_kernels_newton(jfa):
     for f16 in (False, True):
         prob = box_problem(jfa, cells, material="lin", E_cell=E)
         u = jfa.solver(prob, {"newton": {"precision": "mixed", "tol": 0.0, "rel_tol": 1e-9,
-                                         "linear": {"mg": {"fine_f16": f16}}}})[0]
+                                         "linear": {"mg": {"fine_f16": f16, "matrix_free": False}}}})[0]
         info = prob.solve_info
         say("  fine_f16 %s: krylov %s, rel %.2e, fallbacks %d" % (f16, info["krylov_iters"], info["rel"],
                                                                    info["fallbacks"]))
@@ -297,6 +297,84 @@ def test_mg_f16_node_kernels_newton(jfa):
     say("  rel-L2 between the two", rel)
     assert rel <= 1e-7
     assert out[True][1] <= out[False][1] + 3
+
+
+@pytest.mark.parametrize("cells", [(6, 5, 7), (44, 40, 36)])
+def test_cell_operator_matches_stencil(jfa, cells):
+    """A built linear-elastic law with a per-cell modulus on a box lattice: the
+    stencil assembled from the cell operator (one cell matrix, 48 values by the
+    cell's reflection symmetry, per-cell scales) equals the one from the element
+    Jacobians, the matrix-free product equals that stencil's to FP32 rounding, and
+    the V-cycle with both equals the stencil one (8-lane kernels on the small
+    lattice, thread-per-node kernels on the 69,375-node one)."""
+    say("mg cell operator vs stencil", cells)
+    MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
+    lpc = __import__("jax_fem_amd.learned_pc", fromlist=["x"])
+    prob = box_problem(jfa, cells, material="lin", E_cell=lpc.heterogeneous_modulus(cells, 3))
+    mask = MG.dirichlet_mask(prob)
+    mg = MG.Multigrid(MG.lattice(prob), fine_f16=False)
+    u = torch.zeros(prob.N, device=DEV, dtype=torch.float64)
+    g = torch.Generator(device=DEV).manual_seed(4)
+    x = torch.randn(prob.N, generator=g, device=DEV)
+    r = torch.randn(prob.N, generator=g, device=DEV) * mask
+    # the stencil from the element Jacobians, its product and V-cycle
+    L0 = MG.LatticeTangent(prob, u, mg, matrix_free=False)
+    assert L0.cell_op is None
+    mg.setup(L0, mask)
+    y_st = L0.matvec(x)
+    S0 = L0.stencil.clone()
+    z_st = mg.apply(r)
+    # the cell operator, the stencil assembled from it, its product and V-cycle
+    L = MG.LatticeTangent(prob, u, mg)
+    assert L.cell_op is not None
+    K, scale = L.cell_op
+    assert K.shape == (576,) and scale.shape == (prob.num_cells,)
+    E = torch.as_tensor(lpc.heterogeneous_modulus(cells, 3), dtype=torch.float32, device=DEV)
+    assert torch.equal(scale, E)
+    es = float((L.stencil - S0).abs().max() / S0.abs().max())
+    say("  stencil from the cell operator vs element Jacobians: max rel %.2e" % es)
+    assert es <= 2e-6
+    mg.setup(L, mask)
+    y_mf = mg.cell_matvec(x)
+    err = float((y_mf - y_st).abs().max() / y_st.abs().max())
+    say("  product: max rel %.2e" % err)
+    assert err <= 2e-6
+    z_mf = mg.apply(r)
+    e2 = float((z_mf - z_st).abs().max() / z_st.abs().max())
+    say("  V-cycle: max rel %.2e" % e2)
+    assert e2 <= 1e-5
+    # not that form: a Neo-Hookean law, a distorted lattice
+    neo = box_problem(jfa, (4, 3, 5))
+    assert MG.LatticeTangent(neo, torch.zeros(neo.N, device=DEV, dtype=torch.float64),
+                             MG.Multigrid(MG.lattice(neo))).cell_op is None
+    bent = box_problem(jfa, (4, 3, 5), material="lin")
+    bent.fe.points[5, 1] += 0.01
+    assert bent._cell_matrix(MG.lattice(bent), bent._box_corners(MG.lattice(bent)), 0.3) is None
+
+
+def test_mg_matrix_free_newton(jfa):
+    """Mixed Newton with the multigrid on a 74,333-node heterogeneous lattice: the
+    matrix-free fine level reaches the FP64 target with the stencil one's
+    iteration count and solution."""
+    say("mg matrix-free newton")
+    lpc = __import__("jax_fem_amd.learned_pc", fromlist=["x"])
+    cells = (48, 40, 36)
+    E = lpc.heterogeneous_modulus(cells, 6)
+    out = {}
+    for mf in (False, True):
+        prob = box_problem(jfa, cells, material="lin", E_cell=E)
+        u = jfa.solver(prob, {"newton": {"precision": "mixed", "tol": 0.0, "rel_tol": 1e-9,
+                                         "linear": {"mg": {"matrix_free": mf}}}})[0]
+        info = prob.solve_info
+        say("  matrix_free %s: krylov %s, rel %.2e, fallbacks %d" % (mf, info["krylov_iters"], info["rel"],
+                                                                     info["fallbacks"]))
+        assert info["rel"] <= 1e-9 and info["fallbacks"] == 0
+        assert (getattr(prob._mg, "_cell_op", None) is not None) == mf
+        out[mf] = (u, int(info["krylov_iters"]))
+    rel = (torch.linalg.norm(out[True][0] - out[False][0]) / torch.linalg.norm(out[False][0])).item()
+    say("  rel-L2 between the two", rel)
+    assert rel <= 1e-6     # both within the FP64 residual target; the difference is the conditioning's
+    assert out[True][1] <= out[False][1] + 2
 
 
 def test_mg_fcg_converges(jfa):
