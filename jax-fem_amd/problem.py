@@ -711,12 +711,25 @@ class Problem:
         if K is None:
             return None
         prm = self._params()
-        if prm is None:
-            scale = torch.full((self.num_cells,), float(mat._E(None)), dtype=torch.float32, device=self.device)
-        else:
-            if prm.dim() != 2 or not bool((prm == prm[:, :1]).all()):
-                return None
-            scale = mat._E(prm[:, 0]).to(torch.float32).contiguous()
+        # the per-cell moduli live in one buffer (a stable address: the multigrid's
+        # captured iteration keeps it as a kernel argument), refreshed when the
+        # parameter tensor or its contents change; the tensor is referenced by the
+        # key, so its address cannot be reused while the key holds it
+        law = (mat.param, float(mat.E), float(mat.E0), float(mat.Emin), float(mat.penal))
+        key = (None, law) if prm is None else (prm.data_ptr(), prm._version, tuple(prm.shape), law)
+        hit = getattr(self, "_cell_scale_key", None)
+        scale = self._buf("cell_scale", self.num_cells, torch.float32)
+        if hit is None or hit[0] != key or hit[3] is not scale:
+            if prm is None:
+                scale.fill_(float(mat._E(None)))
+                ok = True
+            else:
+                ok = prm.dim() == 2 and bool((prm == prm[:, :1]).all())
+                if ok:
+                    scale.copy_(mat._E(prm[:, 0]))
+            self._cell_scale_key = (key, ok, prm, scale)   # prm kept: its address stays its own
+        if not self._cell_scale_key[1]:
+            return None
         return K, scale
 
     def cell_stencil(self, mg):
