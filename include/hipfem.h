@@ -296,12 +296,11 @@ enum { HF_MG_JACOBI = 0, HF_MG_CHEBYSHEV = 1 };
  * SELL rows; with P the interpolation from every second node it is the 27-point
  * HEX8 operator of the same cells.  Couplings whose coarse parents are more than
  * one coarse step apart are a set-up error.
- * HF_MG_FINE_F16: the fine level's smoothing passes (and the residual restricted
- * from it) read an FP16 copy of the half-stored stencil, made at each set-up:
- * each block's 9 entries in 5 half2 words, scaled by the power of two that puts
- * the largest magnitude in [2^14, 2^15) (280 bytes a node instead of 504).  The
- * outer flexible CG's products keep the FP32 stencil.  Ignored unless the fine
- * stencil is half-stored. */
+ * HF_MG_FINE_F16: the smoothing passes (and the restricted residuals) of every
+ * half-stored level the grid kernels run read an FP16 copy of its stencil, made at
+ * each set-up: each block's 9 entries in 5 half2 words, scaled by the power of two
+ * that puts the level's largest magnitude in [2^14, 2^15) (280 bytes a node
+ * instead of 504).  The outer flexible CG's products keep the FP32 stencil. */
 enum { HF_MG_FINE_FULL = 1, HF_MG_FINE_SELL = 2, HF_MG_FINE_F16 = 4 };
 int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, hf_mg** out);   /* nodes per dimension */
 int hf_mg_destroy(hf_mg* mg);

@@ -2190,7 +2190,8 @@ struct MgLevel {
   float* da = nullptr;    // 3 n smoother directions (two, alternating)
   float* db = nullptr;    // 3 n
   float* res = nullptr;   // 3 n smoother residual / restriction input
-  __half2* S16 = nullptr; // fine level with HF_MG_FINE_F16: 70 n words (mg_to_f16)
+  __half2* S16 = nullptr; // half-stored grid level with HF_MG_FINE_F16: 70 n words (mg_to_f16)
+  bool s16_ok = false;    // S16 made by the last set-up (level 0 skips it while a cell operator is set)
 };
 
 struct hf_mg {
@@ -2204,8 +2205,7 @@ struct hf_mg {
   float* coef = nullptr;     // kMgMaxLevels x kCoefStride smoothing coefficients (mg_coef)
   bool half = false;         // fine-level stencil half-stored (symmetric operator)
   bool f16 = false;          // fine-level smoothing passes read an FP16 copy of the stencil
-  bool s16_valid = false;    // the copy was made by the last set-up (skipped while a cell operator is set)
-  float* scl = nullptr;      // its scale, inverse scale and |S| maximum (mg_f16_scale)
+  float* scl = nullptr;      // per level (4 floats each): scale, inverse scale, |S| maximum (mg_f16_scale)
   const float* s0 = nullptr; // the FP32 fine stencil of the last set-up (the caller's, or lv[0].S)
   // matrix-free fine level (hf_mg_set_cell_operator): the cell matrix on the device,
   // its host copy, and the per-cell scales (the caller's; null: off)
@@ -2269,7 +2269,8 @@ static int mg_free(hf_mg* mg) {
   if (mg->coef) (void)hipFree(mg->coef);
   if (mg->lsm.buf) (void)hipFree(mg->lsm.buf);
   if (mg->sdinv) (void)hipFree(mg->sdinv);
-  if (mg->lv[0].S16) (void)hipFree(mg->lv[0].S16);
+  for (int l = 0; l < mg->nlev; ++l)
+    if (mg->lv[l].S16) (void)hipFree(mg->lv[l].S16);
   if (mg->scl) (void)hipFree(mg->scl);
   if (mg->mf) (void)hipFree(mg->mf);
   return 0;
@@ -2302,8 +2303,8 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
   mg->sell = (flags & HF_MG_FINE_SELL) != 0;
   // a one-level hierarchy is the dense FP64 solve, which reads the full stencil
   mg->half = !(flags & HF_MG_FINE_FULL) && !mg->sell && nlev > 1;
-  // the FP16 copy serves the half-stored fine stencil only (ignored otherwise)
-  mg->f16 = (flags & HF_MG_FINE_F16) && mg->half;
+  // FP16 copies of the half-stored levels the grid kernels smooth (none if no level is)
+  mg->f16 = (flags & HF_MG_FINE_F16) != 0;
   if (mg->sell && nlev < 2) {
     delete mg;
     HF_REQUIRE(false, "hf_mg_create: a SELL fine level needs at least one coarser level");
@@ -2346,12 +2347,14 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
   }
   if (!rc) rc |= alloc(&mg->part, kMaxGrid + kMgMaxLevels);
   if (!rc && mg->f16) {
-    rc |= alloc(&mg->scl, 4);
-    if (!rc && hipMalloc((void**)&mg->lv[0].S16,
-                         (size_t)kHalfSlots * kF16Words * mg->lv[0].g.n * sizeof(__half2)) != hipSuccess) {
-      set_error("hf_mg_create: hipMalloc failed");
-      rc = -2;
-    }
+    // an FP16 copy for every half-stored level the grid kernels smooth
+    rc |= alloc(&mg->scl, 4 * kMgMaxLevels);
+    for (int l = 0; l < lb && !rc; ++l)
+      if (mg->lv[l].half && hipMalloc((void**)&mg->lv[l].S16, (size_t)kHalfSlots * kF16Words * mg->lv[l].g.n *
+                                                                  sizeof(__half2)) != hipSuccess) {
+        set_error("hf_mg_create: hipMalloc failed");
+        rc = -2;
+      }
   }
   if (!rc) rc |= alloc(&mg->coef, (int64_t)kMgMaxLevels * kCoefStride);
   mg->nd = (int)(3 * mg->lv[mg->nlev - 1].g.n);
@@ -2437,7 +2440,7 @@ extern "C" int hf_mg_set_cell_operator(hf_mg* mg, const float* k, const float* c
     if (mg->mf_scale) {
       ++mg->gen;   // the cycle's kernels change
       // a set-up with the cell operator made no FP16 copy: the smoother has none until the next
-      if (mg->f16 && !mg->s16_valid) mg->ready = false;
+      if (mg->lv[0].S16 && !mg->lv[0].s16_ok) mg->ready = false;
     }
     mg->mf_scale = nullptr;
     return 0;
@@ -2566,15 +2569,21 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   const Lat gl = mg->lv[mg->nlev - 1].g;
   mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
   if (mg_coefficients(mg, s)) return -2;
-  mg->s16_valid = false;
-  if (mg->f16 && !mg->mf_scale) {
-    mg->s16_valid = true;
-    const int64_t cnt = (int64_t)kHalfSlots * 9 * g0.n;
+  // FP16 copies of the half-stored levels (level 0 only without a cell operator,
+  // whose matrix-free rows replace its stencil in the cycle)
+  for (int l = 0; l < mg->nlev; ++l) {
+    MgLevel& V = mg->lv[l];
+    V.s16_ok = false;
+    if (!V.S16 || (l == 0 && mg->mf_scale)) continue;
+    const float* Sl = l == 0 ? S0p : V.S;
+    float* sc16 = mg->scl + 4 * l;
+    const int64_t cnt = (int64_t)kHalfSlots * 9 * V.g.n;
     const int ga = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(cnt / 4, kMgBlock), 1), kMaxGrid);
-    mg_absmax<<<ga, kMgBlock, 0, s>>>(S0p, cnt, mg->part);
-    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, ga, mg->scl + 2);
-    mg_f16_scale<<<1, kWave, 0, s>>>(mg->scl);
-    mg_to_f16<<<dim3(gb0, kHalfSlots), kMgBlock, 0, s>>>(S0p, g0.n, mg->scl, mg->lv[0].S16);
+    mg_absmax<<<ga, kMgBlock, 0, s>>>(Sl, cnt, mg->part);
+    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, ga, sc16 + 2);
+    mg_f16_scale<<<1, kWave, 0, s>>>(sc16);
+    mg_to_f16<<<dim3((unsigned)ceil_div(V.g.n, kMgBlock), kHalfSlots), kMgBlock, 0, s>>>(Sl, V.g.n, sc16, V.S16);
+    V.s16_ok = true;
   }
   if (mg->lmode) {
     const MgLevel& F = mg->lv[0];
@@ -2775,8 +2784,8 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       mg_sell_resid_update<<<gg, kMgBlock, 0, s>>>(V.g.n, res, mg->y, dout, sc);
     } else if (l == 0 && mg->mf_scale) {
       down(std::true_type{}, (const MfOp*)mg->mf, mg->mf_scale);
-    } else if (l == 0 && mg->f16) {
-      down(std::true_type{}, (const __half2*)V.S16, (const float*)mg->scl);
+    } else if (V.s16_ok) {
+      down(std::true_type{}, (const __half2*)V.S16, (const float*)(mg->scl + 4 * l));
     } else if (V.half) {
       down(std::true_type{}, (const float*)V.S, (const float*)nullptr);
     } else {
@@ -2818,7 +2827,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       if (mg->mf_scale) {
         mg_poly_resid<true, MfOp><<<gg, kMgBlock, 0, s>>>(mg->mf, mg->mf_scale, V.g, rin(0), cur[0], mg->mask,
                                                           V.res, sc);
-      } else if (mg->f16) {
+      } else if (V.s16_ok) {
         mg_poly_resid<true, __half2><<<gg, kMgBlock, 0, s>>>(V.S16, mg->scl, V.g, rin(0), cur[0], mg->mask, V.res,
                                                              sc);
       } else {
@@ -2878,8 +2887,8 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       }
     } else if (l == 0 && mg->mf_scale) {
       up(std::true_type{}, (const MfOp*)mg->mf, mg->mf_scale);
-    } else if (l == 0 && mg->f16) {
-      up(std::true_type{}, (const __half2*)V.S16, (const float*)mg->scl);
+    } else if (V.s16_ok) {
+      up(std::true_type{}, (const __half2*)V.S16, (const float*)(mg->scl + 4 * l));
     } else if (V.half) {
       up(std::true_type{}, (const float*)V.S, (const float*)nullptr);
     } else {

@@ -14,8 +14,8 @@ re-runs the set-up when the tangent changes and implements the fallback.
     storage      FP32; the fine-level stencil of a symmetric operator half-stored
                  (diagonal + 13 upper offsets, the lower blocks read as the transposes
                  of the neighbours'; fine_full=True keeps all 27 slots); with
-                 fine_f16 the fine level's smoothing passes read an FP16 copy of it
-                 (scaled by a power of two; the outer CG keeps FP32)
+                 fine_f16 the smoothing passes of the half-stored levels read FP16
+                 copies (scaled by a power of two; the outer CG keeps FP32)
     cycle        V-cycle, `degree` smoothing steps per side: Chebyshev in Dinv A on
                  [lam/ratio, lam], lam the Gershgorin bound of Dinv A (Dinv: inverse
                  3x3 diagonal blocks), or damped block Jacobi (omega)
@@ -119,14 +119,14 @@ class Multigrid:
                  fine_sell=False, fine_f16=False):
         """fine_sell: the fine level is the tangent's SELL matrix (HEX27 lattices);
         fine_full: store all 27 fine stencil slots (a non-symmetric operator);
-        fine_f16: the fine level's smoothing passes read an FP16 copy of its
-        half-stored stencil (HF_MG_FINE_F16; ignored with fine_full or fine_sell)."""
+        fine_f16: the smoothing passes of the half-stored levels read FP16 copies of
+        their stencils (HF_MG_FINE_F16; with fine_sell the coarser levels only)."""
         _lib.load()
         self.grid = tuple(int(v) for v in grid)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.fine_full = bool(fine_full)
         self.fine_sell = bool(fine_sell)
-        self.fine_f16 = bool(fine_f16) and not (self.fine_full or self.fine_sell)
+        self.fine_f16 = bool(fine_f16) and not self.fine_full
         h = ctypes.c_void_p()
         flags = (_lib.HF_MG_FINE_FULL if self.fine_full else 0) | (_lib.HF_MG_FINE_SELL if self.fine_sell else 0) | \
             (_lib.HF_MG_FINE_F16 if self.fine_f16 else 0)
@@ -162,7 +162,7 @@ class Multigrid:
         sell = problem.fe.ele_type == "HEX27"
         poly = "chebyshev" if smoother == "learned" else smoother
         mg = getattr(problem, "_mg", None)
-        f16 = bool(fine_f16) and not (fine_full or sell)
+        f16 = bool(fine_f16) and not fine_full
         if mg is None or mg.grid != grid or mg.fine_full != bool(fine_full) or mg.fine_sell != sell or \
                 mg.fine_f16 != f16:
             mg = cls(grid, poly, degree, param, device=problem.device, fine_full=fine_full, fine_sell=sell,

@@ -151,14 +151,21 @@ def test_coarse_operators_annihilate_translations(jfa):
 
 
 def block_dinv(A, unit=None):
+    """Inverses of the 3x3 diagonal blocks; a block whose determinant is not positive
+    (an indefinite tangent: a large random displacement) by its diagonal, as the
+    set-up kernel does (det <= 1e-12 max|diag|^3 -> 1 / positive diagonal entries)."""
     n = A.shape[0] // 3
-    Ad = A.toarray()
+    A = A.tocsr()
     D = []
     for i in range(n):
-        B = Ad[3 * i:3 * i + 3, 3 * i:3 * i + 3].copy()
+        B = A[3 * i:3 * i + 3, 3 * i:3 * i + 3].toarray()
         if unit is not None:
             B += np.diag(1.0 - unit[3 * i:3 * i + 3])
-        D.append(np.linalg.inv(B))
+        sc = np.abs(np.diag(B)).max()
+        if np.linalg.det(B) > 1e-12 * sc ** 3:
+            D.append(np.linalg.inv(B))
+        else:
+            D.append(np.diag([1.0 / v if v > 0 else 0.0 for v in np.diag(B)]))
     return sp.block_diag(D).tocsr()
 
 
@@ -187,10 +194,10 @@ def smoother_ref(A, D, r, e, kind, degree, param, lam):
     return e
 
 
-def vcycle_ref(As, dims, mask, r, kind, degree, param, lams, A0_smooth=None):
-    """One V-cycle with scipy matrices; A0_smooth (optional) is the operator the fine
-    level's smoothing and restricted residual use (its block inverses, bound and the
-    coarse levels still come from As[0])."""
+def vcycle_ref(As, dims, mask, r, kind, degree, param, lams, A_smooth=None):
+    """One V-cycle with scipy matrices; A_smooth (optional, per level, None entries
+    for As[l]) are the operators the levels' smoothing and restricted residuals use
+    (block inverses, bounds and the coarser levels still come from As)."""
     L = len(As)
     Dinv = [block_dinv(As[l], mask if l == 0 else None) for l in range(L - 1)]
     Ps = [interp_3d(dims[l]) for l in range(L - 1)]
@@ -200,7 +207,7 @@ def vcycle_ref(As, dims, mask, r, kind, degree, param, lams, A0_smooth=None):
     def V(l, rl):
         if l == L - 1:
             return Ainv @ rl
-        A = A0_smooth if (l == 0 and A0_smooth is not None) else As[l]
+        A = A_smooth[l] if (A_smooth is not None and l < len(A_smooth) and A_smooth[l] is not None) else As[l]
         e = smoother_ref(A, Dinv[l], rl, np.zeros_like(rl), kind, degree, param, lams[l])
         ec = V(l + 1, Ps[l].T @ (rl - A @ e))
         e = e + (mask if l == 0 else 1.0) * (Ps[l] @ ec)
@@ -243,16 +250,28 @@ def round_f16(A):
     return a
 
 
-@pytest.mark.parametrize("cells", [(6, 5, 7), (8, 6, 7)])
+def grid_levels(dims):
+    """The levels the grid kernels smooth (the rest run in the one-workgroup cycle):
+    up to the first level of at most 128 nodes with at most 4 levels left."""
+    L = len(dims)
+    if L == 1:
+        return 0
+    for l in range(1, L):
+        if int(np.prod(dims[l])) <= 128 and L - l <= 4:
+            return l
+    return L - 1
+
+
+@pytest.mark.parametrize("cells", [(6, 5, 7), (8, 6, 7), (16, 12, 14)])
 def test_vcycle_f16_vs_scipy(jfa, cells):
-    """HF_MG_FINE_F16: the fine level's smoothing passes and the residual restricted
-    from it use the FP16-rounded stencil (packed 5 half2 words per block, lower
-    blocks read transposed from the neighbour, power-of-two scale), the block
-    inverses, bounds and coarse levels the FP32 one."""
+    """HF_MG_FINE_F16: the smoothing passes and restricted residuals of the levels the
+    grid kernels run use FP16-rounded stencils (packed 5 half2 words per block, lower
+    blocks read transposed from the neighbour, a power-of-two scale per level), the
+    block inverses, bounds and coarser levels the FP32 ones."""
     say("mg vcycle fp16 fine level", cells)
     MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
     prob = box_problem(jfa, cells)
-    A = tangent(jfa, prob)
+    A = tangent(jfa, prob, scale=0.002)   # small strains: an SPD tangent, whose V-cycle is a contraction
     mask = MG.dirichlet_mask(prob)
     mg16 = MG.Multigrid(MG.lattice(prob), fine_f16=True)
     mg16.setup(A, mask)
@@ -267,7 +286,11 @@ def test_vcycle_f16_vs_scipy(jfa, cells):
     r = torch.randn(prob.N, generator=g, device=DEV) * mask
     rr = r.double().cpu().numpy()
     z = mg16.apply(r).double().cpu().numpy()
-    z16, _ = vcycle_ref(As, dims, m, rr, "chebyshev", 2, 8.0, lams, A0_smooth=round_f16(A0))
+    lb = grid_levels(dims)
+    # each grid level's own FP32 stencil, rounded as the set-up rounds it
+    A16 = [round_f16(stencil_to_csr(mg16.stencil(l).cpu().numpy(), dims[l])) for l in range(lb)]
+    say("  %d levels, %d smoothed by the grid kernels in FP16" % (len(dims), lb))
+    z16, _ = vcycle_ref(As, dims, m, rr, "chebyshev", 2, 8.0, lams, A_smooth=A16)
     z32, _ = vcycle_ref(As, dims, m, rr, "chebyshev", 2, 8.0, lams)
     e16 = np.abs(z - z16).max() / np.abs(z16).max()
     e32 = np.abs(z - z32).max() / np.abs(z32).max()
