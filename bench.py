@@ -734,10 +734,15 @@ def main():
     # the fine level ran matrix-free when the hierarchy holds a cell operator
     mf_on = use_mg and getattr(getattr(prob, "_mg", None), "_cell_op", None) is not None
     if use_mg and prob.fe.ele_type == "HEX27":
-        # the fine level is the SELL tangent: one smoothing product y = A v
-        kbytes = 4 * nnz + 4 * prob.pattern.nblocks + 8 * N
-        kname = "mg_sell_spmv (fine-level smoothing product y = A v, SELL column-form node blocks)"
-        pmc = "mg_sell_spmv_%s_pmc.md" % tag
+        # the fine level is the SELL tangent: one smoothing product y = A v from the FP16
+        # copy of its values (2 B per value), one column index per node-pair block, v
+        # read and y written (4 B per DOF each)
+        f16 = getattr(getattr(prob, "_mg", None), "fine_f16", False)
+        kbytes = (2 if f16 else 4) * nnz + 4 * prob.pattern.nblocks + 8 * N
+        kname = ("mg_sell_spmv16 (fine-level smoothing product y = A v, SELL column-form node blocks, FP16 "
+                 "values)" if f16 else "mg_sell_spmv (fine-level smoothing product y = A v, SELL column-form "
+                 "node blocks)")
+        pmc = ("mg_sell_spmv16_%s_pmc.md" if f16 else "mg_sell_spmv_%s_pmc.md") % tag
     elif use_mg and mf_on:
         # one fine-level Chebyshev smoothing step with matrix-free rows
         # (mg_poly_step<true, MfOp>): per node the 8 cells' 7 off-diagonal blocks of the

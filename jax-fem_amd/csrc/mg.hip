@@ -1791,6 +1791,63 @@ __global__ void __launch_bounds__(kBlock) mg_sell_spmv(MatArgs m, const float* _
   sweep<float, 3, false>(m, r0, r1, vals, xv, epi);
 }
 
+// The SELL fine level's values in FP16 for the smoothing products (HF_MG_FINE_F16):
+// slot t's value k of lane l at W[(t*3 + k)*L + l] (vals[(t*L + l)*3 + k] times the
+// power-of-two scale scl[0]), so a wavefront's loads of one (t, k) are 126
+// contiguous bytes.
+constexpr int kSellL = SellShape<3>::L;
+__global__ void __launch_bounds__(kMgBlock) mg_sell_to_f16(const float* __restrict__ vals, int64_t nslots,
+                                                           const float* __restrict__ scl, __half* __restrict__ W) {
+  const int64_t idx = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;   // t * L + lane
+  if (idx >= nslots * kSellL) return;
+  const int64_t t = idx / kSellL;
+  const int lane = (int)(idx - t * kSellL);
+  const float sc = scl[0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) W[(t * 3 + k) * kSellL + lane] = __float2half_rn(sc * vals[idx * 3 + k]);
+}
+
+// y = A v as mg_sell_spmv (the same lanes, sums and order) from the FP16 values,
+// scaled back by scl[1]
+__global__ void __launch_bounds__(kBlock) mg_sell_spmv16(MatArgs m, const __half* __restrict__ W,
+                                                         const float* __restrict__ scl, const float* __restrict__ v,
+                                                         float* __restrict__ y, const Scalars* sc) {
+  if (sc && sc->done) return;
+  using S = SellShape<3>;
+  int64_t s0, s1;
+  row_range(m.nslices, s0, s1);
+  const float sinv = scl[1];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int r = lane / 3, i = lane - r * 3;
+  const bool in_slice = lane < S::L;
+  __shared__ float red[kWavesPerBlock][kWave * 3];
+  for (int64_t sl = s0 + w; sl < s1; sl += kWavesPerBlock) {
+    const int32_t t0 = m.slice_ptr[sl], t1 = m.slice_ptr[sl + 1];
+    float yk[3] = {0.f, 0.f, 0.f};
+    if (in_slice) {
+#pragma unroll 8
+      for (int32_t t = t0; t < t1; ++t) {
+        const int64_t cr = (int64_t)t * S::R + r;
+        const float x = v[(int64_t)m.sell_col[cr] * 3 + i];
+        const __half* wt = W + (int64_t)t * 3 * S::L + lane;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) yk[k] += __half2float(wt[k * S::L]) * x;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) red[w][lane * 3 + k] = yk[k];
+    wave_lds_sync();
+    if (in_slice) {
+      float yi = 0.f;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) yi += red[w][(r * 3 + j) * 3 + i];
+      const int32_t n = m.sell_row[sl * S::R + r];
+      if (n >= 0) y[(int64_t)n * 3 + i] = yi * sinv;
+    }
+    wave_lds_sync();  // the next slice rewrites red
+  }
+}
+
 // node-wise halves of the fine-level smoothing passes, with y = A v from mg_sell_spmv:
 //   step:  res = res_in - y (v = d_in); d_out = c1 d_in + c2 Dinv res; e += d_out
 //   resid: t = res_in - y (v = d_in)
@@ -2217,6 +2274,9 @@ struct hf_mg {
   float* y = nullptr;        // 3 n0: SELL products of the fine level
   const hf_pattern* sp = nullptr;  // the fine level's pattern and values (last set-up, caller's memory)
   const float* svals = nullptr;
+  __half* svals16 = nullptr;       // their FP16 copy (HF_MG_FINE_F16), scale in scl[0..2]
+  int64_t svals16_cap = 0;
+  bool svals16_ok = false;
   int kind = HF_MG_CHEBYSHEV;
   int degree = 2;
   int degree_post = 0;       // post-smoothing steps on the grid levels (0: degree)
@@ -2269,6 +2329,7 @@ static int mg_free(hf_mg* mg) {
   if (mg->coef) (void)hipFree(mg->coef);
   if (mg->lsm.buf) (void)hipFree(mg->lsm.buf);
   if (mg->sdinv) (void)hipFree(mg->sdinv);
+  if (mg->svals16) (void)hipFree(mg->svals16);
   for (int l = 0; l < mg->nlev; ++l)
     if (mg->lv[l].S16) (void)hipFree(mg->lv[l].S16);
   if (mg->scl) (void)hipFree(mg->scl);
@@ -2569,6 +2630,27 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   const Lat gl = mg->lv[mg->nlev - 1].g;
   mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
   if (mg_coefficients(mg, s)) return -2;
+  // FP16 copy of a SELL fine level's values (scale in level 0's slot, unused by it)
+  mg->svals16_ok = false;
+  if (mg->f16 && mg->sell) {
+    const int64_t nv = hf_matrix_values(p);
+    if (nv > mg->svals16_cap) {
+      if (mg->svals16) HF_CHECK_HIP(hipFree(mg->svals16));
+      mg->svals16 = nullptr;
+      mg->svals16_cap = 0;
+      HF_CHECK_HIP(hipMalloc((void**)&mg->svals16, (size_t)nv * sizeof(__half)));
+      mg->svals16_cap = nv;
+      ++mg->gen;   // a kernel argument of the captured iteration
+    }
+    const int ga = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(nv / 4, kMgBlock), 1), kMaxGrid);
+    mg_absmax<<<ga, kMgBlock, 0, s>>>(vals, nv, mg->part);
+    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, ga, mg->scl + 2);
+    mg_f16_scale<<<1, kWave, 0, s>>>(mg->scl);
+    const int64_t nslots = nv / (kSellL * 3);
+    mg_sell_to_f16<<<(unsigned)ceil_div(nslots * kSellL, kMgBlock), kMgBlock, 0, s>>>(vals, nslots, mg->scl,
+                                                                                     mg->svals16);
+    mg->svals16_ok = true;
+  }
   // FP16 copies of the half-stored levels (level 0 only without a cell operator,
   // whose matrix-free rows replace its stencil in the cycle)
   for (int l = 0; l < mg->nlev; ++l) {
@@ -2773,14 +2855,21 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       // SELL fine level: each pass is y = A v, then the node-wise update
       const MatArgs m = mat_args(mg->sp);
       const int gr = rows_grid(m.nslices);
+      // y = A v with the FP16 values when the set-up made them
+      auto sell_spmv = [&](const float* vv) {
+        if (mg->svals16_ok)
+          mg_sell_spmv16<<<gr, kBlock, 0, s>>>(m, mg->svals16, mg->scl, vv, mg->y, sc);
+        else
+          mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, vv, mg->y, sc);
+      };
       for (int j = 1; j < k; ++j) {
-        mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
+        sell_spmv(din);
         mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, cf, j, res, mg->y, din, V.res, dout,
                                                     cur[l], sc);
         res = V.res;
         float* tmp = din; din = dout; dout = tmp;
       }
-      mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
+      sell_spmv(din);
       mg_sell_resid_update<<<gg, kMgBlock, 0, s>>>(V.g.n, res, mg->y, dout, sc);
     } else if (l == 0 && mg->mf_scale) {
       down(std::true_type{}, (const MfOp*)mg->mf, mg->mf_scale);
@@ -2866,13 +2955,24 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     if (l == 0 && mg->sell) {
       const MatArgs m = mat_args(mg->sp);
       const int gr = rows_grid(m.nslices);
+      // y = A v with the FP16 values when the set-up made them
+      auto sell_spmv = [&](const float* vv) {
+        if (mg->svals16_ok)
+          mg_sell_spmv16<<<gr, kBlock, 0, s>>>(m, mg->svals16, mg->scl, vv, mg->y, sc);
+        else
+          mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, vv, mg->y, sc);
+      };
       if (timed) {
         hipEvent_t e0, e1;
         if (prof_events(&e0, &e1)) return -2;
-        hipExtLaunchKernelGGL(mg_sell_spmv, dim3(gr), dim3(kBlock), 0, s, e0, e1, 0, m, mg->svals,
-                              (const float*)cur[l], mg->y, sc);
+        if (mg->svals16_ok)
+          hipExtLaunchKernelGGL(mg_sell_spmv16, dim3(gr), dim3(kBlock), 0, s, e0, e1, 0, m,
+                                (const __half*)mg->svals16, (const float*)mg->scl, (const float*)cur[l], mg->y, sc);
+        else
+          hipExtLaunchKernelGGL(mg_sell_spmv, dim3(gr), dim3(kBlock), 0, s, e0, e1, 0, m, mg->svals,
+                                (const float*)cur[l], mg->y, sc);
       } else {
-        mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, cur[l], mg->y, sc);
+        sell_spmv(cur[l]);
       }
       mg_sell_post_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, cf, rin(l), mg->y, cur[l], V.res, V.da, oth[l],
                                                   sc);
@@ -2880,7 +2980,7 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
       float* din = V.da;
       float* dout = V.db;
       for (int j = 1; j < kp; ++j) {
-        mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, din, mg->y, sc);
+        sell_spmv(din);
         mg_sell_step_update<<<gg, kMgBlock, 0, s>>>(V.Dinv, V.g.n, cf, j, V.res, mg->y, din, V.res, dout,
                                                     cur[l], sc);
         float* t2 = din; din = dout; dout = t2;

@@ -650,13 +650,18 @@ def test_hex27_level1_is_the_hex8_operator(jfa):
         np.testing.assert_allclose(G, r, rtol=0, atol=2e-5 * np.abs(r).max(), err_msg="level %d" % l)
 
 
-def test_hex27_vcycle_and_newton(jfa):
-    say("mg hex27 vcycle + newton")
+@pytest.mark.parametrize("f16", [False, True])
+def test_hex27_vcycle_and_newton(jfa, f16):
+    """HEX27 (the fine level is the SELL tangent): the V-cycle against scipy, with the
+    fine smoothing products in FP32 or from the FP16 copy of the SELL values (the
+    reference then smooths with M round16(A) M, the row-eliminated tangent rounded
+    with the same power-of-two scale); Newton through it against the FP64 path."""
+    say("mg hex27 vcycle + newton", "fp16 values" if f16 else "fp32 values")
     MG = __import__("jax_fem_amd.multigrid", fromlist=["x"])
     cells = (3, 2, 3)
     p27 = _hex27_problem(jfa, cells, jfa.NeoHookean(E=10.0, nu=0.3))
     A = tangent(jfa, p27, scale=0.01)
-    mg = MG.Multigrid(MG.lattice(p27), fine_sell=True)
+    mg = MG.Multigrid(MG.lattice(p27), fine_sell=True, fine_f16=f16)
     mask = MG.dirichlet_mask(p27)
     mg.setup(A, mask)
     dims = mg.levels()
@@ -668,8 +673,11 @@ def test_hex27_vcycle_and_newton(jfa):
     g = torch.Generator(device=DEV).manual_seed(4)
     r = torch.randn(p27.N, generator=g, device=DEV) * mask
     z = mg.apply(r).double().cpu().numpy()
-    zr, _ = vcycle_ref(As, dims, m, r.double().cpu().numpy(), "chebyshev", 2, 8.0, lams)
-    np.testing.assert_allclose(z, zr, rtol=0, atol=1e-4 * np.abs(zr).max())
+    A16 = [(sp.diags(m) @ round_f16(A.to_scipy()) @ sp.diags(m)).tocsr()] if f16 else None
+    zr, _ = vcycle_ref(As, dims, m, r.double().cpu().numpy(), "chebyshev", 2, 8.0, lams, A_smooth=A16)
+    err = np.abs(z - zr).max() / np.abs(zr).max()
+    say("  V-cycle vs scipy: max rel %.2e" % err)
+    assert err <= (1e-5 if f16 else 1e-4)
     # Newton through the multigrid vs the FP64 path
     u64 = jfa.solver(p27, {"newton": {"tol": 0.0, "rel_tol": 1e-10, "linear": {"pcg": {"tol": 1e-12, "atol": 1e-14}}}})[0]
     p2 = _hex27_problem(jfa, cells, jfa.NeoHookean(E=10.0, nu=0.3))
