@@ -1302,35 +1302,60 @@ __global__ void __launch_bounds__(kDenseApplyThreads) mg_dense_apply(const doubl
   dense_apply_wg<kDenseApplyThreads>(Ainv, nd, r, e);
 }
 
-// The levels from the first one with at most kSmallNodes nodes down, in one
-// workgroup: the whole sub-cycle with workgroup barriers between its phases.
-// A stencil product uses 8 lanes per node (slots o = q, q+8, ... on lane q,
-// summed with shuffles), so a phase is one round of loads rather than 27.
-// The first level's input and pre-smoother start were written by the
-// restriction above it; its result is left in its e2 buffer (the
-// post-smoother start swaps e and e2 once).
+// The levels from the first small one down run in one workgroup
+// (mg_small_cycle): the whole sub-cycle with workgroup barriers between its
+// phases and every level's data in LDS.  The stencils (in the form the level
+// holds them), the block inverses and the coefficients are read from global
+// memory once per cycle, in one round of loads that are all in flight
+// together; a phase is then a round of LDS reads, not of dependent global
+// loads.  A stencil product uses 8 lanes per node (slots o = q, q+8, ... on lane
+// q, summed with shuffles).  The first level's right-hand side and pre-smoother
+// start were written by the restriction above it; its result goes to its e2
+// buffer (or, with the dense level alone, to its e).
 constexpr int kSmallThreads = 1024;
 constexpr int kGroup = 8;                             // lanes per node
 constexpr int kSmallNodes = kSmallThreads / kGroup;   // 128 nodes a round
-// the cycle starts at the first level of at most kSmallStart nodes: one CU's
-// passes over a 6^3 level (216 nodes) cost more than the 6 launches they save
-constexpr int kSmallStart = 128;
-constexpr int kSmallMax = 4;
-constexpr int kSmallUnroll = 1;  // 2 fits in 100 VGPRs but measured slower here (37 -> 42 us)
+// the cycle takes in levels of at most kSmallStart nodes, at most kSmallMax of
+// them (the dense one included), while their LDS arena fits kSmallArenaWords
+constexpr int kSmallStart = 512;
+constexpr int kSmallMax = 5;
+constexpr int kSmallUnroll = 1;
+// 148 KB of arena + dense_apply_wg's 8.7 KB of LDS: within gfx950's 160 KB per workgroup
+constexpr int kSmallArenaWords = 148 * 1024 / 4;
+// stencil forms of a level in the small cycle
+constexpr int kSmallF16 = 0;     // FP16 words: 5 half2 per half-stored block (the grid levels' S16)
+constexpr int kSmallHalf = 1;    // FP32, diagonal and 13 upper slots
+constexpr int kSmallFull = 2;    // FP32, 27 slots
 struct SmallLevel {
   Lat g;
-  const float* S;
-  const float* Dinv;
-  float *r, *e, *e2, *da, *db, *res;
-  const float* cf;  // the level's smoothing coefficients (device memory)
+  int kind;          // stencil form (smoothed levels)
+  int words;         // its 4-byte words (0 for the dense level)
+  const void* S;     // the stencil (global)
+  const float* scl;  // FP16 form: the scale pair (scl[1] multiplies the products)
+  const float* Dinv; // block inverses (global, smoothed levels)
+  const float* cf;   // smoothing coefficients (global)
+  int oS, oD, oC, oV;  // arena offsets (words): stencil, block inverses, node coordinates, vectors
 };
 struct SmallCycle {
   int nlev;     // levels here, the last one dense
   int degree;
   const double* Ainv;
   int nd;
+  const float *r0, *e0, *d0;   // first level: right-hand side, pre-smoother start (e = d)
+  float* out;                  // first level's result
   SmallLevel lv[kSmallMax];
 };
+// Arena words of one small-cycle level.  Vectors are float4 per node (one
+// 16-byte LDS read per neighbour); a smoothed level holds its stencil, its block
+// inverses, its node coordinates (one packed word per node) and six vectors (r,
+// two e, two d, res); the dense one its coordinates, r and e.
+// (+3 per array: each starts on a 16-byte boundary)
+__host__ __device__ inline int64_t small_level_words(int kind, int64_t n, bool dense) {
+  if (dense) return 9 * n + 6;
+  const int64_t st = kind == kSmallF16 ? (int64_t)kHalfSlots * kF16Words * n
+                     : kind == kSmallHalf ? (int64_t)kHalfSlots * 9 * n : (int64_t)kSt * n;
+  return st + 9 * n + n + 24 * n + 12;
+}
 
 // product of node n's stencil row with v, by the 8 lanes of a group (q = lane
 // in group); every lane of the group ends with the whole sum.  UNROLL (1, 2 or
@@ -1406,95 +1431,389 @@ __device__ __forceinline__ void stencil_row_group(const ST* __restrict__ S, cons
     __syncthreads();                                                                         \
   } while (0)
 
+// n 4-byte words from global memory into LDS (both 16-byte aligned): 16-byte
+// loads, four in flight per thread, then the tail word by word
+__device__ __forceinline__ void lds_fill(float* __restrict__ dst, const float* __restrict__ src, int n) {
+  constexpr int U = 4;
+  const int n4 = n >> 2;
+  const float4* s4 = reinterpret_cast<const float4*>(src);
+  float4* d4 = reinterpret_cast<float4*>(dst);
+  for (int base = threadIdx.x; base < n4; base += U * kSmallThreads) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * kSmallThreads;
+      v[u] = i < n4 ? s4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * kSmallThreads;
+      if (i < n4) d4[i] = v[u];
+    }
+  }
+  const int i = 4 * n4 + (int)threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+// a 3-vector of n nodes (global, 3 floats a node) into LDS float4 slots
+__device__ __forceinline__ void lds_fill3(float* __restrict__ dst, const float* __restrict__ src, int n) {
+  constexpr int U = 4;
+  const int n3 = 3 * n;
+  for (int base = threadIdx.x; base < n3; base += U * kSmallThreads) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * kSmallThreads;
+      v[u] = i < n3 ? src[i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * kSmallThreads;
+      if (i < n3) dst[4 * (i / 3) + i % 3] = v[u];
+    }
+  }
+}
+
+// sum over the 8 lanes of a group, every lane ending with it: quad swaps by DPP
+// (lanes 1,0,3,2 then 2,3,0,1), then each lane adds the lane mirrored in its half
+// row (i <-> 7 - i), which holds the other quad's sum
+__device__ __forceinline__ float group8_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, true));
+  return v;
+}
+
+__device__ __forceinline__ void small_xyz(uint32_t w, int& x, int& y, int& z) {
+  x = (int)(w & 1023u);
+  y = (int)((w >> 10) & 1023u);
+  z = (int)(w >> 20);
+}
+
+// Node n's stencil row (level V, stencil St in LDS, form KIND) times the float4
+// vector v, by the 8 lanes of a group: lane q takes slots q, q + 8, q + 16, q + 24.
+// Slots below 13 of a half-stored stencil are the neighbour's upper slot 13 - o
+// transposed; the slot of a neighbour outside the lattice is skipped (its
+// stored block is zero, or it is a lower block the node does not have).
+template <int KIND>
+__device__ __forceinline__ void small_row(const float* __restrict__ St, int N, const Lat& g, int x, int y, int z,
+                                          int node, const float4* __restrict__ v, int q, float sinv,
+                                          float (&out)[3]) {
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int o = q + 8 * t;
+    if (o >= 27) break;
+    const int ox = o / 9, oy = (o / 3) % 3, oz = o % 3;
+    const int xr = x + ox - 1, yr = y + oy - 1, zr = z + oz - 1;
+    if (xr < 0 || xr >= g.nx || yr < 0 || yr >= g.ny || zr < 0 || zr >= g.nz) continue;
+    const int m = (xr * g.ny + yr) * g.nz + zr;
+    const float4 w = v[m];
+    const float wv[3] = {w.x, w.y, w.z};
+    float T[10];
+    if constexpr (KIND == kSmallFull) {
+#pragma unroll
+      for (int e = 0; e < 9; ++e) T[e] = St[(o * 9 + e) * N + node];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc[i] += T[3 * i] * wv[0] + T[3 * i + 1] * wv[1] + T[3 * i + 2] * wv[2];
+    } else {
+      // slots 0..7 are lower, 16.. upper; 8..15 either (a lane-dependent branch)
+      const bool own = t >= 2 || (t == 1 && o >= 13);
+      const int sl = own ? o - 13 : 13 - o;
+      const int at = own ? node : m;
+      if constexpr (KIND == kSmallF16) {
+        const __half2* W = reinterpret_cast<const __half2*>(St);
+#pragma unroll
+        for (int k = 0; k < kF16Words; ++k) {
+          const float2 f = __half22float2(W[(sl * kF16Words + k) * N + at]);
+          T[2 * k] = f.x;
+          T[2 * k + 1] = f.y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) T[e] = St[(sl * 9 + e) * N + at];
+      }
+      if (own) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i] += T[3 * i] * wv[0] + T[3 * i + 1] * wv[1] + T[3 * i + 2] * wv[2];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i] += T[i] * wv[0] + T[3 + i] * wv[1] + T[6 + i] * wv[2];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out[i] = group8_sum(acc[i]) * sinv;
+}
+
+// rc = P^T t at coarse node I (float4 vectors in LDS); with Dc given, also the
+// coarse pre-smoother's start dc = ec = s0 Dc rc (restrict_node's transfer)
+__device__ __forceinline__ void small_restrict(const float4* __restrict__ t, const Lat& f, const Lat& c,
+                                               const uint32_t* __restrict__ cxyz, float4* __restrict__ rc,
+                                               const float* __restrict__ Dc, float s0, float4* __restrict__ dc,
+                                               float4* __restrict__ ec, int I) {
+  const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
+  int ci[3];
+  small_xyz(cxyz[I], ci[0], ci[1], ci[2]);
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  int fr[3][3];
+  float wr[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int p = cpos(ci[d], nf[d]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int fidx = p + a - 1;
+      const bool ok = fidx >= 0 && fidx <= nf[d];
+      fr[d][a] = ok ? fidx : p;
+      wr[d][a] = ok ? p1(fidx, ci[d], nf[d]) : 0.f;
+    }
+  }
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int ay = 0; ay < 3; ++ay)
+#pragma unroll
+      for (int az = 0; az < 3; ++az) {
+        const float w = wr[0][ax] * wr[1][ay] * wr[2][az];
+        const float4 tv = t[(fr[0][ax] * f.ny + fr[1][ay]) * f.nz + fr[2][az]];
+        acc.x += w * tv.x;
+        acc.y += w * tv.y;
+        acc.z += w * tv.z;
+      }
+  rc[I] = acc;
+  if (Dc) {
+    const int n = (int)c.n;
+    const float a3[3] = {acc.x, acc.y, acc.z};
+    float o[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      o[i] = s0 * (Dc[(3 * i) * n + I] * a3[0] + Dc[(3 * i + 1) * n + I] * a3[1] + Dc[(3 * i + 2) * n + I] * a3[2]);
+    const float4 ov = make_float4(o[0], o[1], o[2], 0.f);
+    dc[I] = ov;
+    ec[I] = ov;
+  }
+}
+
+// e += P ec at fine node i (float4 vectors in LDS; prolong_node's transfer)
+__device__ __forceinline__ void small_prolong(const float4* __restrict__ ec, const Lat& f, const Lat& c,
+                                              const uint32_t* __restrict__ fxyz, float4* __restrict__ e, int i) {
+  const int nf[3] = {f.nx - 1, f.ny - 1, f.nz - 1};
+  const int nc[3] = {c.nx - 1, c.ny - 1, c.nz - 1};
+  int fi[3];
+  small_xyz(fxyz[i], fi[0], fi[1], fi[2]);
+  int pc[3][2];
+  float pw[3][2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int v = fi[d];
+    if (v % 2 == 0) {
+      pc[d][0] = v / 2; pw[d][0] = 1.f; pc[d][1] = v / 2; pw[d][1] = 0.f;
+    } else if (v == nf[d]) {
+      pc[d][0] = nc[d]; pw[d][0] = 1.f; pc[d][1] = nc[d]; pw[d][1] = 0.f;
+    } else {
+      pc[d][0] = (v - 1) / 2; pw[d][0] = 0.5f; pc[d][1] = (v + 1) / 2; pw[d][1] = 0.5f;
+    }
+  }
+  float4 acc = e[i];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const float w = pw[0][a] * pw[1][b] * pw[2][cc];
+        const float4 v = ec[(pc[0][a] * c.ny + pc[1][b]) * c.nz + pc[2][cc]];
+        acc.x += w * v.x;
+        acc.y += w * v.y;
+        acc.z += w * v.z;
+      }
+  e[i] = acc;
+}
+
+// One smoothing phase of a small-cycle level over its nodes, 8 lanes a node; after
+// the row product ad = A din, lane i < 3 of the group finishes component i.
+//   MODE 0 (a Chebyshev step):  rr = res - ad, dn = c1 din + c2 D rr, res_out = rr,
+//                               dout = dn, e += dn
+//   MODE 1 (the residual):      dout = res - ad
+//   MODE 2 (post start, din = e): rr = r - ad, res_out = rr, dout = s0 D rr,
+//                               e_out = e + s0 D rr
+template <int KIND, int MODE>
+__device__ __forceinline__ void small_phase(const SmallLevel& V, const float* __restrict__ St,
+                                            const float* __restrict__ D, const uint32_t* __restrict__ xyz,
+                                            float sinv, const float4* res, const float4* din, float4* res_out,
+                                            float4* dout, float4* e, const float4* ein, float c1, float c2) {
+  const int N = (int)V.g.n;
+  const int q = threadIdx.x % kGroup;
+  for (int node = threadIdx.x / kGroup; node < N; node += kSmallNodes) {
+    int x, y, z;
+    small_xyz(xyz[node], x, y, z);
+    float ad[3];
+    small_row<KIND>(St, N, V.g, x, y, z, node, din, q, sinv, ad);
+    if (q < 3) {
+      const float4 rv = res[node];
+      const float rr[3] = {rv.x - ad[0], rv.y - ad[1], rv.z - ad[2]};
+      float* dv = reinterpret_cast<float*>(dout + node);
+      if (MODE == 1) {
+        dv[q] = rr[q];
+      } else {
+        const float dr = D[(3 * q) * N + node] * rr[0] + D[(3 * q + 1) * N + node] * rr[1] +
+                         D[(3 * q + 2) * N + node] * rr[2];
+        float* ev = reinterpret_cast<float*>(e + node);
+        reinterpret_cast<float*>(res_out + node)[q] = rr[q];
+        if (MODE == 0) {
+          const float dn = c1 * reinterpret_cast<const float*>(din + node)[q] + c2 * dr;
+          dv[q] = dn;
+          ev[q] += dn;
+        } else {
+          dv[q] = c1 * dr;
+          ev[q] = reinterpret_cast<const float*>(ein + node)[q] + c1 * dr;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int MODE>
+__device__ __forceinline__ void small_phase_any(const SmallLevel& V, const float* A, float sinv, const float4* res,
+                                                const float4* din, float4* res_out, float4* dout, float4* e,
+                                                const float4* ein, float c1, float c2) {
+  const float* St = A + V.oS;
+  const float* D = A + V.oD;
+  const uint32_t* xyz = reinterpret_cast<const uint32_t*>(A + V.oC);
+  if (V.kind == kSmallF16)
+    small_phase<kSmallF16, MODE>(V, St, D, xyz, sinv, res, din, res_out, dout, e, ein, c1, c2);
+  else if (V.kind == kSmallHalf)
+    small_phase<kSmallHalf, MODE>(V, St, D, xyz, 1.f, res, din, res_out, dout, e, ein, c1, c2);
+  else
+    small_phase<kSmallFull, MODE>(V, St, D, xyz, 1.f, res, din, res_out, dout, e, ein, c1, c2);
+}
+
+// e = Ainv r for the dense level (dense_apply_wg's scheme), r and e read and
+// written through the index maps ri(i) and ei(i) (global stride 3 or LDS float4)
+template <typename RI, typename EI>
+__device__ __forceinline__ void small_dense(const double* __restrict__ Ainv, int nd, RI ri, EI ei, double* rs,
+                                            double* part) {
+  const int tid = threadIdx.x;
+  if (tid < nd) rs[tid] = (double)ri(tid);
+  __syncthreads();
+  const int ng = kSmallThreads / nd, i = tid % nd, jg = tid / nd;
+  double acc = 0.0;
+  if (jg < ng)
+    for (int j = jg; j < nd; j += ng) acc += Ainv[(int64_t)j * nd + i] * rs[j];
+  part[tid] = acc;
+  __syncthreads();
+  if (tid < nd) {
+    double v = 0.0;
+    for (int kk = 0; kk < ng; ++kk) v += part[kk * nd + tid];
+    ei(tid) = (float)v;
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(kSmallThreads) mg_small_cycle(SmallCycle sc_, const Scalars* sc) {
   if (sc && sc->done) return;
   const int L = sc_.nlev, k = sc_.degree;
-  float* cur[kSmallMax];
-  float* oth[kSmallMax];
+  __shared__ __align__(16) float A[kSmallArenaWords];
+  __shared__ float cfl[kSmallMax][kCoefStride];
+  __shared__ float sinv[kSmallMax];
+  __shared__ double rs[kDenseMax];
+  __shared__ double part[kSmallThreads];
+  // the level descriptors in LDS (indexed by level at run time; the kernel
+  // argument is read with constant indices only)
+  __shared__ SmallLevel slv[kSmallMax];
 #pragma unroll
-  for (int l = 0; l < kSmallMax; ++l) {
-    cur[l] = sc_.lv[l].e;
-    oth[l] = sc_.lv[l].e2;
-  }
-  for (int l = 0; l + 1 < L; ++l) {
-    const SmallLevel& V = sc_.lv[l];
-    const float* res = V.r;
-    float* din = V.da;
-    float* dout = V.db;
-    for (int j = 1; j < k; ++j) {
-      MG_GROUP_LOOP(V.g.n, {
-        float ad[3];
-        stencil_row_group<false, kSmallUnroll>(V.S, V.g, node, din, q, ad);
-        if (q == 0) {
-          const float rr[3] = {res[3 * node] - ad[0], res[3 * node + 1] - ad[1], res[3 * node + 2] - ad[2]};
-          float dr[3];
-          block_apply(V.Dinv, V.g.n, node, rr, dr);
-          for (int i = 0; i < 3; ++i) {
-            const float dn = V.cf[1 + j] * din[3 * node + i] + V.cf[1 + kMaxDegree + j] * dr[i];
-            V.res[3 * node + i] = rr[i];
-            dout[3 * node + i] = dn;
-            cur[l][3 * node + i] += dn;
-          }
-        }
-      });
-      res = V.res;
-      float* tmp = din; din = dout; dout = tmp;
-    }
-    MG_GROUP_LOOP(V.g.n, {
-      float ad[3];
-      stencil_row_group<false, kSmallUnroll>(V.S, V.g, node, din, q, ad);
-      if (q == 0)
-        for (int i = 0; i < 3; ++i) dout[3 * node + i] = res[3 * node + i] - ad[i];
-    });
-    const SmallLevel& C = sc_.lv[l + 1];
-    const bool smooth_c = (l + 2 < L);
-    MG_WG_LOOP(C.g.n, restrict_node(dout, V.g, C.g, C.r, smooth_c ? C.Dinv : nullptr, C.cf[0], C.da, cur[l + 1], node));
-  }
-  dense_apply_wg<kSmallThreads>(sc_.Ainv, sc_.nd, sc_.lv[L - 1].r, cur[L - 1]);
+  for (int l = 0; l < kSmallMax; ++l)
+    if (threadIdx.x == l) slv[l] = sc_.lv[l];
   __syncthreads();
-  for (int l = L - 2; l >= 0; --l) {
-    const SmallLevel& V = sc_.lv[l];
-    const SmallLevel& C = sc_.lv[l + 1];
-    MG_WG_LOOP(V.g.n, prolong_node(cur[l + 1], V.g, C.g, nullptr, cur[l], node));
-    float* ein = cur[l];
-    float* eout = oth[l];
-    MG_GROUP_LOOP(V.g.n, {
-      float ae[3];
-      stencil_row_group<false, kSmallUnroll>(V.S, V.g, node, ein, q, ae);
-      if (q == 0) {
-        const float rr[3] = {V.r[3 * node] - ae[0], V.r[3 * node + 1] - ae[1], V.r[3 * node + 2] - ae[2]};
-        float dr[3];
-        block_apply(V.Dinv, V.g.n, node, rr, dr);
-        for (int i = 0; i < 3; ++i) {
-          V.res[3 * node + i] = rr[i];
-          V.da[3 * node + i] = V.cf[0] * dr[i];
-          eout[3 * node + i] = ein[3 * node + i] + V.cf[0] * dr[i];
-        }
-      }
-    });
-    cur[l] = eout;
-    oth[l] = ein;
-    float* din = V.da;
-    float* dout = V.db;
-    for (int j = 1; j < k; ++j) {
-      MG_GROUP_LOOP(V.g.n, {
-        float ad[3];
-        stencil_row_group<false, kSmallUnroll>(V.S, V.g, node, din, q, ad);
-        if (q == 0) {
-          const float rr[3] = {V.res[3 * node] - ad[0], V.res[3 * node + 1] - ad[1], V.res[3 * node + 2] - ad[2]};
-          float dr[3];
-          block_apply(V.Dinv, V.g.n, node, rr, dr);
-          for (int i = 0; i < 3; ++i) {
-            const float dn = V.cf[1 + j] * din[3 * node + i] + V.cf[1 + kMaxDegree + j] * dr[i];
-            V.res[3 * node + i] = rr[i];
-            dout[3 * node + i] = dn;
-            eout[3 * node + i] += dn;
-          }
-        }
-      });
-      float* t2 = din; din = dout; dout = t2;
+  if (L == 1) {   // the dense level alone
+    const float* r0 = sc_.r0;
+    float* out = sc_.out;
+    small_dense(sc_.Ainv, sc_.nd, [&](int i) { return r0[i]; }, [&](int i) -> float& { return out[i]; }, rs, part);
+    return;
+  }
+  // stage: every smoothed level's stencil, block inverses and coefficients, every
+  // level's node coordinates, the first level's r, e and d starts
+  for (int l = 0; l < L; ++l) {
+    const SmallLevel& V = slv[l];
+    if (l + 1 < L) {
+      lds_fill(A + V.oS, static_cast<const float*>(V.S), V.words);
+      lds_fill(A + V.oD, V.Dinv, (int)(9 * V.g.n));
+      if (threadIdx.x < kCoefStride) cfl[l][threadIdx.x] = V.cf[threadIdx.x];
+      if (threadIdx.x == 0) sinv[l] = V.kind == kSmallF16 ? V.scl[1] : 1.f;
+    }
+    uint32_t* xyz = reinterpret_cast<uint32_t*>(A + V.oC);
+    for (int n = threadIdx.x; n < (int)V.g.n; n += kSmallThreads) {
+      const int z = n % V.g.nz, qq = n / V.g.nz;
+      xyz[n] = (uint32_t)(qq / V.g.ny) | ((uint32_t)(qq % V.g.ny) << 10) | ((uint32_t)z << 20);
     }
   }
+  {
+    const SmallLevel& F = slv[0];
+    const int n = (int)F.g.n;
+    float* v0 = A + F.oV;
+    lds_fill3(v0, sc_.r0, n);            // r
+    lds_fill3(v0 + 4 * n, sc_.e0, n);    // e (the start)
+    lds_fill3(v0 + 12 * n, sc_.d0, n);   // d (the start)
+  }
+  __syncthreads();
+  // level l's vectors: r, e (two buffers), d (two buffers), res; the dense level r, e
+  auto vec = [&](int l, int i) { return reinterpret_cast<float4*>(A + slv[l].oV) + i * slv[l].g.n; };
+  int ecur[kSmallMax];   // which e buffer (1 or 2) holds each level's current e
+#pragma unroll
+  for (int l = 0; l < kSmallMax; ++l) ecur[l] = 1;
+  for (int l = 0; l + 1 < L; ++l) {
+    const SmallLevel& V = slv[l];
+    const float* cf = cfl[l];
+    const float4* res = vec(l, 0);
+    float4* din = vec(l, 3);
+    float4* dout = vec(l, 4);
+    float4* e = vec(l, 1);
+    for (int j = 1; j < k; ++j) {
+      small_phase_any<0>(V, A, sinv[l], res, din, vec(l, 5), dout, e, nullptr, cf[1 + j], cf[1 + kMaxDegree + j]);
+      res = vec(l, 5);
+      float4* tmp = din; din = dout; dout = tmp;
+    }
+    small_phase_any<1>(V, A, sinv[l], res, din, nullptr, dout, nullptr, nullptr, 0.f, 0.f);
+    const SmallLevel& C = slv[l + 1];
+    const bool smooth_c = (l + 2 < L);
+    const uint32_t* cxyz = reinterpret_cast<const uint32_t*>(A + C.oC);
+    for (int I = threadIdx.x; I < (int)C.g.n; I += kSmallThreads)
+      small_restrict(dout, V.g, C.g, cxyz, vec(l + 1, 0), smooth_c ? A + C.oD : nullptr, smooth_c ? cfl[l + 1][0] : 0.f,
+                     vec(l + 1, 3), vec(l + 1, 1), I);
+    __syncthreads();
+  }
+  {
+    // dense level: e = Ainv r (FP64 accumulation) on its float4 slots
+    const float* r = reinterpret_cast<const float*>(vec(L - 1, 0));
+    float* e = reinterpret_cast<float*>(vec(L - 1, 1));
+    small_dense(sc_.Ainv, sc_.nd, [&](int i) { return r[4 * (i / 3) + i % 3]; },
+                [&](int i) -> float& { return e[4 * (i / 3) + i % 3]; }, rs, part);
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    const SmallLevel& V = slv[l];
+    const SmallLevel& C = slv[l + 1];
+    const float* cf = cfl[l];
+    float4* ein = vec(l, ecur[l]);
+    float4* eout = vec(l, 3 - ecur[l]);
+    const float4* ec = vec(l + 1, ecur[l + 1]);
+    const uint32_t* fxyz = reinterpret_cast<const uint32_t*>(A + V.oC);
+    for (int i = threadIdx.x; i < (int)V.g.n; i += kSmallThreads) small_prolong(ec, V.g, C.g, fxyz, ein, i);
+    __syncthreads();
+    small_phase_any<2>(V, A, sinv[l], vec(l, 0), ein, vec(l, 5), vec(l, 3), eout, ein, cf[0], 0.f);
+    ecur[l] = 3 - ecur[l];
+    float4* din = vec(l, 3);
+    float4* dout = vec(l, 4);
+    for (int j = 1; j < k; ++j) {
+      small_phase_any<0>(V, A, sinv[l], vec(l, 5), din, vec(l, 5), dout, eout, nullptr, cf[1 + j],
+                         cf[1 + kMaxDegree + j]);
+      float4* t2 = din; din = dout; dout = t2;
+    }
+  }
+  // the first level's result back to global memory
+  const float* e0 = reinterpret_cast<const float*>(vec(0, ecur[0]));
+  const int n3 = (int)(3 * slv[0].g.n);
+  for (int i = threadIdx.x; i < n3; i += kSmallThreads) sc_.out[i] = e0[4 * (i / 3) + i % 3];
 }
 
 // Levels of up to kLaneNodes nodes above the small cycle run the stencil passes
@@ -2302,14 +2621,28 @@ struct hf_mg {
 
 static const float* coef_of(const hf_mg* mg, int l) { return mg->coef + (int64_t)l * kCoefStride; }
 
-// First level run inside mg_small_cycle (the levels from it down), or nlev - 1
-// (only the dense solve) when no level is small enough.
+// A small-cycle level's stencil form (as the level holds it after a set-up).
+static int small_kind(const MgLevel& V) {
+  return V.S16 ? kSmallF16 : V.half ? kSmallHalf : kSmallFull;
+}
+
+// First level run inside mg_small_cycle (the levels from it down): the coarsest
+// levels, from the dense one up, while a level has at most kSmallStart nodes, the
+// cycle at most kSmallMax levels and their LDS arena fits; nlev - 1 (only the dense
+// solve) when no level above the dense one qualifies.  The fine level never.
 static int small_cycle_start(const hf_mg* mg) {
   const int L = mg->nlev;
   if (L == 1) return 0;
-  for (int l = 1; l < L; ++l)
-    if (mg->lv[l].g.n <= kSmallStart && L - l <= kSmallMax) return l;
-  return L - 1;
+  int lb = L - 1;
+  int64_t words = small_level_words(0, mg->lv[L - 1].g.n, true);
+  for (int l = L - 2; l >= 1; --l) {
+    const MgLevel& V = mg->lv[l];
+    if (V.g.n > kSmallStart || L - l > kSmallMax) break;
+    words += small_level_words(small_kind(V), V.g.n, false);
+    if (words > kSmallArenaWords) break;
+    lb = l;
+  }
+  return lb;
 }
 
 static int mg_free(hf_mg* mg) {
@@ -2371,13 +2704,12 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
     HF_REQUIRE(false, "hf_mg_create: a SELL fine level needs at least one coarser level");
   }
   for (int l = 0; l < nlev; ++l) mg->lv[l].g = make_lat(dims[l][0], dims[l][1], dims[l][2]);
-  // Coarse levels run by grid kernels are half-stored too (Galerkin products of a
-  // symmetric operator); the one-workgroup cycle's levels and the dense one keep
-  // 27 slots.  HF_MG_FINE_FULL (an operator that need not be symmetric) keeps all.
+  // Coarse levels are half-stored too (Galerkin products of a symmetric
+  // operator); the dense one keeps 27 slots (its set-up reads them).
+  // HF_MG_FINE_FULL (an operator that need not be symmetric) keeps all.
   mg->lv[0].half = mg->half;
-  const int lb = small_cycle_start(mg);
   // (a SELL fine level's level 1 comes from mg_rap_ptap, which writes 27 slots)
-  for (int l = mg->sell ? 2 : 1; l < lb; ++l) mg->lv[l].half = !(flags & HF_MG_FINE_FULL);
+  for (int l = mg->sell ? 2 : 1; l + 1 < nlev; ++l) mg->lv[l].half = !(flags & HF_MG_FINE_FULL);
   auto alloc = [&](float** p, int64_t n) -> int {
     HF_CHECK_HIP(hipMalloc((void**)p, (size_t)n * sizeof(float)));
     return 0;
@@ -2408,9 +2740,10 @@ extern "C" int hf_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t flags, h
   }
   if (!rc) rc |= alloc(&mg->part, kMaxGrid + kMgMaxLevels);
   if (!rc && mg->f16) {
-    // an FP16 copy for every half-stored level the grid kernels smooth
+    // an FP16 copy for every half-stored level (its smoothing passes and
+    // restricted residuals read it)
     rc |= alloc(&mg->scl, 4 * kMgMaxLevels);
-    for (int l = 0; l < lb && !rc; ++l)
+    for (int l = 0; l + 1 < mg->nlev && !rc; ++l)
       if (mg->lv[l].half && hipMalloc((void**)&mg->lv[l].S16, (size_t)kHalfSlots * kF16Words * mg->lv[l].g.n *
                                                                   sizeof(__half2)) != hipSuccess) {
         set_error("hf_mg_create: hipMalloc failed");
@@ -2890,17 +3223,38 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     mg_dense_apply<<<1, kDenseApplyThreads, 0, s>>>(mg->Ainv, mg->nd, rin(0), cur[0], sc);
   } else {
     SmallCycle c;
+    memset(&c, 0, sizeof(c));
     c.nlev = L - lb;
     c.degree = k;
     c.Ainv = mg->Ainv;
     c.nd = mg->nd;
+    const MgLevel& F = mg->lv[lb];
+    c.r0 = F.r;
+    c.e0 = F.e;
+    c.d0 = F.da;
+    c.out = (L - lb >= 2) ? F.e2 : F.e;
+    int64_t at = 0;   // arena words
     for (int l = lb; l < L; ++l) {
       const MgLevel& V = mg->lv[l];
       SmallLevel& q = c.lv[l - lb];
-      q.g = V.g; q.S = V.S; q.Dinv = V.Dinv;
-      q.r = V.r; q.e = V.e; q.e2 = V.e2; q.da = V.da; q.db = V.db; q.res = V.res;
+      const bool dense = l == L - 1;
+      q.g = V.g;
+      q.kind = V.s16_ok ? kSmallF16 : V.half ? kSmallHalf : kSmallFull;
+      q.words = dense ? 0 : (int)(small_level_words(q.kind, V.g.n, false) - 34 * V.g.n - 12);
+      q.S = V.s16_ok ? (const void*)V.S16 : (const void*)V.S;
+      q.scl = mg->scl ? mg->scl + 4 * l : nullptr;
+      q.Dinv = V.Dinv;
       q.cf = coef_of(mg, l);
+      // every array 16-byte aligned (the staging copies 16-byte words)
+      auto up4 = [](int64_t w) { return (w + 3) & ~(int64_t)3; };
+      q.oS = (int)at;
+      q.oD = (int)up4(at + q.words);
+      q.oC = dense ? (int)at : (int)up4(q.oD + 9 * V.g.n);
+      q.oV = (int)up4(q.oC + V.g.n);
+      at = up4(q.oV + (dense ? 8 : 24) * V.g.n);
     }
+    HF_REQUIRE(at <= kSmallArenaWords, "vcycle: small-cycle levels need %lld LDS words, the arena holds %d",
+               (long long)at, kSmallArenaWords);
     mg_small_cycle<<<1, kSmallThreads, 0, s>>>(c, sc);
     cur[lb] = (L - lb >= 2) ? mg->lv[lb].e2 : mg->lv[lb].e;
   }

@@ -250,22 +250,17 @@ def round_f16(A):
     return a
 
 
-def grid_levels(dims):
-    """The levels the grid kernels smooth (the rest run in the one-workgroup cycle):
-    up to the first level of at most 128 nodes with at most 4 levels left."""
-    L = len(dims)
-    if L == 1:
-        return 0
-    for l in range(1, L):
-        if int(np.prod(dims[l])) <= 128 and L - l <= 4:
-            return l
-    return L - 1
+def f16_levels(dims):
+    """The levels smoothed from an FP16 copy of their stencil: every half-stored one,
+    i.e. every level but the dense coarsest (the grid kernels' levels and the
+    one-workgroup cycle's alike)."""
+    return max(len(dims) - 1, 0)
 
 
 @pytest.mark.parametrize("cells", [(6, 5, 7), (8, 6, 7), (16, 12, 14)])
 def test_vcycle_f16_vs_scipy(jfa, cells):
-    """HF_MG_FINE_F16: the smoothing passes and restricted residuals of the levels the
-    grid kernels run use FP16-rounded stencils (packed 5 half2 words per block, lower
+    """HF_MG_FINE_F16: the smoothing passes and restricted residuals of every level but
+    the dense one use FP16-rounded stencils (packed 5 half2 words per block, lower
     blocks read transposed from the neighbour, a power-of-two scale per level), the
     block inverses, bounds and coarser levels the FP32 ones."""
     say("mg vcycle fp16 fine level", cells)
@@ -286,10 +281,10 @@ def test_vcycle_f16_vs_scipy(jfa, cells):
     r = torch.randn(prob.N, generator=g, device=DEV) * mask
     rr = r.double().cpu().numpy()
     z = mg16.apply(r).double().cpu().numpy()
-    lb = grid_levels(dims)
-    # each grid level's own FP32 stencil, rounded as the set-up rounds it
+    lb = f16_levels(dims)
+    # each smoothed level's own FP32 stencil, rounded as the set-up rounds it
     A16 = [round_f16(stencil_to_csr(mg16.stencil(l).cpu().numpy(), dims[l])) for l in range(lb)]
-    say("  %d levels, %d smoothed by the grid kernels in FP16" % (len(dims), lb))
+    say("  %d levels, %d smoothed in FP16" % (len(dims), lb))
     z16, _ = vcycle_ref(As, dims, m, rr, "chebyshev", 2, 8.0, lams, A_smooth=A16)
     z32, _ = vcycle_ref(As, dims, m, rr, "chebyshev", 2, 8.0, lams)
     e16 = np.abs(z - z16).max() / np.abs(z16).max()
