@@ -1149,53 +1149,6 @@ __global__ void __launch_bounds__(kMgBlock) mg_prolong(const float* __restrict__
   MG_NODE_LOOP(f.n, prolong_node(ec, f, c, mask, e, node));
 }
 
-// Gershgorin bound of Dinv A: per-workgroup max over nodes of the largest
-// absolute row sum of the node's block row of Dinv A.
-template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S, const float* __restrict__ Dinv, Lat g,
-                                                     const float* __restrict__ mask, float* __restrict__ part) {
-  __shared__ float sm[kMgBlock / kWave];
-  const int64_t n = g.n;
-  float best = 0.f;
-  for (int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x; node < n; node += (int64_t)gridDim.x * kMgBlock) {
-    float M[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) M[e] = Dinv[(int64_t)e * n + node];
-    int x, y, z;
-    lat_xyz(g, node, x, y, z);
-    float rs[3] = {0.f, 0.f, 0.f};
-    for (int t = 0; t < 27; ++t) {
-      const int o = pair_slot(t);
-      const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
-      const bool valid = xr >= 0 && xr < g.nx && yr >= 0 && yr < g.ny && zr >= 0 && zr < g.nz;
-      const int64_t m = lat_id(g, clampi(xr, 0, g.nx - 1), clampi(yr, 0, g.ny - 1), clampi(zr, 0, g.nz - 1));
-      float B[9];
-      load_block<HALF>(S, n, node, o, m, valid, B);
-      // columns of Dirichlet DOFs (M A M; the rows are zero in Dinv already)
-      float cm[3] = {1.f, 1.f, 1.f};
-      if (mask) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) cm[j] = mask[3 * m + j];
-      }
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          rs[i] += fabsf(M[3 * i] * B[j] + M[3 * i + 1] * B[3 + j] + M[3 * i + 2] * B[6 + j]) * cm[j];
-    }
-    best = fmaxf(best, fmaxf(rs[0], fmaxf(rs[1], rs[2])));
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o, 64));
-  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = best;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float m = sm[0];
-    for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
-    part[blockIdx.x] = m;
-  }
-}
-
 __global__ void __launch_bounds__(kMgBlock) mg_max_final(const float* __restrict__ part, int np,
                                                          float* __restrict__ out) {
   __shared__ float sm[kMgBlock / kWave];
@@ -1249,22 +1202,6 @@ __global__ void mg_f16_scale(float* __restrict__ scl) {
   scl[1] = 1.f / sc;
 }
 
-// W[(s*5 + k)*n + node] = half2(scl[0] * S[(s*9 + 2k)*n + node], scl[0] * S[(s*9 + 2k + 1)*n + node]),
-// one thread per (node, slot = blockIdx.y)
-__global__ void __launch_bounds__(kMgBlock) mg_to_f16(const float* __restrict__ S, int64_t n,
-                                                      const float* __restrict__ scl, __half2* __restrict__ W) {
-  const int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
-  if (node >= n) return;
-  const int sl = blockIdx.y;
-  const float sc = scl[0];
-  float v[2 * kF16Words];
-#pragma unroll
-  for (int e = 0; e < 9; ++e) v[e] = sc * S[(int64_t)(sl * 9 + e) * n + node];
-  v[9] = 0.f;
-#pragma unroll
-  for (int k = 0; k < kF16Words; ++k)
-    W[(int64_t)(sl * kF16Words + k) * n + node] = __floats2half2_rn(v[2 * k], v[2 * k + 1]);
-}
 
 // coarsest level: e = Ainv r (FP64 accumulation), one workgroup
 // e = Ainv r for the dense coarsest level (Ainv column-major FP64, nd <= kDenseMax)
@@ -1837,6 +1774,141 @@ inline int lane_grid(int64_t n) {
   if (g >= 64) g = ceil_div(g, kNumXcd) * kNumXcd;  // whole XCDs, rounded up: no workgroup gets a second round
   if (g > kMaxGrid) g = kMaxGrid;
   return (int)(g < 1 ? 1 : g);
+}
+
+// block-wide max of v (non-negative) into the workgroup's partial
+__device__ __forceinline__ void block_max_to(float v, float* __restrict__ part) {
+  __shared__ float sm[kMgBlock / kWave];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = sm[0];
+    for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
+    *part = m;
+  }
+}
+
+// Gershgorin bound of Dinv A: per-workgroup max over nodes of the largest
+// absolute row sum of the node's block row of Dinv A (Dirichlet columns of the
+// fine level masked: M A M; its rows are zero in Dinv already).  8 lanes a node,
+// each summing the |(Dinv B)_ij| of its 3-4 slots' blocks, the group then adding
+// its lanes' sums: the loads of a node's 27 blocks are in flight together.
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S, const float* __restrict__ Dinv, Lat g,
+                                                     const float* __restrict__ mask, float* __restrict__ part) {
+  const int64_t n = g.n;
+  float best = 0.f;
+  const int q = threadIdx.x % kGroup;
+  constexpr int per = kMgBlock / kGroup;
+  for (int64_t node = (int64_t)logical_block() * per + threadIdx.x / kGroup; node < n;
+       node += (int64_t)gridDim.x * per) {
+    float M[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) M[e] = Dinv[(int64_t)e * n + node];
+    int x, y, z;
+    lat_xyz32(g, (uint32_t)node, x, y, z);
+    float rs[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int o = q + 8 * t;
+      if (o >= 27) break;
+      const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
+      const bool valid = xr >= 0 && xr < g.nx && yr >= 0 && yr < g.ny && zr >= 0 && zr < g.nz;
+      const int64_t m = lat_id(g, clampi(xr, 0, g.nx - 1), clampi(yr, 0, g.ny - 1), clampi(zr, 0, g.nz - 1));
+      float B[9];
+      load_block<HALF>(S, n, node, o, m, valid, B);
+      float cm[3] = {1.f, 1.f, 1.f};
+      if (mask) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) cm[j] = mask[3 * m + j];
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          rs[i] += fabsf(M[3 * i] * B[j] + M[3 * i + 1] * B[3 + j] + M[3 * i + 2] * B[6 + j]) * cm[j];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rs[i] = group8_sum(rs[i]);
+    best = fmaxf(best, fmaxf(rs[0], fmaxf(rs[1], rs[2])));
+  }
+  block_max_to(best, part + blockIdx.x);
+}
+
+// FP16 copies of several half-stored levels in two launches: per-workgroup
+// maxima of |S| (mg_f16_max_all), then each workgroup of mg_f16_convert_all
+// reduces its level's maxima itself, forms the scale (mg_f16_scale's rule) and
+// converts its nodes' slots: W[(s*5 + k)*n + node] = half2(sc S[(s*9 + 2k)*n +
+// node], sc S[(s*9 + 2k + 1)*n + node]), the fifth word's upper half zero.
+struct F16Levels {
+  int nlev;
+  const float* S[kMgMaxLevels];   // FP32 half-stored stencils
+  __half2* W[kMgMaxLevels];       // their FP16 words
+  float* scl[kMgMaxLevels];       // scale pairs (scl[2]: the maximum)
+  int64_t n[kMgMaxLevels];        // nodes
+  int p0[kMgMaxLevels + 1];       // first partial (workgroup of mg_f16_max_all) of each level
+  int c0[kMgMaxLevels + 1];       // first workgroup of mg_f16_convert_all of each level
+};
+constexpr int kF16MaxParts = 256;   // mg_f16_max_all workgroups per level at most
+
+__device__ __forceinline__ int f16_level_of(const int* first, int nlev, int b) {
+  int l = 0;
+  while (l + 1 < nlev && b >= first[l + 1]) ++l;
+  return l;
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_f16_max_all(F16Levels J, float* __restrict__ part) {
+  const int b = blockIdx.x;
+  const int l = f16_level_of(J.p0, J.nlev, b);
+  const int nb = J.p0[l + 1] - J.p0[l], bl = b - J.p0[l];
+  const int64_t count = (int64_t)kHalfSlots * 9 * J.n[l];
+  const float* S = J.S[l];
+  const int64_t n4 = count / 4;
+  const float4* S4 = reinterpret_cast<const float4*>(S);
+  float best = 0.f;
+  for (int64_t i = (int64_t)bl * kMgBlock + threadIdx.x; i < n4; i += (int64_t)nb * kMgBlock) {
+    const float4 v = S4[i];
+    best = fmaxf(best, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+  }
+  if (bl == 0)
+    for (int64_t i = 4 * n4 + threadIdx.x; i < count; i += kMgBlock) best = fmaxf(best, fabsf(S[i]));
+  block_max_to(best, part + b);
+}
+
+__global__ void __launch_bounds__(kMgBlock) mg_f16_convert_all(F16Levels J, const float* __restrict__ part) {
+  __shared__ float sm[kMgBlock / kWave];
+  const int b = blockIdx.x;
+  const int l = f16_level_of(J.c0, J.nlev, b);
+  // the level's maximum from its partials (at most kF16MaxParts)
+  float m = 0.f;
+  for (int i = J.p0[l] + threadIdx.x; i < J.p0[l + 1]; i += kMgBlock) m = fmaxf(m, part[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = m;
+  __syncthreads();
+  m = sm[0];
+  for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
+  const float sc = (m > 0.f && isfinite(m)) ? ldexpf(1.f, 14 - ilogbf(m)) : 1.f;
+  if (b == J.c0[l] && threadIdx.x == 0) {
+    J.scl[l][0] = sc;
+    J.scl[l][1] = 1.f / sc;
+    J.scl[l][2] = m;
+  }
+  const int64_t n = J.n[l];
+  const int64_t t = (int64_t)(b - J.c0[l]) * kMgBlock + threadIdx.x;   // (slot, node), node fastest
+  if (t >= n * kHalfSlots) return;
+  const int sl = (int)(t / n);
+  const int64_t node = t - (int64_t)sl * n;
+  const float* S = J.S[l];
+  float v[2 * kF16Words];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) v[e] = sc * S[(int64_t)(sl * 9 + e) * n + node];
+  v[9] = 0.f;
+#pragma unroll
+  for (int k = 0; k < kF16Words; ++k)
+    J.W[l][(int64_t)(sl * kF16Words + k) * n + node] = __floats2half2_rn(v[2 * k], v[2 * k + 1]);
 }
 
 // row q of the node's 3x3 block of D times v
@@ -2566,7 +2638,7 @@ struct MgLevel {
   float* da = nullptr;    // 3 n smoother directions (two, alternating)
   float* db = nullptr;    // 3 n
   float* res = nullptr;   // 3 n smoother residual / restriction input
-  __half2* S16 = nullptr; // half-stored grid level with HF_MG_FINE_F16: 70 n words (mg_to_f16)
+  __half2* S16 = nullptr; // half-stored level with HF_MG_FINE_F16: 70 n words (mg_f16_convert_all)
   bool s16_ok = false;    // S16 made by the last set-up (level 0 skips it while a cell operator is set)
 };
 
@@ -2936,13 +3008,13 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   float* lam_dev = lam_dev0;
   for (int l = mg->sell ? 1 : 0; l + 1 < mg->nlev; ++l) {
     const Lat f = mg->lv[l].g, c = mg->lv[l + 1].g;
-    const int gg = (int)std::min<int64_t>(ceil_div(f.n, kMgBlock), kMaxGrid);
+    const int gl = lane_grid(f.n);   // mg_gersh's workgroups (partials)
     auto level = [&](auto half_tag) {
       constexpr bool H = decltype(half_tag)::value;
       const float* Sp = l == 0 ? S0p : mg->lv[l].S;
       mg_block_dinv<H><<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(Sp, l == 0 ? mask : nullptr, f.n,
                                                                               mg->lv[l].Dinv);
-      mg_gersh<H><<<gg, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, l == 0 ? mask : nullptr, mg->part);
+      mg_gersh<H><<<gl, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, l == 0 ? mask : nullptr, mg->part);
       // P^T A P: x, then y, then z; the two intermediates share the B buffer
       const Lat bx = make_lat(c.nx, f.ny, f.nz), bxy = make_lat(c.nx, c.ny, f.nz);
       float* T1 = mg->B;
@@ -2958,7 +3030,7 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
     };
     if (mg->lv[l].half) level(std::true_type{});
     else level(std::false_type{});
-    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gg, lam_dev + l);
+    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, gl, lam_dev + l);
   }
   const Lat gl = mg->lv[mg->nlev - 1].g;
   mg_dense_setup<<<1, kDenseThreads, 0, s>>>(mg->lv[mg->nlev - 1].S, gl, mg->Ainv, mg->flags);
@@ -2985,20 +3057,33 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
     mg->svals16_ok = true;
   }
   // FP16 copies of the half-stored levels (level 0 only without a cell operator,
-  // whose matrix-free rows replace its stencil in the cycle)
-  for (int l = 0; l < mg->nlev; ++l) {
-    MgLevel& V = mg->lv[l];
-    V.s16_ok = false;
-    if (!V.S16 || (l == 0 && mg->mf_scale)) continue;
-    const float* Sl = l == 0 ? S0p : V.S;
-    float* sc16 = mg->scl + 4 * l;
-    const int64_t cnt = (int64_t)kHalfSlots * 9 * V.g.n;
-    const int ga = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(cnt / 4, kMgBlock), 1), kMaxGrid);
-    mg_absmax<<<ga, kMgBlock, 0, s>>>(Sl, cnt, mg->part);
-    mg_max_final<<<1, kMgBlock, 0, s>>>(mg->part, ga, sc16 + 2);
-    mg_f16_scale<<<1, kWave, 0, s>>>(sc16);
-    mg_to_f16<<<dim3((unsigned)ceil_div(V.g.n, kMgBlock), kHalfSlots), kMgBlock, 0, s>>>(Sl, V.g.n, sc16, V.S16);
-    V.s16_ok = true;
+  // whose matrix-free rows replace its stencil in the cycle), all in two launches
+  {
+    F16Levels J;
+    memset(&J, 0, sizeof(J));
+    int np = 0, nc = 0;
+    for (int l = 0; l < mg->nlev; ++l) {
+      MgLevel& V = mg->lv[l];
+      V.s16_ok = false;
+      if (!V.S16 || (l == 0 && mg->mf_scale)) continue;
+      const int j = J.nlev++;
+      J.S[j] = l == 0 ? S0p : V.S;
+      J.W[j] = V.S16;
+      J.scl[j] = mg->scl + 4 * l;
+      J.n[j] = V.g.n;
+      J.p0[j] = np;
+      J.c0[j] = nc;
+      const int64_t cnt4 = (int64_t)kHalfSlots * 9 * V.g.n / 4;
+      np += (int)std::min<int64_t>(std::max<int64_t>(ceil_div(cnt4, kMgBlock), 1), kF16MaxParts);
+      nc += (int)ceil_div((int64_t)kHalfSlots * V.g.n, kMgBlock);
+      V.s16_ok = true;
+    }
+    J.p0[J.nlev] = np;
+    J.c0[J.nlev] = nc;
+    if (J.nlev) {
+      mg_f16_max_all<<<np, kMgBlock, 0, s>>>(J, mg->part);
+      mg_f16_convert_all<<<nc, kMgBlock, 0, s>>>(J, mg->part);
+    }
   }
   if (mg->lmode) {
     const MgLevel& F = mg->lv[0];
