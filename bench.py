@@ -805,17 +805,19 @@ def main():
     if use_mg and not args.no_pcg:
         _, mixed_out = leg(mixed, "mixed jacobi-pcg", "single run after the timed steps")
     learned_out, sol_l, fcg_out = None, None, None
-    if not args.no_learned and use_mg and prob.fe.ele_type == "HEX8":
+    if not args.no_learned and use_mg:
         # the learned network as the multigrid's fine-level post-smoother (weights trained
-        # on post-coarse-correction residuals of 32^3 heterogeneous elasticity)
+        # on post-coarse-correction residuals of heterogeneous elasticity: 32^3 HEX8 cells,
+        # or 16^3 HEX27 cells for the quadratic lattice)
         from jax_fem_amd.learned_pc import LearnedPC
-        smoother = LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", "lpc_mg_smoother.npz"))
+        wname = "lpc_mg_smoother_hex27.npz" if prob.fe.ele_type == "HEX27" else "lpc_mg_smoother.npz"
+        smoother = LearnedPC.load(os.path.join(ROOT, "jax-fem_amd", "weights", wname))
         learned = {"newton": {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4,
                               "linear": {"mg": {"smoother": "learned", "model": smoother}}}}
         solve_case(jfa, prob, bc, learned, nload, dmax)        # first use: set-up and capture
         sol_l, learned_out = leg(learned, "learned smoother",
                                  "multigrid flexible CG with the learned network as the fine-level post-smoother "
-                                 "(weights/lpc_mg_smoother.npz); second run")
+                                 "(weights/%s); second run" % wname)
         learned_out["method"] = ("FP32 tangent + FP64 Newton refinement; multigrid V-cycles with Chebyshev(2) "
                                  "pre-smoothing and one learned-network step as the fine post-smoother")
     if args.learned_fcg and wl not in ("elastic-hex8-20", "neohookean-hex27-48"):   # HEX8 grids only

@@ -2274,6 +2274,34 @@ __global__ void __launch_bounds__(kMgBlock) mg_sell_resid_update(int64_t n, cons
   });
 }
 
+// 1 / diag(A) of a SELL fine level (the learned smoother's scalar Jacobi, as
+// mg_scalar_dinv); 1 on Dirichlet DOFs
+__global__ void __launch_bounds__(kMgBlock) mg_sell_scalar_dinv(const int32_t* __restrict__ dpos,
+                                                                const float* __restrict__ vals,
+                                                                const float* __restrict__ mask, int64_t n,
+                                                                float* __restrict__ dinv) {
+  const int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x;
+  if (node >= n) return;
+  const int32_t b = dpos[node];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float a = b >= 0 ? vals[(int64_t)b + 4 * i] : 0.f;
+    const bool free = !mask || mask[3 * node + i] != 0.f;
+    dinv[3 * node + i] = (free && a != 0.f) ? 1.f / a : 1.f;
+  }
+}
+
+// res = mask (r - y): the fine residual the learned step reads, y = A e by the SELL product
+__global__ void __launch_bounds__(kMgBlock) mg_sell_masked_resid(int64_t n, const float* __restrict__ r,
+                                                                 const float* __restrict__ y,
+                                                                 const float* __restrict__ mask,
+                                                                 float* __restrict__ res, const Scalars* sc) {
+  if (sc && sc->done) return;
+  MG_NODE_LOOP(n, {
+    for (int i = 0; i < 3; ++i) res[3 * node + i] = mask[3 * node + i] * (r[3 * node + i] - y[3 * node + i]);
+  });
+}
+
 __global__ void __launch_bounds__(kMgBlock) mg_sell_post_update(const float* __restrict__ Dinv, int64_t n,
                                                                 const float* __restrict__ cf,
                                                                 const float* __restrict__ r,
@@ -3087,8 +3115,13 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   }
   if (mg->lmode) {
     const MgLevel& F = mg->lv[0];
-    auto sd = F.half ? mg_scalar_dinv<true> : mg_scalar_dinv<false>;
-    sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(S0p, mask, F.g.n, mg->sdinv);
+    if (mg->sell) {
+      mg_sell_scalar_dinv<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(pattern_dpos(p), vals, mask,
+                                                                                   F.g.n, mg->sdinv);
+    } else {
+      auto sd = F.half ? mg_scalar_dinv<true> : mg_scalar_dinv<false>;
+      sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(S0p, mask, F.g.n, mg->sdinv);
+    }
   }
   mg->s0 = mg->sell ? nullptr : S0p;
   HF_LAUNCH_CHECK();
@@ -3352,7 +3385,15 @@ static int vcycle(hf_mg* mg, const float* r0, float* z0, float* z1, float** out,
     if (l == 0 && skip_fine_post) break;
     // the learned step: res = M (r - A e), then e += D^-1 res + s D^-1/2 mask N(res)
     auto learned_step = [&]() -> int {
-      if (mg->mf_scale) {
+      if (mg->sell) {
+        const MatArgs m = mat_args(mg->sp);
+        const int gr = rows_grid(m.nslices);
+        if (mg->svals16_ok)
+          mg_sell_spmv16<<<gr, kBlock, 0, s>>>(m, mg->svals16, mg->scl, cur[0], mg->y, sc);
+        else
+          mg_sell_spmv<<<gr, kBlock, 0, s>>>(m, mg->svals, cur[0], mg->y, sc);
+        mg_sell_masked_resid<<<gg, kMgBlock, 0, s>>>(V.g.n, rin(0), mg->y, mg->mask, V.res, sc);
+      } else if (mg->mf_scale) {
         mg_poly_resid<true, MfOp><<<gg, kMgBlock, 0, s>>>(mg->mf, mg->mf_scale, V.g, rin(0), cur[0], mg->mask,
                                                           V.res, sc);
       } else if (V.s16_ok) {
@@ -3452,7 +3493,7 @@ extern "C" int hf_mg_set_learned(hf_mg* mg, int32_t mode, const float* const* we
     return 0;
   }
   HF_REQUIRE(weights && gfield, "hf_mg_set_learned: null argument");
-  HF_REQUIRE(!mg->sell && mg->nlev > 1, "hf_mg_set_learned: needs a lattice-stencil fine level and a coarser level");
+  HF_REQUIRE(mg->nlev > 1, "hf_mg_set_learned: needs a coarser level");
   const MgLevel& F = mg->lv[0];
   if (!mg->lsm.buf) {
     HF_CHECK_HIP(hipMalloc(&mg->lsm.buf, lpc_smoother_bytes(F.g.n)));
@@ -3464,8 +3505,13 @@ extern "C" int hf_mg_set_learned(hf_mg* mg, int32_t mode, const float* const* we
   mg->lsm.gfield = gfield;
   if (lpc_smoother_load(mg->lsm, weights, s)) return -2;
   if (mg->ready) {
-    auto sd = F.half ? mg_scalar_dinv<true> : mg_scalar_dinv<false>;
-    sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(mg->s0, mg->mask, F.g.n, mg->sdinv);
+    if (mg->sell) {
+      mg_sell_scalar_dinv<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(pattern_dpos(mg->sp), mg->svals,
+                                                                                   mg->mask, F.g.n, mg->sdinv);
+    } else {
+      auto sd = F.half ? mg_scalar_dinv<true> : mg_scalar_dinv<false>;
+      sd<<<(unsigned)ceil_div(F.g.n, kMgBlock), kMgBlock, 0, s>>>(mg->s0, mg->mask, F.g.n, mg->sdinv);
+    }
     HF_LAUNCH_CHECK();
   }
   mg->lmode = mode;

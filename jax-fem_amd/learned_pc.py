@@ -94,10 +94,13 @@ class LearnedPC:
     # -- problem fields ----------------------------------------------------------
     @staticmethod
     def grid(problem):
+        """The structured node lattice (nx, ny, nz) the network runs on: a box lattice of
+        HEX8 cells, or of HEX27 cells (nodes two per cell edge plus one), vec = 3."""
         g = getattr(problem.fe.mesh, "node_grid", None)
         if g is None or int(np.prod(g)) != problem.fe.num_total_nodes or problem.vec != 3 or \
-                problem.fe.ele_type != "HEX8":
-            raise NotImplementedError("the learned preconditioner needs a structured box_mesh HEX8 grid, vec=3")
+                problem.fe.ele_type not in ("HEX8", "HEX27"):
+            raise NotImplementedError("the learned preconditioner needs a structured box_mesh HEX8 or HEX27 node "
+                                      "lattice, vec=3")
         return tuple(int(x) for x in g)
 
     @staticmethod
@@ -306,8 +309,9 @@ def heterogeneous_modulus(n_cells, seed, sigma=0.5, ell=0.1, lo=0.05, hi=20.0, d
     return np.clip(np.exp(g), lo, hi).reshape(-1)
 
 
-def elasticity_instance(jfa, n, E_cell, Lz=1.0):
-    """Linear elasticity on a box with E per cell, u = 0 on x = 0 (the c0/c2 boundary conditions)."""
+def elasticity_instance(jfa, n, E_cell, Lz=1.0, ele_type="HEX8"):
+    """Linear elasticity on a box with E per cell, u = 0 on x = 0 (the c0/c2 boundary
+    conditions); ele_type HEX27 gives the quadratic lattice (3x3x3 Gauss)."""
     nx, ny, nz = (n, n, n) if np.ndim(n) == 0 else n
 
     class Elastic(jfa.Problem):
@@ -315,10 +319,19 @@ def elasticity_instance(jfa, n, E_cell, Lz=1.0):
             return jfa.LinearElastic(E=1.0, nu=0.3, param="scale_E")
 
     left = lambda p: np.isclose(p[0], 0.0, atol=1e-7)
-    prob = Elastic(jfa.box_mesh(nx, ny, nz, 1.0, ny / nx, Lz * nz / nx), vec=3, dim=3,
-                   dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3])
-    prob.internal_vars = [np.repeat(np.asarray(E_cell, dtype=np.float64)[:, None], 8, axis=1)]
+    bc = [[left] * 3, [0, 1, 2], [lambda p: 0.0] * 3]
+    if ele_type == "HEX27":
+        prob = Elastic(jfa.box_mesh_hex27(nx, ny, nz, 1.0, ny / nx, Lz * nz / nx), vec=3, dim=3, ele_type="HEX27",
+                       quadrature_order=4, dirichlet_bc_info=bc)
+    else:
+        prob = Elastic(jfa.box_mesh(nx, ny, nz, 1.0, ny / nx, Lz * nz / nx), vec=3, dim=3, dirichlet_bc_info=bc)
+    set_modulus(prob, E_cell)
     return prob
+
+
+def set_modulus(prob, E_cell):
+    """The instance's per-cell modulus, held at every quadrature point."""
+    prob.internal_vars = [np.repeat(np.asarray(E_cell, dtype=np.float64)[:, None], prob.fe.num_quads, axis=1)]
 
 
 RHS_KINDS = ("noise", "face", "smooth")

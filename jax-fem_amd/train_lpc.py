@@ -13,6 +13,8 @@ the unit cube (correlation length 12.8 cells instead of 3.2).  Each step takes o
 tangent, draws `batch` Jacobi-PCG residuals (iterations 0..k_max) of a random
 right-hand side and takes an Adam step on the energy loss (learned_pc.train).
 Held-out seeds (9001, ...) measure iterations of learned FCG vs Jacobi-PCG.
+--element HEX27 trains on the quadratic node lattice (n^3 cells, (2n+1)^3 nodes,
+the SELL tangent as the multigrid's fine level) with --sampler mg.
 """
 import argparse
 import importlib.util
@@ -22,6 +24,10 @@ import sys
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# run as a script, the package directory is sys.path[0]: its numpy.py (the
+# jax.numpy stand-in) would shadow NumPy itself
+if sys.path and os.path.abspath(sys.path[0] or ".") == HERE:
+    sys.path.pop(0)
 
 
 def load_pkg():
@@ -51,6 +57,8 @@ def main():
     ap.add_argument("--sampler", default="krylov", choices=("krylov", "mg"),
                     help="krylov: Jacobi-PCG / own-FCG residuals (the preconditioner); mg: residuals after a "
                          "multigrid coarse correction (a fine-level post-smoother)")
+    ap.add_argument("--element", default="HEX8", choices=("HEX8", "HEX27"))
+    ap.add_argument("--cache", type=int, default=256, help="tangent matrices kept on the device")
     ap.add_argument("--out", default=os.path.join(HERE, "weights", "lpc_elastic.npz"))
     args = ap.parse_args()
 
@@ -69,20 +77,20 @@ def main():
         return L.heterogeneous_modulus((n, n, n), seed, domain=(d, d, d))
 
     E0 = field(1)
-    prob = L.elasticity_instance(jfa, n, E0)
+    prob = L.elasticity_instance(jfa, n, E0, ele_type=args.element)
     u0 = torch.zeros(prob.N, dtype=torch.float64, device="cuda")
     cache = {}
 
     def matrix(i):
         seed = 1 + (i % args.instances)
         if seed not in cache:
-            if len(cache) > 256:
+            if len(cache) >= max(args.cache, 1):
                 cache.pop(next(iter(cache)))
             E = field(seed)
-            prob.internal_vars = [np.repeat(E[:, None], 8, axis=1)]
+            L.set_modulus(prob, E)
             cache[seed] = (TangentMatrix(prob, prob.jacobian_bsr(u0, torch.float32)), E)
         A, E = cache[seed]
-        prob.internal_vars = [np.repeat(E[:, None], 8, axis=1)]
+        L.set_modulus(prob, E)
         return A, prob, seed
 
     model = L.LearnedPC.load(args.init) if args.init else L.LearnedPC(seed=0)
@@ -90,7 +98,7 @@ def main():
     sampler = None
     if args.sampler == "mg":
         from jax_fem_amd.multigrid import Multigrid, dirichlet_mask
-        mg = Multigrid(L.LearnedPC.grid(prob))
+        mg = Multigrid(L.LearnedPC.grid(prob), fine_sell=args.element == "HEX27", fine_f16=args.element == "HEX27")
 
         def sampler(model, A, p, ks, sd, g, kind):
             if not mg.is_current(A):
@@ -109,7 +117,7 @@ def main():
     for k in range(args.eval * len(domains)):
         seed = 9001 + k
         E = field(seed)
-        prob.internal_vars = [np.repeat(E[:, None], 8, axis=1)]
+        L.set_modulus(prob, E)
         A = TangentMatrix(prob, prob.jacobian_bsr(u0, torch.float32))
         b = torch.randn(prob.N, device="cuda", generator=torch.Generator(device="cuda").manual_seed(seed))
         b = b * L.LearnedPC.mask(prob)
