@@ -25,7 +25,9 @@ forcing, unless stated):
 roofline: the dominant kernel of the `value` path (the fine-level Chebyshev step
 of the multigrid; the SELL product on HEX27, the Krylov SpMV without multigrid),
 timed live with HIP events; traffic is its HBM bytes per launch from rocprofv3
-PMC counters committed under profiles/ (null if absent).
+PMC counters committed under profiles/ (null if absent).  Every one of these
+kernels has an arithmetic intensity below MI355X's FP32 ridge point, so its
+roofline is HBM; the matrix-free step also reports its flop rate.
 cpu_baseline: the FP64 oracle on this host, in full for elastic-hex8-20,
 extrapolated from a bounded sample otherwise (`--cpu-only full` times a whole
 workload on the CPU alone and prints the sampled estimate beside it).
@@ -882,14 +884,16 @@ def main():
                       "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
                       "traffic": _pmc_traffic(pmc) if full_size else None, "avg_launch_us": kern_us,
                       "launches_timed": int(nl.value), "algorithmic_bytes": kbytes} if kflop is None else
-                     {"bound": "mfma", "kernel": kname,
-                      "achieved": achieved_tf, "peak": 157.3, "unit": "TFLOP/s", "frac": achieved_tf / 157.3,
+                     {"bound": "hbm", "kernel": kname,
+                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
                       "traffic": _pmc_traffic(pmc) if full_size else None, "avg_launch_us": kern_us,
-                      "launches_timed": int(nl.value), "algorithmic_flop": kflop, "algorithmic_bytes": kbytes,
-                      "hbm_achieved_GBps": achieved, "hbm_frac": achieved / 8000.0,
-                      "note": "FP32 FMA on the vector ALU (no matrix cores): the compute bound against MI355X's "
-                              "FP32 peak; without packed-FP32 instructions (not generated here) the vector ALU "
-                              "issues at most half of it, 78.6 TFLOP/s"}),
+                      "launches_timed": int(nl.value), "algorithmic_bytes": kbytes, "algorithmic_flop": kflop,
+                      "flop_achieved_TFLOPs": achieved_tf, "flop_frac_of_fp32_peak": achieved_tf / 157.3,
+                      "intensity_flop_per_byte": kflop / kbytes,
+                      "note": "arithmetic intensity %.1f flop/B is below the FP32 ridge point (157.3 TFLOP/s / "
+                              "8 TB/s = 19.7 flop/B), so HBM is the binding roofline; the FP32 FMA runs on the "
+                              "vector ALU (no packed-FP32 instructions: at most 78.6 TFLOP/s issued)" %
+                              (kflop / kbytes)}),
         "whole_solve": whole,
         "pcg_mixed": mixed_out,
         "learned_pc": learned_out,

@@ -49,23 +49,47 @@ __host__ __device__ constexpr double g3_weight(int i) { return i == 1 ? 8.0 / 18
 
 // quadratic Lagrange through 0, 1/2, 1 and its derivative
 template <typename T>
-__device__ __forceinline__ T lag2(int a, T t) {
+__host__ __device__ constexpr T lag2(int a, T t) {
   return a == 0 ? (T)2 * (t - (T)0.5) * (t - (T)1) : (a == 1 ? (T)(-4) * t * (t - (T)1) : (T)2 * t * (t - (T)0.5));
 }
 template <typename T>
-__device__ __forceinline__ T dlag2(int a, T t) {
+__host__ __device__ constexpr T dlag2(int a, T t) {
   return a == 0 ? (T)4 * t - (T)3 : (a == 1 ? (T)4 - (T)8 * t : (T)4 * t - (T)1);
 }
+
+// The reference gradients dN_n/dxi at the 27 Gauss points, [q][n][e], evaluated
+// in FP64 at compile time (a kernel reads three entries instead of evaluating
+// three products of Lagrange factors with their branches per node and point).
+struct H27Grad {
+  double v[27][27][3];
+};
+constexpr H27Grad make_h27_grad() {
+  constexpr int8_t pos[27][3] = {{0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2},
+                                 {0, 2, 2}, {1, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {1, 0, 2}, {2, 1, 2},
+                                 {1, 2, 2}, {0, 1, 2}, {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1}, {0, 1, 1},
+                                 {2, 1, 1}, {1, 0, 1}, {1, 2, 1}, {1, 1, 0}, {1, 1, 2}, {1, 1, 1}};
+  H27Grad g{};
+  for (int q = 0; q < 27; ++q) {
+    const double tx = g3_point(q / 9), ty = g3_point((q / 3) % 3), tz = g3_point(q % 3);
+    for (int n = 0; n < 27; ++n) {
+      const int a = pos[n][0], b = pos[n][1], c = pos[n][2];
+      const double vx = lag2<double>(a, tx), vy = lag2<double>(b, ty), vz = lag2<double>(c, tz);
+      g.v[q][n][0] = dlag2<double>(a, tx) * vy * vz;
+      g.v[q][n][1] = vx * dlag2<double>(b, ty) * vz;
+      g.v[q][n][2] = vx * vy * dlag2<double>(c, tz);
+    }
+  }
+  return g;
+}
+__constant__ H27Grad c_h27_grad = make_h27_grad();
 
 // reference gradient dN_n/dxi at quadrature point q
 template <typename T>
 __device__ __forceinline__ void ref_grad27(int n, int q, T dN[3]) {
-  const T tx = (T)g3_point(q / 9), ty = (T)g3_point((q / 3) % 3), tz = (T)g3_point(q % 3);
-  const int a = c_h27_node[n][0], b = c_h27_node[n][1], c = c_h27_node[n][2];
-  const T vx = lag2<T>(a, tx), vy = lag2<T>(b, ty), vz = lag2<T>(c, tz);
-  dN[0] = dlag2<T>(a, tx) * vy * vz;
-  dN[1] = vx * dlag2<T>(b, ty) * vz;
-  dN[2] = vx * vy * dlag2<T>(c, tz);
+  const double* g = c_h27_grad.v[q][n];
+  dN[0] = (T)g[0];
+  dN[1] = (T)g[1];
+  dN[2] = (T)g[2];
 }
 
 __device__ __forceinline__ double g3_qweight(int q) {
@@ -81,6 +105,9 @@ __device__ __forceinline__ T hex27_point(int q, const T (*X)[3], const T (*U)[3]
   for (int d = 0; d < 3; ++d)
 #pragma unroll
     for (int e = 0; e < 3; ++e) Jm[d][e] = Hr[d][e] = (T)0;
+  // partly unrolled: fully unrolled, the 81 gradient loads are hoisted and the
+  // FP64 kernels run at one wave per SIMD
+#pragma unroll 3
   for (int n = 0; n < 27; ++n) {
     T dN[3];
     ref_grad27<T>(n, q, dN);
@@ -115,7 +142,7 @@ __device__ __forceinline__ void hex27_load(const double* __restrict__ pts, const
 
 // ---------------------------------------------------------------------------
 template <typename T, int MAT>
-__global__ void __launch_bounds__(64) hex27_residual_kernel(const double* __restrict__ pts,
+__global__ void __launch_bounds__(64, 4) hex27_residual_kernel(const double* __restrict__ pts,
                                                             const int32_t* __restrict__ cells, int64_t ncell,
                                                             const double* __restrict__ u,
                                                             const double* __restrict__ params, hf_material mat,
@@ -141,6 +168,7 @@ __global__ void __launch_bounds__(64) hex27_residual_kernel(const double* __rest
   for (int o = t; o < 81; o += 64) {
     const int n = o / 3, i = o - 3 * (o / 3);
     T r = 0;
+#pragma unroll 3
     for (int q = 0; q < 27; ++q) {
       T dN[3];
       ref_grad27<T>(n, q, dN);
@@ -402,7 +430,7 @@ __global__ void __launch_bounds__(64 * kH27Waves, 2) hex27_jacobian_mfma(const d
 
 // grad[c,q] = - w sum_ij (grad lam)_ij dP_ij/dp
 template <int MAT>
-__global__ void __launch_bounds__(64) hex27_param_vjp_kernel(const double* __restrict__ pts,
+__global__ void __launch_bounds__(64, 4) hex27_param_vjp_kernel(const double* __restrict__ pts,
                                                              const int32_t* __restrict__ cells, int64_t ncell,
                                                              const double* __restrict__ u,
                                                              const double* __restrict__ params, hf_material mat,
