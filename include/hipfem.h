@@ -77,6 +77,17 @@ int hf_pattern_destroy(hf_pattern* p);
 int hf_cell_residual_f64(int32_t ele, const hf_material* mat, const double* points,
                          const int32_t* cells, int64_t ncell, const double* u,
                          const double* params, double* elem_res, hf_stream_t stream);
+/* HEX8 box lattices of identical cells (every cell a translate of cell 0, one
+ * corner order: a box_mesh): the per-point physical gradients and weights are
+ * formed once from cell 0 into geo (hf_hex8_box_geometry_bytes() bytes, device),
+ * and hf_cell_residual_box_f64 evaluates the element residuals as
+ * hf_cell_residual_f64 does with them (no per-cell geometry).  Same role as
+ * hf_cell_residual_f64 (problem.py:261-356). */
+size_t hf_hex8_box_geometry_bytes(void);
+int hf_hex8_box_geometry(const double* points, const int32_t* cells, int64_t ncell, double* geo,
+                         hf_stream_t stream);
+int hf_cell_residual_box_f64(const hf_material* mat, const double* geo, const int32_t* cells, int64_t ncell,
+                             const double* u, const double* params, double* elem_res, hf_stream_t stream);
 int hf_cell_residual_f32(int32_t ele, const hf_material* mat, const double* points,
                          const int32_t* cells, int64_t ncell, const double* u,
                          const double* params, float* elem_res, hf_stream_t stream);

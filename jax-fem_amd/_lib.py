@@ -52,6 +52,9 @@ _SIGS = {
     "hf_pattern_csr": (c_i32, [vp, vp, vp, vp]),
     "hf_pattern_destroy": (c_i32, [vp]),
     "hf_cell_residual_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
+    "hf_hex8_box_geometry_bytes": (c_sz, []),
+    "hf_hex8_box_geometry": (c_i32, [vp, vp, c_i64, vp, vp]),
+    "hf_cell_residual_box_f64": (c_i32, [ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
     "hf_cell_residual_f32": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
     "hf_cell_jacobian_f64": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),
     "hf_cell_jacobian_f32": (c_i32, [c_i32, ctypes.POINTER(HFMaterial), vp, vp, c_i64, vp, vp, vp, vp]),

@@ -12,6 +12,9 @@ void set_error(const char* fmt, ...) {
 }
 const char* get_error() { return g_err; }
 
+int hex8_box_geometry(const double*, const int32_t*, void*, hipStream_t);
+int hex8_residual_box_f64(const hf_material*, const void*, const int32_t*, int64_t, const double*, const double*,
+                          double*, hipStream_t);
 int hex8_residual_f64(const hf_material*, const double*, const int32_t*, int64_t, const double*, const double*,
                       double*, hipStream_t);
 int hex8_residual_f32(const hf_material*, const double*, const int32_t*, int64_t, const double*, const double*,
@@ -69,6 +72,21 @@ extern "C" int hf_cell_residual_f64(int32_t ele, const hf_material* m, const dou
   if (int e = check_args(ele, m, p, c, u, o)) return e;
   return ele == 8 ? hex8_residual_f64(m, p, c, n, u, prm, o, (hipStream_t)s)
                   : hex27_residual_f64(m, p, c, n, u, prm, o, (hipStream_t)s);
+}
+extern "C" size_t hf_hex8_box_geometry_bytes(void) { return (8 * 8 * 3 + 8) * sizeof(double); }
+extern "C" int hf_hex8_box_geometry(const double* p, const int32_t* c, int64_t n, double* geo, hf_stream_t s) {
+  HF_REQUIRE(p && c && geo, "hf_hex8_box_geometry: null argument");
+  HF_REQUIRE(n > 0, "hf_hex8_box_geometry: no cells");
+  return hex8_box_geometry(p, c, geo, (hipStream_t)s);
+}
+extern "C" int hf_cell_residual_box_f64(const hf_material* m, const double* geo, const int32_t* c, int64_t n,
+                                        const double* u, const double* prm, double* o, hf_stream_t s) {
+  HF_REQUIRE(m && geo && c && u && o, "hf_cell_residual_box_f64: null argument");
+  HF_REQUIRE(((uintptr_t)c & 15) == 0, "hf_cell_residual_box_f64: cells not 16-byte aligned");
+  HF_REQUIRE(m->kind >= 0 && m->kind <= 2, "hf_cell_residual_box_f64: unknown material %d", m->kind);
+  HF_REQUIRE(m->quad1d == 0 || m->quad1d == 2, "hf_cell_residual_box_f64: HEX8 kernels use 2x2x2 Gauss points; "
+             "got %d per direction", m->quad1d);
+  return hex8_residual_box_f64(m, geo, c, n, u, prm, o, (hipStream_t)s);
 }
 extern "C" int hf_cell_residual_f32(int32_t ele, const hf_material* m, const double* p, const int32_t* c,
                                     int64_t n, const double* u, const double* prm, float* o, hf_stream_t s) {

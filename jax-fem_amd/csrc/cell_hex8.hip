@@ -499,6 +499,115 @@ int hex8_param_vjp(const hf_material* mat, const double* pts, const int32_t* cel
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Box lattices of identical cells (a box_mesh: every cell a translate of cell 0,
+// nodes in one corner order): the physical shape gradients and weights at the
+// 8 points are the same for every cell, so they are formed once from cell 0
+// (hex8_box_geometry, the same arithmetic as hex8_geometry) and the residual
+// kernel reads them as uniform values (scalar loads) instead of forming the
+// 3x3 Jacobian, its inverse and the gradients per cell and point.
+struct Hex8BoxGeom {
+  double g[8][8][3];   // [q][n][j]
+  double w[8];
+};
+
+__global__ void hex8_box_geometry_kernel(const double* __restrict__ pts, const int32_t* __restrict__ cells,
+                                         Hex8BoxGeom* __restrict__ geo) {
+  const int q = threadIdx.x;
+  if (q >= 8) return;
+  double X[8][3];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int64_t nid = cells[n];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) X[n][d] = pts[3 * nid + d];
+  }
+  double g[8][3], w;
+  hex8_geometry<double>(q, X, g, w);
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) geo->g[q][n][j] = g[n][j];
+  geo->w[q] = w;
+}
+
+template <typename T, int MAT, int VEC>
+__global__ void __launch_bounds__(256) hex8_residual_box_kernel(const Hex8BoxGeom* __restrict__ geo,
+                                                                const int32_t* __restrict__ cells, int64_t ncell,
+                                                                const double* __restrict__ u,
+                                                                const double* __restrict__ params, hf_material mat,
+                                                                T* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int4* cp = reinterpret_cast<const int4*>(cells + c * 8);
+  const int4 c0 = cp[0], c1 = cp[1];
+  const int nid[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+  T U[8][VEC];
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) U[n][i] = (T)u[VEC * (int64_t)nid[n] + i];
+  T r[8][VEC];
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) r[n][i] = 0;
+#pragma unroll 1
+  for (int q = 0; q < 8; ++q) {
+    T g[8][3];
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) g[n][j] = (T)geo->g[q][n][j];
+    const T w = (T)geo->w[q];
+    T ug[VEC][3];
+    grad_u<T, 8, VEC>(U, g, ug);
+    Moduli md = moduli(mat, params ? params[c * 8 + q] : 1.0, params != nullptr);
+    T P[VEC][3];
+    material_stress<T, MAT, VEC>(ug, (T)md.m1, (T)md.m2, P);
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+#pragma unroll
+      for (int i = 0; i < VEC; ++i)
+        r[n][i] += w * (P[i][0] * g[n][0] + P[i][1] * g[n][1] + P[i][2] * g[n][2]);
+  }
+  T* o = out + c * (8 * VEC);
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) o[n * VEC + i] = r[n][i];
+}
+
+int hex8_box_geometry(const double* p, const int32_t* c, void* geo, hipStream_t s) {
+  hex8_box_geometry_kernel<<<1, 64, 0, s>>>(p, c, static_cast<Hex8BoxGeom*>(geo));
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
+int hex8_residual_box_f64(const hf_material* mat, const void* geo, const int32_t* cells, int64_t ncell,
+                          const double* u, const double* params, double* out, hipStream_t s) {
+  if (ncell == 0) return 0;
+  const Hex8BoxGeom* G = static_cast<const Hex8BoxGeom*>(geo);
+  const dim3 blk(256), grd((unsigned)ceil_div(ncell, 256));
+  switch (mat->kind) {
+    case MAT_POISSON:
+      hex8_residual_box_kernel<double, MAT_POISSON, 1><<<grd, blk, 0, s>>>(G, cells, ncell, u, params, *mat, out);
+      break;
+    case MAT_LINEAR_ELASTIC:
+      hex8_residual_box_kernel<double, MAT_LINEAR_ELASTIC, 3><<<grd, blk, 0, s>>>(G, cells, ncell, u, params, *mat,
+                                                                                  out);
+      break;
+    case MAT_NEO_HOOKEAN:
+      hex8_residual_box_kernel<double, MAT_NEO_HOOKEAN, 3><<<grd, blk, 0, s>>>(G, cells, ncell, u, params, *mat,
+                                                                               out);
+      break;
+    default:
+      HF_REQUIRE(false, "unknown material kind %d", mat->kind);
+  }
+  HF_LAUNCH_CHECK();
+  return 0;
+}
+
 int hex8_residual_f64(const hf_material* m, const double* p, const int32_t* c, int64_t n, const double* u,
                       const double* prm, double* o, hipStream_t s) {
   return launch_hex8_residual<double>(m, p, c, n, u, prm, o, s);

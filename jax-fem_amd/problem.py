@@ -603,8 +603,13 @@ class Problem:
             mat = self._hfmat()
             prm = self._params()
             er = self._buf("elem_res", self.num_cells * self.nn * self.vec, torch.float64)
-            _lib.call("hf_cell_residual_f64", self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
-                      self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(er), s)
+            geo = self._box_geometry()
+            if geo is not None:   # identical box cells: the points' gradients formed once
+                _lib.call("hf_cell_residual_box_f64", ctypes.byref(mat), _lib.ptr(geo), _lib.ptr(cells),
+                          self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(er), s)
+            else:
+                _lib.call("hf_cell_residual_f64", self.nn, ctypes.byref(mat), _lib.ptr(pts), _lib.ptr(cells),
+                          self.num_cells, _lib.ptr(u), _lib.ptr(prm), _lib.ptr(er), s)
             ej = None
             if jac:
                 ej = self._buf("elem_jac" + tag, self.num_cells * nd2, dtype)
@@ -800,6 +805,34 @@ class Problem:
                   corner, _lib.ptr(ej), _lib.ptr(S), s)
         self._stencil_version = getattr(self, "_stencil_version", 0) + 1
         return S, self._stencil_version
+
+    box_residual = True   # HEX8 box lattices of identical cells: hf_cell_residual_box_f64
+
+    def _box_geometry(self):
+        """Device buffer of the per-point gradients and weights shared by every cell
+        (hf_hex8_box_geometry) when the mesh is a HEX8 box lattice of identical cells:
+        a lattice cell order with one corner order (_box_corners) and uniform spacing
+        along each axis; None otherwise (checked once per mesh)."""
+        if not (self.box_residual and self.nn == 8 and self.dim == 3):
+            return None
+        if not hasattr(self, "_box_geo"):
+            self._box_geo = None
+            g = getattr(self.fe.mesh, "node_grid", None)
+            if g is not None and int(np.prod(g)) == self.fe.num_total_nodes and \
+                    self._box_corners(tuple(int(v) for v in g)) is not None:
+                nx, ny, nz = (int(v) for v in g)
+                pts = np.asarray(self.fe.points, dtype=np.float64).reshape(nx, ny, nz, 3)
+                axes = [pts[:, 0, 0, 0], pts[0, :, 0, 1], pts[0, 0, :, 2]]
+                uniform = all(np.allclose(np.diff(a), np.diff(a)[0], rtol=1e-12, atol=0.0) for a in axes)
+                lat = np.stack(np.meshgrid(*axes, indexing="ij"), axis=3)
+                if uniform and np.abs(lat - pts).max() <= 1e-12 * max(1.0, np.abs(pts).max()):
+                    pts_d, cells_d = self._dev()
+                    nb = int(_lib.load().hf_hex8_box_geometry_bytes())
+                    geo = torch.empty(nb // 8, dtype=torch.float64, device=self.device)
+                    _lib.call("hf_hex8_box_geometry", _lib.ptr(pts_d), _lib.ptr(cells_d), self.num_cells,
+                              _lib.ptr(geo), _lib.stream())
+                    self._box_geo = geo
+        return self._box_geo
 
     def _box_corners(self, grid):
         """(ctypes int32[8]) local index of corner (dx, dy, dz) at 4dx + 2dy + dz when every

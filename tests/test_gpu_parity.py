@@ -384,3 +384,43 @@ def test_adjoint_vs_oracle_and_fd(jfa):
             vals.append(uu[torch.tensor(nodes, device=DEV), 2].sum().item())
         fd = (vals[0] - vals[1]) / 2e-4
         assert abs(fd - g[c]) <= 1e-5 * abs(g[c]) + 1e-10, (c, fd, g[c])
+
+
+@pytest.mark.parametrize("kind,param", MATS + [("poisson", None)])
+def test_box_residual_vs_oracle(jfa, kind, param):
+    """hf_cell_residual_box_f64 (the points' gradients formed once for a box lattice
+    of identical cells) against the oracle's element residuals and against the
+    per-cell geometry kernel on the same box_mesh (FP64, 1e-12 relative)."""
+    say("box residual", kind, param)
+    n = (4, 3, 5)
+    mesh = jfa.box_mesh(*n, 1.0, 0.75, 1.25)
+    rng = np.random.default_rng(7)
+    if kind == "poisson":
+        mat, vec = jfa.Poisson(), 1
+    else:
+        mat, omat = _mats(jfa, kind, param)
+        vec = 3
+    ncell = int(np.prod(n))
+    params = rng.uniform(0.6, 1.4, (ncell, 8)) if param else None
+    if param:
+        params = np.repeat(params[:, :1], 8, axis=1)     # one value per cell
+    prob = make_problem(jfa, mesh, mat, vec=vec, params=params)
+    assert prob._box_geometry() is not None
+    u = torch.as_tensor(0.05 * rng.standard_normal(prob.N), device=DEV)
+    r_box = prob.residual_vec(u).cpu().numpy()
+    prob.box_residual = False
+    del prob._box_geo
+    assert prob._box_geometry() is None
+    r_gen = prob.residual_vec(u).cpu().numpy()
+    scale = np.abs(r_gen).max()
+    err = np.abs(r_box - r_gen).max() / scale
+    say("  box vs per-cell geometry: %.2e" % err)
+    assert err <= 1e-12
+    if kind != "poisson":
+        pts = np.asarray(mesh.points)
+        op = osv.OracleProblem(pts, np.asarray(mesh.cells), 3, omat,
+                               params=(np.repeat(params[:, :1], 8, axis=1) if param else None))
+        ro = op.compute_residual(u.cpu().numpy())
+        e2 = np.abs(r_box - ro).max() / np.abs(ro).max()
+        say("  box vs oracle: %.2e" % e2)
+        assert e2 <= 1e-11
