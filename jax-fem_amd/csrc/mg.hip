@@ -838,14 +838,16 @@ __global__ void __launch_bounds__(kDenseThreads) mg_dense_setup(const float* __r
   __syncthreads();
   // symmetrise: each owner averages its entries above the diagonal with their mirrors
   double sym[(kDenseMax + kCols - 1) / kCols];
-  if (owner) {
-    int m = 0;
-    for (int j = tj; j < nd; j += kCols, ++m) sym[m] = 0.5 * (A[ti * nd + j] + A[j * nd + ti]);
+#pragma unroll
+  for (int m = 0; m < (kDenseMax + kCols - 1) / kCols; ++m) {
+    const int j = tj + m * kCols;
+    sym[m] = (owner && j < nd) ? 0.5 * (A[ti * nd + j] + A[j * nd + ti]) : 0.0;
   }
   __syncthreads();
-  if (owner) {
-    int m = 0;
-    for (int j = tj; j < nd; j += kCols, ++m) A[ti * nd + j] = sym[m];
+#pragma unroll
+  for (int m = 0; m < (kDenseMax + kCols - 1) / kCols; ++m) {
+    const int j = tj + m * kCols;
+    if (owner && j < nd) A[ti * nd + j] = sym[m];
   }
   __syncthreads();
   // Gauss-Jordan with each owner's row segment in registers.  Step k publishes
