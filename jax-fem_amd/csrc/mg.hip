@@ -674,7 +674,8 @@ __global__ void __launch_bounds__(kMgBlock) mg_block_dinv(const float* __restric
 // dimensions) forms the node's three output slots along dim from the fine blocks
 // (row cpos(K) + a, offset d along dim), at most 3 x 3 of them; the 27 slots of
 // a node are 9 threads in 9 grid planes (blockIdx.y = q), each reading its own
-// blocks.  The intermediate lattices hold all 27 slots.
+// blocks.  The intermediate lattices are stored as the input level is (half-
+// stored for a symmetric operator).
 template <bool HALF>
 __global__ void __launch_bounds__(kMgBlock) mg_rap_dim(const float* __restrict__ S, Lat f, Lat c, int dim,
                                                        const float* __restrict__ mask, float* __restrict__ Out,
@@ -1149,6 +1150,55 @@ __global__ void __launch_bounds__(kMgBlock) mg_prolong(const float* __restrict__
                                                        const Scalars* sc) {
   if (sc && sc->done) return;
   MG_NODE_LOOP(f.n, prolong_node(ec, f, c, mask, e, node));
+}
+
+// Gershgorin bound of Dinv A: per-workgroup max over nodes of the largest
+// absolute row sum of the node's block row of Dinv A, a thread per node (the
+// large levels: each node's block inverse is read once; mg_gersh_lanes below
+// takes the small ones).
+template <bool HALF>
+__global__ void __launch_bounds__(kMgBlock) mg_gersh_nodes(const float* __restrict__ S, const float* __restrict__ Dinv, Lat g,
+                                                     const float* __restrict__ mask, float* __restrict__ part) {
+  __shared__ float sm[kMgBlock / kWave];
+  const int64_t n = g.n;
+  float best = 0.f;
+  for (int64_t node = (int64_t)blockIdx.x * kMgBlock + threadIdx.x; node < n; node += (int64_t)gridDim.x * kMgBlock) {
+    float M[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) M[e] = Dinv[(int64_t)e * n + node];
+    int x, y, z;
+    lat_xyz(g, node, x, y, z);
+    float rs[3] = {0.f, 0.f, 0.f};
+    for (int t = 0; t < 27; ++t) {
+      const int o = pair_slot(t);
+      const int xr = x + o / 9 - 1, yr = y + (o / 3) % 3 - 1, zr = z + o % 3 - 1;
+      const bool valid = xr >= 0 && xr < g.nx && yr >= 0 && yr < g.ny && zr >= 0 && zr < g.nz;
+      const int64_t m = lat_id(g, clampi(xr, 0, g.nx - 1), clampi(yr, 0, g.ny - 1), clampi(zr, 0, g.nz - 1));
+      float B[9];
+      load_block<HALF>(S, n, node, o, m, valid, B);
+      // columns of Dirichlet DOFs (M A M; the rows are zero in Dinv already)
+      float cm[3] = {1.f, 1.f, 1.f};
+      if (mask) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) cm[j] = mask[3 * m + j];
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          rs[i] += fabsf(M[3 * i] * B[j] + M[3 * i + 1] * B[3 + j] + M[3 * i + 2] * B[6 + j]) * cm[j];
+    }
+    best = fmaxf(best, fmaxf(rs[0], fmaxf(rs[1], rs[2])));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o, 64));
+  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = sm[0];
+    for (int k = 1; k < kMgBlock / kWave; ++k) m = fmaxf(m, sm[k]);
+    part[blockIdx.x] = m;
+  }
 }
 
 __global__ void __launch_bounds__(kMgBlock) mg_max_final(const float* __restrict__ part, int np,
@@ -1792,14 +1842,15 @@ __device__ __forceinline__ void block_max_to(float v, float* __restrict__ part) 
   }
 }
 
-// Gershgorin bound of Dinv A: per-workgroup max over nodes of the largest
-// absolute row sum of the node's block row of Dinv A (Dirichlet columns of the
-// fine level masked: M A M; its rows are zero in Dinv already).  8 lanes a node,
+// Gershgorin bound of Dinv A on a level of at most kLaneNodes nodes (as
+// mg_gersh_nodes; Dirichlet columns of the fine level masked: M A M, its rows
+// are zero in Dinv already).  8 lanes a node,
 // each summing the |(Dinv B)_ij| of its 3-4 slots' blocks, the group then adding
 // its lanes' sums: the loads of a node's 27 blocks are in flight together.
 template <bool HALF>
-__global__ void __launch_bounds__(kMgBlock) mg_gersh(const float* __restrict__ S, const float* __restrict__ Dinv, Lat g,
-                                                     const float* __restrict__ mask, float* __restrict__ part) {
+__global__ void __launch_bounds__(kMgBlock) mg_gersh_lanes(const float* __restrict__ S, const float* __restrict__ Dinv,
+                                                           Lat g, const float* __restrict__ mask,
+                                                           float* __restrict__ part) {
   const int64_t n = g.n;
   float best = 0.f;
   const int q = threadIdx.x % kGroup;
@@ -3038,25 +3089,29 @@ static int mg_setup(hf_mg* mg, const hf_pattern* p, const float* vals, const flo
   float* lam_dev = lam_dev0;
   for (int l = mg->sell ? 1 : 0; l + 1 < mg->nlev; ++l) {
     const Lat f = mg->lv[l].g, c = mg->lv[l + 1].g;
-    const int gl = lane_grid(f.n);   // mg_gersh's workgroups (partials)
+    const bool glanes = f.n <= kLaneNodes;
+    // the Gershgorin kernel's workgroups (partials)
+    const int gl = glanes ? lane_grid(f.n) : (int)std::min<int64_t>(ceil_div(f.n, kMgBlock), kMaxGrid);
     auto level = [&](auto half_tag) {
       constexpr bool H = decltype(half_tag)::value;
       const float* Sp = l == 0 ? S0p : mg->lv[l].S;
       mg_block_dinv<H><<<(unsigned)ceil_div(f.n, kMgBlock), kMgBlock, 0, s>>>(Sp, l == 0 ? mask : nullptr, f.n,
                                                                               mg->lv[l].Dinv);
-      mg_gersh<H><<<gl, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, l == 0 ? mask : nullptr, mg->part);
+      auto gersh = glanes ? mg_gersh_lanes<H> : mg_gersh_nodes<H>;
+      gersh<<<gl, kMgBlock, 0, s>>>(Sp, mg->lv[l].Dinv, f, l == 0 ? mask : nullptr, mg->part);
       // P^T A P: x, then y, then z; the two intermediates share the B buffer
       const Lat bx = make_lat(c.nx, f.ny, f.nz), bxy = make_lat(c.nx, c.ny, f.nz);
       float* T1 = mg->B;
       float* T2 = mg->B + (int64_t)kSt * bx.n;
+      // the intermediates P_x^T A P_x and P_y^T (.) P_y are symmetric when A is: with a
+      // half-stored level they are half-stored too (half the bytes written and re-read)
       mg_rap_dim<H><<<dim3((unsigned)ceil_div(bx.n, kMgBlock), 9), kMgBlock, 0, s>>>(Sp, f, bx, 0,
-                                                                                   l == 0 ? mask : nullptr, T1,
-                                                                                   false);
-      mg_rap_dim<false><<<dim3((unsigned)ceil_div(bxy.n, kMgBlock), 9), kMgBlock, 0, s>>>(T1, bx, bxy, 1, nullptr,
-                                                                                         T2, false);
-      mg_rap_dim<false><<<dim3((unsigned)ceil_div(c.n, kMgBlock), 9), kMgBlock, 0, s>>>(T2, bxy, c, 2, nullptr,
-                                                                                      mg->lv[l + 1].S,
-                                                                                      mg->lv[l + 1].half);
+                                                                                   l == 0 ? mask : nullptr, T1, H);
+      mg_rap_dim<H><<<dim3((unsigned)ceil_div(bxy.n, kMgBlock), 9), kMgBlock, 0, s>>>(T1, bx, bxy, 1, nullptr,
+                                                                                     T2, H);
+      mg_rap_dim<H><<<dim3((unsigned)ceil_div(c.n, kMgBlock), 9), kMgBlock, 0, s>>>(T2, bxy, c, 2, nullptr,
+                                                                                  mg->lv[l + 1].S,
+                                                                                  mg->lv[l + 1].half);
     };
     if (mg->lv[l].half) level(std::true_type{});
     else level(std::false_type{});
