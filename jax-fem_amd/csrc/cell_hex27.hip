@@ -19,16 +19,18 @@
 //   workgroup, no workgroup barrier): for each (i,k) the 27 x 27 block
 //     K^{ik}[a][b] = sum_r A[a][r] V^{ik}[r][b],  r = (q, J),
 //     A[a][r] = G[q][a][J],  V^{ik}[r][b] = sum_L C_q[iJkL] G[q][b][L],
-//   is one GEMM on v_mfma_f32_32x32x2f32 (27 padded to 32, K = 84 in 42 steps).
+//   is one GEMM on v_mfma_f32_32x32x2f32 (27 padded to 32, K = 84 in 42 steps);
+//   the tangent is symmetric, so the six blocks i <= k are formed and the other
+//   three written transposed.
 //   Lane l stands for node n = l % 32 as both the row a of A and the column b
 //   of V, and half-wave h = l / 32 owns the quadrature points q = t + 14h
 //   (t = 0..13): step s feeds r = (t = s / 3, J = s % 3), so every operand is a
 //   compile-time register of the lane's own gradients Gl[t][.] (42 floats),
 //   and only the 3 tangent entries C_q[iJk.] come from LDS, a broadcast per
 //   half-wave.  No index division or bound test in the K loop.
-// 2 * 3 * 81 * 81 * 27 = 1.06 MFLOP per cell of algorithmic GEMM work (1.39
-// MFLOP executed on MFMA with the padding), 0.12 MFLOP forming V.  Output
-// block-major (cell, a, b, i, k).
+// 2 * 3 * 81 * 81 * 27 = 1.06 MFLOP per cell of algorithmic GEMM work (the
+// six blocks executed: 0.93 MFLOP on MFMA with the padding), 0.12 MFLOP forming
+// V.  Output block-major (cell, a, b, i, k).
 #include "common.hpp"
 #include "element_math.hpp"
 
@@ -394,26 +396,45 @@ __global__ void __launch_bounds__(64 * kH27Waves, 2) hex27_jacobian_mfma(const d
     }
   }
 
-  // all nine (i, k) blocks stay in accumulators, so that each 3x3 block (a, b) of
-  // the output is written at once: 9 consecutive words per lane, 27 lanes -> 972
-  // contiguous bytes per row a (full cache lines while they are hot in L2)
+  // The tangent is symmetric (C_iJkL = C_kLiJ for a hyperelastic law), so
+  // K^{ik} = (K^{ki})^T: six GEMMs (i <= k) stay in accumulators and the three
+  // blocks below the diagonal are read transposed through LDS, so that each 3x3
+  // block (a, b) of the output is written at once: 9 consecutive words per lane,
+  // 27 lanes -> 972 contiguous bytes per row a (full cache lines while hot in L2)
   const float* Cbase = &S.C[kH27T * h][0];
-  f32x16 acc[9];
+  constexpr int kI[6] = {0, 0, 0, 1, 1, 2}, kK[6] = {0, 1, 2, 1, 2, 2};
+  f32x16 acc[6];
 #pragma unroll
-  for (int ik = 0; ik < 9; ++ik) {
-    const int i = ik / 3, k = ik - 3 * (ik / 3);
+  for (int gk = 0; gk < 6; ++gk) {
+    const int i = kI[gk], k = kK[gk];
     const float* Cik = Cbase + i * 27 + k * 3;   // + t * 81 + J * 9 + L
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[ik][r] = 0.f;
+    for (int r = 0; r < 16; ++r) acc[gk][r] = 0.f;
 #pragma unroll
     for (int s2 = 0; s2 < 3 * kH27T; ++s2) {
       const int t = s2 / 3, J = s2 - 3 * (s2 / 3);
       const float* cq = Cik + t * 81 + J * 9;
       const float bv = cq[0] * Gl[t][0] + cq[1] * Gl[t][1] + cq[2] * Gl[t][2];
-      acc[ik] = __builtin_amdgcn_mfma_f32_32x32x2f32(Gl[t][J], bv, acc[ik], 0, 0, 0);
+      acc[gk] = __builtin_amdgcn_mfma_f32_32x32x2f32(Gl[t][J], bv, acc[gk], 0, 0, 0);
     }
   }
-  // D[m][n]: lane holds column n, rows m = 8 (r / 4) + 4 h + r % 4
+  // D[m][n]: lane holds column n, rows m = 8 (r / 4) + 4 h + r % 4.  The blocks
+  // (0,1), (0,2), (1,2) go to the wave's tangent region (no longer read), 27 x 27 each
+  h27_wave_sync();
+  float* T = &S.C[0][0];
+  static_assert(3 * 27 * 27 <= 2 * kH27T * 81, "transposed blocks exceed the tangent region");
+  if (n < 27) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int m = 8 * (r / 4) + 4 * h + (r % 4);
+      if (m < 27) {
+        T[m * 27 + n] = acc[1][r];
+        T[729 + m * 27 + n] = acc[2][r];
+        T[1458 + m * 27 + n] = acc[4][r];
+      }
+    }
+  }
+  h27_wave_sync();
   if (n < 27) {
     float* o = out + c * (27 * 27 * 9) + (int64_t)n * 9;
 #pragma unroll
@@ -421,8 +442,15 @@ __global__ void __launch_bounds__(64 * kH27Waves, 2) hex27_jacobian_mfma(const d
       const int m = 8 * (r / 4) + 4 * h + (r % 4);
       if (m < 27) {
         float* blk = o + m * (27 * 9);
-#pragma unroll
-        for (int ik = 0; ik < 9; ++ik) blk[ik] = acc[ik][r];
+        blk[0] = acc[0][r];
+        blk[1] = acc[1][r];
+        blk[2] = acc[2][r];
+        blk[3] = T[n * 27 + m];          // (1, 0): K^{01}[n][m]
+        blk[4] = acc[3][r];
+        blk[5] = acc[4][r];
+        blk[6] = T[729 + n * 27 + m];    // (2, 0): K^{02}[n][m]
+        blk[7] = T[1458 + n * 27 + m];   // (2, 1): K^{12}[n][m]
+        blk[8] = acc[5][r];
       }
     }
   }
