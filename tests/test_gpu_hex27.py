@@ -172,3 +172,35 @@ def test_hex27_adjoint_mu_vs_oracle_and_fd(jfa):
             vals.append(float((uu ** 2).sum()))
         fd = (vals[0] - vals[1]) / (2e-4 * mu0)
         assert abs(fd - g[c]) <= 1e-5 * abs(g[c]) + 1e-12, (c, fd, g[c])
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_hex27_row_assembly_bit_exact(jfa, dtype):
+    """hf_assemble on a HEX27 vec-3 pattern (each block the segmented sum of its
+    contribution list) against the same sums formed in NumPy in the same order: every
+    block is the sum of its (cell, a, b) contributions in ascending cell order, from
+    zero, in the element matrices' own precision - bit for bit."""
+    say("hex27 row assembly", dtype)
+    pts, cells = _hex27_box((3, 2, 2), seed=5)
+    mat, _ = _mats(jfa, "neo_hookean", None)
+    prob = _problem(jfa, pts, cells, mat)
+    S = __import__("jax_fem_amd.solver", fromlist=["x"])
+    rng = np.random.default_rng(2)
+    u = torch.as_tensor(0.02 * rng.standard_normal(prob.N), device=DEV)
+    _, ej = prob._element_terms(u, True, dtype)
+    E = ej.reshape(len(cells), 27, 27, 3, 3).cpu().numpy()
+    vals = prob.jacobian_bsr(u, dtype, bc=False)
+    A = S.TangentMatrix(prob, vals).to_scipy().tocsr()
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    # block sums in ascending cell order (np.add.at adds in index order, unbuffered)
+    nn = prob.fe.num_total_nodes
+    keys = (cells[:, :, None].astype(np.int64) * nn + cells[:, None, :]).reshape(-1)
+    uniq, inv = np.unique(keys, return_inverse=True)
+    blocks = np.zeros((len(uniq), 3, 3), dtype=npdt)
+    np.add.at(blocks, inv, E.reshape(-1, 3, 3).astype(npdt))
+    rows, cols = uniq // nn, uniq % nn
+    for i in range(3):
+        for k in range(3):
+            got = np.asarray(A[3 * rows + i, 3 * cols + k]).reshape(-1).astype(npdt)
+            assert np.array_equal(got, blocks[:, i, k]), (i, k, np.abs(got - blocks[:, i, k]).max())
+    say("  %d blocks equal bit for bit" % len(uniq))
