@@ -184,5 +184,12 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream(dev=None):
+    """The current HIP stream of `dev` (default: the current device) as a handle."""
+    if _raw_stream is not None and dev is None:
+        # the raw handle of torch's current stream, without building a Stream object
+        return ctypes.c_void_p(_raw_stream(torch._C._cuda_getDevice()))
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
