@@ -395,7 +395,7 @@ def solve_with_fallback(A, b, x0, problem, rtol, atol, maxiter, rel_r0, poll=1, 
     x_start = (torch.zeros(A.shape[0], dtype=torch.float32, device=b.device) if x0 is None else
                x0.to(torch.float32).clone())
     x, info = mg.solve(A, b, x_start, rtol, atol, maxiter, rel_r0, poll, stats, problem=problem, rebuild=rebuild,
-                       token=stats if stats is not None else object())
+                       token=getattr(stats, "token", stats) if stats is not None else object())
     if info.status in (0, 1):
         return x, info, False
     if isinstance(A, LatticeTangent):

@@ -49,6 +49,19 @@ Added Newton keys:
     in a thin boundary layer whose tangent can be indefinite.  A step that
     raises ||R|| is halved up to 8 times (``backtrack``, default on with 'ew'),
     and a PCG breakdown (p.Ap <= 0) re-solves that step with BiCGSTAB.
+  * ``lag_ratio`` (default 10 with 'ew', 0 otherwise): when ||R||/||R0|| is
+    within this factor of ``rel_tol`` and the last step formed a fresh tangent,
+    the next step reuses that tangent (a chord step): what is left is a small
+    reduction that the lagged tangent gives as well, without a new element
+    Jacobian, assembly and preconditioner set-up.  Convergence is still judged
+    on the FP64 residual, and a chord step is never followed by another.  With
+    ``lag_ratio`` > 0 and a nonlinear law the last tangent of a solve is kept on
+    the problem: the first step of the next solve of the same problem reuses it
+    when that solve starts from the last solution (a load step warm-started from
+    the previous one: the tangent is the one at, or one small correction from,
+    that state) and would form a tangent of the same kind; a LatticeTangent is
+    reused only while no newer stencil was assembled.  It holds the tangent's
+    memory until the next solve.
 Unknown keys are ignored, as in the reference.
 """
 import ctypes
@@ -66,7 +79,7 @@ _METHOD_KEYS = ("newton", "arc_length", "dynamic_relax")
 _LINEAR_KEYS = ("jax_solver", "amgx_solver", "spsolve_solver", "petsc_solver", "custom_solver", "pcg", "bicgstab",
                 "learned", "mg")
 _NEWTON_KEYS = ("tol", "rel_tol", "line_search_flag", "initial_guess", "precision", "forcing", "max_iter",
-                "eta_max", "eta_min", "eta_first", "backtrack")
+                "eta_max", "eta_min", "eta_first", "backtrack", "lag_ratio")
 
 
 def _resolve_solver_options(solver_options):
@@ -196,6 +209,7 @@ def _initial_guess(problem, u):
 class KrylovStats:
     def __init__(self):
         self.calls = []
+        self.token = self    # identifies the solve sequence for a preconditioner's rebuild policy
 
     def add(self, method, iters, status, res, bnorm, seconds):
         self.calls.append({"method": method, "iterations": iters, "status": status, "res": res,
@@ -410,6 +424,7 @@ def _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, 
     eta_min = cfg.get("eta_min", 1e-6 if precision == "mixed" else 1e-10)
     eta_first = cfg.get("eta_first", 1e-3)
     backtrack = cfg.get("backtrack", forcing == "ew")
+    lag_ratio = float(cfg.get("lag_ratio", 10.0 if forcing == "ew" else 0.0))
     timing = {"local_assembly": 0.0, "global_matrix": 0.0, "linear": 0.0}
     stats = KrylovStats()
     torch.cuda.synchronize()
@@ -434,6 +449,25 @@ def _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, 
     eta_prev = None
     A = None
     linear_law = bool(getattr(problem.material, "linear", False))
+    fresh = False     # the last step formed its own tangent
+    chords = 0
+    lag_key = lag_refs = None
+    if lag_ratio > 0.0 and not linear_law:
+        from .problem import _fingerprint
+        # the internal variables by content (arrays) or by storage and version (tensors); the
+        # kept tangent holds references to them, so no new tensor can take a kept one's storage
+        lag_refs = (getattr(problem, "internal_vars", None), getattr(problem, "internal_vars_surfaces", None))
+        lag_key = (smg, adt, (lin_opts.get("mg") or {}).get("matrix_free", True) if smg is not None else None,
+                   _fingerprint(lag_refs[0]), _fingerprint(lag_refs[1]))
+    lag = getattr(problem, "_lag_tangent", None)
+    problem._lag_tangent = None
+    if lag is not None:
+        lagA, key, lag_u = lag[:3]
+        if (lag_ratio <= 0.0 or linear_law or key != lag_key or lag_u.shape != u.shape
+                or (isinstance(lagA, LatticeTangent if smg is not None else ()) and
+                    getattr(problem, "_stencil_version", None) != lagA._version)
+                or not torch.equal(lag_u, u)):
+            lag = None
     def rel_of(r):
         # ||R||/||R0||; a zero initial residual gives NaN as the reference's NumPy
         # division does, so the loop below does not start (solver.py:1283-1295)
@@ -447,10 +481,21 @@ def _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, 
         ev0.record()
         if A is not None and linear_law:
             pass   # a linear law's tangent does not depend on u: the first step's serves the solve
+        elif A is None and lag is not None:
+            # chord step on the last solve's tangent; a multigrid keeps the hierarchy
+            # it had for that solve (its own staleness check still rebuilds it)
+            A, stats.token, lag = lag[0], lag[3], None
+            fresh = False
+            chords += 1
+        elif A is not None and fresh and rel_of(res) <= lag_ratio * rel_tol:
+            fresh = False   # chord step on the last tangent
+            chords += 1
         elif smg is not None:
             A = LatticeTangent(problem, u, smg, matrix_free=(lin_opts["mg"] or {}).get("matrix_free", True))
+            fresh = True
         else:
             A = TangentMatrix(problem, problem.jacobian_bsr(u, adt))
+            fresh = True
         ev1 = torch.cuda.Event(enable_timing=True)
         ev1.record()
         asm_events.append((ev0, ev1))
@@ -512,10 +557,14 @@ def _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, 
     timing["linear"] = sum(a.elapsed_time(b) for a, b in lin_events) * 1e-3
     problem.solve_info = {"newton_iters": n, "res_hist": hist, "res0": res0, "rel": res / res0 if res0 > 0.0 else 0.0,
                           "krylov": stats.calls, "krylov_iters": stats.iterations,
-                          "fallbacks": sum(1 for c in stats.calls if c.get("fallback")),
+                          "fallbacks": sum(1 for c in stats.calls if c.get("fallback")), "chord_steps": chords,
                           "timing": timing, "wall": time.perf_counter() - wall0, "precision": precision}
     problem._newton_u = u
     problem._A_cache = None
+    if A is not None and lag_ratio > 0.0 and not linear_law:
+        problem._lag_tangent = (A, lag_key, u, stats.token, lag_refs)
+    elif lag is not None:
+        problem._lag_tangent = lag     # no step was taken: the kept tangent stays valid
     return problem.unflatten_fn_sol_list(u)
 
 

@@ -15,7 +15,9 @@
   search: same step lengths, residual history and root;
 * a zero initial residual (no load, zero Dirichlet values): no Newton step, as
   the reference's NaN comparison gives (solver.py:1283-1295);
-* a surface traction changed in place between two solves is seen by the second.
+* a surface traction changed in place between two solves is seen by the second;
+* load steps warm-started from the last: the kept tangent (a chord step on the
+  last solve's tangent, lag_ratio) against the oracle's Newton sequence.
 (u-dependent mass and surface maps are element terms: test_gpu_pointwise.py.)
 """
 import os
@@ -197,3 +199,67 @@ def test_surface_traction_changed_in_place(jfa):
         u2 = jfa.solver(prob)[0]
         rel = (torch.linalg.norm(u2 - 2.0 * u1) / torch.linalg.norm(u1)).item()
         assert rel <= 1e-8, (kind, rel)
+
+
+def test_load_steps_reuse_last_tangent(jfa):
+    """Neo-Hookean 12^3 cube (E = 10 scaled per cell by a per-cell field, nu = 0.3),
+    u = 0 on x = 0 and u_x = 0.1 k / 4 on x = 1 in 4 load steps, each warm-started
+    from the last: the mixed multigrid path with the kept tangent (lag_ratio 10, the
+    default) and without it (lag_ratio 0) both reach 1e-8 relative FP64 residual at
+    every step and match the oracle's Newton sequence (direct solves) to rel-L2
+    1e-6.  With the default, steps 2-4 start on the last step's tangent and the
+    hierarchy is set up once; a new parameter field, or a start that is not the
+    last solution, makes the next solve form its own tangent."""
+    say("newton load steps: kept tangent")
+    n = 12
+    left = lambda p: np.isclose(p[0], 0.0, atol=1e-5)
+    right = lambda p: np.isclose(p[0], 1.0, atol=1e-5)
+    bc = lambda d: [[left, left, left, right], [0, 1, 2, 0], [lambda p: 0.0] * 3 + [lambda p: d]]
+    scale = 1.0 + 0.5 * np.random.default_rng(7).random(n ** 3)
+
+    class P(jfa.Problem):
+        def get_tensor_map(self):
+            return jfa.NeoHookean(E=10.0, nu=0.3, param="scale_E")
+
+    pts, cells = ofe.box_mesh(n, n, n)
+    op = osv.OracleProblem(pts, cells, 3, ok.Material("neo_hookean", E=10.0, nu=0.3, param="scale_E"),
+                           dirichlet_bc_info=bc(0.0), params=np.repeat(scale[:, None], 8, axis=1))
+    uo, ref = None, []
+    for k in range(1, 5):
+        op.update_bcs(bc(0.1 * k / 4))
+        uo, _ = osv.newton_solve(op, u0=uo, method="spsolve")
+        ref.append(uo.copy())
+    for lag in (10.0, 0.0):
+        prob = P(jfa.box_mesh(n, n, n), vec=3, dim=3, dirichlet_bc_info=bc(0.0))
+        prob.internal_vars = [np.repeat(scale[:, None], 8, axis=1)]
+        opts = {"precision": "mixed", "tol": 0.0, "eta_first": 1e-4, "lag_ratio": lag, "linear": {"mg": {}}}
+        sol, chords, setups = None, [], []
+        for k in range(1, 5):
+            prob.fe.update_Dirichlet_boundary_conditions(bc(0.1 * k / 4))
+            sol = jfa.solver(prob, {"newton": dict(opts, initial_guess=sol)})
+            si = prob.solve_info
+            chords.append(si["chord_steps"])
+            setups.append(prob._mg.setups)
+            u = sol[0].reshape(-1).cpu().numpy()
+            rel = np.linalg.norm(u - ref[k - 1]) / np.linalg.norm(ref[k - 1])
+            say("  lag %g step %d: newton %d chords %d rel %.2e vs oracle %.2e" % (
+                lag, k, si["newton_iters"], si["chord_steps"], si["rel"], rel))
+            assert si["rel"] <= 1e-8
+            assert rel <= 1e-6
+        if lag > 0.0:
+            assert all(c >= 1 for c in chords[1:])
+            assert setups[-1] == setups[0]
+            # a new parameter field (not a uniform factor: with Dirichlet loading alone that
+            # leaves the solution as it is): the next solve forms its own tangent first
+            scale2 = scale * (1.0 + 0.2 * np.random.default_rng(8).random(n ** 3))
+            prob.internal_vars = [np.repeat(scale2[:, None], 8, axis=1)]
+            sol = jfa.solver(prob, {"newton": dict(opts, initial_guess=sol)})
+            assert prob.solve_info["newton_iters"] >= 1 and prob.solve_info["rel"] <= 1e-8
+            assert prob._mg.setups == setups[-1] + 1
+            # a start that is not the last solution: no reuse either
+            prob.fe.update_Dirichlet_boundary_conditions(bc(0.05))
+            jfa.solver(prob, {"newton": dict(opts, initial_guess=None)})
+            assert prob._mg.setups == setups[-1] + 2
+        else:
+            assert not any(chords)
+            assert prob._lag_tangent is None
