@@ -466,7 +466,7 @@ def _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, 
         if (lag_ratio <= 0.0 or linear_law or key != lag_key or lag_u.shape != u.shape
                 or (isinstance(lagA, LatticeTangent if smg is not None else ()) and
                     getattr(problem, "_stencil_version", None) != lagA._version)
-                or not torch.equal(lag_u, u)):
+                or not (_same_tensors(cfg.get("initial_guess"), lag[5]) or torch.equal(lag_u, u))):
             lag = None
     def rel_of(r):
         # ||R||/||R0||; a zero initial residual gives NaN as the reference's NumPy
@@ -561,11 +561,28 @@ def _newton(problem, cfg, precision, forcing, rel_tol, tol, max_iter, lin_opts, 
                           "timing": timing, "wall": time.perf_counter() - wall0, "precision": precision}
     problem._newton_u = u
     problem._A_cache = None
+    sol = problem.unflatten_fn_sol_list(u)
     if A is not None and lag_ratio > 0.0 and not linear_law:
-        problem._lag_tangent = (A, lag_key, u, stats.token, lag_refs)
+        # the returned tensors (views of u, which the kept tuple holds) by storage, version
+        # and shape: a next solve started from them is known to start from u without a sync
+        problem._lag_tangent = (A, lag_key, u, stats.token, lag_refs, _tensor_sig(sol))
     elif lag is not None:
-        problem._lag_tangent = lag     # no step was taken: the kept tangent stays valid
-    return problem.unflatten_fn_sol_list(u)
+        problem._lag_tangent = lag[:2] + (u,) + lag[3:5] + (_tensor_sig(sol),)   # no step taken: still valid
+    return sol
+
+
+def _tensor_sig(ts):
+    return tuple((t.data_ptr(), t._version, tuple(t.shape)) for t in ts)
+
+
+def _same_tensors(g, sig):
+    """Whether the initial guess g is the tensors a solve returned (signature sig),
+    unmodified since."""
+    if g is None:
+        return False
+    g = list(g) if isinstance(g, (list, tuple)) else [g]
+    return len(g) == len(sig) and all(torch.is_tensor(t) and (t.data_ptr(), t._version, tuple(t.shape)) == q
+                                      for t, q in zip(g, sig))
 
 
 def _line_search(problem, u, inc, residual):
